@@ -1,0 +1,769 @@
+// dgr_kernels.hip -- gfx950 kernels of the 2D p=1 DG shallow-water RK step.
+//
+// One RK stage is four element/node-parallel phases:
+//   k_wd_status   wetDryStatus2D            (2DCode/wetDry2D.c:16-60), wet/dry runs only
+//   k_stage       compute2DL + RK update    (compute2DL.c:12-413, numericalFlux2D.c:6-108,
+//                                            2DInnerProducts.c:8-570, time_evolution.c:72-88,208-217)
+//   k_node_minmax node min/max of element means (SlopeLimiter2D.c:106-153)
+//   k_limit       SlopeLimiter element pass (SlopeLimiter2D.c:157-343) + PDop2D (wetDry2D.c:63-260)
+//
+// The reference's edge loop (compute2DL.c:17-331) scatters each Roe flux to
+// its two elements.  Here every element gathers its three fluxes instead: it
+// evaluates each edge in the reference's canonical orientation (el_in's
+// outward normal, el_in's state as the "in" state) and negates on the el_ex
+// side, so every Fhat slot gets the reference's bits, with no atomics and no
+// dependence on launch order.  Interior fluxes are therefore evaluated twice.
+//
+// Floating point: the kernels are compiled with -ffp-contract=off and keep the
+// reference's evaluation order operation for operation; the only intended
+// differences from the reference binary are pow(x,2) -> x*x and
+// pow(H,4./3) -> dgr_pow43 (see dgr_math.h).
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "dgr_device.h"
+#include "dgr_math.h"
+
+namespace dgr {
+
+#define DGR_MAX(a, b) (((a) > (b)) ? (a) : (b))
+#define DGR_MIN(a, b) (((a) < (b)) ? (a) : (b))
+
+__device__ __forceinline__ int isign(double a) { return a < 0 ? -1 : (a == 0 ? 0 : 1); }
+
+__device__ __forceinline__ void raise_error(Ctrl *c, int code, int where)
+{
+    if (atomicCAS(&c->err_code, 0, code) == 0) c->err_where = where;
+}
+
+// ---------------------------------------------------------------------------
+// Roe flux, numericalFlux2D.c:6-108.  Returns max |lambda| over the six
+// one-sided eigenvalues (:101-105).
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double roe_flux(double F[3], double zi, double ze, double qxi, double qxe,
+                                           double qyi, double qye, double zb, double nx, double ny,
+                                           double lg, double g)
+{
+    const double Hi = zi + zb, He = ze + zb;
+    const double ui = qxi / Hi, ue = qxe / He;
+    const double vi = qyi / Hi, ve = qye / He;
+    const double j0 = zi - ze, j1 = qxi - qxe, j2 = qyi - qye;
+    const double f1xi = Hi * ui, f1xe = He * ue;
+    const double f1yi = Hi * vi, f1ye = He * ve;
+    const double f2xi = Hi * ui * ui + 0.5 * lg * (Hi * Hi - zb * zb);
+    const double f2xe = He * ue * ue + 0.5 * lg * (He * He - zb * zb);
+    const double f2yi = Hi * ui * vi, f2ye = He * ue * ve;
+    const double f3xi = Hi * ui * vi, f3xe = He * ue * ve;
+    const double f3yi = Hi * vi * vi + 0.5 * lg * (Hi * Hi - zb * zb);
+    const double f3ye = He * ve * ve + 0.5 * lg * (He * He - zb * zb);
+    const double fni0 = f1xi * nx + f1yi * ny, fne0 = f1xe * nx + f1ye * ny;
+    const double fni1 = f2xi * nx + f2yi * ny, fne1 = f2xe * nx + f2ye * ny;
+    const double fni2 = f3xi * nx + f3yi * ny, fne2 = f3xe * nx + f3ye * ny;
+
+    const double Hh = 0.5 * (Hi + He);
+    const double sHi = sqrt(Hi), sHe = sqrt(He);
+    const double uh = (ui * sHi + ue * sHe) / (sHi + sHe);
+    const double vh = (vi * sHi + ve * sHe) / (sHi + sHe);
+    const double c = sqrt(g * Hh);
+
+    const double R10 = uh - c * nx, R11 = -ny, R12 = uh + c * nx;
+    const double R20 = vh - c * ny, R21 = nx, R22 = vh + c * ny;
+    const double den = -2 * c;
+    const double Q00 = (-c - nx * uh - ny * vh) / den;
+    const double Q01 = nx / den;
+    const double Q02 = ny / den;
+    const double Q10 = (-2 * c * ny * uh + 2 * c * nx * vh) / den;
+    const double Q11 = 2 * c * ny / den;
+    const double Q12 = -2 * c * nx / den;
+    const double Q20 = (-c + nx * uh + ny * vh) / den;
+    const double Q21 = -nx / den;
+    const double Q22 = -ny / den;
+
+    const double sgi = sqrt(g * Hi), sge = sqrt(g * He);
+    const double l1i = ui * nx + vi * ny - sgi;
+    const double l2i = ui * nx + vi * ny;
+    const double l3i = ui * nx + vi * ny + sgi;
+    const double l1e = ue * nx + ve * ny - sge;
+    const double l2e = ue * nx + ve * ny;
+    const double l3e = ue * nx + ve * ny + sge;
+    double lh1 = uh * nx + vh * ny - c;
+    double lh2 = uh * nx + vh * ny;
+    double lh3 = uh * nx + vh * ny + c;
+
+    double e1 = fmax(0.0, (lh1 - l1i));
+    e1 = fmax(e1, (l1e - lh1));
+    lh1 = fmax(e1, fabs(lh1));
+    double e2 = fmax(0.0, (lh2 - l2i));
+    e2 = fmax(e2, (l2e - lh2));
+    lh2 = fmax(e2, fabs(lh2));
+    double e3 = fmax(0.0, (lh3 - l3i));
+    e3 = fmax(e3, (l3e - lh3));
+    lh3 = fmax(e3, fabs(lh3));
+
+    // MatrixVectorMultiply x3 (mathfunctions.c:24-35), zeros included
+    const double p10 = ((0.0 + Q00 * j0) + Q01 * j1) + Q02 * j2;
+    const double p11 = ((0.0 + Q10 * j0) + Q11 * j1) + Q12 * j2;
+    const double p12 = ((0.0 + Q20 * j0) + Q21 * j1) + Q22 * j2;
+    const double p20 = ((0.0 + lh1 * p10) + 0.0 * p11) + 0.0 * p12;
+    const double p21 = ((0.0 + 0.0 * p10) + lh2 * p11) + 0.0 * p12;
+    const double p22 = ((0.0 + 0.0 * p10) + 0.0 * p11) + lh3 * p12;
+    const double p30 = ((0.0 + 1.0 * p20) + 0.0 * p21) + 1.0 * p22;
+    const double p31 = ((0.0 + R10 * p20) + R11 * p21) + R12 * p22;
+    const double p32 = ((0.0 + R20 * p20) + R21 * p21) + R22 * p22;
+    F[0] = 0.5 * (fni0 + fne0) + 0.5 * p30;
+    F[1] = 0.5 * (fni1 + fne1) + 0.5 * p31;
+    F[2] = 0.5 * (fni2 + fne2) + 0.5 * p32;
+
+    double mx = DGR_MAX(fabs(l1i), fabs(l1e));
+    mx = DGR_MAX(mx, fabs(l2i));
+    mx = DGR_MAX(mx, fabs(l2e));
+    mx = DGR_MAX(mx, fabs(l3i));
+    mx = DGR_MAX(mx, fabs(l3e));
+    return mx;
+}
+
+// Cramer solve of the RT0 fit (2DInnerProducts.c:134-146)
+__device__ __forceinline__ void rt0(double M11, double M12, double M13, double M21, double M22, double M23,
+                                    double M31, double M32, double M33, double den, double R1, double R2,
+                                    double R3, double &a, double &b, double &c)
+{
+    a = (R3 * M13 * M22 - R3 * M12 * M23 - R2 * M13 * M32 + R1 * M23 * M32 + R2 * M12 * M33 - R1 * M22 * M33) / den;
+    b = -(R3 * M13 * M21 - R3 * M11 * M23 - R2 * M13 * M31 + R1 * M23 * M31 + R2 * M11 * M33 - R1 * M21 * M33) / den;
+    c = (R3 * M12 * M21 - R3 * M11 * M22 - R2 * M12 * M31 + R1 * M22 * M31 + R2 * M11 * M32 - R1 * M21 * M32) / den;
+}
+
+// ---------------------------------------------------------------------------
+// Compute2DInnerProducts + the element loop of compute2DL: the 9 RHS values
+// of one element from its state W[f*3+k], its fluxes Fh[f*3+k] and geometry.
+// 2DInnerProducts.c:14-566, compute2DL.c:344-409.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void element_rhs(const double4 P[3], const double W[9], const double Fh[9],
+                                            int wd_self, double g, double rain, double R[9])
+{
+    const double xv1 = P[0].x, xv2 = P[1].x, xv3 = P[2].x;
+    const double yv1 = P[0].y, yv2 = P[1].y, yv3 = P[2].y;
+    const double x1 = (xv1 + xv2) / 2, x2 = (xv2 + xv3) / 2, x3 = (xv3 + xv1) / 2;
+    const double y1 = (yv1 + yv2) / 2, y2 = (yv2 + yv3) / 2, y3 = (yv3 + yv1) / 2;
+    const double el0 = sqrt((xv2 - xv1) * (xv2 - xv1) + (yv2 - yv1) * (yv2 - yv1));
+    const double el1 = sqrt((xv3 - xv2) * (xv3 - xv2) + (yv3 - yv2) * (yv3 - yv2));
+    const double el2 = sqrt((xv3 - xv1) * (xv3 - xv1) + (yv3 - yv1) * (yv3 - yv1));
+    const double ej0 = el0 / 2, ej1 = el1 / 2, ej2 = el2 / 2;
+
+    const double sqrt3 = sqrt(3.0);
+    double xe[3][2], ye[3][2];
+    xe[0][0] = (sqrt3 * x1 - x2 + x3) / sqrt3;  ye[0][0] = (sqrt3 * y1 - y2 + y3) / sqrt3;
+    xe[0][1] = (sqrt3 * x1 + x2 - x3) / sqrt3;  ye[0][1] = (sqrt3 * y1 + y2 - y3) / sqrt3;
+    xe[1][0] = (x1 + sqrt3 * x2 - x3) / sqrt3;  ye[1][0] = (y1 + sqrt3 * y2 - y3) / sqrt3;
+    xe[1][1] = (-x1 + sqrt3 * x2 + x3) / sqrt3; ye[1][1] = (-y1 + sqrt3 * y2 + y3) / sqrt3;
+    xe[2][0] = (-x1 + x2 + sqrt3 * x3) / sqrt3; ye[2][0] = (-y1 + y2 + sqrt3 * y3) / sqrt3;
+    xe[2][1] = (x1 - x2 + sqrt3 * x3) / sqrt3;  ye[2][1] = (y1 - y2 + sqrt3 * y3) / sqrt3;
+
+    // Psi_b at Gauss point n of edge e: Ps[b][e][n]  (:73-87)
+    const double dn0 = x1 * y2 - x1 * y3 + x2 * y3 - x2 * y1 + x3 * y1 - x3 * y2;
+    double Ps[3][3][2];
+#pragma unroll
+    for (int e = 0; e < 3; ++e)
+#pragma unroll
+        for (int n = 0; n < 2; ++n) {
+            const double xq = xe[e][n], yq = ye[e][n];
+            Ps[0][e][n] = -(yq * x2 - yq * x3 - xq * y2 + x3 * y2 + xq * y3 - x2 * y3) / dn0;
+            Ps[1][e][n] = -(-x1 * yq + x3 * yq - x3 * y1 + xq * y1 - xq * y3 + x1 * y3) / dn0;
+            Ps[2][e][n] = -(yq * x1 - yq * x2 - xq * y1 + x2 * y1 + xq * y2 - x1 * y2) / dn0;
+        }
+
+    const double jac = 4 * (x2 - x3) * (y2 - y1) - 4 * (x2 - x1) * (y2 - y3);   // :110
+    const int sj = isign(jac);
+    const double nx1 = sj * 2 * (y2 - y3) / el0, ny1 = sj * 2 * (x3 - x2) / el0;
+    const double nx2 = sj * 2 * (y3 - y1) / el1, ny2 = sj * 2 * (x1 - x3) / el1;
+    const double nx3 = sj * 2 * (y1 - y2) / el2, ny3 = sj * 2 * (x2 - x1) / el2;
+    const double M11 = nx1, M12 = x1 * nx1 + y1 * ny1, M13 = ny1;
+    const double M21 = nx2, M22 = x2 * nx2 + y2 * ny2, M23 = ny2;
+    const double M31 = nx3, M32 = x3 * nx3 + y3 * ny3, M33 = ny3;
+    const double den = M13 * M22 * M31 - M12 * M23 * M31 - M13 * M21 * M32 + M11 * M23 * M32 + M12 * M21 * M33 - M11 * M22 * M33;
+    const double nxs[3] = {nx1, nx2, nx3}, nys[3] = {ny1, ny2, ny3};
+    const double ejs[3] = {ej0, ej1, ej2};
+
+    // surface integrals SI[f*3+b] (:134-239)
+    double SI[9];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        double a, b, c;
+        rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, Fh[f * 3 + 0], Fh[f * 3 + 1], Fh[f * 3 + 2], a, b, c);
+        double Fn[3][2];
+#pragma unroll
+        for (int n = 0; n < 2; ++n)
+#pragma unroll
+            for (int e = 0; e < 3; ++e) Fn[e][n] = (a + b * xe[e][n]) * nxs[e] + (c + b * ye[e][n]) * nys[e];
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+            SI[f * 3 + bb] = ejs[0] * (Fn[0][0] * Ps[bb][0][0] + Fn[0][1] * Ps[bb][0][1]) +
+                             ejs[1] * (Fn[1][0] * Ps[bb][1][0] + Fn[1][1] * Ps[bb][1][1]) +
+                             ejs[2] * (Fn[2][0] * Ps[bb][2][0] + Fn[2][1] * Ps[bb][2][1]);
+    }
+
+    // volume terms (:247-566)
+    const double weight = 1.0 / 6;
+    const double mz[3] = {0.5 * (P[0].z + P[1].z), 0.5 * (P[1].z + P[2].z), 0.5 * (P[2].z + P[0].z)};
+    const double mn[3] = {0.5 * (P[0].w + P[1].w), 0.5 * (P[1].w + P[2].w), 0.5 * (P[2].w + P[0].w)};
+    const double gP[3][2] = {{4 / jac * (y2 - y3), 4 / jac * (x3 - x2)},
+                             {4 / jac * (y3 - y1), 4 / jac * (x1 - x3)},
+                             {4 / jac * (y1 - y2), 4 / jac * (x2 - x1)}};
+    const double lg = (wd_self == 0) ? 0.0 : g;
+    double H[3], tau[3], Fx[3][3], Fy[3][3];
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        const double zr = W[e], qx = W[3 + e], qy = W[6 + e];
+        H[e] = zr + mz[e];
+        const double u = qx / H[e];
+        const double v = qy / H[e];
+        tau[e] = g * mn[e] * mn[e] * sqrt(u * u + v * v) / (dgr_pow43(H[e]));
+        Fx[0][e] = qx;
+        Fx[1][e] = (qx * qx) / H[e] + 0.5 * lg * ((H[e] * H[e]) - (mz[e] * mz[e]));
+        Fx[2][e] = qx * qy / H[e];
+        Fy[0][e] = qy;
+        Fy[1][e] = qx * qy / H[e];
+        Fy[2][e] = (qy * qy) / H[e] + 0.5 * lg * ((H[e] * H[e]) - (mz[e] * mz[e]));
+    }
+    const double gz0 = mz[0] * gP[0][0] + mz[1] * gP[1][0] + mz[2] * gP[2][0];
+    const double gz1 = mz[0] * gP[0][1] + mz[1] * gP[1][1] + mz[2] * gP[2][1];
+    const double aj = fabs(jac);
+    const double sx = x1 + x2 + x3, sy = y1 + y2 + y3;
+    double VF[3][3];   // [field][basis]
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        double a, b, c;
+        rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den,
+            Fx[f][0] * nx1 + Fy[f][0] * ny1, Fx[f][1] * nx2 + Fy[f][1] * ny2, Fx[f][2] * nx3 + Fy[f][2] * ny3,
+            a, b, c);
+#pragma unroll
+        for (int bb = 0; bb < 3; ++bb)
+            VF[f][bb] = aj * (0.5 * a * gP[bb][0] + 0.5 * c * gP[bb][1] + b * weight * (gP[bb][0] * sx + gP[bb][1] * sy));
+    }
+    // rhs combine (compute2DL.c:344-354) and Minv = 6/|jac_vertex| (:380-404)
+    const double jv = P[0].x * (P[1].y - P[2].y) + P[1].x * (P[2].y - P[0].y) + P[2].x * (P[0].y - P[1].y);
+    const double Mi = 6 / fabs(jv);
+    double ze[3], qe[3], re[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ze[k] = -SI[k] + VF[0][k];
+        const double bx = lg * aj * gz0 * weight * W[k];
+        const double fx = aj * weight * tau[k] * W[3 + k];
+        qe[k] = -SI[3 + k] + VF[1][k] + bx - fx;
+        const double by = lg * aj * gz1 * weight * W[k];
+        const double fy = aj * weight * tau[k] * W[6 + k];
+        re[k] = -SI[6 + k] + VF[2][k] + by - fy;
+    }
+    R[0] = (Mi * ze[0] + 0.0 * ze[1]) + 0.0 * ze[2];
+    R[1] = (0.0 * ze[0] + Mi * ze[1]) + 0.0 * ze[2];
+    R[2] = (0.0 * ze[0] + 0.0 * ze[1]) + Mi * ze[2];
+    R[3] = (Mi * qe[0] + 0.0 * qe[1]) + 0.0 * qe[2];
+    R[4] = (0.0 * qe[0] + Mi * qe[1]) + 0.0 * qe[2];
+    R[5] = (0.0 * qe[0] + 0.0 * qe[1]) + Mi * qe[2];
+    R[6] = (Mi * re[0] + 0.0 * re[1]) + 0.0 * re[2];
+    R[7] = (0.0 * re[0] + Mi * re[1]) + 0.0 * re[2];
+    R[8] = (0.0 * re[0] + 0.0 * re[1]) + Mi * re[2];
+    if (rain != 0.0) {
+        R[0] = R[0] + rain;
+        R[1] = R[1] + rain;
+        R[2] = R[2] + rain;
+    }
+}
+
+// wall reflection of (qx, qy) about the edge normal (compute2DL.c:179-189)
+__device__ __forceinline__ void reflect(double qx, double qy, double nx, double ny, double &rx, double &ry)
+{
+    const double tx = -ny, ty = nx;
+    const double qn = qx * nx + qy * ny;
+    const double qt = qx * tx + qy * ty;
+    const double qne = -qn, qte = qt;
+    const double dn = 1. / (nx * ty - ny * tx);
+    rx = (ty * qne - ny * qte) * dn;
+    ry = (-tx * qne + nx * qte) * dn;
+}
+
+template <typename T>
+__device__ __forceinline__ T wave_max(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const T w = __shfl_xor(v, o, 64);
+        v = (w > v) ? w : v;
+    }
+    return v;
+}
+
+// ---------------------------------------------------------------------------
+// k_stage: flux gather + RHS + RK update of one element per thread.
+//   MODE 0: RHS only (compute2DL), written to dbg_rhs/dbg_fhat
+//   MODE 1: w_out = w_in + RHS*dt                       (time_evolution.c:72-88)
+//   MODE 2: w_out = (w_in + w_base)*0.5 + RHS*(dt/2)    (time_evolution.c:176-178,208-217)
+//           w_out may alias w_base (each element reads its own slot first).
+// ---------------------------------------------------------------------------
+template <int MODE, bool WETDRY>
+__global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                               double *wout, const double *wbase,
+                                               const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                               const int *__restrict__ wd, double *__restrict__ means,
+                                               Ctrl *ctrl, int slot, double *__restrict__ dbg_rhs,
+                                               double *__restrict__ dbg_fhat)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const bool valid = e < m.ne;
+    double lam = 0.0;
+    if (valid) {
+        const int P = m.pitch;
+        const double g = ph.g;
+        int v[3];
+        double4 X[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) v[k] = m.ev[k * P + e];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) X[k] = m.nodes[v[k]];
+        double W[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
+        const int wd_self = wd[e];
+
+        double Fh[9];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int inf = m.info[k * P + e];
+            const int nbk = m.nb[k * P + e];
+            const int knb = inf & EI_KNB_MASK;
+            const bool is_in = (inf & EI_IS_IN) != 0;
+            const bool boundary = (inf & EI_BOUNDARY) != 0;
+            const double4 A = X[k], B = X[(k + 1) % 3];
+            const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+            const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
+            double x0, y0, x1, y1;
+            if (inf & EI_DIR_SAME) { x0 = A.x; y0 = A.y; x1 = B.x; y1 = B.y; }
+            else { x0 = B.x; y0 = B.y; x1 = A.x; y1 = A.y; }
+            const double nx = sg * (y1 - y0) / len;
+            const double ny = sg * (x0 - x1) / len;
+            const double tx = -ny, ty = nx;
+            const double zb = (A.z + B.z) / 2;
+            const double oz = W[k], oqx = W[3 + k], oqy = W[6 + k];
+            double zin, qxin, qyin, zex, qxex, qyex;
+            if (boundary) {
+                zin = oz; qxin = oqx; qyin = oqy;
+            } else {
+                const double nz = win[knb * P + nbk], nqx = win[(3 + knb) * P + nbk], nqy = win[(6 + knb) * P + nbk];
+                if (is_in) { zin = oz; qxin = oqx; qyin = oqy; zex = nz; qxex = nqx; qyex = nqy; }
+                else { zin = nz; qxin = nqx; qyin = nqy; zex = oz; qxex = oqx; qyex = oqy; }
+            }
+            if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
+                const int code = (inf >> EI_CODE_SHIFT) & 0xff;
+                const int ed = m.eid[k * P + e];
+                const bool tide = ph.tide_code != 0 && code == ph.tide_code;
+                zex = tide ? ctrl->tide : bzeta[ed];
+                double qnex = tide ? 1000000.0 : bqn[ed];
+                if (zex < 0) zex = zin;
+                if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+                const double qtex = qxin * tx + qyin * ty;
+                const double dn = 1. / (nx * ty - ny * tx);
+                qxex = (ty * qnex - ny * qtex) * dn;
+                qyex = (-tx * qnex + nx * qtex) * dn;
+            } else if (boundary) {                                       // :175-191
+                zex = oz;
+                reflect(qxin, qyin, nx, ny, qxex, qyex);
+            }
+            double F[3], lk;
+            if (WETDRY) {
+                const int wd_in = is_in ? wd_self : wd[nbk];
+                const int wd_ex = boundary ? wd_self : (is_in ? wd[nbk] : wd_self);
+                if (wd_in == 0 && wd_ex == 0) {                          // :203-240
+                    if (is_in) {
+                        double rx, ry;
+                        reflect(qxin, qyin, nx, ny, rx, ry);
+                        lk = roe_flux(F, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
+                        if (boundary) {   // the ex-side reflection flux also feeds max_lambda
+                            double Fd[3], sx_, sy_;
+                            reflect(qxex, qyex, nx, ny, sx_, sy_);
+                            const double le = roe_flux(Fd, zex, zex, sx_, qxex, sy_, qyex, zb, nx, ny, 0.0, g);
+                            lk = DGR_MAX(lk, le);
+                        }
+                    } else {
+                        const double qne = qxex * nx + qyex * ny, qte = qxex * tx + qyex * ty;
+                        const double qni = -qne, qti = qte;
+                        const double dn = 1. / (nx * ty - ny * tx);
+                        const double qxr = (ty * qni - ny * qti) * dn;
+                        const double qyr = (-tx * qni + nx * qte) * dn;
+                        lk = roe_flux(F, zex, zex, qxr, qxex, qyr, qyex, zb, nx, ny, 0.0, g);
+                        F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2];
+                    }
+                } else {                                                 // :241-293
+                    const int my = is_in ? wd_in : wd_ex;
+                    lk = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, (my == 1) ? g : 0.0, g);
+                    if (isnan(F[0]) || isnan(F[1]) || isnan(F[2]))
+                        raise_error(ctrl, (my == 1) ? 3 : (is_in ? 4 : 5), m.eid[k * P + e]);
+                    if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
+                }
+            } else {                                                     // :295-315
+                lk = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
+                if (isnan(F[0]) || isnan(F[1]) || isnan(F[2])) raise_error(ctrl, 3, m.eid[k * P + e]);
+                if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
+            }
+            Fh[k] = F[0];
+            Fh[3 + k] = F[1];
+            Fh[6 + k] = F[2];
+            lam = DGR_MAX(lam, lk);
+        }
+
+        double R[9];
+        element_rhs(X, W, Fh, wd_self, g, ph.rain, R);
+
+        if (MODE == 0) {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) { dbg_rhs[f * P + e] = R[f]; dbg_fhat[f * P + e] = Fh[f]; }
+        } else {
+            const double dt = ctrl->dt;
+            double U[9];
+            if (MODE == 1) {
+#pragma unroll
+                for (int f = 0; f < 9; ++f) U[f] = W[f] + R[f] * dt;
+            } else {
+                const double half = dt / 2;
+#pragma unroll
+                for (int f = 0; f < 9; ++f) U[f] = (W[f] + wbase[f * P + e]) * 0.5 + R[f] * half;
+            }
+#pragma unroll
+            for (int f = 0; f < 9; ++f) wout[f * P + e] = U[f];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) means[f * P + e] = (U[3 * f] + U[3 * f + 1] + U[3 * f + 2]) / 3;
+        }
+    }
+    // max |lambda| (order-free): wave max, one 64-bit atomic max per wave
+    const unsigned long long bits = (unsigned long long)__double_as_longlong(lam);
+    const unsigned long long wm = wave_max(bits);
+    if ((threadIdx.x & 63) == 0 && wm != 0ULL) atomicMax(&ctrl->maxlam[slot], wm);
+}
+
+// element means of the current state (standalone limiter calls)
+__global__ __launch_bounds__(256) void k_means(int ne, int P, const double *__restrict__ w, double *__restrict__ means)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= ne) return;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) means[f * P + e] = (w[(3 * f) * P + e] + w[(3 * f + 1) * P + e] + w[(3 * f + 2) * P + e]) / 3;
+}
+
+// node min/max of element means, ascending element order (SlopeLimiter2D.c:106-153)
+__global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means, double *__restrict__ nodemm)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.nn) return;
+    double mx0 = -99999, mx1 = -99999, mx2 = -99999, mn0 = 99999, mn1 = 99999, mn2 = 99999;
+    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
+    for (int j = b; j < en; ++j) {
+        const int el = m.n2e[j];
+        const double a0 = means[el], a1 = means[m.pitch + el], a2 = means[2 * m.pitch + el];
+        if (a0 < mn0) mn0 = a0;
+        if (a0 > mx0) mx0 = a0;
+        if (a1 < mn1) mn1 = a1;
+        if (a1 > mx1) mx1 = a1;
+        if (a2 < mn2) mn2 = a2;
+        if (a2 > mx2) mx2 = a2;
+    }
+    const int nn = m.nn;
+    nodemm[i] = mn0; nodemm[nn + i] = mn1; nodemm[2 * nn + i] = mn2;
+    nodemm[3 * nn + i] = mx0; nodemm[4 * nn + i] = mx1; nodemm[5 * nn + i] = mx2;
+}
+
+// PDop2D on one element (wetDry2D.c:66-256); zeta/qx/qy are the element's
+// 3+3+3 midpoint values, modified in place.  Returns an error code or 0.
+__device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy[3], const double zv[3],
+                                            double H0, double VZ)
+{
+    const double bz0 = 0.5 * (zv[0] + zv[1]), bz1 = 0.5 * (zv[1] + zv[2]), bz2 = 0.5 * (zv[2] + zv[0]);
+    double mh0 = z[0] + bz0, mh1 = z[1] + bz1, mh2 = z[2] + bz2;
+    double Ht[3] = {mh0 - mh1 + mh2, mh0 + mh1 - mh2, -mh0 + mh1 + mh2};
+    const double avgH = (Ht[0] + Ht[1] + Ht[2]) / 3;
+    if (Ht[0] <= H0 || Ht[1] <= H0 || Ht[2] <= H0) {
+        if (avgH <= H0) {
+            Ht[0] = avgH; Ht[1] = avgH; Ht[2] = avgH;
+        } else {
+            int mi = 0;
+            double mh = Ht[0];
+            if (Ht[1] < mh) { mi = 1; mh = Ht[1]; }
+            if (Ht[2] < mh) { mi = 2; mh = Ht[2]; }
+            int ni = 1;
+            double nh = Ht[1];
+#pragma unroll
+            for (int i = 0; i < 3; ++i) if (Ht[i] < nh && i != mi) { ni = i; nh = Ht[i]; }
+            int ri = 2;
+#pragma unroll
+            for (int i = 2; i >= 0; --i) if (i != mi && i != ni) ri = i;
+            // indexed writes in the reference's order
+            double h[3] = {Ht[0], Ht[1], Ht[2]};
+            h[mi] = H0;
+            h[ni] = fmax(H0, nh - 0.5 * (h[mi] - mh));
+            h[ri] = h[ri] - (h[mi] - mh) - (h[ni] - nh);
+            Ht[0] = h[0]; Ht[1] = h[1]; Ht[2] = h[2];
+        }
+        if (isnan(Ht[0]) || isnan(Ht[1]) || isnan(Ht[2])) return 6;
+        mh0 = 0.5 * (Ht[0] + Ht[1]);
+        mh1 = 0.5 * (Ht[1] + Ht[2]);
+        mh2 = 0.5 * (Ht[2] + Ht[0]);
+        z[0] = mh0 - bz0;
+        z[1] = mh1 - bz1;
+        z[2] = mh2 - bz2;
+    }
+    const int f0 = Ht[0] > H0, f1 = Ht[1] > H0, f2 = Ht[2] > H0;
+    if (f0 == 0 && f1 == 0 && f2 == 0) {
+        qx[0] = 0; qx[1] = 0; qx[2] = 0;
+        qy[0] = 0; qy[1] = 0; qy[2] = 0;
+    } else if (f0 == 0 || f1 == 0 || f2 == 0) {
+        const int f[3] = {f0, f1, f2};
+        double qxe[3] = {qx[0] - qx[1] + qx[2], qx[0] + qx[1] - qx[2], -qx[0] + qx[1] + qx[2]};
+        double qye[3] = {qy[0] - qy[1] + qy[2], qy[0] + qy[1] - qy[2], -qy[0] + qy[1] + qy[2]};
+        double u[3], vv[3];
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { u[i] = qxe[i] / Ht[i]; vv[i] = qye[i] / Ht[i]; }
+        int npos = 0;
+        double du = 0, dv = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            npos += f[i];
+            du += u[i] * (double)(1 - f[i]) * (double)(fabs(qxe[i]) > VZ);
+            dv += vv[i] * (double)(1 - f[i]) * (double)(fabs(qye[i]) > VZ);
+        }
+        int err = 0;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            u[i] = (double)f[i] * (u[i] + du / npos);
+            vv[i] = (double)f[i] * (vv[i] + dv / npos);
+            if (isnan(u[i]) || isnan(vv[i])) err = 7;
+        }
+        if (err) return err;
+#pragma unroll
+        for (int i = 0; i < 3; ++i) { qxe[i] = Ht[i] * u[i]; qye[i] = Ht[i] * vv[i]; }
+#pragma unroll
+        for (int i = 0; i < 3; ++i) {
+            const int i2 = (i + 1) % 3;
+            qx[i] = 0.5 * (qxe[i] + qxe[i2]);
+            qy[i] = 0.5 * (qye[i] + qye[i2]);
+        }
+    }
+    return 0;
+}
+
+// limiter element pass (SlopeLimiter2D.c:157-343), optionally followed by PDop2D
+template <bool PDOP>
+__global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w, const double *__restrict__ nodemm,
+                                               unsigned char *__restrict__ modify_out, Ctrl *ctrl)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    const int P = m.pitch, nn = m.nn;
+    const double H0 = ph.h0;
+    int nd[3];
+    double zv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
+    double S[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) S[f] = w[f * P + e];
+    int modify = 0;
+#pragma unroll
+    for (int iv = 0; iv < 3; ++iv) {
+        const double w1 = S[3 * iv], w2 = S[3 * iv + 1], w3 = S[3 * iv + 2];
+        double V[3] = {w1 - w2 + w3, w1 + w2 - w3, -w1 + w2 + w3};
+        double Vmin[3], Vmax[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { Vmin[k] = nodemm[iv * nn + nd[k]]; Vmax[k] = nodemm[(3 + iv) * nn + nd[k]]; }
+        const double avg = (w1 + w2 + w3) / 3;
+        if (iv == 0) {
+            const double Ha = V[0] + zv[0], Hb = V[1] + zv[1], Hc = V[2] + zv[2];
+            const double Hm0 = Vmin[0] + zv[0], Hm1 = Vmin[1] + zv[1], Hm2 = Vmin[2] + zv[2];
+            if (Ha > H0 && Hb > H0 && Hc > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) V[k] = fmax(DGR_MIN(V[k], Vmax[k]), Vmin[k]);
+#pragma unroll
+        for (int ll = 0; ll < 3; ++ll) {
+            const double sumloc = (V[0] + V[1] + V[2]) / 3;
+            double sumdiff = (sumloc - avg) * 3;
+            const int sd = isign(sumdiff);
+            const double diff[3] = {(V[0] - avg) * sd, (V[1] - avg) * sd, (V[2] - avg) * sd};
+            double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                const double dv = fmax(1.0, kdp);
+                double redfac;
+                if (diff[k] > 0) { redfac = sumdiff * sd / dv; kdp -= 1; }
+                else redfac = 0;
+                const double redmax = (sd > 0) ? V[k] - Vmin[k] : Vmax[k] - V[k];
+                redfac = DGR_MIN(redfac, redmax);
+                sumdiff -= redfac * sd;
+                V[k] -= redfac * sd;
+            }
+        }
+        const double mid1 = (V[0] + V[1]) / 2, mid2 = (V[1] + V[2]) / 2, mid3 = (V[2] + V[0]) / 2;
+        if (iv == 0) {
+            const double Ha = V[0] + zv[0], Hb = V[1] + zv[1], Hc = V[2] + zv[2];
+            if (Ha < H0 || Hb < H0 || Hc < H0) modify = 0;
+        }
+        if (modify) { S[3 * iv] = mid1; S[3 * iv + 1] = mid2; S[3 * iv + 2] = mid3; }
+    }
+    if (PDOP) {
+        const int err = pdop_element(&S[0], &S[3], &S[6], zv, H0, ph.velzero);
+        if (err) raise_error(ctrl, err, e);
+    }
+#pragma unroll
+    for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
+    if (modify_out) modify_out[e] = (unsigned char)modify;
+}
+
+// standalone PDop2D
+__global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, Ctrl *ctrl)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    const int P = m.pitch;
+    double zv[3], S[9];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zv[k] = m.nodes[m.ev[k * P + e]].z;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) S[f] = w[f * P + e];
+    const int err = pdop_element(&S[0], &S[3], &S[6], zv, ph.h0, ph.velzero);
+    if (err) raise_error(ctrl, err, e);
+#pragma unroll
+    for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
+}
+
+// wetDryStatus2D (wetDry2D.c:16-60)
+__global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const double *__restrict__ w, int *__restrict__ wd)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    const int P = m.pitch;
+    const double H0 = ph.h0;
+    const double a = w[e], b = w[P + e], c = w[2 * P + e];
+    const double ze[3] = {a - b + c, a + b - c, -a + b + c};
+    double zz[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zz[k] = m.nodes[m.ev[k * P + e]].z;
+    const double h[3] = {ze[0] + zz[0], ze[1] + zz[1], ze[2] + zz[2]};
+    const double avgH = (h[0] + h[1] + h[2]) / 3;
+    int mi = 0;
+    double mh = h[0];
+    if (h[1] > mh) { mi = 1; mh = h[1]; }
+    if (h[2] > mh) { mi = 2; mh = h[2]; }
+    const double zmax = ze[mi];
+    double zmin = zz[0];
+    if (zz[1] < zmin) zmin = zz[1];
+    if (zz[2] < zmin) zmin = zz[2];
+    const int t1 = (avgH - H0) > 0;
+    const int t2 = (zmax - (H0 - zmin)) > 0;
+    const int old = wd[e];
+    wd[e] = (old == -1) ? t1 * t2 : old * t1 + (1 - old) * t1 * t2;
+}
+
+// per-step bookkeeping (time_evolution.c:311-402): begin evaluates the tide at
+// the step's start time; end advances t and sets the next dt from stage 2's
+// max |lambda|.
+__global__ void k_step_begin(PhysDev ph, Ctrl *c)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    if (ph.tide_code)
+        c->tide = ph.tide_datum + ph.tide_amp * dgr_sin(6.283185307179586 * c->t / ph.tide_period + ph.tide_phase);
+    c->maxlam[0] = 0ULL;
+    c->maxlam[1] = 0ULL;
+}
+
+__global__ void k_step_end(PhysDev ph, Ctrl *c)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const double time = c->t + c->dt;
+    const double ml = __longlong_as_double((long long)c->maxlam[1]);
+    double h = ph.cfl * ph.min_edge / ml;
+    if ((time + h) > ph.final_time) h = ph.final_time - time;
+    c->t = time;
+    c->dt = h;
+}
+
+__global__ void k_reset_maxlam(Ctrl *c)
+{
+    if (threadIdx.x == 0 && blockIdx.x == 0) { c->maxlam[0] = 0ULL; c->maxlam[1] = 0ULL; }
+}
+
+__global__ void k_math_selftest(int n, const double *x, const double *y, double *out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const double a = x[i], b = y[i];
+    out[5 * i + 0] = sqrt(a);
+    out[5 * i + 1] = a / b;
+    out[5 * i + 2] = dgr_pow43(a);
+    out[5 * i + 3] = dgr_sin(a);
+    out[5 * i + 4] = fma(a, b, a);
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+static inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
+
+void launch_stage(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
+                  const double *wbase, const double *bzeta, const double *bqn, const int *wd, double *means,
+                  Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat, hipStream_t s)
+{
+    const dim3 g = grid_for(m.ne), b(256);
+#define DGR_LS(MD, WDF) \
+    hipLaunchKernelGGL((k_stage<MD, WDF>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
+    if (wetdry) {
+        if (mode == 0) DGR_LS(0, true); else if (mode == 1) DGR_LS(1, true); else DGR_LS(2, true);
+    } else {
+        if (mode == 0) DGR_LS(0, false); else if (mode == 1) DGR_LS(1, false); else DGR_LS(2, false);
+    }
+#undef DGR_LS
+}
+
+void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_means, grid_for(m.ne), dim3(256), 0, s, m.ne, m.pitch, w, means);
+}
+
+void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_node_minmax, grid_for(m.nn), dim3(256), 0, s, m, means, nodemm);
+}
+
+void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
+                  unsigned char *modify, Ctrl *ctrl, hipStream_t s)
+{
+    if (pdop) hipLaunchKernelGGL(k_limit<true>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl);
+    else hipLaunchKernelGGL(k_limit<false>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl);
+}
+
+void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pdop, grid_for(m.ne), dim3(256), 0, s, m, ph, w, ctrl);
+}
+
+void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_wd_status, grid_for(m.ne), dim3(256), 0, s, m, ph, w, wd);
+}
+
+void launch_step_begin(const PhysDev &ph, Ctrl *c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, ph, c);
+}
+
+void launch_step_end(const PhysDev &ph, Ctrl *c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, ph, c);
+}
+
+void launch_reset_maxlam(Ctrl *c, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_reset_maxlam, dim3(1), dim3(64), 0, s, c);
+}
+
+void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_math_selftest, grid_for(n), dim3(256), 0, s, n, x, y, out);
+}
+
+}  // namespace dgr

@@ -1,0 +1,127 @@
+// dgr_math.h -- deterministic FP64 helpers for the gfx950 kernels.
+//
+// Every function here uses only IEEE-754 basic operations (+ - * / sqrt) and
+// fused multiply-add, so it returns the same bits on any IEEE machine.
+//
+//  dgr_pow43(x)  x^(4./3), the Manning term of the reference's friction
+//                (2DCode/2DInnerProducts.c:318).  The reference calls glibc
+//                pow(), which is not correctly rounded (about 0.08% of results
+//                are 1 ulp off); no GPU routine can reproduce those bits.  This
+//                one is accurate to ~1e-22 before the final rounding, i.e.
+//                correctly rounded except in vanishingly rare cases.
+//  dgr_sin(x)    sine for the tidal boundary forcing (no reference code).
+//
+// Constants: see oracle/gen_constants.py for how each value was derived.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dgr {
+
+// cubic least-squares seed for t^(-1/3) on [1,8) (max rel. error 1.8e-2)
+constexpr double P43_C0 = 1.2316035297857837;
+constexpr double P43_C1 = -0.29172246132464164;
+constexpr double P43_C2 = 0.044484938428368009;
+constexpr double P43_C3 = -0.0024458486365437917;
+constexpr double THIRD = 0.33333333333333331;          // fl(1/3)
+constexpr double DELTA43 = 7.4014868308343765e-17;     // 4/3 - fl(4/3) = 2^-52/3
+constexpr double LN2 = 0.69314718055994529;
+
+__host__ __device__ inline double pow2i(int k)  // 2^k, -1022 <= k <= 1023
+{
+    return __longlong_as_double((long long)((uint64_t)(k + 1023) << 52));
+}
+
+// x = m 2^e, e = 3q + r, t = m 2^r in [1,8): x^(4/3) = 2^(4q) t t^(1/3).
+// t^(-1/3): seed + 3 Newton steps y += y(1 - t y^3)/3 (no division);
+// t^(1/3) = t y^2, refined once in double-double; the exponent's representation
+// error applied as the factor (1 - DELTA43 ln x).
+__device__ inline double dgr_pow43(double x)
+{
+    if (x != x) return x;
+    if (x <= 0.0) return (x == 0.0) ? 0.0 : __longlong_as_double(0x7ff8000000000000LL);
+    if (x > 1.7976931348623157e308) return x;
+    uint64_t b = (uint64_t)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) {
+        double xs = x * 18014398509481984.0;  // 2^54
+        b = (uint64_t)__double_as_longlong(xs);
+        e = (int)((b >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    const double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+    const int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);
+    const int r = e - 3 * q;
+    const double t = m * (double)(1 << r);
+
+    double y = fma(fma(fma(P43_C3, t, P43_C2), t, P43_C1), t, P43_C0);
+#pragma unroll
+    for (int it = 0; it < 3; ++it) {
+        double w = y * y;
+        w = w * y;
+        const double h = fma(-t, w, 1.0);
+        y = fma(y * h, THIRD, y);
+    }
+    const double c = (t * y) * y;
+    const double a_hi = c * c;
+    const double a_lo = fma(c, c, -a_hi);
+    const double b_hi = a_hi * c;
+    const double b_lo = fma(a_hi, c, -b_hi) + a_lo * c;
+    const double d = (t - b_hi) - b_lo;
+    const double dc = (d * (y * y)) * THIRD;
+    const double p_hi = t * c;
+    const double p_lo = fma(t, c, -p_hi) + t * dc;
+
+    double mm = m;
+    int ee = e;
+    if (mm > 1.4142135623730951) { mm = mm * 0.5; ee += 1; }
+    const double s = (mm - 1.0) / (mm + 1.0);
+    const double s2 = s * s;
+    const double lnm = 2.0 * s * fma(fma(fma(s2, 0.14285714285714285, 0.2), s2, 0.33333333333333331), s2, 1.0);
+    const double lnx = fma((double)ee, LN2, lnm);
+    const double eps = DELTA43 * lnx;
+
+    const double res = p_hi + fma(-p_hi, eps, p_lo);
+    const double sc = pow2i(2 * q);
+    return (res * sc) * sc;
+}
+
+// Cody-Waite reduction by pi/2 (33+33+53-bit split of pi/2) and Taylor
+// polynomials through x^15 (sin) / x^16 (cos) on [-pi/4, pi/4].
+__device__ inline double dgr_sin(double x)
+{
+    if (!(x - x == 0.0)) return x - x;
+    const double TWO_OVER_PI = 0.63661977236758138;
+    const double PIO2_1 = 1.5707963267341256e+00;
+    const double PIO2_2 = 6.077100506303966e-11;
+    const double PIO2_3 = 2.0222662487959506e-21;
+    const double kd = rint(x * TWO_OVER_PI);
+    double r = fma(-kd, PIO2_1, x);
+    r = fma(-kd, PIO2_2, r);
+    r = fma(-kd, PIO2_3, r);
+    const long long k = (long long)kd;
+    const double r2 = r * r;
+    double sp = fma(r2, -7.6471637318198164e-13, 1.6059043836821613e-10);
+    sp = fma(r2, sp, -2.5052108385441720e-08);
+    sp = fma(r2, sp, 2.7557319223985893e-06);
+    sp = fma(r2, sp, -1.9841269841269841e-04);
+    sp = fma(r2, sp, 8.3333333333333332e-03);
+    sp = fma(r2, sp, -1.6666666666666666e-01);
+    const double sv = fma(r * r2, sp, r);
+    double cp = fma(r2, 4.7794773323873853e-14, -1.1470745597729725e-11);
+    cp = fma(r2, cp, 2.08767569878681e-09);
+    cp = fma(r2, cp, -2.7557319223985888e-07);
+    cp = fma(r2, cp, 2.4801587301587302e-05);
+    cp = fma(r2, cp, -1.3888888888888889e-03);
+    cp = fma(r2, cp, 4.1666666666666664e-02);
+    cp = fma(r2, cp, -0.5);
+    const double cv = fma(r2, cp, 1.0);
+    switch ((int)(k & 3)) {
+    case 0: return sv;
+    case 1: return cv;
+    case 2: return -sv;
+    default: return -cv;
+    }
+}
+
+}  // namespace dgr

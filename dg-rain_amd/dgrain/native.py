@@ -1,0 +1,260 @@
+"""ctypes binding of libdgrain.so (include/dgrain.h).
+
+The library is the product: there is no Python or CPU fallback.  Loading
+fails loudly when the shared object has not been built.
+"""
+import ctypes as C
+import math
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(_HERE)                      # .../dg-rain_amd
+REPO_ROOT = os.path.dirname(PKG_ROOT)
+LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdgrain.so")
+
+DGR_OK = 0
+STATUS = {0: "ok", 1: "invalid argument", 2: "inconsistent mesh", 3: "HIP runtime error",
+          4: "NaN flux or wet/dry failure", 5: "out of memory", 6: "no HIP device", 7: "unsupported"}
+
+
+class DgrError(RuntimeError):
+    def __init__(self, code, what=""):
+        self.code = code
+        super().__init__("libdgrain: %s%s" % (STATUS.get(code, "status %d" % code), (" (%s)" % what) if what else ""))
+
+
+class MeshView(C.Structure):
+    _fields_ = [("num_el", C.c_int), ("num_nodes", C.c_int), ("num_edges", C.c_int),
+                ("x", C.POINTER(C.c_double)), ("y", C.POINTER(C.c_double)),
+                ("z", C.POINTER(C.c_double)), ("n", C.POINTER(C.c_double)),
+                ("elto_vert", C.POINTER(C.c_int)), ("edg_to_vert", C.POINTER(C.c_int)),
+                ("edg_to_els", C.POINTER(C.c_int)), ("bdry", C.POINTER(C.c_int))]
+
+
+class Params(C.Structure):
+    _fields_ = [("g", C.c_double), ("h0", C.c_double), ("velzero", C.c_double), ("cfl", C.c_double),
+                ("dt0", C.c_double), ("final_time", C.c_double), ("rain", C.c_double),
+                ("wetdry", C.c_int), ("tide_code", C.c_int),
+                ("tide_datum", C.c_double), ("tide_amp", C.c_double), ("tide_period", C.c_double),
+                ("tide_phase", C.c_double), ("record_modify", C.c_int)]
+
+
+class Report(C.Structure):
+    _fields_ = [("t", C.c_double), ("dt", C.c_double), ("max_lambda", C.c_double),
+                ("error_code", C.c_int), ("error_where", C.c_int)]
+
+
+_lib = None
+
+
+def load():
+    """Load libdgrain.so (raises if it is not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError("libdgrain.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+    lib = C.CDLL(LIB_PATH)
+    dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
+    sig = {
+        "dgr_params_default": (None, [C.POINTER(Params)]),
+        "dgr_status_string": (C.c_char_p, [C.c_int]),
+        "dgr_build_topology": (C.c_int, [C.c_int, C.c_int, ip, dp, dp, C.c_int, ip, C.c_int, ip, C.c_int, ip,
+                                         ip, ip, ip, ip, dp]),
+        "dgr_create": (C.c_int, [C.POINTER(MeshView), C.POINTER(Params), C.POINTER(vp)]),
+        "dgr_destroy": (C.c_int, [vp]),
+        "dgr_set_state": (C.c_int, [vp, dp, dp, dp]),
+        "dgr_get_state": (C.c_int, [vp, dp, dp, dp]),
+        "dgr_set_wd": (C.c_int, [vp, ip]),
+        "dgr_get_wd": (C.c_int, [vp, ip]),
+        "dgr_get_modify": (C.c_int, [vp, C.c_int, C.POINTER(C.c_ubyte)]),
+        "dgr_set_boundary": (C.c_int, [vp, dp, dp]),
+        "dgr_set_time": (C.c_int, [vp, C.c_double, C.c_double]),
+        "dgr_compute_rhs": (C.c_int, [vp, dp, dp, dp, dp, dp, dp, dp]),
+        "dgr_limit": (C.c_int, [vp]),
+        "dgr_wetdry_status": (C.c_int, [vp]),
+        "dgr_pdop": (C.c_int, [vp]),
+        "dgr_one_time_step": (C.c_int, [vp, C.c_double, C.c_double]),
+        "dgr_run": (C.c_int, [vp, C.c_int, C.POINTER(Report)]),
+        "dgr_last_error": (C.c_int, [vp, ip, ip]),
+        "dgr_sync": (C.c_int, [vp]),
+        "dgr_device_state": (C.c_int, [vp, C.POINTER(dp), ip]),
+        "dgr_math_selftest": (C.c_int, [C.c_int, dp, dp, dp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib
+
+
+def exported_symbols():
+    """Names declared in include/dgrain.h."""
+    import re
+    with open(os.path.join(REPO_ROOT, "include", "dgrain.h")) as f:
+        text = f.read()
+    decl = r"^\s*(?:dgr_status|void|const char \*|int|double)\s*\**\s*(dgr_[a-z0-9_]+)\s*\("
+    return sorted(set(re.findall(decl, text, re.M)))
+
+
+def _dp(a):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a):
+    return a.ctypes.data_as(C.POINTER(C.c_int))
+
+
+def check(rc, what=""):
+    if rc != DGR_OK:
+        raise DgrError(rc, what)
+
+
+def default_params(**kw):
+    p = Params()
+    load().dgr_params_default(C.byref(p))
+    for k, v in kw.items():
+        if not hasattr(p, k):
+            raise KeyError(k)
+        setattr(p, k, v)
+    return p
+
+
+def build_topology(elto_vert, x, y, in1=None, in2=None, out=None):
+    """extract_mesh_attributes() in O(N) (see dgr_build_topology)."""
+    lib = load()
+    ev = np.ascontiguousarray(elto_vert, dtype=np.int32).ravel()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    ne, nn = ev.size // 3, x.size
+    lists = [np.ascontiguousarray(a if a is not None else np.zeros(0), dtype=np.int32).ravel()
+             for a in (in1, in2, out)]
+    ned_max = nn + ne - 1
+    e2v = np.empty(2 * ned_max, np.int32)
+    e2e = np.empty(2 * ned_max, np.int32)
+    bdry = np.empty(ned_max, np.int32)
+    ned = C.c_int(0)
+    mel = C.c_double(0)
+    rc = lib.dgr_build_topology(ne, nn, _ip(ev), _dp(x), _dp(y),
+                                lists[0].size // 2, _ip(lists[0]), lists[1].size // 2, _ip(lists[1]),
+                                lists[2].size // 2, _ip(lists[2]),
+                                _ip(e2v), _ip(e2e), _ip(bdry), C.byref(ned), C.byref(mel))
+    check(rc, "dgr_build_topology")
+    return e2v, e2e, bdry, mel.value
+
+
+class Solver:
+    """One libdgrain handle: the reference's global-state step as an object.
+
+    Method names follow the reference functions they replace
+    (compute2DL, SlopeLimiter, wetDryStatus2D, PDop2D, oneTimeStep2D)."""
+
+    def __init__(self, mesh, params=None, **kw):
+        lib = load()
+        self.lib = lib
+        self.mesh = mesh
+        self.params = params if params is not None else default_params(**kw)
+        self._keep = [np.ascontiguousarray(a) for a in
+                      (mesh.x, mesh.y, mesh.z, mesh.n, mesh.elto_vert, mesh.edg_to_vert, mesh.edg_to_els, mesh.bdry)]
+        x, y, z, n, ev, e2v, e2e, bd = self._keep
+        mv = MeshView(mesh.num_el, mesh.num_nodes, mesh.num_edges, _dp(x), _dp(y), _dp(z), _dp(n),
+                      _ip(ev), _ip(e2v), _ip(e2e), _ip(bd))
+        h = C.c_void_p()
+        check(lib.dgr_create(C.byref(mv), C.byref(self.params), C.byref(h)), "dgr_create")
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.dgr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # -- state -------------------------------------------------------------
+    def set_state(self, zeta, qx, qy):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (zeta, qx, qy)]
+        check(self.lib.dgr_set_state(self.h, *[_dp(v) for v in a]), "dgr_set_state")
+
+    def get_state(self):
+        ne3 = 3 * self.mesh.num_el
+        a = [np.empty(ne3) for _ in range(3)]
+        check(self.lib.dgr_get_state(self.h, *[_dp(v) for v in a]), "dgr_get_state")
+        return tuple(a)
+
+    def set_wd(self, wd):
+        a = np.ascontiguousarray(wd, dtype=np.int32)
+        check(self.lib.dgr_set_wd(self.h, _ip(a)))
+
+    def get_wd(self):
+        a = np.empty(self.mesh.num_el, np.int32)
+        check(self.lib.dgr_get_wd(self.h, _ip(a)))
+        return a
+
+    def get_modify(self, stage):
+        a = np.empty(self.mesh.num_el, np.uint8)
+        check(self.lib.dgr_get_modify(self.h, stage, a.ctypes.data_as(C.POINTER(C.c_ubyte))))
+        return a
+
+    def set_boundary(self, bzeta, bqn):
+        a = np.ascontiguousarray(bzeta, dtype=np.float64)
+        b = np.ascontiguousarray(bqn, dtype=np.float64)
+        check(self.lib.dgr_set_boundary(self.h, _dp(a), _dp(b)))
+
+    def set_time(self, t, dt):
+        check(self.lib.dgr_set_time(self.h, float(t), float(dt)))
+
+    # -- operators ---------------------------------------------------------
+    def compute2DL(self):
+        """RHS, edge fluxes and max_lambda of the current state (compute2DL.c:12-413)."""
+        ne3 = 3 * self.mesh.num_el
+        r = [np.empty(ne3) for _ in range(6)]
+        ml = C.c_double(0)
+        rc = self.lib.dgr_compute_rhs(self.h, *[_dp(v) for v in r], C.byref(ml))
+        check(rc, "dgr_compute_rhs")
+        return r[:3], r[3:], ml.value
+
+    def SlopeLimiter(self):
+        check(self.lib.dgr_limit(self.h), "dgr_limit")
+
+    def wetDryStatus2D(self):
+        check(self.lib.dgr_wetdry_status(self.h), "dgr_wetdry_status")
+
+    def PDop2D(self):
+        check(self.lib.dgr_pdop(self.h), "dgr_pdop")
+
+    def oneTimeStep2D(self, t, dt):
+        check(self.lib.dgr_one_time_step(self.h, float(t), float(dt)), "dgr_one_time_step")
+
+    def run(self, nsteps):
+        rep = Report()
+        rc = self.lib.dgr_run(self.h, int(nsteps), C.byref(rep))
+        check(rc, "dgr_run (error %d at %d)" % (rep.error_code, rep.error_where))
+        return rep
+
+    def sync(self):
+        check(self.lib.dgr_sync(self.h))
+
+    def device_state(self):
+        p = C.POINTER(C.c_double)()
+        pitch = C.c_int(0)
+        check(self.lib.dgr_device_state(self.h, C.byref(p), C.byref(pitch)))
+        return C.cast(p, C.c_void_p).value, pitch.value
+
+
+def math_selftest(x, y):
+    lib = load()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    out = np.empty((x.size, 5))
+    check(lib.dgr_math_selftest(x.size, _dp(x), _dp(y), _dp(out)), "dgr_math_selftest")
+    return out
+
+
+INF = math.inf

@@ -1,0 +1,158 @@
+/*
+ * dgrain.h -- C ABI of libdgrain.so, the MI355X (gfx950) implementation of the
+ * DG-RAIN 2D p=1 discontinuous-Galerkin shallow-water step.
+ *
+ * The reference (evalseth/DG-RAIN, directory 2DCode) keeps the mesh and the
+ * state in globals (2DCode/MeshAttributes.h:5-42) and exposes the step as
+ * void functions over them.  This ABI carries exactly the same arrays, with
+ * the same layouts and index conventions, through a handle:
+ *
+ *   nodes    x, y, z, n            double[num_nodes]   (z = depth, positive down;
+ *                                                        n = Manning coefficient)
+ *   elements elto_vert             int[3*num_el]        0-based vertex ids
+ *   edges    edg_to_vert           int[2*num_edges]     v0 < v1
+ *            edg_to_els            int[2*num_edges]     (el_in, el_ex); el_ex == el_in
+ *                                                        on a boundary edge
+ *            bdry                  int[num_edges]       BdryPrescribed code
+ *   state    zeta, qx, qy          double[3*num_el]     value at the midpoint of
+ *                                                        local edge k at el*3+k
+ *   wet/dry  wd                    int[num_el]          -1 unset, 0 dry, 1 wet
+ *
+ * The library copies everything to device memory at create / set time; the
+ * caller keeps ownership of its arrays.  All functions return a dgr_status
+ * (the reference prints and calls exit(); see 2DCode/compute2DL.c:245-249).
+ * One handle binds one HIP stream on the current device; a handle must not
+ * be used from two threads at once.
+ *
+ * Entry points and the reference interface each one replaces:
+ *   dgr_build_topology   extract_mesh_attributes()  extract_mesh_attributes.c:143-361
+ *   dgr_create           store_mesh()+initialize()  extract_mesh_attributes.c:39-141,
+ *                                                   initialize.c:29-128 (allocation part)
+ *   dgr_set_state/…      the zeta/Qx/Qy/WD globals  MeshAttributes.h:23-25,42
+ *   dgr_set_boundary     boundary_conditions()      boundary_conditions.c:10-83
+ *   dgr_compute_rhs      compute2DL(time)           compute2DL.c:12-413
+ *   dgr_limit            SlopeLimiter()             SlopeLimiter2D.c:10-359
+ *   dgr_wetdry_status    wetDryStatus2D()           wetDry2D.c:16-60
+ *   dgr_pdop             PDop2D()                   wetDry2D.c:63-260
+ *   dgr_one_time_step    oneTimeStep2D(time, dt)    time_evolution.c:10-246
+ *   dgr_run              the step loop of time_evolution(FinalTime)
+ *                                                   time_evolution.c:311-402
+ */
+#ifndef DGRAIN_H
+#define DGRAIN_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum {
+    DGR_OK = 0,
+    DGR_ERR_ARG = 1,        /* bad argument (null pointer, size, stage) */
+    DGR_ERR_MESH = 2,       /* inconsistent mesh (compute2DL.c:51-55,63-67;
+                               extract_mesh_attributes.c:252-257) */
+    DGR_ERR_HIP = 3,        /* HIP runtime error */
+    DGR_ERR_NAN = 4,        /* NaN flux or wet/dry failure (compute2DL.c:245-302,
+                               wetDry2D.c:142-146,233-237); see dgr_last_error */
+    DGR_ERR_NOMEM = 5,
+    DGR_ERR_NODEV = 6,      /* no HIP device */
+    DGR_ERR_UNSUPPORTED = 7
+} dgr_status;
+
+typedef struct dgr_handle dgr_handle;
+
+typedef struct {
+    int num_el, num_nodes, num_edges;
+    const double *x, *y, *z, *n;
+    const int *elto_vert;
+    const int *edg_to_vert;
+    const int *edg_to_els;
+    const int *bdry;
+} dgr_mesh_view;
+
+typedef struct {
+    double g;            /* 9.81     initialize.c:22 */
+    double h0;           /* 1e-3     initialize.c:26 */
+    double velzero;      /* 1e-2     initialize.c:27 */
+    double cfl;          /* 0.1      time_evolution.c:399 */
+    double dt0;          /* 1e-7     time_evolution.c:255 */
+    double final_time;   /* dt clamp time_evolution.c:400-401 (HUGE_VAL: none) */
+    double rain;         /* m/s added to every zeta DOF of the RHS (extension; 0 = reference) */
+    int wetdry;          /* 0: 2DCode as shipped; 1: wet/dry on (ChanNet -DWDON ordering) */
+    int tide_code;       /* BdryPrescribed code driven by the tide; 0 = none (extension) */
+    double tide_datum, tide_amp, tide_period, tide_phase;
+    int record_modify;   /* keep the limiter's per-element activation flags */
+} dgr_params;
+
+typedef struct {
+    double t;            /* time after the last step */
+    double dt;           /* dt for the next step */
+    double max_lambda;   /* max |eigenvalue| from the last stage-2 RHS (compute2DL.c:310-313) */
+    int error_code;      /* 0 or the first error raised on the device */
+    int error_where;     /* edge or element id of that error */
+} dgr_report;
+
+/* Reference defaults (2DCode constants, no extensions). */
+void dgr_params_default(dgr_params *p);
+const char *dgr_status_string(dgr_status s);
+
+/* O(N) restatement of extract_mesh_attributes() (extract_mesh_attributes.c:143-361):
+ * same edge numbering, EdgtoEls and BdryPrescribed rules.  Boundary lists are node
+ * pairs (0-based) of the first inflow, second inflow and outflow sections of the
+ * mesh file (extract_mesh_attributes.c:90-136).  Output arrays must hold
+ * 2*(num_nodes+num_el-1) and num_nodes+num_el-1 ints.  Host only, no device use.
+ * *num_edges_out receives NumEdges; DGR_ERR_MESH if Euler's count fails. */
+dgr_status dgr_build_topology(int num_el, int num_nodes, const int *elto_vert,
+                              const double *x, const double *y,
+                              int n_in1, const int *in1, int n_in2, const int *in2,
+                              int n_out, const int *out,
+                              int *edg_to_vert, int *edg_to_els, int *bdry,
+                              int *num_edges_out, double *min_edge_length);
+
+/* Handle lifetime.  Validates the mesh and builds the element-centric neighbour
+ * tables.  Sets the reference's default boundary values (boundary_conditions.c)
+ * and WD = -1 (initialize.c:126). */
+dgr_status dgr_create(const dgr_mesh_view *mesh, const dgr_params *params, dgr_handle **out);
+dgr_status dgr_destroy(dgr_handle *h);
+
+/* Host-array state access in the reference layout (el*3+k). */
+dgr_status dgr_set_state(dgr_handle *h, const double *zeta, const double *qx, const double *qy);
+dgr_status dgr_get_state(dgr_handle *h, double *zeta, double *qx, double *qy);
+dgr_status dgr_set_wd(dgr_handle *h, const int *wd);
+dgr_status dgr_get_wd(dgr_handle *h, int *wd);
+/* limiter activation flags of the last stage-`stage` limiter call (stage 0 or 1);
+ * needs params.record_modify. */
+dgr_status dgr_get_modify(dgr_handle *h, int stage, unsigned char *modify);
+dgr_status dgr_set_boundary(dgr_handle *h, const double *bzeta, const double *bqn);
+dgr_status dgr_set_time(dgr_handle *h, double t, double dt);
+
+/* compute2DL: RHS (and optionally the edge fluxes Fhat{1,2,3}dotn) of the
+ * current state, reference layout; max_lambda may be NULL. */
+dgr_status dgr_compute_rhs(dgr_handle *h, double *rhs_zeta, double *rhs_qx, double *rhs_qy,
+                           double *fhat1, double *fhat2, double *fhat3, double *max_lambda);
+dgr_status dgr_limit(dgr_handle *h);
+dgr_status dgr_wetdry_status(dgr_handle *h);
+dgr_status dgr_pdop(dgr_handle *h);
+
+/* oneTimeStep2D(time, dt) with a host-given dt. */
+dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt);
+/* nsteps steps of the time_evolution loop with the dt rule computed on the
+ * device (dt = cfl*minEdgLength/max_lambda, clamped to final_time), starting
+ * from the time/dt held by the handle (dgr_set_time; initially 0 and dt0). */
+dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *report);
+dgr_status dgr_last_error(dgr_handle *h, int *code, int *where);
+
+/* Synchronise the handle's stream. */
+dgr_status dgr_sync(dgr_handle *h);
+
+/* Device-resident state: pointer to the 9 SoA planes [zeta0..2, qx0..2, qy0..2]
+ * of `pitch` doubles each (pitch >= num_el).  Valid until dgr_destroy. */
+dgr_status dgr_device_state(dgr_handle *h, double **planes, int *pitch);
+
+/* Self-test of the device math against host-supplied inputs: out[i] =
+ * {sqrt(x), x/y, dgr_pow43(x), dgr_sin(x), fma(x,y,x)} for each pair. */
+dgr_status dgr_math_selftest(int n, const double *x, const double *y, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1060 @@
+/*
+ * dgswe_oracle.c -- CPU oracle (test infrastructure only; see dgswe_oracle.h).
+ *
+ * Plain C restatement of the reference 2D p=1 DG shallow-water step.  Every
+ * floating-point expression keeps the reference's evaluation order, because
+ * the oracle is held to bit-identity with the compiled reference.
+ *
+ * How each function is pinned (checked by tests/):
+ *   orc_roe_flux          == 2DCode RoeFlux (numericalFlux2D.c:6-108), bitwise,
+ *                            random inputs; consistency F(u,u)=F(u).n and
+ *                            mirror symmetry as closed forms.
+ *   orc_inner_products    == 2DCode Compute2DInnerProducts (2DInnerProducts.c:8-570),
+ *                            bitwise (ORC_MATH_REF); SI_k = |e_k| Fhat_k and
+ *                            lake-at-rest balance as closed forms.
+ *   orc_compute_L         == 2DCode compute2DL (compute2DL.c:12-413), bitwise, for
+ *                            wetdry=0; == ChanNet compute2DL built -DWDON
+ *                            (ChanNet/compute2DL.c) for wetdry=1 on meshes whose
+ *                            edges are walls or interior.
+ *   orc_slope_limiter     == SlopeLimiter (SlopeLimiter2D.c:10-359), bitwise.
+ *   orc_wetdry_status     == wetDryStatus2D (wetDry2D.c:16-60), bitwise.
+ *   orc_pdop              == PDop2D (wetDry2D.c:63-260), bitwise.
+ *   orc_one_time_step     == oneTimeStep2D (time_evolution.c:10-246) composed from
+ *                            the compiled reference operators by a harness (the
+ *                            reference file itself needs GSL, absent here); the
+ *                            GSL vector ops it replaces are exact IEEE elementwise
+ *                            scale/add.
+ *   orc_build_topology    == extract_mesh_attributes (extract_mesh_attributes.c:143-361),
+ *                            all arrays identical on small meshes.
+ *   orc_pow43             vs quad-precision powq: < 1 ulp on 1e6 samples.
+ *   orc_sin               vs libm sin: < 2 ulp (the tide forcing has no
+ *                            reference code; this function is its definition).
+ *   tide / rain / cfl     extensions with no reference code: pinned by closed
+ *                            forms (closed basin under rain rises at exactly R,
+ *                            tide value equals datum + amp*orc_sin(...)).
+ */
+#include "dgswe_oracle.h"
+
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* libm pow through a volatile pointer: gcc folds pow(x,2) into x*x at -O1
+ * and above, while the reference built with its own flags calls pow(). */
+static double (*volatile orc_libm_pow)(double, double) = pow;
+
+#define ORC_MAX(a, b) (((a) > (b)) ? (a) : (b))   /* mathfunctions.h:8-10 */
+#define ORC_MIN(a, b) (((a) < (b)) ? (a) : (b))   /* mathfunctions.h:12-14 */
+
+struct orc_state {
+    orc_mesh m;
+    orc_params p;
+    int nthreads;
+    double min_edge_length;
+    double max_lambda;
+    int err_code, err_where;
+    /* state, reference layout el*3+k */
+    double *zeta, *qx, *qy;
+    double *f1, *f2, *f3;           /* Fhat{1,2,3}dotn */
+    double *rz, *rqx, *rqy;         /* RHSZeta/Qx/Qy */
+    double *bzeta, *bqn;
+    int *wd;
+    unsigned char *modify[2];
+    /* node -> element adjacency, ascending element order (SlopeLimiter2D.c:39-89) */
+    int *n2e_off, *n2e;
+    double *nmin, *nmax;            /* [3*num_nodes] zeta,qx,qy */
+    /* RK scratch */
+    double *w0, *w1;
+};
+
+/* mathfunctions.c:4-11 */
+static int orc_sign(double a)
+{
+    if (a < 0) return -1;
+    if (a == 0) return 0;
+    return 1;
+}
+
+/* ------------------------------------------------------------------------ */
+/* deterministic pow(x, 4./3)                                                */
+/* ------------------------------------------------------------------------ */
+/*
+ * x = m * 2^e, e = 3q + r, t = m * 2^r in [1,8):  x^(4/3) = 2^(4q) * t * t^(1/3).
+ * t^(-1/3) by a cubic seed and three division-free Newton steps, t^(1/3) by one
+ * double-double Newton correction, and the exponent's representation error
+ * (4./3 = 4/3 - 2^-52/3) applied as the factor (1 - (2^-52/3) ln x).
+ * Only IEEE basic operations and fma, so the same bits on any IEEE machine.
+ */
+static const double ORC_P43_C0 = 1.2316035297857837;
+static const double ORC_P43_C1 = -0.29172246132464164;
+static const double ORC_P43_C2 = 0.044484938428368009;
+static const double ORC_P43_C3 = -0.0024458486365437917;
+static const double ORC_THIRD = 0.33333333333333331;         /* fl(1/3) */
+static const double ORC_DELTA43 = 7.4014868308343765e-17;    /* 4/3 - fl(4/3) = 2^-52/3 */
+static const double ORC_LN2 = 0.69314718055994529;
+
+static double orc_pow2i(int k)  /* 2^k for -1022 <= k <= 1023 */
+{
+    uint64_t b = (uint64_t)(k + 1023) << 52;
+    double d;
+    memcpy(&d, &b, 8);
+    return d;
+}
+
+double orc_pow43(double x)
+{
+    if (x != x) return x;
+    if (x <= 0.0) return (x == 0.0) ? 0.0 : (x - x) / (x - x);
+    if (x > 1.7976931348623157e308) return x;
+    uint64_t b;
+    memcpy(&b, &x, 8);
+    int e = (int)((b >> 52) & 0x7ff);
+    if (e == 0) {               /* subnormal: normalise exactly */
+        double xs = x * 18014398509481984.0;  /* 2^54 */
+        memcpy(&b, &xs, 8);
+        e = (int)((b >> 52) & 0x7ff) - 54;
+    }
+    e -= 1023;
+    uint64_t mb = (b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL;
+    double m;
+    memcpy(&m, &mb, 8);
+    int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);
+    int r = e - 3 * q;
+    double t = m * (double)(1 << r);
+
+    double y = fma(fma(fma(ORC_P43_C3, t, ORC_P43_C2), t, ORC_P43_C1), t, ORC_P43_C0);
+    for (int it = 0; it < 3; ++it) {
+        double w = y * y;
+        w = w * y;
+        double h = fma(-t, w, 1.0);
+        y = fma(y * h, ORC_THIRD, y);
+    }
+    double c = (t * y) * y;
+    double a_hi = c * c;
+    double a_lo = fma(c, c, -a_hi);
+    double b_hi = a_hi * c;
+    double b_lo = fma(a_hi, c, -b_hi) + a_lo * c;
+    double d = (t - b_hi) - b_lo;
+    double dc = (d * (y * y)) * ORC_THIRD;
+    double p_hi = t * c;
+    double p_lo = fma(t, c, -p_hi) + t * dc;
+
+    /* ln x to ~1e-8 absolute (only feeds a 1e-16-sized correction) */
+    double mm = m;
+    int ee = e;
+    if (mm > 1.4142135623730951) { mm = mm * 0.5; ee += 1; }
+    double s = (mm - 1.0) / (mm + 1.0);
+    double s2 = s * s;
+    double lnm = 2.0 * s * fma(fma(fma(s2, 0.14285714285714285, 0.2), s2, 0.33333333333333331), s2, 1.0);
+    double lnx = fma((double)ee, ORC_LN2, lnm);
+    double eps = ORC_DELTA43 * lnx;
+
+    double res = p_hi + fma(-p_hi, eps, p_lo);
+    double sc = orc_pow2i(2 * q);
+    return (res * sc) * sc;
+}
+
+/* ------------------------------------------------------------------------ */
+/* deterministic sine (tide forcing)                                         */
+/* ------------------------------------------------------------------------ */
+/* Cody-Waite reduction by pi/2 (three-part constant) and Taylor polynomials
+ * through x^15 (sin) and x^16 (cos) on [-pi/4, pi/4]. */
+double orc_sin(double x)
+{
+    if (!(x - x == 0.0)) return x - x;   /* inf or nan */
+    const double TWO_OVER_PI = 0.63661977236758138;
+    const double PIO2_1 = 1.5707963267341256e+00;
+    const double PIO2_2 = 6.077100506303966e-11;
+    const double PIO2_3 = 2.0222662487959506e-21;
+    double kd = nearbyint(x * TWO_OVER_PI);
+    double r = fma(-kd, PIO2_1, x);
+    r = fma(-kd, PIO2_2, r);
+    r = fma(-kd, PIO2_3, r);
+    long long k = (long long)kd;
+    double r2 = r * r;
+    double sp = fma(r2, -7.6471637318198164e-13, 1.6059043836821613e-10);
+    sp = fma(r2, sp, -2.5052108385441720e-08);
+    sp = fma(r2, sp, 2.7557319223985893e-06);
+    sp = fma(r2, sp, -1.9841269841269841e-04);
+    sp = fma(r2, sp, 8.3333333333333332e-03);
+    sp = fma(r2, sp, -1.6666666666666666e-01);
+    double sv = fma(r * r2, sp, r);
+    double cp = fma(r2, 4.7794773323873853e-14, -1.1470745597729725e-11);
+    cp = fma(r2, cp, 2.08767569878681e-09);
+    cp = fma(r2, cp, -2.7557319223985888e-07);
+    cp = fma(r2, cp, 2.4801587301587302e-05);
+    cp = fma(r2, cp, -1.3888888888888889e-03);
+    cp = fma(r2, cp, 4.1666666666666664e-02);
+    cp = fma(r2, cp, -0.5);
+    double cv = fma(r2, cp, 1.0);
+    switch ((int)(k & 3)) {
+    case 0: return sv;
+    case 1: return cv;
+    case 2: return -sv;
+    default: return -cv;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* lifetime                                                                  */
+/* ------------------------------------------------------------------------ */
+static void *orc_xcalloc(size_t n, size_t sz)
+{
+    void *p = calloc(n ? n : 1, sz);
+    if (!p) abort();
+    return p;
+}
+
+orc_state *orc_create(const orc_mesh *mesh, const orc_params *p)
+{
+    orc_state *s = (orc_state *)orc_xcalloc(1, sizeof(orc_state));
+    s->m = *mesh;
+    s->p = *p;
+    s->nthreads = 1;
+    int ne = mesh->num_el, nn = mesh->num_nodes, ned = mesh->num_edges;
+    size_t n3 = (size_t)3 * ne;
+    s->zeta = orc_xcalloc(n3, 8); s->qx = orc_xcalloc(n3, 8); s->qy = orc_xcalloc(n3, 8);
+    s->f1 = orc_xcalloc(n3, 8); s->f2 = orc_xcalloc(n3, 8); s->f3 = orc_xcalloc(n3, 8);
+    s->rz = orc_xcalloc(n3, 8); s->rqx = orc_xcalloc(n3, 8); s->rqy = orc_xcalloc(n3, 8);
+    s->w0 = orc_xcalloc(3 * n3, 8); s->w1 = orc_xcalloc(3 * n3, 8);
+    s->bzeta = orc_xcalloc(ned, 8); s->bqn = orc_xcalloc(ned, 8);
+    s->wd = orc_xcalloc(ne, sizeof(int));
+    for (int i = 0; i < ne; ++i) s->wd[i] = -1;           /* initialize.c:126 */
+    s->modify[0] = orc_xcalloc(ne, 1); s->modify[1] = orc_xcalloc(ne, 1);
+    s->nmin = orc_xcalloc((size_t)3 * nn, 8); s->nmax = orc_xcalloc((size_t)3 * nn, 8);
+
+    /* node -> element lists in ascending element order: the order in which
+     * SlopeLimiter2D.c:73-89 fills NtoEl.  Built once (the reference rebuilds
+     * it on every call; the lists are identical). */
+    s->n2e_off = orc_xcalloc((size_t)nn + 1, sizeof(int));
+    s->n2e = orc_xcalloc(n3, sizeof(int));
+    for (int el = 0; el < ne; ++el)
+        for (int k = 0; k < 3; ++k) s->n2e_off[mesh->elto_vert[3 * el + k] + 1]++;
+    for (int i = 0; i < nn; ++i) s->n2e_off[i + 1] += s->n2e_off[i];
+    int *fill = orc_xcalloc((size_t)nn, sizeof(int));
+    for (int el = 0; el < ne; ++el)
+        for (int k = 0; k < 3; ++k) {
+            int nd = mesh->elto_vert[3 * el + k];
+            s->n2e[s->n2e_off[nd] + fill[nd]++] = el;
+        }
+    free(fill);
+
+    /* minEdgLength (extract_mesh_attributes.c:25,297-303) */
+    double mel = 9999999;
+    for (int e = 0; e < ned; ++e) {
+        int n0 = mesh->edg_to_vert[2 * e], n1 = mesh->edg_to_vert[2 * e + 1];
+        double len = sqrt((mesh->x[n1] - mesh->x[n0]) * (mesh->x[n1] - mesh->x[n0]) +
+                          (mesh->y[n1] - mesh->y[n0]) * (mesh->y[n1] - mesh->y[n0]));
+        mel = ORC_MIN(mel, len);
+    }
+    s->min_edge_length = mel;
+    orc_boundary_conditions(s, 0.0);
+    return s;
+}
+
+void orc_destroy(orc_state *s)
+{
+    if (!s) return;
+    free(s->zeta); free(s->qx); free(s->qy);
+    free(s->f1); free(s->f2); free(s->f3);
+    free(s->rz); free(s->rqx); free(s->rqy);
+    free(s->w0); free(s->w1);
+    free(s->bzeta); free(s->bqn); free(s->wd);
+    free(s->modify[0]); free(s->modify[1]);
+    free(s->nmin); free(s->nmax); free(s->n2e_off); free(s->n2e);
+    free(s);
+}
+
+void orc_set_threads(orc_state *s, int nthreads) { s->nthreads = nthreads > 0 ? nthreads : 1; }
+
+void orc_set_state(orc_state *s, const double *z, const double *qx, const double *qy)
+{
+    size_t n = (size_t)3 * s->m.num_el * 8;
+    memcpy(s->zeta, z, n); memcpy(s->qx, qx, n); memcpy(s->qy, qy, n);
+}
+void orc_get_state(const orc_state *s, double *z, double *qx, double *qy)
+{
+    size_t n = (size_t)3 * s->m.num_el * 8;
+    memcpy(z, s->zeta, n); memcpy(qx, s->qx, n); memcpy(qy, s->qy, n);
+}
+void orc_set_wd(orc_state *s, const int *wd) { memcpy(s->wd, wd, (size_t)s->m.num_el * sizeof(int)); }
+void orc_get_wd(const orc_state *s, int *wd) { memcpy(wd, s->wd, (size_t)s->m.num_el * sizeof(int)); }
+void orc_get_modify(const orc_state *s, int stage, unsigned char *mod)
+{
+    memcpy(mod, s->modify[stage & 1], (size_t)s->m.num_el);
+}
+void orc_set_boundary(orc_state *s, const double *bz, const double *bq)
+{
+    memcpy(s->bzeta, bz, (size_t)s->m.num_edges * 8);
+    memcpy(s->bqn, bq, (size_t)s->m.num_edges * 8);
+}
+void orc_get_boundary(const orc_state *s, double *bz, double *bq)
+{
+    memcpy(bz, s->bzeta, (size_t)s->m.num_edges * 8);
+    memcpy(bq, s->bqn, (size_t)s->m.num_edges * 8);
+}
+void orc_get_fhat(const orc_state *s, double *f1, double *f2, double *f3)
+{
+    size_t n = (size_t)3 * s->m.num_el * 8;
+    memcpy(f1, s->f1, n); memcpy(f2, s->f2, n); memcpy(f3, s->f3, n);
+}
+void orc_get_rhs(const orc_state *s, double *a, double *b, double *c)
+{
+    size_t n = (size_t)3 * s->m.num_el * 8;
+    memcpy(a, s->rz, n); memcpy(b, s->rqx, n); memcpy(c, s->rqy, n);
+}
+double orc_max_lambda(const orc_state *s) { return s->max_lambda; }
+double orc_min_edge_length(const orc_state *s) { return s->min_edge_length; }
+int orc_error(const orc_state *s, int *where)
+{
+    if (where) *where = s->err_where;
+    return s->err_code;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Roe flux: numericalFlux2D.c:6-108                                         */
+/* ------------------------------------------------------------------------ */
+/* row-by-row product accumulated from +0 (mathfunctions.c:24-35) */
+static void orc_matvec3(const double M[3][3], const double v[3], double out[3])
+{
+    for (int i = 0; i < 3; ++i) {
+        double sum = 0;
+        for (int j = 0; j < 3; ++j) sum += M[i][j] * v[j];
+        out[i] = sum;
+    }
+}
+
+double orc_roe_flux(double F[3], double zeta_in, double zeta_ex, double qx_in, double qx_ex,
+                    double qy_in, double qy_ex, double zb, double nx, double ny,
+                    double lg, double g)
+{
+    double Hi = zeta_in + zb;                        /* :9-10 */
+    double He = zeta_ex + zb;
+    double ui = qx_in / Hi, ue = qx_ex / He;         /* :12-16 */
+    double vi = qy_in / Hi, ve = qy_ex / He;
+    double jump[3] = {zeta_in - zeta_ex, qx_in - qx_ex, qy_in - qy_ex};   /* :18 */
+
+    double f1xi = Hi * ui, f1xe = He * ue;           /* :20-24 */
+    double f1yi = Hi * vi, f1ye = He * ve;
+    double f2xi = Hi * ui * ui + 0.5 * lg * (Hi * Hi - zb * zb);   /* :26-27 */
+    double f2xe = He * ue * ue + 0.5 * lg * (He * He - zb * zb);
+    double f2yi = Hi * ui * vi, f2ye = He * ue * ve; /* :29-30 */
+    double f3xi = Hi * ui * vi, f3xe = He * ue * ve; /* :32-33 */
+    double f3yi = Hi * vi * vi + 0.5 * lg * (Hi * Hi - zb * zb);   /* :35-36 */
+    double f3ye = He * ve * ve + 0.5 * lg * (He * He - zb * zb);
+    double fni[3] = {f1xi * nx + f1yi * ny, f2xi * nx + f2yi * ny, f3xi * nx + f3yi * ny};
+    double fne[3] = {f1xe * nx + f1ye * ny, f2xe * nx + f2ye * ny, f3xe * nx + f3ye * ny};
+
+    double Hh = 0.5 * (Hi + He);                     /* :45-47 */
+    double uh = (ui * sqrt(Hi) + ue * sqrt(He)) / (sqrt(Hi) + sqrt(He));
+    double vh = (vi * sqrt(Hi) + ve * sqrt(He)) / (sqrt(Hi) + sqrt(He));
+    double c = sqrt(g * Hh);
+
+    double R[3][3] = {{1, 0, 1},                      /* :49 */
+                      {uh - c * nx, -ny, uh + c * nx},
+                      {vh - c * ny, nx, vh + c * ny}};
+    double Ri[3][3];                                 /* :50-60 */
+    double den = -2 * c;
+    Ri[0][0] = (-c - nx * uh - ny * vh) / den;
+    Ri[0][1] = nx / den;
+    Ri[0][2] = ny / den;
+    Ri[1][0] = (-2 * c * ny * uh + 2 * c * nx * vh) / den;
+    Ri[1][1] = 2 * c * ny / den;
+    Ri[1][2] = -2 * c * nx / den;
+    Ri[2][0] = (-c + nx * uh + ny * vh) / den;
+    Ri[2][1] = -nx / den;
+    Ri[2][2] = -ny / den;
+
+    double l1i = ui * nx + vi * ny - sqrt(g * Hi);   /* :62-72 */
+    double l2i = ui * nx + vi * ny;
+    double l3i = ui * nx + vi * ny + sqrt(g * Hi);
+    double l1e = ue * nx + ve * ny - sqrt(g * He);
+    double l2e = ue * nx + ve * ny;
+    double l3e = ue * nx + ve * ny + sqrt(g * He);
+    double lh1 = uh * nx + vh * ny - c;
+    double lh2 = uh * nx + vh * ny;
+    double lh3 = uh * nx + vh * ny + c;
+
+    double e1 = fmax(0, (lh1 - l1i));                /* :74-84 entropy fix */
+    e1 = fmax(e1, (l1e - lh1));
+    lh1 = fmax(e1, fabs(lh1));
+    double e2 = fmax(0, (lh2 - l2i));
+    e2 = fmax(e2, (l2e - lh2));
+    lh2 = fmax(e2, fabs(lh2));
+    double e3 = fmax(0, (lh3 - l3i));
+    e3 = fmax(e3, (l3e - lh3));
+    lh3 = fmax(e3, fabs(lh3));
+
+    double A[3][3] = {{lh1, 0, 0}, {0, lh2, 0}, {0, 0, lh3}};
+    double p1[3], p2[3], p3[3];
+    orc_matvec3(Ri, jump, p1);                       /* :92-94 */
+    orc_matvec3(A, p1, p2);
+    orc_matvec3(R, p2, p3);
+    for (int k = 0; k < 3; ++k) F[k] = 0.5 * (fni[k] + fne[k]) + 0.5 * p3[k];   /* :96-99 */
+
+    double mx = ORC_MAX(fabs(l1i), fabs(l1e));       /* :101-105 */
+    mx = ORC_MAX(mx, fabs(l2i));
+    mx = ORC_MAX(mx, fabs(l2e));
+    mx = ORC_MAX(mx, fabs(l3i));
+    mx = ORC_MAX(mx, fabs(l3e));
+    return mx;
+}
+
+/* ------------------------------------------------------------------------ */
+/* Inner products: 2DInnerProducts.c:8-570                                   */
+/* ------------------------------------------------------------------------ */
+static inline double orc_sq(const orc_state *s, double v)
+{
+    return s->p.math == ORC_MATH_REF ? orc_libm_pow(v, 2) : v * v;
+}
+static inline double orc_p43(const orc_state *s, double v)
+{
+    return s->p.math == ORC_MATH_REF ? orc_libm_pow(v, 4. / 3) : orc_pow43(v);
+}
+
+/* Cramer solve of the RT0 fit F = [a + b x, c + b y] (2DInnerProducts.c:134-146) */
+static void orc_rt0(double M11, double M12, double M13, double M21, double M22, double M23,
+                    double M31, double M32, double M33, double den,
+                    double R1, double R2, double R3, double *a, double *b, double *c)
+{
+    *a = (R3 * M13 * M22 - R3 * M12 * M23 - R2 * M13 * M32 + R1 * M23 * M32 + R2 * M12 * M33 - R1 * M22 * M33) / den;
+    *b = -(R3 * M13 * M21 - R3 * M11 * M23 - R2 * M13 * M31 + R1 * M23 * M31 + R2 * M11 * M33 - R1 * M21 * M33) / den;
+    *c = (R3 * M12 * M21 - R3 * M11 * M22 - R2 * M12 * M31 + R1 * M22 * M31 + R2 * M11 * M32 - R1 * M21 * M32) / den;
+}
+
+void orc_inner_products(const orc_state *s, int i, double SI[9], double VI[21])
+{
+    const orc_mesh *m = &s->m;
+    const double g = s->p.g;
+    int v0 = m->elto_vert[i * 3], v1 = m->elto_vert[i * 3 + 1], v2 = m->elto_vert[i * 3 + 2];
+    double xv1 = m->x[v0], xv2 = m->x[v1], xv3 = m->x[v2];            /* :22-27 */
+    double yv1 = m->y[v0], yv2 = m->y[v1], yv3 = m->y[v2];
+    double x1 = (m->x[v0] + m->x[v1]) / 2;                            /* :30-35 */
+    double x2 = (m->x[v1] + m->x[v2]) / 2;
+    double x3 = (m->x[v2] + m->x[v0]) / 2;
+    double y1 = (m->y[v0] + m->y[v1]) / 2;
+    double y2 = (m->y[v1] + m->y[v2]) / 2;
+    double y3 = (m->y[v2] + m->y[v0]) / 2;
+    double el0 = sqrt((xv2 - xv1) * (xv2 - xv1) + (yv2 - yv1) * (yv2 - yv1));   /* :37-39 */
+    double el1 = sqrt((xv3 - xv2) * (xv3 - xv2) + (yv3 - yv2) * (yv3 - yv2));
+    double el2 = sqrt((xv3 - xv1) * (xv3 - xv1) + (yv3 - yv1) * (yv3 - yv1));
+    double ej0 = el0 / 2, ej1 = el1 / 2, ej2 = el2 / 2;                /* :42-45 */
+
+    double sqrt3 = sqrt(3);                                           /* :51-65 */
+    double xe1[2], xe2[2], xe3[2], ye1[2], ye2[2], ye3[2];
+    xe1[0] = (sqrt3 * x1 - x2 + x3) / sqrt3;  ye1[0] = (sqrt3 * y1 - y2 + y3) / sqrt3;
+    xe1[1] = (sqrt3 * x1 + x2 - x3) / sqrt3;  ye1[1] = (sqrt3 * y1 + y2 - y3) / sqrt3;
+    xe2[0] = (x1 + sqrt3 * x2 - x3) / sqrt3;  ye2[0] = (y1 + sqrt3 * y2 - y3) / sqrt3;
+    xe2[1] = (-x1 + sqrt3 * x2 + x3) / sqrt3; ye2[1] = (-y1 + sqrt3 * y2 + y3) / sqrt3;
+    xe3[0] = (-x1 + x2 + sqrt3 * x3) / sqrt3; ye3[0] = (-y1 + y2 + sqrt3 * y3) / sqrt3;
+    xe3[1] = (x1 - x2 + sqrt3 * x3) / sqrt3;  ye3[1] = (y1 - y2 + sqrt3 * y3) / sqrt3;
+
+    double P1e1[2], P1e2[2], P1e3[2], P2e1[2], P2e2[2], P2e3[2], P3e1[2], P3e2[2], P3e3[2];
+    double den = x1 * y2 - x1 * y3 + x2 * y3 - x2 * y1 + x3 * y1 - x3 * y2;   /* :73 */
+    for (int n = 0; n < 2; ++n) {                                     /* :74-87 */
+        P1e1[n] = -(ye1[n] * x2 - ye1[n] * x3 - xe1[n] * y2 + x3 * y2 + xe1[n] * y3 - x2 * y3) / den;
+        P1e2[n] = -(ye2[n] * x2 - ye2[n] * x3 - xe2[n] * y2 + x3 * y2 + xe2[n] * y3 - x2 * y3) / den;
+        P1e3[n] = -(ye3[n] * x2 - ye3[n] * x3 - xe3[n] * y2 + x3 * y2 + xe3[n] * y3 - x2 * y3) / den;
+        P2e1[n] = -(-x1 * ye1[n] + x3 * ye1[n] - x3 * y1 + xe1[n] * y1 - xe1[n] * y3 + x1 * y3) / den;
+        P2e2[n] = -(-x1 * ye2[n] + x3 * ye2[n] - x3 * y1 + xe2[n] * y1 - xe2[n] * y3 + x1 * y3) / den;
+        P2e3[n] = -(-x1 * ye3[n] + x3 * ye3[n] - x3 * y1 + xe3[n] * y1 - xe3[n] * y3 + x1 * y3) / den;
+        P3e1[n] = -(ye1[n] * x1 - ye1[n] * x2 - xe1[n] * y1 + x2 * y1 + xe1[n] * y2 - x1 * y2) / den;
+        P3e2[n] = -(ye2[n] * x1 - ye2[n] * x2 - xe2[n] * y1 + x2 * y1 + xe2[n] * y2 - x1 * y2) / den;
+        P3e3[n] = -(ye3[n] * x1 - ye3[n] * x2 - xe3[n] * y1 + x2 * y1 + xe3[n] * y2 - x1 * y2) / den;
+    }
+
+    double F1e0 = s->f1[i * 3], F1e1 = s->f1[i * 3 + 1], F1e2 = s->f1[i * 3 + 2];   /* :95-105 */
+    double F2e0 = s->f2[i * 3], F2e1 = s->f2[i * 3 + 1], F2e2 = s->f2[i * 3 + 2];
+    double F3e0 = s->f3[i * 3], F3e1 = s->f3[i * 3 + 1], F3e2 = s->f3[i * 3 + 2];
+
+    double jac = 4 * (x2 - x3) * (y2 - y1) - 4 * (x2 - x1) * (y2 - y3);         /* :110-116 */
+    double nx1 = orc_sign(jac) * 2 * (y2 - y3) / el0;
+    double ny1 = orc_sign(jac) * 2 * (x3 - x2) / el0;
+    double nx2 = orc_sign(jac) * 2 * (y3 - y1) / el1;
+    double ny2 = orc_sign(jac) * 2 * (x1 - x3) / el1;
+    double nx3 = orc_sign(jac) * 2 * (y1 - y2) / el2;
+    double ny3 = orc_sign(jac) * 2 * (x2 - x1) / el2;
+
+    double M11 = nx1, M12 = x1 * nx1 + y1 * ny1, M13 = ny1;                      /* :126-128 */
+    double M21 = nx2, M22 = x2 * nx2 + y2 * ny2, M23 = ny2;
+    double M31 = nx3, M32 = x3 * nx3 + y3 * ny3, M33 = ny3;
+    den = M13 * M22 * M31 - M12 * M23 * M31 - M13 * M21 * M32 + M11 * M23 * M32 + M12 * M21 * M33 - M11 * M22 * M33;
+    double a1, b1, c1, a2, b2, c2, a3, b3, c3;
+    orc_rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, F1e0, F1e1, F1e2, &a1, &b1, &c1);
+    orc_rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, F2e0, F2e1, F2e2, &a2, &b2, &c2);
+    orc_rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, F3e0, F3e1, F3e2, &a3, &b3, &c3);
+
+    double F1n1[2], F1n2[2], F1n3[2], F2n1[2], F2n2[2], F2n3[2], F3n1[2], F3n2[2], F3n3[2];
+    for (int n = 0; n < 2; ++n) {                                     /* :154-167 */
+        F1n1[n] = (a1 + b1 * xe1[n]) * nx1 + (c1 + b1 * ye1[n]) * ny1;
+        F1n2[n] = (a1 + b1 * xe2[n]) * nx2 + (c1 + b1 * ye2[n]) * ny2;
+        F1n3[n] = (a1 + b1 * xe3[n]) * nx3 + (c1 + b1 * ye3[n]) * ny3;
+        F2n1[n] = (a2 + b2 * xe1[n]) * nx1 + (c2 + b2 * ye1[n]) * ny1;
+        F2n2[n] = (a2 + b2 * xe2[n]) * nx2 + (c2 + b2 * ye2[n]) * ny2;
+        F2n3[n] = (a2 + b2 * xe3[n]) * nx3 + (c2 + b2 * ye3[n]) * ny3;
+        F3n1[n] = (a3 + b3 * xe1[n]) * nx1 + (c3 + b3 * ye1[n]) * ny1;
+        F3n2[n] = (a3 + b3 * xe2[n]) * nx2 + (c3 + b3 * ye2[n]) * ny2;
+        F3n3[n] = (a3 + b3 * xe3[n]) * nx3 + (c3 + b3 * ye3[n]) * ny3;
+    }
+    /* :205-239 */
+#define ORC_SI(Fa, Fb, Fc, Pa, Pb, Pc) \
+    (ej0 * (Fa[0] * Pa[0] + Fa[1] * Pa[1]) + ej1 * (Fb[0] * Pb[0] + Fb[1] * Pb[1]) + \
+     ej2 * (Fc[0] * Pc[0] + Fc[1] * Pc[1]))
+    SI[0] = ORC_SI(F1n1, F1n2, F1n3, P1e1, P1e2, P1e3);
+    SI[1] = ORC_SI(F1n1, F1n2, F1n3, P2e1, P2e2, P2e3);
+    SI[2] = ORC_SI(F1n1, F1n2, F1n3, P3e1, P3e2, P3e3);
+    SI[3] = ORC_SI(F2n1, F2n2, F2n3, P1e1, P1e2, P1e3);
+    SI[4] = ORC_SI(F2n1, F2n2, F2n3, P2e1, P2e2, P2e3);
+    SI[5] = ORC_SI(F2n1, F2n2, F2n3, P3e1, P3e2, P3e3);
+    SI[6] = ORC_SI(F3n1, F3n2, F3n3, P1e1, P1e2, P1e3);
+    SI[7] = ORC_SI(F3n1, F3n2, F3n3, P2e1, P2e2, P2e3);
+    SI[8] = ORC_SI(F3n1, F3n2, F3n3, P3e1, P3e2, P3e3);
+#undef ORC_SI
+
+    double weight = 1.0 / 6;                                          /* :247 */
+    double zr[3], qxr[3], qyr[3], H[3], tau[3], gz[2], gP1[2], gP2[2], gP3[2], mz[3], mn[3];
+    int n0 = v0, n1 = v1, n2 = v2;
+    mz[0] = 0.5 * (m->z[n0] + m->z[n1]);                              /* :268-274 */
+    mz[1] = 0.5 * (m->z[n1] + m->z[n2]);
+    mz[2] = 0.5 * (m->z[n2] + m->z[n0]);
+    mn[0] = 0.5 * (m->n[n0] + m->n[n1]);
+    mn[1] = 0.5 * (m->n[n1] + m->n[n2]);
+    mn[2] = 0.5 * (m->n[n2] + m->n[n0]);
+    gP1[0] = 4 / jac * (y2 - y3);                                     /* :303-308 */
+    gP1[1] = 4 / jac * (x3 - x2);
+    gP2[0] = 4 / jac * (y3 - y1);
+    gP2[1] = 4 / jac * (x1 - x3);
+    gP3[0] = 4 / jac * (y1 - y2);
+    gP3[1] = 4 / jac * (x2 - x1);
+    for (int e = 0; e < 3; ++e) {                                     /* :310-319 */
+        zr[e] = s->zeta[i * 3 + e];
+        qxr[e] = s->qx[i * 3 + e];
+        qyr[e] = s->qy[i * 3 + e];
+        H[e] = zr[e] + mz[e];
+        double u = qxr[e] / H[e];
+        double v = qyr[e] / H[e];
+        tau[e] = g * mn[e] * mn[e] * sqrt(u * u + v * v) / (orc_p43(s, H[e]));
+    }
+    double lg = g;                                                    /* :324-326 */
+    if (s->wd[i] == 0) lg = 0;
+    double F1x[3], F2x[3], F3x[3], F1y[3], F2y[3], F3y[3];
+    for (int e = 0; e < 3; ++e) {                                     /* :327-336 */
+        F1x[e] = qxr[e];
+        F2x[e] = orc_sq(s, qxr[e]) / H[e] + 0.5 * lg * (orc_sq(s, H[e]) - orc_sq(s, mz[e]));
+        F3x[e] = qxr[e] * qyr[e] / H[e];
+        F1y[e] = qyr[e];
+        F2y[e] = qxr[e] * qyr[e] / H[e];
+        F3y[e] = orc_sq(s, qyr[e]) / H[e] + 0.5 * lg * (orc_sq(s, H[e]) - orc_sq(s, mz[e]));
+    }
+    double z1 = mz[0], z2 = mz[1], z3 = mz[2];                        /* :339-347 */
+    for (int j = 0; j < 2; ++j) gz[j] = z1 * (j ? gP1[1] : gP1[0]) + z2 * (j ? gP2[1] : gP2[0]) + z3 * (j ? gP3[1] : gP3[0]);
+
+    double R11 = F1x[0] * nx1 + F1y[0] * ny1;                          /* :487-497 */
+    double R12 = F1x[1] * nx2 + F1y[1] * ny2;
+    double R13 = F1x[2] * nx3 + F1y[2] * ny3;
+    double R21 = F2x[0] * nx1 + F2y[0] * ny1;
+    double R22 = F2x[1] * nx2 + F2y[1] * ny2;
+    double R23 = F2x[2] * nx3 + F2y[2] * ny3;
+    double R31 = F3x[0] * nx1 + F3y[0] * ny1;
+    double R32 = F3x[1] * nx2 + F3y[1] * ny2;
+    double R33 = F3x[2] * nx3 + F3y[2] * ny3;
+    /* :499-519: M and den are recomputed from the same values; identical */
+    orc_rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, R11, R12, R13, &a1, &b1, &c1);
+    orc_rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, R21, R22, R23, &a2, &b2, &c2);
+    orc_rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, R31, R32, R33, &a3, &b3, &c3);
+
+    double aj = fabs(jac);                                            /* :522-566 */
+#define ORC_VI(a, b, c, gP) \
+    (aj * (0.5 * a * gP[0] + 0.5 * c * gP[1] + b * weight * (gP[0] * (x1 + x2 + x3) + gP[1] * (y1 + y2 + y3))))
+    VI[0] = ORC_VI(a1, b1, c1, gP1);
+    VI[1] = ORC_VI(a1, b1, c1, gP2);
+    VI[2] = ORC_VI(a1, b1, c1, gP3);
+    VI[3] = ORC_VI(a2, b2, c2, gP1);
+    VI[4] = lg * aj * gz[0] * weight * zr[0];
+    VI[5] = aj * weight * tau[0] * qxr[0];
+    VI[6] = ORC_VI(a2, b2, c2, gP2);
+    VI[7] = lg * aj * gz[0] * weight * zr[1];
+    VI[8] = aj * weight * tau[1] * qxr[1];
+    VI[9] = ORC_VI(a2, b2, c2, gP3);
+    VI[10] = lg * aj * gz[0] * weight * zr[2];
+    VI[11] = aj * weight * tau[2] * qxr[2];
+    VI[12] = ORC_VI(a3, b3, c3, gP1);
+    VI[13] = lg * aj * gz[1] * weight * zr[0];
+    VI[14] = aj * weight * tau[0] * qyr[0];
+    VI[15] = ORC_VI(a3, b3, c3, gP2);
+    VI[16] = lg * aj * gz[1] * weight * zr[1];
+    VI[17] = aj * weight * tau[1] * qyr[1];
+    VI[18] = ORC_VI(a3, b3, c3, gP3);
+    VI[19] = lg * aj * gz[1] * weight * zr[2];
+    VI[20] = aj * weight * tau[2] * qyr[2];
+#undef ORC_VI
+}
+
+/* ------------------------------------------------------------------------ */
+/* compute2DL: compute2DL.c:12-413 (WDON branch :201-293)                    */
+/* ------------------------------------------------------------------------ */
+static void orc_set_err(orc_state *s, int code, int where)
+{
+    if (!s->err_code) { s->err_code = code; s->err_where = where; }
+}
+
+static int orc_local_edge(const int *ev, int v0, int v1)
+{
+    if ((v0 == ev[0] && v1 == ev[1]) || (v0 == ev[1] && v1 == ev[0])) return 0;
+    if ((v0 == ev[1] && v1 == ev[2]) || (v0 == ev[2] && v1 == ev[1])) return 1;
+    if ((v0 == ev[2] && v1 == ev[0]) || (v0 == ev[0] && v1 == ev[2])) return 2;
+    return -1;
+}
+
+int orc_compute_L(orc_state *s)
+{
+    const orc_mesh *m = &s->m;
+    const double g = s->p.g;
+    const int wdon = s->p.wetdry;
+    double maxl = 0;
+    for (int i = 0; i < m->num_edges; ++i) {
+        int el_in = m->edg_to_els[2 * i], el_ex = m->edg_to_els[2 * i + 1];
+        int v0 = m->edg_to_vert[2 * i], v1 = m->edg_to_vert[2 * i + 1];
+        const int *ein = &m->elto_vert[3 * el_in], *eex = &m->elto_vert[3 * el_ex];
+        int kin = orc_local_edge(ein, v0, v1);       /* :45-67 */
+        int kex = orc_local_edge(eex, v0, v1);
+        if (kin < 0) { orc_set_err(s, 1, i); return 1; }
+        if (kex < 0) { orc_set_err(s, 2, i); return 2; }
+        const double *X = m->x, *Y = m->y;
+        double len = sqrt((X[v1] - X[v0]) * (X[v1] - X[v0]) + (Y[v1] - Y[v0]) * (Y[v1] - Y[v0]));  /* :98-99 */
+        double jac = X[ein[0]] * (Y[ein[1]] - Y[ein[2]]) + X[ein[1]] * (Y[ein[2]] - Y[ein[0]]) +
+                     X[ein[2]] * (Y[ein[0]] - Y[ein[1]]);                                        /* :100-102 */
+        int a = ein[kin], b = ein[(kin + 1) % 3];                                                /* :105-130 */
+        double x0 = X[a], x1 = X[b], y0 = Y[a], y1 = Y[b];
+        double nx = orc_sign(jac) * (y1 - y0) / len;                                             /* :131-132 */
+        double ny = orc_sign(jac) * (x0 - x1) / len;
+        double tx = -ny, ty = nx;                                                                /* :138-139 */
+        double zin = s->zeta[3 * el_in + kin], qxin = s->qx[3 * el_in + kin], qyin = s->qy[3 * el_in + kin];
+        double zb = (m->z[v0] + m->z[v1]) / 2;                                                   /* :144 */
+        double zex, qxex, qyex;
+        int bp = m->bdry[i];
+        if (bp) {                                                                                /* :149-164 */
+            zex = s->bzeta[i];
+            double qnex = s->bqn[i];
+            if (zex < 0) zex = zin;
+            if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+            double qtex = qxin * tx + qyin * ty;
+            double dn = 1. / (nx * ty - ny * tx);
+            qxex = (ty * qnex - ny * qtex) * dn;
+            qyex = (-tx * qnex + nx * qtex) * dn;
+        } else if (el_in == el_ex) {                                                             /* :175-191 */
+            zex = s->zeta[3 * el_ex + kex];
+            double qnin = qxin * nx + qyin * ny;
+            double qtin = qxin * tx + qyin * ty;
+            double qnex = -qnin, qtex = qtin;
+            double dn = 1. / (nx * ty - ny * tx);
+            qxex = (ty * qnex - ny * qtex) * dn;
+            qyex = (-tx * qnex + nx * qtex) * dn;
+        } else {                                                                                 /* :194-199 */
+            zex = s->zeta[3 * el_ex + kex];
+            qxex = s->qx[3 * el_ex + kex];
+            qyex = s->qy[3 * el_ex + kex];
+        }
+        double Fin[3], Fex[3], cur;
+        if (wdon && s->wd[el_in] == 0 && s->wd[el_ex] == 0) {                                    /* :203-240 */
+            double zexr = zin;
+            double qnin = qxin * nx + qyin * ny;
+            double qtin = qxin * tx + qyin * ty;
+            double qnex = -qnin, qtex = qtin;
+            double dn = 1. / (nx * ty - ny * tx);
+            double qxexr = (ty * qnex - ny * qtex) * dn;
+            double qyexr = (-tx * qnex + nx * qtex) * dn;
+            double mli = orc_roe_flux(Fin, zin, zexr, qxin, qxexr, qyin, qyexr, zb, nx, ny, 0, g);
+            double zinr = zex;
+            qnex = qxex * nx + qyex * ny;
+            qtex = qxex * tx + qyex * ty;
+            qnin = -qnex;
+            qtin = qtex;
+            double qxinr = (ty * qnin - ny * qtin) * dn;
+            double qyinr = (-tx * qnin + nx * qtex) * dn;
+            double mle = orc_roe_flux(Fex, zinr, zex, qxinr, qxex, qyinr, qyex, zb, nx, ny, 0, g);
+            cur = ORC_MAX(mli, mle);
+        } else {
+            double Fh[3];
+            cur = orc_roe_flux(Fh, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
+            if (isnan(Fh[0]) || isnan(Fh[1]) || isnan(Fh[2])) { orc_set_err(s, 3, i); return 3; }
+            for (int j = 0; j < 3; ++j) { Fin[j] = Fh[j]; Fex[j] = Fh[j]; }
+            if (wdon) {                                                                          /* :251-287 */
+                if (s->wd[el_in] != 1) {
+                    double ml = orc_roe_flux(Fin, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, 0, g);
+                    if (isnan(Fin[0]) || isnan(Fin[1]) || isnan(Fin[2])) { orc_set_err(s, 4, i); return 4; }
+                    cur = ORC_MAX(cur, ml);
+                }
+                if (s->wd[el_ex] != 1) {
+                    double ml = orc_roe_flux(Fex, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, 0, g);
+                    if (isnan(Fex[0]) || isnan(Fex[1]) || isnan(Fex[2])) { orc_set_err(s, 5, i); return 5; }
+                    cur = ORC_MAX(cur, ml);
+                }
+            }
+        }
+        maxl = (i == 0) ? cur : ORC_MAX(maxl, cur);                                              /* :310-313 */
+        s->f1[3 * el_in + kin] = Fin[0];                                                         /* :319-329 */
+        s->f2[3 * el_in + kin] = Fin[1];
+        s->f3[3 * el_in + kin] = Fin[2];
+        if (el_ex != el_in) {
+            s->f1[3 * el_ex + kex] = -Fex[0];
+            s->f2[3 * el_ex + kex] = -Fex[1];
+            s->f3[3 * el_ex + kex] = -Fex[2];
+        }
+    }
+    s->max_lambda = maxl;
+
+    const double rain = s->p.rain;
+    const int ne = m->num_el;
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+#endif
+    for (int k = 0; k < ne; ++k) {                                                               /* :338-411 */
+        double SI[9], VI[21];
+        orc_inner_products(s, k, SI, VI);
+        double ze[3] = {-SI[0] + VI[0], -SI[1] + VI[1], -SI[2] + VI[2]};
+        double qe[3] = {-SI[3] + VI[3] + VI[4] - VI[5], -SI[4] + VI[6] + VI[7] - VI[8],
+                        -SI[5] + VI[9] + VI[10] - VI[11]};
+        double re[3] = {-SI[6] + VI[12] + VI[13] - VI[14], -SI[7] + VI[15] + VI[16] - VI[17],
+                        -SI[8] + VI[18] + VI[19] - VI[20]};
+        const int *ev = &m->elto_vert[3 * k];
+        double x0 = m->x[ev[0]], x1 = m->x[ev[1]], x2 = m->x[ev[2]];
+        double y0 = m->y[ev[0]], y1 = m->y[ev[1]], y2 = m->y[ev[2]];
+        double jac = x0 * (y1 - y2) + x1 * (y2 - y0) + x2 * (y0 - y1);
+        double Mi[3][3] = {{6 / fabs(jac), 0, 0}, {0, 0, 0}, {0, 0, 0}};
+        Mi[1][1] = Mi[0][0];
+        Mi[2][2] = Mi[0][0];
+        for (int j = 0; j < 3; ++j) {
+            double rzj = Mi[j][0] * ze[0] + Mi[j][1] * ze[1] + Mi[j][2] * ze[2];
+            s->rqx[3 * k + j] = Mi[j][0] * qe[0] + Mi[j][1] * qe[1] + Mi[j][2] * qe[2];
+            s->rqy[3 * k + j] = Mi[j][0] * re[0] + Mi[j][1] * re[1] + Mi[j][2] * re[2];
+            s->rz[3 * k + j] = (rain != 0) ? rzj + rain : rzj;
+        }
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* SlopeLimiter: SlopeLimiter2D.c:10-359                                     */
+/* ------------------------------------------------------------------------ */
+void orc_slope_limiter(orc_state *s, unsigned char *modify_out)
+{
+    const orc_mesh *m = &s->m;
+    const double H0 = s->p.h0;
+    const int nn = m->num_nodes, ne = m->num_el;
+    double *F[3] = {s->zeta, s->qx, s->qy};
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+#endif
+    for (int i = 0; i < nn; ++i) {                                    /* :106-153 */
+        double mx[3] = {-99999, -99999, -99999}, mn[3] = {99999, 99999, 99999};
+        for (int j = s->n2e_off[i]; j < s->n2e_off[i + 1]; ++j) {
+            int el = s->n2e[j];
+            for (int v = 0; v < 3; ++v) {
+                double avg = (F[v][3 * el] + F[v][3 * el + 1] + F[v][3 * el + 2]) / 3;
+                if (avg < mn[v]) mn[v] = avg;
+                if (avg > mx[v]) mx[v] = avg;
+            }
+        }
+        for (int v = 0; v < 3; ++v) { s->nmin[3 * i + v] = mn[v]; s->nmax[3 * i + v] = mx[v]; }
+    }
+#ifdef _OPENMP
+#pragma omp parallel for num_threads(s->nthreads) schedule(static)
+#endif
+    for (int el = 0; el < ne; ++el) {                                 /* :157-343 */
+        int nd[3] = {m->elto_vert[3 * el], m->elto_vert[3 * el + 1], m->elto_vert[3 * el + 2]};
+        int modify = 0;
+        for (int iv = 0; iv < 3; ++iv) {
+            double *w = F[iv];
+            double w1 = w[3 * el], w2 = w[3 * el + 1], w3 = w[3 * el + 2];
+            double V[3] = {w1 - w2 + w3, w1 + w2 - w3, -w1 + w2 + w3};   /* :177-179 */
+            double Vmax[3], Vmin[3];
+            for (int k = 0; k < 3; ++k) { Vmax[k] = s->nmax[3 * nd[k] + iv]; Vmin[k] = s->nmin[3 * nd[k] + iv]; }
+            double avg = (w1 + w2 + w3) / 3;
+            if (iv == 0) {                                            /* :189-200 */
+                double z1 = m->z[nd[0]], z2 = m->z[nd[1]], z3 = m->z[nd[2]];
+                double H1 = V[0] + z1, H2 = V[1] + z2, H3 = V[2] + z3;
+                double Hm0 = Vmin[0] + z1, Hm1 = Vmin[1] + z2, Hm2 = Vmin[2] + z3;
+                if (H1 > H0 && H2 > H0 && H3 > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
+            }
+            for (int k = 0; k < 3; ++k) V[k] = fmax(ORC_MIN(V[k], Vmax[k]), Vmin[k]);   /* :244-246 */
+            for (int ll = 0; ll < 3; ++ll) {                          /* :254-299 */
+                double sumloc = (V[0] + V[1] + V[2]) / 3;
+                double sumdiff = (sumloc - avg) * 3;
+                int sd = orc_sign(sumdiff);
+                double diff[3] = {(V[0] - avg) * sd, (V[1] - avg) * sd, (V[2] - avg) * sd};
+                int inc1 = diff[0] > 0, inc2 = diff[1] > 0, inc3 = diff[2] > 0;
+                double kdp = inc1 + inc2 + inc3;
+                for (int k = 0; k < 3; ++k) {
+                    double dv = fmax(1, kdp);
+                    double redfac, redmax;
+                    if (diff[k] > 0) { redfac = sumdiff * sd / dv; kdp -= 1; }
+                    else redfac = 0;
+                    if (sd > 0) redmax = V[k] - Vmin[k];
+                    else redmax = Vmax[k] - V[k];
+                    redfac = ORC_MIN(redfac, redmax);
+                    sumdiff -= redfac * sd;
+                    V[k] -= redfac * sd;
+                }
+            }
+            double mid1 = (V[0] + V[1]) / 2, mid2 = (V[1] + V[2]) / 2, mid3 = (V[2] + V[0]) / 2;   /* :301-303 */
+            if (iv == 0) {                                            /* :305-319 */
+                double H1 = V[0] + m->z[nd[0]], H2 = V[1] + m->z[nd[1]], H3 = V[2] + m->z[nd[2]];
+                if (H1 < H0 || H2 < H0 || H3 < H0) modify = 0;
+            }
+            if (modify) { w[3 * el] = mid1; w[3 * el + 1] = mid2; w[3 * el + 2] = mid3; }
+        }
+        if (modify_out) modify_out[el] = (unsigned char)modify;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* wet/dry: wetDry2D.c:7-260                                                 */
+/* ------------------------------------------------------------------------ */
+static int orc_theta(double a) { return a > 0 ? 1 : 0; }             /* :7-14 */
+
+void orc_wetdry_status(orc_state *s)
+{
+    const orc_mesh *m = &s->m;
+    const double H0 = s->p.h0;
+    for (int el = 0; el < m->num_el; ++el) {                          /* :18-58 */
+        double a = s->zeta[3 * el], b = s->zeta[3 * el + 1], c = s->zeta[3 * el + 2];
+        double ze[3] = {a - b + c, a + b - c, -a + b + c};
+        double zz[3] = {m->z[m->elto_vert[3 * el]], m->z[m->elto_vert[3 * el + 1]], m->z[m->elto_vert[3 * el + 2]]};
+        double h[3] = {ze[0] + zz[0], ze[1] + zz[1], ze[2] + zz[2]};
+        double avgH = (h[0] + h[1] + h[2]) / 3;
+        int mi = 0;
+        double mh = h[0];
+        for (int i = 1; i < 3; ++i) if (h[i] > mh) { mi = i; mh = h[i]; }
+        double zmax = ze[mi];
+        double zmin = zz[0];
+        for (int i = 1; i < 3; ++i) if (zz[i] < zmin) zmin = zz[i];
+        if (s->wd[el] == -1)
+            s->wd[el] = orc_theta(avgH - H0) * orc_theta(zmax - (H0 - zmin));
+        else
+            s->wd[el] = s->wd[el] * orc_theta(avgH - H0) +
+                        (1 - s->wd[el]) * orc_theta(avgH - H0) * orc_theta(zmax - (H0 - zmin));
+    }
+}
+
+void orc_pdop(orc_state *s)
+{
+    const orc_mesh *m = &s->m;
+    const double H0 = s->p.h0, VZ = s->p.velzero;
+    for (int el = 0; el < m->num_el; ++el) {                          /* :66-258 */
+        double mz0 = s->zeta[3 * el], mz1 = s->zeta[3 * el + 1], mz2 = s->zeta[3 * el + 2];
+        double z0 = m->z[m->elto_vert[3 * el]], z1 = m->z[m->elto_vert[3 * el + 1]], z2 = m->z[m->elto_vert[3 * el + 2]];
+        double bz0 = 0.5 * (z0 + z1), bz1 = 0.5 * (z1 + z2), bz2 = 0.5 * (z2 + z0);
+        double mh0 = mz0 + bz0, mh1 = mz1 + bz1, mh2 = mz2 + bz2;
+        double Ht[3] = {mh0 - mh1 + mh2, mh0 + mh1 - mh2, -mh0 + mh1 + mh2};
+        double avgH = (Ht[0] + Ht[1] + Ht[2]) / 3;
+        if (Ht[0] <= H0 || Ht[1] <= H0 || Ht[2] <= H0) {              /* :93-156 */
+            if (avgH <= H0) {
+                for (int i = 0; i < 3; ++i) Ht[i] = avgH;
+            } else {
+                int mi = 0;
+                double mh = Ht[0];
+                for (int i = 1; i < 3; ++i) if (Ht[i] < mh) { mi = i; mh = Ht[i]; }
+                int ni = 1;
+                double nh = Ht[1];
+                for (int i = 0; i < 3; ++i) if (Ht[i] < nh && i != mi) { ni = i; nh = Ht[i]; }
+                int ri = 2;
+                for (int i = 0; i < 3; ++i) if (i != mi && i != ni) { ri = i; break; }
+                Ht[mi] = H0;
+                Ht[ni] = fmax(H0, nh - 0.5 * (Ht[mi] - mh));
+                Ht[ri] = Ht[ri] - (Ht[mi] - mh) - (Ht[ni] - nh);
+            }
+            if (isnan(Ht[0]) || isnan(Ht[1]) || isnan(Ht[2])) { orc_set_err(s, 6, el); return; }
+            mh0 = 0.5 * (Ht[0] + Ht[1]);
+            mh1 = 0.5 * (Ht[1] + Ht[2]);
+            mh2 = 0.5 * (Ht[2] + Ht[0]);
+            s->zeta[3 * el] = mh0 - bz0;
+            s->zeta[3 * el + 1] = mh1 - bz1;
+            s->zeta[3 * el + 2] = mh2 - bz2;
+        }
+        int f[3];
+        for (int i = 0; i < 3; ++i) f[i] = (Ht[i] > H0);              /* :161-165 */
+        if (f[0] == 0 && f[1] == 0 && f[2] == 0) {                    /* :170-178 */
+            for (int i = 0; i < 3; ++i) { s->qx[3 * el + i] = 0; s->qy[3 * el + i] = 0; }
+        } else if (f[0] == 0 || f[1] == 0 || f[2] == 0) {             /* :180-255 */
+            double a = s->qx[3 * el], b = s->qx[3 * el + 1], c = s->qx[3 * el + 2];
+            double d = s->qy[3 * el], e = s->qy[3 * el + 1], ff = s->qy[3 * el + 2];
+            double qxe[3] = {a - b + c, a + b - c, -a + b + c};
+            double qye[3] = {d - e + ff, d + e - ff, -d + e + ff};
+            double u[3], v[3];
+            for (int i = 0; i < 3; ++i) { u[i] = qxe[i] / Ht[i]; v[i] = qye[i] / Ht[i]; }
+            int npos = 0;
+            double du = 0, dv = 0;
+            for (int i = 0; i < 3; ++i) {
+                npos += f[i];
+                du += u[i] * (1 - f[i]) * (fabs(qxe[i]) > VZ);
+                dv += v[i] * (1 - f[i]) * (fabs(qye[i]) > VZ);
+            }
+            for (int i = 0; i < 3; ++i) {
+                u[i] = f[i] * (u[i] + du / npos);
+                v[i] = f[i] * (v[i] + dv / npos);
+                if (isnan(u[i]) || isnan(v[i])) { orc_set_err(s, 7, el); return; }
+            }
+            for (int i = 0; i < 3; ++i) { qxe[i] = Ht[i] * u[i]; qye[i] = Ht[i] * v[i]; }
+            for (int i = 0; i < 3; ++i) {
+                int i1 = i % 3, i2 = (i + 1) % 3;
+                s->qx[3 * el + i] = 0.5 * (qxe[i1] + qxe[i2]);
+                s->qy[3 * el + i] = 0.5 * (qye[i1] + qye[i2]);
+            }
+        }
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundary_conditions.c:10-83 (+ tide extension)                           */
+/* ------------------------------------------------------------------------ */
+void orc_boundary_conditions(orc_state *s, double t)
+{
+    const orc_mesh *m = &s->m;
+    double tide = 0;
+    if (s->p.tide_code)
+        tide = s->p.tide_datum + s->p.tide_amp * orc_sin(6.283185307179586 * t / s->p.tide_period + s->p.tide_phase);
+    for (int e = 0; e < m->num_edges; ++e) {
+        double qn = 0, zb = 0;
+        int bp = m->bdry[e];
+        if (bp == 1) { qn = -0.042 / 0.914; zb = -1000; }
+        if (bp == 2) { zb = 0.296; qn = 1000000; }
+        if (bp == 3) { zb = -10000; qn = -0.1275 / 0.914; }
+        if (s->p.tide_code && bp == s->p.tide_code) { zb = tide; qn = 1000000; }
+        s->bzeta[e] = zb;
+        s->bqn[e] = qn;
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* oneTimeStep2D: time_evolution.c:10-246 (wet/dry order: ChanNet/time_evolution.c:218-427) */
+/* ------------------------------------------------------------------------ */
+int orc_one_time_step(orc_state *s, double t, double dt)
+{
+    (void)t;
+    const size_t n3 = (size_t)3 * s->m.num_el;
+    double *W[3] = {s->zeta, s->qx, s->qy};
+    double *R[3] = {s->rz, s->rqx, s->rqy};
+    for (int f = 0; f < 3; ++f) memcpy(s->w0 + f * n3, W[f], n3 * 8);     /* :14-26 */
+    if (s->p.wetdry) orc_wetdry_status(s);                                 /* :41 */
+    if (orc_compute_L(s)) return s->err_code;                              /* :44 */
+    for (int f = 0; f < 3; ++f)                                            /* :72-110 */
+        for (size_t i = 0; i < n3; ++i) W[f][i] = s->w0[f * n3 + i] + R[f][i] * dt;
+    orc_slope_limiter(s, s->modify[0]);                                    /* :146 */
+    if (s->p.wetdry) { orc_pdop(s); if (s->err_code) return s->err_code; } /* :149 */
+    for (int f = 0; f < 3; ++f)                                            /* :151-178 */
+        for (size_t i = 0; i < n3; ++i) s->w1[f * n3 + i] = W[f][i] + s->w0[f * n3 + i];
+    if (s->p.wetdry) orc_wetdry_status(s);                                 /* :181 */
+    if (orc_compute_L(s)) return s->err_code;                              /* :184 */
+    double half = dt / 2;                                                  /* :208-227 */
+    for (int f = 0; f < 3; ++f)
+        for (size_t i = 0; i < n3; ++i) W[f][i] = s->w1[f * n3 + i] * 0.5 + R[f][i] * half;
+    orc_slope_limiter(s, s->modify[1]);                                    /* :230 */
+    if (s->p.wetdry) { orc_pdop(s); if (s->err_code) return s->err_code; } /* :233 */
+    return 0;
+}
+
+/* time_evolution.c:311-402 */
+int orc_run(orc_state *s, int nsteps, double *t, double *dt)
+{
+    double time = *t, h = *dt;
+    for (int n = 0; n < nsteps; ++n) {
+        orc_boundary_conditions(s, time);
+        time = time + h;
+        int rc = orc_one_time_step(s, time, h);
+        if (rc) { *t = time; *dt = h; return rc; }
+        h = s->p.cfl * s->min_edge_length / s->max_lambda;
+        if ((time + h) > s->p.final_time) h = s->p.final_time - time;
+    }
+    *t = time;
+    *dt = h;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* topology: extract_mesh_attributes.c:143-361, restated with O(N) lookups   */
+/* ------------------------------------------------------------------------ */
+typedef struct { long long key; int val; int val2; } orc_hent;
+
+static long long orc_ekey(int a, int b) { return a < b ? ((long long)a << 32) | (unsigned)b : ((long long)b << 32) | (unsigned)a; }
+
+static size_t orc_hslot(long long k, size_t mask) { return (size_t)((uint64_t)k * 0x9E3779B97F4A7C15ULL >> 20) & mask; }
+
+int orc_build_topology(int ne, int nn, const int *ev, const double *x, const double *y,
+                       int n_in1, const int *in1, int n_in2, const int *in2, int n_out, const int *out,
+                       int *e2v, int *e2e, int *bdry, double *mel_out)
+{
+    /* NtoNodes in first-seen order (:157-214) */
+    int *cnt = orc_xcalloc((size_t)nn, sizeof(int));
+    int *nb = orc_xcalloc((size_t)nn * 16, sizeof(int));
+    for (int el = 0; el < ne; ++el)
+        for (int n = 0; n < 3; ++n) {
+            int node = ev[3 * el + n];
+            int c0 = ev[3 * el + (n + 1) % 3], c1 = ev[3 * el + (n + 2) % 3];
+            int f0 = 0, f1 = 0;
+            for (int j = 0; j < cnt[node]; ++j) { if (nb[16 * node + j] == c0) f0 = 1; if (nb[16 * node + j] == c1) f1 = 1; }
+            if (!f0) { if (cnt[node] >= 16) { free(cnt); free(nb); return -2; } nb[16 * node + cnt[node]++] = c0; }
+            if (!f1) { if (cnt[node] >= 16) { free(cnt); free(nb); return -2; } nb[16 * node + cnt[node]++] = c1; }
+        }
+    int ned = nn + ne - 1;                                            /* :219 */
+    int ec = 0;
+    for (int n = 0; n < nn; ++n)                                      /* :225-250 */
+        for (int k = 0; k < cnt[n]; ++k) {
+            int c = nb[16 * n + k];
+            if (c > n) {
+                if (ec >= ned) { free(cnt); free(nb); return -1; }
+                e2v[2 * ec] = n; e2v[2 * ec + 1] = c; ec++;
+            }
+        }
+    free(cnt); free(nb);
+    if (ec != ned) return -1;                                         /* :252-257 */
+    /* EdgtoEls: the two lowest element indices sharing the edge (:259-295) */
+    size_t cap = 1;
+    while (cap < (size_t)ned * 2) cap <<= 1;
+    orc_hent *ht = orc_xcalloc(cap, sizeof(orc_hent));
+    for (size_t i = 0; i < cap; ++i) ht[i].key = -1;
+    for (int e = 0; e < ned; ++e) {
+        long long k = orc_ekey(e2v[2 * e], e2v[2 * e + 1]);
+        size_t h = orc_hslot(k, cap - 1);
+        while (ht[h].key != -1) h = (h + 1) & (cap - 1);
+        ht[h].key = k; ht[h].val = e;
+        e2e[2 * e] = -1; e2e[2 * e + 1] = -1;
+    }
+    for (int el = 0; el < ne; ++el)
+        for (int k = 0; k < 3; ++k) {
+            long long key = orc_ekey(ev[3 * el + k], ev[3 * el + (k + 1) % 3]);
+            size_t h = orc_hslot(key, cap - 1);
+            while (ht[h].key != -1 && ht[h].key != key) h = (h + 1) & (cap - 1);
+            if (ht[h].key != key) continue;
+            int e = ht[h].val;
+            if (e2e[2 * e] < 0) e2e[2 * e] = el;
+            else if (e2e[2 * e + 1] < 0 && e2e[2 * e] != el) e2e[2 * e + 1] = el;
+        }
+    for (int e = 0; e < ned; ++e)
+        if (e2e[2 * e] >= 0 && e2e[2 * e + 1] < 0) e2e[2 * e + 1] = e2e[2 * e];
+    /* minEdgLength and BdryPrescribed (:297-354) */
+    double mel = 9999999;
+    for (int e = 0; e < ned; ++e) {
+        int n0 = e2v[2 * e], n1 = e2v[2 * e + 1];
+        double len = sqrt((x[n1] - x[n0]) * (x[n1] - x[n0]) + (y[n1] - y[n0]) * (y[n1] - y[n0]));
+        mel = ORC_MIN(mel, len);
+        bdry[e] = 0;
+    }
+    const int *lists[3] = {out, in1, in2};
+    const int nlist[3] = {n_out, n_in1, n_in2};
+    const int code[3] = {2, 1, 3};   /* later lists take precedence: inflow-2 > inflow-1 > outflow */
+    for (int l = 0; l < 3; ++l)
+        for (int k = 0; k < nlist[l]; ++k) {
+            long long key = orc_ekey(lists[l][2 * k], lists[l][2 * k + 1]);
+            size_t h = orc_hslot(key, cap - 1);
+            while (ht[h].key != -1 && ht[h].key != key) h = (h + 1) & (cap - 1);
+            if (ht[h].key == key) bdry[ht[h].val] = code[l];
+        }
+    free(ht);
+    if (mel_out) *mel_out = mel;
+    return ned;
+}

@@ -1,0 +1,63 @@
+"""Shared test inputs: small perturbed meshes with every boundary code."""
+import numpy as np
+
+from dgrain import mesh as M
+
+
+def perturbed_mesh(N=6, L=100.0, seed=1, depth=2.0, codes=True):
+    """Grid with jittered interior nodes, random depth and n, and boundary
+    sections of codes 1 (left, lower half), 3 (bottom) and 2 (right)."""
+    rng = np.random.default_rng(seed)
+    x, y = M.grid_nodes(N, L)
+    inner = (x > 0) & (x < L) & (y > 0) & (y < L)
+    h = L / N
+    x = x + rng.uniform(-0.2 * h, 0.2 * h, x.size) * inner
+    y = y + rng.uniform(-0.2 * h, 0.2 * h, y.size) * inner
+    z = depth + rng.uniform(-0.25 * depth, 0.25 * depth, x.size)
+    n = rng.uniform(0.01, 0.05, x.size)
+    kw = {}
+    if codes:
+        kw = dict(in1=M.side_pairs(N, "left")[: 2 * (N // 2)], in2=M.side_pairs(N, "bottom"),
+                  out=M.side_pairs(N, "right"))
+    return M.from_elements(x, y, z, n, M.grid_elements(N), **kw)
+
+
+def random_state(mesh, seed=2, dz=0.2, q=0.5):
+    rng = np.random.default_rng(seed)
+    ne3 = 3 * mesh.num_el
+    return (rng.uniform(-dz, dz, ne3), rng.uniform(-q, q, ne3), rng.uniform(-q, q, ne3))
+
+
+def beach_mesh(N=10, L=100.0, seed=3):
+    """Sloping beach: depth +1 at x=0 to -0.5 at x=L (dry upper part)."""
+    rng = np.random.default_rng(seed)
+    x, y = M.grid_nodes(N, L)
+    z = 1.0 - 1.5 * x / L + 0.05 * rng.uniform(-1, 1, x.size)
+    n = rng.uniform(0.01, 0.05, x.size)
+    return M.from_elements(x, y, z, n, M.grid_elements(N))
+
+
+def beach_state(mesh, seed=4, H0=1e-3):
+    rng = np.random.default_rng(seed)
+    zn = np.maximum(0.2, H0 - mesh.z)
+    zeta = M.nodal_to_midpoints(mesh, zn)
+    Hm = zeta + M.nodal_to_midpoints(mesh, mesh.z)
+    ne3 = 3 * mesh.num_el
+    qx = rng.uniform(-0.02, 0.02, ne3) * (Hm > 0.1)
+    qy = rng.uniform(-0.02, 0.02, ne3) * (Hm > 0.1)
+    return zeta, qx, qy
+
+
+def same(a, b):
+    """Bitwise equality up to the sign of zero (NaN == NaN)."""
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return a.shape == b.shape and bool(np.all((a == b) | (np.isnan(a) & np.isnan(b))))
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    d = np.linalg.norm(a - b)
+    n = np.linalg.norm(b)
+    return d / n if n > 0 else d

@@ -1,0 +1,155 @@
+"""GPU parity: libdgrain on gfx950, called through the C ABI, against the
+oracle in its device-math mode (squares as x*x, H^(4/3) by the shared
+deterministic algorithm).  The bar is bit-for-bit equality up to the sign of
+zero, for states, RHS, edge fluxes, max_lambda, dt, wet/dry flags and limiter
+activations.  (The oracle's reference-math mode is pinned to the compiled
+reference in test_oracle_vs_reference.py; the two modes are compared to the
+north-star tolerance rel-L2 <= 1e-11 in test_oracle_modes.py.)"""
+import numpy as np
+import pytest
+
+import oracle
+from dgrain import configs, native
+from helpers import beach_mesh, beach_state, perturbed_mesh, random_state, same, rel_l2
+
+pytestmark = pytest.mark.gpu
+
+
+def _pair(mesh, wetdry=0, **kw):
+    s = native.Solver(mesh, native.default_params(wetdry=wetdry, **kw))
+    o = oracle.Oracle(mesh, math=oracle.MATH_DEV, wetdry=wetdry, **kw)
+    return s, o
+
+
+def test_math_selftest_matches_host(gpu):
+    rng = np.random.default_rng(7)
+    n = 200000
+    x = np.concatenate([np.exp(rng.uniform(np.log(1e-6), np.log(1e4), n)),
+                        rng.uniform(-50, 50, 1000), [0.0, -0.0, 1.0, 8.0, 1e-310, 5e-324, 1e300]])
+    y = rng.uniform(-3, 3, x.size)
+    out = native.math_selftest(x, y)
+    assert same(out[:, 0], np.sqrt(x))                  # IEEE sqrt
+    assert same(out[:, 1], x / y)                        # IEEE division
+    o43 = np.array([oracle.pow43(v) for v in x])
+    assert same(out[:, 2], o43)
+    osin = np.array([oracle.dsin(v) for v in x])
+    assert same(out[:, 3], osin)
+
+
+@pytest.mark.parametrize("wetdry", [0, 1])
+@pytest.mark.parametrize("seed", [1, 2])
+def test_compute2DL(gpu, wetdry, seed):
+    mesh = perturbed_mesh(N=8, seed=seed)
+    s, o = _pair(mesh, wetdry)
+    z, qx, qy = random_state(mesh, seed=seed + 10)
+    s.set_state(z, qx, qy)
+    o.set_state(z, qx, qy)
+    if wetdry:
+        wd = np.random.default_rng(seed).integers(0, 2, mesh.num_el).astype(np.int32)
+        s.set_wd(wd)
+        o.set_wd(wd)
+    rg, fg, lg = s.compute2DL()
+    ro, fo, lo = o.compute2DL()
+    for a, b in zip(fg, fo):
+        assert same(a, b)
+    for a, b in zip(rg, ro):
+        assert same(a, b)
+    assert lg == lo
+
+
+def test_limiter_wd_pdop(gpu):
+    mesh = beach_mesh(N=12)
+    s, o = _pair(mesh, 1)
+    z, qx, qy = random_state(mesh, seed=5, dz=0.6, q=0.05)
+    z = z + 0.2
+    for obj in (s, o):
+        obj.set_state(z, qx, qy)
+        obj.wetDryStatus2D()
+    assert np.array_equal(s.get_wd(), o.get_wd())
+    s.SlopeLimiter()
+    mo = o.SlopeLimiter()
+    assert np.array_equal(s.get_modify(0), mo)
+    for a, b in zip(s.get_state(), o.get_state()):
+        assert same(a, b)
+    # PDop on a state with negative vertex depths
+    z2 = z - 0.3
+    for obj in (s, o):
+        obj.set_state(z2, qx, qy)
+        obj.PDop2D()
+    for a, b in zip(s.get_state(), o.get_state()):
+        assert same(a, b)
+
+
+def _compare_run(s, o, nsteps, chunk):
+    done = 0
+    while done < nsteps:
+        k = min(chunk, nsteps - done)
+        rep = s.run(k)
+        t, dt = o.run(k)
+        done += k
+        for a, b in zip(s.get_state(), o.get_state()):
+            assert same(a, b), "state differs after %d steps" % done
+        assert np.array_equal(s.get_wd(), o.get_wd()), "WD differs after %d steps" % done
+        for st in (0, 1):
+            assert np.array_equal(s.get_modify(st), o.get_modify(st)), "limiter flags differ after %d steps" % done
+        assert rep.t == t and rep.dt == dt, (done, rep.t, t, rep.dt, dt)
+
+
+def test_dam_break_trajectory(gpu):
+    wl = configs.dam_break(N=16, nsteps=60)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl)
+    _compare_run(s, o, 60, 20)
+
+
+def test_prescribed_boundaries_trajectory(gpu):
+    mesh = perturbed_mesh(N=10, seed=4)
+    s, o = _pair(mesh, 0, cfl=0.3)
+    z, qx, qy = random_state(mesh, seed=9, dz=0.05, q=0.05)
+    s.set_state(z, qx, qy)
+    o.set_state(z, qx, qy)
+    _compare_run(s, o, 40, 10)
+
+
+def test_wetdry_beach_trajectory(gpu):
+    mesh = beach_mesh(N=16)
+    s, o = _pair(mesh, 1, cfl=0.3)
+    z, qx, qy = beach_state(mesh)
+    s.set_state(z, qx, qy)
+    o.set_state(z, qx, qy)
+    _compare_run(s, o, 80, 20)
+
+
+def test_coastal_tide_rain_trajectory(gpu):
+    wl = configs.coastal(N=32, nsteps=60, rain_mm_h=30.0, bump_scale=0.1)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl)
+    _compare_run(s, o, 60, 20)
+
+
+def test_dam_break_1000_steps(gpu):
+    """The plain dam break (100x100 grid, 20,000 triangles, 1000 SSP-RK2 steps
+    at CFL 0.3, wet/dry and limiter on): bitwise equal to the oracle."""
+    wl = configs.dam_break(N=100, nsteps=1000)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl, threads=16)
+    _compare_run(s, o, 1000, 500)
+    a = np.concatenate(s.get_state())
+    b = np.concatenate(o.get_state())
+    assert rel_l2(a, b) == 0.0
+
+
+def test_one_time_step_host_dt(gpu):
+    mesh = perturbed_mesh(N=6, seed=3)
+    s, o = _pair(mesh, 0)
+    z, qx, qy = random_state(mesh, seed=3, dz=0.05, q=0.05)
+    s.set_state(z, qx, qy)
+    o.set_state(z, qx, qy)
+    for k in range(5):
+        s.oneTimeStep2D(0.0, 0.01)
+        o.oneTimeStep2D(0.0, 0.01)
+    for a, b in zip(s.get_state(), o.get_state()):
+        assert same(a, b)
