@@ -1,0 +1,137 @@
+"""Benchmark: FP64 DOF-updates/s per RK stage of the 2D DG shallow-water step.
+
+Workload (one MI355X): the coastal inundation mesh of 512 x 512 cells split
+into 524,288 triangles over 10 km x 10 km, bed sloping from -10 m to +2 m plus
+32 seeded Gaussian bumps, lognormal Manning n (median 0.025, sigma 0.3), tide
+1.5 m sin(2 pi t / 43200 s) on x = 0, reflecting walls elsewhere, p=1, wet/dry
+and limiter on, SSP-RK2 at CFL 0.3, started fully wet at +2.5 m (the
+still-water start stops on a NaN flux in the reference itself within 3 steps).
+
+A step is one SSP-RK2 step (two RK stages) of the whole mesh; a DOF-update is
+one of the 9 unknowns per element advanced through one stage.  Inputs are
+resident in HBM before timing; dt is computed on the device.
+
+Prints one JSON line (see the "value"/"roofline"/"cpu_baseline" fields).
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for p in (ROOT, os.path.join(ROOT, "dg-rain_amd")):
+    if p not in sys.path:
+        sys.path.insert(0, p)
+
+HBM_PEAK_GBS = 8000.0
+# algorithmic HBM bytes per element for one launch of the flux/RHS/update
+# kernel, averaged over the two stages: state in 72 + state out 72 + step-start
+# state 36 (72 in stage 2 only) + mesh 40 + element means out 24
+KSTAGE_BYTES_PER_EL = 244.0
+# algorithmic bytes per element per RK stage for the whole stage
+STAGE_BYTES_PER_EL = 268.0
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--N", type=int, default=512, help="grid cells per side (2*N*N triangles)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu", action="store_true")
+    return ap.parse_args()
+
+
+def load_traffic():
+    p = os.path.join(ROOT, "profiles", "k_stage_coastal_512_traffic.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            d = json.load(f)
+        return d.get("bytes_per_launch")
+    return None
+
+
+def cpu_baseline(wl, budget_s):
+    """The oracle (C restatement, device-math mode), one thread, timed on a
+    bounded number of steps of the same workload."""
+    import oracle
+    o = oracle.Oracle.for_workload(wl, math=oracle.MATH_DEV, threads=1)
+    o.run(1)                     # first step uses dt0 = 1e-7; not timed
+    steps = 0
+    t0 = time.perf_counter()
+    while True:
+        o.run(1)
+        steps += 1
+        el = time.perf_counter() - t0
+        if el >= budget_s or steps >= 50:
+            break
+    dofs = 9.0 * wl.num_el * 2 * steps
+    return {"value": dofs / el, "unit": "DOF-updates/s per RK stage", "cores": 1, "kind": "port",
+            "sample": "%d SSP-RK2 steps of the same %d-triangle workload, oracle C restatement, 1 thread, %.1f s"
+                      % (steps, wl.num_el, el)}
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
+              file=sys.stderr)
+        sys.exit(2)
+    import torch
+    from dgrain import configs, native
+
+    assert torch.cuda.is_available(), "bench.py needs a GPU"
+    torch.cuda.set_device(0)
+    wl = configs.coastal(N=a.N, start="flood")
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    s.run(a.warmup)
+    s.sync()
+    torch.cuda.synchronize()
+    s.profile_begin()
+    t0 = time.perf_counter()
+    rep = s.run(a.steps)
+    s.sync()
+    torch.cuda.synchronize()
+    el = time.perf_counter() - t0
+    k_ms, k_n = s.profile_end()
+    ne = wl.num_el
+    dof_updates = 9.0 * ne * 2 * a.steps
+    value = dof_updates / el
+    k_avg_s = (k_ms / k_n) * 1e-3
+    achieved = KSTAGE_BYTES_PER_EL * ne / k_avg_s / 1e9
+    traffic = load_traffic()
+    stage_gbs = STAGE_BYTES_PER_EL * ne * 2 * a.steps / el / 1e9
+    out = {
+        "metric": "FP64 DOF-updates/s per RK stage",
+        "value": value,
+        "unit": "DOF-updates/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": el / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": "coastal_inundation_flood_start_%dx%d" % (a.N, a.N), "triangles": ne,
+                   "dofs": 9 * ne, "p": 1, "rk": "SSP-RK2", "cfl": 0.3, "wetdry": True, "limiter": True,
+                   "tide_m": 1.5, "sim_time_s": rep.t},
+        "roofline": {"bound": "hbm", "kernel": "k_stage (flux+RHS+RK update)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "avg_launch_ms": k_avg_s * 1e3, "launches": k_n,
+                     "bytes_per_element": KSTAGE_BYTES_PER_EL},
+        "stage_roofline_frac": stage_gbs / HBM_PEAK_GBS,
+        "ensemble_speedup": None,
+        "adjoint_forward_cost": None,
+    }
+    if not a.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

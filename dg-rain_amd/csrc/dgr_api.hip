@@ -19,7 +19,8 @@ void launch_stage(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, co
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
-                  unsigned char *modify, Ctrl *ctrl, hipStream_t s);
+                  unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s);
+void launch_step_mark(Ctrl *c, double dt, hipStream_t s);
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s);
 void launch_step_begin(const PhysDev &ph, Ctrl *c, hipStream_t s);
@@ -48,6 +49,10 @@ struct dgr_handle {
     Ctrl *d_ctrl;
     // host staging
     std::vector<double> h_planes;
+    // optional HIP-event timing of the flux/RHS/update kernel (k_stage)
+    int profiling;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used;
 };
 
 #define DGR_HIP(call)                                                                   \
@@ -108,6 +113,7 @@ static dgr_status free_handle(dgr_handle *h)
                     h->d_ctrl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
+    for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DGR_OK;
@@ -256,6 +262,8 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
         memset(&c, 0, sizeof(c));
         c.t = 0.0;
         c.dt = prm->dt0;
+        c.err_key = ~0ULL;
+        c.nan_pos[0] = c.nan_pos[1] = -1;
         DGR_TRY(hipMemcpyAsync(h->d_ctrl, &c, sizeof(c), hipMemcpyHostToDevice, s));
     }
     DGR_TRY(hipStreamSynchronize(s));
@@ -312,6 +320,8 @@ static dgr_status planes_down(dgr_handle *h, const double *srcd, double *a, doub
 extern "C" dgr_status dgr_set_state(dgr_handle *h, const double *z, const double *qx, const double *qy)
 {
     if (!h || !z || !qx || !qy) return DGR_ERR_ARG;
+    const unsigned long long none = ~0ULL;   // a new state clears a recorded error
+    DGR_HIP(hipMemcpyAsync(&h->d_ctrl->err_key, &none, sizeof(none), hipMemcpyHostToDevice, h->stream));
     return planes_up(h, h->d_w, z, qx, qy);
 }
 
@@ -385,6 +395,8 @@ extern "C" dgr_status dgr_compute_rhs(dgr_handle *h, double *rz, double *rqx, do
     launch_reset_maxlam(h->d_ctrl, h->stream);
     launch_stage(0, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
                  nullptr, h->d_ctrl, 0, h->d_dbg, h->d_dbg + 9 * (size_t)h->pitch, h->stream);
+    launch_stage(3, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
+                 nullptr, h->d_ctrl, 0, nullptr, nullptr, h->stream);
     DGR_HIP(hipGetLastError());
     if (rz || rqx || rqy) {
         dgr_status st = planes_down(h, h->d_dbg, rz, rqx, rqy);
@@ -397,12 +409,8 @@ extern "C" dgr_status dgr_compute_rhs(dgr_handle *h, double *rz, double *rqx, do
     Ctrl c;
     dgr_status st = read_ctrl(h, &c);
     if (st) return st;
-    if (max_lambda) {
-        double ml;
-        memcpy(&ml, &c.maxlam[0], 8);
-        *max_lambda = ml;
-    }
-    return c.err_code ? DGR_ERR_NAN : DGR_OK;
+    if (max_lambda) *max_lambda = maxlam_of(c, 0);
+    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
 }
 
 extern "C" dgr_status dgr_limit(dgr_handle *h)
@@ -410,7 +418,7 @@ extern "C" dgr_status dgr_limit(dgr_handle *h)
     if (!h) return DGR_ERR_ARG;
     launch_means(h->m, h->d_w, h->d_means, h->stream);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, h->stream);
-    launch_limit(false, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, h->stream);
+    launch_limit(false, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, h->stream);
     DGR_HIP(hipGetLastError());
     DGR_HIP(hipStreamSynchronize(h->stream));
     return DGR_OK;
@@ -433,7 +441,19 @@ extern "C" dgr_status dgr_pdop(dgr_handle *h)
     Ctrl c;
     dgr_status st = read_ctrl(h, &c);
     if (st) return st;
-    return c.err_code ? DGR_ERR_NAN : DGR_OK;
+    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
+}
+
+// event pairs around k_stage launches when profiling is on
+static void mark(dgr_handle *h)
+{
+    if (!h->profiling) return;
+    if (h->ev_used == h->ev_pool.size()) {
+        hipEvent_t e;
+        if (hipEventCreate(&e) != hipSuccess) return;
+        h->ev_pool.push_back(e);
+    }
+    (void)hipEventRecord(h->ev_pool[h->ev_used++], h->stream);
 }
 
 // the two stages of oneTimeStep2D (time_evolution.c:10-246; wet/dry calls at the
@@ -443,29 +463,35 @@ static void enqueue_stages(dgr_handle *h)
     const bool wdon = h->prm.wetdry != 0;
     hipStream_t s = h->stream;
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, s);
+    mark(h);
     launch_stage(1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn, h->d_wd, h->d_means,
                  h->d_ctrl, 0, nullptr, nullptr, s);
+    mark(h);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
-    launch_limit(wdon, h->m, h->ph, h->d_w1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, s);
+    launch_limit(wdon, h->m, h->ph, h->d_w1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, s);
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w1, h->d_wd, s);
+    mark(h);
     launch_stage(2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn, h->d_wd, h->d_means,
                  h->d_ctrl, 1, nullptr, nullptr, s);
+    mark(h);
+    // stage 2's max |lambda| sets the next dt; repair its fold if a NaN operand was seen
+    launch_stage(3, wdon, h->m, h->ph, h->d_w1, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd, nullptr,
+                 h->d_ctrl, 1, nullptr, nullptr, s);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
-    launch_limit(wdon, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, s);
+    launch_limit(wdon, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1, s);
 }
 
 extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)
 {
     if (!h) return DGR_ERR_ARG;
     (void)t;
-    DGR_HIP(hipMemcpyAsync(&h->d_ctrl->dt, &dt, sizeof(double), hipMemcpyHostToDevice, h->stream));
-    launch_reset_maxlam(h->d_ctrl, h->stream);
+    launch_step_mark(h->d_ctrl, dt, h->stream);
     enqueue_stages(h);
     DGR_HIP(hipGetLastError());
     Ctrl c;
     dgr_status st = read_ctrl(h, &c);
     if (st) return st;
-    return c.err_code ? DGR_ERR_NAN : DGR_OK;
+    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
 }
 
 extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
@@ -483,13 +509,11 @@ extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
     if (rep) {
         rep->t = c.t;
         rep->dt = c.dt;
-        double ml;
-        memcpy(&ml, &c.maxlam[1], 8);
-        rep->max_lambda = ml;
-        rep->error_code = c.err_code;
-        rep->error_where = c.err_where;
+        rep->max_lambda = maxlam_of(c, 1);
+        rep->error_code = err_code_of(c.err_key);
+        rep->error_where = err_where_of(c.err_key);
     }
-    return c.err_code ? DGR_ERR_NAN : DGR_OK;
+    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
 }
 
 extern "C" dgr_status dgr_last_error(dgr_handle *h, int *code, int *where)
@@ -498,8 +522,8 @@ extern "C" dgr_status dgr_last_error(dgr_handle *h, int *code, int *where)
     Ctrl c;
     dgr_status st = read_ctrl(h, &c);
     if (st) return st;
-    if (code) *code = c.err_code;
-    if (where) *where = c.err_where;
+    if (code) *code = err_code_of(c.err_key);
+    if (where) *where = err_where_of(c.err_key);
     return DGR_OK;
 }
 
@@ -535,5 +559,32 @@ extern "C" dgr_status dgr_math_selftest(int n, const double *x, const double *y,
     (void)hipFree(dx);
     (void)hipFree(dy);
     (void)hipFree(dout);
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_profile_begin(dgr_handle *h)
+{
+    if (!h) return DGR_ERR_ARG;
+    h->profiling = 1;
+    h->ev_used = 0;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *launches)
+{
+    if (!h || !total_ms || !launches) return DGR_ERR_ARG;
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    double tot = 0.0;
+    int n = 0;
+    for (size_t i = 0; i + 1 < h->ev_used; i += 2) {
+        float ms = 0.f;
+        DGR_HIP(hipEventElapsedTime(&ms, h->ev_pool[i], h->ev_pool[i + 1]));
+        tot += ms;
+        ++n;
+    }
+    h->profiling = 0;
+    h->ev_used = 0;
+    *total_ms = tot;
+    *launches = n;
     return DGR_OK;
 }

@@ -46,9 +46,33 @@ struct Ctrl {
     double tide;              // tide elevation for the current step
     double pad0;
     unsigned long long maxlam[2];   // stage max |lambda| as raw bits (values >= 0)
-    int err_code, err_where;
-    int pad1[2];
+    // earliest error in the reference's execution order (atomic min):
+    // step << 34 | phase << 32 | id << 3 | code, phase = 2*stage + (0 flux, 1 PDop);
+    // all ones when no error has been raised
+    unsigned long long err_key;
+    long long step;                 // steps started (k_step_begin / k_step_mark)
+    // max |lambda| is the reference's left fold of max(a,b)=(a>b?a:b) over the
+    // edge-ordered operands (compute2DL.c:232-238,289-292); a NaN operand resets
+    // the fold.  nan_pos = last NaN operand position (-1: none); the fold's
+    // value is then the max of the operands after it (after_bits, any_after).
+    long long nan_pos[2];
+    unsigned long long after_bits[2];
+    int any_after[2];
 };
+
+__host__ __device__ inline double maxlam_of(const Ctrl &c, int slot)
+{
+    unsigned long long b;
+    if (c.nan_pos[slot] < 0) b = c.maxlam[slot];
+    else if (c.any_after[slot]) b = c.after_bits[slot];
+    else b = 0x7ff8000000000000ULL;
+    double d;
+    __builtin_memcpy(&d, &b, 8);
+    return d;
+}
+
+__host__ __device__ inline int err_code_of(unsigned long long key) { return key == ~0ULL ? 0 : (int)(key & 7); }
+__host__ __device__ inline int err_where_of(unsigned long long key) { return key == ~0ULL ? 0 : (int)((key >> 3) & 0x1fffffff); }
 
 struct PhysDev {
     double g, h0, velzero, rain;

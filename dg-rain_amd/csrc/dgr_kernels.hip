@@ -31,9 +31,14 @@ namespace dgr {
 
 __device__ __forceinline__ int isign(double a) { return a < 0 ? -1 : (a == 0 ? 0 : 1); }
 
-__device__ __forceinline__ void raise_error(Ctrl *c, int code, int where)
+// record an error; the smallest key (earliest step, stage, phase, then
+// lowest edge/element id) wins, which is the error the reference's serial
+// loops would have stopped at (compute2DL.c:245-302, wetDry2D.c:142-146,233-237)
+__device__ __forceinline__ void raise_error(Ctrl *c, int code, int where, int phase)
 {
-    if (atomicCAS(&c->err_code, 0, code) == 0) c->err_where = where;
+    const unsigned long long key = ((unsigned long long)c->step << 34) | ((unsigned long long)phase << 32) |
+                                   ((unsigned long long)(where & 0x1fffffff) << 3) | (unsigned long long)code;
+    atomicMin(&c->err_key, key);
 }
 
 // ---------------------------------------------------------------------------
@@ -292,12 +297,126 @@ __device__ __forceinline__ T wave_max(T v)
     return v;
 }
 
+// One max-|lambda| operand of the reference's fold, at position 2*edge (the
+// flux of a not-both-dry edge, or el_in's reflection flux of a both-dry edge)
+// or 2*edge+1 (el_ex's reflection flux of a both-dry edge).
+struct LamOps {
+    double v[6];
+    long long pos[6];
+    int n;
+};
+
+// The three Roe fluxes of element e in the reference's canonical orientation
+// (compute2DL.c:17-331), negated on the el_ex side, and the max-|lambda|
+// operands this element owns.
+template <bool WETDRY>
+__device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &ph, const double *__restrict__ win,
+                                               const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                               const int *__restrict__ wd, const Ctrl *ctrl, int e,
+                                               const double4 X[3], const double W[9], int wd_self, double Fh[9],
+                                               LamOps &ops, int &err_code, int &err_edge)
+{
+    const int P = m.pitch;
+    const double g = ph.g;
+    ops.n = 0;
+    err_code = 0;
+    err_edge = 0x7fffffff;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int inf = m.info[k * P + e];
+        const int nbk = m.nb[k * P + e];
+        const int ed = m.eid[k * P + e];
+        const int knb = inf & EI_KNB_MASK;
+        const bool is_in = (inf & EI_IS_IN) != 0;
+        const bool boundary = (inf & EI_BOUNDARY) != 0;
+        const double4 A = X[k], B = X[(k + 1) % 3];
+        const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+        const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
+        double x0, y0, x1, y1;
+        if (inf & EI_DIR_SAME) { x0 = A.x; y0 = A.y; x1 = B.x; y1 = B.y; }
+        else { x0 = B.x; y0 = B.y; x1 = A.x; y1 = A.y; }
+        const double nx = sg * (y1 - y0) / len;
+        const double ny = sg * (x0 - x1) / len;
+        const double tx = -ny, ty = nx;
+        const double zb = (A.z + B.z) / 2;
+        const double oz = W[k], oqx = W[3 + k], oqy = W[6 + k];
+        double zin, qxin, qyin, zex, qxex, qyex;
+        if (boundary) {
+            zin = oz; qxin = oqx; qyin = oqy;
+            zex = oz; qxex = oqx; qyex = oqy;
+        } else {
+            const double nz = win[knb * P + nbk], nqx = win[(3 + knb) * P + nbk], nqy = win[(6 + knb) * P + nbk];
+            if (is_in) { zin = oz; qxin = oqx; qyin = oqy; zex = nz; qxex = nqx; qyex = nqy; }
+            else { zin = nz; qxin = nqx; qyin = nqy; zex = oz; qxex = oqx; qyex = oqy; }
+        }
+        if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
+            const int code = (inf >> EI_CODE_SHIFT) & 0xff;
+            const bool tide = ph.tide_code != 0 && code == ph.tide_code;
+            zex = tide ? ctrl->tide : bzeta[ed];
+            double qnex = tide ? 1000000.0 : bqn[ed];
+            if (zex < 0) zex = zin;
+            if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+            const double qtex = qxin * tx + qyin * ty;
+            const double dn = 1. / (nx * ty - ny * tx);
+            qxex = (ty * qnex - ny * qtex) * dn;
+            qyex = (-tx * qnex + nx * qtex) * dn;
+        } else if (boundary) {                                       // :175-191
+            zex = oz;
+            reflect(qxin, qyin, nx, ny, qxex, qyex);
+        }
+        double F[3];
+        const long long p0 = 2LL * ed;
+        if (WETDRY) {
+            const int wd_in = is_in ? wd_self : wd[nbk];
+            const int wd_ex = boundary ? wd_self : (is_in ? wd[nbk] : wd_self);
+            if (wd_in == 0 && wd_ex == 0) {                          // :203-240
+                if (is_in) {
+                    double rx, ry;
+                    reflect(qxin, qyin, nx, ny, rx, ry);
+                    ops.v[ops.n] = roe_flux(F, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
+                    ops.pos[ops.n++] = p0;
+                    if (boundary) {
+                        double Fd[3], sx_, sy_;
+                        reflect(qxex, qyex, nx, ny, sx_, sy_);
+                        ops.v[ops.n] = roe_flux(Fd, zex, zex, sx_, qxex, sy_, qyex, zb, nx, ny, 0.0, g);
+                        ops.pos[ops.n++] = p0 + 1;
+                    }
+                } else {
+                    double rx, ry;
+                    reflect(qxex, qyex, nx, ny, rx, ry);
+                    ops.v[ops.n] = roe_flux(F, zex, zex, rx, qxex, ry, qyex, zb, nx, ny, 0.0, g);
+                    ops.pos[ops.n++] = p0 + 1;
+                    F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2];
+                }
+            } else {                                                 // :241-293
+                const int my = is_in ? wd_in : wd_ex;
+                ops.v[ops.n] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, (my == 1) ? g : 0.0, g);
+                ops.pos[ops.n++] = p0;
+                // a NaN here comes from the state, so the reference's first check
+                // (the g-flux, compute2DL.c:245) would already have stopped
+                if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed < err_edge) { err_code = 3; err_edge = ed; }
+                if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
+            }
+        } else {                                                     // :295-315
+            ops.v[ops.n] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
+            ops.pos[ops.n++] = p0;
+            if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed < err_edge) { err_code = 3; err_edge = ed; }
+            if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
+        }
+        Fh[k] = F[0];
+        Fh[3 + k] = F[1];
+        Fh[6 + k] = F[2];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // k_stage: flux gather + RHS + RK update of one element per thread.
 //   MODE 0: RHS only (compute2DL), written to dbg_rhs/dbg_fhat
 //   MODE 1: w_out = w_in + RHS*dt                       (time_evolution.c:72-88)
 //   MODE 2: w_out = (w_in + w_base)*0.5 + RHS*(dt/2)    (time_evolution.c:176-178,208-217)
 //           w_out may alias w_base (each element reads its own slot first).
+//   MODE 3: max-|lambda| fold repair after a NaN operand (see Ctrl); returns
+//           at once when the stage saw no NaN operand.
 // ---------------------------------------------------------------------------
 template <int MODE, bool WETDRY>
 __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const double *__restrict__ win,
@@ -307,12 +426,17 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
                                                Ctrl *ctrl, int slot, double *__restrict__ dbg_rhs,
                                                double *__restrict__ dbg_fhat)
 {
+    long long nan_after = -1;
+    if (MODE == 3) {
+        nan_after = ctrl->nan_pos[slot];
+        if (nan_after < 0) return;
+    }
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = e < m.ne;
     double lam = 0.0;
+    bool any = false;
     if (valid) {
         const int P = m.pitch;
-        const double g = ph.g;
         int v[3];
         double4 X[3];
 #pragma unroll
@@ -325,117 +449,58 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
         const int wd_self = wd[e];
 
         double Fh[9];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const int inf = m.info[k * P + e];
-            const int nbk = m.nb[k * P + e];
-            const int knb = inf & EI_KNB_MASK;
-            const bool is_in = (inf & EI_IS_IN) != 0;
-            const bool boundary = (inf & EI_BOUNDARY) != 0;
-            const double4 A = X[k], B = X[(k + 1) % 3];
-            const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
-            const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
-            double x0, y0, x1, y1;
-            if (inf & EI_DIR_SAME) { x0 = A.x; y0 = A.y; x1 = B.x; y1 = B.y; }
-            else { x0 = B.x; y0 = B.y; x1 = A.x; y1 = A.y; }
-            const double nx = sg * (y1 - y0) / len;
-            const double ny = sg * (x0 - x1) / len;
-            const double tx = -ny, ty = nx;
-            const double zb = (A.z + B.z) / 2;
-            const double oz = W[k], oqx = W[3 + k], oqy = W[6 + k];
-            double zin, qxin, qyin, zex, qxex, qyex;
-            if (boundary) {
-                zin = oz; qxin = oqx; qyin = oqy;
-            } else {
-                const double nz = win[knb * P + nbk], nqx = win[(3 + knb) * P + nbk], nqy = win[(6 + knb) * P + nbk];
-                if (is_in) { zin = oz; qxin = oqx; qyin = oqy; zex = nz; qxex = nqx; qyex = nqy; }
-                else { zin = nz; qxin = nqx; qyin = nqy; zex = oz; qxex = oqx; qyex = oqy; }
-            }
-            if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
-                const int code = (inf >> EI_CODE_SHIFT) & 0xff;
-                const int ed = m.eid[k * P + e];
-                const bool tide = ph.tide_code != 0 && code == ph.tide_code;
-                zex = tide ? ctrl->tide : bzeta[ed];
-                double qnex = tide ? 1000000.0 : bqn[ed];
-                if (zex < 0) zex = zin;
-                if (qnex > 1000) qnex = qxin * nx + qyin * ny;
-                const double qtex = qxin * tx + qyin * ty;
-                const double dn = 1. / (nx * ty - ny * tx);
-                qxex = (ty * qnex - ny * qtex) * dn;
-                qyex = (-tx * qnex + nx * qtex) * dn;
-            } else if (boundary) {                                       // :175-191
-                zex = oz;
-                reflect(qxin, qyin, nx, ny, qxex, qyex);
-            }
-            double F[3], lk;
-            if (WETDRY) {
-                const int wd_in = is_in ? wd_self : wd[nbk];
-                const int wd_ex = boundary ? wd_self : (is_in ? wd[nbk] : wd_self);
-                if (wd_in == 0 && wd_ex == 0) {                          // :203-240
-                    if (is_in) {
-                        double rx, ry;
-                        reflect(qxin, qyin, nx, ny, rx, ry);
-                        lk = roe_flux(F, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
-                        if (boundary) {   // the ex-side reflection flux also feeds max_lambda
-                            double Fd[3], sx_, sy_;
-                            reflect(qxex, qyex, nx, ny, sx_, sy_);
-                            const double le = roe_flux(Fd, zex, zex, sx_, qxex, sy_, qyex, zb, nx, ny, 0.0, g);
-                            lk = DGR_MAX(lk, le);
-                        }
-                    } else {
-                        const double qne = qxex * nx + qyex * ny, qte = qxex * tx + qyex * ty;
-                        const double qni = -qne, qti = qte;
-                        const double dn = 1. / (nx * ty - ny * tx);
-                        const double qxr = (ty * qni - ny * qti) * dn;
-                        const double qyr = (-tx * qni + nx * qte) * dn;
-                        lk = roe_flux(F, zex, zex, qxr, qxex, qyr, qyex, zb, nx, ny, 0.0, g);
-                        F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2];
-                    }
-                } else {                                                 // :241-293
-                    const int my = is_in ? wd_in : wd_ex;
-                    lk = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, (my == 1) ? g : 0.0, g);
-                    if (isnan(F[0]) || isnan(F[1]) || isnan(F[2]))
-                        raise_error(ctrl, (my == 1) ? 3 : (is_in ? 4 : 5), m.eid[k * P + e]);
-                    if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
-                }
-            } else {                                                     // :295-315
-                lk = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
-                if (isnan(F[0]) || isnan(F[1]) || isnan(F[2])) raise_error(ctrl, 3, m.eid[k * P + e]);
-                if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
-            }
-            Fh[k] = F[0];
-            Fh[3 + k] = F[1];
-            Fh[6 + k] = F[2];
-            lam = DGR_MAX(lam, lk);
-        }
-
-        double R[9];
-        element_rhs(X, W, Fh, wd_self, g, ph.rain, R);
-
-        if (MODE == 0) {
-#pragma unroll
-            for (int f = 0; f < 9; ++f) { dbg_rhs[f * P + e] = R[f]; dbg_fhat[f * P + e] = Fh[f]; }
+        LamOps ops;
+        int ecode, eedge;
+        element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, X, W, wd_self, Fh, ops, ecode, eedge);
+        if (MODE == 3) {
+            for (int i = 0; i < ops.n; ++i)
+                if (ops.pos[i] > nan_after) { lam = (ops.v[i] > lam) ? ops.v[i] : lam; any = true; }
         } else {
-            const double dt = ctrl->dt;
-            double U[9];
-            if (MODE == 1) {
-#pragma unroll
-                for (int f = 0; f < 9; ++f) U[f] = W[f] + R[f] * dt;
-            } else {
-                const double half = dt / 2;
-#pragma unroll
-                for (int f = 0; f < 9; ++f) U[f] = (W[f] + wbase[f * P + e]) * 0.5 + R[f] * half;
+            long long nanp = -1;
+            for (int i = 0; i < ops.n; ++i) {
+                const double x = ops.v[i];
+                if (x != x) nanp = (ops.pos[i] > nanp) ? ops.pos[i] : nanp;
+                else lam = (x > lam) ? x : lam;
             }
+            if (nanp >= 0) atomicMax(&ctrl->nan_pos[slot], nanp);
+            if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
+
+            double R[9];
+            element_rhs(X, W, Fh, wd_self, ph.g, ph.rain, R);
+
+            if (MODE == 0) {
 #pragma unroll
-            for (int f = 0; f < 9; ++f) wout[f * P + e] = U[f];
+                for (int f = 0; f < 9; ++f) { dbg_rhs[f * P + e] = R[f]; dbg_fhat[f * P + e] = Fh[f]; }
+            } else {
+                const double dt = ctrl->dt;
+                double U[9];
+                if (MODE == 1) {
 #pragma unroll
-            for (int f = 0; f < 3; ++f) means[f * P + e] = (U[3 * f] + U[3 * f + 1] + U[3 * f + 2]) / 3;
+                    for (int f = 0; f < 9; ++f) U[f] = W[f] + R[f] * dt;
+                } else {
+                    const double half = dt / 2;
+#pragma unroll
+                    for (int f = 0; f < 9; ++f) U[f] = (W[f] + wbase[f * P + e]) * 0.5 + R[f] * half;
+                }
+#pragma unroll
+                for (int f = 0; f < 9; ++f) wout[f * P + e] = U[f];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) means[f * P + e] = (U[3 * f] + U[3 * f + 1] + U[3 * f + 2]) / 3;
+            }
         }
     }
-    // max |lambda| (order-free): wave max, one 64-bit atomic max per wave
+    // order-free max of non-NaN operands: wave max, one 64-bit atomic per wave
     const unsigned long long bits = (unsigned long long)__double_as_longlong(lam);
     const unsigned long long wm = wave_max(bits);
-    if ((threadIdx.x & 63) == 0 && wm != 0ULL) atomicMax(&ctrl->maxlam[slot], wm);
+    if (MODE == 3) {
+        const unsigned long long anyw = __ballot(any);
+        if ((threadIdx.x & 63) == 0 && anyw) {
+            atomicMax(&ctrl->after_bits[slot], wm);
+            atomicOr(&ctrl->any_after[slot], 1);
+        }
+    } else if ((threadIdx.x & 63) == 0 && wm != 0ULL) {
+        atomicMax(&ctrl->maxlam[slot], wm);
+    }
 }
 
 // element means of the current state (standalone limiter calls)
@@ -550,7 +615,7 @@ __device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy
 // limiter element pass (SlopeLimiter2D.c:157-343), optionally followed by PDop2D
 template <bool PDOP>
 __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w, const double *__restrict__ nodemm,
-                                               unsigned char *__restrict__ modify_out, Ctrl *ctrl)
+                                               unsigned char *__restrict__ modify_out, Ctrl *ctrl, int stage)
 {
     const int e = blockIdx.x * blockDim.x + threadIdx.x;
     if (e >= m.ne) return;
@@ -607,7 +672,7 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w,
     }
     if (PDOP) {
         const int err = pdop_element(&S[0], &S[3], &S[6], zv, H0, ph.velzero);
-        if (err) raise_error(ctrl, err, e);
+        if (err) raise_error(ctrl, err, e, 2 * stage + 1);
     }
 #pragma unroll
     for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
@@ -626,7 +691,7 @@ __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, 
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = w[f * P + e];
     const int err = pdop_element(&S[0], &S[3], &S[6], zv, ph.h0, ph.velzero);
-    if (err) raise_error(ctrl, err, e);
+    if (err) raise_error(ctrl, err, e, 1);
 #pragma unroll
     for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
 }
@@ -669,13 +734,36 @@ __global__ void k_step_begin(PhysDev ph, Ctrl *c)
         c->tide = ph.tide_datum + ph.tide_amp * dgr_sin(6.283185307179586 * c->t / ph.tide_period + ph.tide_phase);
     c->maxlam[0] = 0ULL;
     c->maxlam[1] = 0ULL;
+    c->nan_pos[0] = -1;
+    c->nan_pos[1] = -1;
+    c->after_bits[0] = 0ULL;
+    c->after_bits[1] = 0ULL;
+    c->any_after[0] = 0;
+    c->any_after[1] = 0;
+    c->step += 1;
+}
+
+// oneTimeStep2D with a host dt: no tide update, only the step bookkeeping
+__global__ void k_step_mark(Ctrl *c, double dt)
+{
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->dt = dt;
+    c->maxlam[0] = 0ULL;
+    c->maxlam[1] = 0ULL;
+    c->nan_pos[0] = -1;
+    c->nan_pos[1] = -1;
+    c->after_bits[0] = 0ULL;
+    c->after_bits[1] = 0ULL;
+    c->any_after[0] = 0;
+    c->any_after[1] = 0;
+    c->step += 1;
 }
 
 __global__ void k_step_end(PhysDev ph, Ctrl *c)
 {
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
     const double time = c->t + c->dt;
-    const double ml = __longlong_as_double((long long)c->maxlam[1]);
+    const double ml = maxlam_of(*c, 1);
     double h = ph.cfl * ph.min_edge / ml;
     if ((time + h) > ph.final_time) h = ph.final_time - time;
     c->t = time;
@@ -684,7 +772,15 @@ __global__ void k_step_end(PhysDev ph, Ctrl *c)
 
 __global__ void k_reset_maxlam(Ctrl *c)
 {
-    if (threadIdx.x == 0 && blockIdx.x == 0) { c->maxlam[0] = 0ULL; c->maxlam[1] = 0ULL; }
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->maxlam[0] = 0ULL;
+    c->maxlam[1] = 0ULL;
+    c->nan_pos[0] = -1;
+    c->nan_pos[1] = -1;
+    c->after_bits[0] = 0ULL;
+    c->after_bits[1] = 0ULL;
+    c->any_after[0] = 0;
+    c->any_after[1] = 0;
 }
 
 __global__ void k_math_selftest(int n, const double *x, const double *y, double *out)
@@ -712,9 +808,11 @@ void launch_stage(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, co
 #define DGR_LS(MD, WDF) \
     hipLaunchKernelGGL((k_stage<MD, WDF>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
     if (wetdry) {
-        if (mode == 0) DGR_LS(0, true); else if (mode == 1) DGR_LS(1, true); else DGR_LS(2, true);
+        if (mode == 0) DGR_LS(0, true); else if (mode == 1) DGR_LS(1, true); else if (mode == 2) DGR_LS(2, true);
+        else DGR_LS(3, true);
     } else {
-        if (mode == 0) DGR_LS(0, false); else if (mode == 1) DGR_LS(1, false); else DGR_LS(2, false);
+        if (mode == 0) DGR_LS(0, false); else if (mode == 1) DGR_LS(1, false); else if (mode == 2) DGR_LS(2, false);
+        else DGR_LS(3, false);
     }
 #undef DGR_LS
 }
@@ -730,10 +828,10 @@ void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, h
 }
 
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
-                  unsigned char *modify, Ctrl *ctrl, hipStream_t s)
+                  unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s)
 {
-    if (pdop) hipLaunchKernelGGL(k_limit<true>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl);
-    else hipLaunchKernelGGL(k_limit<false>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl);
+    if (pdop) hipLaunchKernelGGL(k_limit<true>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
+    else hipLaunchKernelGGL(k_limit<false>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
 }
 
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s)
@@ -754,6 +852,11 @@ void launch_step_begin(const PhysDev &ph, Ctrl *c, hipStream_t s)
 void launch_step_end(const PhysDev &ph, Ctrl *c, hipStream_t s)
 {
     hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, ph, c);
+}
+
+void launch_step_mark(Ctrl *c, double dt, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_mark, dim3(1), dim3(64), 0, s, c, dt);
 }
 
 void launch_reset_maxlam(Ctrl *c, hipStream_t s)

@@ -66,22 +66,50 @@ def coastal_bed(x, y, L, nbumps, rng, scale=1.0):
     return b
 
 
-def coastal(N=512, L=1.0e4, nsteps=2000, seed=2, nbumps=32, tide_amp=1.5, rain_mm_h=0.0, bump_scale=1.0):
-    """Wet/dry coastal inundation driven by a tide on x=0."""
+def coastal(N=512, L=1.0e4, nsteps=2000, seed=2, nbumps=32, tide_amp=1.5, rain_mm_h=0.0, bump_scale=1.0,
+            start="flood", level=2.5, tide_phase=0.0):
+    """Coastal inundation driven by a tide on x=0, wet/dry and limiter on.
+
+    Bed: slope from -10 m (x=0) to +2 m (x=L) plus `nbumps` Gaussian bumps;
+    Manning n lognormal (median 0.025, sigma 0.3) per node.
+
+    start="still": still water at level 0 over that bed (the land above 0 m
+        starts dry).  The reference's wet/dry scheme cannot hold this shoreline
+        at rest: the compiled reference and the oracle both stop on a NaN flux
+        within 3 steps (tests/test_oracle_vs_reference.py).
+    start="flood" (default): still water at +`level` m, every element wet, the
+        tide oscillating about that level.  Same mesh, bed, friction, forcing
+        and code paths (wet/dry status, PDop, limiter run every stage); this is
+        the variant the reference runs to 2000 steps.
+    """
     rng = np.random.default_rng(seed)
     x, y = M.grid_nodes(N, L)
     bed = coastal_bed(x, y, L, nbumps, rng, bump_scale)
     depth = -bed
     n = 0.025 * np.exp(0.3 * rng.standard_normal(x.size))
-    D = TIDE_DATUM_SHIFT
-    zeta_nodes = np.maximum(0.0, H0 - depth) + D
+    if start == "still":
+        D, surface = TIDE_DATUM_SHIFT, 0.0
+    elif start == "flood":
+        D, surface = 0.0, float(level)
+    else:
+        raise ValueError(start)
+    zeta_nodes = np.maximum(surface, H0 - depth) + D
     z = depth - D
     mesh = M.from_elements(x, y, z, n, M.grid_elements(N), out=M.side_pairs(N, "left"))
     zeta = M.nodal_to_midpoints(mesh, zeta_nodes)
     zero = np.zeros_like(zeta)
-    params = dict(cfl=0.3, wetdry=1, tide_code=2, tide_datum=D, tide_amp=tide_amp, tide_period=43200.0,
-                  tide_phase=0.0, rain=rain_mm_h / 1000.0 / 3600.0)
-    return Workload("coastal_%dx%d" % (N, N), mesh, zeta, zero, zero.copy(), params, nsteps)
+    params = dict(cfl=0.3, wetdry=1, tide_code=2, tide_datum=D + surface, tide_amp=tide_amp, tide_period=43200.0,
+                  tide_phase=tide_phase, rain=rain_mm_h / 1000.0 / 3600.0)
+    return Workload("coastal_%s_%dx%d" % (start, N, N), mesh, zeta, zero, zero.copy(), params, nsteps)
+
+
+def dry_bed_dam_break(N=16, L=10.0, nsteps=100):
+    """Dam break onto a dry flat bed (the pattern of initialize.c:65-73):
+    H = 0.005 m for x <= L/2, H0 elsewhere; wet/dry on, walls."""
+    mesh = M.grid_mesh(N, L, z=0.0, n=0.02)
+    zeta = M.nodal_to_midpoints(mesh, np.where(mesh.x <= L / 2, 0.005, H0))
+    zero = np.zeros_like(zeta)
+    return Workload("dry_bed_dam_break_%d" % N, mesh, zeta, zero, zero.copy(), dict(cfl=0.3, wetdry=1), nsteps)
 
 
 def lake_at_rest(N=8, L=1000.0, depth=1.0, zeta0=0.1, bumps=True, seed=0):
@@ -102,4 +130,5 @@ WORKLOADS = {
     "dam_break": dam_break,
     "coastal": coastal,
     "lake_at_rest": lake_at_rest,
+    "dry_bed_dam_break": dry_bed_dam_break,
 }

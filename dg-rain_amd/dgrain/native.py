@@ -82,6 +82,8 @@ def load():
         "dgr_sync": (C.c_int, [vp]),
         "dgr_device_state": (C.c_int, [vp, C.POINTER(dp), ip]),
         "dgr_math_selftest": (C.c_int, [C.c_int, dp, dp, dp]),
+        "dgr_profile_begin": (C.c_int, [vp]),
+        "dgr_profile_end": (C.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -240,6 +242,16 @@ class Solver:
 
     def sync(self):
         check(self.lib.dgr_sync(self.h))
+
+    def profile_begin(self):
+        check(self.lib.dgr_profile_begin(self.h))
+
+    def profile_end(self):
+        """(summed ms, launches) of the flux/RHS/update kernel since profile_begin."""
+        ms = C.c_double(0)
+        n = C.c_int(0)
+        check(self.lib.dgr_profile_end(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
 
     def device_state(self):
         p = C.POINTER(C.c_double)()

@@ -141,6 +141,13 @@ dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt);
 dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *report);
 dgr_status dgr_last_error(dgr_handle *h, int *code, int *where);
 
+/* Kernel timing for measurement: between dgr_profile_begin and
+ * dgr_profile_end every flux/RHS/update kernel launch (one per RK stage) is
+ * bracketed by HIP events on the handle's stream; end synchronises and returns
+ * their summed duration and count. */
+dgr_status dgr_profile_begin(dgr_handle *h);
+dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *launches);
+
 /* Synchronise the handle's stream. */
 dgr_status dgr_sync(dgr_handle *h);
 

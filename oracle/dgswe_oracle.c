@@ -677,7 +677,10 @@ int orc_compute_L(orc_state *s)
             double qxinr = (ty * qnin - ny * qtin) * dn;
             double qyinr = (-tx * qnin + nx * qtex) * dn;
             double mle = orc_roe_flux(Fex, zinr, zex, qxinr, qxex, qyinr, qyex, zb, nx, ny, 0, g);
-            cur = ORC_MAX(mli, mle);
+            /* two sequential folds (:232-238); with a NaN operand the order matters */
+            if (i == 0) maxl = ORC_MAX(mli, mle);
+            else { maxl = ORC_MAX(maxl, mli); maxl = ORC_MAX(maxl, mle); }
+            goto store;
         } else {
             double Fh[3];
             cur = orc_roe_flux(Fh, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
@@ -697,6 +700,7 @@ int orc_compute_L(orc_state *s)
             }
         }
         maxl = (i == 0) ? cur : ORC_MAX(maxl, cur);                                              /* :310-313 */
+    store:
         s->f1[3 * el_in + kin] = Fin[0];                                                         /* :319-329 */
         s->f2[3 * el_in + kin] = Fin[1];
         s->f3[3 * el_in + kin] = Fin[2];

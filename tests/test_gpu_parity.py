@@ -5,6 +5,8 @@ zero, for states, RHS, edge fluxes, max_lambda, dt, wet/dry flags and limiter
 activations.  (The oracle's reference-math mode is pinned to the compiled
 reference in test_oracle_vs_reference.py; the two modes are compared to the
 north-star tolerance rel-L2 <= 1e-11 in test_oracle_modes.py.)"""
+import os
+
 import numpy as np
 import pytest
 
@@ -93,6 +95,7 @@ def _compare_run(s, o, nsteps, chunk):
         for st in (0, 1):
             assert np.array_equal(s.get_modify(st), o.get_modify(st)), "limiter flags differ after %d steps" % done
         assert rep.t == t and rep.dt == dt, (done, rep.t, t, rep.dt, dt)
+        print("  %d/%d steps bitwise equal (t=%.6g)" % (done, nsteps, t), flush=True)
 
 
 def test_dam_break_trajectory(gpu):
@@ -112,17 +115,42 @@ def test_prescribed_boundaries_trajectory(gpu):
     _compare_run(s, o, 40, 10)
 
 
-def test_wetdry_beach_trajectory(gpu):
+def test_dry_bed_dam_break_trajectory(gpu):
+    wl = configs.dry_bed_dam_break(N=24, nsteps=120)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl)
+    _compare_run(s, o, 120, 40)
+
+
+def test_wetdry_failure_matches_oracle(gpu):
+    """A sloping beach whose dry films the reference scheme cannot hold: the
+    oracle stops on a NaN flux; the device must report the same error code,
+    edge and step (the earliest error in the reference's loop order)."""
     mesh = beach_mesh(N=16)
     s, o = _pair(mesh, 1, cfl=0.3)
     z, qx, qy = beach_state(mesh)
     s.set_state(z, qx, qy)
     o.set_state(z, qx, qy)
-    _compare_run(s, o, 80, 20)
+    for step in range(1, 50):
+        try:
+            o.run(1)
+        except RuntimeError as ex:
+            msg = str(ex)
+            break
+        s.run(1)
+        for a, b in zip(s.get_state(), o.get_state()):
+            assert same(a, b), "state differs after %d steps" % step
+    else:
+        pytest.fail("expected the oracle to stop on this input")
+    with pytest.raises(native.DgrError) as ei:
+        s.run(1)
+    code, where = msg.split("error ")[1].split(" at ")
+    assert "error %s at %s" % (code, where) in str(ei.value), (msg, str(ei.value))
 
 
 def test_coastal_tide_rain_trajectory(gpu):
-    wl = configs.coastal(N=32, nsteps=60, rain_mm_h=30.0, bump_scale=0.1)
+    wl = configs.coastal(N=32, nsteps=60, rain_mm_h=30.0)
     s = native.Solver(wl.mesh, wl.solver_params())
     s.set_state(wl.zeta, wl.qx, wl.qy)
     o = oracle.Oracle.for_workload(wl)
@@ -135,8 +163,8 @@ def test_dam_break_1000_steps(gpu):
     wl = configs.dam_break(N=100, nsteps=1000)
     s = native.Solver(wl.mesh, wl.solver_params())
     s.set_state(wl.zeta, wl.qx, wl.qy)
-    o = oracle.Oracle.for_workload(wl, threads=16)
-    _compare_run(s, o, 1000, 500)
+    o = oracle.Oracle.for_workload(wl, threads=min(16, os.cpu_count() or 1))
+    _compare_run(s, o, 1000, 100)
     a = np.concatenate(s.get_state())
     b = np.concatenate(o.get_state())
     assert rel_l2(a, b) == 0.0

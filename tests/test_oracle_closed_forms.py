@@ -1,0 +1,246 @@
+"""Closed-form pins of the oracle, independent of the reference binary.
+
+Each check is chosen so that a plausible slip (a dropped term, a wrong sign,
+a swapped index or operand) fails it:
+  * Roe flux == a textbook Roe solver built from the Roe-averaged Jacobian and
+    numpy's eigensolver (Toro, Shock-Capturing Methods, 2nd ed., ch. 10/11),
+    with the reference's entropy fix; consistency and conservation symmetry.
+  * Surface integral SI_k == |e_k| Fhat_k (CR basis, RT0 normal flux).
+  * Free-stream preservation: a uniform state on a flat bed has zero RHS.
+  * Lake at rest over bumps stays at rest; mass is conserved with walls.
+  * Uniform rain over a closed basin raises the surface by exactly R t.
+  * Limiter: element means preserved, vertex values inside node bounds.
+  * PDop: vertex depths >= H0 afterwards.
+  * pow43 against exact decimal arithmetic; the tide sine against libm.
+"""
+import math
+from decimal import Decimal, getcontext
+
+import numpy as np
+import pytest
+
+import oracle
+from dgrain import configs, mesh as M
+from helpers import perturbed_mesh, random_state
+
+G = 9.81
+
+
+def textbook_roe(zl, zr, qxl, qxr, qyl, qyr, zb, nx, ny):
+    """Roe flux for the SWE in (zeta, qx, qy) with the well-balanced pressure
+    0.5 g (H^2 - z^2) and the reference's Harten-type entropy fix."""
+    def flux(z, qx, qy):
+        H = z + zb
+        u, v = qx / H, qy / H
+        p = 0.5 * G * (H * H - zb * zb)
+        return np.array([qx * nx + qy * ny,
+                         (qx * u + p) * nx + qx * v * ny,
+                         qy * u * nx + (qy * v + p) * ny])
+    HL, HR = zl + zb, zr + zb
+    uL, vL, uR, vR = qxl / HL, qyl / HL, qxr / HR, qyr / HR
+    Hh = 0.5 * (HL + HR)
+    sL, sR = math.sqrt(HL), math.sqrt(HR)
+    uh, vh = (uL * sL + uR * sR) / (sL + sR), (vL * sL + vR * sR) / (sL + sR)
+    c = math.sqrt(G * Hh)
+    un = uh * nx + vh * ny
+    # normal flux Jacobian d(F.n)/d(zeta,qx,qy) at the Roe state
+    A = np.array([[0.0, nx, ny],
+                  [(c * c - uh * uh) * nx - uh * vh * ny, 2 * uh * nx + vh * ny, uh * ny],
+                  [-uh * vh * nx + (c * c - vh * vh) * ny, vh * nx, uh * nx + 2 * vh * ny]])
+    lam, V = np.linalg.eig(A)
+    order = np.argsort(lam.real)
+    lam, V = lam.real[order], V.real[:, order]
+    lL = np.array([uL * nx + vL * ny - math.sqrt(G * HL), uL * nx + vL * ny, uL * nx + vL * ny + math.sqrt(G * HL)])
+    lR = np.array([uR * nx + vR * ny - math.sqrt(G * HR), uR * nx + vR * ny, uR * nx + vR * ny + math.sqrt(G * HR)])
+    eps = np.maximum(np.maximum(0.0, lam - lL), lR - lam)
+    absl = np.maximum(eps, np.abs(lam))
+    diss = V @ np.diag(absl) @ np.linalg.inv(V) @ np.array([zl - zr, qxl - qxr, qyl - qyr])
+    F = 0.5 * (flux(zl, qxl, qyl) + flux(zr, qxr, qyr)) + 0.5 * diss
+    mx = max(np.max(np.abs(lL)), np.max(np.abs(lR)))
+    return F, mx, un
+
+
+def test_roe_flux_against_textbook_solver():
+    rng = np.random.default_rng(0)
+    for _ in range(300):
+        zb = rng.uniform(0.5, 5)
+        zl, zr = rng.uniform(-0.3, 0.5, 2)
+        q = rng.uniform(-2, 2, 4)
+        th = rng.uniform(0, 2 * math.pi)
+        nx, ny = math.cos(th), math.sin(th)
+        F, lam = oracle.roe_flux(zl, zr, q[0], q[1], q[2], q[3], zb, nx, ny, G)
+        Ft, lt, _ = textbook_roe(zl, zr, q[0], q[1], q[2], q[3], zb, nx, ny)
+        scale = 1.0 + np.max(np.abs(Ft))
+        assert np.max(np.abs(F - Ft)) <= 1e-12 * scale
+        assert abs(lam - lt) <= 1e-14 * lt
+
+
+def test_roe_flux_consistency_and_symmetry():
+    rng = np.random.default_rng(1)
+    for _ in range(300):
+        zb = rng.uniform(0.5, 5)
+        z = rng.uniform(-0.3, 0.5)
+        qx, qy = rng.uniform(-2, 2, 2)
+        th = rng.uniform(0, 2 * math.pi)
+        nx, ny = math.cos(th), math.sin(th)
+        F, _ = oracle.roe_flux(z, z, qx, qx, qy, qy, zb, nx, ny, G)
+        H = z + zb
+        exact = np.array([qx * nx + qy * ny,
+                          (qx * qx / H + 0.5 * G * (H * H - zb * zb)) * nx + qx * qy / H * ny,
+                          qx * qy / H * nx + (qy * qy / H + 0.5 * G * (H * H - zb * zb)) * ny])
+        assert np.max(np.abs(F - exact)) <= 1e-12 * (1 + np.max(np.abs(exact)))
+        z2 = rng.uniform(-0.3, 0.5)
+        q2 = rng.uniform(-2, 2, 2)
+        F1, l1 = oracle.roe_flux(z, z2, qx, q2[0], qy, q2[1], zb, nx, ny, G)
+        F2, l2 = oracle.roe_flux(z2, z, q2[0], qx, q2[1], qy, zb, -nx, -ny, G)
+        assert np.max(np.abs(F1 + F2)) <= 1e-12 * (1 + np.max(np.abs(F1)))
+        assert l1 == l2
+
+
+def test_surface_integral_is_edge_length_times_flux():
+    m = perturbed_mesh(N=6, seed=2)
+    o = oracle.Oracle(m, math=oracle.MATH_REF)
+    o.set_state(*random_state(m, seed=3))
+    _, f, _ = o.compute2DL()
+    ev = m.elto_vert.reshape(-1, 3)
+    for el in range(m.num_el):
+        SI, _ = o.inner_products(el)
+        for k in range(3):
+            a, b = ev[el, k], ev[el, (k + 1) % 3]
+            L = math.hypot(m.x[b] - m.x[a], m.y[b] - m.y[a])
+            for fld in range(3):
+                expect = L * f[fld][3 * el + k]
+                assert abs(SI[3 * fld + k] - expect) <= 1e-10 * (1 + abs(expect)), (el, k, fld)
+
+
+def test_free_stream_preservation():
+    """Uniform (H, u, v) on a flat bed without friction: every interior
+    element's RHS vanishes (surface and volume terms cancel)."""
+    N = 6
+    x, y = M.grid_nodes(N, 60.0)
+    rng = np.random.default_rng(4)
+    inner = (x > 0) & (x < 60) & (y > 0) & (y < 60)
+    x = x + rng.uniform(-2, 2, x.size) * inner
+    y = y + rng.uniform(-2, 2, y.size) * inner
+    m = M.from_elements(x, y, np.full(x.size, 2.0), np.zeros(x.size), M.grid_elements(N))
+    ne3 = 3 * m.num_el
+    o = oracle.Oracle(m, math=oracle.MATH_REF)
+    o.set_state(np.full(ne3, 0.3), np.full(ne3, 0.7), np.full(ne3, -0.4))
+    r, _, _ = o.compute2DL()
+    e2e = m.edg_to_els.reshape(-1, 2)
+    boundary_els = set(e2e[e2e[:, 0] == e2e[:, 1], 0])
+    interior = np.array([el for el in range(m.num_el) if el not in boundary_els])
+    for fld in range(3):
+        v = r[fld].reshape(-1, 3)[interior]
+        assert np.max(np.abs(v)) < 1e-11
+
+
+def test_lake_at_rest_and_mass_conservation():
+    wl = configs.lake_at_rest(N=8)
+    o = oracle.Oracle.for_workload(wl, math=oracle.MATH_REF)
+    o.run(50)
+    z, qx, qy = o.get_state()
+    assert np.max(np.abs(z - wl.zeta)) < 1e-11
+    assert np.max(np.abs(qx)) < 1e-10 and np.max(np.abs(qy)) < 1e-10
+    # dam break with walls: total mass sum |K| mean(zeta + zmid) constant
+    wl = configs.dam_break(N=12, L=1200.0)
+    m = wl.mesh
+    ev = m.elto_vert.reshape(-1, 3)
+    area = 0.5 * np.abs((m.x[ev[:, 1]] - m.x[ev[:, 0]]) * (m.y[ev[:, 2]] - m.y[ev[:, 0]]) -
+                        (m.x[ev[:, 2]] - m.x[ev[:, 0]]) * (m.y[ev[:, 1]] - m.y[ev[:, 0]]))
+    o = oracle.Oracle.for_workload(wl, math=oracle.MATH_REF)
+    mass0 = np.sum(area * wl.zeta.reshape(-1, 3).mean(1))
+    o.run(60)
+    mass1 = np.sum(area * o.get_state()[0].reshape(-1, 3).mean(1))
+    assert abs(mass1 - mass0) <= 1e-13 * mass0
+
+
+def test_rain_fills_closed_basin_at_rate_R():
+    wl = configs.lake_at_rest(N=6, bumps=False)
+    R = 50e-3 / 3600
+    o = oracle.Oracle(wl.mesh, math=oracle.MATH_DEV, wetdry=1, cfl=0.3, rain=R)
+    o.set_state(wl.zeta, wl.qx, wl.qy)
+    t, _ = o.run(40)
+    z, qx, qy = o.get_state()
+    assert np.max(np.abs(z - (wl.zeta + R * t))) <= 1e-12 * (1 + R * t)
+    assert np.max(np.abs(qx)) < 1e-12 and np.max(np.abs(qy)) < 1e-12
+
+
+def test_limiter_preserves_means_and_respects_node_bounds():
+    m = perturbed_mesh(N=8, seed=9)
+    o = oracle.Oracle(m, math=oracle.MATH_REF)
+    st = random_state(m, seed=9, dz=0.3, q=0.4)
+    o.set_state(*st)
+    mod = o.SlopeLimiter()
+    after = o.get_state()
+    ev = m.elto_vert.reshape(-1, 3)
+    for fld in range(3):
+        before_m = st[fld].reshape(-1, 3).mean(1)
+        after_m = after[fld].reshape(-1, 3).mean(1)
+        assert np.max(np.abs(before_m - after_m)) < 1e-14 * 10
+        w = after[fld].reshape(-1, 3)
+        V = np.stack([w[:, 0] - w[:, 1] + w[:, 2], w[:, 0] + w[:, 1] - w[:, 2], -w[:, 0] + w[:, 1] + w[:, 2]], 1)
+        nmin = np.full(m.num_nodes, np.inf)
+        nmax = np.full(m.num_nodes, -np.inf)
+        for k in range(3):
+            np.minimum.at(nmin, ev[:, k], before_m)
+            np.maximum.at(nmax, ev[:, k], before_m)
+        sel = mod.astype(bool)
+        assert sel.sum() > 0
+        assert np.all(V[sel] >= nmin[ev[sel]] - 1e-12) and np.all(V[sel] <= nmax[ev[sel]] + 1e-12)
+
+
+def test_pdop_restores_positive_depth():
+    """Vertex depths end >= H0 and mass is kept, except in the reference's
+    index-1 quirk: its next-minimum search starts at index 1
+    (wetDry2D.c:115-124), so when vertex 1 is the minimum both indices
+    coincide and vertex 0 absorbs the whole deficit."""
+    m = perturbed_mesh(N=8, seed=12, depth=0.3, codes=False)
+    o = oracle.Oracle(m, math=oracle.MATH_REF, wetdry=1)
+    z, qx, qy = random_state(m, seed=12, dz=0.45, q=0.01)
+    zm = M.nodal_to_midpoints(m, m.z).reshape(-1, 3)
+
+    def vertices(w):
+        return np.stack([w[:, 0] - w[:, 1] + w[:, 2], w[:, 0] + w[:, 1] - w[:, 2], -w[:, 0] + w[:, 1] + w[:, 2]], 1)
+
+    before = vertices(z.reshape(-1, 3) + zm)
+    o.set_state(z, qx, qy)
+    o.PDop2D()
+    z2, _, _ = o.get_state()
+    Ht = vertices(z2.reshape(-1, 3) + zm)
+    wet = before.mean(1) > 1e-3
+    quirk = np.argmin(before, 1) == 1
+    ok = wet & ~quirk & (before.min(1) <= 1e-3)
+    assert ok.sum() > 5
+    assert np.all(Ht[ok] >= 1e-3 - 1e-12)
+    assert np.allclose(Ht[ok].mean(1), before[ok].mean(1), rtol=0, atol=1e-12)
+    q = wet & quirk & (before.min(1) <= 1e-3)
+    assert q.sum() > 0
+    assert np.allclose(Ht[q, 1], 1e-3, rtol=0, atol=1e-12) and np.allclose(Ht[q, 2], before[q, 2], rtol=0, atol=1e-12)
+
+
+def _exact_pow43(x):
+    getcontext().prec = 60
+    y = Decimal(4.0 / 3.0)   # the double 4./3, exactly
+    return float(Decimal(x) ** y)
+
+
+def test_pow43_is_correctly_rounded_on_samples():
+    rng = np.random.default_rng(13)
+    xs = np.concatenate([np.exp(rng.uniform(math.log(1e-4), math.log(1e4), 3000)), [1.0, 8.0, 27.0, 0.001, 1e-200, 1e200]])
+    bad = 0
+    for x in xs:
+        p = oracle.pow43(x)
+        e = _exact_pow43(x)
+        assert abs(p - e) <= math.ulp(e), x
+        bad += p != e
+    assert bad == 0
+    assert oracle.pow43(0.0) == 0.0 and math.isnan(oracle.pow43(-1.0)) and oracle.pow43(math.inf) == math.inf
+    assert oracle.pow43(1e300) == math.inf and oracle.pow43(1e-300) == 0.0
+
+
+def test_tide_sine_against_libm():
+    rng = np.random.default_rng(14)
+    for x in np.concatenate([rng.uniform(-100, 100, 5000), rng.uniform(-1e4, 1e4, 500), [0.0, math.pi, -math.pi / 2]]):
+        s = oracle.dsin(x)
+        assert abs(s - math.sin(x)) <= 2 * math.ulp(max(abs(math.sin(x)), 1e-300)) + 1e-300, x

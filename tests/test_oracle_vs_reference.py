@@ -1,0 +1,107 @@
+"""The oracle against the compiled reference operators themselves
+(oracle/_ref, built from the reference sources by oracle/Makefile).  Random
+inputs beyond the golden vectors; bit-for-bit in reference-math mode.
+Skipped where the reference sources are not mounted."""
+import numpy as np
+import pytest
+
+import oracle
+from helpers import perturbed_mesh, random_state, same
+from conftest import require_ref
+
+pytestmark = pytest.mark.ref
+
+
+@pytest.fixture(autouse=True)
+def _need_ref():
+    require_ref()
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_compute2DL_and_limiter_random(seed):
+    m = perturbed_mesh(N=7 + seed, seed=seed)
+    ref = oracle.Ref2D(m)
+    o = oracle.Oracle(m, math=oracle.MATH_REF)
+    st = random_state(m, seed=100 + seed)
+    ref.set_state(*st)
+    o.set_state(*st)
+    r1, f1, l1 = ref.compute2DL()
+    r2, f2, l2 = o.compute2DL()
+    assert all(same(a, b) for a, b in zip(f1, f2))
+    assert all(same(a, b) for a, b in zip(r1, r2))
+    assert l1 == l2
+    ref.SlopeLimiter()
+    o.SlopeLimiter()
+    assert all(same(a, b) for a, b in zip(ref.get_state(), o.get_state()))
+
+
+def test_roe_flux_random():
+    m = perturbed_mesh(N=3)
+    oracle.Ref2D(m)
+    rng = np.random.default_rng(5)
+    for _ in range(2000):
+        a = list(rng.uniform(-1, 1, 6)) + [rng.uniform(1.2, 4)] + [0.6, 0.8, rng.choice([0.0, 9.81])]
+        F1, l1 = oracle.ref_roe_flux(*a)
+        F2, l2 = oracle.roe_flux(*a)
+        assert same(F1, F2) and l1 == l2
+
+
+def test_2dcode_steps_random():
+    m = perturbed_mesh(N=9, seed=8)
+    ref = oracle.Ref2D(m)
+    o = oracle.Oracle(m, math=oracle.MATH_REF)
+    st = random_state(m, seed=3, dz=0.05, q=0.05)
+    ref.set_state(*st)
+    o.set_state(*st)
+    for _ in range(15):
+        ref.oneTimeStep2D(0.0, 0.05)
+        o.oneTimeStep2D(0.0, 0.05)
+    assert all(same(a, b) for a, b in zip(ref.get_state(), o.get_state()))
+
+
+def test_wetdry_operators_against_both_references():
+    """wetDryStatus2D and PDop2D of 2DCode and ChanNet agree with the oracle."""
+    m = perturbed_mesh(N=8, seed=6, depth=0.3)
+    rng = np.random.default_rng(6)
+    st = list(random_state(m, seed=6, dz=0.25, q=0.05))
+    wd = rng.integers(-1, 2, m.num_el).astype(np.int32)
+    outs = []
+    for impl in (oracle.Ref2D(m), oracle.RefChan(m), oracle.Oracle(m, math=oracle.MATH_REF, wetdry=1)):
+        impl.set_state(*st)
+        impl.set_wd(wd)
+        impl.wetDryStatus2D()
+        impl.PDop2D()
+        outs.append((impl.get_wd(), impl.get_state()))
+    for wd_i, s_i in outs[1:]:
+        assert np.array_equal(wd_i, outs[0][0])
+        assert all(same(a, b) for a, b in zip(s_i, outs[0][1]))
+
+
+def test_wetdry_flux_against_channet():
+    """The -DWDON flux branches (compute2DL.c:201-293) on walls-only meshes with
+    mixed wet/dry flags, against ChanNet's compiled compute2DL."""
+    for seed in range(3):
+        m = perturbed_mesh(N=8, seed=seed, codes=False)
+        st = random_state(m, seed=seed + 40, dz=0.2, q=0.3)
+        wd = np.random.default_rng(seed).integers(0, 2, m.num_el).astype(np.int32)
+        ch = oracle.RefChan(m)
+        o = oracle.Oracle(m, math=oracle.MATH_REF, wetdry=1)
+        for impl in (ch, o):
+            impl.set_state(*st)
+            impl.set_wd(wd)
+        r1, l1 = ch.compute2DL()
+        r2, _, l2 = o.compute2DL()
+        assert all(same(a, b) for a, b in zip(r1, r2)) and l1 == l2
+
+
+def test_topology_random_meshes(tmp_path):
+    from dgrain import mesh as M
+    for N in (4, 9):
+        m = perturbed_mesh(N=N, seed=N)
+        p = str(tmp_path / ("m%d.14" % N))
+        M.write_fort14(p, m)
+        t = oracle.Ref2D(m).topology_from_fort14(p)
+        assert np.array_equal(t["edg_to_vert"], m.edg_to_vert)
+        assert np.array_equal(t["edg_to_els"], m.edg_to_els)
+        assert np.array_equal(t["bdry"], m.bdry)
+        assert t["min_edge_length"] == m.min_edge_length
