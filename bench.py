@@ -41,7 +41,23 @@ def parse():
     ap.add_argument("--N", type=int, default=512, help="grid cells per side (2*N*N triangles)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run 5 device copies of 512 MiB (HBM counter calibration) and exit")
     return ap.parse_args()
+
+
+def copy_calibration():
+    """A plain device-to-device copy of known size: the FETCH_SIZE/WRITE_SIZE
+    counters of this kernel give the scale between counter units and bytes."""
+    import torch
+    n = 64 * 2 ** 20
+    a = torch.ones(n, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "dispatches": 5}))
 
 
 def load_traffic():
@@ -75,6 +91,9 @@ def cpu_baseline(wl, budget_s):
 
 def main():
     a = parse()
+    if a.copy_calibration:
+        copy_calibration()
+        return
     if a.gpus != 1:
         print("bench.py runs on one GPU only (this build has no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)

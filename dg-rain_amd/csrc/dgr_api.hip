@@ -17,6 +17,7 @@ void launch_stage(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, co
                   const double *wbase, const double *bzeta, const double *bqn, const int *wd, double *means,
                   Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat, hipStream_t s);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
+void launch_geometry(const MeshDev &m, double g, double *geo, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
                   unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s);
@@ -44,6 +45,7 @@ struct dgr_handle {
     double *d_bzeta, *d_bqn;
     double *d_w, *d_w1, *d_means, *d_nodemm;
     double *d_dbg;   // 18 planes: rhs (9) then fhat (9)
+    double *d_geo;   // GX_COUNT planes of per-element geometry
     int *d_wd;
     unsigned char *d_modify;   // [2][pitch]
     Ctrl *d_ctrl;
@@ -109,7 +111,7 @@ static dgr_status free_handle(dgr_handle *h)
 {
     if (!h) return DGR_OK;
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_bzeta,
-                    h->d_bqn, h->d_w, h->d_w1, h->d_means, h->d_nodemm, h->d_dbg, h->d_wd, h->d_modify,
+                    h->d_bqn, h->d_w, h->d_w1, h->d_means, h->d_nodemm, h->d_dbg, h->d_geo, h->d_wd, h->d_modify,
                     h->d_ctrl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -230,6 +232,7 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
     DGR_TRY(dalloc(&h->d_wd, P));
     DGR_TRY(dalloc(&h->d_modify, 2 * P));
     DGR_TRY(dalloc(&h->d_ctrl, 1));
+    DGR_TRY(dalloc(&h->d_geo, (size_t)GX_COUNT * P));
     hipStream_t s = h->stream;
     DGR_TRY(hipMemcpyAsync(h->d_nodes, nodes.data(), nn * sizeof(double4), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_ev, ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
@@ -271,6 +274,8 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
     h->m.ne = ne; h->m.nn = nn; h->m.pitch = pitch;
     h->m.nodes = h->d_nodes; h->m.ev = h->d_ev; h->m.nb = h->d_nb; h->m.info = h->d_info; h->m.eid = h->d_eid;
     h->m.n2e_off = h->d_n2e_off; h->m.n2e = h->d_n2e;
+    h->m.geo = h->d_geo;
+    launch_geometry(h->m, prm->g, h->d_geo, s);
     PhysDev &ph = h->ph;
     ph.g = prm->g; ph.h0 = prm->h0; ph.velzero = prm->velzero; ph.rain = prm->rain;
     ph.cfl = prm->cfl; ph.min_edge = mel; ph.final_time = prm->final_time;
@@ -514,6 +519,28 @@ extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
         rep->error_where = err_where_of(c.err_key);
     }
     return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
+}
+
+extern "C" dgr_status dgr_get_report(dgr_handle *h, dgr_report *rep)
+{
+    if (!h || !rep) return DGR_ERR_ARG;
+    Ctrl c;
+    dgr_status st = read_ctrl(h, &c);
+    if (st) return st;
+    rep->t = c.t;
+    rep->dt = c.dt;
+    rep->max_lambda = maxlam_of(c, 1);
+    rep->error_code = err_code_of(c.err_key);
+    rep->error_where = err_where_of(c.err_key);
+    return rep->error_code ? DGR_ERR_NAN : DGR_OK;
+}
+
+extern "C" dgr_status dgr_set_final_time(dgr_handle *h, double final_time)
+{
+    if (!h) return DGR_ERR_ARG;
+    h->prm.final_time = final_time;
+    h->ph.final_time = final_time;
+    return DGR_OK;
 }
 
 extern "C" dgr_status dgr_last_error(dgr_handle *h, int *code, int *where)

@@ -29,8 +29,38 @@ enum : int {
     EI_CODE_SHIFT = 8,      // BdryPrescribed code (clamped to 0..255) in bits 8..15
 };
 
+// Per-element geometry planes (geo[i * pitch + e]), computed once by
+// k_geometry with the reference's own expressions, so every value carries the
+// bits the reference computes on the fly (2DInnerProducts.c:22-308,
+// compute2DL.c:98-144,380-391).
+enum GeoIdx : int {
+    GX_XE = 0,    // Gauss point x on edge e, point n: GX_XE + 2e + n
+    GX_YE = 6,    // Gauss point y
+    GX_PS = 12,   // CR basis b at Gauss point (e, n): GX_PS + 6b + 2e + n
+    GX_NX = 30,   // midpoint-triangle normals nx1..nx3 (2DInnerProducts.c:111-116)
+    GX_NY = 33,
+    GX_M2 = 36,   // M12, M22, M32 of the RT0 fit (:126-128)
+    GX_DEN = 39,  // Cramer determinant (:134)
+    GX_EJ = 40,   // edge jacobians |e_k|/2 (:42-45)
+    GX_GP = 43,   // grad Psi_b, component j: GX_GP + 2b + j (:303-308)
+    GX_GS = 49,   // gradPsi_b . (x1+x2+x3, y1+y2+y3)
+    GX_AJ = 52,   // |jac| of the midpoint triangle
+    GX_AW = 53,   // |jac| * weight
+    GX_MI = 54,   // Minv = 6/|jac_vertex| (compute2DL.c:382)
+    GX_MZ = 55,   // midpoint depths (:268-270)
+    GX_GNN = 58,  // g * n_mid * n_mid (:318)
+    GX_BGX = 61,  // g |jac| dz/dx weight (:535)
+    GX_BGY = 62,  // g |jac| dz/dy weight
+    GX_ENX = 63,  // canonical (el_in) unit normal of local edge k (compute2DL.c:131-132)
+    GX_ENY = 66,
+    GX_EZB = 69,  // z_edge (compute2DL.c:144)
+    GX_EDN = 72,  // 1/(nx*ty - ny*tx) (compute2DL.c:159,187)
+    GX_COUNT = 75
+};
+
 struct MeshDev {
     int ne, nn, pitch;
+    const double *geo;
     const double4 *nodes;
     const int *ev;
     const int *nb;

@@ -138,23 +138,31 @@ __device__ __forceinline__ void rt0(double M11, double M12, double M13, double M
 }
 
 // ---------------------------------------------------------------------------
-// Compute2DInnerProducts + the element loop of compute2DL: the 9 RHS values
-// of one element from its state W[f*3+k], its fluxes Fh[f*3+k] and geometry.
-// 2DInnerProducts.c:14-566, compute2DL.c:344-409.
+// k_geometry: every geometry-only quantity of Compute2DInnerProducts and of
+// the edge loop of compute2DL, per element, with the reference's expressions.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void element_rhs(const double4 P[3], const double W[9], const double Fh[9],
-                                            int wd_self, double g, double rain, double R[9])
+__global__ __launch_bounds__(256) void k_geometry(MeshDev m, double g, double *__restrict__ geo)
 {
-    const double xv1 = P[0].x, xv2 = P[1].x, xv3 = P[2].x;
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    const int Pt = m.pitch;
+    double4 P[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P[k] = m.nodes[m.ev[k * Pt + e]];
+    auto put = [&](int i, double v) { geo[(size_t)i * Pt + e] = v; };
+
+    const double xv1 = P[0].x, xv2 = P[1].x, xv3 = P[2].x;                  // 2DInnerProducts.c:22-27
     const double yv1 = P[0].y, yv2 = P[1].y, yv3 = P[2].y;
-    const double x1 = (xv1 + xv2) / 2, x2 = (xv2 + xv3) / 2, x3 = (xv3 + xv1) / 2;
+    const double x1 = (xv1 + xv2) / 2, x2 = (xv2 + xv3) / 2, x3 = (xv3 + xv1) / 2;   // :30-35
     const double y1 = (yv1 + yv2) / 2, y2 = (yv2 + yv3) / 2, y3 = (yv3 + yv1) / 2;
-    const double el0 = sqrt((xv2 - xv1) * (xv2 - xv1) + (yv2 - yv1) * (yv2 - yv1));
+    const double el0 = sqrt((xv2 - xv1) * (xv2 - xv1) + (yv2 - yv1) * (yv2 - yv1));   // :37-39
     const double el1 = sqrt((xv3 - xv2) * (xv3 - xv2) + (yv3 - yv2) * (yv3 - yv2));
     const double el2 = sqrt((xv3 - xv1) * (xv3 - xv1) + (yv3 - yv1) * (yv3 - yv1));
-    const double ej0 = el0 / 2, ej1 = el1 / 2, ej2 = el2 / 2;
+    put(GX_EJ + 0, el0 / 2);
+    put(GX_EJ + 1, el1 / 2);
+    put(GX_EJ + 2, el2 / 2);
 
-    const double sqrt3 = sqrt(3.0);
+    const double sqrt3 = sqrt(3.0);                                          // :51-65
     double xe[3][2], ye[3][2];
     xe[0][0] = (sqrt3 * x1 - x2 + x3) / sqrt3;  ye[0][0] = (sqrt3 * y1 - y2 + y3) / sqrt3;
     xe[0][1] = (sqrt3 * x1 + x2 - x3) / sqrt3;  ye[0][1] = (sqrt3 * y1 + y2 - y3) / sqrt3;
@@ -162,100 +170,167 @@ __device__ __forceinline__ void element_rhs(const double4 P[3], const double W[9
     xe[1][1] = (-x1 + sqrt3 * x2 + x3) / sqrt3; ye[1][1] = (-y1 + sqrt3 * y2 + y3) / sqrt3;
     xe[2][0] = (-x1 + x2 + sqrt3 * x3) / sqrt3; ye[2][0] = (-y1 + y2 + sqrt3 * y3) / sqrt3;
     xe[2][1] = (x1 - x2 + sqrt3 * x3) / sqrt3;  ye[2][1] = (y1 - y2 + sqrt3 * y3) / sqrt3;
-
-    // Psi_b at Gauss point n of edge e: Ps[b][e][n]  (:73-87)
-    const double dn0 = x1 * y2 - x1 * y3 + x2 * y3 - x2 * y1 + x3 * y1 - x3 * y2;
-    double Ps[3][3][2];
+    const double dn0 = x1 * y2 - x1 * y3 + x2 * y3 - x2 * y1 + x3 * y1 - x3 * y2;   // :73
 #pragma unroll
-    for (int e = 0; e < 3; ++e)
+    for (int ed = 0; ed < 3; ++ed)
 #pragma unroll
         for (int n = 0; n < 2; ++n) {
-            const double xq = xe[e][n], yq = ye[e][n];
-            Ps[0][e][n] = -(yq * x2 - yq * x3 - xq * y2 + x3 * y2 + xq * y3 - x2 * y3) / dn0;
-            Ps[1][e][n] = -(-x1 * yq + x3 * yq - x3 * y1 + xq * y1 - xq * y3 + x1 * y3) / dn0;
-            Ps[2][e][n] = -(yq * x1 - yq * x2 - xq * y1 + x2 * y1 + xq * y2 - x1 * y2) / dn0;
+            const double xq = xe[ed][n], yq = ye[ed][n];
+            put(GX_XE + 2 * ed + n, xq);
+            put(GX_YE + 2 * ed + n, yq);
+            put(GX_PS + 0 + 2 * ed + n, -(yq * x2 - yq * x3 - xq * y2 + x3 * y2 + xq * y3 - x2 * y3) / dn0);   // :74-87
+            put(GX_PS + 6 + 2 * ed + n, -(-x1 * yq + x3 * yq - x3 * y1 + xq * y1 - xq * y3 + x1 * y3) / dn0);
+            put(GX_PS + 12 + 2 * ed + n, -(yq * x1 - yq * x2 - xq * y1 + x2 * y1 + xq * y2 - x1 * y2) / dn0);
         }
-
-    const double jac = 4 * (x2 - x3) * (y2 - y1) - 4 * (x2 - x1) * (y2 - y3);   // :110
+    const double jac = 4 * (x2 - x3) * (y2 - y1) - 4 * (x2 - x1) * (y2 - y3);  // :110
     const int sj = isign(jac);
-    const double nx1 = sj * 2 * (y2 - y3) / el0, ny1 = sj * 2 * (x3 - x2) / el0;
+    const double nx1 = sj * 2 * (y2 - y3) / el0, ny1 = sj * 2 * (x3 - x2) / el0;   // :111-116
     const double nx2 = sj * 2 * (y3 - y1) / el1, ny2 = sj * 2 * (x1 - x3) / el1;
     const double nx3 = sj * 2 * (y1 - y2) / el2, ny3 = sj * 2 * (x2 - x1) / el2;
-    const double M11 = nx1, M12 = x1 * nx1 + y1 * ny1, M13 = ny1;
+    const double M11 = nx1, M12 = x1 * nx1 + y1 * ny1, M13 = ny1;            // :126-128
     const double M21 = nx2, M22 = x2 * nx2 + y2 * ny2, M23 = ny2;
     const double M31 = nx3, M32 = x3 * nx3 + y3 * ny3, M33 = ny3;
-    const double den = M13 * M22 * M31 - M12 * M23 * M31 - M13 * M21 * M32 + M11 * M23 * M32 + M12 * M21 * M33 - M11 * M22 * M33;
+    put(GX_NX + 0, nx1); put(GX_NX + 1, nx2); put(GX_NX + 2, nx3);
+    put(GX_NY + 0, ny1); put(GX_NY + 1, ny2); put(GX_NY + 2, ny3);
+    put(GX_M2 + 0, M12); put(GX_M2 + 1, M22); put(GX_M2 + 2, M32);
+    put(GX_DEN, M13 * M22 * M31 - M12 * M23 * M31 - M13 * M21 * M32 + M11 * M23 * M32 + M12 * M21 * M33 - M11 * M22 * M33);
+
+    const double weight = 1.0 / 6;                                            // :247
+    const double mz[3] = {0.5 * (P[0].z + P[1].z), 0.5 * (P[1].z + P[2].z), 0.5 * (P[2].z + P[0].z)};
+    const double mn[3] = {0.5 * (P[0].w + P[1].w), 0.5 * (P[1].w + P[2].w), 0.5 * (P[2].w + P[0].w)};
+    const double gP[3][2] = {{4 / jac * (y2 - y3), 4 / jac * (x3 - x2)},     // :303-308
+                             {4 / jac * (y3 - y1), 4 / jac * (x1 - x3)},
+                             {4 / jac * (y1 - y2), 4 / jac * (x2 - x1)}};
+    const double sx = x1 + x2 + x3, sy = y1 + y2 + y3;
+    const double aj = fabs(jac);
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        put(GX_GP + 2 * b, gP[b][0]);
+        put(GX_GP + 2 * b + 1, gP[b][1]);
+        put(GX_GS + b, gP[b][0] * sx + gP[b][1] * sy);
+        put(GX_MZ + b, mz[b]);
+        put(GX_GNN + b, g * mn[b] * mn[b]);
+    }
+    put(GX_AJ, aj);
+    put(GX_AW, aj * weight);
+    const double gz0 = mz[0] * gP[0][0] + mz[1] * gP[1][0] + mz[2] * gP[2][0];   // :343-347
+    const double gz1 = mz[0] * gP[0][1] + mz[1] * gP[1][1] + mz[2] * gP[2][1];
+    put(GX_BGX, g * aj * gz0 * weight);                                       // :535
+    put(GX_BGY, g * aj * gz1 * weight);
+    const double jv = P[0].x * (P[1].y - P[2].y) + P[1].x * (P[2].y - P[0].y) + P[2].x * (P[0].y - P[1].y);
+    put(GX_MI, 6 / fabs(jv));                                                 // compute2DL.c:380-382
+
+    // edge loop geometry in the canonical orientation (compute2DL.c:98-144)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int inf = m.info[k * Pt + e];
+        const double4 A = P[k], B = P[(k + 1) % 3];
+        const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+        const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
+        double x0, y0, xx1, yy1;
+        if (inf & EI_DIR_SAME) { x0 = A.x; y0 = A.y; xx1 = B.x; yy1 = B.y; }
+        else { x0 = B.x; y0 = B.y; xx1 = A.x; yy1 = A.y; }
+        const double nx = sg * (yy1 - y0) / len;
+        const double ny = sg * (x0 - xx1) / len;
+        const double tx = -ny, ty = nx;
+        put(GX_ENX + k, nx);
+        put(GX_ENY + k, ny);
+        put(GX_EZB + k, (A.z + B.z) / 2);
+        put(GX_EDN + k, 1. / (nx * ty - ny * tx));
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Compute2DInnerProducts + the element loop of compute2DL: the 9 RHS values
+// of one element from its state W[f*3+k], its fluxes Fh[f*3+k] and its
+// geometry planes.  2DInnerProducts.c:14-566, compute2DL.c:344-409.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void element_rhs(const double *__restrict__ geo, int Pt, int e, const double W[9],
+                                            const double Fh[9], int wd_self, double g, double rain, double R[9])
+{
+    auto G = [&](int i) { return geo[(size_t)i * Pt + e]; };
+    const double nx1 = G(GX_NX), nx2 = G(GX_NX + 1), nx3 = G(GX_NX + 2);
+    const double ny1 = G(GX_NY), ny2 = G(GX_NY + 1), ny3 = G(GX_NY + 2);
+    const double M11 = nx1, M12 = G(GX_M2), M13 = ny1;
+    const double M21 = nx2, M22 = G(GX_M2 + 1), M23 = ny2;
+    const double M31 = nx3, M32 = G(GX_M2 + 2), M33 = ny3;
+    const double den = G(GX_DEN);
     const double nxs[3] = {nx1, nx2, nx3}, nys[3] = {ny1, ny2, ny3};
-    const double ejs[3] = {ej0, ej1, ej2};
 
     // surface integrals SI[f*3+b] (:134-239)
+    double a[3], b[3], c[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+        rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, Fh[f * 3 + 0], Fh[f * 3 + 1], Fh[f * 3 + 2],
+            a[f], b[f], c[f]);
     double SI[9];
 #pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        double a, b, c;
-        rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, Fh[f * 3 + 0], Fh[f * 3 + 1], Fh[f * 3 + 2], a, b, c);
+    for (int f = 0; f < 9; ++f) SI[f] = 0.0;
+#pragma unroll
+    for (int ed = 0; ed < 3; ++ed) {
+        const double ej = G(GX_EJ + ed);
         double Fn[3][2];
 #pragma unroll
-        for (int n = 0; n < 2; ++n)
+        for (int n = 0; n < 2; ++n) {
+            const double xq = G(GX_XE + 2 * ed + n), yq = G(GX_YE + 2 * ed + n);
 #pragma unroll
-            for (int e = 0; e < 3; ++e) Fn[e][n] = (a + b * xe[e][n]) * nxs[e] + (c + b * ye[e][n]) * nys[e];
+            for (int f = 0; f < 3; ++f) Fn[f][n] = (a[f] + b[f] * xq) * nxs[ed] + (c[f] + b[f] * yq) * nys[ed];
+        }
 #pragma unroll
-        for (int bb = 0; bb < 3; ++bb)
-            SI[f * 3 + bb] = ejs[0] * (Fn[0][0] * Ps[bb][0][0] + Fn[0][1] * Ps[bb][0][1]) +
-                             ejs[1] * (Fn[1][0] * Ps[bb][1][0] + Fn[1][1] * Ps[bb][1][1]) +
-                             ejs[2] * (Fn[2][0] * Ps[bb][2][0] + Fn[2][1] * Ps[bb][2][1]);
+        for (int bb = 0; bb < 3; ++bb) {
+            const double p0 = G(GX_PS + 6 * bb + 2 * ed), p1 = G(GX_PS + 6 * bb + 2 * ed + 1);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const double term = ej * (Fn[f][0] * p0 + Fn[f][1] * p1);
+                // ((term_e0 + term_e1) + term_e2), the reference's left-to-right sum
+                SI[f * 3 + bb] = (ed == 0) ? term : SI[f * 3 + bb] + term;
+            }
+        }
     }
 
     // volume terms (:247-566)
     const double weight = 1.0 / 6;
-    const double mz[3] = {0.5 * (P[0].z + P[1].z), 0.5 * (P[1].z + P[2].z), 0.5 * (P[2].z + P[0].z)};
-    const double mn[3] = {0.5 * (P[0].w + P[1].w), 0.5 * (P[1].w + P[2].w), 0.5 * (P[2].w + P[0].w)};
-    const double gP[3][2] = {{4 / jac * (y2 - y3), 4 / jac * (x3 - x2)},
-                             {4 / jac * (y3 - y1), 4 / jac * (x1 - x3)},
-                             {4 / jac * (y1 - y2), 4 / jac * (x2 - x1)}};
     const double lg = (wd_self == 0) ? 0.0 : g;
     double H[3], tau[3], Fx[3][3], Fy[3][3];
 #pragma unroll
-    for (int e = 0; e < 3; ++e) {
-        const double zr = W[e], qx = W[3 + e], qy = W[6 + e];
-        H[e] = zr + mz[e];
-        const double u = qx / H[e];
-        const double v = qy / H[e];
-        tau[e] = g * mn[e] * mn[e] * sqrt(u * u + v * v) / (dgr_pow43(H[e]));
-        Fx[0][e] = qx;
-        Fx[1][e] = (qx * qx) / H[e] + 0.5 * lg * ((H[e] * H[e]) - (mz[e] * mz[e]));
-        Fx[2][e] = qx * qy / H[e];
-        Fy[0][e] = qy;
-        Fy[1][e] = qx * qy / H[e];
-        Fy[2][e] = (qy * qy) / H[e] + 0.5 * lg * ((H[e] * H[e]) - (mz[e] * mz[e]));
+    for (int k = 0; k < 3; ++k) {
+        const double mz = G(GX_MZ + k);
+        const double zr = W[k], qx = W[3 + k], qy = W[6 + k];
+        H[k] = zr + mz;
+        const double u = qx / H[k];
+        const double v = qy / H[k];
+        tau[k] = G(GX_GNN + k) * sqrt(u * u + v * v) / (dgr_pow43(H[k]));
+        Fx[0][k] = qx;
+        Fx[1][k] = (qx * qx) / H[k] + 0.5 * lg * ((H[k] * H[k]) - (mz * mz));
+        Fx[2][k] = qx * qy / H[k];
+        Fy[0][k] = qy;
+        Fy[1][k] = qx * qy / H[k];
+        Fy[2][k] = (qy * qy) / H[k] + 0.5 * lg * ((H[k] * H[k]) - (mz * mz));
     }
-    const double gz0 = mz[0] * gP[0][0] + mz[1] * gP[1][0] + mz[2] * gP[2][0];
-    const double gz1 = mz[0] * gP[0][1] + mz[1] * gP[1][1] + mz[2] * gP[2][1];
-    const double aj = fabs(jac);
-    const double sx = x1 + x2 + x3, sy = y1 + y2 + y3;
+    const double aj = G(GX_AJ);
     double VF[3][3];   // [field][basis]
 #pragma unroll
     for (int f = 0; f < 3; ++f) {
-        double a, b, c;
+        double va, vb, vc;
         rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den,
             Fx[f][0] * nx1 + Fy[f][0] * ny1, Fx[f][1] * nx2 + Fy[f][1] * ny2, Fx[f][2] * nx3 + Fy[f][2] * ny3,
-            a, b, c);
+            va, vb, vc);
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
-            VF[f][bb] = aj * (0.5 * a * gP[bb][0] + 0.5 * c * gP[bb][1] + b * weight * (gP[bb][0] * sx + gP[bb][1] * sy));
+            VF[f][bb] = aj * (0.5 * va * G(GX_GP + 2 * bb) + 0.5 * vc * G(GX_GP + 2 * bb + 1) + vb * weight * G(GX_GS + bb));
     }
-    // rhs combine (compute2DL.c:344-354) and Minv = 6/|jac_vertex| (:380-404)
-    const double jv = P[0].x * (P[1].y - P[2].y) + P[1].x * (P[2].y - P[0].y) + P[2].x * (P[0].y - P[1].y);
-    const double Mi = 6 / fabs(jv);
+    // rhs combine (compute2DL.c:344-354) and Minv (:380-404)
+    const double Mi = G(GX_MI), aw = G(GX_AW), bgx = G(GX_BGX), bgy = G(GX_BGY);
     double ze[3], qe[3], re[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         ze[k] = -SI[k] + VF[0][k];
-        const double bx = lg * aj * gz0 * weight * W[k];
-        const double fx = aj * weight * tau[k] * W[3 + k];
+        // with lg = 0 the bed term is (0*|jac|*dz*weight)*zeta = +-0 (a NaN zeta stays NaN)
+        const double bx = (wd_self == 0) ? 0.0 * W[k] : bgx * W[k];
+        const double fx = aw * tau[k] * W[3 + k];
         qe[k] = -SI[3 + k] + VF[1][k] + bx - fx;
-        const double by = lg * aj * gz1 * weight * W[k];
-        const double fy = aj * weight * tau[k] * W[6 + k];
+        const double by = (wd_self == 0) ? 0.0 * W[k] : bgy * W[k];
+        const double fy = aw * tau[k] * W[6 + k];
         re[k] = -SI[6 + k] + VF[2][k] + by - fy;
     }
     R[0] = (Mi * ze[0] + 0.0 * ze[1]) + 0.0 * ze[2];
@@ -274,14 +349,14 @@ __device__ __forceinline__ void element_rhs(const double4 P[3], const double W[9
     }
 }
 
-// wall reflection of (qx, qy) about the edge normal (compute2DL.c:179-189)
-__device__ __forceinline__ void reflect(double qx, double qy, double nx, double ny, double &rx, double &ry)
+// wall reflection of (qx, qy) about the edge normal (compute2DL.c:179-189);
+// dn = 1/(nx*ty - ny*tx) with (tx, ty) = (-ny, nx)
+__device__ __forceinline__ void reflect(double qx, double qy, double nx, double ny, double dn, double &rx, double &ry)
 {
     const double tx = -ny, ty = nx;
     const double qn = qx * nx + qy * ny;
     const double qt = qx * tx + qy * ty;
     const double qne = -qn, qte = qt;
-    const double dn = 1. / (nx * ty - ny * tx);
     rx = (ty * qne - ny * qte) * dn;
     ry = (-tx * qne + nx * qte) * dn;
 }
@@ -300,10 +375,11 @@ __device__ __forceinline__ T wave_max(T v)
 // One max-|lambda| operand of the reference's fold, at position 2*edge (the
 // flux of a not-both-dry edge, or el_in's reflection flux of a both-dry edge)
 // or 2*edge+1 (el_ex's reflection flux of a both-dry edge).
+// Two fixed slots per local edge (2k, 2k+1) so every index is a compile-time
+// constant after unrolling; pos < 0 marks an unused slot.
 struct LamOps {
     double v[6];
     long long pos[6];
-    int n;
 };
 
 // The three Roe fluxes of element e in the reference's canonical orientation
@@ -313,12 +389,13 @@ template <bool WETDRY>
 __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &ph, const double *__restrict__ win,
                                                const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                const int *__restrict__ wd, const Ctrl *ctrl, int e,
-                                               const double4 X[3], const double W[9], int wd_self, double Fh[9],
+                                               const double W[9], int wd_self, double Fh[9],
                                                LamOps &ops, int &err_code, int &err_edge)
 {
     const int P = m.pitch;
     const double g = ph.g;
-    ops.n = 0;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) { ops.v[i] = 0.0; ops.pos[i] = -1; }
     err_code = 0;
     err_edge = 0x7fffffff;
 #pragma unroll
@@ -329,16 +406,11 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
         const int knb = inf & EI_KNB_MASK;
         const bool is_in = (inf & EI_IS_IN) != 0;
         const bool boundary = (inf & EI_BOUNDARY) != 0;
-        const double4 A = X[k], B = X[(k + 1) % 3];
-        const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
-        const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
-        double x0, y0, x1, y1;
-        if (inf & EI_DIR_SAME) { x0 = A.x; y0 = A.y; x1 = B.x; y1 = B.y; }
-        else { x0 = B.x; y0 = B.y; x1 = A.x; y1 = A.y; }
-        const double nx = sg * (y1 - y0) / len;
-        const double ny = sg * (x0 - x1) / len;
+        const double nx = m.geo[(size_t)(GX_ENX + k) * P + e];
+        const double ny = m.geo[(size_t)(GX_ENY + k) * P + e];
+        const double zb = m.geo[(size_t)(GX_EZB + k) * P + e];
+        const double dn = m.geo[(size_t)(GX_EDN + k) * P + e];
         const double tx = -ny, ty = nx;
-        const double zb = (A.z + B.z) / 2;
         const double oz = W[k], oqx = W[3 + k], oqy = W[6 + k];
         double zin, qxin, qyin, zex, qxex, qyex;
         if (boundary) {
@@ -357,12 +429,11 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
             if (zex < 0) zex = zin;
             if (qnex > 1000) qnex = qxin * nx + qyin * ny;
             const double qtex = qxin * tx + qyin * ty;
-            const double dn = 1. / (nx * ty - ny * tx);
             qxex = (ty * qnex - ny * qtex) * dn;
             qyex = (-tx * qnex + nx * qtex) * dn;
         } else if (boundary) {                                       // :175-191
             zex = oz;
-            reflect(qxin, qyin, nx, ny, qxex, qyex);
+            reflect(qxin, qyin, nx, ny, dn, qxex, qyex);
         }
         double F[3];
         const long long p0 = 2LL * ed;
@@ -372,34 +443,34 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
             if (wd_in == 0 && wd_ex == 0) {                          // :203-240
                 if (is_in) {
                     double rx, ry;
-                    reflect(qxin, qyin, nx, ny, rx, ry);
-                    ops.v[ops.n] = roe_flux(F, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
-                    ops.pos[ops.n++] = p0;
+                    reflect(qxin, qyin, nx, ny, dn, rx, ry);
+                    ops.v[2 * k] = roe_flux(F, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
+                    ops.pos[2 * k] = p0;
                     if (boundary) {
                         double Fd[3], sx_, sy_;
-                        reflect(qxex, qyex, nx, ny, sx_, sy_);
-                        ops.v[ops.n] = roe_flux(Fd, zex, zex, sx_, qxex, sy_, qyex, zb, nx, ny, 0.0, g);
-                        ops.pos[ops.n++] = p0 + 1;
+                        reflect(qxex, qyex, nx, ny, dn, sx_, sy_);
+                        ops.v[2 * k + 1] = roe_flux(Fd, zex, zex, sx_, qxex, sy_, qyex, zb, nx, ny, 0.0, g);
+                        ops.pos[2 * k + 1] = p0 + 1;
                     }
                 } else {
                     double rx, ry;
-                    reflect(qxex, qyex, nx, ny, rx, ry);
-                    ops.v[ops.n] = roe_flux(F, zex, zex, rx, qxex, ry, qyex, zb, nx, ny, 0.0, g);
-                    ops.pos[ops.n++] = p0 + 1;
+                    reflect(qxex, qyex, nx, ny, dn, rx, ry);
+                    ops.v[2 * k] = roe_flux(F, zex, zex, rx, qxex, ry, qyex, zb, nx, ny, 0.0, g);
+                    ops.pos[2 * k] = p0 + 1;
                     F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2];
                 }
             } else {                                                 // :241-293
                 const int my = is_in ? wd_in : wd_ex;
-                ops.v[ops.n] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, (my == 1) ? g : 0.0, g);
-                ops.pos[ops.n++] = p0;
+                ops.v[2 * k] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, (my == 1) ? g : 0.0, g);
+                ops.pos[2 * k] = p0;
                 // a NaN here comes from the state, so the reference's first check
                 // (the g-flux, compute2DL.c:245) would already have stopped
                 if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed < err_edge) { err_code = 3; err_edge = ed; }
                 if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
             }
         } else {                                                     // :295-315
-            ops.v[ops.n] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
-            ops.pos[ops.n++] = p0;
+            ops.v[2 * k] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
+            ops.pos[2 * k] = p0;
             if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed < err_edge) { err_code = 3; err_edge = ed; }
             if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
         }
@@ -437,12 +508,6 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
     bool any = false;
     if (valid) {
         const int P = m.pitch;
-        int v[3];
-        double4 X[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) v[k] = m.ev[k * P + e];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) X[k] = m.nodes[v[k]];
         double W[9];
 #pragma unroll
         for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
@@ -451,14 +516,17 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
         double Fh[9];
         LamOps ops;
         int ecode, eedge;
-        element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, X, W, wd_self, Fh, ops, ecode, eedge);
+        element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, W, wd_self, Fh, ops, ecode, eedge);
         if (MODE == 3) {
-            for (int i = 0; i < ops.n; ++i)
-                if (ops.pos[i] > nan_after) { lam = (ops.v[i] > lam) ? ops.v[i] : lam; any = true; }
+#pragma unroll
+            for (int i = 0; i < 6; ++i)
+                if (ops.pos[i] >= 0 && ops.pos[i] > nan_after) { lam = (ops.v[i] > lam) ? ops.v[i] : lam; any = true; }
         } else {
             long long nanp = -1;
-            for (int i = 0; i < ops.n; ++i) {
+#pragma unroll
+            for (int i = 0; i < 6; ++i) {
                 const double x = ops.v[i];
+                if (ops.pos[i] < 0) continue;
                 if (x != x) nanp = (ops.pos[i] > nanp) ? ops.pos[i] : nanp;
                 else lam = (x > lam) ? x : lam;
             }
@@ -466,7 +534,7 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
             if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
 
             double R[9];
-            element_rhs(X, W, Fh, wd_self, ph.g, ph.rain, R);
+            element_rhs(m.geo, P, e, W, Fh, wd_self, ph.g, ph.rain, R);
 
             if (MODE == 0) {
 #pragma unroll
@@ -815,6 +883,11 @@ void launch_stage(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, co
         else DGR_LS(3, false);
     }
 #undef DGR_LS
+}
+
+void launch_geometry(const MeshDev &m, double g, double *geo, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_geometry, grid_for(m.ne), dim3(256), 0, s, m, g, geo);
 }
 
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s)

@@ -83,6 +83,8 @@ def load():
         "dgr_device_state": (C.c_int, [vp, C.POINTER(dp), ip]),
         "dgr_math_selftest": (C.c_int, [C.c_int, dp, dp, dp]),
         "dgr_profile_begin": (C.c_int, [vp]),
+        "dgr_get_report": (C.c_int, [vp, C.POINTER(Report)]),
+        "dgr_set_final_time": (C.c_int, [vp, C.c_double]),
         "dgr_profile_end": (C.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():

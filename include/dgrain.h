@@ -139,6 +139,10 @@ dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt);
  * device (dt = cfl*minEdgLength/max_lambda, clamped to final_time), starting
  * from the time/dt held by the handle (dgr_set_time; initially 0 and dt0). */
 dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *report);
+/* The report of the last step (or of the last oneTimeStep2D) without stepping. */
+dgr_status dgr_get_report(dgr_handle *h, dgr_report *report);
+/* FinalTime of the dt clamp (time_evolution.c:400-401). */
+dgr_status dgr_set_final_time(dgr_handle *h, double final_time);
 dgr_status dgr_last_error(dgr_handle *h, int *code, int *where);
 
 /* Kernel timing for measurement: between dgr_profile_begin and
