@@ -13,9 +13,10 @@
 #include "dgr_device.h"
 
 namespace dgr {
-void launch_stage(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
-                  const double *wbase, const double *bzeta, const double *bqn, const int *wd, double *means,
-                  Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat, hipStream_t s);
+void launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
+                  double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
+                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
+                  hipStream_t s);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
 void launch_geometry(const MeshDev &m, double g, double *geo, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
@@ -46,6 +47,8 @@ struct dgr_handle {
     double *d_w, *d_w1, *d_means, *d_nodemm;
     double *d_dbg;   // 18 planes: rhs (9) then fhat (9)
     double *d_geo;   // GX_COUNT planes of per-element geometry
+    double *d_fhat;  // 9 planes of edge fluxes (split kernel variant)
+    int variant;     // bit 0: geometry planes, bit 1: split flux/RHS kernels
     int *d_wd;
     unsigned char *d_modify;   // [2][pitch]
     Ctrl *d_ctrl;
@@ -111,7 +114,7 @@ static dgr_status free_handle(dgr_handle *h)
 {
     if (!h) return DGR_OK;
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_bzeta,
-                    h->d_bqn, h->d_w, h->d_w1, h->d_means, h->d_nodemm, h->d_dbg, h->d_geo, h->d_wd, h->d_modify,
+                    h->d_bqn, h->d_w, h->d_w1, h->d_means, h->d_nodemm, h->d_dbg, h->d_geo, h->d_fhat, h->d_wd, h->d_modify,
                     h->d_ctrl};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
@@ -233,6 +236,7 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
     DGR_TRY(dalloc(&h->d_modify, 2 * P));
     DGR_TRY(dalloc(&h->d_ctrl, 1));
     DGR_TRY(dalloc(&h->d_geo, (size_t)GX_COUNT * P));
+    DGR_TRY(dalloc(&h->d_fhat, 9 * P));
     hipStream_t s = h->stream;
     DGR_TRY(hipMemcpyAsync(h->d_nodes, nodes.data(), nn * sizeof(double4), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_ev, ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
@@ -398,10 +402,10 @@ extern "C" dgr_status dgr_compute_rhs(dgr_handle *h, double *rz, double *rqx, do
     if (!h) return DGR_ERR_ARG;
     if (!h->d_dbg) DGR_HIP(dalloc(&h->d_dbg, 18 * (size_t)h->pitch));
     launch_reset_maxlam(h->d_ctrl, h->stream);
-    launch_stage(0, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
-                 nullptr, h->d_ctrl, 0, h->d_dbg, h->d_dbg + 9 * (size_t)h->pitch, h->stream);
-    launch_stage(3, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
-                 nullptr, h->d_ctrl, 0, nullptr, nullptr, h->stream);
+    launch_stage(h->variant, 0, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn,
+                 h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, h->d_dbg, h->d_dbg + 9 * (size_t)h->pitch, h->stream);
+    launch_stage(h->variant, 3, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn,
+                 h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, h->stream);
     DGR_HIP(hipGetLastError());
     if (rz || rqx || rqy) {
         dgr_status st = planes_down(h, h->d_dbg, rz, rqx, rqy);
@@ -469,19 +473,19 @@ static void enqueue_stages(dgr_handle *h)
     hipStream_t s = h->stream;
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, s);
     mark(h);
-    launch_stage(1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn, h->d_wd, h->d_means,
-                 h->d_ctrl, 0, nullptr, nullptr, s);
+    launch_stage(h->variant, 1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
+                 h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
     mark(h);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
     launch_limit(wdon, h->m, h->ph, h->d_w1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, s);
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w1, h->d_wd, s);
     mark(h);
-    launch_stage(2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn, h->d_wd, h->d_means,
-                 h->d_ctrl, 1, nullptr, nullptr, s);
+    launch_stage(h->variant, 2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn, h->d_wd,
+                 h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     mark(h);
     // stage 2's max |lambda| sets the next dt; repair its fold if a NaN operand was seen
-    launch_stage(3, wdon, h->m, h->ph, h->d_w1, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd, nullptr,
-                 h->d_ctrl, 1, nullptr, nullptr, s);
+    launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w1, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
+                 nullptr, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
     launch_limit(wdon, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1, s);
 }
@@ -613,5 +617,72 @@ extern "C" dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *laun
     h->ev_used = 0;
     *total_ms = tot;
     *launches = n;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_set_variant(dgr_handle *h, int variant)
+{
+    if (!h || variant < 0 || variant > 3) return DGR_ERR_ARG;
+    h->variant = variant;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_get_variant(dgr_handle *h, int *variant)
+{
+    if (!h || !variant) return DGR_ERR_ARG;
+    *variant = h->variant;
+    return DGR_OK;
+}
+
+// Times `steps` steps of every kernel variant from the current state and keeps
+// the fastest.  All variants compute the same bits; the state, WD flags and
+// step bookkeeping are restored afterwards.
+extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step)
+{
+    if (!h || steps <= 0) return DGR_ERR_ARG;
+    const size_t P = h->pitch;
+    double *w_save = nullptr;
+    int *wd_save = nullptr;
+    Ctrl c_save;
+    DGR_HIP(dalloc(&w_save, 9 * P));
+    DGR_HIP(dalloc(&wd_save, P));
+    DGR_HIP(hipMemcpyAsync(w_save, h->d_w, 9 * P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+    DGR_HIP(hipMemcpyAsync(wd_save, h->d_wd, P * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+    DGR_HIP(hipMemcpyAsync(&c_save, h->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    hipEvent_t e0, e1;
+    DGR_HIP(hipEventCreate(&e0));
+    DGR_HIP(hipEventCreate(&e1));
+    int best = h->variant;
+    float best_ms = 1e30f;
+    for (int v = 0; v < 4; ++v) {
+        h->variant = v;
+        // one untimed step to warm the code object, then the timed steps
+        launch_step_begin(h->ph, h->d_ctrl, h->stream);
+        enqueue_stages(h);
+        launch_step_end(h->ph, h->d_ctrl, h->stream);
+        DGR_HIP(hipEventRecord(e0, h->stream));
+        for (int n = 0; n < steps; ++n) {
+            launch_step_begin(h->ph, h->d_ctrl, h->stream);
+            enqueue_stages(h);
+            launch_step_end(h->ph, h->d_ctrl, h->stream);
+        }
+        DGR_HIP(hipEventRecord(e1, h->stream));
+        DGR_HIP(hipEventSynchronize(e1));
+        float ms = 0.f;
+        DGR_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (ms_per_step) ms_per_step[v] = ms / steps;
+        if (ms < best_ms) { best_ms = ms; best = v; }
+        DGR_HIP(hipMemcpyAsync(h->d_w, w_save, 9 * P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
+        DGR_HIP(hipMemcpyAsync(h->d_wd, wd_save, P * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
+        DGR_HIP(hipMemcpyAsync(h->d_ctrl, &c_save, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+        DGR_HIP(hipStreamSynchronize(h->stream));
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(w_save);
+    (void)hipFree(wd_save);
+    h->variant = best;
+    if (chosen) *chosen = best;
     return DGR_OK;
 }

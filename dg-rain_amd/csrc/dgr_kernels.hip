@@ -24,6 +24,18 @@
 #include "dgr_device.h"
 #include "dgr_math.h"
 
+// Minimum waves per SIMD asked of the compiler for the stage kernels
+// (register budget 512/waves VGPRs); tuned by measurement.
+#ifndef DGR_STAGE_WAVES
+#define DGR_STAGE_WAVES 1
+#endif
+#ifndef DGR_FLUX_WAVES
+#define DGR_FLUX_WAVES 1
+#endif
+#ifndef DGR_RHS_WAVES
+#define DGR_RHS_WAVES 1
+#endif
+
 namespace dgr {
 
 #define DGR_MAX(a, b) (((a) > (b)) ? (a) : (b))
@@ -138,19 +150,19 @@ __device__ __forceinline__ void rt0(double M11, double M12, double M13, double M
 }
 
 // ---------------------------------------------------------------------------
-// k_geometry: every geometry-only quantity of Compute2DInnerProducts and of
-// the edge loop of compute2DL, per element, with the reference's expressions.
+// geometry_of: every geometry-only quantity of Compute2DInnerProducts and of
+// the edge loop of compute2DL for element e, with the reference's
+// expressions, handed to put(GeoIdx, value).  k_geometry stores them as
+// planes once; the in-register variants call it inside the stage kernels and
+// let dead-code elimination keep only what they use.
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_geometry(MeshDev m, double g, double *__restrict__ geo)
+template <typename Put>
+__device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, Put &&put)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m.ne) return;
     const int Pt = m.pitch;
     double4 P[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) P[k] = m.nodes[m.ev[k * Pt + e]];
-    auto put = [&](int i, double v) { geo[(size_t)i * Pt + e] = v; };
-
     const double xv1 = P[0].x, xv2 = P[1].x, xv3 = P[2].x;                  // 2DInnerProducts.c:22-27
     const double yv1 = P[0].y, yv2 = P[1].y, yv3 = P[2].y;
     const double x1 = (xv1 + xv2) / 2, x2 = (xv2 + xv3) / 2, x3 = (xv3 + xv1) / 2;   // :30-35
@@ -240,15 +252,34 @@ __global__ __launch_bounds__(256) void k_geometry(MeshDev m, double g, double *_
     }
 }
 
+__global__ __launch_bounds__(256) void k_geometry(MeshDev m, double g, double *__restrict__ geo)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    const int Pt = m.pitch;
+    geometry_of(m, e, g, [&](int i, double v) { geo[(size_t)i * Pt + e] = v; });
+}
+
+// geometry accessors: computed into registers, or read from the planes
+struct GeoRegs {
+    double v[GX_COUNT];
+    __device__ __forceinline__ double operator()(int i) const { return v[i]; }
+};
+struct GeoPlanes {
+    const double *geo;
+    int pitch, e;
+    __device__ __forceinline__ double operator()(int i) const { return geo[(size_t)i * pitch + e]; }
+};
+
 // ---------------------------------------------------------------------------
 // Compute2DInnerProducts + the element loop of compute2DL: the 9 RHS values
 // of one element from its state W[f*3+k], its fluxes Fh[f*3+k] and its
 // geometry planes.  2DInnerProducts.c:14-566, compute2DL.c:344-409.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void element_rhs(const double *__restrict__ geo, int Pt, int e, const double W[9],
-                                            const double Fh[9], int wd_self, double g, double rain, double R[9])
+template <typename Geo>
+__device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], const double Fh[9], int wd_self,
+                                            double g, double rain, double R[9])
 {
-    auto G = [&](int i) { return geo[(size_t)i * Pt + e]; };
     const double nx1 = G(GX_NX), nx2 = G(GX_NX + 1), nx3 = G(GX_NX + 2);
     const double ny1 = G(GX_NY), ny2 = G(GX_NY + 1), ny3 = G(GX_NY + 2);
     const double M11 = nx1, M12 = G(GX_M2), M13 = ny1;
@@ -385,11 +416,11 @@ struct LamOps {
 // The three Roe fluxes of element e in the reference's canonical orientation
 // (compute2DL.c:17-331), negated on the el_ex side, and the max-|lambda|
 // operands this element owns.
-template <bool WETDRY>
+template <bool WETDRY, typename Geo>
 __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &ph, const double *__restrict__ win,
                                                const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                const int *__restrict__ wd, const Ctrl *ctrl, int e,
-                                               const double W[9], int wd_self, double Fh[9],
+                                               const Geo &G, const double W[9], int wd_self, double Fh[9],
                                                LamOps &ops, int &err_code, int &err_edge)
 {
     const int P = m.pitch;
@@ -406,10 +437,10 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
         const int knb = inf & EI_KNB_MASK;
         const bool is_in = (inf & EI_IS_IN) != 0;
         const bool boundary = (inf & EI_BOUNDARY) != 0;
-        const double nx = m.geo[(size_t)(GX_ENX + k) * P + e];
-        const double ny = m.geo[(size_t)(GX_ENY + k) * P + e];
-        const double zb = m.geo[(size_t)(GX_EZB + k) * P + e];
-        const double dn = m.geo[(size_t)(GX_EDN + k) * P + e];
+        const double nx = G(GX_ENX + k);
+        const double ny = G(GX_ENY + k);
+        const double zb = G(GX_EZB + k);
+        const double dn = G(GX_EDN + k);
         const double tx = -ny, ty = nx;
         const double oz = W[k], oqx = W[3 + k], oqy = W[6 + k];
         double zin, qxin, qyin, zex, qxex, qyex;
@@ -480,17 +511,64 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
     }
 }
 
+// max-|lambda| bookkeeping of one element's operands; returns the non-NaN max
+__device__ __forceinline__ double lam_fold(const LamOps &ops, Ctrl *ctrl, int slot)
+{
+    double lam = 0.0;
+    long long nanp = -1;
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+        const double x = ops.v[i];
+        if (ops.pos[i] < 0) continue;
+        if (x != x) nanp = (ops.pos[i] > nanp) ? ops.pos[i] : nanp;
+        else lam = (x > lam) ? x : lam;
+    }
+    if (nanp >= 0) atomicMax(&ctrl->nan_pos[slot], nanp);
+    return lam;
+}
+
+// wave max of the element maxima, one 64-bit atomic per wavefront
+__device__ __forceinline__ void lam_commit(double lam, Ctrl *ctrl, int slot)
+{
+    const unsigned long long wm = wave_max((unsigned long long)__double_as_longlong(lam));
+    if ((threadIdx.x & 63) == 0 && wm != 0ULL) atomicMax(&ctrl->maxlam[slot], wm);
+}
+
+// RK update (time_evolution.c:72-88, 176-178, 208-217) and element means
+template <int MODE>
+__device__ __forceinline__ void rk_update(const MeshDev &m, int e, const double W[9], const double R[9],
+                                         double *wout, const double *wbase, double *__restrict__ means,
+                                         const Ctrl *ctrl)
+{
+    const int P = m.pitch;
+    const double dt = ctrl->dt;
+    double U[9];
+    if (MODE == 1) {
+#pragma unroll
+        for (int f = 0; f < 9; ++f) U[f] = W[f] + R[f] * dt;
+    } else {
+        const double half = dt / 2;
+#pragma unroll
+        for (int f = 0; f < 9; ++f) U[f] = (W[f] + wbase[f * P + e]) * 0.5 + R[f] * half;
+    }
+#pragma unroll
+    for (int f = 0; f < 9; ++f) wout[f * P + e] = U[f];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) means[f * P + e] = (U[3 * f] + U[3 * f + 1] + U[3 * f + 2]) / 3;
+}
+
 // ---------------------------------------------------------------------------
-// k_stage: flux gather + RHS + RK update of one element per thread.
+// k_stage: flux gather + RHS + RK update of one element per thread (fused).
 //   MODE 0: RHS only (compute2DL), written to dbg_rhs/dbg_fhat
 //   MODE 1: w_out = w_in + RHS*dt                       (time_evolution.c:72-88)
 //   MODE 2: w_out = (w_in + w_base)*0.5 + RHS*(dt/2)    (time_evolution.c:176-178,208-217)
 //           w_out may alias w_base (each element reads its own slot first).
 //   MODE 3: max-|lambda| fold repair after a NaN operand (see Ctrl); returns
 //           at once when the stage saw no NaN operand.
+// GEOLOAD: geometry from the planes of k_geometry, or computed in registers.
 // ---------------------------------------------------------------------------
-template <int MODE, bool WETDRY>
-__global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const double *__restrict__ win,
+template <int MODE, bool WETDRY, bool GEOLOAD>
+__global__ __launch_bounds__(256, DGR_STAGE_WAVES) void k_stage(MeshDev m, PhysDev ph, const double *__restrict__ win,
                                                double *wout, const double *wbase,
                                                const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                const int *__restrict__ wd, double *__restrict__ means,
@@ -512,62 +590,118 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
 #pragma unroll
         for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
         const int wd_self = wd[e];
-
-        double Fh[9];
-        LamOps ops;
-        int ecode, eedge;
-        element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, W, wd_self, Fh, ops, ecode, eedge);
-        if (MODE == 3) {
+        auto body = [&](const auto &G) {
+            double Fh[9];
+            LamOps ops;
+            int ecode, eedge;
+            element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, G, W, wd_self, Fh, ops, ecode, eedge);
+            if (MODE == 3) {
 #pragma unroll
-            for (int i = 0; i < 6; ++i)
-                if (ops.pos[i] >= 0 && ops.pos[i] > nan_after) { lam = (ops.v[i] > lam) ? ops.v[i] : lam; any = true; }
-        } else {
-            long long nanp = -1;
-#pragma unroll
-            for (int i = 0; i < 6; ++i) {
-                const double x = ops.v[i];
-                if (ops.pos[i] < 0) continue;
-                if (x != x) nanp = (ops.pos[i] > nanp) ? ops.pos[i] : nanp;
-                else lam = (x > lam) ? x : lam;
+                for (int i = 0; i < 6; ++i)
+                    if (ops.pos[i] >= 0 && ops.pos[i] > nan_after) { lam = (ops.v[i] > lam) ? ops.v[i] : lam; any = true; }
+                return;
             }
-            if (nanp >= 0) atomicMax(&ctrl->nan_pos[slot], nanp);
+            lam = lam_fold(ops, ctrl, slot);
             if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
-
             double R[9];
-            element_rhs(m.geo, P, e, W, Fh, wd_self, ph.g, ph.rain, R);
-
+            element_rhs(G, W, Fh, wd_self, ph.g, ph.rain, R);
             if (MODE == 0) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f) { dbg_rhs[f * P + e] = R[f]; dbg_fhat[f * P + e] = Fh[f]; }
             } else {
-                const double dt = ctrl->dt;
-                double U[9];
-                if (MODE == 1) {
-#pragma unroll
-                    for (int f = 0; f < 9; ++f) U[f] = W[f] + R[f] * dt;
-                } else {
-                    const double half = dt / 2;
-#pragma unroll
-                    for (int f = 0; f < 9; ++f) U[f] = (W[f] + wbase[f * P + e]) * 0.5 + R[f] * half;
-                }
-#pragma unroll
-                for (int f = 0; f < 9; ++f) wout[f * P + e] = U[f];
-#pragma unroll
-                for (int f = 0; f < 3; ++f) means[f * P + e] = (U[3 * f] + U[3 * f + 1] + U[3 * f + 2]) / 3;
+                rk_update<MODE>(m, e, W, R, wout, wbase, means, ctrl);
             }
+        };
+        if (GEOLOAD) {
+            body(GeoPlanes{m.geo, P, e});
+        } else {
+            GeoRegs G;
+            geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+            body(G);
         }
     }
-    // order-free max of non-NaN operands: wave max, one 64-bit atomic per wave
-    const unsigned long long bits = (unsigned long long)__double_as_longlong(lam);
-    const unsigned long long wm = wave_max(bits);
     if (MODE == 3) {
+        const unsigned long long wm = wave_max((unsigned long long)__double_as_longlong(lam));
         const unsigned long long anyw = __ballot(any);
         if ((threadIdx.x & 63) == 0 && anyw) {
             atomicMax(&ctrl->after_bits[slot], wm);
             atomicOr(&ctrl->any_after[slot], 1);
         }
-    } else if ((threadIdx.x & 63) == 0 && wm != 0ULL) {
-        atomicMax(&ctrl->maxlam[slot], wm);
+    } else {
+        lam_commit(lam, ctrl, slot);
+    }
+}
+
+// Split variant, part 1: the three edge fluxes of each element (the edge loop
+// of compute2DL in the element-centric orientation) into the Fhat planes.
+template <bool WETDRY, bool GEOLOAD>
+__global__ __launch_bounds__(256, DGR_FLUX_WAVES) void k_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                              const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                              const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrl,
+                                              int slot)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    double lam = 0.0;
+    if (e < m.ne) {
+        const int P = m.pitch;
+        double W[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
+        const int wd_self = wd[e];
+        auto body = [&](const auto &G) {
+            double Fh[9];
+            LamOps ops;
+            int ecode, eedge;
+            element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, G, W, wd_self, Fh, ops, ecode, eedge);
+            lam = lam_fold(ops, ctrl, slot);
+            if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
+#pragma unroll
+            for (int f = 0; f < 9; ++f) fhat[f * P + e] = Fh[f];
+        };
+        if (GEOLOAD) {
+            body(GeoPlanes{m.geo, P, e});
+        } else {
+            GeoRegs G;
+            geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+            body(G);
+        }
+    }
+    lam_commit(lam, ctrl, slot);
+}
+
+// Split variant, part 2: inner products, RHS and RK update from the Fhat planes.
+//   MODE 0: RHS only into dbg_rhs; MODE 1/2 as in k_stage.
+template <int MODE, bool GEOLOAD>
+__global__ __launch_bounds__(256, DGR_RHS_WAVES) void k_rhs(MeshDev m, PhysDev ph, const double *__restrict__ win, double *wout,
+                                             const double *wbase, const double *__restrict__ fhat,
+                                             const int *__restrict__ wd, double *__restrict__ means,
+                                             const Ctrl *ctrl, double *__restrict__ dbg_rhs)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    if (e >= m.ne) return;
+    const int P = m.pitch;
+    double W[9], Fh[9];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) Fh[f] = fhat[f * P + e];
+    const int wd_self = wd[e];
+    auto body = [&](const auto &G) {
+        double R[9];
+        element_rhs(G, W, Fh, wd_self, ph.g, ph.rain, R);
+        if (MODE == 0) {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) dbg_rhs[f * P + e] = R[f];
+        } else {
+            rk_update<MODE>(m, e, W, R, wout, wbase, means, ctrl);
+        }
+    };
+    if (GEOLOAD) {
+        body(GeoPlanes{m.geo, P, e});
+    } else {
+        GeoRegs G;
+        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+        body(G);
     }
 }
 
@@ -868,21 +1002,37 @@ __global__ void k_math_selftest(int n, const double *x, const double *y, double 
 // ---------------------------------------------------------------------------
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
-void launch_stage(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
-                  const double *wbase, const double *bzeta, const double *bqn, const int *wd, double *means,
-                  Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat, hipStream_t s)
+// variant: bit 0 = geometry planes (else in registers), bit 1 = split flux/RHS kernels
+void launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
+                  double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
+                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
+                  hipStream_t s)
 {
     const dim3 g = grid_for(m.ne), b(256);
-#define DGR_LS(MD, WDF) \
-    hipLaunchKernelGGL((k_stage<MD, WDF>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
-    if (wetdry) {
-        if (mode == 0) DGR_LS(0, true); else if (mode == 1) DGR_LS(1, true); else if (mode == 2) DGR_LS(2, true);
-        else DGR_LS(3, true);
-    } else {
-        if (mode == 0) DGR_LS(0, false); else if (mode == 1) DGR_LS(1, false); else if (mode == 2) DGR_LS(2, false);
-        else DGR_LS(3, false);
+    const bool gl = (variant & 1) != 0;
+    const bool split = (variant & 2) != 0 && mode != 3;
+#define DGR_LS(MD, WDF, GL) \
+    hipLaunchKernelGGL((k_stage<MD, WDF, GL>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
+#define DGR_LS_W(WDF, GL) \
+    do { if (mode == 0) DGR_LS(0, WDF, GL); else if (mode == 1) DGR_LS(1, WDF, GL); \
+         else if (mode == 2) DGR_LS(2, WDF, GL); else DGR_LS(3, WDF, GL); } while (0)
+#define DGR_LF(WDF, GL) \
+    hipLaunchKernelGGL((k_flux<WDF, GL>), g, b, 0, s, m, ph, win, bzeta, bqn, wd, mode == 0 ? dbg_fhat : fhat, ctrl, slot)
+#define DGR_LR(MD, GL) \
+    hipLaunchKernelGGL((k_rhs<MD, GL>), g, b, 0, s, m, ph, win, wout, wbase, mode == 0 ? dbg_fhat : fhat, wd, means, ctrl, dbg_rhs)
+    if (!split) {
+        if (wetdry) { if (gl) DGR_LS_W(true, true); else DGR_LS_W(true, false); }
+        else { if (gl) DGR_LS_W(false, true); else DGR_LS_W(false, false); }
+        return;
     }
+    if (wetdry) { if (gl) DGR_LF(true, true); else DGR_LF(true, false); }
+    else { if (gl) DGR_LF(false, true); else DGR_LF(false, false); }
+    if (gl) { if (mode == 0) DGR_LR(0, true); else if (mode == 1) DGR_LR(1, true); else DGR_LR(2, true); }
+    else { if (mode == 0) DGR_LR(0, false); else if (mode == 1) DGR_LR(1, false); else DGR_LR(2, false); }
 #undef DGR_LS
+#undef DGR_LS_W
+#undef DGR_LF
+#undef DGR_LR
 }
 
 void launch_geometry(const MeshDev &m, double g, double *geo, hipStream_t s)

@@ -84,6 +84,9 @@ def load():
         "dgr_math_selftest": (C.c_int, [C.c_int, dp, dp, dp]),
         "dgr_profile_begin": (C.c_int, [vp]),
         "dgr_get_report": (C.c_int, [vp, C.POINTER(Report)]),
+        "dgr_set_variant": (C.c_int, [vp, C.c_int]),
+        "dgr_get_variant": (C.c_int, [vp, ip]),
+        "dgr_autotune": (C.c_int, [vp, C.c_int, ip, dp]),
         "dgr_set_final_time": (C.c_int, [vp, C.c_double]),
         "dgr_profile_end": (C.c_int, [vp, dp, ip]),
     }
@@ -244,6 +247,21 @@ class Solver:
 
     def sync(self):
         check(self.lib.dgr_sync(self.h))
+
+    def set_variant(self, v):
+        check(self.lib.dgr_set_variant(self.h, int(v)))
+
+    def variant(self):
+        v = C.c_int(0)
+        check(self.lib.dgr_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    def autotune(self, steps=5):
+        """Pick the fastest kernel variant; returns (chosen, ms/step of each)."""
+        v = C.c_int(0)
+        ms = np.zeros(4)
+        check(self.lib.dgr_autotune(self.h, int(steps), C.byref(v), _dp(ms)))
+        return v.value, ms
 
     def profile_begin(self):
         check(self.lib.dgr_profile_begin(self.h))
