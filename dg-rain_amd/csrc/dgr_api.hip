@@ -13,7 +13,7 @@
 #include "dgr_device.h"
 
 namespace dgr {
-void launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
+int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
                   double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
                   double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
                   hipStream_t s);
@@ -26,7 +26,8 @@ void launch_step_mark(Ctrl *c, double dt, hipStream_t s);
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s);
 void launch_step_begin(const PhysDev &ph, Ctrl *c, hipStream_t s);
-void launch_step_end(const PhysDev &ph, Ctrl *c, hipStream_t s);
+void launch_step_end(const PhysDev &ph, Ctrl *c, int nparts, hipStream_t s);
+void launch_lam_reduce(const PhysDev &ph, Ctrl *c, int slot, int nparts, hipStream_t s);
 void launch_reset_maxlam(Ctrl *c, hipStream_t s);
 void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s);
 }  // namespace dgr
@@ -48,7 +49,11 @@ struct dgr_handle {
     double *d_dbg;   // 18 planes: rhs (9) then fhat (9)
     double *d_geo;   // GX_COUNT planes of per-element geometry
     double *d_fhat;  // 9 planes of edge fluxes (split kernel variant)
-    int variant;     // bit 0: geometry planes, bit 1: split flux/RHS kernels
+    int variant;     // bit 0: geometry planes, bit 1: split flux/RHS kernels, 4/5: edge fluxes
+    int4 *d_edges;   // {el_in, el_ex, va, vb} per edge
+    int *d_einfo;    // packed edge facts (ED_*/EI_* bits)
+    unsigned long long *d_lamparts;   // [2][maxparts] per-block max |lambda|
+    int nparts[2];   // partials written by the last flux launch of each stage
     int *d_wd;
     unsigned char *d_modify;   // [2][pitch]
     Ctrl *d_ctrl;
@@ -115,7 +120,7 @@ static dgr_status free_handle(dgr_handle *h)
     if (!h) return DGR_OK;
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_bzeta,
                     h->d_bqn, h->d_w, h->d_w1, h->d_means, h->d_nodemm, h->d_dbg, h->d_geo, h->d_fhat, h->d_wd, h->d_modify,
-                    h->d_ctrl};
+                    h->d_ctrl, h->d_edges, h->d_einfo, h->d_lamparts};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -146,6 +151,8 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
     std::vector<int> ev(3 * (size_t)pitch, 0), nb(3 * (size_t)pitch, 0), info(3 * (size_t)pitch, 0),
         eid(3 * (size_t)pitch, 0);
     std::vector<unsigned char> seen(3 * (size_t)ne, 0);
+    std::vector<int4> edges(ned);
+    std::vector<int> einfo(ned);
     for (int el = 0; el < ne; ++el)
         for (int k = 0; k < 3; ++k) {
             const int v = mv->elto_vert[3 * el + k];
@@ -168,6 +175,8 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
         int common = ((sg + 1) << EI_SGN_SHIFT) | ((bp < 0 ? 0 : (bp > 255 ? 255 : bp)) << EI_CODE_SHIFT);
         if (bp) common |= EI_PRESCRIBED;
         if (boundary) common |= EI_BOUNDARY;
+        edges[i] = make_int4(el_in, el_ex, ein[kin], ein[(kin + 1) % 3]);
+        einfo[i] = common | kin | (kex << ED_KEX_SHIFT);
         // el_in slot
         {
             size_t s = (size_t)kin * pitch + el_in;
@@ -237,17 +246,24 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
     DGR_TRY(dalloc(&h->d_ctrl, 1));
     DGR_TRY(dalloc(&h->d_geo, (size_t)GX_COUNT * P));
     DGR_TRY(dalloc(&h->d_fhat, 9 * P));
+    DGR_TRY(dalloc(&h->d_edges, ned));
+    DGR_TRY(dalloc(&h->d_einfo, ned));
+    const int maxparts = ((ne > ned ? ne : ned) + 255) / 256;
+    DGR_TRY(dalloc(&h->d_lamparts, 2 * (size_t)maxparts));
     hipStream_t s = h->stream;
     DGR_TRY(hipMemcpyAsync(h->d_nodes, nodes.data(), nn * sizeof(double4), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_ev, ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_nb, nb.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_info, info.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_eid, eid.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_edges, edges.data(), ned * sizeof(int4), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_einfo, einfo.data(), ned * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_n2e_off, off.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_n2e, n2e.data(), 3 * (size_t)ne * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemsetAsync(h->d_w, 0, 9 * P * sizeof(double), s));
     DGR_TRY(hipMemsetAsync(h->d_w1, 0, 9 * P * sizeof(double), s));
     DGR_TRY(hipMemsetAsync(h->d_modify, 0, 2 * P, s));
+    DGR_TRY(hipMemsetAsync(h->d_lamparts, 0, 2 * (size_t)maxparts * sizeof(unsigned long long), s));
     {
         std::vector<int> wd(P, -1);   // initialize.c:126
         DGR_TRY(hipMemcpyAsync(h->d_wd, wd.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
@@ -279,6 +295,7 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
     h->m.nodes = h->d_nodes; h->m.ev = h->d_ev; h->m.nb = h->d_nb; h->m.info = h->d_info; h->m.eid = h->d_eid;
     h->m.n2e_off = h->d_n2e_off; h->m.n2e = h->d_n2e;
     h->m.geo = h->d_geo;
+    h->m.ned = ned; h->m.edges = h->d_edges; h->m.einfo = h->d_einfo;
     launch_geometry(h->m, prm->g, h->d_geo, s);
     PhysDev &ph = h->ph;
     ph.g = prm->g; ph.h0 = prm->h0; ph.velzero = prm->velzero; ph.rain = prm->rain;
@@ -286,6 +303,7 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
     ph.tide_datum = prm->tide_datum; ph.tide_amp = prm->tide_amp; ph.tide_period = prm->tide_period;
     ph.tide_phase = prm->tide_phase;
     ph.wetdry = prm->wetdry; ph.tide_code = prm->tide_code;
+    ph.lamparts = h->d_lamparts; ph.maxparts = maxparts;
     h->h_planes.resize(9 * P);
     // tide value for t = 0
     launch_step_begin(h->ph, h->d_ctrl, s);
@@ -402,8 +420,10 @@ extern "C" dgr_status dgr_compute_rhs(dgr_handle *h, double *rz, double *rqx, do
     if (!h) return DGR_ERR_ARG;
     if (!h->d_dbg) DGR_HIP(dalloc(&h->d_dbg, 18 * (size_t)h->pitch));
     launch_reset_maxlam(h->d_ctrl, h->stream);
-    launch_stage(h->variant, 0, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn,
-                 h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, h->d_dbg, h->d_dbg + 9 * (size_t)h->pitch, h->stream);
+    h->nparts[0] = launch_stage(h->variant, 0, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta,
+                                h->d_bqn, h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, h->d_dbg,
+                                h->d_dbg + 9 * (size_t)h->pitch, h->stream);
+    launch_lam_reduce(h->ph, h->d_ctrl, 0, h->nparts[0], h->stream);
     launch_stage(h->variant, 3, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn,
                  h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, h->stream);
     DGR_HIP(hipGetLastError());
@@ -473,15 +493,15 @@ static void enqueue_stages(dgr_handle *h)
     hipStream_t s = h->stream;
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, s);
     mark(h);
-    launch_stage(h->variant, 1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
-                 h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
+    h->nparts[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn,
+                                h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
     mark(h);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
     launch_limit(wdon, h->m, h->ph, h->d_w1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, s);
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w1, h->d_wd, s);
     mark(h);
-    launch_stage(h->variant, 2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn, h->d_wd,
-                 h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
+    h->nparts[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn,
+                                h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     mark(h);
     // stage 2's max |lambda| sets the next dt; repair its fold if a NaN operand was seen
     launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w1, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
@@ -496,6 +516,7 @@ extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)
     (void)t;
     launch_step_mark(h->d_ctrl, dt, h->stream);
     enqueue_stages(h);
+    launch_lam_reduce(h->ph, h->d_ctrl, 1, h->nparts[1], h->stream);
     DGR_HIP(hipGetLastError());
     Ctrl c;
     dgr_status st = read_ctrl(h, &c);
@@ -509,7 +530,7 @@ extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
     for (int n = 0; n < nsteps; ++n) {
         launch_step_begin(h->ph, h->d_ctrl, h->stream);
         enqueue_stages(h);
-        launch_step_end(h->ph, h->d_ctrl, h->stream);
+        launch_step_end(h->ph, h->d_ctrl, h->nparts[1], h->stream);
     }
     DGR_HIP(hipGetLastError());
     Ctrl c;
@@ -622,7 +643,7 @@ extern "C" dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *laun
 
 extern "C" dgr_status dgr_set_variant(dgr_handle *h, int variant)
 {
-    if (!h || variant < 0 || variant > 3) return DGR_ERR_ARG;
+    if (!h || variant < 0 || variant >= DGR_NUM_VARIANTS) return DGR_ERR_ARG;
     h->variant = variant;
     return DGR_OK;
 }
@@ -655,17 +676,17 @@ extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double
     DGR_HIP(hipEventCreate(&e1));
     int best = h->variant;
     float best_ms = 1e30f;
-    for (int v = 0; v < 4; ++v) {
+    for (int v = 0; v < DGR_NUM_VARIANTS; ++v) {
         h->variant = v;
         // one untimed step to warm the code object, then the timed steps
         launch_step_begin(h->ph, h->d_ctrl, h->stream);
         enqueue_stages(h);
-        launch_step_end(h->ph, h->d_ctrl, h->stream);
+        launch_step_end(h->ph, h->d_ctrl, h->nparts[1], h->stream);
         DGR_HIP(hipEventRecord(e0, h->stream));
         for (int n = 0; n < steps; ++n) {
             launch_step_begin(h->ph, h->d_ctrl, h->stream);
             enqueue_stages(h);
-            launch_step_end(h->ph, h->d_ctrl, h->stream);
+            launch_step_end(h->ph, h->d_ctrl, h->nparts[1], h->stream);
         }
         DGR_HIP(hipEventRecord(e1, h->stream));
         DGR_HIP(hipEventSynchronize(e1));

@@ -13,6 +13,9 @@
 //   means        double [3][pitch]   element averages of the updated state
 //   nodemm       double [6][num_nodes]  node min (zeta,qx,qy) then max
 //   n2e_off/n2e  node -> element CSR in ascending element order
+//   edges        int4   [num_edges]  {el_in, el_ex, va, vb}: EdgtoEls and el_in's
+//                                     vertices of the edge in el_in's order
+//   einfo        int    [num_edges]  packed edge facts, see ED_* below
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -27,6 +30,12 @@ enum : int {
     EI_PRESCRIBED = 1 << 6, // BdryPrescribed != 0
     EI_BOUNDARY = 1 << 7,   // el_in == el_ex
     EI_CODE_SHIFT = 8,      // BdryPrescribed code (clamped to 0..255) in bits 8..15
+};
+
+// edge info word: bits 4..15 as in the element info word, plus
+enum : int {
+    ED_KIN_MASK = 0x3,      // local edge index in el_in
+    ED_KEX_SHIFT = 2,       // local edge index in el_ex in bits 2..3
 };
 
 // Per-element geometry planes (geo[i * pitch + e]), computed once by
@@ -68,6 +77,9 @@ struct MeshDev {
     const int *eid;
     const int *n2e_off;
     const int *n2e;
+    int ned;
+    const int4 *edges;
+    const int *einfo;
 };
 
 struct Ctrl {
@@ -109,6 +121,11 @@ struct PhysDev {
     double cfl, min_edge, final_time;
     double tide_datum, tide_amp, tide_period, tide_phase;
     int wetdry, tide_code;
+    // per-block max |lambda| partials of the flux kernels, [2][maxparts] raw
+    // bits; reduced into Ctrl::maxlam by k_step_end / k_lam_reduce (one plain
+    // store per block instead of one same-address atomic per wavefront)
+    unsigned long long *lamparts;
+    int maxparts;
 };
 
 }  // namespace dgr

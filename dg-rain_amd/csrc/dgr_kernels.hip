@@ -46,6 +46,18 @@ __device__ __forceinline__ int isign(double a) { return a < 0 ? -1 : (a == 0 ? 0
 // record an error; the smallest key (earliest step, stage, phase, then
 // lowest edge/element id) wins, which is the error the reference's serial
 // loops would have stopped at (compute2DL.c:245-302, wetDry2D.c:142-146,233-237)
+// Global thread index with the blocks renumbered XCD by XCD.  The dispatcher
+// hands workgroup b to XCD b % 8; this bijection gives each XCD a contiguous
+// range of elements (or edges), so neighbour gathers and partial-line stores
+// of one region meet in one L2 instead of eight.
+__device__ __forceinline__ int xcd_gid()
+{
+    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
+    const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
+    const int lb = (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
+    return lb * (int)blockDim.x + (int)threadIdx.x;
+}
+
 __device__ __forceinline__ void raise_error(Ctrl *c, int code, int where, int phase)
 {
     const unsigned long long key = ((unsigned long long)c->step << 34) | ((unsigned long long)phase << 32) |
@@ -527,11 +539,35 @@ __device__ __forceinline__ double lam_fold(const LamOps &ops, Ctrl *ctrl, int sl
     return lam;
 }
 
-// wave max of the element maxima, one 64-bit atomic per wavefront
-__device__ __forceinline__ void lam_commit(double lam, Ctrl *ctrl, int slot)
+// block max of the thread maxima, one plain store per block into this
+// stage's partials (reduced by block_reduce_parts); blockDim.x is 256
+__device__ __forceinline__ void lam_commit(double lam, const PhysDev &ph, int slot)
 {
+    __shared__ unsigned long long red[4];
     const unsigned long long wm = wave_max((unsigned long long)__double_as_longlong(lam));
-    if ((threadIdx.x & 63) == 0 && wm != 0ULL) atomicMax(&ctrl->maxlam[slot], wm);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = wm;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long b = red[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i) b = (red[i] > b) ? red[i] : b;
+        ph.lamparts[(size_t)slot * ph.maxparts + blockIdx.x] = b;
+    }
+}
+
+// max of n partials by one block of 256 threads; the result is valid in thread 0
+__device__ __forceinline__ unsigned long long block_reduce_parts(const unsigned long long *p, int n)
+{
+    __shared__ unsigned long long red[4];
+    unsigned long long b = 0ULL;
+    for (int i = threadIdx.x; i < n; i += 256) b = (p[i] > b) ? p[i] : b;
+    b = wave_max(b);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
+    __syncthreads();
+    b = red[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) b = (red[i] > b) ? red[i] : b;
+    return b;
 }
 
 // RK update (time_evolution.c:72-88, 176-178, 208-217) and element means
@@ -580,7 +616,7 @@ __global__ __launch_bounds__(256, DGR_STAGE_WAVES) void k_stage(MeshDev m, PhysD
         nan_after = ctrl->nan_pos[slot];
         if (nan_after < 0) return;
     }
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_gid();
     const bool valid = e < m.ne;
     double lam = 0.0;
     bool any = false;
@@ -628,7 +664,7 @@ __global__ __launch_bounds__(256, DGR_STAGE_WAVES) void k_stage(MeshDev m, PhysD
             atomicOr(&ctrl->any_after[slot], 1);
         }
     } else {
-        lam_commit(lam, ctrl, slot);
+        lam_commit(lam, ph, slot);
     }
 }
 
@@ -640,7 +676,7 @@ __global__ __launch_bounds__(256, DGR_FLUX_WAVES) void k_flux(MeshDev m, PhysDev
                                               const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrl,
                                               int slot)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_gid();
     double lam = 0.0;
     if (e < m.ne) {
         const int P = m.pitch;
@@ -666,7 +702,105 @@ __global__ __launch_bounds__(256, DGR_FLUX_WAVES) void k_flux(MeshDev m, PhysDev
             body(G);
         }
     }
-    lam_commit(lam, ctrl, slot);
+    lam_commit(lam, ph, slot);
+}
+
+// Edge variant of part 1: one thread per edge, the edge loop of compute2DL
+// (compute2DL.c:17-331) as the reference runs it, each Roe flux evaluated
+// once and stored for el_in (slot in_k) and, negated, for el_ex (slot ex_k).
+// Interior edges need one Roe evaluation, not the two of the element-centric
+// gather; both-dry and mixed wet/dry edges evaluate the reference's second
+// flux.  The max-|lambda| operands are the edge's own positions 2*edge and
+// 2*edge+1, so the fold bookkeeping is that of k_flux.
+template <bool WETDRY>
+__global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                                   const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                                   const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrl,
+                                                   int slot)
+{
+    const int ed = xcd_gid();
+    double lam = 0.0;
+    if (ed < m.ned) {
+        const int P = m.pitch;
+        const double g = ph.g;
+        const int4 E = m.edges[ed];
+        const int inf = m.einfo[ed];
+        const int el_in = E.x, el_ex = E.y;
+        const int kin = inf & ED_KIN_MASK, kex = (inf >> ED_KEX_SHIFT) & 3;
+        const bool boundary = (inf & EI_BOUNDARY) != 0;
+        const double4 A = m.nodes[E.z], B = m.nodes[E.w];
+        // edge geometry from el_in (compute2DL.c:98-133); the squares and the
+        // vertex sum do not depend on the vertex order
+        const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+        const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
+        const double nx = sg * (B.y - A.y) / len;
+        const double ny = sg * (A.x - B.x) / len;
+        const double tx = -ny, ty = nx;
+        const double zb = (A.z + B.z) / 2;
+        const double dn = 1. / (nx * ty - ny * tx);
+        const double zin = win[kin * P + el_in], qxin = win[(3 + kin) * P + el_in], qyin = win[(6 + kin) * P + el_in];
+        double zex, qxex, qyex;
+        if (boundary) { zex = zin; qxex = qxin; qyex = qyin; }
+        else { zex = win[kex * P + el_ex]; qxex = win[(3 + kex) * P + el_ex]; qyex = win[(6 + kex) * P + el_ex]; }
+        if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
+            const int code = (inf >> EI_CODE_SHIFT) & 0xff;
+            const bool tide = ph.tide_code != 0 && code == ph.tide_code;
+            zex = tide ? ctrl->tide : bzeta[ed];
+            double qnex = tide ? 1000000.0 : bqn[ed];
+            if (zex < 0) zex = zin;
+            if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+            const double qtex = qxin * tx + qyin * ty;
+            qxex = (ty * qnex - ny * qtex) * dn;
+            qyex = (-tx * qnex + nx * qtex) * dn;
+        } else if (boundary) {                                       // :175-191
+            reflect(qxin, qyin, nx, ny, dn, qxex, qyex);
+        }
+        LamOps ops;
+#pragma unroll
+        for (int i = 0; i < 6; ++i) { ops.v[i] = 0.0; ops.pos[i] = -1; }
+        const long long p0 = 2LL * ed;
+        double Fi[3], Fx[3];
+        bool nan_flux = false;
+        if (WETDRY) {
+            const int wd_in = wd[el_in];
+            const int wd_ex = boundary ? wd_in : wd[el_ex];
+            if (wd_in == 0 && wd_ex == 0) {                          // :203-240
+                double rx, ry, sx, sy;
+                reflect(qxin, qyin, nx, ny, dn, rx, ry);
+                ops.v[0] = roe_flux(Fi, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
+                ops.pos[0] = p0;
+                reflect(qxex, qyex, nx, ny, dn, sx, sy);
+                ops.v[1] = roe_flux(Fx, zex, zex, sx, qxex, sy, qyex, zb, nx, ny, 0.0, g);
+                ops.pos[1] = p0 + 1;
+            } else {                                                 // :241-293
+                // the flux of a wet side uses g, of a dry side 0; both give the
+                // same |lambda| (the eigenvalues use g, not the flux's gravity)
+                const bool in_wet = wd_in == 1, ex_wet = boundary ? in_wet : (wd_ex == 1);
+                double Fg[3], F0[3];
+                if (in_wet || ex_wet) ops.v[0] = roe_flux(Fg, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
+                if (!in_wet || !ex_wet) ops.v[0] = roe_flux(F0, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, 0.0, g);
+                ops.pos[0] = p0;
+#pragma unroll
+                for (int j = 0; j < 3; ++j) { Fi[j] = in_wet ? Fg[j] : F0[j]; Fx[j] = ex_wet ? Fg[j] : F0[j]; }
+                nan_flux = isnan(Fi[0]) || isnan(Fi[1]) || isnan(Fi[2]) ||
+                           (!boundary && (isnan(Fx[0]) || isnan(Fx[1]) || isnan(Fx[2])));
+            }
+        } else {                                                     // :295-315
+            ops.v[0] = roe_flux(Fi, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
+            ops.pos[0] = p0;
+            Fx[0] = Fi[0]; Fx[1] = Fi[1]; Fx[2] = Fi[2];
+            nan_flux = isnan(Fi[0]) || isnan(Fi[1]) || isnan(Fi[2]);
+        }
+        if (nan_flux) raise_error(ctrl, 3, ed, 2 * slot);
+        lam = lam_fold(ops, ctrl, slot);
+#pragma unroll
+        for (int f = 0; f < 3; ++f) fhat[(f * 3 + kin) * P + el_in] = Fi[f];
+        if (!boundary) {
+#pragma unroll
+            for (int f = 0; f < 3; ++f) fhat[(f * 3 + kex) * P + el_ex] = -Fx[f];
+        }
+    }
+    lam_commit(lam, ph, slot);
 }
 
 // Split variant, part 2: inner products, RHS and RK update from the Fhat planes.
@@ -677,7 +811,7 @@ __global__ __launch_bounds__(256, DGR_RHS_WAVES) void k_rhs(MeshDev m, PhysDev p
                                              const int *__restrict__ wd, double *__restrict__ means,
                                              const Ctrl *ctrl, double *__restrict__ dbg_rhs)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_gid();
     if (e >= m.ne) return;
     const int P = m.pitch;
     double W[9], Fh[9];
@@ -708,7 +842,7 @@ __global__ __launch_bounds__(256, DGR_RHS_WAVES) void k_rhs(MeshDev m, PhysDev p
 // element means of the current state (standalone limiter calls)
 __global__ __launch_bounds__(256) void k_means(int ne, int P, const double *__restrict__ w, double *__restrict__ means)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_gid();
     if (e >= ne) return;
 #pragma unroll
     for (int f = 0; f < 3; ++f) means[f * P + e] = (w[(3 * f) * P + e] + w[(3 * f + 1) * P + e] + w[(3 * f + 2) * P + e]) / 3;
@@ -717,7 +851,7 @@ __global__ __launch_bounds__(256) void k_means(int ne, int P, const double *__re
 // node min/max of element means, ascending element order (SlopeLimiter2D.c:106-153)
 __global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means, double *__restrict__ nodemm)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = xcd_gid();
     if (i >= m.nn) return;
     double mx0 = -99999, mx1 = -99999, mx2 = -99999, mn0 = 99999, mn1 = 99999, mn2 = 99999;
     const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
@@ -819,7 +953,7 @@ template <bool PDOP>
 __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w, const double *__restrict__ nodemm,
                                                unsigned char *__restrict__ modify_out, Ctrl *ctrl, int stage)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_gid();
     if (e >= m.ne) return;
     const int P = m.pitch, nn = m.nn;
     const double H0 = ph.h0;
@@ -884,7 +1018,7 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w,
 // standalone PDop2D
 __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, Ctrl *ctrl)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_gid();
     if (e >= m.ne) return;
     const int P = m.pitch;
     double zv[3], S[9];
@@ -901,7 +1035,7 @@ __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, 
 // wetDryStatus2D (wetDry2D.c:16-60)
 __global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const double *__restrict__ w, int *__restrict__ wd)
 {
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
+    const int e = xcd_gid();
     if (e >= m.ne) return;
     const int P = m.pitch;
     const double H0 = ph.h0;
@@ -961,15 +1095,24 @@ __global__ void k_step_mark(Ctrl *c, double dt)
     c->step += 1;
 }
 
-__global__ void k_step_end(PhysDev ph, Ctrl *c)
+__global__ __launch_bounds__(256) void k_step_end(PhysDev ph, Ctrl *c, int nparts)
 {
+    const unsigned long long b = block_reduce_parts(ph.lamparts + ph.maxparts, nparts);
     if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    c->maxlam[1] = b;
     const double time = c->t + c->dt;
     const double ml = maxlam_of(*c, 1);
     double h = ph.cfl * ph.min_edge / ml;
     if ((time + h) > ph.final_time) h = ph.final_time - time;
     c->t = time;
     c->dt = h;
+}
+
+// Ctrl::maxlam[slot] from the partials of the last flux launch of that slot
+__global__ __launch_bounds__(256) void k_lam_reduce(PhysDev ph, Ctrl *c, int slot, int nparts)
+{
+    const unsigned long long b = block_reduce_parts(ph.lamparts + (size_t)slot * ph.maxparts, nparts);
+    if (threadIdx.x == 0) c->maxlam[slot] = b;
 }
 
 __global__ void k_reset_maxlam(Ctrl *c)
@@ -1002,15 +1145,18 @@ __global__ void k_math_selftest(int n, const double *x, const double *y, double 
 // ---------------------------------------------------------------------------
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
 
-// variant: bit 0 = geometry planes (else in registers), bit 1 = split flux/RHS kernels
-void launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
+// variant: bit 0 = geometry planes (else in registers), bit 1 = split flux/RHS
+// kernels with element-centric fluxes, bit 2 = split with edge-centric fluxes
+// Returns the number of max-|lambda| partials the launch wrote (0 for MODE 3).
+int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
                   double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
                   double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
                   hipStream_t s)
 {
     const dim3 g = grid_for(m.ne), b(256);
     const bool gl = (variant & 1) != 0;
-    const bool split = (variant & 2) != 0 && mode != 3;
+    const bool edge = (variant & 4) != 0 && mode != 3;
+    const bool split = ((variant & 2) != 0 || edge) && mode != 3;
 #define DGR_LS(MD, WDF, GL) \
     hipLaunchKernelGGL((k_stage<MD, WDF, GL>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
 #define DGR_LS_W(WDF, GL) \
@@ -1023,12 +1169,17 @@ void launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Ph
     if (!split) {
         if (wetdry) { if (gl) DGR_LS_W(true, true); else DGR_LS_W(true, false); }
         else { if (gl) DGR_LS_W(false, true); else DGR_LS_W(false, false); }
-        return;
+        return mode == 3 ? 0 : (int)g.x;
     }
-    if (wetdry) { if (gl) DGR_LF(true, true); else DGR_LF(true, false); }
+    if (edge) {
+        double *fh = mode == 0 ? dbg_fhat : fhat;
+        if (wetdry) hipLaunchKernelGGL(k_edge_flux<true>, grid_for(m.ned), b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
+        else hipLaunchKernelGGL(k_edge_flux<false>, grid_for(m.ned), b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
+    } else if (wetdry) { if (gl) DGR_LF(true, true); else DGR_LF(true, false); }
     else { if (gl) DGR_LF(false, true); else DGR_LF(false, false); }
     if (gl) { if (mode == 0) DGR_LR(0, true); else if (mode == 1) DGR_LR(1, true); else DGR_LR(2, true); }
     else { if (mode == 0) DGR_LR(0, false); else if (mode == 1) DGR_LR(1, false); else DGR_LR(2, false); }
+    return edge ? (int)grid_for(m.ned).x : (int)g.x;
 #undef DGR_LS
 #undef DGR_LS_W
 #undef DGR_LF
@@ -1072,9 +1223,14 @@ void launch_step_begin(const PhysDev &ph, Ctrl *c, hipStream_t s)
     hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, ph, c);
 }
 
-void launch_step_end(const PhysDev &ph, Ctrl *c, hipStream_t s)
+void launch_step_end(const PhysDev &ph, Ctrl *c, int nparts, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(64), 0, s, ph, c);
+    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, s, ph, c, nparts);
+}
+
+void launch_lam_reduce(const PhysDev &ph, Ctrl *c, int slot, int nparts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_lam_reduce, dim3(1), dim3(256), 0, s, ph, c, slot, nparts);
 }
 
 void launch_step_mark(Ctrl *c, double dt, hipStream_t s)

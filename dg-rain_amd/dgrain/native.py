@@ -13,6 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)                      # .../dg-rain_amd
 REPO_ROOT = os.path.dirname(PKG_ROOT)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdgrain.so")
+NUM_VARIANTS = 6   # DGR_NUM_VARIANTS in include/dgrain.h
 
 DGR_OK = 0
 STATUS = {0: "ok", 1: "invalid argument", 2: "inconsistent mesh", 3: "HIP runtime error",
@@ -259,9 +260,9 @@ class Solver:
     def autotune(self, steps=5):
         """Pick the fastest kernel variant; returns (chosen, ms/step of each)."""
         v = C.c_int(0)
-        ms = np.zeros(4)
+        ms = np.zeros(NUM_VARIANTS)
         check(self.lib.dgr_autotune(self.h, int(steps), C.byref(v), _dp(ms)))
-        return v.value, ms
+        return v.value, [float(x) for x in ms]
 
     def profile_begin(self):
         check(self.lib.dgr_profile_begin(self.h))

@@ -152,11 +152,16 @@ dgr_status dgr_last_error(dgr_handle *h, int *code, int *where);
 dgr_status dgr_profile_begin(dgr_handle *h);
 dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *launches);
 
-/* Kernel variants (all compute the same bits): bit 0 = per-element geometry
- * read from precomputed planes (else recomputed in registers), bit 1 = edge
- * fluxes and RHS in two kernels (else one fused kernel).  dgr_autotune times
- * `steps` steps of each from the current state (restored afterwards), keeps
- * the fastest and reports ms/step per variant in ms_per_step[4] (may be NULL). */
+/* Kernel variants 0..DGR_NUM_VARIANTS-1 (all compute the same bits):
+ *   bit 0 = per-element geometry read from precomputed planes (else
+ *           recomputed in registers);
+ *   bit 1 = edge fluxes and RHS in two kernels, fluxes gathered per element
+ *           (else one fused kernel);
+ *   value 4/5 = two kernels, fluxes computed once per edge.
+ * dgr_autotune times `steps` steps of each from the current state (restored
+ * afterwards), keeps the fastest and reports ms/step per variant in
+ * ms_per_step[DGR_NUM_VARIANTS] (may be NULL). */
+#define DGR_NUM_VARIANTS 6
 dgr_status dgr_set_variant(dgr_handle *h, int variant);
 dgr_status dgr_get_variant(dgr_handle *h, int *variant);
 dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step);

@@ -181,3 +181,44 @@ def test_one_time_step_host_dt(gpu):
         o.oneTimeStep2D(0.0, 0.01)
     for a, b in zip(s.get_state(), o.get_state()):
         assert same(a, b)
+
+
+@pytest.mark.parametrize("variant", list(range(native.NUM_VARIANTS)))
+def test_stage_variants_bitwise(gpu, variant):
+    """Every stage-kernel variant (fused or split flux/RHS kernels, geometry
+    recomputed in registers or read from precomputed planes) gives the same
+    bits as the oracle: one wet/dry compute2DL and a tide+rain trajectory."""
+    mesh = perturbed_mesh(N=8, seed=5)
+    s, o = _pair(mesh, 1)
+    s.set_variant(variant)
+    assert s.variant() == variant
+    z, qx, qy = random_state(mesh, seed=15)
+    wd = np.random.default_rng(5).integers(0, 2, mesh.num_el).astype(np.int32)
+    for x in (s, o):
+        x.set_state(z, qx, qy)
+        x.set_wd(wd)
+    rg, fg, lg = s.compute2DL()
+    ro, fo, lo = o.compute2DL()
+    for a, b in zip(fg + rg, fo + ro):
+        assert same(a, b)
+    assert lg == lo
+    wl = configs.coastal(N=24, nsteps=40, rain_mm_h=30.0)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_variant(variant)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    _compare_run(s, oracle.Oracle.for_workload(wl), 40, 20)
+
+
+def test_autotune_keeps_state(gpu):
+    """dgr_autotune times every variant from the current state and restores
+    it: the run that follows is still bitwise equal to the oracle."""
+    wl = configs.coastal(N=32, nsteps=30)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl)
+    _compare_run(s, o, 10, 10)
+    chosen, ms = s.autotune(3)
+    assert 0 <= chosen < native.NUM_VARIANTS and s.variant() == chosen
+    assert len(ms) == native.NUM_VARIANTS
+    assert all(m > 0 for m in ms), ms
+    _compare_run(s, o, 20, 10)

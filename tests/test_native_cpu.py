@@ -1,6 +1,7 @@
 """CPU-side checks of the product library (no kernel launches)."""
 import ctypes as C
 import os
+import re
 
 import numpy as np
 import pytest
@@ -8,6 +9,8 @@ import pytest
 import oracle
 from dgrain import mesh as M, native
 from helpers import perturbed_mesh
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 
 
 def test_library_exports_every_declared_symbol():
@@ -84,3 +87,9 @@ def test_create_rejects_bad_mesh():
     with pytest.raises(native.DgrError) as ei:
         native.Solver(bad)
     assert ei.value.code == 2
+
+
+def test_num_variants_matches_header():
+    hdr = open(os.path.join(ROOT, "include", "dgrain.h")).read()
+    m = re.search(r"#define DGR_NUM_VARIANTS (\d+)", hdr)
+    assert m and int(m.group(1)) == native.NUM_VARIANTS
