@@ -48,6 +48,8 @@ def parse():
     ap.add_argument("--N", type=int, default=512, help="grid cells per side (2*N*N triangles)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="stage-kernel variant to time (default: pick the fastest with dgr_autotune)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 512 MiB (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -117,7 +119,11 @@ def main():
     s.sync()
     # pick the fastest bit-identical stage-kernel variant for this mesh
     # (dgr_autotune restores the state it started from)
-    variant, tune_ms = s.autotune(10)
+    if a.variant < 0:
+        variant, tune_ms = s.autotune(10)
+    else:
+        s.set_variant(a.variant)
+        variant, tune_ms = a.variant, None
     torch.cuda.synchronize()
     s.profile_begin()
     t0 = time.perf_counter()

@@ -1,6 +1,7 @@
 // dgr_api.hip -- the C ABI of libdgrain (include/dgrain.h): handle lifetime,
-// element-centric neighbour tables, host<->device transposition between the
-// reference layout (el*3+k) and the SoA planes, and the stage sequence.
+// element-centric and edge tables of one or more members, host<->device
+// transposition between the reference layout (el*3+k) and the SoA planes, the
+// stage sequence and its hipGraph capture.
 #include <hip/hip_runtime.h>
 
 #include <cmath>
@@ -14,55 +15,68 @@
 
 namespace dgr {
 int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
-                  double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
-                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
-                  hipStream_t s);
+                 double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
+                 double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
+                 hipStream_t s);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
-void launch_geometry(const MeshDev &m, double g, double *geo, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
                   unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s);
-void launch_step_mark(Ctrl *c, double dt, hipStream_t s);
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s);
-void launch_step_begin(const PhysDev &ph, Ctrl *c, hipStream_t s);
-void launch_step_end(const PhysDev &ph, Ctrl *c, int nparts, hipStream_t s);
-void launch_lam_reduce(const PhysDev &ph, Ctrl *c, int slot, int nparts, hipStream_t s);
-void launch_reset_maxlam(Ctrl *c, hipStream_t s);
+void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
+void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, hipStream_t s);
+void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
+void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s);
+void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s);
 void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s);
 }  // namespace dgr
 
 using namespace dgr;
 
+struct MemberHost {
+    int ne, nn, ned;              // the member's own sizes
+    int el_off, nd_off, ed_off;   // offsets into the concatenated arrays
+    dgr_params prm;
+    double min_edge;
+};
+
 struct dgr_handle {
-    int ne, nn, ned, pitch;
+    int nmem;
+    int pitch, nn, ned_pad;       // padded element count, total nodes, padded edge count
     int device;
     hipStream_t stream;
-    dgr_params prm;
+    std::vector<MemberHost> mem;
     PhysDev ph;
     MeshDev m;
+    int record_modify;
     // device buffers
     double4 *d_nodes;
     int *d_ev, *d_nb, *d_info, *d_eid, *d_n2e_off, *d_n2e;
+    int4 *d_edges;   // {el_in, el_ex, va, vb} per edge
+    int *d_einfo;    // packed edge facts (ED_*/EI_* bits)
+    MemberDev *d_mem;
+    int *d_eblk_mem, *d_dblk_mem;
     double *d_bzeta, *d_bqn;
     double *d_w, *d_w1, *d_means, *d_nodemm;
     double *d_dbg;   // 18 planes: rhs (9) then fhat (9)
-    double *d_geo;   // GX_COUNT planes of per-element geometry
-    double *d_fhat;  // 9 planes of edge fluxes (split kernel variant)
-    int variant;     // bit 0: geometry planes, bit 1: split flux/RHS kernels, 4/5: edge fluxes
-    int4 *d_edges;   // {el_in, el_ex, va, vb} per edge
-    int *d_einfo;    // packed edge facts (ED_*/EI_* bits)
-    unsigned long long *d_lamparts;   // [2][maxparts] per-block max |lambda|
-    int nparts[2];   // partials written by the last flux launch of each stage
+    double *d_fhat;  // 9 planes of edge fluxes (split kernel variants)
     int *d_wd;
     unsigned char *d_modify;   // [2][pitch]
-    Ctrl *d_ctrl;
+    Ctrl *d_ctrl;              // [nmem]
+    unsigned long long *d_lamparts;   // [2][maxparts] per-workgroup max |lambda|
+    int variant;     // 0 fused, 1 split element-centric, 2 split edge-centric
+    int kind[2];     // partials space of the last flux launch of each stage
     // host staging
     std::vector<double> h_planes;
-    // optional HIP-event timing of the flux/RHS/update kernel (k_stage)
+    // optional HIP-event timing of the flux/RHS/update launches
     int profiling;
     std::vector<hipEvent_t> ev_pool;
     size_t ev_used;
+    // one captured step (dgr_set_graph)
+    int use_graph;
+    int graph_variant;
+    hipGraphExec_t graph_exec;
 };
 
 #define DGR_HIP(call)                                                                   \
@@ -118,12 +132,13 @@ static int local_edge(const int *ev, int v0, int v1)
 static dgr_status free_handle(dgr_handle *h)
 {
     if (!h) return DGR_OK;
-    void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_bzeta,
-                    h->d_bqn, h->d_w, h->d_w1, h->d_means, h->d_nodemm, h->d_dbg, h->d_geo, h->d_fhat, h->d_wd, h->d_modify,
-                    h->d_ctrl, h->d_edges, h->d_einfo, h->d_lamparts};
+    void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_edges,
+                    h->d_einfo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
+                    h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
+    if (h->graph_exec) (void)hipGraphExecDestroy(h->graph_exec);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DGR_OK;
@@ -135,30 +150,29 @@ static hipError_t dalloc(T **p, size_t n)
     return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
 }
 
-extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm, dgr_handle **out)
-{
-    if (!mv || !prm || !out) return DGR_ERR_ARG;
-    *out = nullptr;
-    const int ne = mv->num_el, nn = mv->num_nodes, ned = mv->num_edges;
-    if (ne <= 0 || nn <= 0 || ned <= 0 || !mv->x || !mv->y || !mv->z || !mv->n || !mv->elto_vert ||
-        !mv->edg_to_vert || !mv->edg_to_els || !mv->bdry)
-        return DGR_ERR_ARG;
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DGR_ERR_NODEV;
+static inline int pad256(int n) { return (n + 255) / 256 * 256; }
 
-    // element-centric tables, one slot per (element, local edge)
-    const int pitch = (ne + 63) / 64 * 64;
-    std::vector<int> ev(3 * (size_t)pitch, 0), nb(3 * (size_t)pitch, 0), info(3 * (size_t)pitch, 0),
-        eid(3 * (size_t)pitch, 0);
-    std::vector<unsigned char> seen(3 * (size_t)ne, 0);
-    std::vector<int4> edges(ned);
-    std::vector<int> einfo(ned);
+// Host tables of all members, in the concatenated (padded) index space.
+struct HostTables {
+    std::vector<int> ev, nb, info, eid, off, n2e, einfo, eblk_mem, dblk_mem;
+    std::vector<int4> edges;
+    std::vector<double4> nodes;
+    std::vector<double> bz, bq;
+    std::vector<MemberDev> mdev;
+};
+
+// Validates member `mi` and writes its tables at its offsets.
+static dgr_status build_member(const dgr_mesh_view *mv, const MemberHost &M, int mi, int pitch, HostTables &T)
+{
+    const int ne = M.ne, nn = M.nn, ned = M.ned;
+    const int eo = M.el_off, no = M.nd_off, dof = M.ed_off;
     for (int el = 0; el < ne; ++el)
         for (int k = 0; k < 3; ++k) {
             const int v = mv->elto_vert[3 * el + k];
             if (v < 0 || v >= nn) return DGR_ERR_MESH;
-            ev[(size_t)k * pitch + el] = v;
+            T.ev[(size_t)k * pitch + eo + el] = v + no;
         }
+    std::vector<unsigned char> seen(3 * (size_t)ne, 0);
     const double *X = mv->x, *Y = mv->y;
     for (int i = 0; i < ned; ++i) {
         const int el_in = mv->edg_to_els[2 * i], el_ex = mv->edg_to_els[2 * i + 1];
@@ -175,141 +189,217 @@ extern "C" dgr_status dgr_create(const dgr_mesh_view *mv, const dgr_params *prm,
         int common = ((sg + 1) << EI_SGN_SHIFT) | ((bp < 0 ? 0 : (bp > 255 ? 255 : bp)) << EI_CODE_SHIFT);
         if (bp) common |= EI_PRESCRIBED;
         if (boundary) common |= EI_BOUNDARY;
-        edges[i] = make_int4(el_in, el_ex, ein[kin], ein[(kin + 1) % 3]);
-        einfo[i] = common | kin | (kex << ED_KEX_SHIFT);
-        // el_in slot
+        T.edges[dof + i] = make_int4(eo + el_in, eo + el_ex, no + ein[kin], no + ein[(kin + 1) % 3]);
+        T.einfo[dof + i] = common | kin | (kex << ED_KEX_SHIFT);
         {
-            size_t s = (size_t)kin * pitch + el_in;
+            const size_t s = (size_t)kin * pitch + eo + el_in;
             if (seen[3 * (size_t)el_in + kin]++) return DGR_ERR_MESH;
-            nb[s] = el_ex;
-            info[s] = common | EI_IS_IN | EI_DIR_SAME | (kex & EI_KNB_MASK);
-            eid[s] = i;
+            T.nb[s] = eo + el_ex;
+            T.info[s] = common | EI_IS_IN | EI_DIR_SAME | (kex & EI_KNB_MASK);
+            T.eid[s] = dof + i;
         }
         if (!boundary) {
-            size_t s = (size_t)kex * pitch + el_ex;
+            const size_t s = (size_t)kex * pitch + eo + el_ex;
             if (seen[3 * (size_t)el_ex + kex]++) return DGR_ERR_MESH;
             const bool same = ein[kin] == eex[kex];
-            nb[s] = el_in;
-            info[s] = common | (same ? EI_DIR_SAME : 0) | (kin & EI_KNB_MASK);
-            eid[s] = i;
+            T.nb[s] = eo + el_in;
+            T.info[s] = common | (same ? EI_DIR_SAME : 0) | (kin & EI_KNB_MASK);
+            T.eid[s] = dof + i;
         }
+        // boundary_conditions.c:10-83 defaults
+        if (bp == 1) { T.bq[dof + i] = -0.042 / 0.914; T.bz[dof + i] = -1000; }
+        if (bp == 2) { T.bz[dof + i] = 0.296; T.bq[dof + i] = 1000000; }
+        if (bp == 3) { T.bz[dof + i] = -10000; T.bq[dof + i] = -0.1275 / 0.914; }
     }
     for (size_t i = 0; i < seen.size(); ++i)
         if (seen[i] != 1) return DGR_ERR_MESH;   // every Fhat slot written exactly once
     // node -> element CSR, ascending element order (SlopeLimiter2D.c:73-89)
-    std::vector<int> off(nn + 1, 0), n2e(3 * (size_t)ne);
-    for (int i = 0; i < 3 * ne; ++i) off[mv->elto_vert[i] + 1]++;
-    for (int i = 0; i < nn; ++i) off[i + 1] += off[i];
     {
+        std::vector<int> cnt(nn + 1, 0);
+        for (int i = 0; i < 3 * ne; ++i) cnt[mv->elto_vert[i] + 1]++;
+        for (int i = 0; i < nn; ++i) cnt[i + 1] += cnt[i];
+        const int base = T.off[no];   // CSR entries of earlier members
+        for (int i = 0; i <= nn; ++i) T.off[no + i] = base + cnt[i];
         std::vector<int> fill(nn, 0);
         for (int el = 0; el < ne; ++el)
             for (int k = 0; k < 3; ++k) {
                 const int v = mv->elto_vert[3 * el + k];
-                n2e[off[v] + fill[v]++] = el;
+                T.n2e[base + cnt[v] + fill[v]++] = eo + el;
             }
     }
-    std::vector<double4> nodes(nn);
-    for (int i = 0; i < nn; ++i) nodes[i] = make_double4(X[i], Y[i], mv->z[i], mv->n[i]);
-    // minEdgLength (extract_mesh_attributes.c:297-303)
-    double mel = 9999999;
-    for (int e = 0; e < ned; ++e) {
-        const int n0 = mv->edg_to_vert[2 * e], n1 = mv->edg_to_vert[2 * e + 1];
-        const double len = std::sqrt((X[n1] - X[n0]) * (X[n1] - X[n0]) + (Y[n1] - Y[n0]) * (Y[n1] - Y[n0]));
-        mel = (mel < len) ? mel : len;
+    for (int i = 0; i < nn; ++i) T.nodes[no + i] = make_double4(X[i], Y[i], mv->z[i], mv->n[i]);
+    MemberDev &D = T.mdev[mi];
+    memset(&D, 0, sizeof(D));
+    D.el_off = eo; D.ne = ne; D.ed_off = dof; D.ned = ned;
+    D.eblk0 = eo / 256; D.eblks = pad256(ne) / 256;
+    D.dblk0 = dof / 256; D.dblks = pad256(ned) / 256;
+    D.tide_code = M.prm.tide_code;
+    D.rain = M.prm.rain; D.cfl = M.prm.cfl; D.min_edge = M.min_edge; D.final_time = M.prm.final_time;
+    D.tide_datum = M.prm.tide_datum; D.tide_amp = M.prm.tide_amp; D.tide_period = M.prm.tide_period;
+    D.tide_phase = M.prm.tide_phase;
+    for (int b = 0; b < D.eblks; ++b) T.eblk_mem[D.eblk0 + b] = mi;
+    for (int b = 0; b < D.dblks; ++b) T.dblk_mem[D.dblk0 + b] = mi;
+    return DGR_OK;
+}
+
+static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_params *prms, dgr_handle **out)
+{
+    if (n <= 0 || !mvs || !prms || !out) return DGR_ERR_ARG;
+    *out = nullptr;
+    std::vector<MemberHost> mem(n);
+    int pitch = 0, nn_tot = 0, ned_pad = 0, record_modify = 0;
+    for (int i = 0; i < n; ++i) {
+        const dgr_mesh_view *mv = &mvs[i];
+        const dgr_params *p = &prms[i];
+        const int ne = mv->num_el, nn = mv->num_nodes, ned = mv->num_edges;
+        if (ne <= 0 || nn <= 0 || ned <= 0 || !mv->x || !mv->y || !mv->z || !mv->n || !mv->elto_vert ||
+            !mv->edg_to_vert || !mv->edg_to_els || !mv->bdry)
+            return DGR_ERR_ARG;
+        // constants shared by every member
+        if (p->g != prms[0].g || p->h0 != prms[0].h0 || p->velzero != prms[0].velzero ||
+            (p->wetdry != 0) != (prms[0].wetdry != 0))
+            return DGR_ERR_ARG;
+        if ((long long)pitch + pad256(ne) > (1LL << 29) || (long long)ned_pad + pad256(ned) > (1LL << 29))
+            return DGR_ERR_ARG;
+        MemberHost &M = mem[i];
+        M.ne = ne; M.nn = nn; M.ned = ned;
+        M.el_off = pitch; M.nd_off = nn_tot; M.ed_off = ned_pad;
+        M.prm = *p;
+        // minEdgLength (extract_mesh_attributes.c:297-303)
+        double mel = 9999999;
+        for (int e = 0; e < ned; ++e) {
+            const int n0 = mv->edg_to_vert[2 * e], n1 = mv->edg_to_vert[2 * e + 1];
+            if (n0 < 0 || n0 >= nn || n1 < 0 || n1 >= nn) return DGR_ERR_MESH;
+            const double dx = mv->x[n1] - mv->x[n0], dy = mv->y[n1] - mv->y[n0];
+            const double len = std::sqrt(dx * dx + dy * dy);
+            mel = (mel < len) ? mel : len;
+        }
+        M.min_edge = mel;
+        pitch += pad256(ne);
+        nn_tot += nn;
+        ned_pad += pad256(ned);
+        record_modify |= p->record_modify != 0;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return DGR_ERR_NODEV;
+
+    HostTables T;
+    const size_t P = pitch;
+    T.ev.assign(3 * P, 0); T.nb.assign(3 * P, 0); T.info.assign(3 * P, 0); T.eid.assign(3 * P, 0);
+    T.off.assign((size_t)nn_tot + 1, 0);
+    size_t n2e_tot = 0;
+    for (int i = 0; i < n; ++i) n2e_tot += 3 * (size_t)mem[i].ne;
+    T.n2e.assign(n2e_tot, 0);
+    T.edges.assign(ned_pad, make_int4(0, 0, 0, 0));
+    T.einfo.assign(ned_pad, 0);
+    T.nodes.resize(nn_tot);
+    T.bz.assign(ned_pad, 0.0); T.bq.assign(ned_pad, 0.0);
+    T.mdev.resize(n);
+    T.eblk_mem.assign(P / 256, 0);
+    T.dblk_mem.assign((size_t)ned_pad / 256, 0);
+    for (int i = 0; i < n; ++i) {
+        dgr_status st = build_member(&mvs[i], mem[i], i, pitch, T);
+        if (st) return st;
     }
 
     dgr_handle *h = new dgr_handle();   // value-initialised: all pointers null
-    h->ne = ne; h->nn = nn; h->ned = ned; h->pitch = pitch;
-    h->prm = *prm;
+    h->nmem = n; h->pitch = pitch; h->nn = nn_tot; h->ned_pad = ned_pad;
+    h->mem = mem;
+    h->record_modify = record_modify;
     (void)hipGetDevice(&h->device);
 #define DGR_TRY(call)                                          \
     do {                                                       \
         if ((call) != hipSuccess) { free_handle(h); return DGR_ERR_HIP; } \
     } while (0)
     DGR_TRY(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
-    const size_t P = pitch;
-    DGR_TRY(dalloc(&h->d_nodes, nn));
+    hipStream_t s = h->stream;
+    const int maxparts = (pitch > ned_pad ? pitch : ned_pad) / 256;
+    DGR_TRY(dalloc(&h->d_nodes, nn_tot));
     DGR_TRY(dalloc(&h->d_ev, 3 * P));
     DGR_TRY(dalloc(&h->d_nb, 3 * P));
     DGR_TRY(dalloc(&h->d_info, 3 * P));
     DGR_TRY(dalloc(&h->d_eid, 3 * P));
-    DGR_TRY(dalloc(&h->d_n2e_off, nn + 1));
-    DGR_TRY(dalloc(&h->d_n2e, 3 * (size_t)ne));
-    DGR_TRY(dalloc(&h->d_bzeta, ned));
-    DGR_TRY(dalloc(&h->d_bqn, ned));
+    DGR_TRY(dalloc(&h->d_n2e_off, (size_t)nn_tot + 1));
+    DGR_TRY(dalloc(&h->d_n2e, n2e_tot));
+    DGR_TRY(dalloc(&h->d_edges, ned_pad));
+    DGR_TRY(dalloc(&h->d_einfo, ned_pad));
+    DGR_TRY(dalloc(&h->d_mem, n));
+    DGR_TRY(dalloc(&h->d_eblk_mem, P / 256));
+    DGR_TRY(dalloc(&h->d_dblk_mem, (size_t)ned_pad / 256));
+    DGR_TRY(dalloc(&h->d_bzeta, ned_pad));
+    DGR_TRY(dalloc(&h->d_bqn, ned_pad));
     DGR_TRY(dalloc(&h->d_w, 9 * P));
     DGR_TRY(dalloc(&h->d_w1, 9 * P));
     DGR_TRY(dalloc(&h->d_means, 3 * P));
-    DGR_TRY(dalloc(&h->d_nodemm, 6 * (size_t)nn));
+    DGR_TRY(dalloc(&h->d_nodemm, 6 * (size_t)nn_tot));
+    DGR_TRY(dalloc(&h->d_fhat, 9 * P));
     DGR_TRY(dalloc(&h->d_wd, P));
     DGR_TRY(dalloc(&h->d_modify, 2 * P));
-    DGR_TRY(dalloc(&h->d_ctrl, 1));
-    DGR_TRY(dalloc(&h->d_geo, (size_t)GX_COUNT * P));
-    DGR_TRY(dalloc(&h->d_fhat, 9 * P));
-    DGR_TRY(dalloc(&h->d_edges, ned));
-    DGR_TRY(dalloc(&h->d_einfo, ned));
-    const int maxparts = ((ne > ned ? ne : ned) + 255) / 256;
+    DGR_TRY(dalloc(&h->d_ctrl, n));
     DGR_TRY(dalloc(&h->d_lamparts, 2 * (size_t)maxparts));
-    hipStream_t s = h->stream;
-    DGR_TRY(hipMemcpyAsync(h->d_nodes, nodes.data(), nn * sizeof(double4), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_ev, ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_nb, nb.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_info, info.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_eid, eid.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_edges, edges.data(), ned * sizeof(int4), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_einfo, einfo.data(), ned * sizeof(int), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_n2e_off, off.data(), (nn + 1) * sizeof(int), hipMemcpyHostToDevice, s));
-    DGR_TRY(hipMemcpyAsync(h->d_n2e, n2e.data(), 3 * (size_t)ne * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_nodes, T.nodes.data(), nn_tot * sizeof(double4), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_ev, T.ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_nb, T.nb.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_info, T.info.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_eid, T.eid.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_n2e_off, T.off.data(), ((size_t)nn_tot + 1) * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_n2e, T.n2e.data(), n2e_tot * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_edges, T.edges.data(), ned_pad * sizeof(int4), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_einfo, T.einfo.data(), ned_pad * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_mem, T.mdev.data(), n * sizeof(MemberDev), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_eblk_mem, T.eblk_mem.data(), P / 256 * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_dblk_mem, T.dblk_mem.data(), (size_t)ned_pad / 256 * sizeof(int),
+                           hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_bzeta, T.bz.data(), ned_pad * sizeof(double), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_bqn, T.bq.data(), ned_pad * sizeof(double), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemsetAsync(h->d_w, 0, 9 * P * sizeof(double), s));
     DGR_TRY(hipMemsetAsync(h->d_w1, 0, 9 * P * sizeof(double), s));
+    DGR_TRY(hipMemsetAsync(h->d_means, 0, 3 * P * sizeof(double), s));
+    DGR_TRY(hipMemsetAsync(h->d_fhat, 0, 9 * P * sizeof(double), s));
     DGR_TRY(hipMemsetAsync(h->d_modify, 0, 2 * P, s));
     DGR_TRY(hipMemsetAsync(h->d_lamparts, 0, 2 * (size_t)maxparts * sizeof(unsigned long long), s));
     {
         std::vector<int> wd(P, -1);   // initialize.c:126
         DGR_TRY(hipMemcpyAsync(h->d_wd, wd.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
-    }
-    {
-        // boundary_conditions.c:10-83 defaults
-        std::vector<double> bz(ned, 0.0), bq(ned, 0.0);
-        for (int e = 0; e < ned; ++e) {
-            const int bp = mv->bdry[e];
-            if (bp == 1) { bq[e] = -0.042 / 0.914; bz[e] = -1000; }
-            if (bp == 2) { bz[e] = 0.296; bq[e] = 1000000; }
-            if (bp == 3) { bz[e] = -10000; bq[e] = -0.1275 / 0.914; }
+        std::vector<Ctrl> c(n);
+        for (int i = 0; i < n; ++i) {
+            memset(&c[i], 0, sizeof(Ctrl));
+            c[i].t = 0.0;
+            c[i].dt = prms[i].dt0;
+            c[i].err_key = ~0ULL;
+            c[i].nan_pos[0] = c[i].nan_pos[1] = -1;
         }
-        DGR_TRY(hipMemcpyAsync(h->d_bzeta, bz.data(), ned * sizeof(double), hipMemcpyHostToDevice, s));
-        DGR_TRY(hipMemcpyAsync(h->d_bqn, bq.data(), ned * sizeof(double), hipMemcpyHostToDevice, s));
+        DGR_TRY(hipMemcpyAsync(h->d_ctrl, c.data(), n * sizeof(Ctrl), hipMemcpyHostToDevice, s));
+        DGR_TRY(hipStreamSynchronize(s));
     }
-    {
-        Ctrl c;
-        memset(&c, 0, sizeof(c));
-        c.t = 0.0;
-        c.dt = prm->dt0;
-        c.err_key = ~0ULL;
-        c.nan_pos[0] = c.nan_pos[1] = -1;
-        DGR_TRY(hipMemcpyAsync(h->d_ctrl, &c, sizeof(c), hipMemcpyHostToDevice, s));
-    }
-    DGR_TRY(hipStreamSynchronize(s));
 #undef DGR_TRY
-    h->m.ne = ne; h->m.nn = nn; h->m.pitch = pitch;
-    h->m.nodes = h->d_nodes; h->m.ev = h->d_ev; h->m.nb = h->d_nb; h->m.info = h->d_info; h->m.eid = h->d_eid;
-    h->m.n2e_off = h->d_n2e_off; h->m.n2e = h->d_n2e;
-    h->m.geo = h->d_geo;
-    h->m.ned = ned; h->m.edges = h->d_edges; h->m.einfo = h->d_einfo;
-    launch_geometry(h->m, prm->g, h->d_geo, s);
+    MeshDev &m = h->m;
+    m.pitch = pitch; m.nn = nn_tot; m.ned_pad = ned_pad; m.nmem = n;
+    m.nodes = h->d_nodes; m.ev = h->d_ev; m.nb = h->d_nb; m.info = h->d_info; m.eid = h->d_eid;
+    m.n2e_off = h->d_n2e_off; m.n2e = h->d_n2e; m.edges = h->d_edges; m.einfo = h->d_einfo;
+    m.mem = h->d_mem; m.eblk_mem = h->d_eblk_mem; m.dblk_mem = h->d_dblk_mem;
     PhysDev &ph = h->ph;
-    ph.g = prm->g; ph.h0 = prm->h0; ph.velzero = prm->velzero; ph.rain = prm->rain;
-    ph.cfl = prm->cfl; ph.min_edge = mel; ph.final_time = prm->final_time;
-    ph.tide_datum = prm->tide_datum; ph.tide_amp = prm->tide_amp; ph.tide_period = prm->tide_period;
-    ph.tide_phase = prm->tide_phase;
-    ph.wetdry = prm->wetdry; ph.tide_code = prm->tide_code;
+    ph.g = prms[0].g; ph.h0 = prms[0].h0; ph.velzero = prms[0].velzero; ph.wetdry = prms[0].wetdry;
     ph.lamparts = h->d_lamparts; ph.maxparts = maxparts;
-    h->h_planes.resize(9 * P);
+    size_t maxne = 0;
+    for (int i = 0; i < n; ++i) maxne = maxne > (size_t)mem[i].ne ? maxne : (size_t)mem[i].ne;
+    h->h_planes.resize(9 * maxne);
     // tide value for t = 0
-    launch_step_begin(h->ph, h->d_ctrl, s);
+    launch_step_begin(h->m, h->d_ctrl, s);
     if (hipStreamSynchronize(s) != hipSuccess) { free_handle(h); return DGR_ERR_HIP; }
     *out = h;
     return DGR_OK;
+}
+
+extern "C" dgr_status dgr_create(const dgr_mesh_view *mesh, const dgr_params *params, dgr_handle **out)
+{
+    return create_members(1, mesh, params, out);
+}
+
+extern "C" dgr_status dgr_ensemble_create(int n, const dgr_mesh_view *meshes, const dgr_params *params,
+                                          dgr_handle **out)
+{
+    return create_members(n, meshes, params, out);
 }
 
 extern "C" dgr_status dgr_destroy(dgr_handle *h)
@@ -319,99 +409,192 @@ extern "C" dgr_status dgr_destroy(dgr_handle *h)
     return free_handle(h);
 }
 
-static dgr_status planes_up(dgr_handle *h, double *dst, const double *a, const double *b, const double *c)
+extern "C" dgr_status dgr_num_members(dgr_handle *h, int *n)
 {
-    const size_t P = h->pitch;
+    if (!h || !n) return DGR_ERR_ARG;
+    *n = h->nmem;
+    return DGR_OK;
+}
+
+static inline bool bad_member(dgr_handle *h, int mi) { return !h || mi < 0 || mi >= h->nmem; }
+
+// reference layout (el*3+k) <-> device planes of member mi
+static dgr_status planes_up(dgr_handle *h, int mi, double *dst, const double *a, const double *b, const double *c)
+{
+    const MemberHost &M = h->mem[mi];
+    const size_t ne = M.ne, P = h->pitch;
     const double *src[3] = {a, b, c};
     for (int f = 0; f < 3; ++f)
-        for (int el = 0; el < h->ne; ++el)
-            for (int k = 0; k < 3; ++k) h->h_planes[(size_t)(3 * f + k) * P + el] = src[f][3 * el + k];
-    DGR_HIP(hipMemcpyAsync(dst, h->h_planes.data(), 9 * P * sizeof(double), hipMemcpyHostToDevice, h->stream));
+        for (size_t el = 0; el < ne; ++el)
+            for (int k = 0; k < 3; ++k) h->h_planes[(3 * f + k) * ne + el] = src[f][3 * el + k];
+    for (int pl = 0; pl < 9; ++pl)
+        DGR_HIP(hipMemcpyAsync(dst + pl * P + M.el_off, h->h_planes.data() + pl * ne, ne * sizeof(double),
+                               hipMemcpyHostToDevice, h->stream));
     DGR_HIP(hipStreamSynchronize(h->stream));
     return DGR_OK;
 }
 
-static dgr_status planes_down(dgr_handle *h, const double *srcd, double *a, double *b, double *c)
+static dgr_status planes_down(dgr_handle *h, int mi, const double *srcd, double *a, double *b, double *c)
 {
-    const size_t P = h->pitch;
-    DGR_HIP(hipMemcpyAsync(h->h_planes.data(), srcd, 9 * P * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    const MemberHost &M = h->mem[mi];
+    const size_t ne = M.ne, P = h->pitch;
+    for (int pl = 0; pl < 9; ++pl)
+        DGR_HIP(hipMemcpyAsync(h->h_planes.data() + pl * ne, srcd + pl * P + M.el_off, ne * sizeof(double),
+                               hipMemcpyDeviceToHost, h->stream));
     DGR_HIP(hipStreamSynchronize(h->stream));
     double *dst[3] = {a, b, c};
     for (int f = 0; f < 3; ++f)
         if (dst[f])
-            for (int el = 0; el < h->ne; ++el)
-                for (int k = 0; k < 3; ++k) dst[f][3 * el + k] = h->h_planes[(size_t)(3 * f + k) * P + el];
+            for (size_t el = 0; el < ne; ++el)
+                for (int k = 0; k < 3; ++k) dst[f][3 * el + k] = h->h_planes[(3 * f + k) * ne + el];
     return DGR_OK;
 }
 
+extern "C" dgr_status dgr_member_set_state(dgr_handle *h, int mi, const double *z, const double *qx, const double *qy)
+{
+    if (bad_member(h, mi) || !z || !qx || !qy) return DGR_ERR_ARG;
+    const unsigned long long none = ~0ULL;   // a new state clears a recorded error
+    DGR_HIP(hipMemcpyAsync(&h->d_ctrl[mi].err_key, &none, sizeof(none), hipMemcpyHostToDevice, h->stream));
+    return planes_up(h, mi, h->d_w, z, qx, qy);
+}
+
+extern "C" dgr_status dgr_member_get_state(dgr_handle *h, int mi, double *z, double *qx, double *qy)
+{
+    if (bad_member(h, mi)) return DGR_ERR_ARG;
+    return planes_down(h, mi, h->d_w, z, qx, qy);
+}
+
+extern "C" dgr_status dgr_member_set_wd(dgr_handle *h, int mi, const int *wd)
+{
+    if (bad_member(h, mi) || !wd) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[mi];
+    DGR_HIP(hipMemcpyAsync(h->d_wd + M.el_off, wd, M.ne * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_member_get_wd(dgr_handle *h, int mi, int *wd)
+{
+    if (bad_member(h, mi) || !wd) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[mi];
+    DGR_HIP(hipMemcpyAsync(wd, h->d_wd + M.el_off, M.ne * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_member_get_modify(dgr_handle *h, int mi, int stage, unsigned char *mod)
+{
+    if (bad_member(h, mi) || !mod || stage < 0 || stage > 1) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[mi];
+    DGR_HIP(hipMemcpyAsync(mod, h->d_modify + (size_t)stage * h->pitch + M.el_off, M.ne, hipMemcpyDeviceToHost,
+                           h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_member_set_boundary(dgr_handle *h, int mi, const double *bz, const double *bq)
+{
+    if (bad_member(h, mi) || !bz || !bq) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[mi];
+    DGR_HIP(hipMemcpyAsync(h->d_bzeta + M.ed_off, bz, M.ned * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    DGR_HIP(hipMemcpyAsync(h->d_bqn + M.ed_off, bq, M.ned * sizeof(double), hipMemcpyHostToDevice, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+static dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c)
+{
+    DGR_HIP(hipMemcpyAsync(c, h->d_ctrl + mi, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_member_set_time(dgr_handle *h, int mi, double t, double dt)
+{
+    if (bad_member(h, mi)) return DGR_ERR_ARG;
+    // t, dt, then the tide value at t; the step counter (which only orders
+    // error keys) is left as it was
+    Ctrl c;
+    dgr_status st = read_ctrl(h, mi, &c);
+    if (st) return st;
+    c.t = t;
+    c.dt = dt;
+    const MemberHost &M = h->mem[mi];
+    if (M.prm.tide_code) {
+        // the device evaluates the tide in k_step_begin; run it for every
+        // member and restore the others' step counters
+        DGR_HIP(hipMemcpyAsync(h->d_ctrl + mi, &c, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+        std::vector<Ctrl> all(h->nmem);
+        DGR_HIP(hipMemcpyAsync(all.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+        DGR_HIP(hipStreamSynchronize(h->stream));
+        launch_step_begin(h->m, h->d_ctrl, h->stream);
+        DGR_HIP(hipGetLastError());
+        std::vector<Ctrl> after(h->nmem);
+        DGR_HIP(hipMemcpyAsync(after.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+        DGR_HIP(hipStreamSynchronize(h->stream));
+        all[mi].tide = after[mi].tide;
+        DGR_HIP(hipMemcpyAsync(h->d_ctrl, all.data(), h->nmem * sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+    } else {
+        DGR_HIP(hipMemcpyAsync(h->d_ctrl + mi, &c, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+    }
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+static void fill_report(const Ctrl &c, dgr_report *rep)
+{
+    rep->t = c.t;
+    rep->dt = c.dt;
+    rep->max_lambda = maxlam_of(c, 1);
+    rep->error_code = err_code_of(c.err_key);
+    rep->error_where = err_where_of(c.err_key);
+}
+
+extern "C" dgr_status dgr_member_report(dgr_handle *h, int mi, dgr_report *rep)
+{
+    if (bad_member(h, mi) || !rep) return DGR_ERR_ARG;
+    Ctrl c;
+    dgr_status st = read_ctrl(h, mi, &c);
+    if (st) return st;
+    fill_report(c, rep);
+    return rep->error_code ? DGR_ERR_NAN : DGR_OK;
+}
+
+// any member with a recorded error
+static dgr_status any_error(dgr_handle *h)
+{
+    std::vector<Ctrl> c(h->nmem);
+    DGR_HIP(hipMemcpyAsync(c.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    for (const Ctrl &x : c)
+        if (err_code_of(x.err_key)) return DGR_ERR_NAN;
+    return DGR_OK;
+}
+
+// single-instance accessors: member 0
 extern "C" dgr_status dgr_set_state(dgr_handle *h, const double *z, const double *qx, const double *qy)
 {
-    if (!h || !z || !qx || !qy) return DGR_ERR_ARG;
-    const unsigned long long none = ~0ULL;   // a new state clears a recorded error
-    DGR_HIP(hipMemcpyAsync(&h->d_ctrl->err_key, &none, sizeof(none), hipMemcpyHostToDevice, h->stream));
-    return planes_up(h, h->d_w, z, qx, qy);
+    return dgr_member_set_state(h, 0, z, qx, qy);
 }
-
 extern "C" dgr_status dgr_get_state(dgr_handle *h, double *z, double *qx, double *qy)
 {
-    if (!h) return DGR_ERR_ARG;
-    return planes_down(h, h->d_w, z, qx, qy);
+    return dgr_member_get_state(h, 0, z, qx, qy);
 }
-
-extern "C" dgr_status dgr_set_wd(dgr_handle *h, const int *wd)
-{
-    if (!h || !wd) return DGR_ERR_ARG;
-    DGR_HIP(hipMemcpyAsync(h->d_wd, wd, h->ne * sizeof(int), hipMemcpyHostToDevice, h->stream));
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
-}
-
-extern "C" dgr_status dgr_get_wd(dgr_handle *h, int *wd)
-{
-    if (!h || !wd) return DGR_ERR_ARG;
-    DGR_HIP(hipMemcpyAsync(wd, h->d_wd, h->ne * sizeof(int), hipMemcpyDeviceToHost, h->stream));
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
-}
-
+extern "C" dgr_status dgr_set_wd(dgr_handle *h, const int *wd) { return dgr_member_set_wd(h, 0, wd); }
+extern "C" dgr_status dgr_get_wd(dgr_handle *h, int *wd) { return dgr_member_get_wd(h, 0, wd); }
 extern "C" dgr_status dgr_get_modify(dgr_handle *h, int stage, unsigned char *mod)
 {
-    if (!h || !mod || stage < 0 || stage > 1) return DGR_ERR_ARG;
-    DGR_HIP(hipMemcpyAsync(mod, h->d_modify + (size_t)stage * h->pitch, h->ne, hipMemcpyDeviceToHost, h->stream));
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
+    return dgr_member_get_modify(h, 0, stage, mod);
 }
-
 extern "C" dgr_status dgr_set_boundary(dgr_handle *h, const double *bz, const double *bq)
 {
-    if (!h || !bz || !bq) return DGR_ERR_ARG;
-    DGR_HIP(hipMemcpyAsync(h->d_bzeta, bz, h->ned * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    DGR_HIP(hipMemcpyAsync(h->d_bqn, bq, h->ned * sizeof(double), hipMemcpyHostToDevice, h->stream));
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
+    return dgr_member_set_boundary(h, 0, bz, bq);
 }
-
-extern "C" dgr_status dgr_set_time(dgr_handle *h, double t, double dt)
-{
-    if (!h) return DGR_ERR_ARG;
-    double v[2] = {t, dt};
-    DGR_HIP(hipMemcpyAsync(h->d_ctrl, v, sizeof(v), hipMemcpyHostToDevice, h->stream));
-    launch_step_begin(h->ph, h->d_ctrl, h->stream);   // tide value at t
-    DGR_HIP(hipGetLastError());
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
-}
-
-static dgr_status read_ctrl(dgr_handle *h, Ctrl *c)
-{
-    DGR_HIP(hipMemcpyAsync(c, h->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
-}
+extern "C" dgr_status dgr_set_time(dgr_handle *h, double t, double dt) { return dgr_member_set_time(h, 0, t, dt); }
 
 static inline unsigned char *modify_ptr(dgr_handle *h, int stage)
 {
-    return h->prm.record_modify ? h->d_modify + (size_t)stage * h->pitch : nullptr;
+    return h->record_modify ? h->d_modify + (size_t)stage * h->pitch : nullptr;
 }
 
 extern "C" dgr_status dgr_compute_rhs(dgr_handle *h, double *rz, double *rqx, double *rqy, double *f1, double *f2,
@@ -419,27 +602,28 @@ extern "C" dgr_status dgr_compute_rhs(dgr_handle *h, double *rz, double *rqx, do
 {
     if (!h) return DGR_ERR_ARG;
     if (!h->d_dbg) DGR_HIP(dalloc(&h->d_dbg, 18 * (size_t)h->pitch));
-    launch_reset_maxlam(h->d_ctrl, h->stream);
-    h->nparts[0] = launch_stage(h->variant, 0, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta,
-                                h->d_bqn, h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, h->d_dbg,
-                                h->d_dbg + 9 * (size_t)h->pitch, h->stream);
-    launch_lam_reduce(h->ph, h->d_ctrl, 0, h->nparts[0], h->stream);
-    launch_stage(h->variant, 3, h->prm.wetdry != 0, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn,
-                 h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, h->stream);
+    const bool wdon = h->ph.wetdry != 0;
+    launch_reset_maxlam(h->m, h->d_ctrl, h->stream);
+    h->kind[0] = launch_stage(h->variant, 0, wdon, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn,
+                              h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, h->d_dbg, h->d_dbg + 9 * (size_t)h->pitch,
+                              h->stream);
+    launch_lam_reduce(h->m, h->ph, h->d_ctrl, 0, h->kind[0], h->stream);
+    launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd, nullptr,
+                 h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, h->stream);
     DGR_HIP(hipGetLastError());
     if (rz || rqx || rqy) {
-        dgr_status st = planes_down(h, h->d_dbg, rz, rqx, rqy);
+        dgr_status st = planes_down(h, 0, h->d_dbg, rz, rqx, rqy);
         if (st) return st;
     }
     if (f1 || f2 || f3) {
-        dgr_status st = planes_down(h, h->d_dbg + 9 * (size_t)h->pitch, f1, f2, f3);
+        dgr_status st = planes_down(h, 0, h->d_dbg + 9 * (size_t)h->pitch, f1, f2, f3);
         if (st) return st;
     }
     Ctrl c;
-    dgr_status st = read_ctrl(h, &c);
+    dgr_status st = read_ctrl(h, 0, &c);
     if (st) return st;
     if (max_lambda) *max_lambda = maxlam_of(c, 0);
-    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
+    return any_error(h);
 }
 
 extern "C" dgr_status dgr_limit(dgr_handle *h)
@@ -467,13 +651,10 @@ extern "C" dgr_status dgr_pdop(dgr_handle *h)
     if (!h) return DGR_ERR_ARG;
     launch_pdop(h->m, h->ph, h->d_w, h->d_ctrl, h->stream);
     DGR_HIP(hipGetLastError());
-    Ctrl c;
-    dgr_status st = read_ctrl(h, &c);
-    if (st) return st;
-    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
+    return any_error(h);
 }
 
-// event pairs around k_stage launches when profiling is on
+// event pairs around the flux/RHS/update launches when profiling is on
 static void mark(dgr_handle *h)
 {
     if (!h->profiling) return;
@@ -487,21 +668,22 @@ static void mark(dgr_handle *h)
 
 // the two stages of oneTimeStep2D (time_evolution.c:10-246; wet/dry calls at the
 // places ChanNet/time_evolution.c:218-427 puts them), enqueued on h->stream
+// for every member at once
 static void enqueue_stages(dgr_handle *h)
 {
-    const bool wdon = h->prm.wetdry != 0;
+    const bool wdon = h->ph.wetdry != 0;
     hipStream_t s = h->stream;
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, s);
     mark(h);
-    h->nparts[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn,
-                                h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
+    h->kind[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn,
+                              h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
     mark(h);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
     launch_limit(wdon, h->m, h->ph, h->d_w1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, s);
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w1, h->d_wd, s);
     mark(h);
-    h->nparts[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn,
-                                h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
+    h->kind[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn,
+                              h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     mark(h);
     // stage 2's max |lambda| sets the next dt; repair its fold if a NaN operand was seen
     launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w1, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
@@ -510,61 +692,94 @@ static void enqueue_stages(dgr_handle *h)
     launch_limit(wdon, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1, s);
 }
 
+static void enqueue_step(dgr_handle *h)
+{
+    launch_step_begin(h->m, h->d_ctrl, h->stream);
+    enqueue_stages(h);
+    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], h->stream);
+}
+
 extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)
 {
     if (!h) return DGR_ERR_ARG;
     (void)t;
-    launch_step_mark(h->d_ctrl, dt, h->stream);
+    launch_step_mark(h->m, h->d_ctrl, dt, h->stream);
     enqueue_stages(h);
-    launch_lam_reduce(h->ph, h->d_ctrl, 1, h->nparts[1], h->stream);
+    launch_lam_reduce(h->m, h->ph, h->d_ctrl, 1, h->kind[1], h->stream);
     DGR_HIP(hipGetLastError());
-    Ctrl c;
-    dgr_status st = read_ctrl(h, &c);
-    if (st) return st;
-    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
+    return any_error(h);
+}
+
+// one step of every member captured as a hipGraph (rebuilt when the variant changes)
+static dgr_status step_graph(dgr_handle *h)
+{
+    if (h->graph_exec && h->graph_variant == h->variant) return DGR_OK;
+    if (h->graph_exec) {
+        DGR_HIP(hipGraphExecDestroy(h->graph_exec));
+        h->graph_exec = nullptr;
+    }
+    hipGraph_t g = nullptr;
+    DGR_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_step(h);
+    const hipError_t le = hipGetLastError();
+    const hipError_t ce = hipStreamEndCapture(h->stream, &g);
+    if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        fprintf(stderr, "libdgrain: step capture failed: %s\n", hipGetErrorString(le != hipSuccess ? le : ce));
+        return DGR_ERR_HIP;
+    }
+    const hipError_t ie = hipGraphInstantiate(&h->graph_exec, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        h->graph_exec = nullptr;
+        fprintf(stderr, "libdgrain: graph instantiate failed: %s\n", hipGetErrorString(ie));
+        return DGR_ERR_HIP;
+    }
+    h->graph_variant = h->variant;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_set_graph(dgr_handle *h, int on)
+{
+    if (!h) return DGR_ERR_ARG;
+    h->use_graph = on != 0;
+    return DGR_OK;
 }
 
 extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
 {
     if (!h || nsteps < 0) return DGR_ERR_ARG;
-    for (int n = 0; n < nsteps; ++n) {
-        launch_step_begin(h->ph, h->d_ctrl, h->stream);
-        enqueue_stages(h);
-        launch_step_end(h->ph, h->d_ctrl, h->nparts[1], h->stream);
+    if (h->use_graph && !h->profiling && nsteps > 0) {
+        dgr_status st = step_graph(h);
+        if (st) return st;
+        for (int n = 0; n < nsteps; ++n) DGR_HIP(hipGraphLaunch(h->graph_exec, h->stream));
+    } else {
+        for (int n = 0; n < nsteps; ++n) enqueue_step(h);
     }
     DGR_HIP(hipGetLastError());
-    Ctrl c;
-    dgr_status st = read_ctrl(h, &c);
-    if (st) return st;
     if (rep) {
-        rep->t = c.t;
-        rep->dt = c.dt;
-        rep->max_lambda = maxlam_of(c, 1);
-        rep->error_code = err_code_of(c.err_key);
-        rep->error_where = err_where_of(c.err_key);
+        Ctrl c;
+        dgr_status st = read_ctrl(h, 0, &c);
+        if (st) return st;
+        fill_report(c, rep);
     }
-    return err_code_of(c.err_key) ? DGR_ERR_NAN : DGR_OK;
+    return any_error(h);
 }
 
-extern "C" dgr_status dgr_get_report(dgr_handle *h, dgr_report *rep)
-{
-    if (!h || !rep) return DGR_ERR_ARG;
-    Ctrl c;
-    dgr_status st = read_ctrl(h, &c);
-    if (st) return st;
-    rep->t = c.t;
-    rep->dt = c.dt;
-    rep->max_lambda = maxlam_of(c, 1);
-    rep->error_code = err_code_of(c.err_key);
-    rep->error_where = err_where_of(c.err_key);
-    return rep->error_code ? DGR_ERR_NAN : DGR_OK;
-}
+extern "C" dgr_status dgr_get_report(dgr_handle *h, dgr_report *rep) { return dgr_member_report(h, 0, rep); }
 
 extern "C" dgr_status dgr_set_final_time(dgr_handle *h, double final_time)
 {
     if (!h) return DGR_ERR_ARG;
-    h->prm.final_time = final_time;
-    h->ph.final_time = final_time;
+    std::vector<MemberDev> md(h->nmem);
+    DGR_HIP(hipMemcpyAsync(md.data(), h->d_mem, h->nmem * sizeof(MemberDev), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < h->nmem; ++i) {
+        h->mem[i].prm.final_time = final_time;
+        md[i].final_time = final_time;
+    }
+    DGR_HIP(hipMemcpyAsync(h->d_mem, md.data(), h->nmem * sizeof(MemberDev), hipMemcpyHostToDevice, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
     return DGR_OK;
 }
 
@@ -572,7 +787,7 @@ extern "C" dgr_status dgr_last_error(dgr_handle *h, int *code, int *where)
 {
     if (!h) return DGR_ERR_ARG;
     Ctrl c;
-    dgr_status st = read_ctrl(h, &c);
+    dgr_status st = read_ctrl(h, 0, &c);
     if (st) return st;
     if (code) *code = err_code_of(c.err_key);
     if (where) *where = err_where_of(c.err_key);
@@ -591,6 +806,14 @@ extern "C" dgr_status dgr_device_state(dgr_handle *h, double **planes, int *pitc
     if (!h || !planes || !pitch) return DGR_ERR_ARG;
     *planes = h->d_w;
     *pitch = h->pitch;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_member_layout(dgr_handle *h, int mi, int *el_offset, int *num_el)
+{
+    if (bad_member(h, mi)) return DGR_ERR_ARG;
+    if (el_offset) *el_offset = h->mem[mi].el_off;
+    if (num_el) *num_el = h->mem[mi].ne;
     return DGR_OK;
 }
 
@@ -657,37 +880,33 @@ extern "C" dgr_status dgr_get_variant(dgr_handle *h, int *variant)
 
 // Times `steps` steps of every kernel variant from the current state and keeps
 // the fastest.  All variants compute the same bits; the state, WD flags and
-// step bookkeeping are restored afterwards.
+// step bookkeeping of every member are restored afterwards.
 extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step)
 {
     if (!h || steps <= 0) return DGR_ERR_ARG;
     const size_t P = h->pitch;
     double *w_save = nullptr;
     int *wd_save = nullptr;
-    Ctrl c_save;
+    std::vector<Ctrl> c_save(h->nmem);
     DGR_HIP(dalloc(&w_save, 9 * P));
     DGR_HIP(dalloc(&wd_save, P));
     DGR_HIP(hipMemcpyAsync(w_save, h->d_w, 9 * P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
     DGR_HIP(hipMemcpyAsync(wd_save, h->d_wd, P * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
-    DGR_HIP(hipMemcpyAsync(&c_save, h->d_ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipMemcpyAsync(c_save.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
     DGR_HIP(hipStreamSynchronize(h->stream));
     hipEvent_t e0, e1;
     DGR_HIP(hipEventCreate(&e0));
     DGR_HIP(hipEventCreate(&e1));
+    const int prof = h->profiling;
+    h->profiling = 0;
     int best = h->variant;
     float best_ms = 1e30f;
     for (int v = 0; v < DGR_NUM_VARIANTS; ++v) {
         h->variant = v;
         // one untimed step to warm the code object, then the timed steps
-        launch_step_begin(h->ph, h->d_ctrl, h->stream);
-        enqueue_stages(h);
-        launch_step_end(h->ph, h->d_ctrl, h->nparts[1], h->stream);
+        enqueue_step(h);
         DGR_HIP(hipEventRecord(e0, h->stream));
-        for (int n = 0; n < steps; ++n) {
-            launch_step_begin(h->ph, h->d_ctrl, h->stream);
-            enqueue_stages(h);
-            launch_step_end(h->ph, h->d_ctrl, h->nparts[1], h->stream);
-        }
+        for (int n = 0; n < steps; ++n) enqueue_step(h);
         DGR_HIP(hipEventRecord(e1, h->stream));
         DGR_HIP(hipEventSynchronize(e1));
         float ms = 0.f;
@@ -696,9 +915,10 @@ extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double
         if (ms < best_ms) { best_ms = ms; best = v; }
         DGR_HIP(hipMemcpyAsync(h->d_w, w_save, 9 * P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
         DGR_HIP(hipMemcpyAsync(h->d_wd, wd_save, P * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
-        DGR_HIP(hipMemcpyAsync(h->d_ctrl, &c_save, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+        DGR_HIP(hipMemcpyAsync(h->d_ctrl, c_save.data(), h->nmem * sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
         DGR_HIP(hipStreamSynchronize(h->stream));
     }
+    h->profiling = prof;
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     (void)hipFree(w_save);

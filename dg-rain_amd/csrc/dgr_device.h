@@ -1,21 +1,30 @@
 // dgr_device.h -- device-side data layout shared by the kernels and the host
 // code of libdgrain.
 //
-// HBM layout (one instance):
+// A handle holds one or more independent instances ("members": an ensemble of
+// meshes, each with its own forcing, time and dt).  Their elements, nodes and
+// edges are concatenated; each member's element range and edge range start on
+// a 256-entry boundary, so every workgroup of an element or edge kernel
+// belongs to exactly one member and reads that member's parameters and
+// control block with scalar loads.  A single instance is the one-member case.
+//
+// HBM layout (all members):
 //   state        double [9][pitch]   planes zeta0..2, qx0..2, qy0..2 (plane f*3+k
 //                                     holds the reference's field f at el*3+k)
-//   nodes        double4[num_nodes]  {x, y, z, n}: one 32-byte gather per vertex
-//   ev           int    [3][pitch]   EltoVert, SoA
+//   nodes        double4[nn]         {x, y, z, n}: one 32-byte gather per vertex
+//   ev           int    [3][pitch]   EltoVert, SoA (global node ids)
 //   nb           int    [3][pitch]   neighbour element across local edge k
 //                                     (the element itself on a boundary edge)
 //   info         int    [3][pitch]   packed edge facts, see EI_* below
 //   eid          int    [3][pitch]   global edge id (boundary data lookup)
 //   means        double [3][pitch]   element averages of the updated state
-//   nodemm       double [6][num_nodes]  node min (zeta,qx,qy) then max
+//   nodemm       double [6][nn]      node min (zeta,qx,qy) then max
 //   n2e_off/n2e  node -> element CSR in ascending element order
-//   edges        int4   [num_edges]  {el_in, el_ex, va, vb}: EdgtoEls and el_in's
+//   edges        int4   [ned_pad]    {el_in, el_ex, va, vb}: EdgtoEls and el_in's
 //                                     vertices of the edge in el_in's order
-//   einfo        int    [num_edges]  packed edge facts, see ED_* below
+//   einfo        int    [ned_pad]    packed edge facts, see ED_* below
+//   eblk_mem     int    [pitch/256]  member of each element workgroup
+//   dblk_mem     int    [ned_pad/256] member of each edge workgroup
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -38,10 +47,8 @@ enum : int {
     ED_KEX_SHIFT = 2,       // local edge index in el_ex in bits 2..3
 };
 
-// Per-element geometry planes (geo[i * pitch + e]), computed once by
-// k_geometry with the reference's own expressions, so every value carries the
-// bits the reference computes on the fly (2DInnerProducts.c:22-308,
-// compute2DL.c:98-144,380-391).
+// Indices of the per-element geometry quantities that geometry_of() hands
+// out (2DInnerProducts.c:22-308, compute2DL.c:98-144,380-391).
 enum GeoIdx : int {
     GX_XE = 0,    // Gauss point x on edge e, point n: GX_XE + 2e + n
     GX_YE = 6,    // Gauss point y
@@ -67,9 +74,19 @@ enum GeoIdx : int {
     GX_COUNT = 75
 };
 
+// per-member constants
+struct MemberDev {
+    int el_off, ne;           // elements [el_off, el_off + ne)
+    int ed_off, ned;          // edges [ed_off, ed_off + ned)
+    int eblk0, eblks;         // element workgroups of the member
+    int dblk0, dblks;         // edge workgroups of the member
+    int tide_code, pad;
+    double rain, cfl, min_edge, final_time;
+    double tide_datum, tide_amp, tide_period, tide_phase;
+};
+
 struct MeshDev {
-    int ne, nn, pitch;
-    const double *geo;
+    int pitch, nn, ned_pad, nmem;
     const double4 *nodes;
     const int *ev;
     const int *nb;
@@ -77,11 +94,14 @@ struct MeshDev {
     const int *eid;
     const int *n2e_off;
     const int *n2e;
-    int ned;
     const int4 *edges;
     const int *einfo;
+    const MemberDev *mem;
+    const int *eblk_mem;
+    const int *dblk_mem;
 };
 
+// one per member
 struct Ctrl {
     double t;                 // time at the start of the next step
     double dt;                // dt of the next step
@@ -90,7 +110,7 @@ struct Ctrl {
     unsigned long long maxlam[2];   // stage max |lambda| as raw bits (values >= 0)
     // earliest error in the reference's execution order (atomic min):
     // step << 34 | phase << 32 | id << 3 | code, phase = 2*stage + (0 flux, 1 PDop);
-    // all ones when no error has been raised
+    // id is member-local; all ones when no error has been raised
     unsigned long long err_key;
     long long step;                 // steps started (k_step_begin / k_step_mark)
     // max |lambda| is the reference's left fold of max(a,b)=(a>b?a:b) over the
@@ -117,15 +137,16 @@ __host__ __device__ inline int err_code_of(unsigned long long key) { return key 
 __host__ __device__ inline int err_where_of(unsigned long long key) { return key == ~0ULL ? 0 : (int)((key >> 3) & 0x1fffffff); }
 
 struct PhysDev {
-    double g, h0, velzero, rain;
-    double cfl, min_edge, final_time;
-    double tide_datum, tide_amp, tide_period, tide_phase;
-    int wetdry, tide_code;
-    // per-block max |lambda| partials of the flux kernels, [2][maxparts] raw
-    // bits; reduced into Ctrl::maxlam by k_step_end / k_lam_reduce (one plain
-    // store per block instead of one same-address atomic per wavefront)
+    double g, h0, velzero;
+    int wetdry;
+    // per-workgroup max |lambda| partials of the flux kernels, [2][maxparts]
+    // raw bits; reduced per member into Ctrl::maxlam by k_step_end /
+    // k_lam_reduce (one plain store per workgroup, no same-address atomics)
     unsigned long long *lamparts;
     int maxparts;
 };
+
+// which workgroup space a flux launch wrote its partials in
+enum PartKind : int { PARTS_ELEM = 0, PARTS_EDGE = 1 };
 
 }  // namespace dgr

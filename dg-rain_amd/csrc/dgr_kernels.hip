@@ -24,18 +24,6 @@
 #include "dgr_device.h"
 #include "dgr_math.h"
 
-// Minimum waves per SIMD asked of the compiler for the stage kernels
-// (register budget 512/waves VGPRs); tuned by measurement.
-#ifndef DGR_STAGE_WAVES
-#define DGR_STAGE_WAVES 1
-#endif
-#ifndef DGR_FLUX_WAVES
-#define DGR_FLUX_WAVES 1
-#endif
-#ifndef DGR_RHS_WAVES
-#define DGR_RHS_WAVES 1
-#endif
-
 namespace dgr {
 
 #define DGR_MAX(a, b) (((a) > (b)) ? (a) : (b))
@@ -43,21 +31,21 @@ namespace dgr {
 
 __device__ __forceinline__ int isign(double a) { return a < 0 ? -1 : (a == 0 ? 0 : 1); }
 
-// record an error; the smallest key (earliest step, stage, phase, then
-// lowest edge/element id) wins, which is the error the reference's serial
-// loops would have stopped at (compute2DL.c:245-302, wetDry2D.c:142-146,233-237)
-// Global thread index with the blocks renumbered XCD by XCD.  The dispatcher
-// hands workgroup b to XCD b % 8; this bijection gives each XCD a contiguous
-// range of elements (or edges), so neighbour gathers and partial-line stores
-// of one region meet in one L2 instead of eight.
-__device__ __forceinline__ int xcd_gid()
+// Logical workgroup index: the dispatcher hands workgroup b to XCD b % 8;
+// this bijection gives each XCD a contiguous range of workgroups (elements or
+// edges), so neighbour gathers and partial-line stores of one region meet in
+// one L2 instead of eight.
+__device__ __forceinline__ int xcd_block()
 {
     const int nb = (int)gridDim.x, b = (int)blockIdx.x;
     const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
-    const int lb = (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
-    return lb * (int)blockDim.x + (int)threadIdx.x;
+    return (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
 }
 
+// record an error; the smallest key (earliest step, stage, phase, then
+// lowest member-local edge/element id) wins, which is the error the
+// reference's serial loops would have stopped at (compute2DL.c:245-302,
+// wetDry2D.c:142-146,233-237)
 __device__ __forceinline__ void raise_error(Ctrl *c, int code, int where, int phase)
 {
     const unsigned long long key = ((unsigned long long)c->step << 34) | ((unsigned long long)phase << 32) |
@@ -164,9 +152,7 @@ __device__ __forceinline__ void rt0(double M11, double M12, double M13, double M
 // ---------------------------------------------------------------------------
 // geometry_of: every geometry-only quantity of Compute2DInnerProducts and of
 // the edge loop of compute2DL for element e, with the reference's
-// expressions, handed to put(GeoIdx, value).  k_geometry stores them as
-// planes once; the in-register variants call it inside the stage kernels and
-// let dead-code elimination keep only what they use.
+// expressions, handed to put(GeoIdx, value).
 // ---------------------------------------------------------------------------
 template <typename Put>
 __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, Put &&put)
@@ -264,23 +250,11 @@ __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, P
     }
 }
 
-__global__ __launch_bounds__(256) void k_geometry(MeshDev m, double g, double *__restrict__ geo)
-{
-    const int e = blockIdx.x * blockDim.x + threadIdx.x;
-    if (e >= m.ne) return;
-    const int Pt = m.pitch;
-    geometry_of(m, e, g, [&](int i, double v) { geo[(size_t)i * Pt + e] = v; });
-}
-
-// geometry accessors: computed into registers, or read from the planes
+// geometry computed into registers; dead-code elimination keeps only what
+// the caller reads
 struct GeoRegs {
     double v[GX_COUNT];
     __device__ __forceinline__ double operator()(int i) const { return v[i]; }
-};
-struct GeoPlanes {
-    const double *geo;
-    int pitch, e;
-    __device__ __forceinline__ double operator()(int i) const { return geo[(size_t)i * pitch + e]; }
 };
 
 // ---------------------------------------------------------------------------
@@ -429,7 +403,8 @@ struct LamOps {
 // (compute2DL.c:17-331), negated on the el_ex side, and the max-|lambda|
 // operands this element owns.
 template <bool WETDRY, typename Geo>
-__device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &ph, const double *__restrict__ win,
+__device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &ph, const MemberDev &M,
+                                               const double *__restrict__ win,
                                                const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                const int *__restrict__ wd, const Ctrl *ctrl, int e,
                                                const Geo &G, const double W[9], int wd_self, double Fh[9],
@@ -466,7 +441,7 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
         }
         if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
             const int code = (inf >> EI_CODE_SHIFT) & 0xff;
-            const bool tide = ph.tide_code != 0 && code == ph.tide_code;
+            const bool tide = M.tide_code != 0 && code == M.tide_code;
             zex = tide ? ctrl->tide : bzeta[ed];
             double qnex = tide ? 1000000.0 : bqn[ed];
             if (zex < 0) zex = zin;
@@ -479,7 +454,7 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
             reflect(qxin, qyin, nx, ny, dn, qxex, qyex);
         }
         double F[3];
-        const long long p0 = 2LL * ed;
+        const long long p0 = 2LL * (ed - M.ed_off);   // member-local operand position
         if (WETDRY) {
             const int wd_in = is_in ? wd_self : wd[nbk];
             const int wd_ex = boundary ? wd_self : (is_in ? wd[nbk] : wd_self);
@@ -508,13 +483,19 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
                 ops.pos[2 * k] = p0;
                 // a NaN here comes from the state, so the reference's first check
                 // (the g-flux, compute2DL.c:245) would already have stopped
-                if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed < err_edge) { err_code = 3; err_edge = ed; }
+                if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed - M.ed_off < err_edge) {
+                    err_code = 3;
+                    err_edge = ed - M.ed_off;
+                }
                 if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
             }
         } else {                                                     // :295-315
             ops.v[2 * k] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
             ops.pos[2 * k] = p0;
-            if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed < err_edge) { err_code = 3; err_edge = ed; }
+            if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed - M.ed_off < err_edge) {
+                err_code = 3;
+                err_edge = ed - M.ed_off;
+            }
             if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
         }
         Fh[k] = F[0];
@@ -541,7 +522,7 @@ __device__ __forceinline__ double lam_fold(const LamOps &ops, Ctrl *ctrl, int sl
 
 // block max of the thread maxima, one plain store per block into this
 // stage's partials (reduced by block_reduce_parts); blockDim.x is 256
-__device__ __forceinline__ void lam_commit(double lam, const PhysDev &ph, int slot)
+__device__ __forceinline__ void lam_commit(double lam, const PhysDev &ph, int slot, int lb)
 {
     __shared__ unsigned long long red[4];
     const unsigned long long wm = wave_max((unsigned long long)__double_as_longlong(lam));
@@ -551,7 +532,7 @@ __device__ __forceinline__ void lam_commit(double lam, const PhysDev &ph, int sl
         unsigned long long b = red[0];
 #pragma unroll
         for (int i = 1; i < 4; ++i) b = (red[i] > b) ? red[i] : b;
-        ph.lamparts[(size_t)slot * ph.maxparts + blockIdx.x] = b;
+        ph.lamparts[(size_t)slot * ph.maxparts + lb] = b;
     }
 }
 
@@ -594,66 +575,79 @@ __device__ __forceinline__ void rk_update(const MeshDev &m, int e, const double 
 }
 
 // ---------------------------------------------------------------------------
+// Workgroup -> member.  Element kernels run on pitch/256 workgroups, edge
+// kernels on ned_pad/256; every workgroup lies inside one member's range.
+// ---------------------------------------------------------------------------
+struct ElemLane {
+    int lb, mid, e;
+    bool valid;
+};
+__device__ __forceinline__ ElemLane elem_lane(const MeshDev &m)
+{
+    ElemLane L;
+    L.lb = xcd_block();
+    L.mid = m.eblk_mem[L.lb];
+    L.e = L.lb * 256 + (int)threadIdx.x;
+    const MemberDev &M = m.mem[L.mid];
+    L.valid = L.e < M.el_off + M.ne;
+    return L;
+}
+
+// ---------------------------------------------------------------------------
 // k_stage: flux gather + RHS + RK update of one element per thread (fused).
 //   MODE 0: RHS only (compute2DL), written to dbg_rhs/dbg_fhat
 //   MODE 1: w_out = w_in + RHS*dt                       (time_evolution.c:72-88)
 //   MODE 2: w_out = (w_in + w_base)*0.5 + RHS*(dt/2)    (time_evolution.c:176-178,208-217)
 //           w_out may alias w_base (each element reads its own slot first).
 //   MODE 3: max-|lambda| fold repair after a NaN operand (see Ctrl); returns
-//           at once when the stage saw no NaN operand.
-// GEOLOAD: geometry from the planes of k_geometry, or computed in registers.
+//           at once for members whose stage saw no NaN operand.
 // ---------------------------------------------------------------------------
-template <int MODE, bool WETDRY, bool GEOLOAD>
-__global__ __launch_bounds__(256, DGR_STAGE_WAVES) void k_stage(MeshDev m, PhysDev ph, const double *__restrict__ win,
+template <int MODE, bool WETDRY>
+__global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const double *__restrict__ win,
                                                double *wout, const double *wbase,
                                                const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                const int *__restrict__ wd, double *__restrict__ means,
-                                               Ctrl *ctrl, int slot, double *__restrict__ dbg_rhs,
+                                               Ctrl *ctrls, int slot, double *__restrict__ dbg_rhs,
                                                double *__restrict__ dbg_fhat)
 {
+    const ElemLane L = elem_lane(m);
+    const MemberDev &M = m.mem[L.mid];
+    Ctrl *ctrl = ctrls + L.mid;
     long long nan_after = -1;
     if (MODE == 3) {
         nan_after = ctrl->nan_pos[slot];
-        if (nan_after < 0) return;
+        if (nan_after < 0) return;   // uniform over the workgroup
     }
-    const int e = xcd_gid();
-    const bool valid = e < m.ne;
+    const int e = L.e;
     double lam = 0.0;
     bool any = false;
-    if (valid) {
+    if (L.valid) {
         const int P = m.pitch;
         double W[9];
 #pragma unroll
         for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
         const int wd_self = wd[e];
-        auto body = [&](const auto &G) {
-            double Fh[9];
-            LamOps ops;
-            int ecode, eedge;
-            element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, G, W, wd_self, Fh, ops, ecode, eedge);
-            if (MODE == 3) {
+        GeoRegs G;
+        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+        double Fh[9];
+        LamOps ops;
+        int ecode, eedge;
+        element_fluxes<WETDRY>(m, ph, M, win, bzeta, bqn, wd, ctrl, e, G, W, wd_self, Fh, ops, ecode, eedge);
+        if (MODE == 3) {
 #pragma unroll
-                for (int i = 0; i < 6; ++i)
-                    if (ops.pos[i] >= 0 && ops.pos[i] > nan_after) { lam = (ops.v[i] > lam) ? ops.v[i] : lam; any = true; }
-                return;
-            }
+            for (int i = 0; i < 6; ++i)
+                if (ops.pos[i] >= 0 && ops.pos[i] > nan_after) { lam = (ops.v[i] > lam) ? ops.v[i] : lam; any = true; }
+        } else {
             lam = lam_fold(ops, ctrl, slot);
             if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
             double R[9];
-            element_rhs(G, W, Fh, wd_self, ph.g, ph.rain, R);
+            element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R);
             if (MODE == 0) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f) { dbg_rhs[f * P + e] = R[f]; dbg_fhat[f * P + e] = Fh[f]; }
             } else {
                 rk_update<MODE>(m, e, W, R, wout, wbase, means, ctrl);
             }
-        };
-        if (GEOLOAD) {
-            body(GeoPlanes{m.geo, P, e});
-        } else {
-            GeoRegs G;
-            geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
-            body(G);
         }
     }
     if (MODE == 3) {
@@ -664,45 +658,41 @@ __global__ __launch_bounds__(256, DGR_STAGE_WAVES) void k_stage(MeshDev m, PhysD
             atomicOr(&ctrl->any_after[slot], 1);
         }
     } else {
-        lam_commit(lam, ph, slot);
+        lam_commit(lam, ph, slot, L.lb);
     }
 }
 
 // Split variant, part 1: the three edge fluxes of each element (the edge loop
 // of compute2DL in the element-centric orientation) into the Fhat planes.
-template <bool WETDRY, bool GEOLOAD>
-__global__ __launch_bounds__(256, DGR_FLUX_WAVES) void k_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
+template <bool WETDRY>
+__global__ __launch_bounds__(256) void k_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
                                               const double *__restrict__ bzeta, const double *__restrict__ bqn,
-                                              const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrl,
+                                              const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrls,
                                               int slot)
 {
-    const int e = xcd_gid();
+    const ElemLane L = elem_lane(m);
+    const MemberDev &M = m.mem[L.mid];
+    Ctrl *ctrl = ctrls + L.mid;
+    const int e = L.e;
     double lam = 0.0;
-    if (e < m.ne) {
+    if (L.valid) {
         const int P = m.pitch;
         double W[9];
 #pragma unroll
         for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
         const int wd_self = wd[e];
-        auto body = [&](const auto &G) {
-            double Fh[9];
-            LamOps ops;
-            int ecode, eedge;
-            element_fluxes<WETDRY>(m, ph, win, bzeta, bqn, wd, ctrl, e, G, W, wd_self, Fh, ops, ecode, eedge);
-            lam = lam_fold(ops, ctrl, slot);
-            if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
+        GeoRegs G;
+        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+        double Fh[9];
+        LamOps ops;
+        int ecode, eedge;
+        element_fluxes<WETDRY>(m, ph, M, win, bzeta, bqn, wd, ctrl, e, G, W, wd_self, Fh, ops, ecode, eedge);
+        lam = lam_fold(ops, ctrl, slot);
+        if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
 #pragma unroll
-            for (int f = 0; f < 9; ++f) fhat[f * P + e] = Fh[f];
-        };
-        if (GEOLOAD) {
-            body(GeoPlanes{m.geo, P, e});
-        } else {
-            GeoRegs G;
-            geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
-            body(G);
-        }
+        for (int f = 0; f < 9; ++f) fhat[f * P + e] = Fh[f];
     }
-    lam_commit(lam, ph, slot);
+    lam_commit(lam, ph, slot, L.lb);
 }
 
 // Edge variant of part 1: one thread per edge, the edge loop of compute2DL
@@ -715,12 +705,16 @@ __global__ __launch_bounds__(256, DGR_FLUX_WAVES) void k_flux(MeshDev m, PhysDev
 template <bool WETDRY>
 __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
                                                    const double *__restrict__ bzeta, const double *__restrict__ bqn,
-                                                   const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrl,
+                                                   const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrls,
                                                    int slot)
 {
-    const int ed = xcd_gid();
+    const int lb = xcd_block();
+    const int mid = m.dblk_mem[lb];
+    const MemberDev &M = m.mem[mid];
+    Ctrl *ctrl = ctrls + mid;
+    const int ed = lb * 256 + (int)threadIdx.x;
     double lam = 0.0;
-    if (ed < m.ned) {
+    if (ed < M.ed_off + M.ned) {
         const int P = m.pitch;
         const double g = ph.g;
         const int4 E = m.edges[ed];
@@ -744,7 +738,7 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
         else { zex = win[kex * P + el_ex]; qxex = win[(3 + kex) * P + el_ex]; qyex = win[(6 + kex) * P + el_ex]; }
         if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
             const int code = (inf >> EI_CODE_SHIFT) & 0xff;
-            const bool tide = ph.tide_code != 0 && code == ph.tide_code;
+            const bool tide = M.tide_code != 0 && code == M.tide_code;
             zex = tide ? ctrl->tide : bzeta[ed];
             double qnex = tide ? 1000000.0 : bqn[ed];
             if (zex < 0) zex = zin;
@@ -758,7 +752,8 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
         LamOps ops;
 #pragma unroll
         for (int i = 0; i < 6; ++i) { ops.v[i] = 0.0; ops.pos[i] = -1; }
-        const long long p0 = 2LL * ed;
+        const int led = ed - M.ed_off;                 // member-local edge id
+        const long long p0 = 2LL * led;
         double Fi[3], Fx[3];
         bool nan_flux = false;
         if (WETDRY) {
@@ -791,7 +786,7 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             Fx[0] = Fi[0]; Fx[1] = Fi[1]; Fx[2] = Fi[2];
             nan_flux = isnan(Fi[0]) || isnan(Fi[1]) || isnan(Fi[2]);
         }
-        if (nan_flux) raise_error(ctrl, 3, ed, 2 * slot);
+        if (nan_flux) raise_error(ctrl, 3, led, 2 * slot);
         lam = lam_fold(ops, ctrl, slot);
 #pragma unroll
         for (int f = 0; f < 3; ++f) fhat[(f * 3 + kin) * P + el_in] = Fi[f];
@@ -800,19 +795,22 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             for (int f = 0; f < 3; ++f) fhat[(f * 3 + kex) * P + el_ex] = -Fx[f];
         }
     }
-    lam_commit(lam, ph, slot);
+    lam_commit(lam, ph, slot, lb);
 }
 
-// Split variant, part 2: inner products, RHS and RK update from the Fhat planes.
+// Split variants, part 2: inner products, RHS and RK update from the Fhat planes.
 //   MODE 0: RHS only into dbg_rhs; MODE 1/2 as in k_stage.
-template <int MODE, bool GEOLOAD>
-__global__ __launch_bounds__(256, DGR_RHS_WAVES) void k_rhs(MeshDev m, PhysDev ph, const double *__restrict__ win, double *wout,
+template <int MODE>
+__global__ __launch_bounds__(256) void k_rhs(MeshDev m, PhysDev ph, const double *__restrict__ win, double *wout,
                                              const double *wbase, const double *__restrict__ fhat,
                                              const int *__restrict__ wd, double *__restrict__ means,
-                                             const Ctrl *ctrl, double *__restrict__ dbg_rhs)
+                                             const Ctrl *ctrls, double *__restrict__ dbg_rhs)
 {
-    const int e = xcd_gid();
-    if (e >= m.ne) return;
+    const ElemLane L = elem_lane(m);
+    if (!L.valid) return;
+    const MemberDev &M = m.mem[L.mid];
+    const Ctrl *ctrl = ctrls + L.mid;
+    const int e = L.e;
     const int P = m.pitch;
     double W[9], Fh[9];
 #pragma unroll
@@ -820,30 +818,24 @@ __global__ __launch_bounds__(256, DGR_RHS_WAVES) void k_rhs(MeshDev m, PhysDev p
 #pragma unroll
     for (int f = 0; f < 9; ++f) Fh[f] = fhat[f * P + e];
     const int wd_self = wd[e];
-    auto body = [&](const auto &G) {
-        double R[9];
-        element_rhs(G, W, Fh, wd_self, ph.g, ph.rain, R);
-        if (MODE == 0) {
+    GeoRegs G;
+    geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+    double R[9];
+    element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R);
+    if (MODE == 0) {
 #pragma unroll
-            for (int f = 0; f < 9; ++f) dbg_rhs[f * P + e] = R[f];
-        } else {
-            rk_update<MODE>(m, e, W, R, wout, wbase, means, ctrl);
-        }
-    };
-    if (GEOLOAD) {
-        body(GeoPlanes{m.geo, P, e});
+        for (int f = 0; f < 9; ++f) dbg_rhs[f * P + e] = R[f];
     } else {
-        GeoRegs G;
-        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
-        body(G);
+        rk_update<MODE>(m, e, W, R, wout, wbase, means, ctrl);
     }
 }
 
 // element means of the current state (standalone limiter calls)
-__global__ __launch_bounds__(256) void k_means(int ne, int P, const double *__restrict__ w, double *__restrict__ means)
+__global__ __launch_bounds__(256) void k_means(MeshDev m, const double *__restrict__ w, double *__restrict__ means)
 {
-    const int e = xcd_gid();
-    if (e >= ne) return;
+    const ElemLane L = elem_lane(m);
+    if (!L.valid) return;
+    const int P = m.pitch, e = L.e;
 #pragma unroll
     for (int f = 0; f < 3; ++f) means[f * P + e] = (w[(3 * f) * P + e] + w[(3 * f + 1) * P + e] + w[(3 * f + 2) * P + e]) / 3;
 }
@@ -851,7 +843,7 @@ __global__ __launch_bounds__(256) void k_means(int ne, int P, const double *__re
 // node min/max of element means, ascending element order (SlopeLimiter2D.c:106-153)
 __global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means, double *__restrict__ nodemm)
 {
-    const int i = xcd_gid();
+    const int i = xcd_block() * 256 + (int)threadIdx.x;
     if (i >= m.nn) return;
     double mx0 = -99999, mx1 = -99999, mx2 = -99999, mn0 = 99999, mn1 = 99999, mn2 = 99999;
     const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
@@ -951,10 +943,11 @@ __device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy
 // limiter element pass (SlopeLimiter2D.c:157-343), optionally followed by PDop2D
 template <bool PDOP>
 __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w, const double *__restrict__ nodemm,
-                                               unsigned char *__restrict__ modify_out, Ctrl *ctrl, int stage)
+                                               unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage)
 {
-    const int e = xcd_gid();
-    if (e >= m.ne) return;
+    const ElemLane L = elem_lane(m);
+    if (!L.valid) return;
+    const int e = L.e;
     const int P = m.pitch, nn = m.nn;
     const double H0 = ph.h0;
     int nd[3];
@@ -1008,7 +1001,7 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w,
     }
     if (PDOP) {
         const int err = pdop_element(&S[0], &S[3], &S[6], zv, H0, ph.velzero);
-        if (err) raise_error(ctrl, err, e, 2 * stage + 1);
+        if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 2 * stage + 1);
     }
 #pragma unroll
     for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
@@ -1016,10 +1009,11 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w,
 }
 
 // standalone PDop2D
-__global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, Ctrl *ctrl)
+__global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, Ctrl *ctrls)
 {
-    const int e = xcd_gid();
-    if (e >= m.ne) return;
+    const ElemLane L = elem_lane(m);
+    if (!L.valid) return;
+    const int e = L.e;
     const int P = m.pitch;
     double zv[3], S[9];
 #pragma unroll
@@ -1027,7 +1021,7 @@ __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, 
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = w[f * P + e];
     const int err = pdop_element(&S[0], &S[3], &S[6], zv, ph.h0, ph.velzero);
-    if (err) raise_error(ctrl, err, e, 1);
+    if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 1);
 #pragma unroll
     for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
 }
@@ -1035,8 +1029,9 @@ __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, 
 // wetDryStatus2D (wetDry2D.c:16-60)
 __global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const double *__restrict__ w, int *__restrict__ wd)
 {
-    const int e = xcd_gid();
-    if (e >= m.ne) return;
+    const ElemLane L = elem_lane(m);
+    if (!L.valid) return;
+    const int e = L.e;
     const int P = m.pitch;
     const double H0 = ph.h0;
     const double a = w[e], b = w[P + e], c = w[2 * P + e];
@@ -1060,14 +1055,8 @@ __global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const 
     wd[e] = (old == -1) ? t1 * t2 : old * t1 + (1 - old) * t1 * t2;
 }
 
-// per-step bookkeeping (time_evolution.c:311-402): begin evaluates the tide at
-// the step's start time; end advances t and sets the next dt from stage 2's
-// max |lambda|.
-__global__ void k_step_begin(PhysDev ph, Ctrl *c)
+__device__ __forceinline__ void clear_fold(Ctrl *c)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    if (ph.tide_code)
-        c->tide = ph.tide_datum + ph.tide_amp * dgr_sin(6.283185307179586 * c->t / ph.tide_period + ph.tide_phase);
     c->maxlam[0] = 0ULL;
     c->maxlam[1] = 0ULL;
     c->nan_pos[0] = -1;
@@ -1076,56 +1065,73 @@ __global__ void k_step_begin(PhysDev ph, Ctrl *c)
     c->after_bits[1] = 0ULL;
     c->any_after[0] = 0;
     c->any_after[1] = 0;
+}
+
+// per-step bookkeeping (time_evolution.c:311-402), one thread per member:
+// begin evaluates the tide at the step's start time; end advances t and sets
+// the next dt from stage 2's max |lambda|.
+__global__ __launch_bounds__(64) void k_step_begin(MeshDev m, Ctrl *ctrls)
+{
+    const int mid = blockIdx.x * 64 + threadIdx.x;
+    if (mid >= m.nmem) return;
+    const MemberDev &M = m.mem[mid];
+    Ctrl *c = ctrls + mid;
+    if (M.tide_code)
+        c->tide = M.tide_datum + M.tide_amp * dgr_sin(6.283185307179586 * c->t / M.tide_period + M.tide_phase);
+    clear_fold(c);
     c->step += 1;
 }
 
 // oneTimeStep2D with a host dt: no tide update, only the step bookkeeping
-__global__ void k_step_mark(Ctrl *c, double dt)
+__global__ __launch_bounds__(64) void k_step_mark(MeshDev m, Ctrl *ctrls, double dt)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const int mid = blockIdx.x * 64 + threadIdx.x;
+    if (mid >= m.nmem) return;
+    Ctrl *c = ctrls + mid;
     c->dt = dt;
-    c->maxlam[0] = 0ULL;
-    c->maxlam[1] = 0ULL;
-    c->nan_pos[0] = -1;
-    c->nan_pos[1] = -1;
-    c->after_bits[0] = 0ULL;
-    c->after_bits[1] = 0ULL;
-    c->any_after[0] = 0;
-    c->any_after[1] = 0;
+    clear_fold(c);
     c->step += 1;
 }
 
-__global__ __launch_bounds__(256) void k_step_end(PhysDev ph, Ctrl *c, int nparts)
+// the partials one member's workgroups wrote for a slot
+__device__ __forceinline__ unsigned long long member_lam(const MeshDev &m, const PhysDev &ph, int mid, int slot,
+                                                          int kind)
 {
-    const unsigned long long b = block_reduce_parts(ph.lamparts + ph.maxparts, nparts);
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    const MemberDev &M = m.mem[mid];
+    const int b0 = kind == PARTS_EDGE ? M.dblk0 : M.eblk0;
+    const int nb = kind == PARTS_EDGE ? M.dblks : M.eblks;
+    return block_reduce_parts(ph.lamparts + (size_t)slot * ph.maxparts + b0, nb);
+}
+
+// one workgroup per member
+__global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind)
+{
+    const int mid = blockIdx.x;
+    const unsigned long long b = member_lam(m, ph, mid, 1, kind);
+    if (threadIdx.x != 0) return;
+    const MemberDev &M = m.mem[mid];
+    Ctrl *c = ctrls + mid;
     c->maxlam[1] = b;
     const double time = c->t + c->dt;
     const double ml = maxlam_of(*c, 1);
-    double h = ph.cfl * ph.min_edge / ml;
-    if ((time + h) > ph.final_time) h = ph.final_time - time;
+    double h = M.cfl * M.min_edge / ml;
+    if ((time + h) > M.final_time) h = M.final_time - time;
     c->t = time;
     c->dt = h;
 }
 
-// Ctrl::maxlam[slot] from the partials of the last flux launch of that slot
-__global__ __launch_bounds__(256) void k_lam_reduce(PhysDev ph, Ctrl *c, int slot, int nparts)
+// Ctrl::maxlam[slot] of every member from the partials of the last flux launch
+__global__ __launch_bounds__(256) void k_lam_reduce(MeshDev m, PhysDev ph, Ctrl *ctrls, int slot, int kind)
 {
-    const unsigned long long b = block_reduce_parts(ph.lamparts + (size_t)slot * ph.maxparts, nparts);
-    if (threadIdx.x == 0) c->maxlam[slot] = b;
+    const int mid = blockIdx.x;
+    const unsigned long long b = member_lam(m, ph, mid, slot, kind);
+    if (threadIdx.x == 0) ctrls[mid].maxlam[slot] = b;
 }
 
-__global__ void k_reset_maxlam(Ctrl *c)
+__global__ __launch_bounds__(64) void k_reset_maxlam(MeshDev m, Ctrl *ctrls)
 {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
-    c->maxlam[0] = 0ULL;
-    c->maxlam[1] = 0ULL;
-    c->nan_pos[0] = -1;
-    c->nan_pos[1] = -1;
-    c->after_bits[0] = 0ULL;
-    c->after_bits[1] = 0ULL;
-    c->any_after[0] = 0;
-    c->any_after[1] = 0;
+    const int mid = blockIdx.x * 64 + threadIdx.x;
+    if (mid < m.nmem) clear_fold(ctrls + mid);
 }
 
 __global__ void k_math_selftest(int n, const double *x, const double *y, double *out)
@@ -1144,56 +1150,52 @@ __global__ void k_math_selftest(int n, const double *x, const double *y, double 
 // launchers
 // ---------------------------------------------------------------------------
 static inline dim3 grid_for(int n) { return dim3((unsigned)((n + 255) / 256)); }
+static inline dim3 member_grid(const MeshDev &m) { return dim3((unsigned)((m.nmem + 63) / 64)); }
 
-// variant: bit 0 = geometry planes (else in registers), bit 1 = split flux/RHS
-// kernels with element-centric fluxes, bit 2 = split with edge-centric fluxes
-// Returns the number of max-|lambda| partials the launch wrote (0 for MODE 3).
+// variant 0: fused k_stage; 1: k_flux (element-centric) + k_rhs;
+// 2: k_edge_flux (edge-centric) + k_rhs.  MODE 3 always runs k_stage.
+// Returns the workgroup space the launch wrote its max-|lambda| partials in.
 int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
-                  double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
-                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
-                  hipStream_t s)
+                 double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
+                 double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
+                 hipStream_t s)
 {
-    const dim3 g = grid_for(m.ne), b(256);
-    const bool gl = (variant & 1) != 0;
-    const bool edge = (variant & 4) != 0 && mode != 3;
-    const bool split = ((variant & 2) != 0 || edge) && mode != 3;
-#define DGR_LS(MD, WDF, GL) \
-    hipLaunchKernelGGL((k_stage<MD, WDF, GL>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
-#define DGR_LS_W(WDF, GL) \
-    do { if (mode == 0) DGR_LS(0, WDF, GL); else if (mode == 1) DGR_LS(1, WDF, GL); \
-         else if (mode == 2) DGR_LS(2, WDF, GL); else DGR_LS(3, WDF, GL); } while (0)
-#define DGR_LF(WDF, GL) \
-    hipLaunchKernelGGL((k_flux<WDF, GL>), g, b, 0, s, m, ph, win, bzeta, bqn, wd, mode == 0 ? dbg_fhat : fhat, ctrl, slot)
-#define DGR_LR(MD, GL) \
-    hipLaunchKernelGGL((k_rhs<MD, GL>), g, b, 0, s, m, ph, win, wout, wbase, mode == 0 ? dbg_fhat : fhat, wd, means, ctrl, dbg_rhs)
-    if (!split) {
-        if (wetdry) { if (gl) DGR_LS_W(true, true); else DGR_LS_W(true, false); }
-        else { if (gl) DGR_LS_W(false, true); else DGR_LS_W(false, false); }
-        return mode == 3 ? 0 : (int)g.x;
+    const dim3 g = grid_for(m.pitch), b(256);
+#define DGR_LS(MD, WDF) \
+    hipLaunchKernelGGL((k_stage<MD, WDF>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
+#define DGR_LR(MD) \
+    hipLaunchKernelGGL((k_rhs<MD>), g, b, 0, s, m, ph, win, wout, wbase, mode == 0 ? dbg_fhat : fhat, wd, means, ctrl, dbg_rhs)
+    if (variant == 0 || mode == 3) {
+        if (wetdry) {
+            if (mode == 0) DGR_LS(0, true); else if (mode == 1) DGR_LS(1, true);
+            else if (mode == 2) DGR_LS(2, true); else DGR_LS(3, true);
+        } else {
+            if (mode == 0) DGR_LS(0, false); else if (mode == 1) DGR_LS(1, false);
+            else if (mode == 2) DGR_LS(2, false); else DGR_LS(3, false);
+        }
+        return PARTS_ELEM;
     }
-    if (edge) {
-        double *fh = mode == 0 ? dbg_fhat : fhat;
-        if (wetdry) hipLaunchKernelGGL(k_edge_flux<true>, grid_for(m.ned), b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
-        else hipLaunchKernelGGL(k_edge_flux<false>, grid_for(m.ned), b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
-    } else if (wetdry) { if (gl) DGR_LF(true, true); else DGR_LF(true, false); }
-    else { if (gl) DGR_LF(false, true); else DGR_LF(false, false); }
-    if (gl) { if (mode == 0) DGR_LR(0, true); else if (mode == 1) DGR_LR(1, true); else DGR_LR(2, true); }
-    else { if (mode == 0) DGR_LR(0, false); else if (mode == 1) DGR_LR(1, false); else DGR_LR(2, false); }
-    return edge ? (int)grid_for(m.ned).x : (int)g.x;
+    double *fh = mode == 0 ? dbg_fhat : fhat;
+    int kind;
+    if (variant == 1) {
+        if (wetdry) hipLaunchKernelGGL(k_flux<true>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
+        else hipLaunchKernelGGL(k_flux<false>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
+        kind = PARTS_ELEM;
+    } else {
+        const dim3 ge = grid_for(m.ned_pad);
+        if (wetdry) hipLaunchKernelGGL(k_edge_flux<true>, ge, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
+        else hipLaunchKernelGGL(k_edge_flux<false>, ge, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
+        kind = PARTS_EDGE;
+    }
+    if (mode == 0) DGR_LR(0); else if (mode == 1) DGR_LR(1); else DGR_LR(2);
 #undef DGR_LS
-#undef DGR_LS_W
-#undef DGR_LF
 #undef DGR_LR
-}
-
-void launch_geometry(const MeshDev &m, double g, double *geo, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_geometry, grid_for(m.ne), dim3(256), 0, s, m, g, geo);
+    return kind;
 }
 
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_means, grid_for(m.ne), dim3(256), 0, s, m.ne, m.pitch, w, means);
+    hipLaunchKernelGGL(k_means, grid_for(m.pitch), dim3(256), 0, s, m, w, means);
 }
 
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s)
@@ -1204,43 +1206,43 @@ void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, h
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
                   unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s)
 {
-    if (pdop) hipLaunchKernelGGL(k_limit<true>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
-    else hipLaunchKernelGGL(k_limit<false>, grid_for(m.ne), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
+    if (pdop) hipLaunchKernelGGL(k_limit<true>, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
+    else hipLaunchKernelGGL(k_limit<false>, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
 }
 
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pdop, grid_for(m.ne), dim3(256), 0, s, m, ph, w, ctrl);
+    hipLaunchKernelGGL(k_pdop, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, ctrl);
 }
 
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_wd_status, grid_for(m.ne), dim3(256), 0, s, m, ph, w, wd);
+    hipLaunchKernelGGL(k_wd_status, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, wd);
 }
 
-void launch_step_begin(const PhysDev &ph, Ctrl *c, hipStream_t s)
+void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(64), 0, s, ph, c);
+    hipLaunchKernelGGL(k_step_begin, member_grid(m), dim3(64), 0, s, m, c);
 }
 
-void launch_step_end(const PhysDev &ph, Ctrl *c, int nparts, hipStream_t s)
+void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_end, dim3(1), dim3(256), 0, s, ph, c, nparts);
+    hipLaunchKernelGGL(k_step_end, dim3((unsigned)m.nmem), dim3(256), 0, s, m, ph, c, kind);
 }
 
-void launch_lam_reduce(const PhysDev &ph, Ctrl *c, int slot, int nparts, hipStream_t s)
+void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_lam_reduce, dim3(1), dim3(256), 0, s, ph, c, slot, nparts);
+    hipLaunchKernelGGL(k_lam_reduce, dim3((unsigned)m.nmem), dim3(256), 0, s, m, ph, c, slot, kind);
 }
 
-void launch_step_mark(Ctrl *c, double dt, hipStream_t s)
+void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_mark, dim3(1), dim3(64), 0, s, c, dt);
+    hipLaunchKernelGGL(k_step_mark, member_grid(m), dim3(64), 0, s, m, c, dt);
 }
 
-void launch_reset_maxlam(Ctrl *c, hipStream_t s)
+void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_reset_maxlam, dim3(1), dim3(64), 0, s, c);
+    hipLaunchKernelGGL(k_reset_maxlam, member_grid(m), dim3(64), 0, s, m, c);
 }
 
 void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s)

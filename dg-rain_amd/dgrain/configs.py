@@ -103,6 +103,27 @@ def coastal(N=512, L=1.0e4, nsteps=2000, seed=2, nbumps=32, tide_amp=1.5, rain_m
     return Workload("coastal_%s_%dx%d" % (start, N, N), mesh, zeta, zero, zero.copy(), params, nsteps)
 
 
+def ensemble(n=64, seed=3, nmin=64, nmax=512, nsteps=500, L_per_cell=1.0e4 / 512):
+    """Batched ensemble of `n` coastal members of uneven sizes: N x N grids
+    with N = floor(exp(U(ln nmin, ln nmax))) from `seed`; per member a tide
+    amplitude U(0.5, 2.5) m, uniform rain U(0, 50) mm/h and its own seeded
+    bumps and Manning n field.  The cell size is that of the 512 x 512 coastal
+    mesh, so every member has the same minimum edge.  Returns the workloads."""
+    rng = np.random.default_rng(seed)
+    Ns = np.floor(np.exp(rng.uniform(np.log(nmin), np.log(nmax), n))).astype(int)
+    amps = rng.uniform(0.5, 2.5, n)
+    rains = rng.uniform(0.0, 50.0, n)
+    seeds = rng.integers(0, 2**31 - 1, n)
+    out = []
+    for i in range(n):
+        N = int(Ns[i])
+        wl = coastal(N=N, L=N * L_per_cell, nsteps=nsteps, seed=int(seeds[i]), tide_amp=float(amps[i]),
+                     rain_mm_h=float(rains[i]))
+        wl.name = "ensemble_member_%d_%dx%d" % (i, N, N)
+        out.append(wl)
+    return out
+
+
 def dry_bed_dam_break(N=16, L=10.0, nsteps=100):
     """Dam break onto a dry flat bed (the pattern of initialize.c:65-73):
     H = 0.005 m for x <= L/2, H0 elsewhere; wet/dry on, walls."""

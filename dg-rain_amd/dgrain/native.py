@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)                      # .../dg-rain_amd
 REPO_ROOT = os.path.dirname(PKG_ROOT)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdgrain.so")
-NUM_VARIANTS = 6   # DGR_NUM_VARIANTS in include/dgrain.h
+NUM_VARIANTS = 3   # DGR_NUM_VARIANTS in include/dgrain.h
 
 DGR_OK = 0
 STATUS = {0: "ok", 1: "invalid argument", 2: "inconsistent mesh", 3: "HIP runtime error",
@@ -89,6 +89,18 @@ def load():
         "dgr_get_variant": (C.c_int, [vp, ip]),
         "dgr_autotune": (C.c_int, [vp, C.c_int, ip, dp]),
         "dgr_set_final_time": (C.c_int, [vp, C.c_double]),
+        "dgr_set_graph": (C.c_int, [vp, C.c_int]),
+        "dgr_ensemble_create": (C.c_int, [C.c_int, C.POINTER(MeshView), C.POINTER(Params), C.POINTER(vp)]),
+        "dgr_num_members": (C.c_int, [vp, ip]),
+        "dgr_member_set_state": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "dgr_member_get_state": (C.c_int, [vp, C.c_int, dp, dp, dp]),
+        "dgr_member_set_wd": (C.c_int, [vp, C.c_int, ip]),
+        "dgr_member_get_wd": (C.c_int, [vp, C.c_int, ip]),
+        "dgr_member_get_modify": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_ubyte)]),
+        "dgr_member_set_boundary": (C.c_int, [vp, C.c_int, dp, dp]),
+        "dgr_member_set_time": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
+        "dgr_member_report": (C.c_int, [vp, C.c_int, C.POINTER(Report)]),
+        "dgr_member_layout": (C.c_int, [vp, C.c_int, ip, ip]),
         "dgr_profile_end": (C.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():
@@ -154,25 +166,17 @@ def build_topology(elto_vert, x, y, in1=None, in2=None, out=None):
     return e2v, e2e, bdry, mel.value
 
 
-class Solver:
-    """One libdgrain handle: the reference's global-state step as an object.
+def _mesh_view(mesh, keep):
+    arrs = [np.ascontiguousarray(a) for a in
+            (mesh.x, mesh.y, mesh.z, mesh.n, mesh.elto_vert, mesh.edg_to_vert, mesh.edg_to_els, mesh.bdry)]
+    keep.extend(arrs)
+    x, y, z, n, ev, e2v, e2e, bd = arrs
+    return MeshView(mesh.num_el, mesh.num_nodes, mesh.num_edges, _dp(x), _dp(y), _dp(z), _dp(n),
+                    _ip(ev), _ip(e2v), _ip(e2e), _ip(bd))
 
-    Method names follow the reference functions they replace
-    (compute2DL, SlopeLimiter, wetDryStatus2D, PDop2D, oneTimeStep2D)."""
 
-    def __init__(self, mesh, params=None, **kw):
-        lib = load()
-        self.lib = lib
-        self.mesh = mesh
-        self.params = params if params is not None else default_params(**kw)
-        self._keep = [np.ascontiguousarray(a) for a in
-                      (mesh.x, mesh.y, mesh.z, mesh.n, mesh.elto_vert, mesh.edg_to_vert, mesh.edg_to_els, mesh.bdry)]
-        x, y, z, n, ev, e2v, e2e, bd = self._keep
-        mv = MeshView(mesh.num_el, mesh.num_nodes, mesh.num_edges, _dp(x), _dp(y), _dp(z), _dp(n),
-                      _ip(ev), _ip(e2v), _ip(e2e), _ip(bd))
-        h = C.c_void_p()
-        check(lib.dgr_create(C.byref(mv), C.byref(self.params), C.byref(h)), "dgr_create")
-        self.h = h
+class _Handle:
+    """What single-instance and ensemble handles share."""
 
     def close(self):
         if getattr(self, "h", None):
@@ -184,6 +188,72 @@ class Solver:
             self.close()
         except Exception:
             pass
+
+    def run(self, nsteps):
+        """nsteps steps of every member; returns member 0's report."""
+        rep = Report()
+        rc = self.lib.dgr_run(self.h, int(nsteps), C.byref(rep))
+        check(rc, "dgr_run (error %d at %d)" % (rep.error_code, rep.error_where))
+        return rep
+
+    def sync(self):
+        check(self.lib.dgr_sync(self.h))
+
+    def set_variant(self, v):
+        check(self.lib.dgr_set_variant(self.h, int(v)))
+
+    def variant(self):
+        v = C.c_int(0)
+        check(self.lib.dgr_get_variant(self.h, C.byref(v)))
+        return v.value
+
+    def set_graph(self, on=True):
+        """Replay one captured hipGraph per step in run()."""
+        check(self.lib.dgr_set_graph(self.h, 1 if on else 0))
+
+    def autotune(self, steps=5):
+        """Pick the fastest kernel variant; returns (chosen, ms/step of each)."""
+        v = C.c_int(0)
+        ms = np.zeros(NUM_VARIANTS)
+        check(self.lib.dgr_autotune(self.h, int(steps), C.byref(v), _dp(ms)))
+        return v.value, [float(x) for x in ms]
+
+    def set_final_time(self, t):
+        check(self.lib.dgr_set_final_time(self.h, float(t)))
+
+    def profile_begin(self):
+        check(self.lib.dgr_profile_begin(self.h))
+
+    def profile_end(self):
+        """(summed ms, launches) of the flux/RHS/update kernels since profile_begin."""
+        ms = C.c_double(0)
+        n = C.c_int(0)
+        check(self.lib.dgr_profile_end(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def device_state(self):
+        p = C.POINTER(C.c_double)()
+        pitch = C.c_int(0)
+        check(self.lib.dgr_device_state(self.h, C.byref(p), C.byref(pitch)))
+        return C.cast(p, C.c_void_p).value, pitch.value
+
+
+class Solver(_Handle):
+    """One libdgrain handle: the reference's global-state step as an object.
+
+    Method names follow the reference functions they replace
+    (compute2DL, SlopeLimiter, wetDryStatus2D, PDop2D, oneTimeStep2D)."""
+
+    def __init__(self, mesh, params=None, **kw):
+        lib = load()
+        self.lib = lib
+        self.mesh = mesh
+        self.params = params if params is not None else default_params(**kw)
+        self._keep = []
+        mv = _mesh_view(mesh, self._keep)
+        h = C.c_void_p()
+        check(lib.dgr_create(C.byref(mv), C.byref(self.params), C.byref(h)), "dgr_create")
+        self.h = h
 
     # -- state -------------------------------------------------------------
     def set_state(self, zeta, qx, qy):
@@ -240,45 +310,72 @@ class Solver:
     def oneTimeStep2D(self, t, dt):
         check(self.lib.dgr_one_time_step(self.h, float(t), float(dt)), "dgr_one_time_step")
 
-    def run(self, nsteps):
+
+class Ensemble(_Handle):
+    """n independent members (own mesh, state, forcing, time and dt) advanced
+    together, one launch per kernel for all of them (dgr_ensemble_create)."""
+
+    def __init__(self, meshes, params):
+        if len(meshes) != len(params) or not meshes:
+            raise ValueError("one params object per mesh")
+        lib = load()
+        self.lib = lib
+        self.meshes = list(meshes)
+        self._keep = []
+        mvs = (MeshView * len(meshes))(*[_mesh_view(m, self._keep) for m in meshes])
+        prms = (Params * len(params))(*params)
+        self.params = list(params)
+        h = C.c_void_p()
+        check(lib.dgr_ensemble_create(len(meshes), mvs, prms, C.byref(h)), "dgr_ensemble_create")
+        self.h = h
+
+    def __len__(self):
+        return len(self.meshes)
+
+    def set_state(self, i, zeta, qx, qy):
+        a = [np.ascontiguousarray(v, dtype=np.float64) for v in (zeta, qx, qy)]
+        check(self.lib.dgr_member_set_state(self.h, int(i), *[_dp(v) for v in a]), "dgr_member_set_state")
+
+    def get_state(self, i):
+        ne3 = 3 * self.meshes[i].num_el
+        a = [np.empty(ne3) for _ in range(3)]
+        check(self.lib.dgr_member_get_state(self.h, int(i), *[_dp(v) for v in a]), "dgr_member_get_state")
+        return tuple(a)
+
+    def get_wd(self, i):
+        a = np.empty(self.meshes[i].num_el, np.int32)
+        check(self.lib.dgr_member_get_wd(self.h, int(i), _ip(a)))
+        return a
+
+    def get_modify(self, i, stage):
+        a = np.empty(self.meshes[i].num_el, np.uint8)
+        check(self.lib.dgr_member_get_modify(self.h, int(i), stage, a.ctypes.data_as(C.POINTER(C.c_ubyte))))
+        return a
+
+    def set_boundary(self, i, bzeta, bqn):
+        a = np.ascontiguousarray(bzeta, dtype=np.float64)
+        b = np.ascontiguousarray(bqn, dtype=np.float64)
+        check(self.lib.dgr_member_set_boundary(self.h, int(i), _dp(a), _dp(b)))
+
+    def report(self, i):
+        """member i's report; raises DgrError if it recorded an error"""
         rep = Report()
-        rc = self.lib.dgr_run(self.h, int(nsteps), C.byref(rep))
-        check(rc, "dgr_run (error %d at %d)" % (rep.error_code, rep.error_where))
+        rc = self.lib.dgr_member_report(self.h, int(i), C.byref(rep))
+        check(rc, "member %d (error %d at %d)" % (i, rep.error_code, rep.error_where))
         return rep
 
-    def sync(self):
-        check(self.lib.dgr_sync(self.h))
-
-    def set_variant(self, v):
-        check(self.lib.dgr_set_variant(self.h, int(v)))
-
-    def variant(self):
-        v = C.c_int(0)
-        check(self.lib.dgr_get_variant(self.h, C.byref(v)))
-        return v.value
-
-    def autotune(self, steps=5):
-        """Pick the fastest kernel variant; returns (chosen, ms/step of each)."""
-        v = C.c_int(0)
-        ms = np.zeros(NUM_VARIANTS)
-        check(self.lib.dgr_autotune(self.h, int(steps), C.byref(v), _dp(ms)))
-        return v.value, [float(x) for x in ms]
-
-    def profile_begin(self):
-        check(self.lib.dgr_profile_begin(self.h))
-
-    def profile_end(self):
-        """(summed ms, launches) of the flux/RHS/update kernel since profile_begin."""
-        ms = C.c_double(0)
-        n = C.c_int(0)
-        check(self.lib.dgr_profile_end(self.h, C.byref(ms), C.byref(n)))
-        return ms.value, n.value
-
-    def device_state(self):
-        p = C.POINTER(C.c_double)()
-        pitch = C.c_int(0)
-        check(self.lib.dgr_device_state(self.h, C.byref(p), C.byref(pitch)))
-        return C.cast(p, C.c_void_p).value, pitch.value
+    def run(self, nsteps):
+        """nsteps steps of every member; returns the per-member reports."""
+        rc = self.lib.dgr_run(self.h, int(nsteps), None)
+        reps = []
+        for i in range(len(self)):
+            rep = Report()
+            self.lib.dgr_member_report(self.h, i, C.byref(rep))
+            reps.append(rep)
+        if rc != DGR_OK:
+            bad = [(i, r.error_code, r.error_where) for i, r in enumerate(reps) if r.error_code]
+            raise DgrError(rc, "dgr_run: member errors (member, code, where) %s" % bad)
+        return reps
 
 
 def math_selftest(x, y):

@@ -153,18 +153,47 @@ dgr_status dgr_profile_begin(dgr_handle *h);
 dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *launches);
 
 /* Kernel variants 0..DGR_NUM_VARIANTS-1 (all compute the same bits):
- *   bit 0 = per-element geometry read from precomputed planes (else
- *           recomputed in registers);
- *   bit 1 = edge fluxes and RHS in two kernels, fluxes gathered per element
- *           (else one fused kernel);
- *   value 4/5 = two kernels, fluxes computed once per edge.
+ *   0 = one fused kernel per stage (element-centric flux gather, inner
+ *       products and RK update);
+ *   1 = element-centric fluxes into Fhat planes, then the RHS/update kernel;
+ *   2 = edge-centric fluxes (one Roe evaluation per edge) into Fhat planes,
+ *       then the RHS/update kernel.
  * dgr_autotune times `steps` steps of each from the current state (restored
  * afterwards), keeps the fastest and reports ms/step per variant in
  * ms_per_step[DGR_NUM_VARIANTS] (may be NULL). */
-#define DGR_NUM_VARIANTS 6
+#define DGR_NUM_VARIANTS 3
 dgr_status dgr_set_variant(dgr_handle *h, int variant);
 dgr_status dgr_get_variant(dgr_handle *h, int *variant);
 dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step);
+
+/* Run dgr_run's steps as replays of one captured hipGraph of the step
+ * (on != 0) instead of enqueueing each launch; same kernels, same bits. */
+dgr_status dgr_set_graph(dgr_handle *h, int on);
+
+/* Ensembles (extension; no counterpart in the reference, which runs one
+ * mesh per process).  n independent members, each with its own mesh, state,
+ * boundary data, forcing, time and dt, advanced together: every kernel of a
+ * step covers all members in one launch.  g, h0, velzero and wetdry must be
+ * the same for every member.  dgr_run / dgr_autotune / dgr_set_graph /
+ * dgr_set_final_time act on all members; the dgr_member_* calls address one
+ * member (0-based) and the plain single-instance calls address member 0.
+ * Each member's trajectory is bit-for-bit that of its own single-instance
+ * handle.  dgr_run returns DGR_ERR_NAN if any member recorded an error; after
+ * an error that member's state is undefined, the others are unaffected. */
+dgr_status dgr_ensemble_create(int n, const dgr_mesh_view *meshes, const dgr_params *params,
+                               dgr_handle **out);
+dgr_status dgr_num_members(dgr_handle *h, int *n);
+dgr_status dgr_member_set_state(dgr_handle *h, int member, const double *zeta, const double *qx,
+                                const double *qy);
+dgr_status dgr_member_get_state(dgr_handle *h, int member, double *zeta, double *qx, double *qy);
+dgr_status dgr_member_set_wd(dgr_handle *h, int member, const int *wd);
+dgr_status dgr_member_get_wd(dgr_handle *h, int member, int *wd);
+dgr_status dgr_member_get_modify(dgr_handle *h, int member, int stage, unsigned char *modify);
+dgr_status dgr_member_set_boundary(dgr_handle *h, int member, const double *bzeta, const double *bqn);
+dgr_status dgr_member_set_time(dgr_handle *h, int member, double t, double dt);
+dgr_status dgr_member_report(dgr_handle *h, int member, dgr_report *report);
+/* element range of a member inside the device planes of dgr_device_state */
+dgr_status dgr_member_layout(dgr_handle *h, int member, int *el_offset, int *num_el);
 
 /* Synchronise the handle's stream. */
 dgr_status dgr_sync(dgr_handle *h);
