@@ -32,12 +32,9 @@ KSTAGE_BYTES_PER_EL = 244.0
 # algorithmic bytes per element per RK stage for the whole stage
 STAGE_BYTES_PER_EL = 268.0
 # the kernel(s) each stage-kernel variant launches for the flux/RHS/update
-STAGE_KERNELS = {0: "k_stage (flux+RHS+RK update, geometry in registers)",
-                 1: "k_stage (flux+RHS+RK update, geometry planes)",
-                 2: "k_flux + k_rhs (split, geometry in registers)",
-                 3: "k_flux + k_rhs (split, geometry planes)",
-                 4: "k_edge_flux + k_rhs (edge-centric fluxes, geometry in registers)",
-                 5: "k_edge_flux + k_rhs (edge-centric fluxes, geometry planes)"}
+STAGE_KERNELS = {0: "k_stage (fused flux gather + RHS + RK update)",
+                 1: "k_flux + k_rhs (element-centric fluxes, then RHS + RK update)",
+                 2: "k_edge_flux + k_rhs (edge-centric fluxes, then RHS + RK update)"}
 
 
 def parse():
@@ -48,6 +45,9 @@ def parse():
     ap.add_argument("--N", type=int, default=512, help="grid cells per side (2*N*N triangles)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--ens-members", type=int, default=64)
+    ap.add_argument("--ens-steps", type=int, default=20, help="timed steps of the ensemble and of each solo run")
     ap.add_argument("--variant", type=int, default=-1,
                     help="stage-kernel variant to time (default: pick the fastest with dgr_autotune)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -69,12 +69,16 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "dispatches": 5}))
 
 
-def load_traffic():
-    p = os.path.join(ROOT, "profiles", "k_stage_coastal_512_traffic.json")
+def load_traffic(variant):
+    """HBM bytes per stage launch of the stage kernels, from the FETCH_SIZE /
+    WRITE_SIZE passes summarised in profiles/ (None when they cover another
+    variant)."""
+    p = os.path.join(ROOT, "profiles", "stage_kernels_coastal_512_traffic.json")
     if os.path.exists(p):
         with open(p) as f:
             d = json.load(f)
-        return d.get("bytes_per_launch")
+        if d.get("variant") == variant:
+            return d.get("bytes_per_launch")
     return None
 
 
@@ -96,6 +100,51 @@ def cpu_baseline(wl, budget_s):
     return {"value": dofs / el, "unit": "DOF-updates/s per RK stage", "cores": 1, "kind": "port",
             "sample": "%d SSP-RK2 steps of the same %d-triangle workload, oracle C restatement, 1 thread, %.1f s"
                       % (steps, wl.num_el, el)}
+
+
+def ensemble_bench(a, variant):
+    """The batched-ensemble workload (64 coastal members, N log-uniform in
+    [64, 512], own tide amplitude, rain, bumps and n field each): every
+    member advanced by one launch per kernel, steps replayed from a captured
+    hipGraph, against the plain one-instance-per-launch path (each member on
+    its own handle, launches enqueued per step) over the same steps."""
+    from dgrain import configs, native
+    wls = configs.ensemble(n=a.ens_members, seed=3)
+    ne_tot = sum(w.num_el for w in wls)
+    ens = native.Ensemble([w.mesh for w in wls], [w.solver_params() for w in wls])
+    for i, w in enumerate(wls):
+        ens.set_state(i, w.zeta, w.qx, w.qy)
+    ens.set_graph(True)
+    ens.run(2)
+    ens.sync()
+    ens_variant, ens_tune = ens.autotune(3)
+    ens.run(1)
+    ens.sync()
+    t0 = time.perf_counter()
+    ens.run(a.ens_steps)
+    ens.sync()
+    t_ens = time.perf_counter() - t0
+    ens.close()
+    t_solo = 0.0
+    for w in wls:
+        s = native.Solver(w.mesh, w.solver_params())
+        s.set_variant(ens_variant)
+        s.set_state(w.zeta, w.qx, w.qy)
+        s.run(3)
+        s.sync()
+        t0 = time.perf_counter()
+        s.run(a.ens_steps)
+        s.sync()
+        t_solo += time.perf_counter() - t0
+        s.close()
+    dofs = 9.0 * ne_tot * 2 * a.ens_steps
+    v_ens, v_solo = dofs / t_ens, dofs / t_solo
+    return {"members": len(wls), "triangles": ne_tot, "sizes_N": [int(round((w.num_el / 2) ** 0.5)) for w in wls],
+            "steps": a.ens_steps, "variant": ens_variant, "autotune_ms_per_step": ens_tune,
+            "value": v_ens, "unit": "DOF-updates/s", "ms_per_step": t_ens / a.ens_steps * 1e3,
+            "solo_value": v_solo, "solo_ms_per_step": t_solo / a.ens_steps * 1e3,
+            "speedup": v_ens / v_solo,
+            "stage_roofline_frac": STAGE_BYTES_PER_EL * ne_tot * 2 * a.ens_steps / t_ens / 1e9 / HBM_PEAK_GBS}
 
 
 def main():
@@ -137,7 +186,7 @@ def main():
     value = dof_updates / el
     k_avg_s = (k_ms / k_n) * 1e-3
     achieved = KSTAGE_BYTES_PER_EL * ne / k_avg_s / 1e9
-    traffic = load_traffic()
+    traffic = load_traffic(variant) if a.N == 512 else None
     stage_gbs = STAGE_BYTES_PER_EL * ne * 2 * a.steps / el / 1e9
     out = {
         "metric": "FP64 DOF-updates/s per RK stage",
@@ -164,6 +213,11 @@ def main():
         "ensemble_speedup": None,
         "adjoint_forward_cost": None,
     }
+    s.close()
+    if not a.no_ensemble:
+        e = ensemble_bench(a, variant)
+        out["ensemble"] = e
+        out["ensemble_speedup"] = e["speedup"]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
     print(json.dumps(out))

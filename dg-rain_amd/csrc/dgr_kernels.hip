@@ -801,7 +801,7 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
 // Split variants, part 2: inner products, RHS and RK update from the Fhat planes.
 //   MODE 0: RHS only into dbg_rhs; MODE 1/2 as in k_stage.
 template <int MODE>
-__global__ __launch_bounds__(256) void k_rhs(MeshDev m, PhysDev ph, const double *__restrict__ win, double *wout,
+__global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const double *__restrict__ win, double *wout,
                                              const double *wbase, const double *__restrict__ fhat,
                                              const int *__restrict__ wd, double *__restrict__ means,
                                              const Ctrl *ctrls, double *__restrict__ dbg_rhs)
