@@ -1,0 +1,485 @@
+// dgr_adjoint.h -- hand-written reverse-mode derivatives of the discrete 2D
+// p=1 DG step's pieces, for the gradient of a loss with respect to the state,
+// nodal Manning n and nodal depth z.
+//
+// Each *_adj function re-evaluates the forward operation from its inputs and
+// accumulates  xbar += (d out / d x)^T outbar  for every input x.  The forward
+// operations are the ones of dgr_kernels.hip (numericalFlux2D.c:6-108,
+// 2DInnerProducts.c:8-570, compute2DL.c:338-411, SlopeLimiter2D.c:157-343);
+// their non-smooth parts (max, |.|, min/max clipping, limiter branches) are
+// differentiated on the branch the forward evaluation takes, which is the
+// forward pass's active set.  Derivatives need not reproduce the forward's
+// rounding, so algebraically equal forms are used where they are simpler.
+//
+// Host and device: the same functions are compiled into the GPU kernels and
+// into the host-side finite-difference checks.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+namespace dgr {
+
+#define DGR_HD __host__ __device__ __forceinline__
+
+DGR_HD double sgn_d(double a) { return a > 0 ? 1.0 : (a < 0 ? -1.0 : 0.0); }
+
+// ---------------------------------------------------------------------------
+// Roe flux (numericalFlux2D.c:6-108): F[3] and max|lambda| from
+// (zi, ze, qxi, qxe, qyi, qye, zb) given (nx, ny, lg, g).  In[7] holds the
+// inputs in that order; inbar[7] receives the accumulated adjoints.
+// ---------------------------------------------------------------------------
+struct RoeIn {
+    double zi, ze, qxi, qxe, qyi, qye, zb;
+};
+
+DGR_HD double roe_eval(double F[3], const RoeIn &a, double nx, double ny, double lg, double g)
+{
+    const double Hi = a.zi + a.zb, He = a.ze + a.zb;
+    const double ui = a.qxi / Hi, ue = a.qxe / He, vi = a.qyi / Hi, ve = a.qye / He;
+    const double j0 = a.zi - a.ze, j1 = a.qxi - a.qxe, j2 = a.qyi - a.qye;
+    const double fni0 = (Hi * ui) * nx + (Hi * vi) * ny, fne0 = (He * ue) * nx + (He * ve) * ny;
+    const double fni1 = (Hi * ui * ui + 0.5 * lg * (Hi * Hi - a.zb * a.zb)) * nx + (Hi * ui * vi) * ny;
+    const double fne1 = (He * ue * ue + 0.5 * lg * (He * He - a.zb * a.zb)) * nx + (He * ue * ve) * ny;
+    const double fni2 = (Hi * ui * vi) * nx + (Hi * vi * vi + 0.5 * lg * (Hi * Hi - a.zb * a.zb)) * ny;
+    const double fne2 = (He * ue * ve) * nx + (He * ve * ve + 0.5 * lg * (He * He - a.zb * a.zb)) * ny;
+    const double Hh = 0.5 * (Hi + He);
+    const double sHi = sqrt(Hi), sHe = sqrt(He), S = sHi + sHe;
+    const double uh = (ui * sHi + ue * sHe) / S, vh = (vi * sHi + ve * sHe) / S;
+    const double c = sqrt(g * Hh);
+    const double den = -2 * c;
+    const double Q[3][3] = {{(-c - nx * uh - ny * vh) / den, nx / den, ny / den},
+                            {(-2 * c * ny * uh + 2 * c * nx * vh) / den, 2 * c * ny / den, -2 * c * nx / den},
+                            {(-c + nx * uh + ny * vh) / den, -nx / den, -ny / den}};
+    const double sgi = sqrt(g * Hi), sge = sqrt(g * He);
+    const double li[3] = {ui * nx + vi * ny - sgi, ui * nx + vi * ny, ui * nx + vi * ny + sgi};
+    const double le[3] = {ue * nx + ve * ny - sge, ue * nx + ve * ny, ue * nx + ve * ny + sge};
+    const double lh[3] = {uh * nx + vh * ny - c, uh * nx + vh * ny, uh * nx + vh * ny + c};
+    double lam[3];
+    for (int k = 0; k < 3; ++k) {
+        double e = fmax(0.0, lh[k] - li[k]);
+        e = fmax(e, le[k] - lh[k]);
+        lam[k] = fmax(e, fabs(lh[k]));
+    }
+    double p1[3], p2[3];
+    for (int r = 0; r < 3; ++r) p1[r] = Q[r][0] * j0 + Q[r][1] * j1 + Q[r][2] * j2;
+    for (int r = 0; r < 3; ++r) p2[r] = lam[r] * p1[r];
+    const double R1[3] = {uh - c * nx, -ny, uh + c * nx}, R2[3] = {vh - c * ny, nx, vh + c * ny};
+    F[0] = 0.5 * (fni0 + fne0) + 0.5 * (p2[0] + p2[2]);
+    F[1] = 0.5 * (fni1 + fne1) + 0.5 * (R1[0] * p2[0] + R1[1] * p2[1] + R1[2] * p2[2]);
+    F[2] = 0.5 * (fni2 + fne2) + 0.5 * (R2[0] * p2[0] + R2[1] * p2[1] + R2[2] * p2[2]);
+    double mx = fabs(li[0]);
+    const double ops[5] = {fabs(le[0]), fabs(li[1]), fabs(le[1]), fabs(li[2]), fabs(le[2])};
+    for (int i = 0; i < 5; ++i) mx = ops[i] > mx ? ops[i] : mx;
+    return mx;
+}
+
+// Reverse of roe_eval: Fb[3] and lamb (adjoint of the returned max|lambda|)
+// accumulate into ib.
+DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, const double Fb[3], double lamb,
+                    RoeIn &ib)
+{
+    // forward values
+    const double Hi = a.zi + a.zb, He = a.ze + a.zb;
+    const double ui = a.qxi / Hi, ue = a.qxe / He, vi = a.qyi / Hi, ve = a.qye / He;
+    const double j[3] = {a.zi - a.ze, a.qxi - a.qxe, a.qyi - a.qye};
+    const double Hh = 0.5 * (Hi + He);
+    const double sHi = sqrt(Hi), sHe = sqrt(He), S = sHi + sHe;
+    const double uh = (ui * sHi + ue * sHe) / S, vh = (vi * sHi + ve * sHe) / S;
+    const double c = sqrt(g * Hh);
+    const double den = -2 * c;
+    const double num[3] = {-c - nx * uh - ny * vh, -2 * c * ny * uh + 2 * c * nx * vh, -c + nx * uh + ny * vh};
+    const double Q[3][3] = {{num[0] / den, nx / den, ny / den},
+                            {num[1] / den, 2 * c * ny / den, -2 * c * nx / den},
+                            {num[2] / den, -nx / den, -ny / den}};
+    const double sgi = sqrt(g * Hi), sge = sqrt(g * He);
+    const double li[3] = {ui * nx + vi * ny - sgi, ui * nx + vi * ny, ui * nx + vi * ny + sgi};
+    const double le[3] = {ue * nx + ve * ny - sge, ue * nx + ve * ny, ue * nx + ve * ny + sge};
+    const double lh[3] = {uh * nx + vh * ny - c, uh * nx + vh * ny, uh * nx + vh * ny + c};
+    double lam[3];
+    int act[3];   // 0: zero, 1: lh-li, 2: le-lh, 3: |lh|
+    for (int k = 0; k < 3; ++k) {
+        double e = 0.0;
+        int ae = 0;
+        const double t1 = lh[k] - li[k];
+        if (t1 > e) { e = t1; ae = 1; }
+        const double t2 = le[k] - lh[k];
+        if (t2 > e) { e = t2; ae = 2; }
+        const double t3 = fabs(lh[k]);
+        if (t3 > e) { e = t3; ae = 3; }
+        lam[k] = e;
+        act[k] = ae;
+    }
+    double p1[3], p2[3];
+    for (int r = 0; r < 3; ++r) p1[r] = Q[r][0] * j[0] + Q[r][1] * j[1] + Q[r][2] * j[2];
+    for (int r = 0; r < 3; ++r) p2[r] = lam[r] * p1[r];
+    const double R1[3] = {uh - c * nx, -ny, uh + c * nx}, R2[3] = {vh - c * ny, nx, vh + c * ny};
+
+    // reverse
+    double Hib = 0, Heb = 0, uib = 0, ueb = 0, vib = 0, veb = 0, zbb = 0;
+    double uhb = 0, vhb = 0, cb = 0, sgib = 0, sgeb = 0;
+    // F = 0.5 (fni + fne) + 0.5 p3
+    const double fnb[3] = {0.5 * Fb[0], 0.5 * Fb[1], 0.5 * Fb[2]};
+    const double p3b[3] = {0.5 * Fb[0], 0.5 * Fb[1], 0.5 * Fb[2]};
+    // p3 = [1 0 1; R1; R2] p2
+    double p2b[3];
+    for (int k = 0; k < 3; ++k) p2b[k] = R1[k] * p3b[1] + R2[k] * p3b[2];
+    p2b[0] += p3b[0];
+    p2b[2] += p3b[0];
+    const double R1b0 = p2[0] * p3b[1], R1b2 = p2[2] * p3b[1];
+    const double R2b0 = p2[0] * p3b[2], R2b2 = p2[2] * p3b[2];
+    uhb += R1b0 + R1b2;
+    cb += -nx * R1b0 + nx * R1b2;
+    vhb += R2b0 + R2b2;
+    cb += -ny * R2b0 + ny * R2b2;
+    // p2 = lam * p1
+    double p1b[3], lamb_k[3];
+    for (int k = 0; k < 3; ++k) { p1b[k] = lam[k] * p2b[k]; lamb_k[k] = p1[k] * p2b[k]; }
+    // p1 = Q j
+    double jb[3] = {0, 0, 0}, Qb[3][3];
+    for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) { Qb[r][s] = p1b[r] * j[s]; jb[s] += Q[r][s] * p1b[r]; }
+    // entropy fix
+    double lib[3] = {0, 0, 0}, leb[3] = {0, 0, 0}, lhb[3] = {0, 0, 0};
+    for (int k = 0; k < 3; ++k) {
+        if (act[k] == 1) { lhb[k] += lamb_k[k]; lib[k] -= lamb_k[k]; }
+        else if (act[k] == 2) { leb[k] += lamb_k[k]; lhb[k] -= lamb_k[k]; }
+        else if (act[k] == 3) lhb[k] += sgn_d(lh[k]) * lamb_k[k];
+    }
+    // max |lambda| (left fold, ties keep the earlier operand)
+    // (an exact tie shares the adjoint among the tied operands, the value a
+    // central difference gives for |x| at a kink such as l1 = -l3 at rest)
+    if (lamb != 0.0) {
+        const double vals[6] = {li[0], le[0], li[1], le[1], li[2], le[2]};
+        double mx = 0.0;
+        for (int i = 0; i < 6; ++i) mx = fabs(vals[i]) > mx ? fabs(vals[i]) : mx;
+        int nt = 0;
+        for (int i = 0; i < 6; ++i) nt += fabs(vals[i]) == mx;
+        for (int i = 0; i < 6; ++i)
+            if (fabs(vals[i]) == mx) {
+                const double d = sgn_d(vals[i]) * lamb / nt;
+                if (i & 1) leb[i >> 1] += d; else lib[i >> 1] += d;
+            }
+    }
+    // eigenvalues
+    uib += nx * (lib[0] + lib[1] + lib[2]);
+    vib += ny * (lib[0] + lib[1] + lib[2]);
+    sgib += -lib[0] + lib[2];
+    ueb += nx * (leb[0] + leb[1] + leb[2]);
+    veb += ny * (leb[0] + leb[1] + leb[2]);
+    sgeb += -leb[0] + leb[2];
+    uhb += nx * (lhb[0] + lhb[1] + lhb[2]);
+    vhb += ny * (lhb[0] + lhb[1] + lhb[2]);
+    cb += -lhb[0] + lhb[2];
+    Hib += sgib * g / (2 * sgi);
+    Heb += sgeb * g / (2 * sge);
+    // Q = num / den and the constant-numerator entries
+    double denb = 0;
+    for (int r = 0; r < 3; ++r)
+        for (int s = 0; s < 3; ++s) denb += -Q[r][s] * Qb[r][s] / den;
+    const double n0b = Qb[0][0] / den, n1b = Qb[1][0] / den, n2b = Qb[2][0] / den;
+    cb += -n0b;
+    uhb += -nx * n0b;
+    vhb += -ny * n0b;
+    cb += n1b * (-2 * ny * uh + 2 * nx * vh);
+    uhb += n1b * (-2 * c * ny);
+    vhb += n1b * (2 * c * nx);
+    cb += -n2b;
+    uhb += nx * n2b;
+    vhb += ny * n2b;
+    cb += 2 * ny * Qb[1][1] / den - 2 * nx * Qb[1][2] / den;
+    cb += -2 * denb;
+    // c = sqrt(g Hh)
+    const double Hhb = cb * g / (2 * c);
+    Hib += 0.5 * Hhb;
+    Heb += 0.5 * Hhb;
+    // Roe averages
+    const double numub = uhb / S, numvb = vhb / S;
+    const double Sb = -uh * uhb / S - vh * vhb / S;
+    uib += sHi * numub;
+    ueb += sHe * numub;
+    vib += sHi * numvb;
+    veb += sHe * numvb;
+    const double sHib = ui * numub + vi * numvb + Sb, sHeb = ue * numub + ve * numvb + Sb;
+    Hib += sHib / (2 * sHi);
+    Heb += sHeb / (2 * sHe);
+    // physical normal fluxes
+    {
+        const double f1x = nx * fnb[0], f1y = ny * fnb[0], f2x = nx * fnb[1], f2y = ny * fnb[1];
+        const double f3x = nx * fnb[2], f3y = ny * fnb[2];
+        // in side
+        Hib += ui * f1x + vi * f1y + (ui * ui + lg * Hi) * f2x + ui * vi * f2y + ui * vi * f3x + (vi * vi + lg * Hi) * f3y;
+        uib += Hi * f1x + 2 * Hi * ui * f2x + Hi * vi * f2y + Hi * vi * f3x;
+        vib += Hi * f1y + Hi * ui * f2y + Hi * ui * f3x + 2 * Hi * vi * f3y;
+        zbb += -lg * a.zb * (f2x + f3y);
+        // ex side
+        Heb += ue * f1x + ve * f1y + (ue * ue + lg * He) * f2x + ue * ve * f2y + ue * ve * f3x + (ve * ve + lg * He) * f3y;
+        ueb += He * f1x + 2 * He * ue * f2x + He * ve * f2y + He * ve * f3x;
+        veb += He * f1y + He * ue * f2y + He * ue * f3x + 2 * He * ve * f3y;
+        zbb += -lg * a.zb * (f2x + f3y);
+    }
+    // velocities
+    ib.qxi += uib / Hi;
+    Hib += -ui * uib / Hi;
+    ib.qyi += vib / Hi;
+    Hib += -vi * vib / Hi;
+    ib.qxe += ueb / He;
+    Heb += -ue * ueb / He;
+    ib.qye += veb / He;
+    Heb += -ve * veb / He;
+    // jumps
+    ib.zi += jb[0];
+    ib.ze -= jb[0];
+    ib.qxi += jb[1];
+    ib.qxe -= jb[1];
+    ib.qyi += jb[2];
+    ib.qye -= jb[2];
+    // depths
+    ib.zi += Hib;
+    ib.ze += Heb;
+    ib.zb += Hib + Heb + zbb;
+}
+
+// ---------------------------------------------------------------------------
+// RT0 fit (2DInnerProducts.c:134-146) as the linear map (R1,R2,R3) -> (a,b,c)
+// and its transpose.
+// ---------------------------------------------------------------------------
+struct Rt0 {
+    double A[3], B[3], C[3];   // a = A.R, b = B.R, c = C.R
+};
+
+DGR_HD Rt0 rt0_map(double M11, double M12, double M13, double M21, double M22, double M23, double M31, double M32,
+                   double M33, double den)
+{
+    Rt0 m;
+    m.A[0] = (M23 * M32 - M22 * M33) / den;
+    m.A[1] = (M12 * M33 - M13 * M32) / den;
+    m.A[2] = (M13 * M22 - M12 * M23) / den;
+    m.B[0] = -(M23 * M31 - M21 * M33) / den;
+    m.B[1] = -(M11 * M33 - M13 * M31) / den;
+    m.B[2] = -(M13 * M21 - M11 * M23) / den;
+    m.C[0] = (M22 * M31 - M21 * M32) / den;
+    m.C[1] = (M11 * M32 - M12 * M31) / den;
+    m.C[2] = (M12 * M21 - M11 * M22) / den;
+    return m;
+}
+
+// ---------------------------------------------------------------------------
+// Element RHS (2DInnerProducts.c:14-566 + compute2DL.c:344-409) reverse.
+// Geometry quantities come from geometry_of (GeoIdx); the ones that depend on
+// the nodal depths are the midpoint depths mz (GX_MZ) and the bed-slope terms
+// (GX_BGX/BGY), the one that depends on Manning n is GX_GNN.
+// Inputs: W[9] state, wd_self; outputs adjoint Rb[9].
+// Accumulates Wb[9], Fhb[9], mzb[3], gnnb[3], bgxb, bgyb.
+// ---------------------------------------------------------------------------
+template <typename Geo>
+DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double g, const double Rb[9], double Wb[9],
+                            double Fhb[9], double mzb[3], double gnnb[3], double &bgxb, double &bgyb)
+{
+    const double nxs[3] = {G(GX_NX), G(GX_NX + 1), G(GX_NX + 2)};
+    const double nys[3] = {G(GX_NY), G(GX_NY + 1), G(GX_NY + 2)};
+    const Rt0 T = rt0_map(nxs[0], G(GX_M2), nys[0], nxs[1], G(GX_M2 + 1), nys[1], nxs[2], G(GX_M2 + 2), nys[2],
+                          G(GX_DEN));
+    const double Mi = G(GX_MI), aw = G(GX_AW), aj = G(GX_AJ), bgx = G(GX_BGX), bgy = G(GX_BGY);
+    const double weight = 1.0 / 6;
+    const double lg = (wd_self == 0) ? 0.0 : g;
+    // R = Mi (..) : adjoints of ze, qe, re
+    double eb[9];
+    for (int i = 0; i < 9; ++i) eb[i] = Mi * Rb[i];
+    // surface term SI = -..: SIb = -eb
+    // SI[f*3+b] = sum_ed ej (Fn[f][ed][0] p0 + Fn[f][ed][1] p1),
+    // Fn = (a + b xq) nx + (c + b yq) ny
+    double ab[3] = {0, 0, 0}, bb_[3] = {0, 0, 0}, cb_[3] = {0, 0, 0};
+    for (int ed = 0; ed < 3; ++ed) {
+        const double ej = G(GX_EJ + ed);
+        for (int n = 0; n < 2; ++n) {
+            const double xq = G(GX_XE + 2 * ed + n), yq = G(GX_YE + 2 * ed + n);
+            for (int f = 0; f < 3; ++f) {
+                double Fnb = 0;
+                for (int b = 0; b < 3; ++b) Fnb += ej * G(GX_PS + 6 * b + 2 * ed + n) * (-eb[f * 3 + b]);
+                ab[f] += nxs[ed] * Fnb;
+                cb_[f] += nys[ed] * Fnb;
+                bb_[f] += (xq * nxs[ed] + yq * nys[ed]) * Fnb;
+            }
+        }
+    }
+    for (int f = 0; f < 3; ++f)
+        for (int k = 0; k < 3; ++k) Fhb[f * 3 + k] += T.A[k] * ab[f] + T.B[k] * bb_[f] + T.C[k] * cb_[f];
+    // volume term VF[f][b] = aj (0.5 va GPx_b + 0.5 vc GPy_b + vb weight GS_b), (va,vb,vc) = rt0(Fnp)
+    double Fnpb[3][3];   // [field][k]
+    for (int f = 0; f < 3; ++f) {
+        double vab = 0, vbb = 0, vcb = 0;
+        for (int b = 0; b < 3; ++b) {
+            const double x = eb[f * 3 + b];
+            vab += aj * 0.5 * G(GX_GP + 2 * b) * x;
+            vcb += aj * 0.5 * G(GX_GP + 2 * b + 1) * x;
+            vbb += aj * weight * G(GX_GS + b) * x;
+        }
+        for (int k = 0; k < 3; ++k) Fnpb[f][k] = T.A[k] * vab + T.B[k] * vbb + T.C[k] * vcb;
+    }
+    // point fluxes, friction and bed terms per midpoint k
+    for (int k = 0; k < 3; ++k) {
+        const double mz = G(GX_MZ + k), gnn = G(GX_GNN + k);
+        const double zr = W[k], qx = W[3 + k], qy = W[6 + k];
+        const double H = zr + mz;
+        const double u = qx / H, v = qy / H;
+        const double sp = sqrt(u * u + v * v);
+        const double H43 = H * cbrt(H);
+        const double tau = gnn * sp / H43;
+        double Hb = 0, qxb = 0, qyb = 0, zrb = 0, mzb_explicit = 0;
+        // Fx/Fy adjoints from the normal point fluxes
+        const double fxb[3] = {nxs[k] * Fnpb[0][k], nxs[k] * Fnpb[1][k], nxs[k] * Fnpb[2][k]};
+        const double fyb[3] = {nys[k] * Fnpb[0][k], nys[k] * Fnpb[1][k], nys[k] * Fnpb[2][k]};
+        // Fx0 = qx, Fy0 = qy
+        qxb += fxb[0];
+        qyb += fyb[0];
+        // Fx1 = qx^2/H + 0.5 lg (H^2 - mz^2)
+        qxb += 2 * qx / H * fxb[1];
+        Hb += (-qx * qx / (H * H) + lg * H) * fxb[1];
+        mzb_explicit += -lg * mz * fxb[1];
+        // Fx2 = Fy1 = qx qy / H
+        const double x2 = fxb[2] + fyb[1];
+        qxb += qy / H * x2;
+        qyb += qx / H * x2;
+        Hb += -qx * qy / (H * H) * x2;
+        // Fy2 = qy^2/H + 0.5 lg (H^2 - mz^2)
+        qyb += 2 * qy / H * fyb[2];
+        Hb += (-qy * qy / (H * H) + lg * H) * fyb[2];
+        mzb_explicit += -lg * mz * fyb[2];
+        // friction fx = aw tau qx, fy = aw tau qy enter with a minus sign
+        const double qeb = eb[3 + k], reb = eb[6 + k];
+        const double taub = -aw * (qx * qeb + qy * reb);
+        qxb += -aw * tau * qeb;
+        qyb += -aw * tau * reb;
+        // tau = gnn sp / H^(4/3)
+        gnnb[k] += sp / H43 * taub;
+        const double spb = gnn / H43 * taub;
+        Hb += -(4.0 / 3.0) * tau / H * taub;
+        if (sp > 0) {
+            const double ub = u / sp * spb, vb = v / sp * spb;
+            qxb += ub / H;
+            qyb += vb / H;
+            Hb += -(u * ub + v * vb) / H;
+        }
+        // bed slope terms bx = bgx zeta, by = bgy zeta (zero when dry)
+        if (wd_self != 0) {
+            zrb += bgx * qeb + bgy * reb;
+            bgxb += zr * qeb;
+            bgyb += zr * reb;
+        }
+        // H = zr + mz
+        zrb += Hb;
+        mzb[k] += Hb + mzb_explicit;
+        Wb[k] += zrb;
+        Wb[3 + k] += qxb;
+        Wb[6 + k] += qyb;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Limiter element pass (SlopeLimiter2D.c:157-343) reverse for one element:
+// S[9] is the element's state before limiting, Vmin/Vmax[f][k] the node
+// bounds at its vertices, zv[3] the vertex depths; Sb[9] the adjoint of the
+// limited state.  Returns the modify flag.  Accumulates Sin_b[9] (into the
+// element's own pre-limit state) and Vminb/Vmaxb[f][k].
+// ---------------------------------------------------------------------------
+DGR_HD int limit_adj(const double S[9], const double Vmin[3][3], const double Vmax[3][3], const double zv[3],
+                     double H0, const double Sb[9], double Sin_b[9], double Vminb[3][3], double Vmaxb[3][3])
+{
+    // forward decisions
+    int modify = 0;
+    int clampsel[3][3];          // 0: V, 1: Vmax, 2: Vmin
+    int sd[3][3];                // [field][pass]
+    int pos[3][3][3], dvv[3][3][3], use0[3][3][3];   // [field][pass][k]
+    double Vfin[3][3];
+    for (int iv = 0; iv < 3; ++iv) {
+        const double w1 = S[3 * iv], w2 = S[3 * iv + 1], w3 = S[3 * iv + 2];
+        double V[3] = {w1 - w2 + w3, w1 + w2 - w3, -w1 + w2 + w3};
+        const double avg = (w1 + w2 + w3) / 3;
+        if (iv == 0) {
+            const double Ha = V[0] + zv[0], Hb_ = V[1] + zv[1], Hc = V[2] + zv[2];
+            const double Hm0 = Vmin[0][0] + zv[0], Hm1 = Vmin[0][1] + zv[1], Hm2 = Vmin[0][2] + zv[2];
+            if (Ha > H0 && Hb_ > H0 && Hc > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
+        }
+        for (int k = 0; k < 3; ++k) {
+            const double lo = (V[k] < Vmax[iv][k]) ? V[k] : Vmax[iv][k];
+            int sel = (V[k] < Vmax[iv][k]) ? 0 : 1;
+            if (Vmin[iv][k] > lo) sel = 2;
+            clampsel[iv][k] = sel;
+            V[k] = fmax(lo, Vmin[iv][k]);
+        }
+        for (int ll = 0; ll < 3; ++ll) {
+            const double sumloc = (V[0] + V[1] + V[2]) / 3;
+            double sumdiff = (sumloc - avg) * 3;
+            const int s = sumdiff < 0 ? -1 : (sumdiff == 0 ? 0 : 1);
+            sd[iv][ll] = s;
+            const double diff[3] = {(V[0] - avg) * s, (V[1] - avg) * s, (V[2] - avg) * s};
+            double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
+            for (int k = 0; k < 3; ++k) {
+                const double dv = fmax(1.0, kdp);
+                double redfac;
+                pos[iv][ll][k] = diff[k] > 0;
+                dvv[iv][ll][k] = (int)dv;
+                if (diff[k] > 0) { redfac = sumdiff * s / dv; kdp -= 1; }
+                else redfac = 0;
+                const double redmax = (s > 0) ? V[k] - Vmin[iv][k] : Vmax[iv][k] - V[k];
+                use0[iv][ll][k] = redfac < redmax;
+                redfac = (redfac < redmax) ? redfac : redmax;
+                sumdiff -= redfac * s;
+                V[k] -= redfac * s;
+            }
+        }
+        for (int k = 0; k < 3; ++k) Vfin[iv][k] = V[k];
+        if (iv == 0) {
+            const double Ha = V[0] + zv[0], Hb_ = V[1] + zv[1], Hc = V[2] + zv[2];
+            if (Ha < H0 || Hb_ < H0 || Hc < H0) modify = 0;
+        }
+    }
+    (void)Vfin;
+    if (!modify) {
+        for (int i = 0; i < 9; ++i) Sin_b[i] += Sb[i];
+        return 0;
+    }
+    // reverse, field by field
+    for (int iv = 0; iv < 3; ++iv) {
+        // mids: S_out = ((V0+V1)/2, (V1+V2)/2, (V2+V0)/2)
+        double Vb[3];
+        Vb[0] = 0.5 * (Sb[3 * iv] + Sb[3 * iv + 2]);
+        Vb[1] = 0.5 * (Sb[3 * iv] + Sb[3 * iv + 1]);
+        Vb[2] = 0.5 * (Sb[3 * iv + 1] + Sb[3 * iv + 2]);
+        double avgb = 0;
+        for (int ll = 2; ll >= 0; --ll) {
+            const int s = sd[iv][ll];
+            double sumdiffb = 0;   // adjoint of the running sumdiff after the k loop
+            for (int k = 2; k >= 0; --k) {
+                // V_k -= red*s ; sumdiff -= red*s
+                const double redb = -s * Vb[k] - s * sumdiffb;
+                if (use0[iv][ll][k]) {
+                    if (pos[iv][ll][k]) sumdiffb += redb * s / dvv[iv][ll][k];
+                } else {
+                    if (s > 0) { Vb[k] += redb; Vminb[iv][k] -= redb; }
+                    else { Vmaxb[iv][k] += redb; Vb[k] -= redb; }
+                }
+            }
+            // sumdiff = (sum V / 3 - avg) * 3 = sum V - 3 avg
+            for (int k = 0; k < 3; ++k) Vb[k] += sumdiffb;
+            avgb += -3 * sumdiffb;
+        }
+        // clamp
+        double V0b[3];
+        for (int k = 0; k < 3; ++k) {
+            V0b[k] = 0;
+            if (clampsel[iv][k] == 0) V0b[k] = Vb[k];
+            else if (clampsel[iv][k] == 1) Vmaxb[iv][k] += Vb[k];
+            else Vminb[iv][k] += Vb[k];
+        }
+        // V0 = (w1-w2+w3, w1+w2-w3, -w1+w2+w3), avg = (w1+w2+w3)/3
+        Sin_b[3 * iv] += V0b[0] + V0b[1] - V0b[2] + avgb / 3;
+        Sin_b[3 * iv + 1] += -V0b[0] + V0b[1] + V0b[2] + avgb / 3;
+        Sin_b[3 * iv + 2] += V0b[0] - V0b[1] + V0b[2] + avgb / 3;
+    }
+    return 1;
+}
+
+#undef DGR_HD
+
+}  // namespace dgr

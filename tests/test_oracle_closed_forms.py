@@ -14,6 +14,7 @@ a swapped index or operand) fails it:
   * pow43 against exact decimal arithmetic; the tide sine against libm.
 """
 import math
+import os
 from decimal import Decimal, getcontext
 
 import numpy as np
@@ -244,3 +245,17 @@ def test_tide_sine_against_libm():
     for x in np.concatenate([rng.uniform(-100, 100, 5000), rng.uniform(-1e4, 1e4, 500), [0.0, math.pi, -math.pi / 2]]):
         s = oracle.dsin(x)
         assert abs(s - math.sin(x)) <= 2 * math.ulp(max(abs(math.sin(x)), 1e-300)) + 1e-300, x
+
+
+def test_adjoint_pieces_against_finite_differences(tmp_path):
+    """The reverse-mode pieces of the gradient (Roe flux, element RHS,
+    limiter; dg-rain_amd/csrc/dgr_adjoint.h) against central differences of
+    the forward operations, on the host (tests/adjoint_check.cpp)."""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    exe = str(tmp_path / "adjoint_check")
+    subprocess.run(["/opt/rocm/bin/hipcc", "-O2", "-std=c++17", "-o", exe,
+                    os.path.join(root, "tests", "adjoint_check.cpp")], check=True, capture_output=True)
+    r = subprocess.run([exe], capture_output=True, text=True)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
