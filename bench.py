@@ -46,6 +46,9 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost measurement")
+    ap.add_argument("--adj-N", type=int, default=256)
+    ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=20, help="timed steps of the ensemble and of each solo run")
     ap.add_argument("--variant", type=int, default=-1,
@@ -147,6 +150,48 @@ def ensemble_bench(a, variant):
             "stage_roofline_frac": STAGE_BYTES_PER_EL * ne_tot * 2 * a.ens_steps / t_ens / 1e9 / HBM_PEAK_GBS}
 
 
+def adjoint_bench(a, variant):
+    """Adjoint Manning-n inversion workload: the coastal physics on an N x N
+    grid (256: 131,072 triangles), a gauge misfit over 64 seeded gauge
+    elements sampled every 10 steps, observations from the seeded 'true' n
+    field, the gradient taken at a perturbed n (truth times a lognormal
+    factor, sigma 0.2) with checkpoints every 50 steps.  Cost = time of one
+    dgr_adjoint_gauges call (forward with checkpoints + recompute + reverse)
+    over the time of the plain forward run of the same steps."""
+    import numpy as np
+    from dgrain import configs, native
+    wl = configs.coastal(N=a.adj_N, start="flood")
+    rng = np.random.default_rng(4)
+    gauges = np.sort(rng.choice(wl.num_el, 64, replace=False)).astype(np.int32)
+    n_true = wl.mesh.n
+    n0 = n_true * np.exp(0.2 * rng.standard_normal(n_true.size))
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_variant(variant)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    obs = s.gauge_run(a.adj_steps, 10, gauges)
+    s.close()
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_variant(variant)
+    s.set_nodal(n=n0)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    s.adjoint_gauges(50, 10, 50, gauges, obs[:, :5])   # warm the code objects
+    s.sync()
+    t0 = time.perf_counter()
+    loss, gz, gn = s.adjoint_gauges(a.adj_steps, 10, 50, gauges, obs)
+    s.sync()
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    h = s.gauge_run(a.adj_steps, 10, gauges)
+    s.sync()
+    t_fwd = time.perf_counter() - t0
+    s.close()
+    return {"workload": "coastal_flood_%dx%d_manning_inversion" % (a.adj_N, a.adj_N), "triangles": wl.num_el,
+            "steps": a.adj_steps, "gauges": 64, "sample_every": 10, "checkpoint_every": 50,
+            "loss": float(loss[0]), "loss_forward_only": float(((h - obs) ** 2).sum()),
+            "grad_n_norm": float(np.linalg.norm(gn)), "grad_z_norm": float(np.linalg.norm(gz)),
+            "forward_s": t_fwd, "adjoint_s": t_adj, "ratio": t_adj / t_fwd}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -218,6 +263,10 @@ def main():
         e = ensemble_bench(a, variant)
         out["ensemble"] = e
         out["ensemble_speedup"] = e["speedup"]
+    if not a.no_adjoint:
+        ad = adjoint_bench(a, variant)
+        out["adjoint"] = ad
+        out["adjoint_forward_cost"] = ad["ratio"]
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
     print(json.dumps(out))

@@ -17,6 +17,8 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 
+#include "dgr_device.h"
+
 namespace dgr {
 
 #define DGR_HD __host__ __device__ __forceinline__
@@ -380,10 +382,12 @@ DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double
 // S[9] is the element's state before limiting, Vmin/Vmax[f][k] the node
 // bounds at its vertices, zv[3] the vertex depths; Sb[9] the adjoint of the
 // limited state.  Returns the modify flag.  Accumulates Sin_b[9] (into the
-// element's own pre-limit state) and Vminb/Vmaxb[f][k].
+// element's own pre-limit state) and Vminb/Vmaxb[f][k]; Sout (optional)
+// receives the limited state.
 // ---------------------------------------------------------------------------
 DGR_HD int limit_adj(const double S[9], const double Vmin[3][3], const double Vmax[3][3], const double zv[3],
-                     double H0, const double Sb[9], double Sin_b[9], double Vminb[3][3], double Vmaxb[3][3])
+                     double H0, const double Sb[9], double Sin_b[9], double Vminb[3][3], double Vmaxb[3][3],
+                     double Sout[9] = nullptr)
 {
     // forward decisions
     int modify = 0;
@@ -401,8 +405,12 @@ DGR_HD int limit_adj(const double S[9], const double Vmin[3][3], const double Vm
             if (Ha > H0 && Hb_ > H0 && Hc > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
         }
         for (int k = 0; k < 3; ++k) {
+            // at an exact tie the element's own value is taken as the active
+            // branch: the clip is then inactive, as it is in both directions
+            // of a central difference when several elements share the bound
+            // (lake at rest), and no adjoint is routed through a tied bound
             const double lo = (V[k] < Vmax[iv][k]) ? V[k] : Vmax[iv][k];
-            int sel = (V[k] < Vmax[iv][k]) ? 0 : 1;
+            int sel = (V[k] <= Vmax[iv][k]) ? 0 : 1;
             if (Vmin[iv][k] > lo) sel = 2;
             clampsel[iv][k] = sel;
             V[k] = fmax(lo, Vmin[iv][k]);
@@ -434,7 +442,15 @@ DGR_HD int limit_adj(const double S[9], const double Vmin[3][3], const double Vm
             if (Ha < H0 || Hb_ < H0 || Hc < H0) modify = 0;
         }
     }
-    (void)Vfin;
+    if (Sout) {
+        for (int i = 0; i < 9; ++i) Sout[i] = S[i];
+        if (modify)
+            for (int iv = 0; iv < 3; ++iv) {
+                Sout[3 * iv] = (Vfin[iv][0] + Vfin[iv][1]) / 2;
+                Sout[3 * iv + 1] = (Vfin[iv][1] + Vfin[iv][2]) / 2;
+                Sout[3 * iv + 2] = (Vfin[iv][2] + Vfin[iv][0]) / 2;
+            }
+    }
     if (!modify) {
         for (int i = 0; i < 9; ++i) Sin_b[i] += Sb[i];
         return 0;

@@ -1,0 +1,273 @@
+// dgr_adjoint_api.hip -- C ABI of the gradient: a gauge-misfit loss over a
+// forward run and its adjoint with respect to nodal depth z and Manning n
+// (per member), by checkpointing and recomputing the forward.
+//
+// Forward: nsteps steps from the handle's state, a checkpoint (state, wet/dry
+// flags, control block) every `checkpoint_every` steps, the gauge depths
+// sampled after every `sample_every`-th step.  Reverse: for each segment from
+// the last, restore its checkpoint, recompute its steps with a tape (each
+// step's start state, the two stage outputs before and after the limiter and
+// the wet/dry flags), then run the reverse stages step by step back to the
+// segment start.  The same kernels recompute the same bits, so the reverse
+// sees the forward pass's active sets exactly.  The handle's state is
+// restored to the start state on return.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <vector>
+
+#include "dgr_handle.h"
+
+namespace {
+
+struct AdjBuffers {
+    std::vector<void *> owned;
+    template <typename T>
+    hipError_t alloc(T **p, size_t n)
+    {
+        hipError_t e = dalloc(p, n);
+        if (e == hipSuccess) owned.push_back((void *)*p);
+        return e;
+    }
+    ~AdjBuffers()
+    {
+        for (void *p : owned) (void)hipFree(p);
+    }
+};
+
+}  // namespace
+
+extern "C" dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every, int ngauges, const int *gauge_el,
+                                    double *h_out)
+{
+    if (!h || nsteps < 0 || sample_every <= 0 || ngauges <= 0 || !gauge_el || !h_out) return DGR_ERR_ARG;
+    for (int mi = 0; mi < h->nmem; ++mi)
+        for (int g = 0; g < ngauges; ++g) {
+            const int e = gauge_el[mi * ngauges + g];
+            if (e < 0 || e >= h->mem[mi].ne) return DGR_ERR_ARG;
+        }
+    const int nsamp = nsteps / sample_every;
+    const size_t per = (size_t)h->nmem * ngauges;
+    AdjBuffers B;
+    int *d_gel = nullptr;
+    double *d_h = nullptr;
+    DGR_HIP(B.alloc(&d_gel, per));
+    DGR_HIP(B.alloc(&d_h, per * (nsamp > 0 ? nsamp : 1)));
+    DGR_HIP(hipMemcpyAsync(d_gel, gauge_el, per * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    for (int n = 0; n < nsteps; ++n) {
+        enqueue_step(h, nullptr);
+        if ((n + 1) % sample_every == 0) {
+            const int k = (n + 1) / sample_every - 1;
+            launch_gauge_sample(h->m, ngauges, d_gel, h->d_w, d_h + (size_t)k * per, nullptr, nullptr, h->stream);
+        }
+    }
+    DGR_HIP(hipGetLastError());
+    std::vector<double> hv(per * (nsamp > 0 ? nsamp : 1));
+    DGR_HIP(hipMemcpyAsync(hv.data(), d_h, hv.size() * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    // [member][sample][gauge]
+    for (int mi = 0; mi < h->nmem; ++mi)
+        for (int k = 0; k < nsamp; ++k)
+            for (int g = 0; g < ngauges; ++g)
+                h_out[((size_t)mi * nsamp + k) * ngauges + g] = hv[(size_t)k * per + (size_t)mi * ngauges + g];
+    return any_error(h);
+}
+
+extern "C" dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, const double *n)
+{
+    if (!h || member < 0 || member >= h->nmem || (!z && !n)) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[member];
+    std::vector<double4> nodes(M.nn);
+    DGR_HIP(hipMemcpyAsync(nodes.data(), h->d_nodes + M.nd_off, M.nn * sizeof(double4), hipMemcpyDeviceToHost,
+                           h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    for (int i = 0; i < M.nn; ++i) {
+        if (z) nodes[i].z = z[i];
+        if (n) nodes[i].w = n[i];
+    }
+    DGR_HIP(hipMemcpyAsync(h->d_nodes + M.nd_off, nodes.data(), M.nn * sizeof(double4), hipMemcpyHostToDevice,
+                           h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *sp, double *loss, double *grad_z,
+                                         double *grad_n)
+{
+    if (!h || !sp || !loss || sp->nsteps <= 0 || sp->sample_every <= 0 || sp->checkpoint_every <= 0 ||
+        sp->ngauges <= 0 || !sp->gauge_el || !sp->obs)
+        return DGR_ERR_ARG;
+    const int N = sp->nsteps, K = sp->checkpoint_every, ng = sp->ngauges, nm = h->nmem;
+    for (int mi = 0; mi < nm; ++mi)
+        for (int g = 0; g < ng; ++g) {
+            const int e = sp->gauge_el[mi * ng + g];
+            if (e < 0 || e >= h->mem[mi].ne) return DGR_ERR_ARG;
+        }
+    if (h->profiling) return DGR_ERR_ARG;
+    const int nseg = (N + K - 1) / K, nsamp = N / sp->sample_every;
+    const size_t P = h->pitch, W9 = 9 * P, nn = h->nn, per = (size_t)nm * ng;
+    const int maxparts = h->ph.maxparts;
+    hipStream_t s = h->stream;
+    AdjBuffers B;
+    // checkpoints, step records, gauges
+    double *ckW;
+    int *ckWD;
+    Ctrl *ckC;
+    StepRec *rec;
+    int *d_gel;
+    double *d_obs, *d_hv;
+    AdjCtrl *actl;
+    DGR_HIP(B.alloc(&ckW, W9 * nseg));
+    DGR_HIP(B.alloc(&ckWD, P * nseg));
+    DGR_HIP(B.alloc(&ckC, (size_t)nm * nseg));
+    DGR_HIP(B.alloc(&rec, (size_t)nm * N));
+    DGR_HIP(B.alloc(&d_gel, per));
+    DGR_HIP(B.alloc(&d_obs, per * (nsamp > 0 ? nsamp : 1)));
+    DGR_HIP(B.alloc(&d_hv, per * (nsamp > 0 ? nsamp : 1)));
+    DGR_HIP(B.alloc(&actl, nm));
+    // tape of one segment
+    double *tw0, *tu1, *tp1, *tu2;
+    int *twa, *twb;
+    DGR_HIP(B.alloc(&tw0, W9 * K));
+    DGR_HIP(B.alloc(&tu1, W9 * K));
+    DGR_HIP(B.alloc(&tp1, W9 * K));
+    DGR_HIP(B.alloc(&tu2, W9 * K));
+    DGR_HIP(B.alloc(&twa, P * K));
+    DGR_HIP(B.alloc(&twb, P * K));
+    // reverse buffers
+    double *wb, *wb0, *pb, *ub, *fhb, *slotb, *zbe, *vmb, *nodeb, *nodemm, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
+    int *nodearg;
+    DGR_HIP(B.alloc(&wb, W9));
+    DGR_HIP(B.alloc(&wb0, W9));
+    DGR_HIP(B.alloc(&pb, W9));
+    DGR_HIP(B.alloc(&ub, W9));
+    DGR_HIP(B.alloc(&fhb, W9));
+    DGR_HIP(B.alloc(&slotb, W9));
+    DGR_HIP(B.alloc(&zbe, 3 * P));
+    DGR_HIP(B.alloc(&vmb, 18 * P));
+    DGR_HIP(B.alloc(&nodeb, 6 * nn));
+    DGR_HIP(B.alloc(&nodemm, 6 * nn));
+    DGR_HIP(B.alloc(&nodearg, 6 * nn));
+    DGR_HIP(B.alloc(&zvb, 3 * P));
+    DGR_HIP(B.alloc(&nvb, 3 * P));
+    DGR_HIP(B.alloc(&dtp, 2 * (size_t)maxparts));
+    DGR_HIP(B.alloc(&tdp, 2 * (size_t)maxparts));
+    DGR_HIP(B.alloc(&gz, nn));
+    DGR_HIP(B.alloc(&gn, nn));
+    DGR_HIP(hipMemcpyAsync(d_gel, sp->gauge_el, per * sizeof(int), hipMemcpyHostToDevice, s));
+    {
+        // obs arrive [member][sample][gauge]; device keeps [sample][member][gauge]
+        std::vector<double> o(per * (nsamp > 0 ? nsamp : 1));
+        for (int mi = 0; mi < nm; ++mi)
+            for (int k = 0; k < nsamp; ++k)
+                for (int g = 0; g < ng; ++g)
+                    o[(size_t)k * per + (size_t)mi * ng + g] = sp->obs[((size_t)mi * nsamp + k) * ng + g];
+        DGR_HIP(hipMemcpyAsync(d_obs, o.data(), o.size() * sizeof(double), hipMemcpyHostToDevice, s));
+        DGR_HIP(hipMemsetAsync(actl, 0, nm * sizeof(AdjCtrl), s));
+        DGR_HIP(hipStreamSynchronize(s));
+    }
+    const size_t Wb = W9 * sizeof(double), Ib = P * sizeof(int), Cb = (size_t)nm * sizeof(Ctrl);
+
+    // forward with checkpoints
+    for (int n = 0; n < N; ++n) {
+        if (n % K == 0) {
+            const int c = n / K;
+            DGR_HIP(hipMemcpyAsync(ckW + W9 * c, h->d_w, Wb, hipMemcpyDeviceToDevice, s));
+            DGR_HIP(hipMemcpyAsync(ckWD + P * c, h->d_wd, Ib, hipMemcpyDeviceToDevice, s));
+            DGR_HIP(hipMemcpyAsync(ckC + (size_t)nm * c, h->d_ctrl, Cb, hipMemcpyDeviceToDevice, s));
+        }
+        launch_step_begin(h->m, h->d_ctrl, s);
+        launch_step_record(h->m, h->d_ctrl, rec + (size_t)nm * n, 0, s);
+        enqueue_stages(h, nullptr);
+        launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], s);
+        launch_step_record(h->m, h->d_ctrl, rec + (size_t)nm * n, 1, s);
+        if ((n + 1) % sp->sample_every == 0) {
+            const int k = (n + 1) / sp->sample_every - 1;
+            launch_gauge_sample(h->m, ng, d_gel, h->d_w, d_hv + (size_t)k * per, d_obs + (size_t)k * per, actl, s);
+        }
+    }
+    DGR_HIP(hipGetLastError());
+    dgr_status fst = any_error(h);
+    auto restore = [&](int c) -> dgr_status {
+        DGR_HIP(hipMemcpyAsync(h->d_w, ckW + W9 * c, Wb, hipMemcpyDeviceToDevice, s));
+        DGR_HIP(hipMemcpyAsync(h->d_wd, ckWD + P * c, Ib, hipMemcpyDeviceToDevice, s));
+        DGR_HIP(hipMemcpyAsync(h->d_ctrl, ckC + (size_t)nm * c, Cb, hipMemcpyDeviceToDevice, s));
+        return DGR_OK;
+    };
+    if (fst != DGR_OK) {
+        restore(0);
+        DGR_HIP(hipStreamSynchronize(s));
+        return fst;
+    }
+
+    // reverse
+    DGR_HIP(hipMemsetAsync(wb, 0, Wb, s));
+    DGR_HIP(hipMemsetAsync(zvb, 0, 3 * P * sizeof(double), s));
+    DGR_HIP(hipMemsetAsync(nvb, 0, 3 * P * sizeof(double), s));
+    for (int c = nseg - 1; c >= 0; --c) {
+        const int n0 = c * K, n1 = (n0 + K < N) ? n0 + K : N;
+        dgr_status st = restore(c);
+        if (st) return st;
+        for (int n = n0; n < n1; ++n) {
+            const int k = n - n0;
+            const StepTape tp = {tw0 + W9 * k, tu1 + W9 * k, tp1 + W9 * k, tu2 + W9 * k, twa + P * k, twb + P * k};
+            enqueue_step(h, &tp);
+        }
+        for (int n = n1 - 1; n >= n0; --n) {
+            const int k = n - n0;
+            double *w0 = tw0 + W9 * k, *u1 = tu1 + W9 * k, *p1 = tp1 + W9 * k, *u2 = tu2 + W9 * k;
+            int *wda = twa + P * k, *wdb = twb + P * k;
+            const StepRec *rn = rec + (size_t)nm * n;
+            if ((n + 1) % sp->sample_every == 0) {
+                const int j = (n + 1) / sp->sample_every - 1;
+                launch_adj_gauges(h->m, ng, d_gel, d_hv + (size_t)j * per, d_obs + (size_t)j * per, wb, zvb, actl, s);
+            }
+            DGR_HIP(hipMemsetAsync(wb0, 0, Wb, s));
+            // stage 2: input p1, base w0, output u2 -> limited -> w^{n+1} (adjoint wb)
+            launch_node_arg(h->m, u2, nodemm, nodearg, s);
+            launch_adj_limit(h->m, h->ph, u2, nodemm, wb, ub, vmb, actl, s);
+            launch_adj_node(h->m, vmb, nodeb, s);
+            launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, nodearg, wdb, rn, pb, wb0, fhb, zvb, nvb,
+                           dtp + maxparts, s);
+            launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slotb, zbe, tdp + maxparts,
+                            n + 1 < N, s);
+            launch_adj_combine(h->m, pb, slotb, zbe, zvb, s);
+            // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb)
+            launch_node_arg(h->m, u1, nodemm, nodearg, s);
+            launch_adj_limit(h->m, h->ph, u1, nodemm, pb, ub, vmb, actl, s);
+            launch_adj_node(h->m, vmb, nodeb, s);
+            launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nodeb, nodearg, wda, rn, nullptr, wb0, fhb, zvb, nvb,
+                           dtp, s);
+            launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slotb, zbe, tdp, false, s);
+            launch_adj_combine(h->m, wb0, slotb, zbe, zvb, s);
+            launch_adj_scalar(h->m, dtp, tdp, maxparts, rn, n > 0 ? rn - nm : rn, n > 0, actl, s);
+            double *t = wb;
+            wb = wb0;
+            wb0 = t;
+        }
+        DGR_HIP(hipGetLastError());
+    }
+    launch_adj_gather(h->m, zvb, nvb, gz, gn, s);
+    DGR_HIP(hipGetLastError());
+    dgr_status st = restore(0);
+    if (st) return st;
+    std::vector<AdjCtrl> a(nm);
+    DGR_HIP(hipMemcpyAsync(a.data(), actl, nm * sizeof(AdjCtrl), hipMemcpyDeviceToHost, s));
+    std::vector<double> hz(nn), hn(nn);
+    DGR_HIP(hipMemcpyAsync(hz.data(), gz, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    DGR_HIP(hipMemcpyAsync(hn.data(), gn, nn * sizeof(double), hipMemcpyDeviceToHost, s));
+    DGR_HIP(hipStreamSynchronize(s));
+    int unsupported = 0;
+    size_t o = 0;
+    for (int mi = 0; mi < nm; ++mi) {
+        loss[mi] = a[mi].loss;
+        unsupported |= a[mi].unsupported;
+        const MemberHost &M = h->mem[mi];
+        for (int i = 0; i < M.nn; ++i) {
+            if (grad_z) grad_z[o + i] = hz[M.nd_off + i];
+            if (grad_n) grad_n[o + i] = hn[M.nd_off + i];
+        }
+        o += M.nn;
+    }
+    return unsupported ? DGR_ERR_UNSUPPORTED : DGR_OK;
+}

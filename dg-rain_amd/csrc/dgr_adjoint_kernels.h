@@ -1,0 +1,705 @@
+// dgr_adjoint_kernels.h -- the reverse sweep of the RK step on the GPU.
+// Included by dgr_kernels.hip inside namespace dgr: the kernels re-evaluate
+// the forward with its device functions (geometry_of, reflect, the member
+// mapping) and differentiate with dgr_adjoint.h.
+//
+// One reverse RK stage, for every member at once (element kernels on the
+// element workgroups, edge kernels on the edge workgroups):
+//   k_node_arg      node min/max of the stage output's element means and the
+//                   element that attains each (the limiter's active bounds)
+//   k_adj_limit     limiter pass reverse: adjoint of the pre-limit output
+//                   (own part) and of the node bounds at the element's vertices
+//   k_adj_node      node sums of the bound adjoints
+//   k_adj_rhs       bound adjoints -> element means -> stage output; RK update
+//                   and element RHS reverse: adjoint of the stage input, of
+//                   Fhat, of dt (per-workgroup partial) and of nodal z, n
+//   k_adj_edge      Roe flux reverse per edge, written to the two element
+//                   slots the edge feeds (each slot has exactly one edge, so
+//                   no atomics): adjoint of the stage input at the edge
+//                   midpoints, of z_edge, of the tide value, of max |lambda|
+//   k_adj_combine   adds the edge part to the stage input adjoint
+// Then k_adj_scalar turns the dt and tide adjoints into the previous step's
+// max-|lambda| adjoint (dt = cfl * minEdgLength / max|lambda|).
+#pragma once
+
+// ---------------------------------------------------------------------------
+// node min/max (SlopeLimiter2D.c:106-153) of the element means of U, and the
+// first element in ascending order that attains each (-1: the sentinel stood)
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(256) void k_node_arg(MeshDev m, const double *__restrict__ U, double *__restrict__ nodemm,
+                                                  int *__restrict__ nodearg)
+{
+    const int i = xcd_block() * 256 + (int)threadIdx.x;
+    if (i >= m.nn) return;
+    const int P = m.pitch;
+    double mn[3] = {99999, 99999, 99999}, mx[3] = {-99999, -99999, -99999};
+    int amn[3] = {-1, -1, -1}, amx[3] = {-1, -1, -1};
+    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
+    for (int j = b; j < en; ++j) {
+        const int el = m.n2e[j];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const double a = (U[(3 * f) * P + el] + U[(3 * f + 1) * P + el] + U[(3 * f + 2) * P + el]) / 3;
+            if (a < mn[f]) { mn[f] = a; amn[f] = el; }
+            if (a > mx[f]) { mx[f] = a; amx[f] = el; }
+        }
+    }
+    const int nn = m.nn;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        nodemm[f * nn + i] = mn[f];
+        nodemm[(3 + f) * nn + i] = mx[f];
+        nodearg[f * nn + i] = amn[f];
+        nodearg[(3 + f) * nn + i] = amx[f];
+    }
+}
+
+// limiter reverse.  U: pre-limit stage output; Sb: adjoint of the limited
+// (and PDop'd) output.  Writes Ub (own part of the pre-limit adjoint) and
+// vmb[18][pitch]: adjoint of Vmin (f*3+k) and Vmax (9+f*3+k) at vertex k.
+// PDop is the identity when every vertex depth exceeds H0; an element where
+// it is not raises the member's `unsupported` flag.
+__global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const double *__restrict__ U,
+                                                   const double *__restrict__ nodemm, const double *__restrict__ Sb,
+                                                   double *__restrict__ Ub, double *__restrict__ vmb, AdjCtrl *actl)
+{
+    const ElemLane L = elem_lane(m);
+    if (!L.valid) return;
+    const int e = L.e, P = m.pitch, nn = m.nn;
+    int nd[3];
+    double zv[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
+    double S[9], sb[9], ub[9], Vmin[3][3], Vmax[3][3], Vminb[3][3], Vmaxb[3][3];
+#pragma unroll
+    for (int f = 0; f < 9; ++f) { S[f] = U[f * P + e]; sb[f] = Sb[f * P + e]; ub[f] = 0.0; }
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Vmin[f][k] = nodemm[f * nn + nd[k]];
+            Vmax[f][k] = nodemm[(3 + f) * nn + nd[k]];
+            Vminb[f][k] = 0.0;
+            Vmaxb[f][k] = 0.0;
+        }
+    double so[9];
+    limit_adj(S, Vmin, Vmax, zv, ph.h0, sb, ub, Vminb, Vmaxb, so);
+    if (ph.wetdry) {
+        // PDop2D (wetDry2D.c:66-256) leaves an element unchanged iff every
+        // vertex depth it forms from the limited state exceeds H0
+        const double bz0 = 0.5 * (zv[0] + zv[1]), bz1 = 0.5 * (zv[1] + zv[2]), bz2 = 0.5 * (zv[2] + zv[0]);
+        const double mh0 = so[0] + bz0, mh1 = so[1] + bz1, mh2 = so[2] + bz2;
+        const double H0 = ph.h0;
+        if (mh0 - mh1 + mh2 <= H0 || mh0 + mh1 - mh2 <= H0 || -mh0 + mh1 + mh2 <= H0) actl[L.mid].unsupported = 1;
+    }
+#pragma unroll
+    for (int f = 0; f < 9; ++f) Ub[f * P + e] = ub[f];
+#pragma unroll
+    for (int f = 0; f < 3; ++f)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            vmb[(f * 3 + k) * P + e] = Vminb[f][k];
+            vmb[(9 + f * 3 + k) * P + e] = Vmaxb[f][k];
+        }
+}
+
+// node sums of the bound adjoints: nodeb[f][i] (min) and nodeb[3+f][i] (max)
+__global__ __launch_bounds__(256) void k_adj_node(MeshDev m, const double *__restrict__ vmb, double *__restrict__ nodeb)
+{
+    const int i = xcd_block() * 256 + (int)threadIdx.x;
+    if (i >= m.nn) return;
+    const int P = m.pitch, nn = m.nn;
+    double s[6] = {0, 0, 0, 0, 0, 0};
+    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
+    for (int j = b; j < en; ++j) {
+        const int el = m.n2e[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (m.ev[k * P + el] == i) {
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    s[f] += vmb[(f * 3 + k) * P + el];
+                    s[3 + f] += vmb[(9 + f * 3 + k) * P + el];
+                }
+            }
+    }
+#pragma unroll
+    for (int f = 0; f < 6; ++f) nodeb[f * nn + i] = s[f];
+}
+
+// sum over a workgroup of 256 threads, valid in thread 0
+__device__ __forceinline__ double block_sum(double v)
+{
+    __shared__ double red[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = v;
+    __syncthreads();
+    return red[0] + red[1] + red[2] + red[3];
+}
+
+// RK update + element RHS reverse.
+//   MODE 1: U = W + R dt             -> Wb += Ub + RHS'(W)^T (dt Ub)
+//   MODE 2: U = (W + B) 0.5 + R dt/2 -> Inb = 0.5 Ub + RHS'(W)^T (dt/2 Ub), Bb += 0.5 Ub
+// W: stage input, B: step-start state (MODE 2), U: pre-limit output, Ubown:
+// own part of the pre-limit adjoint; the node-bound part is added here from
+// nodeb for the elements that attain a bound.  Outputs: Inb (MODE 2: the
+// stage-input adjoint, written; MODE 1: added to Bb, which is the step-start
+// adjoint), Fhb (adjoint of Fhat), zvb/nvb (nodal z/n adjoints per element
+// vertex, added), dtparts (this workgroup's dt adjoint).
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const double *__restrict__ W,
+                                                    const double *__restrict__ B, const double *__restrict__ U,
+                                                    const double *__restrict__ Ubown, const double *__restrict__ nodeb,
+                                                    const int *__restrict__ nodearg, const int *__restrict__ wd,
+                                                    const StepRec *__restrict__ rec, double *Inb, double *Bb,
+                                                    double *__restrict__ Fhb, double *__restrict__ zvb,
+                                                    double *__restrict__ nvb, double *__restrict__ dtparts)
+{
+    const ElemLane L = elem_lane(m);
+    double dtb = 0.0;
+    if (L.valid) {
+        const int e = L.e, P = m.pitch, nn = m.nn;
+        const double dt = rec[L.mid].dt;
+        double w[9], u[9], ub[9];
+#pragma unroll
+        for (int f = 0; f < 9; ++f) { w[f] = W[f * P + e]; u[f] = U[f * P + e]; ub[f] = Ubown[f * P + e]; }
+        // the element mean enters the node bounds it attains
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int nd = m.ev[k * P + e];
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                double add = 0.0;
+                if (nodearg[f * nn + nd] == e) add += nodeb[f * nn + nd];
+                if (nodearg[(3 + f) * nn + nd] == e) add += nodeb[(3 + f) * nn + nd];
+                add *= 1.0 / 3.0;
+                ub[3 * f] += add;
+                ub[3 * f + 1] += add;
+                ub[3 * f + 2] += add;
+            }
+        }
+        double Rb[9], R[9];
+        if (MODE == 1) {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) { R[f] = (u[f] - w[f]) / dt; Rb[f] = dt * ub[f]; dtb += R[f] * ub[f]; }
+        } else {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                const double bf = B[f * P + e];
+                R[f] = (u[f] - (w[f] + bf) * 0.5) * 2.0 / dt;
+                Rb[f] = 0.5 * dt * ub[f];
+                dtb += 0.5 * R[f] * ub[f];
+            }
+        }
+        GeoRegs G;
+        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+        double Wb[9], Fb[9], mzb[3] = {0, 0, 0}, gnnb[3] = {0, 0, 0}, bgxb = 0.0, bgyb = 0.0;
+#pragma unroll
+        for (int f = 0; f < 9; ++f) { Wb[f] = 0.0; Fb[f] = 0.0; }
+        element_rhs_adj(G, w, wd[e], ph.g, Rb, Wb, Fb, mzb, gnnb, bgxb, bgyb);
+        if (MODE == 1) {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) Bb[f * P + e] += ub[f] + Wb[f];
+        } else {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) {
+                Inb[f * P + e] = 0.5 * ub[f] + Wb[f];
+                Bb[f * P + e] += 0.5 * ub[f];
+            }
+        }
+#pragma unroll
+        for (int f = 0; f < 9; ++f) Fhb[f * P + e] = Fb[f];
+        // geometry -> nodal z and n.  bgx = g |jac| weight sum_k mz_k gP_k,x
+        const double cbed = ph.g * G(GX_AJ) * (1.0 / 6);
+#pragma unroll
+        for (int k = 0; k < 3; ++k) mzb[k] += cbed * (G(GX_GP + 2 * k) * bgxb + G(GX_GP + 2 * k + 1) * bgyb);
+        double4 Pv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Pv[k] = m.nodes[m.ev[k * P + e]];
+        double zb_[3] = {0, 0, 0}, nb_[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const int k1 = (k + 1) % 3;
+            // mz_k = 0.5 (z_k + z_k1), gnn_k = g mn_k^2, mn_k = 0.5 (n_k + n_k1)
+            zb_[k] += 0.5 * mzb[k];
+            zb_[k1] += 0.5 * mzb[k];
+            const double mn = 0.5 * (Pv[k].w + Pv[k1].w);
+            nb_[k] += ph.g * mn * gnnb[k];
+            nb_[k1] += ph.g * mn * gnnb[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { zvb[k * P + e] += zb_[k]; nvb[k * P + e] += nb_[k]; }
+    }
+    const double s = block_sum(dtb);
+    if (threadIdx.x == 0) dtparts[L.lb] = s;
+}
+
+// wall reflection (compute2DL.c:179-189) as a 2x2 map r = A q
+__device__ __forceinline__ void reflect_matrix(double nx, double ny, double dn, double A[2][2])
+{
+    const double tx = -ny, ty = nx;
+    // rx = (ty*(-qn) - ny*qt) dn, ry = (-tx*(-qn) + nx*qt) dn, qn = q.n, qt = q.t
+    A[0][0] = (-ty * nx - ny * tx) * dn;
+    A[0][1] = (-ty * ny - ny * ty) * dn;
+    A[1][0] = (tx * nx + nx * tx) * dn;
+    A[1][1] = (tx * ny + nx * ty) * dn;
+}
+
+// the edge's inputs, as the forward builds them (compute2DL.c:98-199)
+struct EdgeState {
+    int el_in, el_ex, kin, kex, inf, code;
+    bool boundary, prescribed, tide, zex_from_in, qn_from_in;
+    double nx, ny, dn, zb;
+    double zin, qxin, qyin, zex, qxex, qyex;
+};
+
+__device__ __forceinline__ EdgeState edge_state(const MeshDev &m, const MemberDev &M, const StepRec &rc,
+                                                const double *__restrict__ win, const double *__restrict__ bzeta,
+                                                const double *__restrict__ bqn, int ed)
+{
+    EdgeState s;
+    const int P = m.pitch;
+    const int4 E = m.edges[ed];
+    s.inf = m.einfo[ed];
+    s.el_in = E.x;
+    s.el_ex = E.y;
+    s.kin = s.inf & ED_KIN_MASK;
+    s.kex = (s.inf >> ED_KEX_SHIFT) & 3;
+    s.boundary = (s.inf & EI_BOUNDARY) != 0;
+    s.prescribed = (s.inf & EI_PRESCRIBED) != 0;
+    s.code = (s.inf >> EI_CODE_SHIFT) & 0xff;
+    const double4 A = m.nodes[E.z], B = m.nodes[E.w];
+    const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+    const double sg = (double)(((s.inf >> EI_SGN_SHIFT) & 3) - 1);
+    s.nx = sg * (B.y - A.y) / len;
+    s.ny = sg * (A.x - B.x) / len;
+    const double tx = -s.ny, ty = s.nx;
+    s.zb = (A.z + B.z) / 2;
+    s.dn = 1. / (s.nx * ty - s.ny * tx);
+    s.zin = win[s.kin * P + s.el_in];
+    s.qxin = win[(3 + s.kin) * P + s.el_in];
+    s.qyin = win[(6 + s.kin) * P + s.el_in];
+    s.tide = false;
+    s.zex_from_in = false;
+    s.qn_from_in = false;
+    if (s.boundary) { s.zex = s.zin; s.qxex = s.qxin; s.qyex = s.qyin; }
+    else {
+        s.zex = win[s.kex * P + s.el_ex];
+        s.qxex = win[(3 + s.kex) * P + s.el_ex];
+        s.qyex = win[(6 + s.kex) * P + s.el_ex];
+    }
+    if (s.prescribed) {
+        s.tide = M.tide_code != 0 && s.code == M.tide_code;
+        s.zex = s.tide ? rc.tide : bzeta[ed];
+        double qnex = s.tide ? 1000000.0 : bqn[ed];
+        if (s.zex < 0) { s.zex = s.zin; s.zex_from_in = true; }
+        if (qnex > 1000) { qnex = s.qxin * s.nx + s.qyin * s.ny; s.qn_from_in = true; }
+        const double qtex = s.qxin * tx + s.qyin * ty;
+        s.qxex = (ty * qnex - s.ny * qtex) * s.dn;
+        s.qyex = (-tx * qnex + s.nx * qtex) * s.dn;
+    } else if (s.boundary) {
+        reflect(s.qxin, s.qyin, s.nx, s.ny, s.dn, s.qxex, s.qyex);
+        s.zex_from_in = true;
+    }
+    return s;
+}
+
+// the |lambda| operands of an edge (its own positions 2*edge, 2*edge+1)
+__device__ __forceinline__ int edge_lams(const EdgeState &s, const int *__restrict__ wd, bool wetdry, double g,
+                                         double lam[2])
+{
+    double F[3];
+    RoeIn a;
+    if (wetdry) {
+        const int wd_in = wd[s.el_in], wd_ex = s.boundary ? wd_in : wd[s.el_ex];
+        if (wd_in == 0 && wd_ex == 0) {
+            double rx, ry, sx, sy;
+            reflect(s.qxin, s.qyin, s.nx, s.ny, s.dn, rx, ry);
+            a = {s.zin, s.zin, s.qxin, rx, s.qyin, ry, s.zb};
+            lam[0] = roe_eval(F, a, s.nx, s.ny, 0.0, g);
+            reflect(s.qxex, s.qyex, s.nx, s.ny, s.dn, sx, sy);
+            a = {s.zex, s.zex, sx, s.qxex, sy, s.qyex, s.zb};
+            lam[1] = roe_eval(F, a, s.nx, s.ny, 0.0, g);
+            return 2;
+        }
+    }
+    a = {s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
+    lam[0] = roe_eval(F, a, s.nx, s.ny, g, g);
+    return 1;
+}
+
+// count per member the edges whose |lambda| operand equals the recorded max
+__global__ __launch_bounds__(256) void k_adj_ties(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                                  const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                                  const int *__restrict__ wd, const StepRec *__restrict__ rec_lam,
+                                                  const StepRec *__restrict__ rec, AdjCtrl *actl)
+{
+    const int lb = xcd_block();
+    const int mid = m.dblk_mem[lb];
+    const MemberDev &M = m.mem[mid];
+    const int ed = lb * 256 + (int)threadIdx.x;
+    if (ed >= M.ed_off + M.ned) return;
+    if (actl[mid].lamb == 0.0) return;
+    const EdgeState s = edge_state(m, M, rec[mid], win, bzeta, bqn, ed);
+    double lam[2];
+    const int n = edge_lams(s, wd, ph.wetdry != 0, ph.g, lam);
+    const double target = __longlong_as_double((long long)rec_lam[mid].lam2);
+    int c = 0;
+    for (int i = 0; i < n; ++i) c += lam[i] == target;
+    if (c) atomicAdd(&actl[mid].ntie, c);
+}
+
+// Roe flux reverse per edge.  W: stage input; Fhb: adjoint of the Fhat slots.
+// Writes slotb[9][pitch] (adjoint of the stage input at the two slots the
+// edge feeds; a boundary edge writes its one slot) and zbe[3][pitch] (the
+// z_edge adjoint in el_in's slot, 0 in el_ex's), and tideparts (workgroup sum
+// of the tide value's adjoint).  With lamb != 0 the edges whose |lambda|
+// operand equals the recorded max (rec_lam) share lamb.
+template <bool WETDRY>
+__global__ __launch_bounds__(256) void k_adj_edge(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                                  const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                                  const int *__restrict__ wd, const double *__restrict__ Fhb,
+                                                  const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_lam,
+                                                  const AdjCtrl *__restrict__ actl, int use_lambda,
+                                                  double *__restrict__ slotb, double *__restrict__ zbe,
+                                                  double *__restrict__ tideparts)
+{
+    const int lb = xcd_block();
+    const int mid = m.dblk_mem[lb];
+    const MemberDev &M = m.mem[mid];
+    const int ed = lb * 256 + (int)threadIdx.x;
+    double tideb = 0.0;
+    if (ed < M.ed_off + M.ned) {
+        const int P = m.pitch;
+        const double g = ph.g;
+        const EdgeState s = edge_state(m, M, rec[mid], win, bzeta, bqn, ed);
+        double lam_share = 0.0;
+        const AdjCtrl &ac = actl[mid];
+        if (use_lambda && ac.lamb != 0.0 && ac.ntie > 0) lam_share = ac.lamb / ac.ntie;
+        const double target = __longlong_as_double((long long)rec_lam[mid].lam2);
+        double Fin_b[3], Fex_b[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            Fin_b[f] = Fhb[(f * 3 + s.kin) * P + s.el_in];
+            Fex_b[f] = s.boundary ? 0.0 : -Fhb[(f * 3 + s.kex) * P + s.el_ex];
+        }
+        // adjoints of the edge-level inputs
+        double zinb = 0, qxinb = 0, qyinb = 0, zexb = 0, qxexb = 0, qyexb = 0, zbb = 0;
+        double Ar[2][2];
+        reflect_matrix(s.nx, s.ny, s.dn, Ar);
+        bool done = false;
+        if (WETDRY) {
+            const int wd_in = wd[s.el_in], wd_ex = s.boundary ? wd_in : wd[s.el_ex];
+            if (wd_in == 0 && wd_ex == 0) {
+                double rx, ry, sx, sy, F[3];
+                reflect(s.qxin, s.qyin, s.nx, s.ny, s.dn, rx, ry);
+                RoeIn a = {s.zin, s.zin, s.qxin, rx, s.qyin, ry, s.zb};
+                double l0 = roe_eval(F, a, s.nx, s.ny, 0.0, g);
+                RoeIn ib = {0, 0, 0, 0, 0, 0, 0};
+                roe_adj(a, s.nx, s.ny, 0.0, g, Fin_b, l0 == target ? lam_share : 0.0, ib);
+                zinb += ib.zi + ib.ze;
+                qxinb += ib.qxi + Ar[0][0] * ib.qxe + Ar[1][0] * ib.qye;
+                qyinb += ib.qyi + Ar[0][1] * ib.qxe + Ar[1][1] * ib.qye;
+                zbb += ib.zb;
+                reflect(s.qxex, s.qyex, s.nx, s.ny, s.dn, sx, sy);
+                RoeIn c = {s.zex, s.zex, sx, s.qxex, sy, s.qyex, s.zb};
+                const double l1 = roe_eval(F, c, s.nx, s.ny, 0.0, g);
+                RoeIn ic = {0, 0, 0, 0, 0, 0, 0};
+                roe_adj(c, s.nx, s.ny, 0.0, g, Fex_b, l1 == target ? lam_share : 0.0, ic);
+                zexb += ic.zi + ic.ze;
+                qxexb += ic.qxe + Ar[0][0] * ic.qxi + Ar[1][0] * ic.qyi;
+                qyexb += ic.qye + Ar[0][1] * ic.qxi + Ar[1][1] * ic.qyi;
+                zbb += ic.zb;
+                done = true;
+            } else {
+                const bool in_wet = wd_in == 1, ex_wet = s.boundary ? in_wet : (wd_ex == 1);
+                if (!(in_wet && ex_wet)) {
+                    // a side on the dry flux (lg = 0)
+                    const RoeIn a = {s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
+                    double F[3];
+                    const double l0 = roe_eval(F, a, s.nx, s.ny, g, g);
+                    double Fg_b[3], F0_b[3];
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) {
+                        Fg_b[f] = (in_wet ? Fin_b[f] : 0.0) + (ex_wet ? Fex_b[f] : 0.0);
+                        F0_b[f] = (in_wet ? 0.0 : Fin_b[f]) + (ex_wet ? 0.0 : Fex_b[f]);
+                    }
+                    RoeIn ib = {0, 0, 0, 0, 0, 0, 0};
+                    roe_adj(a, s.nx, s.ny, g, g, Fg_b, l0 == target ? lam_share : 0.0, ib);
+                    roe_adj(a, s.nx, s.ny, 0.0, g, F0_b, 0.0, ib);
+                    zinb += ib.zi; zexb += ib.ze; qxinb += ib.qxi; qxexb += ib.qxe;
+                    qyinb += ib.qyi; qyexb += ib.qye; zbb += ib.zb;
+                    done = true;
+                }
+            }
+        }
+        if (!done) {
+            const RoeIn a = {s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
+            double F[3], Fb[3];
+            const double l0 = roe_eval(F, a, s.nx, s.ny, g, g);
+#pragma unroll
+            for (int f = 0; f < 3; ++f) Fb[f] = Fin_b[f] + Fex_b[f];
+            RoeIn ib = {0, 0, 0, 0, 0, 0, 0};
+            roe_adj(a, s.nx, s.ny, g, g, Fb, l0 == target ? lam_share : 0.0, ib);
+            zinb += ib.zi; zexb += ib.ze; qxinb += ib.qxi; qxexb += ib.qxe;
+            qyinb += ib.qyi; qyexb += ib.qye; zbb += ib.zb;
+        }
+        // boundary data -> slots (compute2DL.c:149-199)
+        double exz = 0, exqx = 0, exqy = 0;   // adjoint of the el_ex slot
+        if (s.prescribed) {
+            if (s.zex_from_in) zinb += zexb;
+            else if (s.tide) tideb += zexb;
+            const double tx = -s.ny, ty = s.nx;
+            const double ax = s.qn_from_in ? s.nx : 0.0, ay = s.qn_from_in ? s.ny : 0.0;
+            // qxex = (ty qn - ny qt) dn, qyex = (-tx qn + nx qt) dn, qt = qxin tx + qyin ty
+            qxinb += ((ty * ax - s.ny * tx) * qxexb + (-tx * ax + s.nx * tx) * qyexb) * s.dn;
+            qyinb += ((ty * ay - s.ny * ty) * qxexb + (-tx * ay + s.nx * ty) * qyexb) * s.dn;
+        } else if (s.boundary) {
+            zinb += zexb;
+            qxinb += Ar[0][0] * qxexb + Ar[1][0] * qyexb;
+            qyinb += Ar[0][1] * qxexb + Ar[1][1] * qyexb;
+        } else {
+            exz = zexb; exqx = qxexb; exqy = qyexb;
+        }
+        slotb[s.kin * P + s.el_in] = zinb;
+        slotb[(3 + s.kin) * P + s.el_in] = qxinb;
+        slotb[(6 + s.kin) * P + s.el_in] = qyinb;
+        zbe[s.kin * P + s.el_in] = zbb;
+        if (!s.boundary) {
+            slotb[s.kex * P + s.el_ex] = exz;
+            slotb[(3 + s.kex) * P + s.el_ex] = exqx;
+            slotb[(6 + s.kex) * P + s.el_ex] = exqy;
+            zbe[s.kex * P + s.el_ex] = 0.0;
+        }
+    }
+    const double t = block_sum(tideb);
+    if (threadIdx.x == 0) tideparts[lb] = t;
+}
+
+// Dst += slotb (stage input adjoint); the z_edge adjoints to the edge's two
+// vertices (z_edge = (z_a + z_b)/2)
+__global__ __launch_bounds__(256) void k_adj_combine(MeshDev m, double *Dst, const double *__restrict__ slotb,
+                                                     const double *__restrict__ zbe, double *__restrict__ zvb)
+{
+    const ElemLane L = elem_lane(m);
+    if (!L.valid) return;
+    const int e = L.e, P = m.pitch;
+#pragma unroll
+    for (int f = 0; f < 9; ++f) Dst[f * P + e] += slotb[f * P + e];
+    double z[3] = {0, 0, 0};
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double b = zbe[k * P + e];
+        z[k] += 0.5 * b;
+        z[(k + 1) % 3] += 0.5 * b;
+    }
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zvb[k * P + e] += z[k];
+}
+
+// per member, after the two stages of step n are reversed: the dt adjoint of
+// step n (workgroup partials of both stages plus the start-time term), the
+// tide time derivative, and the adjoint of the previous step's max |lambda|
+// (dt_n = cfl minEdgLength / lambda_{n-1} unless the final-time clamp acted).
+__global__ void k_adj_scalar(MeshDev m, const double *__restrict__ dtparts, const double *__restrict__ tideparts,
+                             int maxparts, const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_prev,
+                             int has_prev, AdjCtrl *actl)
+{
+    const int mid = blockIdx.x;
+    const MemberDev &M = m.mem[mid];
+    __shared__ double red[2][4];
+    double dtb = 0.0, tb = 0.0;
+    for (int i = threadIdx.x; i < M.eblks; i += 256)
+        dtb += dtparts[M.eblk0 + i] + dtparts[maxparts + M.eblk0 + i];
+    for (int i = threadIdx.x; i < M.dblks; i += 256)
+        tb += tideparts[M.dblk0 + i] + tideparts[maxparts + M.dblk0 + i];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) { dtb += __shfl_xor(dtb, o, 64); tb += __shfl_xor(tb, o, 64); }
+    if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = dtb; red[1][threadIdx.x >> 6] = tb; }
+    __syncthreads();
+    if (threadIdx.x != 0) return;
+    dtb = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+    tb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+    AdjCtrl &a = actl[mid];
+    const StepRec r = rec[mid];
+    dtb += a.S;   // t_k = t_0 + sum_{j<k} dt_j
+    double tbar = 0.0;
+    if (M.tide_code) {
+        const double w = 6.283185307179586 / M.tide_period;
+        tbar = tb * M.tide_amp * w * cos(w * r.t + M.tide_phase);
+    }
+    a.S += tbar;
+    a.lamb = 0.0;
+    a.ntie = 0;
+    if (has_prev) {
+        const double lam = __longlong_as_double((long long)rec_prev[mid].lam2);
+        if (r.dt == M.cfl * M.min_edge / lam) a.lamb = -r.dt / lam * dtb;
+    }
+}
+
+// loss gradient of the gauges sampled at this state: h_g = mean over the
+// element's midpoints of zeta + z_mid, loss = sum (h_g - obs)^2
+__global__ void k_adj_gauges(MeshDev m, int ngauge, const int *__restrict__ gauge_el,
+                             const double *__restrict__ hval, const double *__restrict__ obs, double *Wb,
+                             double *zvb, AdjCtrl *actl)
+{
+    const int mid = blockIdx.x * 64 + threadIdx.x;
+    if (mid >= m.nmem) return;
+    const MemberDev &M = m.mem[mid];
+    const int P = m.pitch;
+    for (int gi = 0; gi < ngauge; ++gi) {
+        const int e = M.el_off + gauge_el[mid * ngauge + gi];
+        const double r = hval[mid * ngauge + gi] - obs[mid * ngauge + gi];
+        const double w = 2.0 * r / 3.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            Wb[k * P + e] += w;
+            zvb[k * P + e] += w;   // z_mid_k = (z_k + z_k1)/2: each vertex in two midpoints
+        }
+    }
+}
+
+// forward sample: h at the gauges; adds the misfit to the member's loss
+__global__ void k_gauge_sample(MeshDev m, int ngauge, const int *__restrict__ gauge_el, const double *__restrict__ w,
+                               double *__restrict__ hval, const double *__restrict__ obs, AdjCtrl *actl)
+{
+    const int mid = blockIdx.x * 64 + threadIdx.x;
+    if (mid >= m.nmem) return;
+    const MemberDev &M = m.mem[mid];
+    const int P = m.pitch;
+    double loss = 0.0;
+    for (int gi = 0; gi < ngauge; ++gi) {
+        const int e = M.el_off + gauge_el[mid * ngauge + gi];
+        double h = 0.0;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double za = m.nodes[m.ev[k * P + e]].z, zb = m.nodes[m.ev[((k + 1) % 3) * P + e]].z;
+            h += w[k * P + e] + 0.5 * (za + zb);
+        }
+        h /= 3.0;
+        hval[mid * ngauge + gi] = h;
+        if (obs) {
+            const double r = h - obs[mid * ngauge + gi];
+            loss += r * r;
+        }
+    }
+    if (actl) actl[mid].loss += loss;
+}
+
+// step record: at the start (t, dt, tide) or the end (max |lambda| of stage 2)
+__global__ void k_step_record(MeshDev m, const Ctrl *ctrls, StepRec *rec, int at_end)
+{
+    const int mid = blockIdx.x * 64 + threadIdx.x;
+    if (mid >= m.nmem) return;
+    const Ctrl &c = ctrls[mid];
+    if (at_end) rec[mid].lam2 = c.maxlam[1];
+    else { rec[mid].t = c.t; rec[mid].dt = c.dt; rec[mid].tide = c.tide; }
+}
+
+// nodal gradients: sum of the per-element-vertex adjoints over the elements
+// around each node
+__global__ __launch_bounds__(256) void k_adj_gather(MeshDev m, const double *__restrict__ zvb,
+                                                    const double *__restrict__ nvb, double *__restrict__ gz,
+                                                    double *__restrict__ gn)
+{
+    const int i = xcd_block() * 256 + (int)threadIdx.x;
+    if (i >= m.nn) return;
+    const int P = m.pitch;
+    double sz = 0.0, sn = 0.0;
+    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
+    for (int j = b; j < en; ++j) {
+        const int el = m.n2e[j];
+#pragma unroll
+        for (int k = 0; k < 3; ++k)
+            if (m.ev[k * P + el] == i) { sz += zvb[k * P + el]; sn += nvb[k * P + el]; }
+    }
+    gz[i] = sz;
+    gn[i] = sn;
+}
+
+// ---------------------------------------------------------------------------
+// launchers
+// ---------------------------------------------------------------------------
+void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nodearg, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_node_arg, dim3((unsigned)((m.nn + 255) / 256)), dim3(256), 0, s, m, U, nodemm, nodearg);
+}
+
+void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
+                      double *Ub, double *vmb, AdjCtrl *actl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_limit, dim3((unsigned)(m.pitch / 256)), dim3(256), 0, s, m, ph, U, nodemm, Sb, Ub, vmb,
+                       actl);
+}
+
+void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_node, dim3((unsigned)((m.nn + 255) / 256)), dim3(256), 0, s, m, vmb, nodeb);
+}
+
+void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
+                    const double *U, const double *Ubown, const double *nodeb, const int *nodearg, const int *wd,
+                    const StepRec *rec, double *Inb, double *Bb, double *Fhb, double *zvb, double *nvb,
+                    double *dtparts, hipStream_t s)
+{
+    const dim3 g((unsigned)(m.pitch / 256)), b(256);
+    if (mode == 1)
+        hipLaunchKernelGGL(k_adj_rhs<1>, g, b, 0, s, m, ph, W, B, U, Ubown, nodeb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
+                           nvb, dtparts);
+    else
+        hipLaunchKernelGGL(k_adj_rhs<2>, g, b, 0, s, m, ph, W, B, U, Ubown, nodeb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
+                           nvb, dtparts);
+}
+
+void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
+                     const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
+                     double *slotb, double *zbe, double *tideparts, bool with_lambda, hipStream_t s)
+{
+    const dim3 g((unsigned)(m.ned_pad / 256)), b(256);
+    if (with_lambda)
+        hipLaunchKernelGGL(k_adj_ties, g, b, 0, s, m, ph, win, bzeta, bqn, wd, rec_lam, rec, actl);
+    if (ph.wetdry)
+        hipLaunchKernelGGL(k_adj_edge<true>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
+                           with_lambda ? 1 : 0, slotb, zbe, tideparts);
+    else
+        hipLaunchKernelGGL(k_adj_edge<false>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
+                           with_lambda ? 1 : 0, slotb, zbe, tideparts);
+}
+
+void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, const double *zbe, double *zvb,
+                        hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_combine, dim3((unsigned)(m.pitch / 256)), dim3(256), 0, s, m, Dst, slotb, zbe, zvb);
+}
+
+void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
+                       const StepRec *rec, const StepRec *rec_prev, int has_prev, AdjCtrl *actl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_scalar, dim3((unsigned)m.nmem), dim3(256), 0, s, m, dtparts, tideparts, maxparts, rec,
+                       rec_prev, has_prev, actl);
+}
+
+void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,
+                       double *Wb, double *zvb, AdjCtrl *actl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_gauges, dim3((unsigned)((m.nmem + 63) / 64)), dim3(64), 0, s, m, ngauge, gauge_el, hval,
+                       obs, Wb, zvb, actl);
+}
+
+void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
+                         const double *obs, AdjCtrl *actl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_gauge_sample, dim3((unsigned)((m.nmem + 63) / 64)), dim3(64), 0, s, m, ngauge, gauge_el, w,
+                       hval, obs, actl);
+}
+
+void launch_step_record(const MeshDev &m, const Ctrl *ctrls, StepRec *rec, int at_end, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_step_record, dim3((unsigned)((m.nmem + 63) / 64)), dim3(64), 0, s, m, ctrls, rec, at_end);
+}
+
+void launch_adj_gather(const MeshDev &m, const double *zvb, const double *nvb, double *gz, double *gn, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_gather, dim3((unsigned)((m.nn + 255) / 256)), dim3(256), 0, s, m, zvb, nvb, gz, gn);
+}

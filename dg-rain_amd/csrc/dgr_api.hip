@@ -13,80 +13,7 @@
 #include "../../include/dgrain.h"
 #include "dgr_device.h"
 
-namespace dgr {
-int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
-                 double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
-                 double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
-                 hipStream_t s);
-void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
-void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
-void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
-                  unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s);
-void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
-void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s);
-void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
-void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, hipStream_t s);
-void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
-void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s);
-void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s);
-void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s);
-}  // namespace dgr
-
-using namespace dgr;
-
-struct MemberHost {
-    int ne, nn, ned;              // the member's own sizes
-    int el_off, nd_off, ed_off;   // offsets into the concatenated arrays
-    dgr_params prm;
-    double min_edge;
-};
-
-struct dgr_handle {
-    int nmem;
-    int pitch, nn, ned_pad;       // padded element count, total nodes, padded edge count
-    int device;
-    hipStream_t stream;
-    std::vector<MemberHost> mem;
-    PhysDev ph;
-    MeshDev m;
-    int record_modify;
-    // device buffers
-    double4 *d_nodes;
-    int *d_ev, *d_nb, *d_info, *d_eid, *d_n2e_off, *d_n2e;
-    int4 *d_edges;   // {el_in, el_ex, va, vb} per edge
-    int *d_einfo;    // packed edge facts (ED_*/EI_* bits)
-    MemberDev *d_mem;
-    int *d_eblk_mem, *d_dblk_mem;
-    double *d_bzeta, *d_bqn;
-    double *d_w, *d_w1, *d_means, *d_nodemm;
-    double *d_dbg;   // 18 planes: rhs (9) then fhat (9)
-    double *d_fhat;  // 9 planes of edge fluxes (split kernel variants)
-    int *d_wd;
-    unsigned char *d_modify;   // [2][pitch]
-    Ctrl *d_ctrl;              // [nmem]
-    unsigned long long *d_lamparts;   // [2][maxparts] per-workgroup max |lambda|
-    int variant;     // 0 fused, 1 split element-centric, 2 split edge-centric
-    int kind[2];     // partials space of the last flux launch of each stage
-    // host staging
-    std::vector<double> h_planes;
-    // optional HIP-event timing of the flux/RHS/update launches
-    int profiling;
-    std::vector<hipEvent_t> ev_pool;
-    size_t ev_used;
-    // one captured step (dgr_set_graph)
-    int use_graph;
-    int graph_variant;
-    hipGraphExec_t graph_exec;
-};
-
-#define DGR_HIP(call)                                                                   \
-    do {                                                                                \
-        hipError_t e_ = (call);                                                         \
-        if (e_ != hipSuccess) {                                                         \
-            fprintf(stderr, "libdgrain: %s failed: %s\n", #call, hipGetErrorString(e_)); \
-            return DGR_ERR_HIP;                                                         \
-        }                                                                               \
-    } while (0)
+#include "dgr_handle.h"
 
 extern "C" void dgr_params_default(dgr_params *p)
 {
@@ -142,12 +69,6 @@ static dgr_status free_handle(dgr_handle *h)
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
     return DGR_OK;
-}
-
-template <typename T>
-static hipError_t dalloc(T **p, size_t n)
-{
-    return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
 }
 
 static inline int pad256(int n) { return (n + 255) / 256 * 256; }
@@ -502,7 +423,7 @@ extern "C" dgr_status dgr_member_set_boundary(dgr_handle *h, int mi, const doubl
     return DGR_OK;
 }
 
-static dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c)
+dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c)
 {
     DGR_HIP(hipMemcpyAsync(c, h->d_ctrl + mi, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
     DGR_HIP(hipStreamSynchronize(h->stream));
@@ -561,7 +482,7 @@ extern "C" dgr_status dgr_member_report(dgr_handle *h, int mi, dgr_report *rep)
 }
 
 // any member with a recorded error
-static dgr_status any_error(dgr_handle *h)
+dgr_status any_error(dgr_handle *h)
 {
     std::vector<Ctrl> c(h->nmem);
     DGR_HIP(hipMemcpyAsync(c.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
@@ -592,7 +513,7 @@ extern "C" dgr_status dgr_set_boundary(dgr_handle *h, const double *bz, const do
 }
 extern "C" dgr_status dgr_set_time(dgr_handle *h, double t, double dt) { return dgr_member_set_time(h, 0, t, dt); }
 
-static inline unsigned char *modify_ptr(dgr_handle *h, int stage)
+unsigned char *modify_ptr(dgr_handle *h, int stage)
 {
     return h->record_modify ? h->d_modify + (size_t)stage * h->pitch : nullptr;
 }
@@ -669,22 +590,31 @@ static void mark(dgr_handle *h)
 // the two stages of oneTimeStep2D (time_evolution.c:10-246; wet/dry calls at the
 // places ChanNet/time_evolution.c:218-427 puts them), enqueued on h->stream
 // for every member at once
-static void enqueue_stages(dgr_handle *h)
+// With a tape, the step's start state, each stage's output before and after
+// the limiter and the wet/dry flags each stage used are copied out as well.
+void enqueue_stages(dgr_handle *h, const StepTape *tp)
 {
     const bool wdon = h->ph.wetdry != 0;
     hipStream_t s = h->stream;
+    const size_t W = 9 * (size_t)h->pitch * sizeof(double), I = (size_t)h->pitch * sizeof(int);
+    if (tp) (void)hipMemcpyAsync(tp->w0, h->d_w, W, hipMemcpyDeviceToDevice, s);
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, s);
+    if (tp) (void)hipMemcpyAsync(tp->wda, h->d_wd, I, hipMemcpyDeviceToDevice, s);
     mark(h);
     h->kind[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn,
                               h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
     mark(h);
+    if (tp) (void)hipMemcpyAsync(tp->u1, h->d_w1, W, hipMemcpyDeviceToDevice, s);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
     launch_limit(wdon, h->m, h->ph, h->d_w1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, s);
+    if (tp) (void)hipMemcpyAsync(tp->p1, h->d_w1, W, hipMemcpyDeviceToDevice, s);
     if (wdon) launch_wd_status(h->m, h->ph, h->d_w1, h->d_wd, s);
+    if (tp) (void)hipMemcpyAsync(tp->wdb, h->d_wd, I, hipMemcpyDeviceToDevice, s);
     mark(h);
     h->kind[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn,
                               h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     mark(h);
+    if (tp) (void)hipMemcpyAsync(tp->u2, h->d_w, W, hipMemcpyDeviceToDevice, s);
     // stage 2's max |lambda| sets the next dt; repair its fold if a NaN operand was seen
     launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w1, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
                  nullptr, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
@@ -692,10 +622,10 @@ static void enqueue_stages(dgr_handle *h)
     launch_limit(wdon, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1, s);
 }
 
-static void enqueue_step(dgr_handle *h)
+void enqueue_step(dgr_handle *h, const StepTape *tp)
 {
     launch_step_begin(h->m, h->d_ctrl, h->stream);
-    enqueue_stages(h);
+    enqueue_stages(h, tp);
     launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], h->stream);
 }
 
@@ -704,7 +634,7 @@ extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)
     if (!h) return DGR_ERR_ARG;
     (void)t;
     launch_step_mark(h->m, h->d_ctrl, dt, h->stream);
-    enqueue_stages(h);
+    enqueue_stages(h, nullptr);
     launch_lam_reduce(h->m, h->ph, h->d_ctrl, 1, h->kind[1], h->stream);
     DGR_HIP(hipGetLastError());
     return any_error(h);
@@ -720,7 +650,7 @@ static dgr_status step_graph(dgr_handle *h)
     }
     hipGraph_t g = nullptr;
     DGR_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(h);
+    enqueue_step(h, nullptr);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(h->stream, &g);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -754,7 +684,7 @@ extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
         if (st) return st;
         for (int n = 0; n < nsteps; ++n) DGR_HIP(hipGraphLaunch(h->graph_exec, h->stream));
     } else {
-        for (int n = 0; n < nsteps; ++n) enqueue_step(h);
+        for (int n = 0; n < nsteps; ++n) enqueue_step(h, nullptr);
     }
     DGR_HIP(hipGetLastError());
     if (rep) {
@@ -904,9 +834,9 @@ extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double
     for (int v = 0; v < DGR_NUM_VARIANTS; ++v) {
         h->variant = v;
         // one untimed step to warm the code object, then the timed steps
-        enqueue_step(h);
+        enqueue_step(h, nullptr);
         DGR_HIP(hipEventRecord(e0, h->stream));
-        for (int n = 0; n < steps; ++n) enqueue_step(h);
+        for (int n = 0; n < steps; ++n) enqueue_step(h, nullptr);
         DGR_HIP(hipEventRecord(e1, h->stream));
         DGR_HIP(hipEventSynchronize(e1));
         float ms = 0.f;

@@ -146,6 +146,21 @@ struct PhysDev {
     int maxparts;
 };
 
+// per-member bookkeeping of the reverse sweep
+struct AdjCtrl {
+    double S;        // sum of the adjoints of the step start times after this step
+    double lamb;     // adjoint of the max |lambda| that set the dt of the step being reversed
+    double loss;
+    int ntie;        // edges whose |lambda| operand equals the recorded max
+    int unsupported; // set if the forward took a branch the reverse does not cover
+};
+
+// per-step record of the forward pass, per member
+struct StepRec {
+    double t, dt, tide;         // at the step's start
+    unsigned long long lam2;    // max |lambda| of the step's second stage (raw bits)
+};
+
 // which workgroup space a flux launch wrote its partials in
 enum PartKind : int { PARTS_ELEM = 0, PARTS_EDGE = 1 };
 

@@ -1,0 +1,131 @@
+// dgr_handle.h -- the handle of libdgrain and the host-side pieces its C ABI
+// files share (dgr_api.hip: forward; dgr_adjoint_api.hip: gradients).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <vector>
+
+#include "../../include/dgrain.h"
+#include "dgr_device.h"
+
+namespace dgr {
+int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
+                 double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
+                 double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
+                 hipStream_t s);
+void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
+void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
+void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
+                  unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s);
+void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
+void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s);
+void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
+void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, hipStream_t s);
+void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
+void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s);
+void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s);
+void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s);
+
+// adjoint sweep (dgr_adjoint_kernels.h)
+void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nodearg, hipStream_t s);
+void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
+                      double *Ub, double *vmb, AdjCtrl *actl, hipStream_t s);
+void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s);
+void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
+                    const double *U, const double *Ubown, const double *nodeb, const int *nodearg, const int *wd,
+                    const StepRec *rec, double *Inb, double *Bb, double *Fhb, double *zvb, double *nvb,
+                    double *dtparts, hipStream_t s);
+void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
+                     const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
+                     double *slotb, double *zbe, double *tideparts, bool with_lambda, hipStream_t s);
+void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, const double *zbe, double *zvb,
+                        hipStream_t s);
+void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
+                       const StepRec *rec, const StepRec *rec_prev, int has_prev, AdjCtrl *actl, hipStream_t s);
+void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,
+                       double *Wb, double *zvb, AdjCtrl *actl, hipStream_t s);
+void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
+                         const double *obs, AdjCtrl *actl, hipStream_t s);
+void launch_step_record(const MeshDev &m, const Ctrl *ctrls, StepRec *rec, int at_end, hipStream_t s);
+void launch_adj_gather(const MeshDev &m, const double *zvb, const double *nvb, double *gz, double *gn, hipStream_t s);
+}  // namespace dgr
+
+using namespace dgr;
+
+struct MemberHost {
+    int ne, nn, ned;              // the member's own sizes
+    int el_off, nd_off, ed_off;   // offsets into the concatenated arrays
+    dgr_params prm;
+    double min_edge;
+};
+
+struct dgr_handle {
+    int nmem;
+    int pitch, nn, ned_pad;       // padded element count, total nodes, padded edge count
+    int device;
+    hipStream_t stream;
+    std::vector<MemberHost> mem;
+    PhysDev ph;
+    MeshDev m;
+    int record_modify;
+    // device buffers
+    double4 *d_nodes;
+    int *d_ev, *d_nb, *d_info, *d_eid, *d_n2e_off, *d_n2e;
+    int4 *d_edges;   // {el_in, el_ex, va, vb} per edge
+    int *d_einfo;    // packed edge facts (ED_*/EI_* bits)
+    MemberDev *d_mem;
+    int *d_eblk_mem, *d_dblk_mem;
+    double *d_bzeta, *d_bqn;
+    double *d_w, *d_w1, *d_means, *d_nodemm;
+    double *d_dbg;   // 18 planes: rhs (9) then fhat (9)
+    double *d_fhat;  // 9 planes of edge fluxes (split kernel variants)
+    int *d_wd;
+    unsigned char *d_modify;   // [2][pitch]
+    Ctrl *d_ctrl;              // [nmem]
+    unsigned long long *d_lamparts;   // [2][maxparts] per-workgroup max |lambda|
+    int variant;     // 0 fused, 1 split element-centric, 2 split edge-centric
+    int kind[2];     // partials space of the last flux launch of each stage
+    // host staging
+    std::vector<double> h_planes;
+    // optional HIP-event timing of the flux/RHS/update launches
+    int profiling;
+    std::vector<hipEvent_t> ev_pool;
+    size_t ev_used;
+    // one captured step (dgr_set_graph)
+    int use_graph;
+    int graph_variant;
+    hipGraphExec_t graph_exec;
+};
+
+#define DGR_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            fprintf(stderr, "libdgrain: %s failed: %s\n", #call, hipGetErrorString(e_)); \
+            return DGR_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+
+template <typename T>
+static inline hipError_t dalloc(T **p, size_t n)
+{
+    return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
+}
+
+// device copies taken during one taped step (enqueue_stages)
+struct StepTape {
+    double *w0;   // state at the step's start
+    double *u1;   // stage 1 output before the limiter
+    double *p1;   // stage 1 output after limiter/PDop (stage 2 input)
+    double *u2;   // stage 2 output before the limiter
+    int *wda;     // wet/dry flags stage 1 used
+    int *wdb;     // wet/dry flags stage 2 used
+};
+
+void enqueue_stages(dgr_handle *h, const StepTape *tp);
+void enqueue_step(dgr_handle *h, const StepTape *tp);
+dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c);
+dgr_status any_error(dgr_handle *h);
+unsigned char *modify_ptr(dgr_handle *h, int stage);

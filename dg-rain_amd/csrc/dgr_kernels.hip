@@ -21,6 +21,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "dgr_adjoint.h"
 #include "dgr_device.h"
 #include "dgr_math.h"
 
@@ -1249,5 +1250,8 @@ void launch_math_selftest(int n, const double *x, const double *y, double *out, 
 {
     hipLaunchKernelGGL(k_math_selftest, grid_for(n), dim3(256), 0, s, n, x, y, out);
 }
+
+// the reverse sweep (shares the device functions above)
+#include "dgr_adjoint_kernels.h"
 
 }  // namespace dgr

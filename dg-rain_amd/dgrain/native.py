@@ -42,6 +42,11 @@ class Params(C.Structure):
                 ("tide_phase", C.c_double), ("record_modify", C.c_int)]
 
 
+class AdjointSpec(C.Structure):
+    _fields_ = [("nsteps", C.c_int), ("sample_every", C.c_int), ("checkpoint_every", C.c_int),
+                ("ngauges", C.c_int), ("gauge_el", C.POINTER(C.c_int)), ("obs", C.POINTER(C.c_double))]
+
+
 class Report(C.Structure):
     _fields_ = [("t", C.c_double), ("dt", C.c_double), ("max_lambda", C.c_double),
                 ("error_code", C.c_int), ("error_where", C.c_int)]
@@ -101,6 +106,9 @@ def load():
         "dgr_member_set_time": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
         "dgr_member_report": (C.c_int, [vp, C.c_int, C.POINTER(Report)]),
         "dgr_member_layout": (C.c_int, [vp, C.c_int, ip, ip]),
+        "dgr_adjoint_gauges": (C.c_int, [vp, C.POINTER(AdjointSpec), dp, dp, dp]),
+        "dgr_gauge_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, ip, dp]),
+        "dgr_set_nodal": (C.c_int, [vp, C.c_int, dp, dp]),
         "dgr_profile_end": (C.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():
@@ -230,6 +238,46 @@ class _Handle:
         n = C.c_int(0)
         check(self.lib.dgr_profile_end(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
+
+    # -- gauges and gradients ---------------------------------------------
+    def _num_nodes(self):
+        ms = getattr(self, "meshes", None) or [self.mesh]
+        return sum(m.num_nodes for m in ms)
+
+    def _nmem(self):
+        return len(getattr(self, "meshes", None) or [self.mesh])
+
+    def gauge_run(self, nsteps, sample_every, gauge_el):
+        """Run nsteps steps; gauge depths after every sample_every-th step,
+        shape [members, nsteps // sample_every, ngauges]."""
+        gel = np.ascontiguousarray(np.reshape(gauge_el, (self._nmem(), -1)), dtype=np.int32)
+        nsamp = nsteps // sample_every
+        out = np.zeros((gel.shape[0], nsamp, gel.shape[1]))
+        check(self.lib.dgr_gauge_run(self.h, int(nsteps), int(sample_every), gel.shape[1], _ip(gel), _dp(out)),
+              "dgr_gauge_run")
+        return out
+
+    def adjoint_gauges(self, nsteps, sample_every, checkpoint_every, gauge_el, obs):
+        """(loss[members], dL/dz, dL/dn) of the gauge misfit over nsteps steps
+        from the current state (which is restored afterwards)."""
+        gel = np.ascontiguousarray(np.reshape(gauge_el, (self._nmem(), -1)), dtype=np.int32)
+        obs = np.ascontiguousarray(obs, dtype=np.float64)
+        assert obs.shape == (gel.shape[0], nsteps // sample_every, gel.shape[1]), obs.shape
+        spec = AdjointSpec(int(nsteps), int(sample_every), int(checkpoint_every), gel.shape[1], _ip(gel), _dp(obs))
+        loss = np.zeros(gel.shape[0])
+        gz = np.zeros(self._num_nodes())
+        gn = np.zeros(self._num_nodes())
+        check(self.lib.dgr_adjoint_gauges(self.h, C.byref(spec), _dp(loss), _dp(gz), _dp(gn)), "dgr_adjoint_gauges")
+        return loss, gz, gn
+
+    def set_nodal(self, z=None, n=None, member=0):
+        """Replace the nodal depth z and/or Manning n of a member."""
+        if z is None and n is None:
+            return
+        za = None if z is None else np.ascontiguousarray(z, dtype=np.float64)
+        na = None if n is None else np.ascontiguousarray(n, dtype=np.float64)
+        check(self.lib.dgr_set_nodal(self.h, int(member), None if za is None else _dp(za),
+                                     None if na is None else _dp(na)), "dgr_set_nodal")
 
     def device_state(self):
         p = C.POINTER(C.c_double)()

@@ -195,6 +195,35 @@ dgr_status dgr_member_report(dgr_handle *h, int member, dgr_report *report);
 /* element range of a member inside the device planes of dgr_device_state */
 dgr_status dgr_member_layout(dgr_handle *h, int member, int *el_offset, int *num_el);
 
+/* Gradients (extension; no counterpart in the reference).  A gauge-misfit
+ * loss over a forward run from the handle's current state,
+ *     L = sum over samples s, gauges g of (h_g(s) - obs[s][g])^2,
+ * with h_g the mean depth (zeta + z at the midpoints) of gauge element g after
+ * every `sample_every`-th step, and its gradient with respect to the nodal
+ * depth z and Manning n of each member, by a hand-written reverse of the RK
+ * step (Roe flux, inner products, Manning friction, RK update, limiter with
+ * its active set, dt = cfl*minEdgLength/max|lambda| and the tide's time) over
+ * checkpoints every `checkpoint_every` steps.  The forward's wet/dry flags
+ * enter as recorded; an element where PDop2D changed the state makes the call
+ * return DGR_ERR_UNSUPPORTED.  The final-time clamp of dt is taken as inactive.
+ * The handle's state, flags and time are restored on return. */
+typedef struct {
+    int nsteps;
+    int sample_every;
+    int checkpoint_every;
+    int ngauges;              /* gauges per member */
+    const int *gauge_el;      /* [members][ngauges] member-local element ids */
+    const double *obs;        /* [members][nsteps/sample_every][ngauges] */
+} dgr_adjoint_spec;
+/* loss[members]; grad_z, grad_n: [sum of members' num_nodes], member order (may be NULL) */
+dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *spec, double *loss, double *grad_z,
+                              double *grad_n);
+/* Forward run that samples the gauge depths: h_out[members][nsteps/sample_every][ngauges]. */
+dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every, int ngauges, const int *gauge_el,
+                         double *h_out);
+/* Replace a member's nodal depth z and/or Manning n (NULL keeps it). */
+dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, const double *n);
+
 /* Synchronise the handle's stream. */
 dgr_status dgr_sync(dgr_handle *h);
 

@@ -1,0 +1,180 @@
+"""Gradients on the GPU: the hand-written reverse of the RK step (through the
+C ABI, dgr_adjoint_gauges) against central finite differences of the forward
+solver, for a gauge-misfit loss with respect to nodal depth z and Manning n.
+
+The loss sums (h - obs)^2 over gauge elements sampled every few steps; the
+observations come from a run with a perturbed Manning field.  Each check
+compares <grad, d> with (L(p + eps d) - L(p - eps d)) / (2 eps) for random
+directions d; the tolerance is written in each test."""
+import dataclasses
+
+import numpy as np
+import pytest
+
+import oracle
+from dgrain import configs, native
+
+pytestmark = pytest.mark.gpu
+
+
+def _problem(N=10, nsteps=30, sample_every=5, ngauge=6, seed=0, rain_mm_h=0.0):
+    """Coastal problem started from a generic state: the flood level plus
+    small random elevations and flows, so that no limiter bound, node
+    extremum or max-|lambda| operand is an exact tie (a central difference
+    across an exact tie averages two branches; the reverse takes one)."""
+    wl = configs.coastal(N=N, nsteps=nsteps, rain_mm_h=rain_mm_h)
+    rng = np.random.default_rng(seed)
+    n3 = wl.zeta.size
+    wl = dataclasses.replace(wl, zeta=wl.zeta + 0.05 * rng.standard_normal(n3),
+                             qx=0.05 * rng.standard_normal(n3), qy=0.05 * rng.standard_normal(n3))
+    gauges = np.sort(rng.choice(wl.num_el, ngauge, replace=False)).astype(np.int32)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    s.set_nodal(n=wl.mesh.n * np.exp(0.2 * rng.standard_normal(wl.mesh.num_nodes)))
+    obs = s.gauge_run(nsteps, sample_every, gauges)
+    s.close()
+    return wl, gauges, obs
+
+
+def _loss_gpu(wl, gauges, obs, nsteps, sample_every, z=None, n=None):
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_nodal(z=z, n=n)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    h = s.gauge_run(nsteps, sample_every, gauges)
+    s.close()
+    return float(((h - obs) ** 2).sum())
+
+
+def _smooth_direction(mesh, rng):
+    # a smooth random field over the nodes (a few Fourier modes)
+    x = (mesh.x - mesh.x.min()) / (np.ptp(mesh.x) + 1e-300)
+    y = (mesh.y - mesh.y.min()) / (np.ptp(mesh.y) + 1e-300)
+    d = np.zeros(mesh.num_nodes)
+    for _ in range(4):
+        kx, ky = rng.integers(0, 3, 2)
+        d += rng.standard_normal() * np.cos(np.pi * kx * x + rng.uniform(0, 6.3)) * np.cos(np.pi * ky * y)
+    return d
+
+
+def test_adjoint_loss_matches_forward(gpu):
+    nsteps, every = 30, 5
+    wl, gauges, obs = _problem(nsteps=nsteps, sample_every=every)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    loss, gz, gn = s.adjoint_gauges(nsteps, every, 7, gauges, obs)
+    ref = _loss_gpu(wl, gauges, obs, nsteps, every)
+    assert abs(loss[0] - ref) <= 1e-13 * abs(ref), (loss, ref)
+    # the handle's state is restored: a second call gives the same bits
+    loss2, gz2, gn2 = s.adjoint_gauges(nsteps, every, 7, gauges, obs)
+    assert loss2[0] == loss[0] and np.array_equal(gz, gz2) and np.array_equal(gn, gn2)
+    assert np.all(np.isfinite(gz)) and np.all(np.isfinite(gn)) and np.abs(gn).max() > 0
+
+
+@pytest.mark.parametrize("param", ["z", "n"])
+def test_adjoint_directional_derivatives(gpu, param):
+    """Tolerance: 1e-6 relative between <grad, d> and the central difference,
+    in each of 4 random directions; eps relative to the field is 1e-6 for z
+    and 1e-4 for n (Manning n ~ 0.025: a smaller step drowns the difference
+    in the loss's rounding)."""
+    nsteps, every = 40, 5
+    wl, gauges, obs = _problem(N=10, nsteps=nsteps, sample_every=every, seed=1, rain_mm_h=20.0)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    loss, gz, gn = s.adjoint_gauges(nsteps, every, 8, gauges, obs)
+    s.close()
+    g = gz if param == "z" else gn
+    base = wl.mesh.z if param == "z" else wl.mesh.n
+    rng = np.random.default_rng(5)
+    worst = 0.0
+    for _ in range(4):
+        d = _smooth_direction(wl.mesh, rng) * np.abs(base).mean()
+        eps = 1e-6 if param == "z" else 1e-4
+        kw_p = {param: base + eps * d}
+        kw_m = {param: base - eps * d}
+        fd = (_loss_gpu(wl, gauges, obs, nsteps, every, **kw_p) -
+              _loss_gpu(wl, gauges, obs, nsteps, every, **kw_m)) / (2 * eps)
+        ad = float(g @ d)
+        rel = abs(ad - fd) / max(abs(fd), 1e-300)
+        print("  %s: adjoint %.12e  central difference %.12e  rel %.2e" % (param, ad, fd, rel), flush=True)
+        worst = max(worst, rel)
+    assert worst <= 1e-6, worst
+
+
+def test_adjoint_against_oracle_differences(gpu):
+    """The gradient against central differences of the CPU oracle (the
+    restatement of the reference solver) on the same problem; tolerance 1e-6
+    relative, 2 directions for z and 2 for n."""
+    nsteps, every = 20, 5
+    wl, gauges, obs = _problem(N=8, nsteps=nsteps, sample_every=every, seed=2)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    loss, gz, gn = s.adjoint_gauges(nsteps, every, 5, gauges, obs)
+    s.close()
+
+    def oracle_loss(mesh):
+        w2 = dataclasses.replace(wl, mesh=mesh)
+        o = oracle.Oracle.for_workload(w2)
+        hs = []
+        for k in range(nsteps // every):
+            o.run(every)
+            z, _, _ = o.get_state()
+            z = z.reshape(-1, 3)
+            ev = mesh.elto_vert.reshape(-1, 3)
+            zm = 0.5 * (mesh.z[ev] + mesh.z[np.roll(ev, -1, axis=1)])
+            hs.append(((z + zm).sum(axis=1) / 3.0)[gauges])
+        return float(((np.array(hs)[None] - obs) ** 2).sum())
+
+    rng = np.random.default_rng(9)
+    worst = 0.0
+    for param, g in (("z", gz), ("n", gn), ("z", gz), ("n", gn)):
+        base = getattr(wl.mesh, param)
+        d = _smooth_direction(wl.mesh, rng) * np.abs(base).mean()
+        eps = 1e-6 if param == "z" else 1e-4
+        mp = dataclasses.replace(wl.mesh, **{param: base + eps * d})
+        mm = dataclasses.replace(wl.mesh, **{param: base - eps * d})
+        fd = (oracle_loss(mp) - oracle_loss(mm)) / (2 * eps)
+        ad = float(g @ d)
+        rel = abs(ad - fd) / max(abs(fd), 1e-300)
+        print("  %s: adjoint %.12e  oracle central difference %.12e  rel %.2e" % (param, ad, fd, rel), flush=True)
+        worst = max(worst, rel)
+    assert worst <= 1e-6, worst
+
+
+def test_autograd_harness(gpu):
+    """torch.autograd over the C ABI: backward gives the adjoint gradients,
+    and a few gradient steps on Manning n reduce the misfit."""
+    import torch
+    from dgrain import autograd
+    nsteps, every = 30, 5
+    wl, gauges, obs = _problem(N=10, nsteps=nsteps, sample_every=every, seed=3)
+    prob = autograd.GaugeProblem(wl.mesh, wl.solver_params(), wl.zeta, wl.qx, wl.qy, gauges, obs[0], nsteps,
+                                 every, 6)
+    z = torch.tensor(wl.mesh.z, dtype=torch.float64, requires_grad=True)
+    n = torch.tensor(wl.mesh.n, dtype=torch.float64, requires_grad=True)
+    L = autograd.gauge_loss(prob, z, n)
+    L.backward()
+    ref_loss, gz, gn = prob.loss_and_grads(wl.mesh.z, wl.mesh.n)
+    assert float(L) == ref_loss
+    assert np.array_equal(z.grad.numpy(), gz) and np.array_equal(n.grad.numpy(), gn)
+    # the misfit of the forward-only path is the same number
+    assert abs(prob.loss(wl.mesh.z, wl.mesh.n) - ref_loss) <= 1e-13 * ref_loss
+    # steepest descent on log n with a backtracking step
+    logn = torch.log(torch.tensor(wl.mesh.n, dtype=torch.float64)).requires_grad_(True)
+    zt = torch.tensor(wl.mesh.z, dtype=torch.float64)
+    losses = []
+    step = 1.0
+    for it in range(4):
+        L = autograd.gauge_loss(prob, zt, torch.exp(logn))
+        L.backward()
+        losses.append(float(L))
+        with torch.no_grad():
+            g = logn.grad
+            while True:
+                trial = logn - step * g / (g.abs().max() + 1e-300) * 0.05
+                if prob.loss(wl.mesh.z, torch.exp(trial).numpy()) < float(L) or step < 1e-3:
+                    break
+                step *= 0.5
+            logn.copy_(trial)
+            logn.grad.zero_()
+    print("  misfit by iteration:", ["%.4e" % v for v in losses])
+    assert losses[-1] < losses[0]
