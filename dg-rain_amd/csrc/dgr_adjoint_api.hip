@@ -134,16 +134,20 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&tu2, W9 * K));
     DGR_HIP(B.alloc(&twa, P * K));
     DGR_HIP(B.alloc(&twb, P * K));
-    // reverse buffers
-    double *wb, *wb0, *pb, *ub, *fhb, *slotb, *zbe, *vmb, *nodeb, *nodemm, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
+    // reverse buffers: slot/zbe pairs per stage (the edge part of a stage's
+    // input adjoint is consumed by the limiter reverse that comes next)
+    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *nodeb, *nodemm, *zvb, *nvb, *dtp, *tdp,
+        *gz, *gn;
     int *nodearg;
     DGR_HIP(B.alloc(&wb, W9));
     DGR_HIP(B.alloc(&wb0, W9));
     DGR_HIP(B.alloc(&pb, W9));
     DGR_HIP(B.alloc(&ub, W9));
     DGR_HIP(B.alloc(&fhb, W9));
-    DGR_HIP(B.alloc(&slotb, W9));
-    DGR_HIP(B.alloc(&zbe, 3 * P));
+    DGR_HIP(B.alloc(&slot1, W9));
+    DGR_HIP(B.alloc(&slot2, W9));
+    DGR_HIP(B.alloc(&zbe1, 3 * P));
+    DGR_HIP(B.alloc(&zbe2, 3 * P));
     DGR_HIP(B.alloc(&vmb, 18 * P));
     DGR_HIP(B.alloc(&nodeb, 6 * nn));
     DGR_HIP(B.alloc(&nodemm, 6 * nn));
@@ -167,6 +171,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         DGR_HIP(hipStreamSynchronize(s));
     }
     const size_t Wb = W9 * sizeof(double), Ib = P * sizeof(int), Cb = (size_t)nm * sizeof(Ctrl);
+    const bool wdon = h->ph.wetdry != 0;
 
     // forward with checkpoints
     for (int n = 0; n < N; ++n) {
@@ -204,42 +209,58 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(hipMemsetAsync(wb, 0, Wb, s));
     DGR_HIP(hipMemsetAsync(zvb, 0, 3 * P * sizeof(double), s));
     DGR_HIP(hipMemsetAsync(nvb, 0, 3 * P * sizeof(double), s));
+    bool have_slot1 = false;
     for (int c = nseg - 1; c >= 0; --c) {
         const int n0 = c * K, n1 = (n0 + K < N) ? n0 + K : N;
-        dgr_status st = restore(c);
-        if (st) return st;
+        // recompute the segment into the tape: the checkpoint goes to slot 0,
+        // every step reads its slot and writes the next one (no copies)
+        DGR_HIP(hipMemcpyAsync(tw0, ckW + W9 * c, Wb, hipMemcpyDeviceToDevice, s));
+        DGR_HIP(hipMemcpyAsync(h->d_wd, ckWD + P * c, Ib, hipMemcpyDeviceToDevice, s));
+        DGR_HIP(hipMemcpyAsync(h->d_ctrl, ckC + (size_t)nm * c, Cb, hipMemcpyDeviceToDevice, s));
         for (int n = n0; n < n1; ++n) {
             const int k = n - n0;
-            const StepTape tp = {tw0 + W9 * k, tu1 + W9 * k, tp1 + W9 * k, tu2 + W9 * k, twa + P * k, twb + P * k};
-            enqueue_step(h, &tp);
+            const bool last = n + 1 == n1;
+            StepBufs bf;
+            bf.w0 = tw0 + W9 * k;
+            bf.u1 = tu1 + W9 * k;
+            bf.p1 = tp1 + W9 * k;
+            bf.u2 = tu2 + W9 * k;
+            bf.wout = last ? h->d_w : tw0 + W9 * (k + 1);
+            if (wdon) {
+                bf.wdprev = k == 0 ? h->d_wd : twb + P * (k - 1);
+                bf.wda = twa + P * k;
+                bf.wdb = twb + P * k;
+            } else {
+                bf.wdprev = bf.wda = bf.wdb = h->d_wd;   // never written without wet/dry
+            }
+            enqueue_step(h, &bf);
         }
         for (int n = n1 - 1; n >= n0; --n) {
             const int k = n - n0;
             double *w0 = tw0 + W9 * k, *u1 = tu1 + W9 * k, *p1 = tp1 + W9 * k, *u2 = tu2 + W9 * k;
-            int *wda = twa + P * k, *wdb = twb + P * k;
+            const int *wda = wdon ? twa + P * k : h->d_wd, *wdb = wdon ? twb + P * k : h->d_wd;
             const StepRec *rn = rec + (size_t)nm * n;
             if ((n + 1) % sp->sample_every == 0) {
                 const int j = (n + 1) / sp->sample_every - 1;
-                launch_adj_gauges(h->m, ng, d_gel, d_hv + (size_t)j * per, d_obs + (size_t)j * per, wb, zvb, actl, s);
+                launch_adj_gauges(h->m, ng, d_gel, d_hv + (size_t)j * per, d_obs + (size_t)j * per, wb, zvb, s);
             }
-            DGR_HIP(hipMemsetAsync(wb0, 0, Wb, s));
-            // stage 2: input p1, base w0, output u2 -> limited -> w^{n+1} (adjoint wb)
+            // stage 2: input p1, base w0, output u2 -> limited -> w^{n+1} (adjoint wb + slot1)
             launch_node_arg(h->m, u2, nodemm, nodearg, s);
-            launch_adj_limit(h->m, h->ph, u2, nodemm, wb, ub, vmb, actl, s);
+            launch_adj_limit(h->m, h->ph, u2, nodemm, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
+                             zvb, ub, vmb, actl, s);
             launch_adj_node(h->m, vmb, nodeb, s);
             launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, nodearg, wdb, rn, pb, wb0, fhb, zvb, nvb,
                            dtp + maxparts, s);
-            launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slotb, zbe, tdp + maxparts,
+            launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slot2, zbe2, tdp + maxparts,
                             n + 1 < N, s);
-            launch_adj_combine(h->m, pb, slotb, zbe, zvb, s);
-            // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb)
+            // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)
             launch_node_arg(h->m, u1, nodemm, nodearg, s);
-            launch_adj_limit(h->m, h->ph, u1, nodemm, pb, ub, vmb, actl, s);
+            launch_adj_limit(h->m, h->ph, u1, nodemm, pb, slot2, zbe2, zvb, ub, vmb, actl, s);
             launch_adj_node(h->m, vmb, nodeb, s);
             launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nodeb, nodearg, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);
-            launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slotb, zbe, tdp, false, s);
-            launch_adj_combine(h->m, wb0, slotb, zbe, zvb, s);
+            launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, false, s);
+            have_slot1 = true;
             launch_adj_scalar(h->m, dtp, tdp, maxparts, rn, n > 0 ? rn - nm : rn, n > 0, actl, s);
             double *t = wb;
             wb = wb0;
@@ -247,6 +268,9 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         }
         DGR_HIP(hipGetLastError());
     }
+    // the last stage-1 edge part: its z_edge adjoints (the state adjoint of
+    // the start is not returned)
+    launch_adj_combine(h->m, wb, slot1, zbe1, zvb, s);
     launch_adj_gather(h->m, zvb, nvb, gz, gn, s);
     DGR_HIP(hipGetLastError());
     dgr_status st = restore(0);

@@ -54,14 +54,18 @@ __global__ __launch_bounds__(256) void k_node_arg(MeshDev m, const double *__res
     }
 }
 
-// limiter reverse.  U: pre-limit stage output; Sb: adjoint of the limited
-// (and PDop'd) output.  Writes Ub (own part of the pre-limit adjoint) and
-// vmb[18][pitch]: adjoint of Vmin (f*3+k) and Vmax (9+f*3+k) at vertex k.
-// PDop is the identity when every vertex depth exceeds H0; an element where
-// it is not raises the member's `unsupported` flag.
+// limiter reverse.  U: pre-limit stage output; Sb (+ slotb when given): the
+// adjoint of the limited (and PDop'd) output, slotb being the edge part left
+// by the following stage's k_adj_edge, whose z_edge adjoints (zbe) are moved
+// to the element's vertices here too.  Writes Ub (own part of the pre-limit
+// adjoint) and vmb[18][pitch]: adjoint of Vmin (f*3+k) and Vmax (9+f*3+k) at
+// vertex k.  PDop is the identity when every vertex depth exceeds H0; an
+// element where it is not raises the member's `unsupported` flag.
 __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const double *__restrict__ U,
                                                    const double *__restrict__ nodemm, const double *__restrict__ Sb,
-                                                   double *__restrict__ Ub, double *__restrict__ vmb, AdjCtrl *actl)
+                                                   const double *__restrict__ slotb, const double *__restrict__ zbe,
+                                                   double *__restrict__ zvb, double *__restrict__ Ub,
+                                                   double *__restrict__ vmb, AdjCtrl *actl)
 {
     const ElemLane L = elem_lane(m);
     if (!L.valid) return;
@@ -72,7 +76,23 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
     for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
     double S[9], sb[9], ub[9], Vmin[3][3], Vmax[3][3], Vminb[3][3], Vmaxb[3][3];
 #pragma unroll
-    for (int f = 0; f < 9; ++f) { S[f] = U[f * P + e]; sb[f] = Sb[f * P + e]; ub[f] = 0.0; }
+    for (int f = 0; f < 9; ++f) {
+        S[f] = U[f * P + e];
+        sb[f] = Sb[f * P + e] + (slotb ? slotb[f * P + e] : 0.0);
+        ub[f] = 0.0;
+    }
+    if (zbe) {
+        // z_edge = (z_a + z_b)/2 of local edge k
+        double z[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double b = zbe[k * P + e];
+            z[k] += 0.5 * b;
+            z[(k + 1) % 3] += 0.5 * b;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) zvb[k * P + e] += z[k];
+    }
 #pragma unroll
     for (int f = 0; f < 3; ++f)
 #pragma unroll
@@ -140,7 +160,7 @@ __device__ __forceinline__ double block_sum(double v)
 
 // RK update + element RHS reverse.
 //   MODE 1: U = W + R dt             -> Wb += Ub + RHS'(W)^T (dt Ub)
-//   MODE 2: U = (W + B) 0.5 + R dt/2 -> Inb = 0.5 Ub + RHS'(W)^T (dt/2 Ub), Bb += 0.5 Ub
+//   MODE 2: U = (W + B) 0.5 + R dt/2 -> Inb = 0.5 Ub + RHS'(W)^T (dt/2 Ub), Bb = 0.5 Ub
 // W: stage input, B: step-start state (MODE 2), U: pre-limit output, Ubown:
 // own part of the pre-limit adjoint; the node-bound part is added here from
 // nodeb for the elements that attain a bound.  Outputs: Inb (MODE 2: the
@@ -205,7 +225,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
                 Inb[f * P + e] = 0.5 * ub[f] + Wb[f];
-                Bb[f * P + e] += 0.5 * ub[f];
+                Bb[f * P + e] = 0.5 * ub[f];   // first write of the step-start adjoint
             }
         }
 #pragma unroll
@@ -539,52 +559,55 @@ __global__ void k_adj_scalar(MeshDev m, const double *__restrict__ dtparts, cons
 }
 
 // loss gradient of the gauges sampled at this state: h_g = mean over the
-// element's midpoints of zeta + z_mid, loss = sum (h_g - obs)^2
+// element's midpoints of zeta + z_mid, loss = sum (h_g - obs)^2; one thread
+// per (member, gauge), atomic adds because a gauge element may repeat
 __global__ void k_adj_gauges(MeshDev m, int ngauge, const int *__restrict__ gauge_el,
-                             const double *__restrict__ hval, const double *__restrict__ obs, double *Wb,
-                             double *zvb, AdjCtrl *actl)
+                             const double *__restrict__ hval, const double *__restrict__ obs, double *Wb, double *zvb)
 {
-    const int mid = blockIdx.x * 64 + threadIdx.x;
-    if (mid >= m.nmem) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nmem * ngauge) return;
+    const int mid = t / ngauge;
     const MemberDev &M = m.mem[mid];
     const int P = m.pitch;
-    for (int gi = 0; gi < ngauge; ++gi) {
-        const int e = M.el_off + gauge_el[mid * ngauge + gi];
-        const double r = hval[mid * ngauge + gi] - obs[mid * ngauge + gi];
-        const double w = 2.0 * r / 3.0;
+    const int e = M.el_off + gauge_el[t];
+    const double w = 2.0 * (hval[t] - obs[t]) / 3.0;
 #pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            Wb[k * P + e] += w;
-            zvb[k * P + e] += w;   // z_mid_k = (z_k + z_k1)/2: each vertex in two midpoints
-        }
+    for (int k = 0; k < 3; ++k) {
+        atomicAdd(&Wb[k * P + e], w);
+        atomicAdd(&zvb[k * P + e], w);   // z_mid_k = (z_k + z_k1)/2: each vertex in two midpoints
     }
 }
 
-// forward sample: h at the gauges; adds the misfit to the member's loss
+// forward sample: h at the gauges, one thread per (member, gauge); with obs
+// the squared misfits are summed per member in gauge order
 __global__ void k_gauge_sample(MeshDev m, int ngauge, const int *__restrict__ gauge_el, const double *__restrict__ w,
-                               double *__restrict__ hval, const double *__restrict__ obs, AdjCtrl *actl)
+                               double *__restrict__ hval)
+{
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nmem * ngauge) return;
+    const MemberDev &M = m.mem[t / ngauge];
+    const int P = m.pitch;
+    const int e = M.el_off + gauge_el[t];
+    double h = 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double za = m.nodes[m.ev[k * P + e]].z, zb = m.nodes[m.ev[((k + 1) % 3) * P + e]].z;
+        h += w[k * P + e] + 0.5 * (za + zb);
+    }
+    hval[t] = h / 3.0;
+}
+
+__global__ void k_gauge_loss(MeshDev m, int ngauge, const double *__restrict__ hval, const double *__restrict__ obs,
+                             AdjCtrl *actl)
 {
     const int mid = blockIdx.x * 64 + threadIdx.x;
     if (mid >= m.nmem) return;
-    const MemberDev &M = m.mem[mid];
-    const int P = m.pitch;
     double loss = 0.0;
-    for (int gi = 0; gi < ngauge; ++gi) {
-        const int e = M.el_off + gauge_el[mid * ngauge + gi];
-        double h = 0.0;
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const double za = m.nodes[m.ev[k * P + e]].z, zb = m.nodes[m.ev[((k + 1) % 3) * P + e]].z;
-            h += w[k * P + e] + 0.5 * (za + zb);
-        }
-        h /= 3.0;
-        hval[mid * ngauge + gi] = h;
-        if (obs) {
-            const double r = h - obs[mid * ngauge + gi];
-            loss += r * r;
-        }
+    for (int g = 0; g < ngauge; ++g) {
+        const double r = hval[mid * ngauge + g] - obs[mid * ngauge + g];
+        loss += r * r;
     }
-    if (actl) actl[mid].loss += loss;
+    actl[mid].loss += loss;
 }
 
 // step record: at the start (t, dt, tide) or the end (max |lambda| of stage 2)
@@ -627,10 +650,11 @@ void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nod
 }
 
 void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
-                      double *Ub, double *vmb, AdjCtrl *actl, hipStream_t s)
+                      const double *slotb, const double *zbe, double *zvb, double *Ub, double *vmb, AdjCtrl *actl,
+                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_limit, dim3((unsigned)(m.pitch / 256)), dim3(256), 0, s, m, ph, U, nodemm, Sb, Ub, vmb,
-                       actl);
+    hipLaunchKernelGGL(k_adj_limit, dim3((unsigned)(m.pitch / 256)), dim3(256), 0, s, m, ph, U, nodemm, Sb, slotb, zbe,
+                       zvb, Ub, vmb, actl);
 }
 
 void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s)
@@ -681,17 +705,22 @@ void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *ti
 }
 
 void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,
-                       double *Wb, double *zvb, AdjCtrl *actl, hipStream_t s)
+                       double *Wb, double *zvb, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_gauges, dim3((unsigned)((m.nmem + 63) / 64)), dim3(64), 0, s, m, ngauge, gauge_el, hval,
-                       obs, Wb, zvb, actl);
+    const int n = m.nmem * ngauge;
+    hipLaunchKernelGGL(k_adj_gauges, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, ngauge, gauge_el, hval,
+                       obs, Wb, zvb);
 }
 
 void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
                          const double *obs, AdjCtrl *actl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_gauge_sample, dim3((unsigned)((m.nmem + 63) / 64)), dim3(64), 0, s, m, ngauge, gauge_el, w,
-                       hval, obs, actl);
+    const int n = m.nmem * ngauge;
+    hipLaunchKernelGGL(k_gauge_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, ngauge, gauge_el, w,
+                       hval);
+    if (obs && actl)
+        hipLaunchKernelGGL(k_gauge_loss, dim3((unsigned)((m.nmem + 63) / 64)), dim3(64), 0, s, m, ngauge, hval, obs,
+                           actl);
 }
 
 void launch_step_record(const MeshDev &m, const Ctrl *ctrls, StepRec *rec, int at_end, hipStream_t s)

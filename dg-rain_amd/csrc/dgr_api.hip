@@ -552,7 +552,8 @@ extern "C" dgr_status dgr_limit(dgr_handle *h)
     if (!h) return DGR_ERR_ARG;
     launch_means(h->m, h->d_w, h->d_means, h->stream);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, h->stream);
-    launch_limit(false, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, h->stream);
+    launch_limit(false, h->m, h->ph, h->d_w, h->d_w, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, nullptr, nullptr,
+                 h->stream);
     DGR_HIP(hipGetLastError());
     DGR_HIP(hipStreamSynchronize(h->stream));
     return DGR_OK;
@@ -561,7 +562,7 @@ extern "C" dgr_status dgr_limit(dgr_handle *h)
 extern "C" dgr_status dgr_wetdry_status(dgr_handle *h)
 {
     if (!h) return DGR_ERR_ARG;
-    launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, h->stream);
+    launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, h->d_wd, h->stream);
     DGR_HIP(hipGetLastError());
     DGR_HIP(hipStreamSynchronize(h->stream));
     return DGR_OK;
@@ -589,43 +590,38 @@ static void mark(dgr_handle *h)
 
 // the two stages of oneTimeStep2D (time_evolution.c:10-246; wet/dry calls at the
 // places ChanNet/time_evolution.c:218-427 puts them), enqueued on h->stream
-// for every member at once
-// With a tape, the step's start state, each stage's output before and after
-// the limiter and the wet/dry flags each stage used are copied out as well.
-void enqueue_stages(dgr_handle *h, const StepTape *tp)
+// for every member at once.  The wetDryStatus2D call between the stages runs
+// inside the stage-1 limiter kernel.  b == nullptr: the plain in-place buffers.
+void enqueue_stages(dgr_handle *h, const StepBufs *b)
 {
     const bool wdon = h->ph.wetdry != 0;
     hipStream_t s = h->stream;
-    const size_t W = 9 * (size_t)h->pitch * sizeof(double), I = (size_t)h->pitch * sizeof(int);
-    if (tp) (void)hipMemcpyAsync(tp->w0, h->d_w, W, hipMemcpyDeviceToDevice, s);
-    if (wdon) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, s);
-    if (tp) (void)hipMemcpyAsync(tp->wda, h->d_wd, I, hipMemcpyDeviceToDevice, s);
+    const StepBufs plain = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wd, h->d_wd};
+    if (!b) b = &plain;
+    if (wdon) launch_wd_status(h->m, h->ph, b->w0, b->wdprev, b->wda, s);
     mark(h);
-    h->kind[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, h->d_w, h->d_w1, nullptr, h->d_bzeta, h->d_bqn,
-                              h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
+    h->kind[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, b->w0, b->u1, nullptr, h->d_bzeta, h->d_bqn, b->wda,
+                              h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
     mark(h);
-    if (tp) (void)hipMemcpyAsync(tp->u1, h->d_w1, W, hipMemcpyDeviceToDevice, s);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
-    launch_limit(wdon, h->m, h->ph, h->d_w1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, s);
-    if (tp) (void)hipMemcpyAsync(tp->p1, h->d_w1, W, hipMemcpyDeviceToDevice, s);
-    if (wdon) launch_wd_status(h->m, h->ph, h->d_w1, h->d_wd, s);
-    if (tp) (void)hipMemcpyAsync(tp->wdb, h->d_wd, I, hipMemcpyDeviceToDevice, s);
+    launch_limit(wdon, h->m, h->ph, b->u1, b->p1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, b->wda,
+                 wdon ? b->wdb : nullptr, s);
+    const int *wd2 = wdon ? b->wdb : b->wda;
     mark(h);
-    h->kind[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, h->d_w1, h->d_w, h->d_w, h->d_bzeta, h->d_bqn,
-                              h->d_wd, h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
+    h->kind[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, b->p1, b->u2, b->w0, h->d_bzeta, h->d_bqn, wd2,
+                              h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     mark(h);
-    if (tp) (void)hipMemcpyAsync(tp->u2, h->d_w, W, hipMemcpyDeviceToDevice, s);
     // stage 2's max |lambda| sets the next dt; repair its fold if a NaN operand was seen
-    launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w1, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
-                 nullptr, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
+    launch_stage(h->variant, 3, wdon, h->m, h->ph, b->p1, nullptr, nullptr, h->d_bzeta, h->d_bqn, wd2, nullptr,
+                 h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
-    launch_limit(wdon, h->m, h->ph, h->d_w, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1, s);
+    launch_limit(wdon, h->m, h->ph, b->u2, b->wout, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1, nullptr, nullptr, s);
 }
 
-void enqueue_step(dgr_handle *h, const StepTape *tp)
+void enqueue_step(dgr_handle *h, const StepBufs *b)
 {
     launch_step_begin(h->m, h->d_ctrl, h->stream);
-    enqueue_stages(h, tp);
+    enqueue_stages(h, b);
     launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], h->stream);
 }
 

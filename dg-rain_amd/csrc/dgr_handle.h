@@ -16,10 +16,12 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
                  hipStream_t s);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
-void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
-                  unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s);
+void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
+                  const double *nodemm, unsigned char *modify, Ctrl *ctrl, int stage, const int *wd_in, int *wd_out,
+                  hipStream_t s);
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
-void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s);
+void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
+                      hipStream_t s);
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, hipStream_t s);
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
@@ -30,7 +32,8 @@ void launch_math_selftest(int n, const double *x, const double *y, double *out, 
 // adjoint sweep (dgr_adjoint_kernels.h)
 void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nodearg, hipStream_t s);
 void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
-                      double *Ub, double *vmb, AdjCtrl *actl, hipStream_t s);
+                      const double *slotb, const double *zbe, double *zvb, double *Ub, double *vmb, AdjCtrl *actl,
+                      hipStream_t s);
 void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s);
 void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
                     const double *U, const double *Ubown, const double *nodeb, const int *nodearg, const int *wd,
@@ -44,7 +47,7 @@ void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, cons
 void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
                        const StepRec *rec, const StepRec *rec_prev, int has_prev, AdjCtrl *actl, hipStream_t s);
 void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,
-                       double *Wb, double *zvb, AdjCtrl *actl, hipStream_t s);
+                       double *Wb, double *zvb, hipStream_t s);
 void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
                          const double *obs, AdjCtrl *actl, hipStream_t s);
 void launch_step_record(const MeshDev &m, const Ctrl *ctrls, StepRec *rec, int at_end, hipStream_t s);
@@ -114,18 +117,22 @@ static inline hipError_t dalloc(T **p, size_t n)
     return hipMalloc((void **)p, (n ? n : 1) * sizeof(T));
 }
 
-// device copies taken during one taped step (enqueue_stages)
-struct StepTape {
-    double *w0;   // state at the step's start
-    double *u1;   // stage 1 output before the limiter
-    double *p1;   // stage 1 output after limiter/PDop (stage 2 input)
-    double *u2;   // stage 2 output before the limiter
-    int *wda;     // wet/dry flags stage 1 used
-    int *wdb;     // wet/dry flags stage 2 used
+// Buffers of one step.  The plain step works in place (w0 = u2 = wout =
+// d_w, u1 = p1 = d_w1, every flag set in d_wd); a taped step keeps every
+// stage output and flag set in its own buffer for the reverse sweep.
+struct StepBufs {
+    double *w0;     // state at the step's start
+    double *u1;     // stage 1 output before the limiter
+    double *p1;     // stage 1 output after limiter/PDop (stage 2 input)
+    double *u2;     // stage 2 output before the limiter
+    double *wout;   // stage 2 output after limiter/PDop (next step's start)
+    const int *wdprev;   // wet/dry flags on entry (the previous step's stage-2 flags)
+    int *wda;       // wet/dry flags stage 1 uses
+    int *wdb;       // wet/dry flags stage 2 uses
 };
 
-void enqueue_stages(dgr_handle *h, const StepTape *tp);
-void enqueue_step(dgr_handle *h, const StepTape *tp);
+void enqueue_stages(dgr_handle *h, const StepBufs *b);
+void enqueue_step(dgr_handle *h, const StepBufs *b);
 dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c);
 dgr_status any_error(dgr_handle *h);
 unsigned char *modify_ptr(dgr_handle *h, int stage);

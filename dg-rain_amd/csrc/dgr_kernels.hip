@@ -941,10 +941,35 @@ __device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy
     return 0;
 }
 
+// wetDryStatus2D (wetDry2D.c:16-60) of one element from its zeta midpoints
+// (a, b, c), vertex depths zz and previous flag
+__device__ __forceinline__ int wd_status_of(double a, double b, double c, const double zz[3], double H0, int old)
+{
+    const double ze[3] = {a - b + c, a + b - c, -a + b + c};
+    const double h[3] = {ze[0] + zz[0], ze[1] + zz[1], ze[2] + zz[2]};
+    const double avgH = (h[0] + h[1] + h[2]) / 3;
+    int mi = 0;
+    double mh = h[0];
+    if (h[1] > mh) { mi = 1; mh = h[1]; }
+    if (h[2] > mh) { mi = 2; mh = h[2]; }
+    const double zmax = ze[mi];
+    double zmin = zz[0];
+    if (zz[1] < zmin) zmin = zz[1];
+    if (zz[2] < zmin) zmin = zz[2];
+    const int t1 = (avgH - H0) > 0;
+    const int t2 = (zmax - (H0 - zmin)) > 0;
+    return (old == -1) ? t1 * t2 : old * t1 + (1 - old) * t1 * t2;
+}
+
 // limiter element pass (SlopeLimiter2D.c:157-343), optionally followed by PDop2D
+// win -> wout (may alias).  With PDOP, the wet/dry status of the limited
+// state follows (the reference's next wetDryStatus2D call, which always comes
+// right after the limiter): wd_out from wd_in (may alias).
 template <bool PDOP>
-__global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w, const double *__restrict__ nodemm,
-                                               unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage)
+__global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const double *win, double *wout,
+                                               const double *__restrict__ nodemm,
+                                               unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
+                                               const int *wd_in, int *wd_out)
 {
     const ElemLane L = elem_lane(m);
     if (!L.valid) return;
@@ -957,7 +982,7 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w,
     for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
     double S[9];
 #pragma unroll
-    for (int f = 0; f < 9; ++f) S[f] = w[f * P + e];
+    for (int f = 0; f < 9; ++f) S[f] = win[f * P + e];
     int modify = 0;
 #pragma unroll
     for (int iv = 0; iv < 3; ++iv) {
@@ -1003,9 +1028,10 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, double *w,
     if (PDOP) {
         const int err = pdop_element(&S[0], &S[3], &S[6], zv, H0, ph.velzero);
         if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 2 * stage + 1);
+        if (wd_out) wd_out[e] = wd_status_of(S[0], S[1], S[2], zv, H0, wd_in[e]);
     }
 #pragma unroll
-    for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
+    for (int f = 0; f < 9; ++f) wout[f * P + e] = S[f];
     if (modify_out) modify_out[e] = (unsigned char)modify;
 }
 
@@ -1027,33 +1053,18 @@ __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, 
     for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
 }
 
-// wetDryStatus2D (wetDry2D.c:16-60)
-__global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const double *__restrict__ w, int *__restrict__ wd)
+// standalone wetDryStatus2D: wd_out from the state and wd_in (may alias)
+__global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const double *__restrict__ w,
+                                                   const int *wd_in, int *wd_out)
 {
     const ElemLane L = elem_lane(m);
     if (!L.valid) return;
     const int e = L.e;
     const int P = m.pitch;
-    const double H0 = ph.h0;
-    const double a = w[e], b = w[P + e], c = w[2 * P + e];
-    const double ze[3] = {a - b + c, a + b - c, -a + b + c};
     double zz[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) zz[k] = m.nodes[m.ev[k * P + e]].z;
-    const double h[3] = {ze[0] + zz[0], ze[1] + zz[1], ze[2] + zz[2]};
-    const double avgH = (h[0] + h[1] + h[2]) / 3;
-    int mi = 0;
-    double mh = h[0];
-    if (h[1] > mh) { mi = 1; mh = h[1]; }
-    if (h[2] > mh) { mi = 2; mh = h[2]; }
-    const double zmax = ze[mi];
-    double zmin = zz[0];
-    if (zz[1] < zmin) zmin = zz[1];
-    if (zz[2] < zmin) zmin = zz[2];
-    const int t1 = (avgH - H0) > 0;
-    const int t2 = (zmax - (H0 - zmin)) > 0;
-    const int old = wd[e];
-    wd[e] = (old == -1) ? t1 * t2 : old * t1 + (1 - old) * t1 * t2;
+    wd_out[e] = wd_status_of(w[e], w[P + e], w[2 * P + e], zz, ph.h0, wd_in[e]);
 }
 
 __device__ __forceinline__ void clear_fold(Ctrl *c)
@@ -1204,11 +1215,16 @@ void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, h
     hipLaunchKernelGGL(k_node_minmax, grid_for(m.nn), dim3(256), 0, s, m, means, nodemm);
 }
 
-void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, double *w, const double *nodemm,
-                  unsigned char *modify, Ctrl *ctrl, int stage, hipStream_t s)
+void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
+                  const double *nodemm, unsigned char *modify, Ctrl *ctrl, int stage, const int *wd_in, int *wd_out,
+                  hipStream_t s)
 {
-    if (pdop) hipLaunchKernelGGL(k_limit<true>, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
-    else hipLaunchKernelGGL(k_limit<false>, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, nodemm, modify, ctrl, stage);
+    if (pdop)
+        hipLaunchKernelGGL(k_limit<true>, grid_for(m.pitch), dim3(256), 0, s, m, ph, win, wout, nodemm, modify, ctrl,
+                           stage, wd_in, wd_out);
+    else
+        hipLaunchKernelGGL(k_limit<false>, grid_for(m.pitch), dim3(256), 0, s, m, ph, win, wout, nodemm, modify, ctrl,
+                           stage, wd_in, wd_out);
 }
 
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s)
@@ -1216,9 +1232,10 @@ void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hip
     hipLaunchKernelGGL(k_pdop, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, ctrl);
 }
 
-void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, int *wd, hipStream_t s)
+void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
+                      hipStream_t s)
 {
-    hipLaunchKernelGGL(k_wd_status, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, wd);
+    hipLaunchKernelGGL(k_wd_status, grid_for(m.pitch), dim3(256), 0, s, m, ph, w, wd_in, wd_out);
 }
 
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s)
