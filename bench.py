@@ -37,6 +37,11 @@ STAGE_KERNELS = {0: "k_stage (fused flux gather + RHS + RK update)",
                  2: "k_edge_flux + k_rhs (edge-centric fluxes, then RHS + RK update)"}
 
 
+def progress(msg):
+    """A progress line on stderr (the JSON line alone goes to stdout)."""
+    print("bench: " + msg, file=sys.stderr, flush=True)
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -48,6 +53,9 @@ def parse():
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost measurement")
     ap.add_argument("--adj-N", type=int, default=256)
+    ap.add_argument("--no-large", action="store_true", help="skip the 2048x2048 run and the batched adjoints")
+    ap.add_argument("--large-steps", type=int, default=20)
+    ap.add_argument("--batch-adj-steps", type=int, default=100)
     ap.add_argument("--adj-steps", type=int, default=2000)
     ap.add_argument("--ens-members", type=int, default=64)
     ap.add_argument("--ens-steps", type=int, default=20, help="timed steps of the ensemble and of each solo run")
@@ -192,6 +200,60 @@ def adjoint_bench(a, variant):
             "forward_s": t_fwd, "adjoint_s": t_adj, "ratio": t_adj / t_fwd}
 
 
+def large_bench(a, variant):
+    """part 1, the 2048 x 2048 mesh (8,388,608 triangles, 40 km,
+    128 bumps, tide + 20 mm/h rain) timed over a few steps; part 2, 16 gauge
+    gradients on one 512 x 512 sub-mesh (64 seeded gauges each), batched as
+    the 16 members of one ensemble handle (one launch per reverse kernel for
+    all 16), against the forward run of the same batch."""
+    import numpy as np
+    from dgrain import configs, native
+    wl = configs.large()
+    progress("large mesh built (%d triangles)" % wl.num_el)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_variant(variant)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    s.run(3)
+    s.sync()
+    t0 = time.perf_counter()
+    s.run(a.large_steps)
+    s.sync()
+    el = time.perf_counter() - t0
+    s.close()
+    ne = wl.num_el
+    part1 = {"workload": wl.name, "triangles": ne, "steps": a.large_steps,
+             "value": 9.0 * ne * 2 * a.large_steps / el, "unit": "DOF-updates/s",
+             "ms_per_step": el / a.large_steps * 1e3,
+             "stage_roofline_frac": STAGE_BYTES_PER_EL * ne * 2 * a.large_steps / el / 1e9 / HBM_PEAK_GBS}
+    del wl
+    sub = configs.coastal(N=512, start="flood")
+    nb = 16
+    rng = np.random.default_rng(6)
+    gauges = np.stack([np.sort(rng.choice(sub.num_el, 64, replace=False)) for _ in range(nb)]).astype(np.int32)
+    ens = native.Ensemble([sub.mesh] * nb, [sub.solver_params() for _ in range(nb)])
+    progress("batched-adjoint ensemble created")
+    ens.set_variant(variant)
+    for i in range(nb):
+        ens.set_state(i, sub.zeta, sub.qx, sub.qy)
+    S = a.batch_adj_steps
+    obs = ens.gauge_run(S, 10, gauges) * (1.0 + 1e-4)
+    for i in range(nb):
+        ens.set_state(i, sub.zeta, sub.qx, sub.qy)
+    t0 = time.perf_counter()
+    loss, gz, gn = ens.adjoint_gauges(S, 10, 50, gauges, obs)
+    ens.sync()
+    t_adj = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    ens.gauge_run(S, 10, gauges)
+    ens.sync()
+    t_fwd = time.perf_counter() - t0
+    ens.close()
+    part2 = {"workload": "16 gradients, coastal_flood_512x512 sub-mesh, 64 gauges each", "members": nb,
+             "triangles": nb * sub.num_el, "steps": S, "checkpoint_every": 50, "forward_s": t_fwd,
+             "adjoint_s": t_adj, "ratio": t_adj / t_fwd, "loss_per_member": [float(x) for x in loss]}
+    return {"forward": part1, "batched_adjoint": part2}
+
+
 def main():
     a = parse()
     if a.copy_calibration:
@@ -259,14 +321,21 @@ def main():
         "adjoint_forward_cost": None,
     }
     s.close()
+    progress("single instance: %.3e DOF-updates/s, %.4f ms/step" % (value, el / a.steps * 1e3))
     if not a.no_ensemble:
         e = ensemble_bench(a, variant)
         out["ensemble"] = e
         out["ensemble_speedup"] = e["speedup"]
+        progress("ensemble: speedup %.2f" % e["speedup"])
     if not a.no_adjoint:
         ad = adjoint_bench(a, variant)
         out["adjoint"] = ad
         out["adjoint_forward_cost"] = ad["ratio"]
+        progress("adjoint: %.2f x forward" % ad["ratio"])
+    if not a.no_large:
+        out["large"] = large_bench(a, variant)
+        progress("large: %.3e DOF-updates/s; batched adjoint %.2f x forward"
+                 % (out["large"]["forward"]["value"], out["large"]["batched_adjoint"]["ratio"]))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
     print(json.dumps(out))

@@ -735,6 +735,59 @@ extern "C" dgr_status dgr_device_state(dgr_handle *h, double **planes, int *pitc
     return DGR_OK;
 }
 
+// Restart file of one member: "DGRCKPT1", int32 num_el, double t, dt, then
+// zeta, Qx, Qy (3*num_el doubles each, reference layout) and WD (num_el
+// int32), host byte order.
+extern "C" dgr_status dgr_checkpoint_write(dgr_handle *h, int mi, const char *path)
+{
+    if (bad_member(h, mi) || !path) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[mi];
+    const size_t n3 = 3 * (size_t)M.ne;
+    std::vector<double> st(3 * n3);
+    std::vector<int> wd(M.ne);
+    dgr_status r = dgr_member_get_state(h, mi, st.data(), st.data() + n3, st.data() + 2 * n3);
+    if (r) return r;
+    r = dgr_member_get_wd(h, mi, wd.data());
+    if (r) return r;
+    Ctrl c;
+    r = read_ctrl(h, mi, &c);
+    if (r) return r;
+    FILE *f = fopen(path, "wb");
+    if (!f) return DGR_ERR_ARG;
+    const int ne = M.ne;
+    bool ok = fwrite("DGRCKPT1", 1, 8, f) == 8 && fwrite(&ne, sizeof(int), 1, f) == 1 &&
+              fwrite(&c.t, sizeof(double), 1, f) == 1 && fwrite(&c.dt, sizeof(double), 1, f) == 1 &&
+              fwrite(st.data(), sizeof(double), st.size(), f) == st.size() &&
+              fwrite(wd.data(), sizeof(int), wd.size(), f) == wd.size();
+    ok = (fclose(f) == 0) && ok;
+    return ok ? DGR_OK : DGR_ERR_ARG;
+}
+
+extern "C" dgr_status dgr_checkpoint_read(dgr_handle *h, int mi, const char *path)
+{
+    if (bad_member(h, mi) || !path) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[mi];
+    FILE *f = fopen(path, "rb");
+    if (!f) return DGR_ERR_ARG;
+    char magic[8];
+    int ne = 0;
+    double t = 0, dt = 0;
+    const size_t n3 = 3 * (size_t)M.ne;
+    std::vector<double> st(3 * n3);
+    std::vector<int> wd(M.ne);
+    bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "DGRCKPT1", 8) == 0 && fread(&ne, sizeof(int), 1, f) == 1 &&
+              ne == M.ne && fread(&t, sizeof(double), 1, f) == 1 && fread(&dt, sizeof(double), 1, f) == 1 &&
+              fread(st.data(), sizeof(double), st.size(), f) == st.size() &&
+              fread(wd.data(), sizeof(int), wd.size(), f) == wd.size();
+    fclose(f);
+    if (!ok) return DGR_ERR_ARG;
+    dgr_status r = dgr_member_set_state(h, mi, st.data(), st.data() + n3, st.data() + 2 * n3);
+    if (r) return r;
+    r = dgr_member_set_wd(h, mi, wd.data());
+    if (r) return r;
+    return dgr_member_set_time(h, mi, t, dt);
+}
+
 extern "C" dgr_status dgr_member_layout(dgr_handle *h, int mi, int *el_offset, int *num_el)
 {
     if (bad_member(h, mi)) return DGR_ERR_ARG;

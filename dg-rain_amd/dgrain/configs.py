@@ -127,6 +127,15 @@ def ensemble(n=64, seed=3, nmin=64, nmax=512, nsteps=500, L_per_cell=1.0e4 / 512
     return out
 
 
+def large(N=2048, L=4.0e4, nsteps=1000, seed=5):
+    """The large single-GPU run: N x N over L x L (2048: 8,388,608 triangles),
+    the coastal bathymetry statistics scaled to the domain (128 bumps, radii
+    x4), tide 1.5 m plus rain 20 mm/h, flood start."""
+    wl = coastal(N=N, L=L, nsteps=nsteps, seed=seed, nbumps=128, rain_mm_h=20.0, bump_scale=L / 1.0e4, level=3.5)
+    wl.name = "large_%dx%d" % (N, N)
+    return wl
+
+
 def dry_bed_dam_break(N=16, L=10.0, nsteps=100):
     """Dam break onto a dry flat bed (the pattern of initialize.c:65-73):
     H = 0.005 m for x <= L/2, H0 elsewhere; wet/dry on, walls."""
@@ -155,4 +164,6 @@ WORKLOADS = {
     "coastal": coastal,
     "lake_at_rest": lake_at_rest,
     "dry_bed_dam_break": dry_bed_dam_break,
+    "ensemble": ensemble,
+    "large": large,
 }

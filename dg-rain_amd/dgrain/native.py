@@ -109,6 +109,8 @@ def load():
         "dgr_adjoint_gauges": (C.c_int, [vp, C.POINTER(AdjointSpec), dp, dp, dp]),
         "dgr_gauge_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, ip, dp]),
         "dgr_set_nodal": (C.c_int, [vp, C.c_int, dp, dp]),
+        "dgr_checkpoint_write": (C.c_int, [vp, C.c_int, C.c_char_p]),
+        "dgr_checkpoint_read": (C.c_int, [vp, C.c_int, C.c_char_p]),
         "dgr_profile_end": (C.c_int, [vp, dp, ip]),
     }
     for name, (res, args) in sig.items():
@@ -269,6 +271,13 @@ class _Handle:
         gn = np.zeros(self._num_nodes())
         check(self.lib.dgr_adjoint_gauges(self.h, C.byref(spec), _dp(loss), _dp(gz), _dp(gn)), "dgr_adjoint_gauges")
         return loss, gz, gn
+
+    def checkpoint_write(self, path, member=0):
+        """Restart file of a member: state, wet/dry flags, t and dt."""
+        check(self.lib.dgr_checkpoint_write(self.h, int(member), os.fsencode(path)), "dgr_checkpoint_write")
+
+    def checkpoint_read(self, path, member=0):
+        check(self.lib.dgr_checkpoint_read(self.h, int(member), os.fsencode(path)), "dgr_checkpoint_read")
 
     def set_nodal(self, z=None, n=None, member=0):
         """Replace the nodal depth z and/or Manning n of a member."""

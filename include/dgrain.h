@@ -224,6 +224,13 @@ dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every, int ngauge
 /* Replace a member's nodal depth z and/or Manning n (NULL keeps it). */
 dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, const double *n);
 
+/* Restart files (extension): one member's state, wet/dry flags, time and dt.
+ * Layout: "DGRCKPT1", int32 num_el, double t, double dt, zeta/Qx/Qy
+ * (3*num_el doubles each, el*3+k), WD (num_el int32), host byte order.
+ * Reading checks the tag and num_el (DGR_ERR_ARG otherwise). */
+dgr_status dgr_checkpoint_write(dgr_handle *h, int member, const char *path);
+dgr_status dgr_checkpoint_read(dgr_handle *h, int member, const char *path);
+
 /* Synchronise the handle's stream. */
 dgr_status dgr_sync(dgr_handle *h);
 

@@ -222,3 +222,55 @@ def test_autotune_keeps_state(gpu):
     assert len(ms) == native.NUM_VARIANTS
     assert all(m > 0 for m in ms), ms
     _compare_run(s, o, 20, 10)
+
+
+@pytest.mark.parametrize("wetdry", [0, 1])
+def test_lake_at_rest_on_gpu(gpu, wetdry):
+    """Still water over a bumpy bed (walls, 32x32 grid, 500 steps, limiter on):
+    the surface stays flat to 1e-13 (the north-star bar on h).  The momentum
+    is not exactly balanced by the reference scheme itself: it drifts by
+    rounding to ~6e-12 after 500 steps, identically in the oracle's
+    reference-math mode (checked here to 1e-11) and bit for bit in its
+    device-math mode."""
+    wl = configs.lake_at_rest(N=32)
+    p = dict(wl.params, wetdry=wetdry)
+    s = native.Solver(wl.mesh, native.default_params(**p))
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    s.run(500)
+    z, qx, qy = s.get_state()
+    assert np.max(np.abs(z - wl.zeta)) <= 1e-13, np.max(np.abs(z - wl.zeta))
+    assert np.max(np.abs(qx)) <= 1e-11 and np.max(np.abs(qy)) <= 1e-11, (np.max(np.abs(qx)), np.max(np.abs(qy)))
+    o = oracle.Oracle(wl.mesh, math=oracle.MATH_DEV, **p)
+    o.set_state(wl.zeta, wl.qx, wl.qy)
+    o.run(500)
+    for a, b in zip((z, qx, qy), o.get_state()):
+        assert same(a, b)
+    r = oracle.Oracle(wl.mesh, math=oracle.MATH_REF, **p)
+    r.set_state(wl.zeta, wl.qx, wl.qy)
+    r.run(500)
+    zr, qxr, qyr = r.get_state()
+    assert np.max(np.abs(zr - wl.zeta)) <= 1e-13
+    assert np.max(np.abs(qxr)) <= 1e-11 and np.max(np.abs(qyr)) <= 1e-11
+
+
+def test_checkpoint_restart(gpu, tmp_path):
+    """A restart file written after 15 steps and read into a new handle gives
+    the bits of the uninterrupted run (state, flags, t, dt)."""
+    wl = configs.coastal(N=16, rain_mm_h=10.0)
+    a = native.Solver(wl.mesh, wl.solver_params())
+    a.set_state(wl.zeta, wl.qx, wl.qy)
+    a.run(15)
+    path = str(tmp_path / "member0.ckpt")
+    a.checkpoint_write(path)
+    ra = a.run(15)
+    b = native.Solver(wl.mesh, wl.solver_params())
+    b.checkpoint_read(path)
+    rb = b.run(15)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert same(x, y)
+    assert np.array_equal(a.get_wd(), b.get_wd())
+    assert ra.t == rb.t and ra.dt == rb.dt
+    with open(path, "r+b") as f:
+        f.write(b"XXXXXXXX")
+    with pytest.raises(native.DgrError):
+        b.checkpoint_read(path)
