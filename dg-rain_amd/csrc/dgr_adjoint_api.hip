@@ -184,7 +184,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         launch_step_begin(h->m, h->d_ctrl, s);
         launch_step_record(h->m, h->d_ctrl, rec + (size_t)nm * n, 0, s);
         enqueue_stages(h, nullptr);
-        launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], s);
+        launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], false, s);
         launch_step_record(h->m, h->d_ctrl, rec + (size_t)nm * n, 1, s);
         if ((n + 1) % sp->sample_every == 0) {
             const int k = (n + 1) / sp->sample_every - 1;
@@ -220,7 +220,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         for (int n = n0; n < n1; ++n) {
             const int k = n - n0;
             const bool last = n + 1 == n1;
-            StepBufs bf;
+            StepBufs bf{};
             bf.w0 = tw0 + W9 * k;
             bf.u1 = tu1 + W9 * k;
             bf.p1 = tp1 + W9 * k;

@@ -61,7 +61,8 @@ static dgr_status free_handle(dgr_handle *h)
     if (!h) return DGR_OK;
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_edges,
                     h->d_einfo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
-                    h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts};
+                    h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts,
+                    h->d_nanparts, h->d_wdn};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -258,6 +259,8 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     DGR_TRY(dalloc(&h->d_modify, 2 * P));
     DGR_TRY(dalloc(&h->d_ctrl, n));
     DGR_TRY(dalloc(&h->d_lamparts, 2 * (size_t)maxparts));
+    DGR_TRY(dalloc(&h->d_nanparts, 2 * (size_t)maxparts));
+    DGR_TRY(dalloc(&h->d_wdn, P));
     DGR_TRY(hipMemcpyAsync(h->d_nodes, T.nodes.data(), nn_tot * sizeof(double4), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_ev, T.ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_nb, T.nb.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
@@ -279,9 +282,11 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     DGR_TRY(hipMemsetAsync(h->d_fhat, 0, 9 * P * sizeof(double), s));
     DGR_TRY(hipMemsetAsync(h->d_modify, 0, 2 * P, s));
     DGR_TRY(hipMemsetAsync(h->d_lamparts, 0, 2 * (size_t)maxparts * sizeof(unsigned long long), s));
+    DGR_TRY(hipMemsetAsync(h->d_nanparts, 0xff, 2 * (size_t)maxparts * sizeof(int), s));
     {
         std::vector<int> wd(P, -1);   // initialize.c:126
         DGR_TRY(hipMemcpyAsync(h->d_wd, wd.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
+        DGR_TRY(hipMemcpyAsync(h->d_wdn, wd.data(), P * sizeof(int), hipMemcpyHostToDevice, s));
         std::vector<Ctrl> c(n);
         for (int i = 0; i < n; ++i) {
             memset(&c[i], 0, sizeof(Ctrl));
@@ -301,7 +306,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     m.mem = h->d_mem; m.eblk_mem = h->d_eblk_mem; m.dblk_mem = h->d_dblk_mem;
     PhysDev &ph = h->ph;
     ph.g = prms[0].g; ph.h0 = prms[0].h0; ph.velzero = prms[0].velzero; ph.wetdry = prms[0].wetdry;
-    ph.lamparts = h->d_lamparts; ph.maxparts = maxparts;
+    ph.lamparts = h->d_lamparts; ph.nanparts = h->d_nanparts; ph.maxparts = maxparts;
     size_t maxne = 0;
     for (int i = 0; i < n; ++i) maxne = maxne > (size_t)mem[i].ne ? maxne : (size_t)mem[i].ne;
     h->h_planes.resize(9 * maxne);
@@ -528,9 +533,12 @@ extern "C" dgr_status dgr_compute_rhs(dgr_handle *h, double *rz, double *rqx, do
     h->kind[0] = launch_stage(h->variant, 0, wdon, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn,
                               h->d_wd, nullptr, h->d_fhat, h->d_ctrl, 0, h->d_dbg, h->d_dbg + 9 * (size_t)h->pitch,
                               h->stream);
+    // element partials: repair the fold after a NaN operand before reducing
+    // (the edge kernel's partials carry the NaN position themselves)
+    if (h->kind[0] == PARTS_ELEM)
+        launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd,
+                     nullptr, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, h->stream);
     launch_lam_reduce(h->m, h->ph, h->d_ctrl, 0, h->kind[0], h->stream);
-    launch_stage(h->variant, 3, wdon, h->m, h->ph, h->d_w, nullptr, nullptr, h->d_bzeta, h->d_bqn, h->d_wd, nullptr,
-                 h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, h->stream);
     DGR_HIP(hipGetLastError());
     if (rz || rqx || rqy) {
         dgr_status st = planes_down(h, 0, h->d_dbg, rz, rqx, rqy);
@@ -596,9 +604,9 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b)
 {
     const bool wdon = h->ph.wetdry != 0;
     hipStream_t s = h->stream;
-    const StepBufs plain = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wd, h->d_wd};
+    const StepBufs plain = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wd, h->d_wd, nullptr, false};
     if (!b) b = &plain;
-    if (wdon) launch_wd_status(h->m, h->ph, b->w0, b->wdprev, b->wda, s);
+    if (wdon && !b->wda_ready) launch_wd_status(h->m, h->ph, b->w0, b->wdprev, b->wda, s);
     mark(h);
     h->kind[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, b->w0, b->u1, nullptr, h->d_bzeta, h->d_bqn, b->wda,
                               h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
@@ -611,18 +619,41 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b)
     h->kind[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, b->p1, b->u2, b->w0, h->d_bzeta, h->d_bqn, wd2,
                               h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     mark(h);
-    // stage 2's max |lambda| sets the next dt; repair its fold if a NaN operand was seen
-    launch_stage(h->variant, 3, wdon, h->m, h->ph, b->p1, nullptr, nullptr, h->d_bzeta, h->d_bqn, wd2, nullptr,
-                 h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
+    // stage 2's max |lambda| sets the next dt.  Element partials: repair the
+    // fold if a NaN operand was seen (edge partials carry the NaN position).
+    if (h->kind[1] == PARTS_ELEM)
+        launch_stage(h->variant, 3, wdon, h->m, h->ph, b->p1, nullptr, nullptr, h->d_bzeta, h->d_bqn, wd2, nullptr,
+                     h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
-    launch_limit(wdon, h->m, h->ph, b->u2, b->wout, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1, nullptr, nullptr, s);
+    const bool wdn = wdon && b->wdnext;
+    launch_limit(wdon, h->m, h->ph, b->u2, b->wout, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1,
+                 wdn ? wd2 : nullptr, wdn ? b->wdnext : nullptr, s);
 }
 
 void enqueue_step(dgr_handle *h, const StepBufs *b)
 {
     launch_step_begin(h->m, h->d_ctrl, h->stream);
     enqueue_stages(h, b);
-    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], h->stream);
+    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], false, h->stream);
+}
+
+// The steps of dgr_run.  Before the first: the step bookkeeping and the
+// start flags into d_wdn.  Each step then takes its start flags from d_wdn,
+// keeps the stage-2 flags in d_wd (as the reference leaves WD after a step),
+// writes the next step's start flags into d_wdn from its stage-2 limiter and
+// ends with the next step's bookkeeping: no separate status or step-begin
+// launch per step.
+static void enqueue_run_prologue(dgr_handle *h)
+{
+    launch_step_begin(h->m, h->d_ctrl, h->stream);
+    if (h->ph.wetdry) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, h->d_wdn, h->stream);
+}
+
+static void enqueue_run_step(dgr_handle *h)
+{
+    const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true};
+    enqueue_stages(h, &run);
+    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], true, h->stream);
 }
 
 extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)
@@ -646,7 +677,7 @@ static dgr_status step_graph(dgr_handle *h)
     }
     hipGraph_t g = nullptr;
     DGR_HIP(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    enqueue_step(h, nullptr);
+    enqueue_run_step(h);
     const hipError_t le = hipGetLastError();
     const hipError_t ce = hipStreamEndCapture(h->stream, &g);
     if (le != hipSuccess || ce != hipSuccess) {
@@ -675,12 +706,14 @@ extern "C" dgr_status dgr_set_graph(dgr_handle *h, int on)
 extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
 {
     if (!h || nsteps < 0) return DGR_ERR_ARG;
-    if (h->use_graph && !h->profiling && nsteps > 0) {
+    if (nsteps > 0 && h->use_graph && !h->profiling) {
         dgr_status st = step_graph(h);
         if (st) return st;
+        enqueue_run_prologue(h);
         for (int n = 0; n < nsteps; ++n) DGR_HIP(hipGraphLaunch(h->graph_exec, h->stream));
-    } else {
-        for (int n = 0; n < nsteps; ++n) enqueue_step(h, nullptr);
+    } else if (nsteps > 0) {
+        enqueue_run_prologue(h);
+        for (int n = 0; n < nsteps; ++n) enqueue_run_step(h);
     }
     DGR_HIP(hipGetLastError());
     if (rep) {
@@ -804,12 +837,12 @@ extern "C" dgr_status dgr_math_selftest(int n, const double *x, const double *y,
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
     DGR_HIP(dalloc(&dx, n));
     DGR_HIP(dalloc(&dy, n));
-    DGR_HIP(dalloc(&dout, 5 * (size_t)n));
+    DGR_HIP(dalloc(&dout, 7 * (size_t)n));
     DGR_HIP(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
     DGR_HIP(hipMemcpy(dy, y, n * sizeof(double), hipMemcpyHostToDevice));
     launch_math_selftest(n, dx, dy, dout, 0);
     DGR_HIP(hipGetLastError());
-    DGR_HIP(hipMemcpy(out, dout, 5 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    DGR_HIP(hipMemcpy(out, dout, 7 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(dx);
     (void)hipFree(dy);
     (void)hipFree(dout);

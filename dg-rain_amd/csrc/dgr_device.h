@@ -143,6 +143,10 @@ struct PhysDev {
     // raw bits; reduced per member into Ctrl::maxlam by k_step_end /
     // k_lam_reduce (one plain store per workgroup, no same-address atomics)
     unsigned long long *lamparts;
+    // edge kernel only, [2][maxparts]: member-local position of the last NaN
+    // operand in the workgroup's edge range (-1: none); lamparts then holds
+    // the max of the operands after it (-1.0: none)
+    int *nanparts;
     int maxparts;
 };
 

@@ -23,7 +23,7 @@ void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hip
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
                       hipStream_t s);
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
-void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, hipStream_t s);
+void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s);
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
 void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s);
 void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s);
@@ -87,6 +87,8 @@ struct dgr_handle {
     unsigned char *d_modify;   // [2][pitch]
     Ctrl *d_ctrl;              // [nmem]
     unsigned long long *d_lamparts;   // [2][maxparts] per-workgroup max |lambda|
+    int *d_nanparts;                  // [2][maxparts] last NaN operand per edge workgroup
+    int *d_wdn;      // [pitch] wet/dry flags for the next step's start (dgr_run)
     int variant;     // 0 fused, 1 split element-centric, 2 split edge-centric
     int kind[2];     // partials space of the last flux launch of each stage
     // host staging
@@ -129,6 +131,10 @@ struct StepBufs {
     const int *wdprev;   // wet/dry flags on entry (the previous step's stage-2 flags)
     int *wda;       // wet/dry flags stage 1 uses
     int *wdb;       // wet/dry flags stage 2 uses
+    int *wdnext;    // if set: the stage-2 limiter also writes the next step's
+                    // start flags (wetDryStatus2D of wout from wdb) here
+    bool wda_ready; // wda already holds this step's start flags (a previous
+                    // step wrote them through wdnext): no status launch
 };
 
 void enqueue_stages(dgr_handle *h, const StepBufs *b);

@@ -537,6 +537,42 @@ __device__ __forceinline__ void lam_commit(double lam, const PhysDev &ph, int sl
     }
 }
 
+// Workgroup partial of the reference's in-order max fold over the edge
+// kernel's contiguous edge range.  Thread t passes the position of its last
+// NaN operand tp (-1: none), the max tm of its operands after it (all of them
+// if none; -1.0: none) and its first operand position.  The partial is the
+// last NaN position P of the range and the max of the operands after it:
+// tm of the thread holding P and of every thread that starts after P.
+// blockDim.x is 256.
+__device__ __forceinline__ void seg_commit(int tp, double tm, int first, const PhysDev &ph, int slot, int lb)
+{
+    __shared__ int rp[4];
+    __shared__ double rm[4];
+    // one pass for P and the max of every tm (the answer when P = -1), a
+    // second one only for a workgroup that saw a NaN
+    int P = wave_max(tp);
+    double M = wave_max(tm);
+    if ((threadIdx.x & 63) == 0) { rp[threadIdx.x >> 6] = P; rm[threadIdx.x >> 6] = M; }
+    __syncthreads();
+    P = rp[0];
+    M = rm[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) { P = (rp[i] > P) ? rp[i] : P; M = (rm[i] > M) ? rm[i] : M; }
+    if (P >= 0) {   // uniform over the workgroup
+        __syncthreads();
+        M = wave_max((tp == P || first > P) ? tm : -1.0);
+        if ((threadIdx.x & 63) == 0) rm[threadIdx.x >> 6] = M;
+        __syncthreads();
+        M = rm[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i) M = (rm[i] > M) ? rm[i] : M;
+    }
+    if (threadIdx.x == 0) {
+        ph.lamparts[(size_t)slot * ph.maxparts + lb] = (unsigned long long)__double_as_longlong(M);
+        ph.nanparts[(size_t)slot * ph.maxparts + lb] = P;
+    }
+}
+
 // max of n partials by one block of 256 threads; the result is valid in thread 0
 __device__ __forceinline__ unsigned long long block_reduce_parts(const unsigned long long *p, int n)
 {
@@ -714,7 +750,8 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
     const MemberDev &M = m.mem[mid];
     Ctrl *ctrl = ctrls + mid;
     const int ed = lb * 256 + (int)threadIdx.x;
-    double lam = 0.0;
+    int tp = -1, first = -2;
+    double tm = -1.0;
     if (ed < M.ed_off + M.ned) {
         const int P = m.pitch;
         const double g = ph.g;
@@ -788,7 +825,16 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             nan_flux = isnan(Fi[0]) || isnan(Fi[1]) || isnan(Fi[2]);
         }
         if (nan_flux) raise_error(ctrl, 3, led, 2 * slot);
-        lam = lam_fold(ops, ctrl, slot);
+        // in-order fold of this edge's operands (positions p0, p0 + 1); a NaN
+        // restarts the reference's fold
+#pragma unroll
+        for (int i = 0; i < 2; ++i) {
+            if (ops.pos[i] < 0) continue;
+            const double x = ops.v[i];
+            if (x != x) { tp = (int)ops.pos[i]; tm = -1.0; }
+            else tm = (x > tm) ? x : tm;
+        }
+        first = (int)p0;
 #pragma unroll
         for (int f = 0; f < 3; ++f) fhat[(f * 3 + kin) * P + el_in] = Fi[f];
         if (!boundary) {
@@ -796,7 +842,7 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             for (int f = 0; f < 3; ++f) fhat[(f * 3 + kex) * P + el_ex] = -Fx[f];
         }
     }
-    lam_commit(lam, ph, slot, lb);
+    seg_commit(tp, tm, first, ph, slot, lb);
 }
 
 // Split variants, part 2: inner products, RHS and RK update from the Fhat planes.
@@ -984,6 +1030,7 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const doub
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = win[f * P + e];
     int modify = 0;
+    const Divisor D3 = divisor(3.0);
 #pragma unroll
     for (int iv = 0; iv < 3; ++iv) {
         const double w1 = S[3 * iv], w2 = S[3 * iv + 1], w3 = S[3 * iv + 2];
@@ -991,7 +1038,7 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const doub
         double Vmin[3], Vmax[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) { Vmin[k] = nodemm[iv * nn + nd[k]]; Vmax[k] = nodemm[(3 + iv) * nn + nd[k]]; }
-        const double avg = (w1 + w2 + w3) / 3;
+        const double avg = quot(w1 + w2 + w3, D3);
         if (iv == 0) {
             const double Ha = V[0] + zv[0], Hb = V[1] + zv[1], Hc = V[2] + zv[2];
             const double Hm0 = Vmin[0] + zv[0], Hm1 = Vmin[1] + zv[1], Hm2 = Vmin[2] + zv[2];
@@ -1001,17 +1048,21 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const doub
         for (int k = 0; k < 3; ++k) V[k] = fmax(DGR_MIN(V[k], Vmax[k]), Vmin[k]);
 #pragma unroll
         for (int ll = 0; ll < 3; ++ll) {
-            const double sumloc = (V[0] + V[1] + V[2]) / 3;
+            const double sumloc = quot(V[0] + V[1] + V[2], D3);
             double sumdiff = (sumloc - avg) * 3;
             const int sd = isign(sumdiff);
             const double diff[3] = {(V[0] - avg) * sd, (V[1] - avg) * sd, (V[2] - avg) * sd};
             double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                const double dv = fmax(1.0, kdp);
+                // sumdiff*sd / max(1, kdp) with kdp in {0,1,2,3}: x/1 = x and
+                // x/2 = x*0.5 exactly
                 double redfac;
-                if (diff[k] > 0) { redfac = sumdiff * sd / dv; kdp -= 1; }
-                else redfac = 0;
+                if (diff[k] > 0) {
+                    const double num = sumdiff * sd;
+                    redfac = (kdp > 2.0) ? quot(num, D3) : (kdp > 1.0) ? num * 0.5 : num;
+                    kdp -= 1;
+                } else redfac = 0;
                 const double redmax = (sd > 0) ? V[k] - Vmin[k] : Vmax[k] - V[k];
                 redfac = DGR_MIN(redfac, redmax);
                 sumdiff -= redfac * sd;
@@ -1082,14 +1133,31 @@ __device__ __forceinline__ void clear_fold(Ctrl *c)
 // per-step bookkeeping (time_evolution.c:311-402), one thread per member:
 // begin evaluates the tide at the step's start time; end advances t and sets
 // the next dt from stage 2's max |lambda|.
+// the NaN bookkeeping of the element-kernel folds (Ctrl::maxlam is written by
+// the reductions before it is read)
+__device__ __forceinline__ void clear_nan_fold(Ctrl *c)
+{
+    c->nan_pos[0] = -1;
+    c->nan_pos[1] = -1;
+    c->after_bits[0] = 0ULL;
+    c->after_bits[1] = 0ULL;
+    c->any_after[0] = 0;
+    c->any_after[1] = 0;
+}
+
+__device__ __forceinline__ void step_tide(const MemberDev &M, Ctrl *c)
+{
+    if (M.tide_code)
+        c->tide = M.tide_datum + M.tide_amp * dgr_sin(6.283185307179586 * c->t / M.tide_period + M.tide_phase);
+}
+
 __global__ __launch_bounds__(64) void k_step_begin(MeshDev m, Ctrl *ctrls)
 {
     const int mid = blockIdx.x * 64 + threadIdx.x;
     if (mid >= m.nmem) return;
     const MemberDev &M = m.mem[mid];
     Ctrl *c = ctrls + mid;
-    if (M.tide_code)
-        c->tide = M.tide_datum + M.tide_amp * dgr_sin(6.283185307179586 * c->t / M.tide_period + M.tide_phase);
+    step_tide(M, c);
     clear_fold(c);
     c->step += 1;
 }
@@ -1115,29 +1183,101 @@ __device__ __forceinline__ unsigned long long member_lam(const MeshDev &m, const
     return block_reduce_parts(ph.lamparts + (size_t)slot * ph.maxparts + b0, nb);
 }
 
-// one workgroup per member
-__global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind)
+// The reference's in-order max fold (compute2DL.c:232-238,289-292) over one
+// member's operands from the edge kernel's workgroup partials (seg_commit);
+// member-local workgroup b holds operand positions [512 b, 512 b + 512).  The
+// fold starts at 0; a NaN operand restarts it, so the result is the max of
+// the operands after the last NaN, or NaN if none follows.  One workgroup of
+// 256 threads; the result is valid in every thread.
+__device__ __forceinline__ double member_fold_edges(const MeshDev &m, const PhysDev &ph, int mid, int slot)
+{
+    __shared__ int rp[4];
+    __shared__ double rm[4];
+    const MemberDev &M = m.mem[mid];
+    const int *np = ph.nanparts + (size_t)slot * ph.maxparts + M.dblk0;
+    const unsigned long long *lp = ph.lamparts + (size_t)slot * ph.maxparts + M.dblk0;
+    // one pass for P and the max of all partials (the answer when P = -1),
+    // a second one only when the member saw a NaN operand
+    int P = -1;
+    double R = -1.0;
+    for (int b = threadIdx.x; b < M.dblks; b += 256) {
+        const double v = __longlong_as_double((long long)lp[b]);
+        P = (np[b] > P) ? np[b] : P;
+        R = (v > R) ? v : R;
+    }
+    P = wave_max(P);
+    R = wave_max(R);
+    if ((threadIdx.x & 63) == 0) { rp[threadIdx.x >> 6] = P; rm[threadIdx.x >> 6] = R; }
+    __syncthreads();
+    P = rp[0];
+    R = rm[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) { P = (rp[i] > P) ? rp[i] : P; R = (rm[i] > R) ? rm[i] : R; }
+    if (P < 0) return (R > 0.0) ? R : 0.0;
+    __syncthreads();
+    R = -1.0;
+    for (int b = threadIdx.x; b < M.dblks; b += 256) {
+        const double v = __longlong_as_double((long long)lp[b]);
+        if ((np[b] == P || 512LL * b > P) && v > R) R = v;
+    }
+    R = wave_max(R);
+    if ((threadIdx.x & 63) == 0) rm[threadIdx.x >> 6] = R;
+    __syncthreads();
+    R = rm[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) R = (rm[i] > R) ? rm[i] : R;
+    return (R < 0.0) ? __longlong_as_double(0x7ff8000000000000LL) : R;
+}
+
+// the fold of one member and stage from the last flux launch's partials,
+// stored resolved in Ctrl::maxlam[slot] (thread 0)
+__device__ __forceinline__ double resolve_lam(const MeshDev &m, const PhysDev &ph, Ctrl *c, int mid, int slot,
+                                              int kind)
+{
+    if (kind == PARTS_EDGE) {
+        const double ml = member_fold_edges(m, ph, mid, slot);
+        if (threadIdx.x == 0) {
+            c->maxlam[slot] = (unsigned long long)__double_as_longlong(ml);
+            c->nan_pos[slot] = -1;
+        }
+        return ml;
+    }
+    const unsigned long long b = member_lam(m, ph, mid, slot, kind);
+    if (threadIdx.x != 0) return 0.0;
+    c->maxlam[slot] = b;
+    const double ml = maxlam_of(*c, slot);
+    c->maxlam[slot] = (unsigned long long)__double_as_longlong(ml);
+    c->nan_pos[slot] = -1;
+    return ml;
+}
+
+// one workgroup per member.  With next != 0 the next step's bookkeeping
+// follows (the tide at the new time, the NaN bookkeeping cleared, the step
+// counted), which k_step_begin does otherwise.
+__global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind, int next)
 {
     const int mid = blockIdx.x;
-    const unsigned long long b = member_lam(m, ph, mid, 1, kind);
+    Ctrl *c = ctrls + mid;
+    const double ml = resolve_lam(m, ph, c, mid, 1, kind);
     if (threadIdx.x != 0) return;
     const MemberDev &M = m.mem[mid];
-    Ctrl *c = ctrls + mid;
-    c->maxlam[1] = b;
     const double time = c->t + c->dt;
-    const double ml = maxlam_of(*c, 1);
     double h = M.cfl * M.min_edge / ml;
     if ((time + h) > M.final_time) h = M.final_time - time;
     c->t = time;
     c->dt = h;
+    if (next) {
+        step_tide(M, c);
+        clear_nan_fold(c);
+        c->step += 1;
+    }
 }
 
 // Ctrl::maxlam[slot] of every member from the partials of the last flux launch
 __global__ __launch_bounds__(256) void k_lam_reduce(MeshDev m, PhysDev ph, Ctrl *ctrls, int slot, int kind)
 {
     const int mid = blockIdx.x;
-    const unsigned long long b = member_lam(m, ph, mid, slot, kind);
-    if (threadIdx.x == 0) ctrls[mid].maxlam[slot] = b;
+    (void)resolve_lam(m, ph, ctrls + mid, mid, slot, kind);
 }
 
 __global__ __launch_bounds__(64) void k_reset_maxlam(MeshDev m, Ctrl *ctrls)
@@ -1151,11 +1291,13 @@ __global__ void k_math_selftest(int n, const double *x, const double *y, double 
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double a = x[i], b = y[i];
-    out[5 * i + 0] = sqrt(a);
-    out[5 * i + 1] = a / b;
-    out[5 * i + 2] = dgr_pow43(a);
-    out[5 * i + 3] = dgr_sin(a);
-    out[5 * i + 4] = fma(a, b, a);
+    out[7 * i + 0] = sqrt(a);
+    out[7 * i + 1] = a / b;
+    out[7 * i + 2] = dgr_pow43(a);
+    out[7 * i + 3] = dgr_sin(a);
+    out[7 * i + 4] = fma(a, b, a);
+    out[7 * i + 5] = quot(a, divisor(b));
+    out[7 * i + 6] = quot(a, divisor(3.0));
 }
 
 // ---------------------------------------------------------------------------
@@ -1243,9 +1385,9 @@ void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s)
     hipLaunchKernelGGL(k_step_begin, member_grid(m), dim3(64), 0, s, m, c);
 }
 
-void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, hipStream_t s)
+void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_step_end, dim3((unsigned)m.nmem), dim3(256), 0, s, m, ph, c, kind);
+    hipLaunchKernelGGL(k_step_end, dim3((unsigned)m.nmem), dim3(256), 0, s, m, ph, c, kind, next ? 1 : 0);
 }
 
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s)

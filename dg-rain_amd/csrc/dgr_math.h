@@ -86,6 +86,56 @@ __device__ inline double dgr_pow43(double x)
     return (res * sc) * sc;
 }
 
+// ---------------------------------------------------------------------------
+// Several divisions by one denominator, each bit-identical to x / y.
+//
+// gfx950 divides doubles (correctly rounded) with
+//   d = v_div_scale(y, y, x); r = v_rcp(d);
+//   r = fma(r, fma(-d, r, 1), r) twice;
+//   n = v_div_scale(x, y, x); q = n * r; e = fma(-d, q, n);
+//   result = v_div_fixup(v_div_fmas(e, r, q), y, x).
+// When no operand needs exponent scaling (|y| in [2^-320, 2^320] and |x/y| in
+// [2^-300, 2^300]: far from zero, denormals and overflow) both div_scale
+// steps return their first operand with the scale flag clear, div_fmas is the
+// plain fma and div_fixup returns its first operand.  Only q, e and the final
+// fma then depend on x; Divisor holds the rest, and quot() repeats the
+// x-dependent part for each numerator (3 instructions instead of 11).  Any
+// result outside the range above (also 0, inf, NaN) is recomputed as x / y.
+// ---------------------------------------------------------------------------
+struct Divisor {
+    double y, r;
+    bool ok;
+};
+
+__device__ __forceinline__ Divisor divisor(double y)
+{
+    double d = y;
+    // opaque to the optimiser: a constant y must still go through v_rcp_f64
+    // (constant folding would give the correctly rounded 1/y instead)
+    asm volatile("" : "+v"(d));
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    const double a = fabs(y);
+    return Divisor{y, r, a >= 0x1p-320 && a <= 0x1p320};
+}
+
+__device__ __forceinline__ double quot(double x, const Divisor &D)
+{
+    const double q = x * D.r;
+    const double e = fma(-D.y, q, x);
+    const double res = fma(e, D.r, q);
+    const double a = fabs(res);
+    if (D.ok && a >= 0x1p-300 && a <= 0x1p300) return res;
+    // the barrier keeps the full division inside this rarely taken branch
+    // (otherwise it is computed for every call and selected)
+    double y = D.y;
+    asm volatile("" : "+v"(y));
+    return x / y;
+}
+
 // Cody-Waite reduction by pi/2 (33+33+53-bit split of pi/2) and Taylor
 // polynomials through x^15 (sin) / x^16 (cos) on [-pi/4, pi/4].
 __device__ inline double dgr_sin(double x)

@@ -439,7 +439,7 @@ def math_selftest(x, y):
     lib = load()
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
-    out = np.empty((x.size, 5))
+    out = np.empty((x.size, 7))
     check(lib.dgr_math_selftest(x.size, _dp(x), _dp(y), _dp(out)), "dgr_math_selftest")
     return out
 

@@ -239,7 +239,8 @@ dgr_status dgr_sync(dgr_handle *h);
 dgr_status dgr_device_state(dgr_handle *h, double **planes, int *pitch);
 
 /* Self-test of the device math against host-supplied inputs: out[i] =
- * {sqrt(x), x/y, dgr_pow43(x), dgr_sin(x), fma(x,y,x)} for each pair. */
+ * {sqrt(x), x/y, dgr_pow43(x), dgr_sin(x), fma(x,y,x), x/y and x/3 by the
+ * shared-denominator division} for each pair (7 doubles). */
 dgr_status dgr_math_selftest(int n, const double *x, const double *y, double *out);
 
 #ifdef __cplusplus

@@ -38,6 +38,40 @@ def test_math_selftest_matches_host(gpu):
     assert same(out[:, 3], osin)
 
 
+def _bits_equal(a, b):
+    """Bitwise equality, the sign of zero included (any NaN == any NaN)."""
+    return bool(np.all((a.view(np.uint64) == b.view(np.uint64)) | (np.isnan(a) & np.isnan(b))))
+
+
+def test_shared_denominator_division(gpu):
+    """quot(x, divisor(y)) is x / y bit for bit, on ordinary operands (the
+    3-instruction path) and on zeros, denormals, huge values, inf and NaN
+    (the fallback), including values around the fallback's thresholds."""
+    rng = np.random.default_rng(11)
+    n = 2_000_000
+    sx = rng.choice([-1.0, 1.0], n)
+    sy = rng.choice([-1.0, 1.0], n)
+    x = sx * np.exp2(rng.uniform(-60, 60, n)) * rng.uniform(1, 2, n)
+    y = sy * np.exp2(rng.uniform(-60, 60, n)) * rng.uniform(1, 2, n)
+    # operands whose quotient or divisor straddles 2^-300, 2^300, 2^-320, 2^320
+    m = 200_000
+    ex = rng.choice([-330.0, -320.0, -310.0, -300.0, -290.0, 290.0, 300.0, 310.0, 320.0, 330.0], m)
+    x2 = np.exp2(ex + rng.uniform(-2, 2, m)) * rng.uniform(1, 2, m)
+    y2 = np.exp2(rng.choice([-330.0, -320.0, -10.0, 0.0, 10.0, 320.0, 330.0], m) + rng.uniform(-2, 2, m))
+    special = np.array([0.0, -0.0, np.inf, -np.inf, np.nan, 5e-324, -5e-324, 1e-310, 1e308, -1e308,
+                        1.0, 3.0, 1.5, 2.0 ** 1023, 2.0 ** -1022])
+    xs, ys = np.meshgrid(special, special)
+    x = np.concatenate([x, x2, y2, xs.ravel(), rng.uniform(-1e4, 1e4, 1000)])
+    y = np.concatenate([y, y2, x2, ys.ravel(), rng.uniform(-1e4, 1e4, 1000)])
+    out = native.math_selftest(x, y)
+    with np.errstate(all="ignore"):
+        q = x / y
+        q3 = x / 3.0
+    assert _bits_equal(out[:, 1], q)
+    assert _bits_equal(out[:, 5], q)
+    assert _bits_equal(out[:, 6], q3)
+
+
 @pytest.mark.parametrize("wetdry", [0, 1])
 @pytest.mark.parametrize("seed", [1, 2])
 def test_compute2DL(gpu, wetdry, seed):
@@ -57,6 +91,67 @@ def test_compute2DL(gpu, wetdry, seed):
     for a, b in zip(rg, ro):
         assert same(a, b)
     assert lg == lo
+
+
+def _dry_state(mesh, dry, seed):
+    """Random state whose `dry` elements hold still water: zero depth (zeta =
+    -z_edge) at the midpoints of edges shared with another dry element or on
+    the boundary, 5 cm elsewhere.  The both-dry edges then give NaN
+    max-|lambda| operands (0/0 velocities), which restart the reference's
+    fold, while every wet/dry edge stays finite."""
+    z, qx, qy = random_state(mesh, seed=seed)
+    ev = np.asarray(mesh.elto_vert).reshape(-1, 3)
+    zn = np.asarray(mesh.z)
+    owner = {}
+    for e in range(ev.shape[0]):
+        for k in range(3):
+            owner.setdefault(frozenset((ev[e, k], ev[e, (k + 1) % 3])), []).append(e)
+    for e in np.flatnonzero(dry):
+        for k in range(3):
+            a, b = ev[e, k], ev[e, (k + 1) % 3]
+            els = owner[frozenset((a, b))]
+            other = [x for x in els if x != e]
+            ze = (zn[a] + zn[b]) / 2
+            z[3 * e + k] = -ze if (not other or dry[other[0]]) else 0.05 - ze
+            qx[3 * e + k] = 0.0
+            qy[3 * e + k] = 0.0
+    return z, qx, qy
+
+
+@pytest.mark.parametrize("variant", range(native.NUM_VARIANTS))
+@pytest.mark.parametrize("pattern", ["band", "tail", "all"])
+def test_max_lambda_fold_with_nan_operands(gpu, variant, pattern):
+    """max |lambda| when NaN operands occur (both-dry edges at zero depth):
+    the fold restarts after each NaN, so the result is the max of the operands
+    after the last one (band: NaNs in the middle of the edge order; tail: NaNs
+    up to the last few edges, whose operands alone give the result; all: every
+    operand NaN, result NaN), for every kernel variant, against the oracle's
+    left fold."""
+    mesh = perturbed_mesh(N=12, seed=4)
+    ne = mesh.num_el
+    dry = np.zeros(ne, bool)
+    if pattern == "band":
+        dry[ne // 3: ne // 2] = True
+    elif pattern == "tail":
+        dry[ne - ne // 4: ne - 4] = True
+    else:
+        dry[:] = True
+    s, o = _pair(mesh, 1)
+    s.set_variant(variant)
+    z, qx, qy = _dry_state(mesh, dry, seed=variant + 30)
+    wd = np.where(dry, 0, 1).astype(np.int32)
+    for obj in (s, o):
+        obj.set_state(z, qx, qy)
+        obj.set_wd(wd)
+    rg, fg, lg = s.compute2DL()
+    ro, fo, lo = o.compute2DL()
+    assert same(np.array([lg]), np.array([lo])), (lg, lo)
+    if pattern == "all":
+        assert np.isnan(lg)
+    else:
+        assert np.isfinite(lg) and lg > 0
+    for a, b in zip(fg, fo):
+        assert same(a, b)
 
 
 def test_limiter_wd_pdop(gpu):
