@@ -172,13 +172,17 @@ DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, c
     uhb += nx * (lhb[0] + lhb[1] + lhb[2]);
     vhb += ny * (lhb[0] + lhb[1] + lhb[2]);
     cb += -lhb[0] + lhb[2];
-    Hib += sgib * g / (2 * sgi);
-    Heb += sgeb * g / (2 * sge);
+    // the reverse sweep multiplies by reciprocals (the forward values above
+    // keep the reference's divisions: they decide the active branches)
+    Hib += sgib * (0.5 * g / sgi);
+    Heb += sgeb * (0.5 * g / sge);
     // Q = num / den and the constant-numerator entries
+    const double rden = 1.0 / den;
     double denb = 0;
     for (int r = 0; r < 3; ++r)
-        for (int s = 0; s < 3; ++s) denb += -Q[r][s] * Qb[r][s] / den;
-    const double n0b = Qb[0][0] / den, n1b = Qb[1][0] / den, n2b = Qb[2][0] / den;
+        for (int s = 0; s < 3; ++s) denb += -Q[r][s] * Qb[r][s];
+    denb *= rden;
+    const double n0b = Qb[0][0] * rden, n1b = Qb[1][0] * rden, n2b = Qb[2][0] * rden;
     cb += -n0b;
     uhb += -nx * n0b;
     vhb += -ny * n0b;
@@ -188,22 +192,23 @@ DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, c
     cb += -n2b;
     uhb += nx * n2b;
     vhb += ny * n2b;
-    cb += 2 * ny * Qb[1][1] / den - 2 * nx * Qb[1][2] / den;
+    cb += (2 * ny * Qb[1][1] - 2 * nx * Qb[1][2]) * rden;
     cb += -2 * denb;
     // c = sqrt(g Hh)
-    const double Hhb = cb * g / (2 * c);
+    const double Hhb = cb * (0.5 * g / c);
     Hib += 0.5 * Hhb;
     Heb += 0.5 * Hhb;
     // Roe averages
-    const double numub = uhb / S, numvb = vhb / S;
-    const double Sb = -uh * uhb / S - vh * vhb / S;
+    const double rS = 1.0 / S;
+    const double numub = uhb * rS, numvb = vhb * rS;
+    const double Sb = -(uh * uhb + vh * vhb) * rS;
     uib += sHi * numub;
     ueb += sHe * numub;
     vib += sHi * numvb;
     veb += sHe * numvb;
     const double sHib = ui * numub + vi * numvb + Sb, sHeb = ue * numub + ve * numvb + Sb;
-    Hib += sHib / (2 * sHi);
-    Heb += sHeb / (2 * sHe);
+    Hib += sHib * (0.5 / sHi);
+    Heb += sHeb * (0.5 / sHe);
     // physical normal fluxes
     {
         const double f1x = nx * fnb[0], f1y = ny * fnb[0], f2x = nx * fnb[1], f2y = ny * fnb[1];
@@ -220,14 +225,13 @@ DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, c
         zbb += -lg * a.zb * (f2x + f3y);
     }
     // velocities
-    ib.qxi += uib / Hi;
-    Hib += -ui * uib / Hi;
-    ib.qyi += vib / Hi;
-    Hib += -vi * vib / Hi;
-    ib.qxe += ueb / He;
-    Heb += -ue * ueb / He;
-    ib.qye += veb / He;
-    Heb += -ve * veb / He;
+    const double rHi = 1.0 / Hi, rHe = 1.0 / He;
+    ib.qxi += uib * rHi;
+    ib.qyi += vib * rHi;
+    Hib += -(ui * uib + vi * vib) * rHi;
+    ib.qxe += ueb * rHe;
+    ib.qye += veb * rHe;
+    Heb += -(ue * ueb + ve * veb) * rHe;
     // jumps
     ib.zi += jb[0];
     ib.ze -= jb[0];
@@ -253,15 +257,16 @@ DGR_HD Rt0 rt0_map(double M11, double M12, double M13, double M21, double M22, d
                    double M33, double den)
 {
     Rt0 m;
-    m.A[0] = (M23 * M32 - M22 * M33) / den;
-    m.A[1] = (M12 * M33 - M13 * M32) / den;
-    m.A[2] = (M13 * M22 - M12 * M23) / den;
-    m.B[0] = -(M23 * M31 - M21 * M33) / den;
-    m.B[1] = -(M11 * M33 - M13 * M31) / den;
-    m.B[2] = -(M13 * M21 - M11 * M23) / den;
-    m.C[0] = (M22 * M31 - M21 * M32) / den;
-    m.C[1] = (M11 * M32 - M12 * M31) / den;
-    m.C[2] = (M12 * M21 - M11 * M22) / den;
+    const double r = 1.0 / den;
+    m.A[0] = (M23 * M32 - M22 * M33) * r;
+    m.A[1] = (M12 * M33 - M13 * M32) * r;
+    m.A[2] = (M13 * M22 - M12 * M23) * r;
+    m.B[0] = -(M23 * M31 - M21 * M33) * r;
+    m.B[1] = -(M11 * M33 - M13 * M31) * r;
+    m.B[2] = -(M13 * M21 - M11 * M23) * r;
+    m.C[0] = (M22 * M31 - M21 * M32) * r;
+    m.C[1] = (M11 * M32 - M12 * M31) * r;
+    m.C[2] = (M12 * M21 - M11 * M22) * r;
     return m;
 }
 
@@ -323,10 +328,11 @@ DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double
         const double mz = G(GX_MZ + k), gnn = G(GX_GNN + k);
         const double zr = W[k], qx = W[3 + k], qy = W[6 + k];
         const double H = zr + mz;
-        const double u = qx / H, v = qy / H;
+        const double rH = 1.0 / H;
+        const double u = qx * rH, v = qy * rH;
         const double sp = sqrt(u * u + v * v);
-        const double H43 = H * cbrt(H);
-        const double tau = gnn * sp / H43;
+        const double rH43 = 1.0 / (H * cbrt(H));
+        const double tau = gnn * sp * rH43;
         double Hb = 0, qxb = 0, qyb = 0, zrb = 0, mzb_explicit = 0;
         // Fx/Fy adjoints from the normal point fluxes
         const double fxb[3] = {nxs[k] * Fnpb[0][k], nxs[k] * Fnpb[1][k], nxs[k] * Fnpb[2][k]};
@@ -335,17 +341,17 @@ DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double
         qxb += fxb[0];
         qyb += fyb[0];
         // Fx1 = qx^2/H + 0.5 lg (H^2 - mz^2)
-        qxb += 2 * qx / H * fxb[1];
-        Hb += (-qx * qx / (H * H) + lg * H) * fxb[1];
+        qxb += 2 * u * fxb[1];
+        Hb += (-u * u + lg * H) * fxb[1];
         mzb_explicit += -lg * mz * fxb[1];
         // Fx2 = Fy1 = qx qy / H
         const double x2 = fxb[2] + fyb[1];
-        qxb += qy / H * x2;
-        qyb += qx / H * x2;
-        Hb += -qx * qy / (H * H) * x2;
+        qxb += v * x2;
+        qyb += u * x2;
+        Hb += -u * v * x2;
         // Fy2 = qy^2/H + 0.5 lg (H^2 - mz^2)
-        qyb += 2 * qy / H * fyb[2];
-        Hb += (-qy * qy / (H * H) + lg * H) * fyb[2];
+        qyb += 2 * v * fyb[2];
+        Hb += (-v * v + lg * H) * fyb[2];
         mzb_explicit += -lg * mz * fyb[2];
         // friction fx = aw tau qx, fy = aw tau qy enter with a minus sign
         const double qeb = eb[3 + k], reb = eb[6 + k];
@@ -353,14 +359,15 @@ DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double
         qxb += -aw * tau * qeb;
         qyb += -aw * tau * reb;
         // tau = gnn sp / H^(4/3)
-        gnnb[k] += sp / H43 * taub;
-        const double spb = gnn / H43 * taub;
-        Hb += -(4.0 / 3.0) * tau / H * taub;
+        gnnb[k] += sp * rH43 * taub;
+        const double spb = gnn * rH43 * taub;
+        Hb += -(4.0 / 3.0) * tau * rH * taub;
         if (sp > 0) {
-            const double ub = u / sp * spb, vb = v / sp * spb;
-            qxb += ub / H;
-            qyb += vb / H;
-            Hb += -(u * ub + v * vb) / H;
+            const double rsp = spb / sp;
+            const double ub = u * rsp, vb = v * rsp;
+            qxb += ub * rH;
+            qyb += vb * rH;
+            Hb += -(u * ub + v * vb) * rH;
         }
         // bed slope terms bx = bgx zeta, by = bgy zeta (zero when dry)
         if (wd_self != 0) {

@@ -377,7 +377,7 @@ __global__ __launch_bounds__(256) void k_adj_ties(MeshDev m, PhysDev ph, const d
 // of the tide value's adjoint).  With lamb != 0 the edges whose |lambda|
 // operand equals the recorded max (rec_lam) share lamb.
 template <bool WETDRY>
-__global__ __launch_bounds__(256) void k_adj_edge(MeshDev m, PhysDev ph, const double *__restrict__ win,
+__global__ __launch_bounds__(256, 2) void k_adj_edge(MeshDev m, PhysDev ph, const double *__restrict__ win,
                                                   const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                   const int *__restrict__ wd, const double *__restrict__ Fhb,
                                                   const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_lam,
@@ -404,66 +404,79 @@ __global__ __launch_bounds__(256) void k_adj_edge(MeshDev m, PhysDev ph, const d
             Fin_b[f] = Fhb[(f * 3 + s.kin) * P + s.el_in];
             Fex_b[f] = s.boundary ? 0.0 : -Fhb[(f * 3 + s.kex) * P + s.el_ex];
         }
-        // adjoints of the edge-level inputs
+        // adjoints of the edge-level inputs.  Up to two Roe evaluations per
+        // edge: (wet) one with g; (one side dry) the g flux and the lg = 0
+        // flux of the same states; (both dry) el_in's and el_ex's reflection
+        // fluxes.  One roe_eval/roe_adj in a loop keeps a single copy of the
+        // reverse code (and its registers) for every case.
         double zinb = 0, qxinb = 0, qyinb = 0, zexb = 0, qxexb = 0, qyexb = 0, zbb = 0;
         double Ar[2][2];
         reflect_matrix(s.nx, s.ny, s.dn, Ar);
-        bool done = false;
+        int mode = 0;   // 0 wet, 1 one side dry, 2 both dry
+        bool in_wet = true, ex_wet = true;
         if (WETDRY) {
             const int wd_in = wd[s.el_in], wd_ex = s.boundary ? wd_in : wd[s.el_ex];
-            if (wd_in == 0 && wd_ex == 0) {
-                double rx, ry, sx, sy, F[3];
-                reflect(s.qxin, s.qyin, s.nx, s.ny, s.dn, rx, ry);
-                RoeIn a = {s.zin, s.zin, s.qxin, rx, s.qyin, ry, s.zb};
-                double l0 = roe_eval(F, a, s.nx, s.ny, 0.0, g);
-                RoeIn ib = {0, 0, 0, 0, 0, 0, 0};
-                roe_adj(a, s.nx, s.ny, 0.0, g, Fin_b, l0 == target ? lam_share : 0.0, ib);
+            in_wet = wd_in == 1;
+            ex_wet = s.boundary ? in_wet : (wd_ex == 1);
+            if (wd_in == 0 && wd_ex == 0) mode = 2;
+            else if (!(in_wet && ex_wet)) mode = 1;
+        }
+        RoeIn acc = {0, 0, 0, 0, 0, 0, 0};   // identity-mapped input adjoints (modes 0, 1)
+        const int nev = (mode == 0) ? 1 : 2;
+#pragma unroll 1
+        for (int j = 0; j < nev; ++j) {
+            RoeIn a;
+            double lg = g, Fb[3];
+            bool lam_op = true;
+            if (mode == 2) {
+                lg = 0.0;
+                if (j == 0) {
+                    double rx, ry;
+                    reflect(s.qxin, s.qyin, s.nx, s.ny, s.dn, rx, ry);
+                    a = RoeIn{s.zin, s.zin, s.qxin, rx, s.qyin, ry, s.zb};
+                } else {
+                    double sx, sy;
+                    reflect(s.qxex, s.qyex, s.nx, s.ny, s.dn, sx, sy);
+                    a = RoeIn{s.zex, s.zex, sx, s.qxex, sy, s.qyex, s.zb};
+                }
+#pragma unroll
+                for (int f = 0; f < 3; ++f) Fb[f] = (j == 0) ? Fin_b[f] : Fex_b[f];
+            } else {
+                a = RoeIn{s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
+                if (mode == 0) {
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) Fb[f] = Fin_b[f] + Fex_b[f];
+                } else if (j == 0) {   // the g flux feeds the wet sides
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) Fb[f] = (in_wet ? Fin_b[f] : 0.0) + (ex_wet ? Fex_b[f] : 0.0);
+                } else {               // the lg = 0 flux the dry sides (|lambda| counted once)
+                    lg = 0.0;
+                    lam_op = false;
+#pragma unroll
+                    for (int f = 0; f < 3; ++f) Fb[f] = (in_wet ? 0.0 : Fin_b[f]) + (ex_wet ? 0.0 : Fex_b[f]);
+                }
+            }
+            double F[3];
+            const double l = roe_eval(F, a, s.nx, s.ny, g, g);   // |lambda| does not depend on lg
+            RoeIn ib = (mode == 2) ? RoeIn{0, 0, 0, 0, 0, 0, 0} : acc;
+            roe_adj(a, s.nx, s.ny, lg, g, Fb, (lam_op && l == target) ? lam_share : 0.0, ib);
+            if (mode != 2) {
+                acc = ib;
+            } else if (j == 0) {
                 zinb += ib.zi + ib.ze;
                 qxinb += ib.qxi + Ar[0][0] * ib.qxe + Ar[1][0] * ib.qye;
                 qyinb += ib.qyi + Ar[0][1] * ib.qxe + Ar[1][1] * ib.qye;
                 zbb += ib.zb;
-                reflect(s.qxex, s.qyex, s.nx, s.ny, s.dn, sx, sy);
-                RoeIn c = {s.zex, s.zex, sx, s.qxex, sy, s.qyex, s.zb};
-                const double l1 = roe_eval(F, c, s.nx, s.ny, 0.0, g);
-                RoeIn ic = {0, 0, 0, 0, 0, 0, 0};
-                roe_adj(c, s.nx, s.ny, 0.0, g, Fex_b, l1 == target ? lam_share : 0.0, ic);
-                zexb += ic.zi + ic.ze;
-                qxexb += ic.qxe + Ar[0][0] * ic.qxi + Ar[1][0] * ic.qyi;
-                qyexb += ic.qye + Ar[0][1] * ic.qxi + Ar[1][1] * ic.qyi;
-                zbb += ic.zb;
-                done = true;
             } else {
-                const bool in_wet = wd_in == 1, ex_wet = s.boundary ? in_wet : (wd_ex == 1);
-                if (!(in_wet && ex_wet)) {
-                    // a side on the dry flux (lg = 0)
-                    const RoeIn a = {s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
-                    double F[3];
-                    const double l0 = roe_eval(F, a, s.nx, s.ny, g, g);
-                    double Fg_b[3], F0_b[3];
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) {
-                        Fg_b[f] = (in_wet ? Fin_b[f] : 0.0) + (ex_wet ? Fex_b[f] : 0.0);
-                        F0_b[f] = (in_wet ? 0.0 : Fin_b[f]) + (ex_wet ? 0.0 : Fex_b[f]);
-                    }
-                    RoeIn ib = {0, 0, 0, 0, 0, 0, 0};
-                    roe_adj(a, s.nx, s.ny, g, g, Fg_b, l0 == target ? lam_share : 0.0, ib);
-                    roe_adj(a, s.nx, s.ny, 0.0, g, F0_b, 0.0, ib);
-                    zinb += ib.zi; zexb += ib.ze; qxinb += ib.qxi; qxexb += ib.qxe;
-                    qyinb += ib.qyi; qyexb += ib.qye; zbb += ib.zb;
-                    done = true;
-                }
+                zexb += ib.zi + ib.ze;
+                qxexb += ib.qxe + Ar[0][0] * ib.qxi + Ar[1][0] * ib.qyi;
+                qyexb += ib.qye + Ar[0][1] * ib.qxi + Ar[1][1] * ib.qyi;
+                zbb += ib.zb;
             }
         }
-        if (!done) {
-            const RoeIn a = {s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
-            double F[3], Fb[3];
-            const double l0 = roe_eval(F, a, s.nx, s.ny, g, g);
-#pragma unroll
-            for (int f = 0; f < 3; ++f) Fb[f] = Fin_b[f] + Fex_b[f];
-            RoeIn ib = {0, 0, 0, 0, 0, 0, 0};
-            roe_adj(a, s.nx, s.ny, g, g, Fb, l0 == target ? lam_share : 0.0, ib);
-            zinb += ib.zi; zexb += ib.ze; qxinb += ib.qxi; qxexb += ib.qxe;
-            qyinb += ib.qyi; qyexb += ib.qye; zbb += ib.zb;
+        if (mode != 2) {
+            zinb += acc.zi; zexb += acc.ze; qxinb += acc.qxi; qxexb += acc.qxe;
+            qyinb += acc.qyi; qyexb += acc.qye; zbb += acc.zb;
         }
         // boundary data -> slots (compute2DL.c:149-199)
         double exz = 0, exqx = 0, exqy = 0;   // adjoint of the el_ex slot

@@ -58,13 +58,15 @@ __device__ __forceinline__ void raise_error(Ctrl *c, int code, int where, int ph
 // Roe flux, numericalFlux2D.c:6-108.  Returns max |lambda| over the six
 // one-sided eigenvalues (:101-105).
 // ---------------------------------------------------------------------------
+template <class DV>
 __device__ __forceinline__ double roe_flux(double F[3], double zi, double ze, double qxi, double qxe,
                                            double qyi, double qye, double zb, double nx, double ny,
-                                           double lg, double g)
+                                           double lg, double g, DV &D)
 {
     const double Hi = zi + zb, He = ze + zb;
-    const double ui = qxi / Hi, ue = qxe / He;
-    const double vi = qyi / Hi, ve = qye / He;
+    const typename DV::Den DHi = D.den(Hi), DHe = D.den(He);
+    const double ui = D.q(qxi, DHi), ue = D.q(qxe, DHe);
+    const double vi = D.q(qyi, DHi), ve = D.q(qye, DHe);
     const double j0 = zi - ze, j1 = qxi - qxe, j2 = qyi - qye;
     const double f1xi = Hi * ui, f1xe = He * ue;
     const double f1yi = Hi * vi, f1ye = He * ve;
@@ -80,22 +82,24 @@ __device__ __forceinline__ double roe_flux(double F[3], double zi, double ze, do
 
     const double Hh = 0.5 * (Hi + He);
     const double sHi = sqrt(Hi), sHe = sqrt(He);
-    const double uh = (ui * sHi + ue * sHe) / (sHi + sHe);
-    const double vh = (vi * sHi + ve * sHe) / (sHi + sHe);
+    const typename DV::Den DS = D.den(sHi + sHe);
+    const double uh = D.q(ui * sHi + ue * sHe, DS);
+    const double vh = D.q(vi * sHi + ve * sHe, DS);
     const double c = sqrt(g * Hh);
 
     const double R10 = uh - c * nx, R11 = -ny, R12 = uh + c * nx;
     const double R20 = vh - c * ny, R21 = nx, R22 = vh + c * ny;
-    const double den = -2 * c;
-    const double Q00 = (-c - nx * uh - ny * vh) / den;
-    const double Q01 = nx / den;
-    const double Q02 = ny / den;
-    const double Q10 = (-2 * c * ny * uh + 2 * c * nx * vh) / den;
-    const double Q11 = 2 * c * ny / den;
-    const double Q12 = -2 * c * nx / den;
-    const double Q20 = (-c + nx * uh + ny * vh) / den;
-    const double Q21 = -nx / den;
-    const double Q22 = -ny / den;
+    const typename DV::Den den = D.den(-2 * c);
+    const double Q00 = D.q(-c - nx * uh - ny * vh, den);
+    const double Q01 = D.q(nx, den);
+    const double Q02 = D.q(ny, den);
+    const double Q10 = D.q(-2 * c * ny * uh + 2 * c * nx * vh, den);
+    const double Q11 = D.q(2 * c * ny, den);
+    const double Q12 = D.q(-2 * c * nx, den);
+    const double Q20 = D.q(-c + nx * uh + ny * vh, den);
+    // (-nx)/den is -(nx/den): rounding to nearest is symmetric
+    const double Q21 = -Q01;
+    const double Q22 = -Q02;
 
     const double sgi = sqrt(g * Hi), sge = sqrt(g * He);
     const double l1i = ui * nx + vi * ny - sgi;
@@ -141,13 +145,14 @@ __device__ __forceinline__ double roe_flux(double F[3], double zi, double ze, do
 }
 
 // Cramer solve of the RT0 fit (2DInnerProducts.c:134-146)
+template <class DV>
 __device__ __forceinline__ void rt0(double M11, double M12, double M13, double M21, double M22, double M23,
-                                    double M31, double M32, double M33, double den, double R1, double R2,
-                                    double R3, double &a, double &b, double &c)
+                                    double M31, double M32, double M33, const typename DV::Den &den, double R1,
+                                    double R2, double R3, double &a, double &b, double &c, DV &D)
 {
-    a = (R3 * M13 * M22 - R3 * M12 * M23 - R2 * M13 * M32 + R1 * M23 * M32 + R2 * M12 * M33 - R1 * M22 * M33) / den;
-    b = -(R3 * M13 * M21 - R3 * M11 * M23 - R2 * M13 * M31 + R1 * M23 * M31 + R2 * M11 * M33 - R1 * M21 * M33) / den;
-    c = (R3 * M12 * M21 - R3 * M11 * M22 - R2 * M12 * M31 + R1 * M22 * M31 + R2 * M11 * M32 - R1 * M21 * M32) / den;
+    a = D.q(R3 * M13 * M22 - R3 * M12 * M23 - R2 * M13 * M32 + R1 * M23 * M32 + R2 * M12 * M33 - R1 * M22 * M33, den);
+    b = D.q(-(R3 * M13 * M21 - R3 * M11 * M23 - R2 * M13 * M31 + R1 * M23 * M31 + R2 * M11 * M33 - R1 * M21 * M33), den);
+    c = D.q(R3 * M12 * M21 - R3 * M11 * M22 - R2 * M12 * M31 + R1 * M22 * M31 + R2 * M11 * M32 - R1 * M21 * M32, den);
 }
 
 // ---------------------------------------------------------------------------
@@ -155,8 +160,8 @@ __device__ __forceinline__ void rt0(double M11, double M12, double M13, double M
 // the edge loop of compute2DL for element e, with the reference's
 // expressions, handed to put(GeoIdx, value).
 // ---------------------------------------------------------------------------
-template <typename Put>
-__device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, Put &&put)
+template <typename Put, class DV>
+__device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, Put &&put, DV &D)
 {
     const int Pt = m.pitch;
     double4 P[3];
@@ -174,14 +179,16 @@ __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, P
     put(GX_EJ + 2, el2 / 2);
 
     const double sqrt3 = sqrt(3.0);                                          // :51-65
+    const typename DV::Den D3 = D.den_const(sqrt3);
     double xe[3][2], ye[3][2];
-    xe[0][0] = (sqrt3 * x1 - x2 + x3) / sqrt3;  ye[0][0] = (sqrt3 * y1 - y2 + y3) / sqrt3;
-    xe[0][1] = (sqrt3 * x1 + x2 - x3) / sqrt3;  ye[0][1] = (sqrt3 * y1 + y2 - y3) / sqrt3;
-    xe[1][0] = (x1 + sqrt3 * x2 - x3) / sqrt3;  ye[1][0] = (y1 + sqrt3 * y2 - y3) / sqrt3;
-    xe[1][1] = (-x1 + sqrt3 * x2 + x3) / sqrt3; ye[1][1] = (-y1 + sqrt3 * y2 + y3) / sqrt3;
-    xe[2][0] = (-x1 + x2 + sqrt3 * x3) / sqrt3; ye[2][0] = (-y1 + y2 + sqrt3 * y3) / sqrt3;
-    xe[2][1] = (x1 - x2 + sqrt3 * x3) / sqrt3;  ye[2][1] = (y1 - y2 + sqrt3 * y3) / sqrt3;
+    xe[0][0] = D.q(sqrt3 * x1 - x2 + x3, D3);  ye[0][0] = D.q(sqrt3 * y1 - y2 + y3, D3);
+    xe[0][1] = D.q(sqrt3 * x1 + x2 - x3, D3);  ye[0][1] = D.q(sqrt3 * y1 + y2 - y3, D3);
+    xe[1][0] = D.q(x1 + sqrt3 * x2 - x3, D3);  ye[1][0] = D.q(y1 + sqrt3 * y2 - y3, D3);
+    xe[1][1] = D.q(-x1 + sqrt3 * x2 + x3, D3); ye[1][1] = D.q(-y1 + sqrt3 * y2 + y3, D3);
+    xe[2][0] = D.q(-x1 + x2 + sqrt3 * x3, D3); ye[2][0] = D.q(-y1 + y2 + sqrt3 * y3, D3);
+    xe[2][1] = D.q(x1 - x2 + sqrt3 * x3, D3);  ye[2][1] = D.q(y1 - y2 + sqrt3 * y3, D3);
     const double dn0 = x1 * y2 - x1 * y3 + x2 * y3 - x2 * y1 + x3 * y1 - x3 * y2;   // :73
+    const typename DV::Den Ddn = D.den(dn0);
 #pragma unroll
     for (int ed = 0; ed < 3; ++ed)
 #pragma unroll
@@ -189,15 +196,16 @@ __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, P
             const double xq = xe[ed][n], yq = ye[ed][n];
             put(GX_XE + 2 * ed + n, xq);
             put(GX_YE + 2 * ed + n, yq);
-            put(GX_PS + 0 + 2 * ed + n, -(yq * x2 - yq * x3 - xq * y2 + x3 * y2 + xq * y3 - x2 * y3) / dn0);   // :74-87
-            put(GX_PS + 6 + 2 * ed + n, -(-x1 * yq + x3 * yq - x3 * y1 + xq * y1 - xq * y3 + x1 * y3) / dn0);
-            put(GX_PS + 12 + 2 * ed + n, -(yq * x1 - yq * x2 - xq * y1 + x2 * y1 + xq * y2 - x1 * y2) / dn0);
+            put(GX_PS + 0 + 2 * ed + n, D.q(-(yq * x2 - yq * x3 - xq * y2 + x3 * y2 + xq * y3 - x2 * y3), Ddn));   // :74-87
+            put(GX_PS + 6 + 2 * ed + n, D.q(-(-x1 * yq + x3 * yq - x3 * y1 + xq * y1 - xq * y3 + x1 * y3), Ddn));
+            put(GX_PS + 12 + 2 * ed + n, D.q(-(yq * x1 - yq * x2 - xq * y1 + x2 * y1 + xq * y2 - x1 * y2), Ddn));
         }
     const double jac = 4 * (x2 - x3) * (y2 - y1) - 4 * (x2 - x1) * (y2 - y3);  // :110
     const int sj = isign(jac);
-    const double nx1 = sj * 2 * (y2 - y3) / el0, ny1 = sj * 2 * (x3 - x2) / el0;   // :111-116
-    const double nx2 = sj * 2 * (y3 - y1) / el1, ny2 = sj * 2 * (x1 - x3) / el1;
-    const double nx3 = sj * 2 * (y1 - y2) / el2, ny3 = sj * 2 * (x2 - x1) / el2;
+    const typename DV::Den De0 = D.den(el0), De1 = D.den(el1), De2 = D.den(el2);   // :111-116
+    const double nx1 = D.q(sj * 2 * (y2 - y3), De0), ny1 = D.q(sj * 2 * (x3 - x2), De0);
+    const double nx2 = D.q(sj * 2 * (y3 - y1), De1), ny2 = D.q(sj * 2 * (x1 - x3), De1);
+    const double nx3 = D.q(sj * 2 * (y1 - y2), De2), ny3 = D.q(sj * 2 * (x2 - x1), De2);
     const double M11 = nx1, M12 = x1 * nx1 + y1 * ny1, M13 = ny1;            // :126-128
     const double M21 = nx2, M22 = x2 * nx2 + y2 * ny2, M23 = ny2;
     const double M31 = nx3, M32 = x3 * nx3 + y3 * ny3, M33 = ny3;
@@ -209,9 +217,10 @@ __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, P
     const double weight = 1.0 / 6;                                            // :247
     const double mz[3] = {0.5 * (P[0].z + P[1].z), 0.5 * (P[1].z + P[2].z), 0.5 * (P[2].z + P[0].z)};
     const double mn[3] = {0.5 * (P[0].w + P[1].w), 0.5 * (P[1].w + P[2].w), 0.5 * (P[2].w + P[0].w)};
-    const double gP[3][2] = {{4 / jac * (y2 - y3), 4 / jac * (x3 - x2)},     // :303-308
-                             {4 / jac * (y3 - y1), 4 / jac * (x1 - x3)},
-                             {4 / jac * (y1 - y2), 4 / jac * (x2 - x1)}};
+    const double fj = D.q(4.0, D.den(jac));                                  // 4 / jac
+    const double gP[3][2] = {{fj * (y2 - y3), fj * (x3 - x2)},               // :303-308
+                             {fj * (y3 - y1), fj * (x1 - x3)},
+                             {fj * (y1 - y2), fj * (x2 - x1)}};
     const double sx = x1 + x2 + x3, sy = y1 + y2 + y3;
     const double aj = fabs(jac);
 #pragma unroll
@@ -229,7 +238,7 @@ __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, P
     put(GX_BGX, g * aj * gz0 * weight);                                       // :535
     put(GX_BGY, g * aj * gz1 * weight);
     const double jv = P[0].x * (P[1].y - P[2].y) + P[1].x * (P[2].y - P[0].y) + P[2].x * (P[0].y - P[1].y);
-    put(GX_MI, 6 / fabs(jv));                                                 // compute2DL.c:380-382
+    put(GX_MI, D.q(6.0, D.den(fabs(jv))));                                    // compute2DL.c:380-382
 
     // edge loop geometry in the canonical orientation (compute2DL.c:98-144)
 #pragma unroll
@@ -241,14 +250,22 @@ __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, P
         double x0, y0, xx1, yy1;
         if (inf & EI_DIR_SAME) { x0 = A.x; y0 = A.y; xx1 = B.x; yy1 = B.y; }
         else { x0 = B.x; y0 = B.y; xx1 = A.x; yy1 = A.y; }
-        const double nx = sg * (yy1 - y0) / len;
-        const double ny = sg * (x0 - xx1) / len;
+        const typename DV::Den Dl = D.den(len);
+        const double nx = D.q(sg * (yy1 - y0), Dl);
+        const double ny = D.q(sg * (x0 - xx1), Dl);
         const double tx = -ny, ty = nx;
         put(GX_ENX + k, nx);
         put(GX_ENY + k, ny);
         put(GX_EZB + k, (A.z + B.z) / 2);
-        put(GX_EDN + k, 1. / (nx * ty - ny * tx));
+        put(GX_EDN + k, D.q(1.0, D.den(nx * ty - ny * tx)));
     }
+}
+
+template <typename Put>
+__device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, Put &&put)
+{
+    DivExact D;
+    geometry_of(m, e, g, put, D);
 }
 
 // geometry computed into registers; dead-code elimination keeps only what
@@ -263,16 +280,16 @@ struct GeoRegs {
 // of one element from its state W[f*3+k], its fluxes Fh[f*3+k] and its
 // geometry planes.  2DInnerProducts.c:14-566, compute2DL.c:344-409.
 // ---------------------------------------------------------------------------
-template <typename Geo>
+template <typename Geo, class DV>
 __device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], const double Fh[9], int wd_self,
-                                            double g, double rain, double R[9])
+                                            double g, double rain, double R[9], DV &D)
 {
     const double nx1 = G(GX_NX), nx2 = G(GX_NX + 1), nx3 = G(GX_NX + 2);
     const double ny1 = G(GX_NY), ny2 = G(GX_NY + 1), ny3 = G(GX_NY + 2);
     const double M11 = nx1, M12 = G(GX_M2), M13 = ny1;
     const double M21 = nx2, M22 = G(GX_M2 + 1), M23 = ny2;
     const double M31 = nx3, M32 = G(GX_M2 + 2), M33 = ny3;
-    const double den = G(GX_DEN);
+    const typename DV::Den den = D.den(G(GX_DEN));
     const double nxs[3] = {nx1, nx2, nx3}, nys[3] = {ny1, ny2, ny3};
 
     // surface integrals SI[f*3+b] (:134-239)
@@ -280,7 +297,7 @@ __device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], con
 #pragma unroll
     for (int f = 0; f < 3; ++f)
         rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, Fh[f * 3 + 0], Fh[f * 3 + 1], Fh[f * 3 + 2],
-            a[f], b[f], c[f]);
+            a[f], b[f], c[f], D);
     double SI[9];
 #pragma unroll
     for (int f = 0; f < 9; ++f) SI[f] = 0.0;
@@ -315,15 +332,17 @@ __device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], con
         const double mz = G(GX_MZ + k);
         const double zr = W[k], qx = W[3 + k], qy = W[6 + k];
         H[k] = zr + mz;
-        const double u = qx / H[k];
-        const double v = qy / H[k];
-        tau[k] = G(GX_GNN + k) * sqrt(u * u + v * v) / (dgr_pow43(H[k]));
+        const typename DV::Den DH = D.den(H[k]);
+        const double u = D.q(qx, DH);
+        const double v = D.q(qy, DH);
+        tau[k] = D.q(G(GX_GNN + k) * sqrt(u * u + v * v), D.den(dgr_pow43(H[k])));
+        const double uv = D.q(qx * qy, DH);
         Fx[0][k] = qx;
-        Fx[1][k] = (qx * qx) / H[k] + 0.5 * lg * ((H[k] * H[k]) - (mz * mz));
-        Fx[2][k] = qx * qy / H[k];
+        Fx[1][k] = D.q(qx * qx, DH) + 0.5 * lg * ((H[k] * H[k]) - (mz * mz));
+        Fx[2][k] = uv;
         Fy[0][k] = qy;
-        Fy[1][k] = qx * qy / H[k];
-        Fy[2][k] = (qy * qy) / H[k] + 0.5 * lg * ((H[k] * H[k]) - (mz * mz));
+        Fy[1][k] = uv;
+        Fy[2][k] = D.q(qy * qy, DH) + 0.5 * lg * ((H[k] * H[k]) - (mz * mz));
     }
     const double aj = G(GX_AJ);
     double VF[3][3];   // [field][basis]
@@ -332,7 +351,7 @@ __device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], con
         double va, vb, vc;
         rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den,
             Fx[f][0] * nx1 + Fy[f][0] * ny1, Fx[f][1] * nx2 + Fy[f][1] * ny2, Fx[f][2] * nx3 + Fy[f][2] * ny3,
-            va, vb, vc);
+            va, vb, vc, D);
 #pragma unroll
         for (int bb = 0; bb < 3; ++bb)
             VF[f][bb] = aj * (0.5 * va * G(GX_GP + 2 * bb) + 0.5 * vc * G(GX_GP + 2 * bb + 1) + vb * weight * G(GX_GS + bb));
@@ -413,6 +432,7 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
 {
     const int P = m.pitch;
     const double g = ph.g;
+    DivExact D;
 #pragma unroll
     for (int i = 0; i < 6; ++i) { ops.v[i] = 0.0; ops.pos[i] = -1; }
     err_code = 0;
@@ -463,24 +483,24 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
                 if (is_in) {
                     double rx, ry;
                     reflect(qxin, qyin, nx, ny, dn, rx, ry);
-                    ops.v[2 * k] = roe_flux(F, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
+                    ops.v[2 * k] = roe_flux(F, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g, D);
                     ops.pos[2 * k] = p0;
                     if (boundary) {
                         double Fd[3], sx_, sy_;
                         reflect(qxex, qyex, nx, ny, dn, sx_, sy_);
-                        ops.v[2 * k + 1] = roe_flux(Fd, zex, zex, sx_, qxex, sy_, qyex, zb, nx, ny, 0.0, g);
+                        ops.v[2 * k + 1] = roe_flux(Fd, zex, zex, sx_, qxex, sy_, qyex, zb, nx, ny, 0.0, g, D);
                         ops.pos[2 * k + 1] = p0 + 1;
                     }
                 } else {
                     double rx, ry;
                     reflect(qxex, qyex, nx, ny, dn, rx, ry);
-                    ops.v[2 * k] = roe_flux(F, zex, zex, rx, qxex, ry, qyex, zb, nx, ny, 0.0, g);
+                    ops.v[2 * k] = roe_flux(F, zex, zex, rx, qxex, ry, qyex, zb, nx, ny, 0.0, g, D);
                     ops.pos[2 * k] = p0 + 1;
                     F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2];
                 }
             } else {                                                 // :241-293
                 const int my = is_in ? wd_in : wd_ex;
-                ops.v[2 * k] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, (my == 1) ? g : 0.0, g);
+                ops.v[2 * k] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, (my == 1) ? g : 0.0, g, D);
                 ops.pos[2 * k] = p0;
                 // a NaN here comes from the state, so the reference's first check
                 // (the g-flux, compute2DL.c:245) would already have stopped
@@ -491,7 +511,7 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
                 if (!is_in) { F[0] = -F[0]; F[1] = -F[1]; F[2] = -F[2]; }
             }
         } else {                                                     // :295-315
-            ops.v[2 * k] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
+            ops.v[2 * k] = roe_flux(F, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g, D);
             ops.pos[2 * k] = p0;
             if ((isnan(F[0]) || isnan(F[1]) || isnan(F[2])) && ed - M.ed_off < err_edge) {
                 err_code = 3;
@@ -678,7 +698,8 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
             lam = lam_fold(ops, ctrl, slot);
             if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
             double R[9];
-            element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R);
+            DivExact D;
+            element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R, D);
             if (MODE == 0) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f) { dbg_rhs[f * P + e] = R[f]; dbg_fhat[f * P + e] = Fh[f]; }
@@ -761,68 +782,81 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
         const int kin = inf & ED_KIN_MASK, kex = (inf >> ED_KEX_SHIFT) & 3;
         const bool boundary = (inf & EI_BOUNDARY) != 0;
         const double4 A = m.nodes[E.z], B = m.nodes[E.w];
-        // edge geometry from el_in (compute2DL.c:98-133); the squares and the
-        // vertex sum do not depend on the vertex order
-        const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
-        const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
-        const double nx = sg * (B.y - A.y) / len;
-        const double ny = sg * (A.x - B.x) / len;
-        const double tx = -ny, ty = nx;
-        const double zb = (A.z + B.z) / 2;
-        const double dn = 1. / (nx * ty - ny * tx);
         const double zin = win[kin * P + el_in], qxin = win[(3 + kin) * P + el_in], qyin = win[(6 + kin) * P + el_in];
-        double zex, qxex, qyex;
-        if (boundary) { zex = zin; qxex = qxin; qyex = qyin; }
-        else { zex = win[kex * P + el_ex]; qxex = win[(3 + kex) * P + el_ex]; qyex = win[(6 + kex) * P + el_ex]; }
-        if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
-            const int code = (inf >> EI_CODE_SHIFT) & 0xff;
-            const bool tide = M.tide_code != 0 && code == M.tide_code;
-            zex = tide ? ctrl->tide : bzeta[ed];
-            double qnex = tide ? 1000000.0 : bqn[ed];
-            if (zex < 0) zex = zin;
-            if (qnex > 1000) qnex = qxin * nx + qyin * ny;
-            const double qtex = qxin * tx + qyin * ty;
-            qxex = (ty * qnex - ny * qtex) * dn;
-            qyex = (-tx * qnex + nx * qtex) * dn;
-        } else if (boundary) {                                       // :175-191
-            reflect(qxin, qyin, nx, ny, dn, qxex, qyex);
-        }
-        LamOps ops;
-#pragma unroll
-        for (int i = 0; i < 6; ++i) { ops.v[i] = 0.0; ops.pos[i] = -1; }
+        double zex0 = zin, qxex0 = qxin, qyex0 = qyin;
+        if (!boundary) { zex0 = win[kex * P + el_ex]; qxex0 = win[(3 + kex) * P + el_ex]; qyex0 = win[(6 + kex) * P + el_ex]; }
+        const int code = (inf >> EI_CODE_SHIFT) & 0xff;
+        const bool tide = (inf & EI_PRESCRIBED) && M.tide_code != 0 && code == M.tide_code;
+        double bz = 0.0, bq = 0.0;
+        if (inf & EI_PRESCRIBED) { bz = tide ? ctrl->tide : bzeta[ed]; bq = tide ? 1000000.0 : bqn[ed]; }
+        const int wd_in = WETDRY ? wd[el_in] : 1;
+        const int wd_ex = WETDRY ? (boundary ? wd_in : wd[el_ex]) : 1;
         const int led = ed - M.ed_off;                 // member-local edge id
         const long long p0 = 2LL * led;
+        LamOps ops;
         double Fi[3], Fx[3];
         bool nan_flux = false;
-        if (WETDRY) {
-            const int wd_in = wd[el_in];
-            const int wd_ex = boundary ? wd_in : wd[el_ex];
-            if (wd_in == 0 && wd_ex == 0) {                          // :203-240
+        // the edge's geometry, boundary data and Roe flux(es) with division
+        // policy D.  Geometry from el_in (compute2DL.c:98-133); the squares and
+        // the vertex sum do not depend on the vertex order.
+        auto eval = [&](auto &D) {
+            const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+            const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
+            const auto Dl = D.den(len);
+            const double nx = D.q(sg * (B.y - A.y), Dl);
+            const double ny = D.q(sg * (A.x - B.x), Dl);
+            const double tx = -ny, ty = nx;
+            const double zb = (A.z + B.z) / 2;
+            const double dn = D.q(1.0, D.den(nx * ty - ny * tx));
+            double zex = zex0, qxex = qxex0, qyex = qyex0;
+            if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
+                zex = bz;
+                double qnex = bq;
+                if (zex < 0) zex = zin;
+                if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+                const double qtex = qxin * tx + qyin * ty;
+                qxex = (ty * qnex - ny * qtex) * dn;
+                qyex = (-tx * qnex + nx * qtex) * dn;
+            } else if (boundary) {                                       // :175-191
+                reflect(qxin, qyin, nx, ny, dn, qxex, qyex);
+            }
+#pragma unroll
+            for (int i = 0; i < 6; ++i) { ops.v[i] = 0.0; ops.pos[i] = -1; }
+            nan_flux = false;
+            if (WETDRY && wd_in == 0 && wd_ex == 0) {                    // :203-240
                 double rx, ry, sx, sy;
                 reflect(qxin, qyin, nx, ny, dn, rx, ry);
-                ops.v[0] = roe_flux(Fi, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g);
+                ops.v[0] = roe_flux(Fi, zin, zin, qxin, rx, qyin, ry, zb, nx, ny, 0.0, g, D);
                 ops.pos[0] = p0;
                 reflect(qxex, qyex, nx, ny, dn, sx, sy);
-                ops.v[1] = roe_flux(Fx, zex, zex, sx, qxex, sy, qyex, zb, nx, ny, 0.0, g);
+                ops.v[1] = roe_flux(Fx, zex, zex, sx, qxex, sy, qyex, zb, nx, ny, 0.0, g, D);
                 ops.pos[1] = p0 + 1;
-            } else {                                                 // :241-293
+            } else if (WETDRY) {                                         // :241-293
                 // the flux of a wet side uses g, of a dry side 0; both give the
                 // same |lambda| (the eigenvalues use g, not the flux's gravity)
                 const bool in_wet = wd_in == 1, ex_wet = boundary ? in_wet : (wd_ex == 1);
                 double Fg[3], F0[3];
-                if (in_wet || ex_wet) ops.v[0] = roe_flux(Fg, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
-                if (!in_wet || !ex_wet) ops.v[0] = roe_flux(F0, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, 0.0, g);
+                if (in_wet || ex_wet) ops.v[0] = roe_flux(Fg, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g, D);
+                if (!in_wet || !ex_wet) ops.v[0] = roe_flux(F0, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, 0.0, g, D);
                 ops.pos[0] = p0;
 #pragma unroll
                 for (int j = 0; j < 3; ++j) { Fi[j] = in_wet ? Fg[j] : F0[j]; Fx[j] = ex_wet ? Fg[j] : F0[j]; }
                 nan_flux = isnan(Fi[0]) || isnan(Fi[1]) || isnan(Fi[2]) ||
                            (!boundary && (isnan(Fx[0]) || isnan(Fx[1]) || isnan(Fx[2])));
+            } else {                                                     // :295-315
+                ops.v[0] = roe_flux(Fi, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g, D);
+                ops.pos[0] = p0;
+                Fx[0] = Fi[0]; Fx[1] = Fi[1]; Fx[2] = Fi[2];
+                nan_flux = isnan(Fi[0]) || isnan(Fi[1]) || isnan(Fi[2]);
             }
-        } else {                                                     // :295-315
-            ops.v[0] = roe_flux(Fi, zin, zex, qxin, qxex, qyin, qyex, zb, nx, ny, g, g);
-            ops.pos[0] = p0;
-            Fx[0] = Fi[0]; Fx[1] = Fi[1]; Fx[2] = Fi[2];
-            nan_flux = isnan(Fi[0]) || isnan(Fi[1]) || isnan(Fi[2]);
+        };
+        {
+            DivFast D;
+            eval(D);
+            if (!D.ok()) {   // an operand outside the fast division's range: plain divisions
+                DivExact X;
+                eval(X);
+            }
         }
         if (nan_flux) raise_error(ctrl, 3, led, 2 * slot);
         // in-order fold of this edge's operands (positions p0, p0 + 1); a NaN
@@ -865,10 +899,29 @@ __global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const dou
 #pragma unroll
     for (int f = 0; f < 9; ++f) Fh[f] = fhat[f * P + e];
     const int wd_self = wd[e];
-    GeoRegs G;
-    geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
     double R[9];
-    element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R);
+    {
+        DivFast D;
+        GeoRegs G;
+        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
+        element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R, D);
+        if (!D.ok()) {
+            // an operand outside the fast division's range: plain divisions.
+            // The inputs are loaded again through pointers the compiler
+            // cannot match with the first loads, so nothing of the fast
+            // evaluation has to stay live for this path.
+            MeshDev m2 = m;
+            const double *fh2 = fhat;
+            asm volatile("" : "+s"(m2.nodes), "+s"(m2.ev), "+s"(fh2));
+            double Fh2[9];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) Fh2[f] = fh2[f * P + e];
+            DivExact X;
+            GeoRegs G2;
+            geometry_of(m2, e, ph.g, [&](int i, double v) { G2.v[i] = v; }, X);
+            element_rhs(G2, W, Fh2, wd_self, ph.g, M.rain, R, X);
+        }
+    }
     if (MODE == 0) {
 #pragma unroll
         for (int f = 0; f < 9; ++f) dbg_rhs[f * P + e] = R[f];

@@ -136,6 +136,66 @@ __device__ __forceinline__ double quot(double x, const Divisor &D)
     return x / y;
 }
 
+// ---------------------------------------------------------------------------
+// Division policies for code that evaluates a whole element (or edge) with
+// the 3-instruction division above and evaluates it again with plain
+// division when any operand left the range where the two agree.  DivFast
+// checks without branching: it folds the operands into three running bounds
+// and ok() tests them once at the end.
+//   divisors   |y| in [2^-320, 2^320]
+//   numerators x = 0, or |x| in [2^-700, 2^320] (frexp exponent >= -699)
+// so |x/y| < 2^640 and no operand needs div_scale's exponent scaling.  A NaN
+// operand gives NaN either way.  A zero quotient takes the sign of x*r, the
+// sign div_fixup gives it (the final fma rounds -0 + +0 to +0).
+// ---------------------------------------------------------------------------
+struct DivExact {
+    struct Den {
+        double y;
+    };
+    __device__ __forceinline__ Den den(double y) { return Den{y}; }
+    __device__ __forceinline__ Den den_const(double y) { return Den{y}; }
+    __device__ __forceinline__ double q(double x, const Den &d) { return x / d.y; }
+};
+
+struct DivFast {
+    struct Den {
+        double y, r;
+    };
+    int emin = 0;
+    double amax = 0.0;
+    double dmin = 0x1p320;
+
+    __device__ __forceinline__ Den make(double y, double d)
+    {
+        double r = __builtin_amdgcn_rcp(d);
+        double e = fma(-d, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-d, r, 1.0);
+        r = fma(r, e, r);
+        amax = fmax(amax, fabs(y));
+        dmin = fmin(dmin, fabs(y));
+        return Den{y, r};
+    }
+    __device__ __forceinline__ Den den(double y) { return make(y, y); }
+    // a compile-time constant divisor must still go through v_rcp_f64
+    __device__ __forceinline__ Den den_const(double y)
+    {
+        double d = y;
+        asm volatile("" : "+v"(d));
+        return make(y, d);
+    }
+    __device__ __forceinline__ double q(double x, const Den &d)
+    {
+        const int ex = __builtin_amdgcn_frexp_exp(x);
+        emin = (ex < emin) ? ex : emin;
+        amax = fmax(amax, fabs(x));
+        const double qq = x * d.r;
+        const double e = fma(-d.y, qq, x);
+        return copysign(fma(e, d.r, qq), qq);
+    }
+    __device__ __forceinline__ bool ok() const { return emin >= -699 && amax <= 0x1p320 && dmin >= 0x1p-320; }
+};
+
 // Cody-Waite reduction by pi/2 (33+33+53-bit split of pi/2) and Taylor
 // polynomials through x^15 (sin) / x^16 (cos) on [-pi/4, pi/4].
 __device__ inline double dgr_sin(double x)
