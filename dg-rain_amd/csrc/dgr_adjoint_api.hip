@@ -134,11 +134,16 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&tu2, W9 * K));
     DGR_HIP(B.alloc(&twa, P * K));
     DGR_HIP(B.alloc(&twb, P * K));
+    // node bounds of both stages of every step and the elements attaining
+    // them, as the forward limiter saw them (the reverse routes the bound
+    // adjoints to those elements)
+    double *tnm;
+    int *tna;
+    DGR_HIP(B.alloc(&tnm, 12 * nn * K));
+    DGR_HIP(B.alloc(&tna, 12 * nn * K));
     // reverse buffers: slot/zbe pairs per stage (the edge part of a stage's
     // input adjoint is consumed by the limiter reverse that comes next)
-    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *nodeb, *nodemm, *zvb, *nvb, *dtp, *tdp,
-        *gz, *gn;
-    int *nodearg;
+    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *nodeb, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
     DGR_HIP(B.alloc(&wb, W9));
     DGR_HIP(B.alloc(&wb0, W9));
     DGR_HIP(B.alloc(&pb, W9));
@@ -150,8 +155,6 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&zbe2, 3 * P));
     DGR_HIP(B.alloc(&vmb, 18 * P));
     DGR_HIP(B.alloc(&nodeb, 6 * nn));
-    DGR_HIP(B.alloc(&nodemm, 6 * nn));
-    DGR_HIP(B.alloc(&nodearg, 6 * nn));
     DGR_HIP(B.alloc(&zvb, 3 * P));
     DGR_HIP(B.alloc(&nvb, 3 * P));
     DGR_HIP(B.alloc(&dtp, 2 * (size_t)maxparts));
@@ -233,6 +236,10 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             } else {
                 bf.wdprev = bf.wda = bf.wdb = h->d_wd;   // never written without wet/dry
             }
+            bf.nodemm1 = tnm + 6 * nn * (2 * k);
+            bf.nodemm2 = tnm + 6 * nn * (2 * k + 1);
+            bf.nodearg1 = tna + 6 * nn * (2 * k);
+            bf.nodearg2 = tna + 6 * nn * (2 * k + 1);
             enqueue_step(h, &bf);
         }
         for (int n = n1 - 1; n >= n0; --n) {
@@ -244,20 +251,20 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
                 const int j = (n + 1) / sp->sample_every - 1;
                 launch_adj_gauges(h->m, ng, d_gel, d_hv + (size_t)j * per, d_obs + (size_t)j * per, wb, zvb, s);
             }
+            const double *nm1 = tnm + 6 * nn * (2 * k), *nm2 = tnm + 6 * nn * (2 * k + 1);
+            const int *na1 = tna + 6 * nn * (2 * k), *na2 = tna + 6 * nn * (2 * k + 1);
             // stage 2: input p1, base w0, output u2 -> limited -> w^{n+1} (adjoint wb + slot1)
-            launch_node_arg(h->m, u2, nodemm, nodearg, s);
-            launch_adj_limit(h->m, h->ph, u2, nodemm, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
+            launch_adj_limit(h->m, h->ph, u2, nm2, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
                              zvb, ub, vmb, actl, s);
             launch_adj_node(h->m, vmb, nodeb, s);
-            launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, nodearg, wdb, rn, pb, wb0, fhb, zvb, nvb,
+            launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
                            dtp + maxparts, s);
             launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slot2, zbe2, tdp + maxparts,
                             n + 1 < N, s);
             // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)
-            launch_node_arg(h->m, u1, nodemm, nodearg, s);
-            launch_adj_limit(h->m, h->ph, u1, nodemm, pb, slot2, zbe2, zvb, ub, vmb, actl, s);
+            launch_adj_limit(h->m, h->ph, u1, nm1, pb, slot2, zbe2, zvb, ub, vmb, actl, s);
             launch_adj_node(h->m, vmb, nodeb, s);
-            launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nodeb, nodearg, wda, rn, nullptr, wb0, fhb, zvb, nvb,
+            launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nodeb, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);
             launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, false, s);
             have_slot1 = true;

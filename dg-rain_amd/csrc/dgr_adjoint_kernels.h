@@ -36,7 +36,7 @@ __global__ __launch_bounds__(256) void k_node_arg(MeshDev m, const double *__res
     int amn[3] = {-1, -1, -1}, amx[3] = {-1, -1, -1};
     const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
     for (int j = b; j < en; ++j) {
-        const int el = m.n2e[j];
+        const int el = m.n2e[j] & N2E_EL_MASK;
 #pragma unroll
         for (int f = 0; f < 3; ++f) {
             const double a = (U[(3 * f) * P + el] + U[(3 * f + 1) * P + el] + U[(3 * f + 2) * P + el]) / 3;
@@ -132,16 +132,13 @@ __global__ __launch_bounds__(256) void k_adj_node(MeshDev m, const double *__res
     double s[6] = {0, 0, 0, 0, 0, 0};
     const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
     for (int j = b; j < en; ++j) {
-        const int el = m.n2e[j];
+        const int v = m.n2e[j];
+        const int el = v & N2E_EL_MASK, k = (unsigned)v >> N2E_K_SHIFT;
 #pragma unroll
-        for (int k = 0; k < 3; ++k)
-            if (m.ev[k * P + el] == i) {
-#pragma unroll
-                for (int f = 0; f < 3; ++f) {
-                    s[f] += vmb[(f * 3 + k) * P + el];
-                    s[3 + f] += vmb[(9 + f * 3 + k) * P + el];
-                }
-            }
+        for (int f = 0; f < 3; ++f) {
+            s[f] += vmb[(f * 3 + k) * P + el];
+            s[3 + f] += vmb[(9 + f * 3 + k) * P + el];
+        }
     }
 #pragma unroll
     for (int f = 0; f < 6; ++f) nodeb[f * nn + i] = s[f];
@@ -199,21 +196,26 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
                 ub[3 * f + 2] += add;
             }
         }
+        // the RHS the forward used, recovered from the update (it only
+        // weighs the dt adjoint, so reciprocals do)
         double Rb[9], R[9];
+        const double rdt = 1.0 / dt;
         if (MODE == 1) {
 #pragma unroll
-            for (int f = 0; f < 9; ++f) { R[f] = (u[f] - w[f]) / dt; Rb[f] = dt * ub[f]; dtb += R[f] * ub[f]; }
+            for (int f = 0; f < 9; ++f) { R[f] = (u[f] - w[f]) * rdt; Rb[f] = dt * ub[f]; dtb += R[f] * ub[f]; }
         } else {
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
                 const double bf = B[f * P + e];
-                R[f] = (u[f] - (w[f] + bf) * 0.5) * 2.0 / dt;
+                R[f] = (u[f] - (w[f] + bf) * 0.5) * (2.0 * rdt);
                 Rb[f] = 0.5 * dt * ub[f];
                 dtb += 0.5 * R[f] * ub[f];
             }
         }
+        // geometry enters derivatives only: reciprocal products
         GeoRegs G;
-        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; });
+        DivRecip D;
+        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
         double Wb[9], Fb[9], mzb[3] = {0, 0, 0}, gnnb[3] = {0, 0, 0}, bgxb = 0.0, bgyb = 0.0;
 #pragma unroll
         for (int f = 0; f < 9; ++f) { Wb[f] = 0.0; Fb[f] = 0.0; }
@@ -645,10 +647,10 @@ __global__ __launch_bounds__(256) void k_adj_gather(MeshDev m, const double *__r
     double sz = 0.0, sn = 0.0;
     const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
     for (int j = b; j < en; ++j) {
-        const int el = m.n2e[j];
-#pragma unroll
-        for (int k = 0; k < 3; ++k)
-            if (m.ev[k * P + el] == i) { sz += zvb[k * P + el]; sn += nvb[k * P + el]; }
+        const int v = m.n2e[j];
+        const int el = v & N2E_EL_MASK, k = (unsigned)v >> N2E_K_SHIFT;
+        sz += zvb[k * P + el];
+        sn += nvb[k * P + el];
     }
     gz[i] = sz;
     gn[i] = sn;

@@ -135,7 +135,9 @@ static dgr_status build_member(const dgr_mesh_view *mv, const MemberHost &M, int
     }
     for (size_t i = 0; i < seen.size(); ++i)
         if (seen[i] != 1) return DGR_ERR_MESH;   // every Fhat slot written exactly once
-    // node -> element CSR, ascending element order (SlopeLimiter2D.c:73-89)
+    // node -> element CSR, ascending element order (SlopeLimiter2D.c:73-89);
+    // each entry also carries the node's local vertex index in the element
+    // (bits 30-31, see N2E_EL_MASK)
     {
         std::vector<int> cnt(nn + 1, 0);
         for (int i = 0; i < 3 * ne; ++i) cnt[mv->elto_vert[i] + 1]++;
@@ -146,7 +148,7 @@ static dgr_status build_member(const dgr_mesh_view *mv, const MemberHost &M, int
         for (int el = 0; el < ne; ++el)
             for (int k = 0; k < 3; ++k) {
                 const int v = mv->elto_vert[3 * el + k];
-                T.n2e[base + cnt[v] + fill[v]++] = eo + el;
+                T.n2e[base + cnt[v] + fill[v]++] = (eo + el) | (k << N2E_K_SHIFT);
             }
     }
     for (int i = 0; i < nn; ++i) T.nodes[no + i] = make_double4(X[i], Y[i], mv->z[i], mv->n[i]);
@@ -559,7 +561,7 @@ extern "C" dgr_status dgr_limit(dgr_handle *h)
 {
     if (!h) return DGR_ERR_ARG;
     launch_means(h->m, h->d_w, h->d_means, h->stream);
-    launch_node_minmax(h->m, h->d_means, h->d_nodemm, h->stream);
+    launch_node_minmax(h->m, h->d_means, h->d_nodemm, nullptr, h->stream);
     launch_limit(false, h->m, h->ph, h->d_w, h->d_w, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, nullptr, nullptr,
                  h->stream);
     DGR_HIP(hipGetLastError());
@@ -604,15 +606,17 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b)
 {
     const bool wdon = h->ph.wetdry != 0;
     hipStream_t s = h->stream;
-    const StepBufs plain = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wd, h->d_wd, nullptr, false};
+    const StepBufs plain = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wd, h->d_wd, nullptr, false,
+                            nullptr, nullptr, nullptr, nullptr};
     if (!b) b = &plain;
     if (wdon && !b->wda_ready) launch_wd_status(h->m, h->ph, b->w0, b->wdprev, b->wda, s);
     mark(h);
     h->kind[0] = launch_stage(h->variant, 1, wdon, h->m, h->ph, b->w0, b->u1, nullptr, h->d_bzeta, h->d_bqn, b->wda,
                               h->d_means, h->d_fhat, h->d_ctrl, 0, nullptr, nullptr, s);
     mark(h);
-    launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
-    launch_limit(wdon, h->m, h->ph, b->u1, b->p1, h->d_nodemm, modify_ptr(h, 0), h->d_ctrl, 0, b->wda,
+    double *nm1 = b->nodemm1 ? b->nodemm1 : h->d_nodemm;
+    launch_node_minmax(h->m, h->d_means, nm1, b->nodearg1, s);
+    launch_limit(wdon, h->m, h->ph, b->u1, b->p1, nm1, modify_ptr(h, 0), h->d_ctrl, 0, b->wda,
                  wdon ? b->wdb : nullptr, s);
     const int *wd2 = wdon ? b->wdb : b->wda;
     mark(h);
@@ -624,9 +628,10 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b)
     if (h->kind[1] == PARTS_ELEM)
         launch_stage(h->variant, 3, wdon, h->m, h->ph, b->p1, nullptr, nullptr, h->d_bzeta, h->d_bqn, wd2, nullptr,
                      h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
-    launch_node_minmax(h->m, h->d_means, h->d_nodemm, s);
+    double *nm2 = b->nodemm2 ? b->nodemm2 : h->d_nodemm;
+    launch_node_minmax(h->m, h->d_means, nm2, b->nodearg2, s);
     const bool wdn = wdon && b->wdnext;
-    launch_limit(wdon, h->m, h->ph, b->u2, b->wout, h->d_nodemm, modify_ptr(h, 1), h->d_ctrl, 1,
+    launch_limit(wdon, h->m, h->ph, b->u2, b->wout, nm2, modify_ptr(h, 1), h->d_ctrl, 1,
                  wdn ? wd2 : nullptr, wdn ? b->wdnext : nullptr, s);
 }
 
@@ -651,7 +656,8 @@ static void enqueue_run_prologue(dgr_handle *h)
 
 static void enqueue_run_step(dgr_handle *h)
 {
-    const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true};
+    const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true,
+                          nullptr, nullptr, nullptr, nullptr};
     enqueue_stages(h, &run);
     launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], true, h->stream);
 }

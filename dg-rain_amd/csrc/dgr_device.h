@@ -19,7 +19,8 @@
 //   eid          int    [3][pitch]   global edge id (boundary data lookup)
 //   means        double [3][pitch]   element averages of the updated state
 //   nodemm       double [6][nn]      node min (zeta,qx,qy) then max
-//   n2e_off/n2e  node -> element CSR in ascending element order
+//   n2e_off/n2e  node -> element CSR in ascending element order; an entry is
+//                el | k << 30, k the node's local vertex index in el
 //   edges        int4   [ned_pad]    {el_in, el_ex, va, vb}: EdgtoEls and el_in's
 //                                     vertices of the edge in el_in's order
 //   einfo        int    [ned_pad]    packed edge facts, see ED_* below
@@ -39,6 +40,12 @@ enum : int {
     EI_PRESCRIBED = 1 << 6, // BdryPrescribed != 0
     EI_BOUNDARY = 1 << 7,   // el_in == el_ex
     EI_CODE_SHIFT = 8,      // BdryPrescribed code (clamped to 0..255) in bits 8..15
+};
+
+// node -> element CSR entry: element id and the node's local vertex index
+enum : int {
+    N2E_EL_MASK = 0x3fffffff,
+    N2E_K_SHIFT = 30,
 };
 
 // edge info word: bits 4..15 as in the element info word, plus

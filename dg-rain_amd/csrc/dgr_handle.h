@@ -15,7 +15,7 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
                  hipStream_t s);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
-void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s);
+void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
                   const double *nodemm, unsigned char *modify, Ctrl *ctrl, int stage, const int *wd_in, int *wd_out,
                   hipStream_t s);
@@ -135,6 +135,10 @@ struct StepBufs {
                     // start flags (wetDryStatus2D of wout from wdb) here
     bool wda_ready; // wda already holds this step's start flags (a previous
                     // step wrote them through wdnext): no status launch
+    // if set: each stage's node bounds [6][nn] (and the elements attaining
+    // them) go here instead of the handle's scratch buffer
+    double *nodemm1, *nodemm2;
+    int *nodearg1, *nodearg2;
 };
 
 void enqueue_stages(dgr_handle *h, const StepBufs *b);

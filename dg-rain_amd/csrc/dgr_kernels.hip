@@ -940,26 +940,37 @@ __global__ __launch_bounds__(256) void k_means(MeshDev m, const double *__restri
     for (int f = 0; f < 3; ++f) means[f * P + e] = (w[(3 * f) * P + e] + w[(3 * f + 1) * P + e] + w[(3 * f + 2) * P + e]) / 3;
 }
 
-// node min/max of element means, ascending element order (SlopeLimiter2D.c:106-153)
-__global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means, double *__restrict__ nodemm)
+// node min/max of element means, ascending element order (SlopeLimiter2D.c:106-153).
+// ARG: also the first element that attains each bound (-1: the sentinel
+// stood), which the limiter's reverse routes the bound adjoints to.
+template <bool ARG>
+__global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means,
+                                                     double *__restrict__ nodemm, int *__restrict__ nodearg)
 {
     const int i = xcd_block() * 256 + (int)threadIdx.x;
     if (i >= m.nn) return;
-    double mx0 = -99999, mx1 = -99999, mx2 = -99999, mn0 = 99999, mn1 = 99999, mn2 = 99999;
+    double mn[3] = {99999, 99999, 99999}, mx[3] = {-99999, -99999, -99999};
+    int amn[3] = {-1, -1, -1}, amx[3] = {-1, -1, -1};
     const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
     for (int j = b; j < en; ++j) {
-        const int el = m.n2e[j];
-        const double a0 = means[el], a1 = means[m.pitch + el], a2 = means[2 * m.pitch + el];
-        if (a0 < mn0) mn0 = a0;
-        if (a0 > mx0) mx0 = a0;
-        if (a1 < mn1) mn1 = a1;
-        if (a1 > mx1) mx1 = a1;
-        if (a2 < mn2) mn2 = a2;
-        if (a2 > mx2) mx2 = a2;
+        const int el = m.n2e[j] & N2E_EL_MASK;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            const double a = means[f * m.pitch + el];
+            if (a < mn[f]) { mn[f] = a; if (ARG) amn[f] = el; }
+            if (a > mx[f]) { mx[f] = a; if (ARG) amx[f] = el; }
+        }
     }
     const int nn = m.nn;
-    nodemm[i] = mn0; nodemm[nn + i] = mn1; nodemm[2 * nn + i] = mn2;
-    nodemm[3 * nn + i] = mx0; nodemm[4 * nn + i] = mx1; nodemm[5 * nn + i] = mx2;
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        nodemm[f * nn + i] = mn[f];
+        nodemm[(3 + f) * nn + i] = mx[f];
+        if (ARG) {
+            nodearg[f * nn + i] = amn[f];
+            nodearg[(3 + f) * nn + i] = amx[f];
+        }
+    }
 }
 
 // PDop2D on one element (wetDry2D.c:66-256); zeta/qx/qy are the element's
@@ -1405,9 +1416,10 @@ void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t 
     hipLaunchKernelGGL(k_means, grid_for(m.pitch), dim3(256), 0, s, m, w, means);
 }
 
-void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, hipStream_t s)
+void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_node_minmax, grid_for(m.nn), dim3(256), 0, s, m, means, nodemm);
+    if (nodearg) hipLaunchKernelGGL(k_node_minmax<true>, grid_for(m.nn), dim3(256), 0, s, m, means, nodemm, nodearg);
+    else hipLaunchKernelGGL(k_node_minmax<false>, grid_for(m.nn), dim3(256), 0, s, m, means, nodemm, nodearg);
 }
 
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,

@@ -196,6 +196,17 @@ struct DivFast {
     __device__ __forceinline__ bool ok() const { return emin >= -699 && amax <= 0x1p320 && dmin >= 0x1p-320; }
 };
 
+// quotients as products with a rounded reciprocal (within a few ulps of
+// x / y): for values that only enter derivatives, never a forward result
+struct DivRecip {
+    struct Den {
+        double r;
+    };
+    __device__ __forceinline__ Den den(double y) { return Den{1.0 / y}; }
+    __device__ __forceinline__ Den den_const(double y) { return Den{1.0 / y}; }
+    __device__ __forceinline__ double q(double x, const Den &d) { return x * d.r; }
+};
+
 // Cody-Waite reduction by pi/2 (33+33+53-bit split of pi/2) and Taylor
 // polynomials through x^15 (sin) / x^16 (cos) on [-pi/4, pi/4].
 __device__ inline double dgr_sin(double x)
