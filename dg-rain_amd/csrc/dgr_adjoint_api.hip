@@ -54,8 +54,9 @@ extern "C" dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every,
     DGR_HIP(B.alloc(&d_gel, per));
     DGR_HIP(B.alloc(&d_h, per * (nsamp > 0 ? nsamp : 1)));
     DGR_HIP(hipMemcpyAsync(d_gel, gauge_el, per * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (nsteps > 0) enqueue_run_prologue(h);
     for (int n = 0; n < nsteps; ++n) {
-        enqueue_step(h, nullptr);
+        enqueue_run_step(h);
         if ((n + 1) % sample_every == 0) {
             const int k = (n + 1) / sample_every - 1;
             launch_gauge_sample(h->m, ngauges, d_gel, h->d_w, d_h + (size_t)k * per, nullptr, nullptr, h->stream);
@@ -176,7 +177,10 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     const size_t Wb = W9 * sizeof(double), Ib = P * sizeof(int), Cb = (size_t)nm * sizeof(Ctrl);
     const bool wdon = h->ph.wetdry != 0;
 
-    // forward with checkpoints
+    // forward with checkpoints (run steps: each step's end records its max
+    // |lambda| and the next step's t, dt and tide)
+    enqueue_run_prologue(h);
+    launch_step_record(h->m, h->d_ctrl, rec, 0, s);
     for (int n = 0; n < N; ++n) {
         if (n % K == 0) {
             const int c = n / K;
@@ -184,11 +188,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             DGR_HIP(hipMemcpyAsync(ckWD + P * c, h->d_wd, Ib, hipMemcpyDeviceToDevice, s));
             DGR_HIP(hipMemcpyAsync(ckC + (size_t)nm * c, h->d_ctrl, Cb, hipMemcpyDeviceToDevice, s));
         }
-        launch_step_begin(h->m, h->d_ctrl, s);
-        launch_step_record(h->m, h->d_ctrl, rec + (size_t)nm * n, 0, s);
-        enqueue_stages(h, nullptr);
-        launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], false, s);
-        launch_step_record(h->m, h->d_ctrl, rec + (size_t)nm * n, 1, s);
+        enqueue_run_step(h, nullptr, rec + (size_t)nm * n, n + 1 < N ? rec + (size_t)nm * (n + 1) : nullptr);
         if ((n + 1) % sp->sample_every == 0) {
             const int k = (n + 1) / sp->sample_every - 1;
             launch_gauge_sample(h->m, ng, d_gel, h->d_w, d_hv + (size_t)k * per, d_obs + (size_t)k * per, actl, s);
@@ -216,10 +216,13 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     for (int c = nseg - 1; c >= 0; --c) {
         const int n0 = c * K, n1 = (n0 + K < N) ? n0 + K : N;
         // recompute the segment into the tape: the checkpoint goes to slot 0,
-        // every step reads its slot and writes the next one (no copies)
+        // every step reads its slot and writes the next one (no copies); the
+        // stage-2 limiter of each step writes the next step's start flags
         DGR_HIP(hipMemcpyAsync(tw0, ckW + W9 * c, Wb, hipMemcpyDeviceToDevice, s));
         DGR_HIP(hipMemcpyAsync(h->d_wd, ckWD + P * c, Ib, hipMemcpyDeviceToDevice, s));
         DGR_HIP(hipMemcpyAsync(h->d_ctrl, ckC + (size_t)nm * c, Cb, hipMemcpyDeviceToDevice, s));
+        launch_step_begin(h->m, h->d_ctrl, s);
+        if (wdon) launch_wd_status(h->m, h->ph, tw0, h->d_wd, twa, s);
         for (int n = n0; n < n1; ++n) {
             const int k = n - n0;
             const bool last = n + 1 == n1;
@@ -233,6 +236,8 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
                 bf.wdprev = k == 0 ? h->d_wd : twb + P * (k - 1);
                 bf.wda = twa + P * k;
                 bf.wdb = twb + P * k;
+                bf.wdnext = last ? h->d_wdn : twa + P * (k + 1);
+                bf.wda_ready = true;
             } else {
                 bf.wdprev = bf.wda = bf.wdb = h->d_wd;   // never written without wet/dry
             }
@@ -240,7 +245,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             bf.nodemm2 = tnm + 6 * nn * (2 * k + 1);
             bf.nodearg1 = tna + 6 * nn * (2 * k);
             bf.nodearg2 = tna + 6 * nn * (2 * k + 1);
-            enqueue_step(h, &bf);
+            enqueue_run_step(h, &bf);
         }
         for (int n = n1 - 1; n >= n0; --n) {
             const int k = n - n0;

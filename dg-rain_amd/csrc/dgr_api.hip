@@ -648,18 +648,18 @@ void enqueue_step(dgr_handle *h, const StepBufs *b)
 // writes the next step's start flags into d_wdn from its stage-2 limiter and
 // ends with the next step's bookkeeping: no separate status or step-begin
 // launch per step.
-static void enqueue_run_prologue(dgr_handle *h)
+void enqueue_run_prologue(dgr_handle *h)
 {
     launch_step_begin(h->m, h->d_ctrl, h->stream);
     if (h->ph.wetdry) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, h->d_wdn, h->stream);
 }
 
-static void enqueue_run_step(dgr_handle *h)
+void enqueue_run_step(dgr_handle *h, const StepBufs *b, StepRec *rec_end, StepRec *rec_next)
 {
     const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true,
                           nullptr, nullptr, nullptr, nullptr};
-    enqueue_stages(h, &run);
-    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], true, h->stream);
+    enqueue_stages(h, b ? b : &run);
+    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], true, h->stream, rec_end, rec_next);
 }
 
 extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)

@@ -23,7 +23,8 @@ void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hip
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
                       hipStream_t s);
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
-void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s);
+void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
+                     StepRec *rec_end = nullptr, StepRec *rec_next = nullptr);
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
 void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s);
 void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s);
@@ -143,6 +144,13 @@ struct StepBufs {
 
 void enqueue_stages(dgr_handle *h, const StepBufs *b);
 void enqueue_step(dgr_handle *h, const StepBufs *b);
+// the steps of a run: the prologue once (step bookkeeping, start flags into
+// d_wdn), then run steps that chain the start flags through the stage-2
+// limiter and end with the next step's bookkeeping.  b = nullptr: the
+// handle's own buffers (wda = d_wdn, wdb = d_wd, in place).
+void enqueue_run_prologue(dgr_handle *h);
+void enqueue_run_step(dgr_handle *h, const StepBufs *b = nullptr, StepRec *rec_end = nullptr,
+                      StepRec *rec_next = nullptr);
 dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c);
 dgr_status any_error(dgr_handle *h);
 unsigned char *modify_ptr(dgr_handle *h, int stage);

@@ -1317,8 +1317,11 @@ __device__ __forceinline__ double resolve_lam(const MeshDev &m, const PhysDev &p
 
 // one workgroup per member.  With next != 0 the next step's bookkeeping
 // follows (the tide at the new time, the NaN bookkeeping cleared, the step
-// counted), which k_step_begin does otherwise.
-__global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind, int next)
+// counted), which k_step_begin does otherwise.  Optional step records: the
+// step's max |lambda| into rec_end, the next step's t, dt and tide into
+// rec_next (after the next step's bookkeeping).
+__global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind, int next,
+                                                  StepRec *rec_end, StepRec *rec_next)
 {
     const int mid = blockIdx.x;
     Ctrl *c = ctrls + mid;
@@ -1334,6 +1337,12 @@ __global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *c
         step_tide(M, c);
         clear_nan_fold(c);
         c->step += 1;
+    }
+    if (rec_end) rec_end[mid].lam2 = c->maxlam[1];
+    if (rec_next) {
+        rec_next[mid].t = c->t;
+        rec_next[mid].dt = c->dt;
+        rec_next[mid].tide = c->tide;
     }
 }
 
@@ -1450,9 +1459,11 @@ void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s)
     hipLaunchKernelGGL(k_step_begin, member_grid(m), dim3(64), 0, s, m, c);
 }
 
-void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s)
+void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
+                     StepRec *rec_end, StepRec *rec_next)
 {
-    hipLaunchKernelGGL(k_step_end, dim3((unsigned)m.nmem), dim3(256), 0, s, m, ph, c, kind, next ? 1 : 0);
+    hipLaunchKernelGGL(k_step_end, dim3((unsigned)m.nmem), dim3(256), 0, s, m, ph, c, kind, next ? 1 : 0, rec_end,
+                       rec_next);
 }
 
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s)
