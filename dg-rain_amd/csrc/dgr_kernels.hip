@@ -911,21 +911,27 @@ __global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const dou
             // cannot match with the first loads, so nothing of the fast
             // evaluation has to stay live for this path.
             MeshDev m2 = m;
-            const double *fh2 = fhat;
-            asm volatile("" : "+s"(m2.nodes), "+s"(m2.ev), "+s"(fh2));
-            double Fh2[9];
+            const double *fh2 = fhat, *w2 = win;
+            asm volatile("" : "+s"(m2.nodes), "+s"(m2.ev), "+s"(fh2), "+s"(w2));
+            double Fh2[9], W2[9];
 #pragma unroll
-            for (int f = 0; f < 9; ++f) Fh2[f] = fh2[f * P + e];
+            for (int f = 0; f < 9; ++f) { Fh2[f] = fh2[f * P + e]; W2[f] = w2[f * P + e]; }
             DivExact X;
             GeoRegs G2;
             geometry_of(m2, e, ph.g, [&](int i, double v) { G2.v[i] = v; }, X);
-            element_rhs(G2, W, Fh2, wd_self, ph.g, M.rain, R, X);
+            element_rhs(G2, W2, Fh2, wd_self, ph.g, M.rain, R, X);
         }
     }
     if (MODE == 0) {
 #pragma unroll
         for (int f = 0; f < 9; ++f) dbg_rhs[f * P + e] = R[f];
     } else {
+        // the stage input again (a cache hit) rather than kept live through
+        // the RHS
+        const double *w3 = win;
+        asm volatile("" : "+s"(w3));
+#pragma unroll
+        for (int f = 0; f < 9; ++f) W[f] = w3[f * P + e];
         rk_update<MODE>(m, e, W, R, wout, wbase, means, ctrl);
     }
 }
