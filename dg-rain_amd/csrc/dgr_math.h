@@ -36,21 +36,18 @@ __host__ __device__ inline double pow2i(int k)  // 2^k, -1022 <= k <= 1023
 // t^(-1/3): seed + 3 Newton steps y += y(1 - t y^3)/3 (no division);
 // t^(1/3) = t y^2, refined once in double-double; the exponent's representation
 // error applied as the factor (1 - DELTA43 ln x).
+// Branch-free: the special inputs (NaN, x <= 0, inf) run the arithmetic on
+// whatever bits they have and their result is selected at the end, so the
+// function adds no basic blocks to the kernels that inline it.
 __device__ inline double dgr_pow43(double x)
 {
-    if (x != x) return x;
-    if (x <= 0.0) return (x == 0.0) ? 0.0 : __longlong_as_double(0x7ff8000000000000LL);
-    if (x > 1.7976931348623157e308) return x;
-    uint64_t b = (uint64_t)__double_as_longlong(x);
-    int e = (int)((b >> 52) & 0x7ff);
-    if (e == 0) {
-        double xs = x * 18014398509481984.0;  // 2^54
-        b = (uint64_t)__double_as_longlong(xs);
-        e = (int)((b >> 52) & 0x7ff) - 54;
-    }
+    const bool den = x < 2.2250738585072014e-308;   // denormal (or not positive)
+    const double xs = den ? x * 18014398509481984.0 : x;   // 2^54
+    const uint64_t b = (uint64_t)__double_as_longlong(xs);
+    int e = (int)((b >> 52) & 0x7ff) - (den ? 54 : 0);
     e -= 1023;
     const double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
-    const int q = (e >= 0) ? e / 3 : -((-e + 2) / 3);
+    const int q = (e + 3000) / 3 - 1000;   // floor(e / 3) for the exponent range
     const int r = e - 3 * q;
     const double t = m * (double)(1 << r);
 
@@ -83,7 +80,13 @@ __device__ inline double dgr_pow43(double x)
 
     const double res = p_hi + fma(-p_hi, eps, p_lo);
     const double sc = pow2i(2 * q);
-    return (res * sc) * sc;
+    double out = (res * sc) * sc;
+    // computed for every input (keeps the selects below from becoming branches)
+    asm volatile("" : "+v"(out));
+    out = (x > 1.7976931348623157e308) ? x : out;
+    out = (x == 0.0) ? 0.0 : out;
+    out = (x < 0.0) ? __longlong_as_double(0x7ff8000000000000LL) : out;
+    return (x != x) ? x : out;
 }
 
 // ---------------------------------------------------------------------------

@@ -27,13 +27,16 @@ def test_math_selftest_matches_host(gpu):
     rng = np.random.default_rng(7)
     n = 200000
     x = np.concatenate([np.exp(rng.uniform(np.log(1e-6), np.log(1e4), n)),
-                        rng.uniform(-50, 50, 1000), [0.0, -0.0, 1.0, 8.0, 1e-310, 5e-324, 1e300]])
+                        rng.uniform(-50, 50, 1000), [0.0, -0.0, 1.0, 8.0, 1e-310, 5e-324, 1e300],
+                        [np.inf, -np.inf, np.nan, -1.0, 2.2250738585072014e-308, 2.2250738585072009e-308,
+                         1.7976931348623157e308]])
     y = rng.uniform(-3, 3, x.size)
     out = native.math_selftest(x, y)
     assert same(out[:, 0], np.sqrt(x))                  # IEEE sqrt
     assert same(out[:, 1], x / y)                        # IEEE division
     o43 = np.array([oracle.pow43(v) for v in x])
     assert same(out[:, 2], o43)
+    assert np.isnan(out[-7:-4, 2]).sum() == 2 and out[-7, 2] == np.inf   # inf, -inf -> NaN, NaN
     osin = np.array([oracle.dsin(v) for v in x])
     assert same(out[:, 3], osin)
 
