@@ -781,25 +781,43 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
         const int el_in = E.x, el_ex = E.y;
         const int kin = inf & ED_KIN_MASK, kex = (inf >> ED_KEX_SHIFT) & 3;
         const bool boundary = (inf & EI_BOUNDARY) != 0;
-        const double4 A = m.nodes[E.z], B = m.nodes[E.w];
-        const double zin = win[kin * P + el_in], qxin = win[(3 + kin) * P + el_in], qyin = win[(6 + kin) * P + el_in];
-        double zex0 = zin, qxex0 = qxin, qyex0 = qyin;
-        if (!boundary) { zex0 = win[kex * P + el_ex]; qxex0 = win[(3 + kex) * P + el_ex]; qyex0 = win[(6 + kex) * P + el_ex]; }
         const int code = (inf >> EI_CODE_SHIFT) & 0xff;
         const bool tide = (inf & EI_PRESCRIBED) && M.tide_code != 0 && code == M.tide_code;
-        double bz = 0.0, bq = 0.0;
-        if (inf & EI_PRESCRIBED) { bz = tide ? ctrl->tide : bzeta[ed]; bq = tide ? 1000000.0 : bqn[ed]; }
-        const int wd_in = WETDRY ? wd[el_in] : 1;
-        const int wd_ex = WETDRY ? (boundary ? wd_in : wd[el_ex]) : 1;
         const int led = ed - M.ed_off;                 // member-local edge id
         const long long p0 = 2LL * led;
+        // the edge's inputs: vertices, the two midpoint states, boundary data
+        struct EdgeIn {
+            double4 A, B;
+            double zin, qxin, qyin, zex0, qxex0, qyex0, bz, bq;
+            int wd_in, wd_ex;
+        };
+        auto load = [&](const double4 *nodes, const double *w, const double *bzp, const double *bqp,
+                        const int *wdp) {
+            EdgeIn in;
+            in.A = nodes[E.z];
+            in.B = nodes[E.w];
+            in.zin = w[kin * P + el_in]; in.qxin = w[(3 + kin) * P + el_in]; in.qyin = w[(6 + kin) * P + el_in];
+            in.zex0 = in.zin; in.qxex0 = in.qxin; in.qyex0 = in.qyin;
+            if (!boundary) {
+                in.zex0 = w[kex * P + el_ex]; in.qxex0 = w[(3 + kex) * P + el_ex]; in.qyex0 = w[(6 + kex) * P + el_ex];
+            }
+            in.bz = 0.0;
+            in.bq = 0.0;
+            if (inf & EI_PRESCRIBED) { in.bz = tide ? ctrl->tide : bzp[ed]; in.bq = tide ? 1000000.0 : bqp[ed]; }
+            in.wd_in = WETDRY ? wdp[el_in] : 1;
+            in.wd_ex = WETDRY ? (boundary ? in.wd_in : wdp[el_ex]) : 1;
+            return in;
+        };
         LamOps ops;
         double Fi[3], Fx[3];
         bool nan_flux = false;
         // the edge's geometry, boundary data and Roe flux(es) with division
         // policy D.  Geometry from el_in (compute2DL.c:98-133); the squares and
         // the vertex sum do not depend on the vertex order.
-        auto eval = [&](auto &D) {
+        auto eval = [&](const EdgeIn &in, auto &D) {
+            const double4 A = in.A, B = in.B;
+            const double zin = in.zin, qxin = in.qxin, qyin = in.qyin;
+            const int wd_in = in.wd_in, wd_ex = in.wd_ex;
             const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
             const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
             const auto Dl = D.den(len);
@@ -808,10 +826,10 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             const double tx = -ny, ty = nx;
             const double zb = (A.z + B.z) / 2;
             const double dn = D.q(1.0, D.den(nx * ty - ny * tx));
-            double zex = zex0, qxex = qxex0, qyex = qyex0;
+            double zex = in.zex0, qxex = in.qxex0, qyex = in.qyex0;
             if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
-                zex = bz;
-                double qnex = bq;
+                zex = in.bz;
+                double qnex = in.bq;
                 if (zex < 0) zex = zin;
                 if (qnex > 1000) qnex = qxin * nx + qyin * ny;
                 const double qtex = qxin * tx + qyin * ty;
@@ -852,10 +870,18 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
         };
         {
             DivFast D;
-            eval(D);
-            if (!D.ok()) {   // an operand outside the fast division's range: plain divisions
+            eval(load(m.nodes, win, bzeta, bqn, wd), D);
+            if (!D.ok()) {
+                // an operand outside the fast division's range: plain
+                // divisions, on inputs loaded again through pointers the
+                // compiler cannot match with the first loads (so none of them
+                // stays live through the fast evaluation)
+                const double4 *nodes2 = m.nodes;
+                const double *win2 = win, *bz2 = bzeta, *bq2 = bqn;
+                const int *wd2 = wd;
+                asm volatile("" : "+s"(nodes2), "+s"(win2), "+s"(bz2), "+s"(bq2), "+s"(wd2));
                 DivExact X;
-                eval(X);
+                eval(load(nodes2, win2, bz2, bq2, wd2), X);
             }
         }
         if (nan_flux) raise_error(ctrl, 3, led, 2 * slot);
