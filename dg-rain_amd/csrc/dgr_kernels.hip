@@ -1007,13 +1007,14 @@ __global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__
 
 // PDop2D on one element (wetDry2D.c:66-256); zeta/qx/qy are the element's
 // 3+3+3 midpoint values, modified in place.  Returns an error code or 0.
+template <class DV>
 __device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy[3], const double zv[3],
-                                            double H0, double VZ)
+                                            double H0, double VZ, DV &D)
 {
     const double bz0 = 0.5 * (zv[0] + zv[1]), bz1 = 0.5 * (zv[1] + zv[2]), bz2 = 0.5 * (zv[2] + zv[0]);
     double mh0 = z[0] + bz0, mh1 = z[1] + bz1, mh2 = z[2] + bz2;
     double Ht[3] = {mh0 - mh1 + mh2, mh0 + mh1 - mh2, -mh0 + mh1 + mh2};
-    const double avgH = (Ht[0] + Ht[1] + Ht[2]) / 3;
+    const double avgH = D.q(Ht[0] + Ht[1] + Ht[2], D.den_const(3.0));
     if (Ht[0] <= H0 || Ht[1] <= H0 || Ht[2] <= H0) {
         if (avgH <= H0) {
             Ht[0] = avgH; Ht[1] = avgH; Ht[2] = avgH;
@@ -1054,7 +1055,11 @@ __device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy
         double qye[3] = {qy[0] - qy[1] + qy[2], qy[0] + qy[1] - qy[2], -qy[0] + qy[1] + qy[2]};
         double u[3], vv[3];
 #pragma unroll
-        for (int i = 0; i < 3; ++i) { u[i] = qxe[i] / Ht[i]; vv[i] = qye[i] / Ht[i]; }
+        for (int i = 0; i < 3; ++i) {
+            const typename DV::Den DH = D.den(Ht[i]);
+            u[i] = D.q(qxe[i], DH);
+            vv[i] = D.q(qye[i], DH);
+        }
         int npos = 0;
         double du = 0, dv = 0;
 #pragma unroll
@@ -1064,10 +1069,12 @@ __device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy
             dv += vv[i] * (double)(1 - f[i]) * (double)(fabs(qye[i]) > VZ);
         }
         int err = 0;
+        const typename DV::Den Dn = D.den((double)npos);
+        const double dun = D.q(du, Dn), dvn = D.q(dv, Dn);
 #pragma unroll
         for (int i = 0; i < 3; ++i) {
-            u[i] = (double)f[i] * (u[i] + du / npos);
-            vv[i] = (double)f[i] * (vv[i] + dv / npos);
+            u[i] = (double)f[i] * (u[i] + dun);
+            vv[i] = (double)f[i] * (vv[i] + dvn);
             if (isnan(u[i]) || isnan(vv[i])) err = 7;
         }
         if (err) return err;
@@ -1085,11 +1092,13 @@ __device__ __forceinline__ int pdop_element(double z[3], double qx[3], double qy
 
 // wetDryStatus2D (wetDry2D.c:16-60) of one element from its zeta midpoints
 // (a, b, c), vertex depths zz and previous flag
-__device__ __forceinline__ int wd_status_of(double a, double b, double c, const double zz[3], double H0, int old)
+template <class DV>
+__device__ __forceinline__ int wd_status_of(double a, double b, double c, const double zz[3], double H0, int old,
+                                            DV &D)
 {
     const double ze[3] = {a - b + c, a + b - c, -a + b + c};
     const double h[3] = {ze[0] + zz[0], ze[1] + zz[1], ze[2] + zz[2]};
-    const double avgH = (h[0] + h[1] + h[2]) / 3;
+    const double avgH = D.q(h[0] + h[1] + h[2], D.den_const(3.0));
     int mi = 0;
     double mh = h[0];
     if (h[1] > mh) { mi = 1; mh = h[1]; }
@@ -1118,65 +1127,101 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const doub
     const int e = L.e;
     const int P = m.pitch, nn = m.nn;
     const double H0 = ph.h0;
-    int nd[3];
-    double zv[3];
+    struct LimIn {
+        double zv[3], S[9], Vmin[3][3], Vmax[3][3];
+        int old;
+    };
+    auto load = [&](const int *ev, const double4 *nodes, const double *w, const double *nm, const int *wdi) {
+        LimIn in;
+        int nd[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
-    double S[9];
+        for (int k = 0; k < 3; ++k) { nd[k] = ev[k * P + e]; in.zv[k] = nodes[nd[k]].z; }
 #pragma unroll
-    for (int f = 0; f < 9; ++f) S[f] = win[f * P + e];
-    int modify = 0;
-    const Divisor D3 = divisor(3.0);
+        for (int f = 0; f < 9; ++f) in.S[f] = w[f * P + e];
 #pragma unroll
-    for (int iv = 0; iv < 3; ++iv) {
-        const double w1 = S[3 * iv], w2 = S[3 * iv + 1], w3 = S[3 * iv + 2];
-        double V[3] = {w1 - w2 + w3, w1 + w2 - w3, -w1 + w2 + w3};
-        double Vmin[3], Vmax[3];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) { Vmin[k] = nodemm[iv * nn + nd[k]]; Vmax[k] = nodemm[(3 + iv) * nn + nd[k]]; }
-        const double avg = quot(w1 + w2 + w3, D3);
-        if (iv == 0) {
-            const double Ha = V[0] + zv[0], Hb = V[1] + zv[1], Hc = V[2] + zv[2];
-            const double Hm0 = Vmin[0] + zv[0], Hm1 = Vmin[1] + zv[1], Hm2 = Vmin[2] + zv[2];
-            if (Ha > H0 && Hb > H0 && Hc > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
-        }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) V[k] = fmax(DGR_MIN(V[k], Vmax[k]), Vmin[k]);
-#pragma unroll
-        for (int ll = 0; ll < 3; ++ll) {
-            const double sumloc = quot(V[0] + V[1] + V[2], D3);
-            double sumdiff = (sumloc - avg) * 3;
-            const int sd = isign(sumdiff);
-            const double diff[3] = {(V[0] - avg) * sd, (V[1] - avg) * sd, (V[2] - avg) * sd};
-            double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
+        for (int iv = 0; iv < 3; ++iv)
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                // sumdiff*sd / max(1, kdp) with kdp in {0,1,2,3}: x/1 = x and
-                // x/2 = x*0.5 exactly
-                double redfac;
-                if (diff[k] > 0) {
-                    const double num = sumdiff * sd;
-                    redfac = (kdp > 2.0) ? quot(num, D3) : (kdp > 1.0) ? num * 0.5 : num;
-                    kdp -= 1;
-                } else redfac = 0;
-                const double redmax = (sd > 0) ? V[k] - Vmin[k] : Vmax[k] - V[k];
-                redfac = DGR_MIN(redfac, redmax);
-                sumdiff -= redfac * sd;
-                V[k] -= redfac * sd;
+                in.Vmin[iv][k] = nm[iv * nn + nd[k]];
+                in.Vmax[iv][k] = nm[(3 + iv) * nn + nd[k]];
             }
+        in.old = (PDOP && wd_out) ? wdi[e] : 0;
+        return in;
+    };
+    double S[9];
+    int modify = 0, err = 0, wdv = 0;
+    // the limiter pass, PDop and the wet/dry status with division policy D
+    auto eval = [&](const LimIn &in, auto &D) {
+        const double *zv = in.zv;
+#pragma unroll
+        for (int f = 0; f < 9; ++f) S[f] = in.S[f];
+        modify = 0;
+        const auto D3 = D.den_const(3.0);
+#pragma unroll
+        for (int iv = 0; iv < 3; ++iv) {
+            const double w1 = S[3 * iv], w2 = S[3 * iv + 1], w3 = S[3 * iv + 2];
+            double V[3] = {w1 - w2 + w3, w1 + w2 - w3, -w1 + w2 + w3};
+            const double *Vmin = in.Vmin[iv], *Vmax = in.Vmax[iv];
+            const double avg = D.q(w1 + w2 + w3, D3);
+            if (iv == 0) {
+                const double Ha = V[0] + zv[0], Hb = V[1] + zv[1], Hc = V[2] + zv[2];
+                const double Hm0 = Vmin[0] + zv[0], Hm1 = Vmin[1] + zv[1], Hm2 = Vmin[2] + zv[2];
+                if (Ha > H0 && Hb > H0 && Hc > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
+            }
+#pragma unroll
+            for (int k = 0; k < 3; ++k) V[k] = fmax(DGR_MIN(V[k], Vmax[k]), Vmin[k]);
+#pragma unroll
+            for (int ll = 0; ll < 3; ++ll) {
+                const double sumloc = D.q(V[0] + V[1] + V[2], D3);
+                double sumdiff = (sumloc - avg) * 3;
+                const int sd = isign(sumdiff);
+                const double diff[3] = {(V[0] - avg) * sd, (V[1] - avg) * sd, (V[2] - avg) * sd};
+                double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    // sumdiff*sd / max(1, kdp) with kdp in {0,1,2,3}: x/1 = x and
+                    // x/2 = x*0.5 exactly
+                    double redfac;
+                    if (diff[k] > 0) {
+                        const double num = sumdiff * sd;
+                        redfac = (kdp > 2.0) ? D.q(num, D3) : (kdp > 1.0) ? num * 0.5 : num;
+                        kdp -= 1;
+                    } else redfac = 0;
+                    const double redmax = (sd > 0) ? V[k] - Vmin[k] : Vmax[k] - V[k];
+                    redfac = DGR_MIN(redfac, redmax);
+                    sumdiff -= redfac * sd;
+                    V[k] -= redfac * sd;
+                }
+            }
+            const double mid1 = (V[0] + V[1]) / 2, mid2 = (V[1] + V[2]) / 2, mid3 = (V[2] + V[0]) / 2;
+            if (iv == 0) {
+                const double Ha = V[0] + zv[0], Hb = V[1] + zv[1], Hc = V[2] + zv[2];
+                if (Ha < H0 || Hb < H0 || Hc < H0) modify = 0;
+            }
+            if (modify) { S[3 * iv] = mid1; S[3 * iv + 1] = mid2; S[3 * iv + 2] = mid3; }
         }
-        const double mid1 = (V[0] + V[1]) / 2, mid2 = (V[1] + V[2]) / 2, mid3 = (V[2] + V[0]) / 2;
-        if (iv == 0) {
-            const double Ha = V[0] + zv[0], Hb = V[1] + zv[1], Hc = V[2] + zv[2];
-            if (Ha < H0 || Hb < H0 || Hc < H0) modify = 0;
+        err = 0;
+        if (PDOP) {
+            err = pdop_element(&S[0], &S[3], &S[6], zv, H0, ph.velzero, D);
+            if (wd_out) wdv = wd_status_of(S[0], S[1], S[2], zv, H0, in.old, D);
         }
-        if (modify) { S[3 * iv] = mid1; S[3 * iv + 1] = mid2; S[3 * iv + 2] = mid3; }
+    };
+    {
+        DivFast D;
+        eval(load(m.ev, m.nodes, win, nodemm, wd_in), D);
+        if (!D.ok()) {
+            // an operand outside the fast division's range: plain divisions on
+            // inputs loaded again (nothing of the fast pass stays live for it)
+            const int *ev2 = m.ev, *wd2 = wd_in;
+            const double4 *nodes2 = m.nodes;
+            const double *win2 = win, *nm2 = nodemm;
+            asm volatile("" : "+s"(ev2), "+s"(nodes2), "+s"(win2), "+s"(nm2), "+s"(wd2));
+            DivExact X;
+            eval(load(ev2, nodes2, win2, nm2, wd2), X);
+        }
     }
-    if (PDOP) {
-        const int err = pdop_element(&S[0], &S[3], &S[6], zv, H0, ph.velzero);
-        if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 2 * stage + 1);
-        if (wd_out) wd_out[e] = wd_status_of(S[0], S[1], S[2], zv, H0, wd_in[e]);
-    }
+    if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 2 * stage + 1);
+    if (PDOP && wd_out) wd_out[e] = wdv;
 #pragma unroll
     for (int f = 0; f < 9; ++f) wout[f * P + e] = S[f];
     if (modify_out) modify_out[e] = (unsigned char)modify;
@@ -1194,7 +1239,8 @@ __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, 
     for (int k = 0; k < 3; ++k) zv[k] = m.nodes[m.ev[k * P + e]].z;
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = w[f * P + e];
-    const int err = pdop_element(&S[0], &S[3], &S[6], zv, ph.h0, ph.velzero);
+    DivExact D;
+    const int err = pdop_element(&S[0], &S[3], &S[6], zv, ph.h0, ph.velzero, D);
     if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 1);
 #pragma unroll
     for (int f = 0; f < 9; ++f) w[f * P + e] = S[f];
@@ -1211,7 +1257,8 @@ __global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const 
     double zz[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) zz[k] = m.nodes[m.ev[k * P + e]].z;
-    wd_out[e] = wd_status_of(w[e], w[P + e], w[2 * P + e], zz, ph.h0, wd_in[e]);
+    DivExact D;
+    wd_out[e] = wd_status_of(w[e], w[P + e], w[2 * P + e], zz, ph.h0, wd_in[e], D);
 }
 
 __device__ __forceinline__ void clear_fold(Ctrl *c)
@@ -1401,8 +1448,15 @@ __global__ void k_math_selftest(int n, const double *x, const double *y, double 
     out[7 * i + 2] = dgr_pow43(a);
     out[7 * i + 3] = dgr_sin(a);
     out[7 * i + 4] = fma(a, b, a);
-    out[7 * i + 5] = quot(a, divisor(b));
-    out[7 * i + 6] = quot(a, divisor(3.0));
+    DivFast D;
+    const double q1 = D.q(a, D.den(b));
+    const bool ok1 = D.ok();
+    DivFast D3;
+    const double q3 = D3.q(a, D3.den_const(3.0));
+    const bool ok3 = D3.ok();
+    // the fast quotient where its range check passes, else plain division
+    out[7 * i + 5] = ok1 ? q1 : a / b;
+    out[7 * i + 6] = ok3 ? q3 : a / 3.0;
 }
 
 // ---------------------------------------------------------------------------

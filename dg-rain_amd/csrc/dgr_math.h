@@ -1,7 +1,7 @@
 // dgr_math.h -- deterministic FP64 helpers for the gfx950 kernels.
 //
-// Every function here uses only IEEE-754 basic operations (+ - * / sqrt) and
-// fused multiply-add, so it returns the same bits on any IEEE machine.
+// dgr_pow43 and dgr_sin use only IEEE-754 basic operations (+ - * / sqrt)
+// and fused multiply-add, so they return the same bits on any IEEE machine.
 //
 //  dgr_pow43(x)  x^(4./3), the Manning term of the reference's friction
 //                (2DCode/2DInnerProducts.c:318).  The reference calls glibc
@@ -10,6 +10,8 @@
 //                one is accurate to ~1e-22 before the final rounding, i.e.
 //                correctly rounded except in vanishingly rare cases.
 //  dgr_sin(x)    sine for the tidal boundary forcing (no reference code).
+//  DivExact / DivFast / DivRecip   division policies (below); DivFast gives
+//                the bits of x / y with fewer instructions, DivRecip does not.
 //
 // Constants: see oracle/gen_constants.py for how each value was derived.
 #pragma once
@@ -90,66 +92,29 @@ __device__ inline double dgr_pow43(double x)
 }
 
 // ---------------------------------------------------------------------------
-// Several divisions by one denominator, each bit-identical to x / y.
+// Division policies.
 //
 // gfx950 divides doubles (correctly rounded) with
 //   d = v_div_scale(y, y, x); r = v_rcp(d);
 //   r = fma(r, fma(-d, r, 1), r) twice;
 //   n = v_div_scale(x, y, x); q = n * r; e = fma(-d, q, n);
 //   result = v_div_fixup(v_div_fmas(e, r, q), y, x).
-// When no operand needs exponent scaling (|y| in [2^-320, 2^320] and |x/y| in
-// [2^-300, 2^300]: far from zero, denormals and overflow) both div_scale
-// steps return their first operand with the scale flag clear, div_fmas is the
-// plain fma and div_fixup returns its first operand.  Only q, e and the final
-// fma then depend on x; Divisor holds the rest, and quot() repeats the
-// x-dependent part for each numerator (3 instructions instead of 11).  Any
-// result outside the range above (also 0, inf, NaN) is recomputed as x / y.
-// ---------------------------------------------------------------------------
-struct Divisor {
-    double y, r;
-    bool ok;
-};
-
-__device__ __forceinline__ Divisor divisor(double y)
-{
-    double d = y;
-    // opaque to the optimiser: a constant y must still go through v_rcp_f64
-    // (constant folding would give the correctly rounded 1/y instead)
-    asm volatile("" : "+v"(d));
-    double r = __builtin_amdgcn_rcp(d);
-    double e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    e = fma(-d, r, 1.0);
-    r = fma(r, e, r);
-    const double a = fabs(y);
-    return Divisor{y, r, a >= 0x1p-320 && a <= 0x1p320};
-}
-
-__device__ __forceinline__ double quot(double x, const Divisor &D)
-{
-    const double q = x * D.r;
-    const double e = fma(-D.y, q, x);
-    const double res = fma(e, D.r, q);
-    const double a = fabs(res);
-    if (D.ok && a >= 0x1p-300 && a <= 0x1p300) return res;
-    // the barrier keeps the full division inside this rarely taken branch
-    // (otherwise it is computed for every call and selected)
-    double y = D.y;
-    asm volatile("" : "+v"(y));
-    return x / y;
-}
-
-// ---------------------------------------------------------------------------
-// Division policies for code that evaluates a whole element (or edge) with
-// the 3-instruction division above and evaluates it again with plain
-// division when any operand left the range where the two agree.  DivFast
-// checks without branching: it folds the operands into three running bounds
-// and ok() tests them once at the end.
+// When no operand needs exponent scaling both div_scale steps return their
+// first operand with the scale flag clear, div_fmas is the plain fma and
+// div_fixup returns its first operand (up to the sign of a zero quotient,
+// which it sets to sign(x) xor sign(y)).  Only q, e and the final fma then
+// depend on x: DivFast keeps the rest per divisor (Den) and computes each
+// quotient in 3 instructions, bit-identical to x / y.  It is exact for
 //   divisors   |y| in [2^-320, 2^320]
 //   numerators x = 0, or |x| in [2^-700, 2^320] (frexp exponent >= -699)
-// so |x/y| < 2^640 and no operand needs div_scale's exponent scaling.  A NaN
-// operand gives NaN either way.  A zero quotient takes the sign of x*r, the
-// sign div_fixup gives it (the final fma rounds -0 + +0 to +0).
+// (then |x/y| < 2^640 and nothing is scaled).  It checks this without
+// branching, by folding the operands into three running bounds that ok()
+// tests once: code evaluates a whole element (or edge) with DivFast and
+// evaluates it again with DivExact when ok() fails.  A NaN operand gives NaN
+// either way.  A zero quotient takes the sign of x*r, as div_fixup gives it
+// (the final fma alone would round -0 + +0 to +0).
+// DivRecip (reciprocal products, a few ulps off) is for values that only
+// enter derivatives, never a forward result.
 // ---------------------------------------------------------------------------
 struct DivExact {
     struct Den {
