@@ -25,6 +25,8 @@ for p in (ROOT, os.path.join(ROOT, "dg-rain_amd")):
         sys.path.insert(0, p)
 
 HBM_PEAK_GBS = 8000.0
+# FP64 VALU issue peak: 256 CUs x 4 SIMDs x 2.4 GHz / 4 cycles per wave64 instruction
+VALU_PEAK_GWIS = 256 * 4 * 2.4 / 4
 # algorithmic HBM bytes per element for one launch of the flux/RHS/update
 # kernel, averaged over the two stages: state in 72 + state out 72 + step-start
 # state 36 (72 in stage 2 only) + mesh 40 + element means out 24
@@ -90,6 +92,19 @@ def load_traffic(variant):
             d = json.load(f)
         if d.get("variant") == variant:
             return d.get("bytes_per_launch")
+    return None
+
+
+def load_valu(variant):
+    """Wave-level VALU instructions per stage launch of the stage kernel pair,
+    from the SQ_INSTS_VALU pass summarised in profiles/ (None when it covers
+    another variant)."""
+    p = os.path.join(ROOT, "profiles", "stage_kernels_coastal_512_valu.json")
+    if os.path.exists(p):
+        with open(p) as f:
+            d = json.load(f)
+        if d.get("variant") == variant:
+            return d.get("valu_wave_insts_per_launch")
     return None
 
 
@@ -294,6 +309,7 @@ def main():
     k_avg_s = (k_ms / k_n) * 1e-3
     achieved = KSTAGE_BYTES_PER_EL * ne / k_avg_s / 1e9
     traffic = load_traffic(variant) if a.N == 512 else None
+    valu = load_valu(variant) if a.N == 512 else None
     stage_gbs = STAGE_BYTES_PER_EL * ne * 2 * a.steps / el / 1e9
     out = {
         "metric": "FP64 DOF-updates/s per RK stage",
@@ -315,6 +331,11 @@ def main():
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "avg_launch_ms": k_avg_s * 1e3, "launches": k_n,
                      "bytes_per_element": KSTAGE_BYTES_PER_EL},
+        # the pair's real bound: FP64 VALU issue (wave-instructions/s against
+        # 1024 SIMDs x 2.4 GHz / 4 cycles), instruction count from the PMC pass
+        "valu_issue": None if valu is None else {
+            "achieved": valu / k_avg_s / 1e9, "peak": VALU_PEAK_GWIS, "unit": "G wave-instructions/s",
+            "frac": valu / k_avg_s / 1e9 / VALU_PEAK_GWIS, "insts_per_launch": valu},
         "stage_roofline_frac": stage_gbs / HBM_PEAK_GBS,
         "stage_variant": {"chosen": variant, "autotune_ms_per_step": tune_ms},
         "ensemble_speedup": None,
