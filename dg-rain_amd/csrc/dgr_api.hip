@@ -62,7 +62,7 @@ static dgr_status free_handle(dgr_handle *h)
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_edges,
                     h->d_einfo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
                     h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts,
-                    h->d_nanparts, h->d_wdn};
+                    h->d_nanparts, h->d_wdn, h->d_hydro};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -418,6 +418,57 @@ extern "C" dgr_status dgr_member_get_modify(dgr_handle *h, int mi, int stage, un
                            h->stream));
     DGR_HIP(hipStreamSynchronize(h->stream));
     return DGR_OK;
+}
+
+// Hydrographs: every member's tables are concatenated into one device array
+// (MeshDev::hydro), rebuilt on each change; the captured step graph holds the
+// mesh descriptor by value, so it is dropped.
+extern "C" dgr_status dgr_member_set_hydrograph(dgr_handle *h, int mi, int code, int npts, const double *t,
+                                                const double *zeta_b, const double *qn_b)
+{
+    if (bad_member(h, mi) || code < 1 || code > 3 || npts < 0 || (npts > 0 && (!t || !zeta_b || !qn_b)))
+        return DGR_ERR_ARG;
+    if (code == h->mem[mi].prm.tide_code) return DGR_ERR_ARG;
+    for (int i = 1; i < npts; ++i)
+        if (!(t[i] > t[i - 1])) return DGR_ERR_ARG;
+    std::vector<double> &v = h->mem[mi].hyd[code - 1];
+    v.assign(3 * (size_t)npts, 0.0);
+    for (int i = 0; i < npts; ++i) { v[i] = t[i]; v[npts + i] = zeta_b[i]; v[2 * npts + i] = qn_b[i]; }
+    std::vector<MemberDev> md(h->nmem);
+    DGR_HIP(hipMemcpyAsync(md.data(), h->d_mem, h->nmem * sizeof(MemberDev), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    std::vector<double> all;
+    for (int i = 0; i < h->nmem; ++i)
+        for (int c = 0; c < 3; ++c) {
+            const std::vector<double> &hv = h->mem[i].hyd[c];
+            md[i].hyd_n[c] = (int)(hv.size() / 3);
+            md[i].hyd_off[c] = (int)all.size();
+            all.insert(all.end(), hv.begin(), hv.end());
+        }
+    if (h->d_hydro) {
+        DGR_HIP(hipStreamSynchronize(h->stream));
+        DGR_HIP(hipFree(h->d_hydro));
+        h->d_hydro = nullptr;
+    }
+    if (!all.empty()) {
+        DGR_HIP(dalloc(&h->d_hydro, all.size()));
+        DGR_HIP(hipMemcpyAsync(h->d_hydro, all.data(), all.size() * sizeof(double), hipMemcpyHostToDevice,
+                               h->stream));
+    }
+    DGR_HIP(hipMemcpyAsync(h->d_mem, md.data(), h->nmem * sizeof(MemberDev), hipMemcpyHostToDevice, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    h->m.hydro = h->d_hydro;
+    if (h->graph_exec) {
+        DGR_HIP(hipGraphExecDestroy(h->graph_exec));
+        h->graph_exec = nullptr;
+    }
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_set_hydrograph(dgr_handle *h, int code, int npts, const double *t, const double *zeta_b,
+                                         const double *qn_b)
+{
+    return dgr_member_set_hydrograph(h, 0, code, npts, t, zeta_b, qn_b);
 }
 
 extern "C" dgr_status dgr_member_set_boundary(dgr_handle *h, int mi, const double *bz, const double *bq)

@@ -87,7 +87,10 @@ struct MemberDev {
     int ed_off, ned;          // edges [ed_off, ed_off + ned)
     int eblk0, eblks;         // element workgroups of the member
     int dblk0, dblks;         // edge workgroups of the member
-    int tide_code, pad;
+    int tide_code;
+    int hyd_n[3], hyd_off[3];  // hydrograph of boundary code c+1: hyd_n points at
+                               // MeshDev::hydro + hyd_off (t, then zeta_b, then qn_b)
+    int pad;
     double rain, cfl, min_edge, final_time;
     double tide_datum, tide_amp, tide_period, tide_phase;
 };
@@ -106,6 +109,7 @@ struct MeshDev {
     const MemberDev *mem;
     const int *eblk_mem;
     const int *dblk_mem;
+    const double *hydro;      // hydrograph tables of all members
 };
 
 // one per member
@@ -127,7 +131,25 @@ struct Ctrl {
     long long nan_pos[2];
     unsigned long long after_bits[2];
     int any_after[2];
+    // boundary values of the codes with a hydrograph at the current step
+    double hyd_z[3], hyd_q[3];
 };
+
+// Hydrograph value at `time`: piecewise linear through n points (t strictly
+// increasing), held at the end values outside them.  The oracle's
+// orc_hydro_eval is the same function operation for operation.
+__host__ __device__ inline double hydro_eval(int n, const double *t, const double *v, double time)
+{
+    if (time <= t[0]) return v[0];
+    if (time >= t[n - 1]) return v[n - 1];
+    int lo = 0, hi = n - 1;          // t[lo] <= time < t[hi]
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (t[mid] <= time) lo = mid;
+        else hi = mid;
+    }
+    return v[lo] + (v[hi] - v[lo]) * ((time - t[lo]) / (t[hi] - t[lo]));
+}
 
 __host__ __device__ inline double maxlam_of(const Ctrl &c, int slot)
 {

@@ -62,6 +62,7 @@ struct MemberHost {
     int el_off, nd_off, ed_off;   // offsets into the concatenated arrays
     dgr_params prm;
     double min_edge;
+    std::vector<double> hyd[3];   // hydrograph of code c+1: t, zeta_b, qn_b (3n values)
 };
 
 struct dgr_handle {
@@ -90,6 +91,7 @@ struct dgr_handle {
     unsigned long long *d_lamparts;   // [2][maxparts] per-workgroup max |lambda|
     int *d_nanparts;                  // [2][maxparts] last NaN operand per edge workgroup
     int *d_wdn;      // [pitch] wet/dry flags for the next step's start (dgr_run)
+    double *d_hydro; // hydrograph tables of all members (MeshDev::hydro), or null
     int variant;     // 0 fused, 1 split element-centric, 2 split edge-centric
     int kind[2];     // partials space of the last flux launch of each stage
     // host staging

@@ -54,6 +54,20 @@ __device__ __forceinline__ void raise_error(Ctrl *c, int code, int where, int ph
     atomicMin(&c->err_key, key);
 }
 
+// Prescribed boundary data of an edge with BdryPrescribed code `code`: the
+// tide on the tide code, the current hydrograph values on a code that has
+// one, else the edge's bzeta/bqn (boundary_conditions.c:10-83).
+__device__ __forceinline__ void prescribed_values(const MemberDev &M, const Ctrl *ctrl, int code,
+                                                  const double *bzeta, const double *bqn, int ed, double &zb,
+                                                  double &qn)
+{
+    const bool tide = M.tide_code != 0 && code == M.tide_code;
+    const int ci = (code >= 1 && code <= 3) ? code - 1 : 0;
+    const bool hyd = code >= 1 && code <= 3 && M.hyd_n[ci] != 0;
+    zb = tide ? ctrl->tide : hyd ? ctrl->hyd_z[ci] : bzeta[ed];
+    qn = tide ? 1000000.0 : hyd ? ctrl->hyd_q[ci] : bqn[ed];
+}
+
 // ---------------------------------------------------------------------------
 // Roe flux, numericalFlux2D.c:6-108.  Returns max |lambda| over the six
 // one-sided eigenvalues (:101-105).
@@ -462,9 +476,8 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
         }
         if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
             const int code = (inf >> EI_CODE_SHIFT) & 0xff;
-            const bool tide = M.tide_code != 0 && code == M.tide_code;
-            zex = tide ? ctrl->tide : bzeta[ed];
-            double qnex = tide ? 1000000.0 : bqn[ed];
+            double qnex;
+            prescribed_values(M, ctrl, code, bzeta, bqn, ed, zex, qnex);
             if (zex < 0) zex = zin;
             if (qnex > 1000) qnex = qxin * nx + qyin * ny;
             const double qtex = qxin * tx + qyin * ty;
@@ -782,7 +795,6 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
         const int kin = inf & ED_KIN_MASK, kex = (inf >> ED_KEX_SHIFT) & 3;
         const bool boundary = (inf & EI_BOUNDARY) != 0;
         const int code = (inf >> EI_CODE_SHIFT) & 0xff;
-        const bool tide = (inf & EI_PRESCRIBED) && M.tide_code != 0 && code == M.tide_code;
         const int led = ed - M.ed_off;                 // member-local edge id
         const long long p0 = 2LL * led;
         // the edge's inputs: vertices, the two midpoint states, boundary data
@@ -803,7 +815,7 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             }
             in.bz = 0.0;
             in.bq = 0.0;
-            if (inf & EI_PRESCRIBED) { in.bz = tide ? ctrl->tide : bzp[ed]; in.bq = tide ? 1000000.0 : bqp[ed]; }
+            if (inf & EI_PRESCRIBED) prescribed_values(M, ctrl, code, bzp, bqp, ed, in.bz, in.bq);
             in.wd_in = WETDRY ? wdp[el_in] : 1;
             in.wd_ex = WETDRY ? (boundary ? in.wd_in : wdp[el_ex]) : 1;
             return in;
@@ -1288,10 +1300,21 @@ __device__ __forceinline__ void clear_nan_fold(Ctrl *c)
     c->any_after[1] = 0;
 }
 
-__device__ __forceinline__ void step_tide(const MemberDev &M, Ctrl *c)
+// the time-dependent boundary values of the step starting at c->t: the tide
+// and the hydrographs (boundary_conditions.c:10-83 is called once per step)
+__device__ __forceinline__ void step_tide(const MeshDev &m, const MemberDev &M, Ctrl *c)
 {
     if (M.tide_code)
         c->tide = M.tide_datum + M.tide_amp * dgr_sin(6.283185307179586 * c->t / M.tide_period + M.tide_phase);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const int n = M.hyd_n[k];
+        if (n) {
+            const double *h = m.hydro + M.hyd_off[k];
+            c->hyd_z[k] = hydro_eval(n, h, h + n, c->t);
+            c->hyd_q[k] = hydro_eval(n, h, h + 2 * n, c->t);
+        }
+    }
 }
 
 __global__ __launch_bounds__(64) void k_step_begin(MeshDev m, Ctrl *ctrls)
@@ -1300,7 +1323,7 @@ __global__ __launch_bounds__(64) void k_step_begin(MeshDev m, Ctrl *ctrls)
     if (mid >= m.nmem) return;
     const MemberDev &M = m.mem[mid];
     Ctrl *c = ctrls + mid;
-    step_tide(M, c);
+    step_tide(m, M, c);
     clear_fold(c);
     c->step += 1;
 }
@@ -1413,7 +1436,7 @@ __global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *c
     c->t = time;
     c->dt = h;
     if (next) {
-        step_tide(M, c);
+        step_tide(m, M, c);
         clear_nan_fold(c);
         c->step += 1;
     }

@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdgrain.so")
 NUM_VARIANTS = 3   # DGR_NUM_VARIANTS in include/dgrain.h
 
 DGR_OK = 0
+DGR_ERR_UNSUPPORTED = 7
 STATUS = {0: "ok", 1: "invalid argument", 2: "inconsistent mesh", 3: "HIP runtime error",
           4: "NaN flux or wet/dry failure", 5: "out of memory", 6: "no HIP device", 7: "unsupported"}
 
@@ -103,6 +104,8 @@ def load():
         "dgr_member_get_wd": (C.c_int, [vp, C.c_int, ip]),
         "dgr_member_get_modify": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_ubyte)]),
         "dgr_member_set_boundary": (C.c_int, [vp, C.c_int, dp, dp]),
+        "dgr_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
+        "dgr_member_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
         "dgr_member_set_time": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
         "dgr_member_report": (C.c_int, [vp, C.c_int, C.POINTER(Report)]),
         "dgr_member_layout": (C.c_int, [vp, C.c_int, ip, ip]),
@@ -271,6 +274,15 @@ class _Handle:
         gn = np.zeros(self._num_nodes())
         check(self.lib.dgr_adjoint_gauges(self.h, C.byref(spec), _dp(loss), _dp(gz), _dp(gn)), "dgr_adjoint_gauges")
         return loss, gz, gn
+
+    def set_hydrograph(self, code, t, zeta_b, qn_b, member=0):
+        """Boundary code `code` (1-3) follows (zeta_b, qn_b)(t), piecewise
+        linear through the points; empty arrays remove it."""
+        t, z, q = (np.ascontiguousarray(a, dtype=np.float64) for a in (t, zeta_b, qn_b))
+        if not (t.size == z.size == q.size):
+            raise ValueError("hydrograph arrays differ in length")
+        check(self.lib.dgr_member_set_hydrograph(self.h, int(member), int(code), t.size, _dp(t), _dp(z), _dp(q)),
+              "dgr_member_set_hydrograph")
 
     def checkpoint_write(self, path, member=0):
         """Restart file of a member: state, wet/dry flags, t and dt."""

@@ -124,6 +124,16 @@ dgr_status dgr_get_wd(dgr_handle *h, int *wd);
 dgr_status dgr_get_modify(dgr_handle *h, int stage, unsigned char *modify);
 dgr_status dgr_set_boundary(dgr_handle *h, const double *bzeta, const double *bqn);
 dgr_status dgr_set_time(dgr_handle *h, double t, double dt);
+/* Hydrograph (extension; the reference's boundary_conditions.c:10-83 holds
+ * codes 1-3 constant): the prescribed edges of BdryPrescribed code `code`
+ * (1, 2 or 3; not the tide code) take zeta_b(t) and qn_b(t), piecewise linear
+ * through npts points (t strictly increasing), held at the end values outside
+ * them, evaluated at each step's start time by dgr_run.  The reference's
+ * sentinels still apply (zeta_b < 0: interior zeta; qn_b > 1000: interior
+ * normal discharge).  npts = 0 removes it.  dgr_adjoint_gauges returns
+ * DGR_ERR_UNSUPPORTED while any hydrograph is set. */
+dgr_status dgr_set_hydrograph(dgr_handle *h, int code, int npts, const double *t, const double *zeta_b,
+                              const double *qn_b);
 
 /* compute2DL: RHS (and optionally the edge fluxes Fhat{1,2,3}dotn) of the
  * current state, reference layout; max_lambda may be NULL. */
@@ -191,6 +201,8 @@ dgr_status dgr_member_get_wd(dgr_handle *h, int member, int *wd);
 dgr_status dgr_member_get_modify(dgr_handle *h, int member, int stage, unsigned char *modify);
 dgr_status dgr_member_set_boundary(dgr_handle *h, int member, const double *bzeta, const double *bqn);
 dgr_status dgr_member_set_time(dgr_handle *h, int member, double t, double dt);
+dgr_status dgr_member_set_hydrograph(dgr_handle *h, int member, int code, int npts, const double *t,
+                                     const double *zeta_b, const double *qn_b);
 dgr_status dgr_member_report(dgr_handle *h, int member, dgr_report *report);
 /* element range of a member inside the device planes of dgr_device_state */
 dgr_status dgr_member_layout(dgr_handle *h, int member, int *el_offset, int *num_el);

@@ -90,6 +90,8 @@ def _load_oracle():
             "orc_wetdry_status": (None, [vp]),
             "orc_pdop": (None, [vp]),
             "orc_boundary_conditions": (None, [vp, C.c_double]),
+            "orc_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
+            "orc_hydro_eval": (C.c_double, [C.c_int, dp, dp, C.c_double]),
             "orc_one_time_step": (C.c_int, [vp, C.c_double, C.c_double]),
             "orc_run": (C.c_int, [vp, C.c_int, dp, dp]),
             "orc_pow43": (C.c_double, [C.c_double]),
@@ -103,6 +105,12 @@ def _load_oracle():
             fn.argtypes = args
         _olib = lib
     return _olib
+
+
+def hydro_eval(t, v, time):
+    t = np.ascontiguousarray(t, np.float64)
+    v = np.ascontiguousarray(v, np.float64)
+    return _load_oracle().orc_hydro_eval(t.size, _dp(t), _dp(v), float(time))
 
 
 def pow43(x):
@@ -211,6 +219,13 @@ class Oracle:
 
     def boundary_conditions(self, t):
         self.lib.orc_boundary_conditions(self.h, float(t))
+
+    def set_hydrograph(self, code, t, zeta, qn):
+        t, z, q = (np.ascontiguousarray(a, np.float64) for a in (t, zeta, qn))
+        if not (t.size == z.size == q.size):
+            raise ValueError("hydrograph arrays differ in length")
+        if self.lib.orc_set_hydrograph(self.h, int(code), t.size, _dp(t), _dp(z), _dp(q)):
+            raise ValueError("bad hydrograph")
 
     def compute2DL(self):
         rc = self.lib.orc_compute_L(self.h)

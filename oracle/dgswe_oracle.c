@@ -69,6 +69,10 @@ struct orc_state {
     double *nmin, *nmax;            /* [3*num_nodes] zeta,qx,qy */
     /* RK scratch */
     double *w0, *w1;
+    /* hydrographs (extension): per boundary code 1..3, hyd_n points of
+     * (t, zeta_b, qn_b); 0 points = the code's constant values */
+    int hyd_n[3];
+    double *hyd[3];                 /* [3 * hyd_n]: t, then zeta_b, then qn_b */
 };
 
 /* mathfunctions.c:4-11 */
@@ -259,6 +263,7 @@ orc_state *orc_create(const orc_mesh *mesh, const orc_params *p)
 void orc_destroy(orc_state *s)
 {
     if (!s) return;
+    for (int c = 0; c < 3; ++c) free(s->hyd[c]);
     free(s->zeta); free(s->qx); free(s->qy);
     free(s->f1); free(s->f2); free(s->f3);
     free(s->rz); free(s->rqx); free(s->rqy);
@@ -914,7 +919,42 @@ void orc_pdop(orc_state *s)
 }
 
 /* ------------------------------------------------------------------------ */
-/* boundary_conditions.c:10-83 (+ tide extension)                           */
+/* hydrograph extension (no reference code): piecewise-linear in time,      */
+/* held at the end values outside [t_0, t_{n-1}]; t strictly increasing.    */
+/* ------------------------------------------------------------------------ */
+double orc_hydro_eval(int n, const double *t, const double *v, double time)
+{
+    if (time <= t[0]) return v[0];
+    if (time >= t[n - 1]) return v[n - 1];
+    int lo = 0, hi = n - 1;          /* t[lo] <= time < t[hi] */
+    while (hi - lo > 1) {
+        const int mid = (lo + hi) / 2;
+        if (t[mid] <= time) lo = mid;
+        else hi = mid;
+    }
+    return v[lo] + (v[hi] - v[lo]) * ((time - t[lo]) / (t[hi] - t[lo]));
+}
+
+int orc_set_hydrograph(orc_state *s, int code, int n, const double *t, const double *zb, const double *qn)
+{
+    if (code < 1 || code > 3 || n < 0 || code == s->p.tide_code) return -1;
+    for (int i = 1; i < n; ++i)
+        if (!(t[i] > t[i - 1])) return -1;
+    free(s->hyd[code - 1]);
+    s->hyd[code - 1] = NULL;
+    s->hyd_n[code - 1] = 0;
+    if (n == 0) return 0;
+    double *h = (double *)orc_xcalloc((size_t)3 * n, sizeof(double));
+    memcpy(h, t, (size_t)n * 8);
+    memcpy(h + n, zb, (size_t)n * 8);
+    memcpy(h + 2 * n, qn, (size_t)n * 8);
+    s->hyd[code - 1] = h;
+    s->hyd_n[code - 1] = n;
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* boundary_conditions.c:10-83 (+ tide and hydrograph extensions)           */
 /* ------------------------------------------------------------------------ */
 void orc_boundary_conditions(orc_state *s, double t)
 {
@@ -922,6 +962,15 @@ void orc_boundary_conditions(orc_state *s, double t)
     double tide = 0;
     if (s->p.tide_code)
         tide = s->p.tide_datum + s->p.tide_amp * orc_sin(6.283185307179586 * t / s->p.tide_period + s->p.tide_phase);
+    double hz[3], hq[3];
+    for (int c = 0; c < 3; ++c) {
+        const int n = s->hyd_n[c];
+        hz[c] = hq[c] = 0.0;
+        if (n) {
+            hz[c] = orc_hydro_eval(n, s->hyd[c], s->hyd[c] + n, t);
+            hq[c] = orc_hydro_eval(n, s->hyd[c], s->hyd[c] + 2 * n, t);
+        }
+    }
     for (int e = 0; e < m->num_edges; ++e) {
         double qn = 0, zb = 0;
         int bp = m->bdry[e];
@@ -929,6 +978,7 @@ void orc_boundary_conditions(orc_state *s, double t)
         if (bp == 2) { zb = 0.296; qn = 1000000; }
         if (bp == 3) { zb = -10000; qn = -0.1275 / 0.914; }
         if (s->p.tide_code && bp == s->p.tide_code) { zb = tide; qn = 1000000; }
+        if (bp >= 1 && bp <= 3 && s->hyd_n[bp - 1]) { zb = hz[bp - 1]; qn = hq[bp - 1]; }
         s->bzeta[e] = zb;
         s->bqn[e] = qn;
     }

@@ -84,6 +84,12 @@ void orc_slope_limiter(orc_state *s, unsigned char *modify_out);
 void orc_wetdry_status(orc_state *s);
 void orc_pdop(orc_state *s);
 void orc_boundary_conditions(orc_state *s, double t);
+/* Hydrograph extension: boundary code 1..3 takes zeta_b(t), qn_b(t),
+ * piecewise linear through n points (t strictly increasing), held at the end
+ * values outside them, evaluated at each step's start time.  n = 0 restores
+ * the code's constants.  Not allowed on the tide code.  0 or -1 (bad input). */
+int orc_set_hydrograph(orc_state *s, int code, int n, const double *t, const double *zb, const double *qn);
+double orc_hydro_eval(int n, const double *t, const double *v, double time);
 int orc_one_time_step(orc_state *s, double t, double dt);
 
 /* Step loop of time_evolution.c:311-402 for nsteps steps.  *t and *dt carry

@@ -213,6 +213,49 @@ def test_prescribed_boundaries_trajectory(gpu):
     _compare_run(s, o, 40, 10)
 
 
+def _set_hydrographs(obj, **kw):
+    """Hydrographs over a ~30 s run: code 1 an inflow discharge with a
+    falling limb (zeta_b at the "use interior" sentinel), code 3 a stage
+    series that crosses into the sentinel range (zeta_b < 0) and back, code 2
+    a falling stage with its discharge held at the qn > 1000 sentinel.  (A
+    ramp from a sentinel to a real discharge would pass through huge
+    discharges, so sentinels are held over whole segments.)"""
+    t = [0.0, 8.0, 19.0, 54.0]
+    obj.set_hydrograph(1, t, [-1000.0] * 4, [-0.02, -0.08, -0.05, -0.01], **kw)
+    obj.set_hydrograph(3, t, [0.1, 0.25, -1.0, 0.05], [0.03, 0.05, 0.02, -0.03], **kw)
+    obj.set_hydrograph(2, t, [0.3, 0.2, 0.05, 0.1], [2000.0] * 4, **kw)
+
+
+@pytest.mark.parametrize("wetdry", [0, 1])
+def test_hydrograph_trajectory(gpu, wetdry):
+    """Time-dependent boundary data (hydrographs on codes 1 and 3) evaluated
+    at each step's start, bit for bit against the oracle."""
+    mesh = perturbed_mesh(N=10, seed=4)
+    s, o = _pair(mesh, wetdry, cfl=0.3)
+    _set_hydrographs(s)
+    _set_hydrographs(o)
+    z, qx, qy = random_state(mesh, seed=9, dz=0.05, q=0.05)
+    s.set_state(z, qx, qy)
+    o.set_state(z, qx, qy)
+    _compare_run(s, o, 40, 10)
+
+
+def test_hydrograph_rejected_by_adjoint(gpu):
+    mesh = perturbed_mesh(N=6, seed=4)
+    s = native.Solver(mesh, native.default_params(cfl=0.3))
+    _set_hydrographs(s)
+    z, qx, qy = random_state(mesh, seed=9, dz=0.05, q=0.05)
+    s.set_state(z, qx, qy)
+    gel = np.array([0, 5, 9], np.int32)
+    obs = np.zeros((1, 2, 3))
+    with pytest.raises(native.DgrError) as ei:
+        s.adjoint_gauges(4, 2, 2, gel, obs)
+    assert ei.value.code == native.DGR_ERR_UNSUPPORTED
+    for code in (1, 2, 3):
+        s.set_hydrograph(code, [], [], [])
+    s.adjoint_gauges(4, 2, 2, gel, obs)
+
+
 def test_dry_bed_dam_break_trajectory(gpu):
     wl = configs.dry_bed_dam_break(N=24, nsteps=120)
     s = native.Solver(wl.mesh, wl.solver_params())

@@ -247,6 +247,53 @@ def test_tide_sine_against_libm():
         assert abs(s - math.sin(x)) <= 2 * math.ulp(max(abs(math.sin(x)), 1e-300)) + 1e-300, x
 
 
+def test_hydrograph_interpolation_closed_forms():
+    """The hydrograph extension (no reference code): the knot values exactly,
+    the end values held outside, and straight lines between knots (against
+    numpy's interpolation to rounding, and exactly at segment midpoints of a
+    dyadic table)."""
+    t = np.array([0.0, 10.0, 25.0, 100.0, 1000.0])
+    v = np.array([1.0, 2.0, -1.0, 0.5, 0.5])
+    for ti, vi in zip(t, v):
+        assert oracle.hydro_eval(t, v, ti) == vi
+    assert oracle.hydro_eval(t, v, -3.0) == 1.0
+    assert oracle.hydro_eval(t, v, 1e9) == 0.5
+    rng = np.random.default_rng(3)
+    xs = rng.uniform(-10, 1100, 2000)
+    got = np.array([oracle.hydro_eval(t, v, x) for x in xs])
+    assert np.allclose(got, np.interp(xs, t, v), rtol=0, atol=1e-14)
+    td = np.array([0.0, 4.0, 8.0, 16.0])
+    vd = np.array([0.0, 1.0, 3.0, -5.0])
+    for a, b_, va, vb in zip(td[:-1], td[1:], vd[:-1], vd[1:]):
+        assert oracle.hydro_eval(td, vd, (a + b_) / 2) == (va + vb) / 2
+
+
+def test_constant_hydrograph_is_the_static_boundary():
+    """A hydrograph holding a code's own constants leaves a run bit-for-bit
+    unchanged; a varying one changes it."""
+    mesh = perturbed_mesh(N=10, seed=4)
+    z, qx, qy = random_state(mesh, seed=9, dz=0.05, q=0.05)
+    runs = []
+    for kind in ("none", "constant", "varying"):
+        o = oracle.Oracle(mesh, math=oracle.MATH_DEV, wetdry=0, cfl=0.3)
+        t = [0.0, 5.0, 20.0]
+        if kind == "constant":
+            o.set_hydrograph(2, t, [0.296] * 3, [1000000.0] * 3)
+            o.set_hydrograph(3, t, [-10000.0] * 3, [-0.1275 / 0.914] * 3)
+        elif kind == "varying":
+            o.set_hydrograph(3, t, [0.1, 0.2, 0.05], [-0.1, 0.05, -0.02])
+        o.set_state(z, qx, qy)
+        o.run(30)
+        runs.append(np.concatenate(o.get_state()))
+    assert np.array_equal(runs[0], runs[1])
+    assert not np.array_equal(runs[0], runs[2])
+    o = oracle.Oracle(mesh, math=oracle.MATH_DEV, wetdry=0, tide_code=2)
+    with pytest.raises(ValueError):
+        o.set_hydrograph(2, [0.0, 1.0], [0.1, 0.2], [0.0, 0.0])     # the tide code
+    with pytest.raises(ValueError):
+        o.set_hydrograph(1, [0.0, 0.0], [0.1, 0.2], [0.0, 0.0])     # not increasing
+
+
 def test_adjoint_pieces_against_finite_differences(tmp_path):
     """The reverse-mode pieces of the gradient (Roe flux, element RHS,
     limiter; dg-rain_amd/csrc/dgr_adjoint.h) against central differences of
