@@ -105,10 +105,13 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             if (e < 0 || e >= h->mem[mi].ne) return DGR_ERR_ARG;
         }
     if (h->profiling) return DGR_ERR_ARG;
-    // the reverse has no adjoint of a hydrograph's time dependence
-    for (int mi = 0; mi < h->nmem; ++mi)
+    // the reverse has no adjoint of a hydrograph's time dependence, nor of
+    // the infiltration rate's dependence on t and dt
+    for (int mi = 0; mi < h->nmem; ++mi) {
         for (int c = 0; c < 3; ++c)
             if (!h->mem[mi].hyd[c].empty()) return DGR_ERR_UNSUPPORTED;
+        if (h->mem[mi].infil_K > 0.0) return DGR_ERR_UNSUPPORTED;
+    }
     const int nseg = (N + K - 1) / K, nsamp = N / sp->sample_every;
     const size_t P = h->pitch, W9 = 9 * P, nn = h->nn, per = (size_t)nm * ng;
     const int maxparts = h->ph.maxparts;

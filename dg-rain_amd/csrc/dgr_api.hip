@@ -471,6 +471,37 @@ extern "C" dgr_status dgr_set_hydrograph(dgr_handle *h, int code, int npts, cons
     return dgr_member_set_hydrograph(h, 0, code, npts, t, zeta_b, qn_b);
 }
 
+extern "C" dgr_status dgr_member_set_infiltration(dgr_handle *h, int mi, double K, double psi, double theta_e,
+                                                  double s_e)
+{
+    if (bad_member(h, mi)) return DGR_ERR_ARG;
+    double kk = 0.0, pp = 0.0;
+    if (K > 0.0) {
+        if (!(psi > 0.0) || !(theta_e > 0.0) || theta_e > 1.0 || s_e < 0.0 || !(s_e < 1.0) || K > 1e300 ||
+            psi > 1e300)
+            return DGR_ERR_ARG;
+        kk = K;
+        pp = psi * ((1 - s_e) * theta_e);   // KinField/Infiltration.c:16,49
+    }
+    MemberHost &M = h->mem[mi];
+    M.infil_K = kk;
+    M.infil_P = pp;
+    // the members' device records are read at every step; no graph holds them
+    MemberDev md;
+    DGR_HIP(hipMemcpyAsync(&md, h->d_mem + mi, sizeof(MemberDev), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    md.infil_K = kk;
+    md.infil_P = pp;
+    DGR_HIP(hipMemcpyAsync(h->d_mem + mi, &md, sizeof(MemberDev), hipMemcpyHostToDevice, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_set_infiltration(dgr_handle *h, double K, double psi, double theta_e, double s_e)
+{
+    return dgr_member_set_infiltration(h, 0, K, psi, theta_e, s_e);
+}
+
 extern "C" dgr_status dgr_member_set_boundary(dgr_handle *h, int mi, const double *bz, const double *bq)
 {
     if (bad_member(h, mi) || !bz || !bq) return DGR_ERR_ARG;
@@ -894,12 +925,12 @@ extern "C" dgr_status dgr_math_selftest(int n, const double *x, const double *y,
     double *dx = nullptr, *dy = nullptr, *dout = nullptr;
     DGR_HIP(dalloc(&dx, n));
     DGR_HIP(dalloc(&dy, n));
-    DGR_HIP(dalloc(&dout, 7 * (size_t)n));
+    DGR_HIP(dalloc(&dout, 9 * (size_t)n));
     DGR_HIP(hipMemcpy(dx, x, n * sizeof(double), hipMemcpyHostToDevice));
     DGR_HIP(hipMemcpy(dy, y, n * sizeof(double), hipMemcpyHostToDevice));
     launch_math_selftest(n, dx, dy, dout, 0);
     DGR_HIP(hipGetLastError());
-    DGR_HIP(hipMemcpy(out, dout, 7 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
+    DGR_HIP(hipMemcpy(out, dout, 9 * (size_t)n * sizeof(double), hipMemcpyDeviceToHost));
     (void)hipFree(dx);
     (void)hipFree(dy);
     (void)hipFree(dout);

@@ -93,6 +93,7 @@ struct MemberDev {
     int pad;
     double rain, cfl, min_edge, final_time;
     double tide_datum, tide_amp, tide_period, tide_phase;
+    double infil_K, infil_P;  // Green-Ampt conductivity and psi * delta_theta (K = 0: off)
 };
 
 struct MeshDev {
@@ -117,7 +118,7 @@ struct Ctrl {
     double t;                 // time at the start of the next step
     double dt;                // dt of the next step
     double tide;              // tide elevation for the current step
-    double pad0;
+    double infil;             // Green-Ampt mean infiltration rate over the current step
     unsigned long long maxlam[2];   // stage max |lambda| as raw bits (values >= 0)
     // earliest error in the reference's execution order (atomic min):
     // step << 34 | phase << 32 | id << 3 | code, phase = 2*stage + (0 flux, 1 PDop);

@@ -63,6 +63,7 @@ struct MemberHost {
     dgr_params prm;
     double min_edge;
     std::vector<double> hyd[3];   // hydrograph of code c+1: t, zeta_b, qn_b (3n values)
+    double infil_K = 0.0, infil_P = 0.0;   // Green-Ampt K and psi * delta_theta (K = 0: off)
 };
 
 struct dgr_handle {

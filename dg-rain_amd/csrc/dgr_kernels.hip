@@ -296,7 +296,7 @@ struct GeoRegs {
 // ---------------------------------------------------------------------------
 template <typename Geo, class DV>
 __device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], const double Fh[9], int wd_self,
-                                            double g, double rain, double R[9], DV &D)
+                                            double g, double rain, double infil, double R[9], DV &D)
 {
     const double nx1 = G(GX_NX), nx2 = G(GX_NX + 1), nx3 = G(GX_NX + 2);
     const double ny1 = G(GX_NY), ny2 = G(GX_NY + 1), ny3 = G(GX_NY + 2);
@@ -397,6 +397,12 @@ __device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], con
         R[0] = R[0] + rain;
         R[1] = R[1] + rain;
         R[2] = R[2] + rain;
+    }
+    // infiltration leaves elements not flagged dry
+    if (infil != 0.0 && wd_self != 0) {
+        R[0] = R[0] - infil;
+        R[1] = R[1] - infil;
+        R[2] = R[2] - infil;
     }
 }
 
@@ -712,7 +718,7 @@ __global__ __launch_bounds__(256) void k_stage(MeshDev m, PhysDev ph, const doub
             if (ecode) raise_error(ctrl, ecode, eedge, 2 * slot);
             double R[9];
             DivExact D;
-            element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R, D);
+            element_rhs(G, W, Fh, wd_self, ph.g, M.rain, ctrl->infil, R, D);
             if (MODE == 0) {
 #pragma unroll
                 for (int f = 0; f < 9; ++f) { dbg_rhs[f * P + e] = R[f]; dbg_fhat[f * P + e] = Fh[f]; }
@@ -942,7 +948,7 @@ __global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const dou
         DivFast D;
         GeoRegs G;
         geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
-        element_rhs(G, W, Fh, wd_self, ph.g, M.rain, R, D);
+        element_rhs(G, W, Fh, wd_self, ph.g, M.rain, ctrl->infil, R, D);
         if (!D.ok()) {
             // an operand outside the fast division's range: plain divisions.
             // The inputs are loaded again through pointers the compiler
@@ -957,7 +963,7 @@ __global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const dou
             DivExact X;
             GeoRegs G2;
             geometry_of(m2, e, ph.g, [&](int i, double v) { G2.v[i] = v; }, X);
-            element_rhs(G2, W2, Fh2, wd_self, ph.g, M.rain, R, X);
+            element_rhs(G2, W2, Fh2, wd_self, ph.g, M.rain, ctrl->infil, R, X);
         }
     }
     if (MODE == 0) {
@@ -1315,6 +1321,10 @@ __device__ __forceinline__ void step_tide(const MeshDev &m, const MemberDev &M, 
             c->hyd_q[k] = hydro_eval(n, h, h + 2 * n, c->t);
         }
     }
+    // the step's mean infiltration rate (F(t + dt) - F(t)) / dt
+    c->infil = (M.infil_K > 0.0 && c->dt > 0.0) ? (ga_cumulative(M.infil_K, M.infil_P, c->t + c->dt) -
+                                                   ga_cumulative(M.infil_K, M.infil_P, c->t)) / c->dt
+                                                : 0.0;
 }
 
 __global__ __launch_bounds__(64) void k_step_begin(MeshDev m, Ctrl *ctrls)
@@ -1461,16 +1471,17 @@ __global__ __launch_bounds__(64) void k_reset_maxlam(MeshDev m, Ctrl *ctrls)
     if (mid < m.nmem) clear_fold(ctrls + mid);
 }
 
+constexpr int NSELF = 9;   // outputs per input pair
 __global__ void k_math_selftest(int n, const double *x, const double *y, double *out)
 {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
     const double a = x[i], b = y[i];
-    out[7 * i + 0] = sqrt(a);
-    out[7 * i + 1] = a / b;
-    out[7 * i + 2] = dgr_pow43(a);
-    out[7 * i + 3] = dgr_sin(a);
-    out[7 * i + 4] = fma(a, b, a);
+    out[NSELF * i + 0] = sqrt(a);
+    out[NSELF * i + 1] = a / b;
+    out[NSELF * i + 2] = dgr_pow43(a);
+    out[NSELF * i + 3] = dgr_sin(a);
+    out[NSELF * i + 4] = fma(a, b, a);
     DivFast D;
     const double q1 = D.q(a, D.den(b));
     const bool ok1 = D.ok();
@@ -1478,8 +1489,10 @@ __global__ void k_math_selftest(int n, const double *x, const double *y, double 
     const double q3 = D3.q(a, D3.den_const(3.0));
     const bool ok3 = D3.ok();
     // the fast quotient where its range check passes, else plain division
-    out[7 * i + 5] = ok1 ? q1 : a / b;
-    out[7 * i + 6] = ok3 ? q3 : a / 3.0;
+    out[NSELF * i + 5] = ok1 ? q1 : a / b;
+    out[NSELF * i + 6] = ok3 ? q3 : a / 3.0;
+    out[NSELF * i + 7] = dgr_log1p(fabs(a));
+    out[NSELF * i + 8] = ga_cumulative(1e-6 + 1e-5 * fabs(b), 0.033, fabs(a));
 }
 
 // ---------------------------------------------------------------------------

@@ -10,6 +10,9 @@
 //                one is accurate to ~1e-22 before the final rounding, i.e.
 //                correctly rounded except in vanishingly rare cases.
 //  dgr_sin(x)    sine for the tidal boundary forcing (no reference code).
+//  dgr_log(x), dgr_log1p(x)   natural log of x >= 1 and of 1 + x, x >= 0, for
+//                the Green-Ampt infiltration depth.
+//  ga_cumulative Green-Ampt cumulative infiltration depth F(t).
 //  DivExact / DivFast / DivRecip   division policies (below); DivFast gives
 //                the bits of x / y with fewer instructions, DivRecip does not.
 //
@@ -211,6 +214,67 @@ __device__ inline double dgr_sin(double x)
     case 2: return -sv;
     default: return -cv;
     }
+}
+
+// ln x for finite x >= 1: x = m 2^e with m in [sqrt(1/2), sqrt(2)),
+// ln m = 2 atanh(s), s = (m-1)/(m+1), by the odd series through s^21
+// (truncation below 1e-18 relative), ln 2 split so that e * LN2_HI is exact.
+// The oracle's orc_log is the same operations in the same order.
+__device__ inline double dgr_log(double x)
+{
+    const uint64_t b = (uint64_t)__double_as_longlong(x);
+    int e = (int)((b >> 52) & 0x7ff) - 1023;
+    double m = __longlong_as_double((long long)((b & 0x000fffffffffffffULL) | 0x3ff0000000000000ULL));
+    if (m > 1.4142135623730951) { m = m * 0.5; e += 1; }
+    const double s = (m - 1.0) / (m + 1.0);
+    const double s2 = s * s;
+    double p = 1.0 / 21;
+    p = p * s2 + 1.0 / 19;
+    p = p * s2 + 1.0 / 17;
+    p = p * s2 + 1.0 / 15;
+    p = p * s2 + 1.0 / 13;
+    p = p * s2 + 1.0 / 11;
+    p = p * s2 + 1.0 / 9;
+    p = p * s2 + 1.0 / 7;
+    p = p * s2 + 1.0 / 5;
+    p = p * s2 + 1.0 / 3;
+    p = p * s2 + 1.0;
+    const double lnm = 2.0 * s * p;
+    return (double)e * 6.9314718036912382e-01 + ((double)e * 1.9082149292705877e-10 + lnm);
+}
+
+// ln(1 + x) for x >= 0 by Goldberg's correction (What Every Computer
+// Scientist Should Know About Floating-Point Arithmetic, Theorem 4):
+// u = 1 + x, ln(u) x / (u - 1), which keeps ln's relative accuracy where
+// 1 + x rounds.  The oracle's orc_log1p is the same.
+__device__ inline double dgr_log1p(double x)
+{
+    const double u = 1.0 + x;
+    if (u == 1.0) return x;
+    return dgr_log(u) * (x / (u - 1.0));
+}
+
+// Green-Ampt infiltration under ponding from t = 0 (the reference's kinematic
+// model, KinField/Infiltration.c:8-87): the cumulative depth F solves
+//   F - K t - P ln(1 + F/P) = 0,   P = psi * delta_theta.
+// Newton from max(sqrt(2 P K t), K t) (below the root; the first step lands
+// above it and the rest descend) until an iterate repeats, at most 40
+// iterations; the stop depends only on the bits, so the oracle's
+// orc_ga_cumulative, the same operations, stops at the same iterate.
+__device__ inline double ga_cumulative(double K, double P, double t)
+{
+    if (!(t > 0)) return 0.0;
+    const double Kt = K * t;
+    double F = sqrt(2.0 * P * Kt);
+    if (Kt > F) F = Kt;
+    for (int i = 0; i < 40; ++i) {
+        const double g = F - Kt - P * dgr_log1p(F / P);
+        const double dg = F / (P + F);
+        const double Fn = F - g / dg;
+        if (Fn == F) break;
+        F = Fn;
+    }
+    return F;
 }
 
 }  // namespace dgr

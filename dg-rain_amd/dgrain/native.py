@@ -16,6 +16,7 @@ LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdgrain.so")
 NUM_VARIANTS = 3   # DGR_NUM_VARIANTS in include/dgrain.h
 
 DGR_OK = 0
+DGR_ERR_ARG = 1
 DGR_ERR_UNSUPPORTED = 7
 STATUS = {0: "ok", 1: "invalid argument", 2: "inconsistent mesh", 3: "HIP runtime error",
           4: "NaN flux or wet/dry failure", 5: "out of memory", 6: "no HIP device", 7: "unsupported"}
@@ -106,6 +107,8 @@ def load():
         "dgr_member_set_boundary": (C.c_int, [vp, C.c_int, dp, dp]),
         "dgr_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
         "dgr_member_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
+        "dgr_set_infiltration": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "dgr_member_set_infiltration": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
         "dgr_member_set_time": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
         "dgr_member_report": (C.c_int, [vp, C.c_int, C.POINTER(Report)]),
         "dgr_member_layout": (C.c_int, [vp, C.c_int, ip, ip]),
@@ -284,6 +287,11 @@ class _Handle:
         check(self.lib.dgr_member_set_hydrograph(self.h, int(member), int(code), t.size, _dp(t), _dp(z), _dp(q)),
               "dgr_member_set_hydrograph")
 
+    def set_infiltration(self, K, psi=0.0, theta_e=0.0, s_e=0.0, member=0):
+        """Green-Ampt infiltration of a member (K <= 0 turns it off)."""
+        check(self.lib.dgr_member_set_infiltration(self.h, int(member), float(K), float(psi), float(theta_e),
+                                                   float(s_e)), "dgr_member_set_infiltration")
+
     def checkpoint_write(self, path, member=0):
         """Restart file of a member: state, wet/dry flags, t and dt."""
         check(self.lib.dgr_checkpoint_write(self.h, int(member), os.fsencode(path)), "dgr_checkpoint_write")
@@ -451,7 +459,7 @@ def math_selftest(x, y):
     lib = load()
     x = np.ascontiguousarray(x, dtype=np.float64)
     y = np.ascontiguousarray(y, dtype=np.float64)
-    out = np.empty((x.size, 7))
+    out = np.empty((x.size, 9))
     check(lib.dgr_math_selftest(x.size, _dp(x), _dp(y), _dp(out)), "dgr_math_selftest")
     return out
 

@@ -134,6 +134,18 @@ dgr_status dgr_set_time(dgr_handle *h, double t, double dt);
  * DGR_ERR_UNSUPPORTED while any hydrograph is set. */
 dgr_status dgr_set_hydrograph(dgr_handle *h, int code, int npts, const double *t, const double *zeta_b,
                               const double *qn_b);
+/* Green-Ampt infiltration (extension; the reference's kinematic model,
+ * KinField/Infiltration.c:8-87 and its use in KinField/computeL.c:546-555,
+ * which 2DCode does not have): conductivity K (m/s), suction head psi (m),
+ * effective porosity theta_e and effective saturation s_e give
+ * P = psi ((1 - s_e) theta_e), and the cumulative depth F(t) solves
+ * F - K t - P ln(1 + F/P) = 0 (ponding from t = 0).  dgr_run subtracts the
+ * step's mean rate (F(t + dt) - F(t)) / dt from every zeta DOF of the RHS of
+ * each element not flagged dry (after params.rain).  K <= 0 turns it off;
+ * otherwise psi > 0, 0 < theta_e <= 1 and 0 <= s_e < 1 or DGR_ERR_ARG.  The
+ * rate is updated by dgr_run only.  dgr_adjoint_gauges returns
+ * DGR_ERR_UNSUPPORTED while it is on. */
+dgr_status dgr_set_infiltration(dgr_handle *h, double K, double psi, double theta_e, double s_e);
 
 /* compute2DL: RHS (and optionally the edge fluxes Fhat{1,2,3}dotn) of the
  * current state, reference layout; max_lambda may be NULL. */
@@ -203,6 +215,8 @@ dgr_status dgr_member_set_boundary(dgr_handle *h, int member, const double *bzet
 dgr_status dgr_member_set_time(dgr_handle *h, int member, double t, double dt);
 dgr_status dgr_member_set_hydrograph(dgr_handle *h, int member, int code, int npts, const double *t,
                                      const double *zeta_b, const double *qn_b);
+dgr_status dgr_member_set_infiltration(dgr_handle *h, int member, double K, double psi, double theta_e,
+                                       double s_e);
 dgr_status dgr_member_report(dgr_handle *h, int member, dgr_report *report);
 /* element range of a member inside the device planes of dgr_device_state */
 dgr_status dgr_member_layout(dgr_handle *h, int member, int *el_offset, int *num_el);
@@ -252,7 +266,8 @@ dgr_status dgr_device_state(dgr_handle *h, double **planes, int *pitch);
 
 /* Self-test of the device math against host-supplied inputs: out[i] =
  * {sqrt(x), x/y, dgr_pow43(x), dgr_sin(x), fma(x,y,x), x/y and x/3 by the
- * shared-denominator division} for each pair (7 doubles). */
+ * shared-denominator division, ln(1 + |x|), the Green-Ampt depth F(t = |x|)
+ * for K = 1e-6 + 1e-5 |y|, P = 0.033} for each pair (9 doubles). */
 dgr_status dgr_math_selftest(int n, const double *x, const double *y, double *out);
 
 #ifdef __cplusplus

@@ -92,6 +92,10 @@ def _load_oracle():
             "orc_boundary_conditions": (None, [vp, C.c_double]),
             "orc_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
             "orc_hydro_eval": (C.c_double, [C.c_int, dp, dp, C.c_double]),
+            "orc_set_infiltration": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+            "orc_ga_cumulative": (C.c_double, [C.c_double, C.c_double, C.c_double]),
+            "orc_log": (C.c_double, [C.c_double]),
+            "orc_log1p": (C.c_double, [C.c_double]),
             "orc_one_time_step": (C.c_int, [vp, C.c_double, C.c_double]),
             "orc_run": (C.c_int, [vp, C.c_int, dp, dp]),
             "orc_pow43": (C.c_double, [C.c_double]),
@@ -111,6 +115,18 @@ def hydro_eval(t, v, time):
     t = np.ascontiguousarray(t, np.float64)
     v = np.ascontiguousarray(v, np.float64)
     return _load_oracle().orc_hydro_eval(t.size, _dp(t), _dp(v), float(time))
+
+
+def ga_cumulative(K, P, t):
+    return _load_oracle().orc_ga_cumulative(float(K), float(P), float(t))
+
+
+def dlog(x):
+    return _load_oracle().orc_log(float(x))
+
+
+def dlog1p(x):
+    return _load_oracle().orc_log1p(float(x))
 
 
 def pow43(x):
@@ -219,6 +235,10 @@ class Oracle:
 
     def boundary_conditions(self, t):
         self.lib.orc_boundary_conditions(self.h, float(t))
+
+    def set_infiltration(self, K, psi, theta_e, s_e=0.0):
+        if self.lib.orc_set_infiltration(self.h, float(K), float(psi), float(theta_e), float(s_e)):
+            raise ValueError("bad infiltration parameters")
 
     def set_hydrograph(self, code, t, zeta, qn):
         t, z, q = (np.ascontiguousarray(a, np.float64) for a in (t, zeta, qn))

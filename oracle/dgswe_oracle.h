@@ -90,6 +90,14 @@ void orc_boundary_conditions(orc_state *s, double t);
  * the code's constants.  Not allowed on the tide code.  0 or -1 (bad input). */
 int orc_set_hydrograph(orc_state *s, int code, int n, const double *t, const double *zb, const double *qn);
 double orc_hydro_eval(int n, const double *t, const double *v, double time);
+/* Green-Ampt infiltration extension (the reference's kinematic model,
+ * KinField/Infiltration.c): cumulative depth F(t) of ponded infiltration;
+ * orc_run takes the step's mean rate (F(t+dt) - F(t))/dt off the RHS of zeta
+ * of every element not flagged dry.  K <= 0 turns it off.  0 or -1. */
+int orc_set_infiltration(orc_state *s, double K, double psi, double theta_e, double s_e);
+double orc_ga_cumulative(double K, double P, double t);
+double orc_log(double x);
+double orc_log1p(double x);
 int orc_one_time_step(orc_state *s, double t, double dt);
 
 /* Step loop of time_evolution.c:311-402 for nsteps steps.  *t and *dt carry

@@ -77,6 +77,34 @@ def test_ensemble_member_hydrographs(gpu):
         s.close()
 
 
+def test_ensemble_member_infiltration(gpu):
+    """Per-member Green-Ampt parameters (none on member 1): each member bit
+    for bit its solo run under graph replay."""
+    from helpers import perturbed_mesh, random_state
+    meshes = [perturbed_mesh(N=n, seed=sd) for n, sd in ((6, 1), (9, 2), (7, 3))]
+    params = [native.default_params(cfl=0.3, rain=10e-3 / 3600) for _ in meshes]
+    ga = {0: (1e-5, 0.11, 0.43, 0.25), 2: (4e-6, 0.2, 0.35, 0.1)}
+    states = [random_state(m, seed=20 + i, dz=0.05, q=0.05) for i, m in enumerate(meshes)]
+    ens = native.Ensemble(meshes, params)
+    for i, g in ga.items():
+        ens.set_infiltration(*g, member=i)
+    for i, st in enumerate(states):
+        ens.set_state(i, *st)
+    ens.set_graph(True)
+    reps = ens.run(12)
+    reps = ens.run(13)
+    for i, m in enumerate(meshes):
+        s = native.Solver(m, params[i])
+        if i in ga:
+            s.set_infiltration(*ga[i])
+        s.set_state(*states[i])
+        r = s.run(25)
+        for a, b in zip(ens.get_state(i), s.get_state()):
+            assert same(a, b), "member %d differs from its solo run" % i
+        assert reps[i].t == r.t and reps[i].dt == r.dt, i
+        s.close()
+
+
 def test_ensemble_matches_oracle(gpu):
     wls = _members()[:3]
     ens = native.Ensemble([w.mesh for w in wls], [w.solver_params() for w in wls])

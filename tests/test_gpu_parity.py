@@ -75,6 +75,19 @@ def test_shared_denominator_division(gpu):
     assert _bits_equal(out[:, 6], q3)
 
 
+def test_green_ampt_math_matches_oracle(gpu):
+    """The device's ln(1 + x) and Green-Ampt depth F(t) against the oracle's,
+    bit for bit, over the time range a run meets (and t = 0, denormals)."""
+    rng = np.random.default_rng(12)
+    x = np.concatenate([np.exp(rng.uniform(np.log(1e-9), np.log(1e8), 20000)), [0.0, 5e-324, 1e-310, 1.0, 2.0]])
+    y = rng.uniform(-3, 3, x.size)
+    out = native.math_selftest(x, y)
+    l1p = np.array([oracle.dlog1p(abs(v)) for v in x])
+    assert _bits_equal(out[:, 7], l1p)
+    F = np.array([oracle.ga_cumulative(1e-6 + 1e-5 * abs(b), 0.033, abs(a)) for a, b in zip(x, y)])
+    assert _bits_equal(out[:, 8], F)
+
+
 @pytest.mark.parametrize("wetdry", [0, 1])
 @pytest.mark.parametrize("seed", [1, 2])
 def test_compute2DL(gpu, wetdry, seed):
@@ -254,6 +267,94 @@ def test_hydrograph_rejected_by_adjoint(gpu):
     for code in (1, 2, 3):
         s.set_hydrograph(code, [], [], [])
     s.adjoint_gauges(4, 2, 2, gel, obs)
+
+
+GA = (0.11, 0.43, 0.25)    # suction head psi (m), effective porosity, effective saturation
+
+
+def test_infiltration_trajectory(gpu):
+    """Rain and Green-Ampt infiltration (the step's mean rate computed on the
+    device at each step's start) on a wet domain, bit for bit against the
+    oracle, with and without the wet/dry scheme."""
+    mesh = perturbed_mesh(N=10, seed=4)
+    z, qx, qy = random_state(mesh, seed=9, dz=0.05, q=0.05)
+    for wetdry in (0, 1):
+        s, o = _pair(mesh, wetdry, cfl=0.3, rain=20e-3 / 3600)
+        s.set_infiltration(1e-5, *GA)
+        o.set_infiltration(1e-5, *GA)
+        s.set_state(z, qx, qy)
+        o.set_state(z, qx, qy)
+        _compare_run(s, o, 40, 20)
+        s.close()
+
+
+def test_infiltration_dries_elements(gpu):
+    """Infiltration on the dry-bed dam break: about a third of the elements
+    drain dry within 120 steps (none do without it); flags, limiter
+    activations and states bit for bit against the oracle."""
+    wl = configs.dry_bed_dam_break(N=24, nsteps=120)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_infiltration(1e-6, *GA)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl)
+    o.set_infiltration(1e-6, *GA)
+    o.set_state(wl.zeta, wl.qx, wl.qy)
+    _compare_run(s, o, 120, 40)
+    assert 100 < int(np.sum(s.get_wd() == 0)) < 1000
+
+
+def test_infiltration_failure_matches_oracle(gpu):
+    """Infiltration faster than the thin films hold: the oracle stops with a
+    wet/dry failure; the device reports the same error code, element and step."""
+    wl = configs.dry_bed_dam_break(N=24, nsteps=120)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_infiltration(1e-4, *GA)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl)
+    o.set_infiltration(1e-4, *GA)
+    o.set_state(wl.zeta, wl.qx, wl.qy)
+    for step in range(1, 120):
+        try:
+            o.run(1)
+        except RuntimeError as ex:
+            msg = str(ex)
+            break
+        s.run(1)
+        for a, b in zip(s.get_state(), o.get_state()):
+            assert same(a, b), "state differs after %d steps" % step
+    else:
+        pytest.fail("expected the oracle to stop on this input")
+    with pytest.raises(native.DgrError) as ei:
+        s.run(1)
+    code, where = msg.split("error ")[1].split(" at ")
+    assert "error %s at %s" % (code, where) in str(ei.value), (msg, str(ei.value))
+
+
+def test_infiltration_arguments_and_adjoint(gpu):
+    mesh = perturbed_mesh(N=6, seed=4)
+    s = native.Solver(mesh, native.default_params(cfl=0.3))
+    for bad in ((1e-5, 0.0, 0.4, 0.0), (1e-5, 0.1, 1.5, 0.0), (1e-5, 0.1, 0.4, 1.0), (1e-5, 0.1, 0.4, -0.1)):
+        with pytest.raises(native.DgrError) as ei:
+            s.set_infiltration(*bad)
+        assert ei.value.code == native.DGR_ERR_ARG
+    z, qx, qy = random_state(mesh, seed=9, dz=0.05, q=0.05)
+    s.set_state(z, qx, qy)
+    s.set_infiltration(1e-5, *GA)
+    gel = np.array([0, 5, 9], np.int32)
+    obs = np.zeros((1, 2, 3))
+    with pytest.raises(native.DgrError) as ei:
+        s.adjoint_gauges(4, 2, 2, gel, obs)
+    assert ei.value.code == native.DGR_ERR_UNSUPPORTED
+    s.set_infiltration(0.0)
+    s.adjoint_gauges(4, 2, 2, gel, obs)
+    # off again: the run is the plain one
+    s2 = native.Solver(mesh, native.default_params(cfl=0.3))
+    for h in (s, s2):
+        h.set_state(z, qx, qy)
+        h.set_time(0.0, native.default_params().dt0)
+        h.run(10)
+    for a, b in zip(s.get_state(), s2.get_state()):
+        assert same(a, b)
 
 
 def test_dry_bed_dam_break_trajectory(gpu):

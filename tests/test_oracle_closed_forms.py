@@ -12,6 +12,10 @@ a swapped index or operand) fails it:
   * Limiter: element means preserved, vertex values inside node bounds.
   * PDop: vertex depths >= H0 afterwards.
   * pow43 against exact decimal arithmetic; the tide sine against libm.
+  * Green-Ampt depth: the implicit equation's residual (with libm's log1p),
+    its time derivative against the reference's rate K (P/F + 1), early-time
+    sqrt(2 P K t); infiltration over a closed basin lowers the surface by
+    exactly F(t) (rain minus infiltration: R t - F(t)).
 """
 import math
 import os
@@ -165,6 +169,66 @@ def test_rain_fills_closed_basin_at_rate_R():
     z, qx, qy = o.get_state()
     assert np.max(np.abs(z - (wl.zeta + R * t))) <= 1e-12 * (1 + R * t)
     assert np.max(np.abs(qx)) < 1e-12 and np.max(np.abs(qy)) < 1e-12
+
+
+def test_log_against_libm():
+    rng = np.random.default_rng(15)
+    xs = np.concatenate([1.0 + rng.uniform(0, 1, 3000), np.exp(rng.uniform(0, 40, 3000)),
+                         [1.0, 2.0, math.sqrt(2.0), 1.4142135623730951, 1.4142135623730954, 1e300]])
+    for x in xs:
+        ref = math.log(x)
+        assert abs(oracle.dlog(x) - ref) <= 2 * math.ulp(max(ref, 1e-300)), x
+    assert oracle.dlog(1.0) == 0.0
+    for x in np.concatenate([np.geomspace(1e-300, 1e3, 3000), [0.0, 1e-16, 2.2e-16, 1.0]]):
+        ref = math.log1p(x)
+        assert abs(oracle.dlog1p(x) - ref) <= 4 * math.ulp(max(ref, 1e-320)), x
+
+
+@pytest.mark.parametrize("K,P", [(1e-5, 0.11 * 0.3), (3.3e-7, 0.2 * 0.4), (1e-3, 0.05)])
+def test_green_ampt_depth_closed_forms(K, P):
+    """F(t) of KinField/Infiltration.c:8-87 (F - K t - P ln(1 + F/P) = 0)."""
+    assert oracle.ga_cumulative(K, P, 0.0) == 0.0
+    prev = 0.0
+    for t in np.geomspace(1e-3, 1e7, 60):
+        F = oracle.ga_cumulative(K, P, t)
+        assert F > prev
+        prev = F
+        # the equation itself, with libm's log1p
+        assert abs(F - K * t - P * math.log1p(F / P)) <= 4e-16 * (F + K * t)
+        # implicit differentiation: dF/dt (1 - P/(P+F)) = K, i.e. the
+        # reference's rate f = K (P/F + 1) (KinField/computeL.c:555)
+        d = 1e-4 * t
+        rate = (oracle.ga_cumulative(K, P, t + d) - oracle.ga_cumulative(K, P, t - d)) / (2 * d)
+        assert abs(rate / (K * (P / F + 1)) - 1) < 1e-6, t
+    # early time: F ~ sqrt(2 P K t) (sorptivity), with relative correction O(sqrt(K t / P))
+    t = 1e-6 * P / K
+    F = oracle.ga_cumulative(K, P, t)
+    assert abs(F / math.sqrt(2 * P * K * t) - 1) < 3e-3
+
+
+def test_infiltration_drains_closed_basin():
+    """Rain R and Green-Ampt infiltration over a closed flat basin that stays
+    wet: the surface stays flat and moves by R t - F(t) (the per-step mean
+    rates telescope to F), velocities stay zero."""
+    wl = configs.lake_at_rest(N=6, bumps=False)
+    R = 50e-3 / 3600
+    K, psi, theta_e, s_e = 1e-5, 0.11, 0.43, 0.25
+    P = psi * ((1 - s_e) * theta_e)
+    for wetdry in (0, 1):
+        o = oracle.Oracle(wl.mesh, math=oracle.MATH_DEV, wetdry=wetdry, cfl=0.3, rain=R)
+        o.set_infiltration(K, psi, theta_e, s_e)
+        o.set_state(wl.zeta, wl.qx, wl.qy)
+        t, _ = o.run(40)
+        z, qx, qy = o.get_state()
+        F = oracle.ga_cumulative(K, P, t)
+        assert F > 1e-3
+        assert np.max(np.abs(z - (wl.zeta + R * t - F))) <= 1e-12 * (1 + F)
+        assert np.max(np.abs(qx)) < 1e-12 and np.max(np.abs(qy)) < 1e-12
+    o = oracle.Oracle(wl.mesh, math=oracle.MATH_DEV, wetdry=0)
+    for bad in ((1e-5, 0.0, 0.4, 0.0), (1e-5, 0.1, 1.5, 0.0), (1e-5, 0.1, 0.4, 1.0), (1e-5, 0.1, 0.4, -0.1)):
+        with pytest.raises(ValueError):
+            o.set_infiltration(*bad)
+    o.set_infiltration(0.0, 0.0, 0.0, 0.0)     # off
 
 
 def test_limiter_preserves_means_and_respects_node_bounds():
