@@ -384,121 +384,151 @@ DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double
     }
 }
 
+// forward decisions of one field's limiter pass (SlopeLimiter2D.c:166-303):
+// the clamp branch per vertex, the sign of each redistribution pass, which
+// vertices were positive in each pass and which took the uncapped
+// reduction (bit 3*pass + k), and the final vertex values
+struct LimFieldDec {
+    int clampsel[3];   // 0: V, 1: Vmax, 2: Vmin
+    int sd[3];         // per pass
+    unsigned pos, use0;
+    double Vfin[3];
+};
+
+DGR_HD void limit_field_forward(const double S[9], int iv, const double Vmin[3], const double Vmax[3],
+                                LimFieldDec &d)
+{
+    const double w1 = S[3 * iv], w2 = S[3 * iv + 1], w3 = S[3 * iv + 2];
+    double V[3] = {w1 - w2 + w3, w1 + w2 - w3, -w1 + w2 + w3};
+    const double avg = (w1 + w2 + w3) / 3;
+    for (int k = 0; k < 3; ++k) {
+        // at an exact tie the element's own value is taken as the active
+        // branch: the clip is then inactive, as it is in both directions
+        // of a central difference when several elements share the bound
+        // (lake at rest), and no adjoint is routed through a tied bound
+        const double lo = (V[k] < Vmax[k]) ? V[k] : Vmax[k];
+        int sel = (V[k] <= Vmax[k]) ? 0 : 1;
+        if (Vmin[k] > lo) sel = 2;
+        d.clampsel[k] = sel;
+        V[k] = fmax(lo, Vmin[k]);
+    }
+    d.pos = 0;
+    d.use0 = 0;
+    for (int ll = 0; ll < 3; ++ll) {
+        const double sumloc = (V[0] + V[1] + V[2]) / 3;
+        double sumdiff = (sumloc - avg) * 3;
+        const int s = sumdiff < 0 ? -1 : (sumdiff == 0 ? 0 : 1);
+        d.sd[ll] = s;
+        const double diff[3] = {(V[0] - avg) * s, (V[1] - avg) * s, (V[2] - avg) * s};
+        double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
+        for (int k = 0; k < 3; ++k) {
+            const double dv = fmax(1.0, kdp);
+            double redfac;
+            if (diff[k] > 0) {
+                d.pos |= 1u << (3 * ll + k);
+                redfac = sumdiff * s / dv;
+                kdp -= 1;
+            } else {
+                redfac = 0;
+            }
+            const double redmax = (s > 0) ? V[k] - Vmin[k] : Vmax[k] - V[k];
+            if (redfac < redmax) d.use0 |= 1u << (3 * ll + k);
+            redfac = (redfac < redmax) ? redfac : redmax;
+            sumdiff -= redfac * s;
+            V[k] -= redfac * s;
+        }
+    }
+    for (int k = 0; k < 3; ++k) d.Vfin[k] = V[k];
+}
+
+// reverse of one field's pass given its decisions: Sb the adjoint of the
+// limited midpoints; accumulates Sin_b and the bound adjoints
+DGR_HD void limit_field_reverse(const LimFieldDec &d, int iv, const double Sb[9], double Sin_b[9], double Vminb[3],
+                                double Vmaxb[3])
+{
+    // mids: S_out = ((V0+V1)/2, (V1+V2)/2, (V2+V0)/2)
+    double Vb[3];
+    Vb[0] = 0.5 * (Sb[3 * iv] + Sb[3 * iv + 2]);
+    Vb[1] = 0.5 * (Sb[3 * iv] + Sb[3 * iv + 1]);
+    Vb[2] = 0.5 * (Sb[3 * iv + 1] + Sb[3 * iv + 2]);
+    double avgb = 0;
+    for (int ll = 2; ll >= 0; --ll) {
+        const int s = d.sd[ll];
+        double sumdiffb = 0;   // adjoint of the running sumdiff after the k loop
+        for (int k = 2; k >= 0; --k) {
+            // V_k -= red*s ; sumdiff -= red*s
+            const double redb = -s * Vb[k] - s * sumdiffb;
+            const unsigned bit = 1u << (3 * ll + k);
+            if (d.use0 & bit) {
+                if (d.pos & bit) {
+                    // the divisor max(1, kdp): positives at k and after in this pass
+                    const unsigned later = (d.pos >> (3 * ll + k)) & (7u >> k);
+                    const int cnt = (int)(later & 1u) + (int)((later >> 1) & 1u) + (int)((later >> 2) & 1u);
+                    sumdiffb += redb * s / (cnt > 1 ? cnt : 1);
+                }
+            } else {
+                if (s > 0) { Vb[k] += redb; Vminb[k] -= redb; }
+                else { Vmaxb[k] += redb; Vb[k] -= redb; }
+            }
+        }
+        // sumdiff = (sum V / 3 - avg) * 3 = sum V - 3 avg
+        for (int k = 0; k < 3; ++k) Vb[k] += sumdiffb;
+        avgb += -3 * sumdiffb;
+    }
+    // clamp
+    double V0b[3];
+    for (int k = 0; k < 3; ++k) {
+        V0b[k] = 0;
+        if (d.clampsel[k] == 0) V0b[k] = Vb[k];
+        else if (d.clampsel[k] == 1) Vmaxb[k] += Vb[k];
+        else Vminb[k] += Vb[k];
+    }
+    // V0 = (w1-w2+w3, w1+w2-w3, -w1+w2+w3), avg = (w1+w2+w3)/3
+    Sin_b[3 * iv] += V0b[0] + V0b[1] - V0b[2] + avgb / 3;
+    Sin_b[3 * iv + 1] += -V0b[0] + V0b[1] + V0b[2] + avgb / 3;
+    Sin_b[3 * iv + 2] += V0b[0] - V0b[1] + V0b[2] + avgb / 3;
+}
+
 // ---------------------------------------------------------------------------
 // Limiter element pass (SlopeLimiter2D.c:157-343) reverse for one element:
 // S[9] is the element's state before limiting, Vmin/Vmax[f][k] the node
 // bounds at its vertices, zv[3] the vertex depths; Sb[9] the adjoint of the
 // limited state.  Returns the modify flag.  Accumulates Sin_b[9] (into the
 // element's own pre-limit state) and Vminb/Vmaxb[f][k]; Sout (optional)
-// receives the limited state.
+// receives the limited state.  The zeta pass decides modify; each field is then
+// replayed and reversed on its own, so one field's decisions are live at a
+// time.
 // ---------------------------------------------------------------------------
 DGR_HD int limit_adj(const double S[9], const double Vmin[3][3], const double Vmax[3][3], const double zv[3],
                      double H0, const double Sb[9], double Sin_b[9], double Vminb[3][3], double Vmaxb[3][3],
                      double Sout[9] = nullptr)
 {
-    // forward decisions
+    LimFieldDec d;
+    limit_field_forward(S, 0, Vmin[0], Vmax[0], d);
     int modify = 0;
-    int clampsel[3][3];          // 0: V, 1: Vmax, 2: Vmin
-    int sd[3][3];                // [field][pass]
-    int pos[3][3][3], dvv[3][3][3], use0[3][3][3];   // [field][pass][k]
-    double Vfin[3][3];
-    for (int iv = 0; iv < 3; ++iv) {
-        const double w1 = S[3 * iv], w2 = S[3 * iv + 1], w3 = S[3 * iv + 2];
-        double V[3] = {w1 - w2 + w3, w1 + w2 - w3, -w1 + w2 + w3};
-        const double avg = (w1 + w2 + w3) / 3;
-        if (iv == 0) {
-            const double Ha = V[0] + zv[0], Hb_ = V[1] + zv[1], Hc = V[2] + zv[2];
-            const double Hm0 = Vmin[0][0] + zv[0], Hm1 = Vmin[0][1] + zv[1], Hm2 = Vmin[0][2] + zv[2];
-            if (Ha > H0 && Hb_ > H0 && Hc > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
-        }
-        for (int k = 0; k < 3; ++k) {
-            // at an exact tie the element's own value is taken as the active
-            // branch: the clip is then inactive, as it is in both directions
-            // of a central difference when several elements share the bound
-            // (lake at rest), and no adjoint is routed through a tied bound
-            const double lo = (V[k] < Vmax[iv][k]) ? V[k] : Vmax[iv][k];
-            int sel = (V[k] <= Vmax[iv][k]) ? 0 : 1;
-            if (Vmin[iv][k] > lo) sel = 2;
-            clampsel[iv][k] = sel;
-            V[k] = fmax(lo, Vmin[iv][k]);
-        }
-        for (int ll = 0; ll < 3; ++ll) {
-            const double sumloc = (V[0] + V[1] + V[2]) / 3;
-            double sumdiff = (sumloc - avg) * 3;
-            const int s = sumdiff < 0 ? -1 : (sumdiff == 0 ? 0 : 1);
-            sd[iv][ll] = s;
-            const double diff[3] = {(V[0] - avg) * s, (V[1] - avg) * s, (V[2] - avg) * s};
-            double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
-            for (int k = 0; k < 3; ++k) {
-                const double dv = fmax(1.0, kdp);
-                double redfac;
-                pos[iv][ll][k] = diff[k] > 0;
-                dvv[iv][ll][k] = (int)dv;
-                if (diff[k] > 0) { redfac = sumdiff * s / dv; kdp -= 1; }
-                else redfac = 0;
-                const double redmax = (s > 0) ? V[k] - Vmin[iv][k] : Vmax[iv][k] - V[k];
-                use0[iv][ll][k] = redfac < redmax;
-                redfac = (redfac < redmax) ? redfac : redmax;
-                sumdiff -= redfac * s;
-                V[k] -= redfac * s;
-            }
-        }
-        for (int k = 0; k < 3; ++k) Vfin[iv][k] = V[k];
-        if (iv == 0) {
-            const double Ha = V[0] + zv[0], Hb_ = V[1] + zv[1], Hc = V[2] + zv[2];
-            if (Ha < H0 || Hb_ < H0 || Hc < H0) modify = 0;
-        }
-    }
-    if (Sout) {
-        for (int i = 0; i < 9; ++i) Sout[i] = S[i];
-        if (modify)
-            for (int iv = 0; iv < 3; ++iv) {
-                Sout[3 * iv] = (Vfin[iv][0] + Vfin[iv][1]) / 2;
-                Sout[3 * iv + 1] = (Vfin[iv][1] + Vfin[iv][2]) / 2;
-                Sout[3 * iv + 2] = (Vfin[iv][2] + Vfin[iv][0]) / 2;
-            }
+    {
+        const double w1 = S[0], w2 = S[1], w3 = S[2];
+        const double V0 = w1 - w2 + w3, V1 = w1 + w2 - w3, V2 = -w1 + w2 + w3;
+        const double Ha = V0 + zv[0], Hb_ = V1 + zv[1], Hc = V2 + zv[2];
+        const double Hm0 = Vmin[0][0] + zv[0], Hm1 = Vmin[0][1] + zv[1], Hm2 = Vmin[0][2] + zv[2];
+        if (Ha > H0 && Hb_ > H0 && Hc > H0 && Hm0 > H0 && Hm1 > H0 && Hm2 > H0) modify = 1;
+        const double Fa = d.Vfin[0] + zv[0], Fb = d.Vfin[1] + zv[1], Fc = d.Vfin[2] + zv[2];
+        if (Fa < H0 || Fb < H0 || Fc < H0) modify = 0;
     }
     if (!modify) {
+        if (Sout)
+            for (int i = 0; i < 9; ++i) Sout[i] = S[i];
         for (int i = 0; i < 9; ++i) Sin_b[i] += Sb[i];
         return 0;
     }
-    // reverse, field by field
     for (int iv = 0; iv < 3; ++iv) {
-        // mids: S_out = ((V0+V1)/2, (V1+V2)/2, (V2+V0)/2)
-        double Vb[3];
-        Vb[0] = 0.5 * (Sb[3 * iv] + Sb[3 * iv + 2]);
-        Vb[1] = 0.5 * (Sb[3 * iv] + Sb[3 * iv + 1]);
-        Vb[2] = 0.5 * (Sb[3 * iv + 1] + Sb[3 * iv + 2]);
-        double avgb = 0;
-        for (int ll = 2; ll >= 0; --ll) {
-            const int s = sd[iv][ll];
-            double sumdiffb = 0;   // adjoint of the running sumdiff after the k loop
-            for (int k = 2; k >= 0; --k) {
-                // V_k -= red*s ; sumdiff -= red*s
-                const double redb = -s * Vb[k] - s * sumdiffb;
-                if (use0[iv][ll][k]) {
-                    if (pos[iv][ll][k]) sumdiffb += redb * s / dvv[iv][ll][k];
-                } else {
-                    if (s > 0) { Vb[k] += redb; Vminb[iv][k] -= redb; }
-                    else { Vmaxb[iv][k] += redb; Vb[k] -= redb; }
-                }
-            }
-            // sumdiff = (sum V / 3 - avg) * 3 = sum V - 3 avg
-            for (int k = 0; k < 3; ++k) Vb[k] += sumdiffb;
-            avgb += -3 * sumdiffb;
+        if (iv > 0) limit_field_forward(S, iv, Vmin[iv], Vmax[iv], d);
+        if (Sout) {
+            Sout[3 * iv] = (d.Vfin[0] + d.Vfin[1]) / 2;
+            Sout[3 * iv + 1] = (d.Vfin[1] + d.Vfin[2]) / 2;
+            Sout[3 * iv + 2] = (d.Vfin[2] + d.Vfin[0]) / 2;
         }
-        // clamp
-        double V0b[3];
-        for (int k = 0; k < 3; ++k) {
-            V0b[k] = 0;
-            if (clampsel[iv][k] == 0) V0b[k] = Vb[k];
-            else if (clampsel[iv][k] == 1) Vmaxb[iv][k] += Vb[k];
-            else Vminb[iv][k] += Vb[k];
-        }
-        // V0 = (w1-w2+w3, w1+w2-w3, -w1+w2+w3), avg = (w1+w2+w3)/3
-        Sin_b[3 * iv] += V0b[0] + V0b[1] - V0b[2] + avgb / 3;
-        Sin_b[3 * iv + 1] += -V0b[0] + V0b[1] + V0b[2] + avgb / 3;
-        Sin_b[3 * iv + 2] += V0b[0] - V0b[1] + V0b[2] + avgb / 3;
+        limit_field_reverse(d, iv, Sb, Sin_b, Vminb[iv], Vmaxb[iv]);
     }
     return 1;
 }

@@ -13,6 +13,7 @@
 // restored to the start state on return.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 #include <vector>
 
@@ -92,6 +93,19 @@ extern "C" dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, 
     return DGR_OK;
 }
 
+extern "C" dgr_status dgr_set_tape_budget(dgr_handle *h, long long bytes)
+{
+    if (!h) return DGR_ERR_ARG;
+    h->tape_budget = bytes < 0 ? -1 : bytes;
+    if (h->d_tape) {
+        DGR_HIP(hipStreamSynchronize(h->stream));
+        DGR_HIP(hipFree(h->d_tape));
+        h->d_tape = nullptr;
+        h->tape_bytes = 0;
+    }
+    return DGR_OK;
+}
+
 extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *sp, double *loss, double *grad_z,
                                          double *grad_n)
 {
@@ -133,22 +147,88 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&d_obs, per * (nsamp > 0 ? nsamp : 1)));
     DGR_HIP(B.alloc(&d_hv, per * (nsamp > 0 ? nsamp : 1)));
     DGR_HIP(B.alloc(&actl, nm));
-    // tape of one segment
-    double *tw0, *tu1, *tp1, *tu2;
-    int *twa, *twb;
-    DGR_HIP(B.alloc(&tw0, W9 * K));
-    DGR_HIP(B.alloc(&tu1, W9 * K));
-    DGR_HIP(B.alloc(&tp1, W9 * K));
-    DGR_HIP(B.alloc(&tu2, W9 * K));
-    DGR_HIP(B.alloc(&twa, P * K));
-    DGR_HIP(B.alloc(&twb, P * K));
-    // node bounds of both stages of every step and the elements attaining
-    // them, as the forward limiter saw them (the reverse routes the bound
-    // adjoints to those elements)
-    double *tnm;
-    int *tna;
-    DGR_HIP(B.alloc(&tnm, 12 * nn * K));
-    DGR_HIP(B.alloc(&tna, 12 * nn * K));
+    // Tapes: each step's start state, both stage outputs before and after
+    // the limiter, both stages' flags, node bounds and the elements attaining
+    // them.  The last R segments are taped by the forward pass itself and
+    // stay resident (R from the tape budget); the others are recomputed from
+    // their checkpoints into slot 0 when the reverse reaches them.  Both
+    // ways run the same kernels on the same inputs: the tapes are identical.
+    const size_t Wb = W9 * sizeof(double), Ib = P * sizeof(int), Cb = (size_t)nm * sizeof(Ctrl);
+    const bool wdon = h->ph.wetdry != 0;
+    auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t b_w = up(Wb * K), b_i = up(Ib * K), b_nm = up(12 * nn * K * sizeof(double)),
+                 b_na = up(12 * nn * K * sizeof(int));
+    const size_t seg_bytes = 4 * b_w + 2 * b_i + b_nm + b_na;
+    long long budget = h->tape_budget;
+    if (budget < 0) {
+        // half the free memory, counting the tape space the handle holds
+        size_t fr = 0, tot = 0;
+        budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? (long long)((fr + h->tape_bytes) / 2) : 0;
+    }
+    int R = (int)std::min<long long>(nseg, budget / (long long)seg_bytes);
+    if (h->tape_bytes < seg_bytes * (size_t)(R > 1 ? R : 1)) {
+        // grow the handle's tape space (kept for the next call)
+        if (h->d_tape) {
+            DGR_HIP(hipStreamSynchronize(s));
+            DGR_HIP(hipFree(h->d_tape));
+            h->d_tape = nullptr;
+            h->tape_bytes = 0;
+        }
+        for (;;) {
+            const size_t want = seg_bytes * (size_t)(R > 1 ? R : 1);
+            if (hipMalloc((void **)&h->d_tape, want) == hipSuccess) {
+                h->tape_bytes = want;
+                break;
+            }
+            (void)hipGetLastError();
+            h->d_tape = nullptr;
+            if (R <= 1) return DGR_ERR_HIP;
+            R /= 2;
+        }
+    }
+    char *const tape = h->d_tape;
+    struct Tape {
+        double *w0, *u1, *p1, *u2, *nm;
+        int *wa, *wb, *na;
+    };
+    auto slot = [&](int j) {
+        char *b = tape + seg_bytes * (size_t)j;
+        Tape t;
+        t.w0 = (double *)b; b += b_w;
+        t.u1 = (double *)b; b += b_w;
+        t.p1 = (double *)b; b += b_w;
+        t.u2 = (double *)b; b += b_w;
+        t.wa = (int *)b; b += b_i;
+        t.wb = (int *)b; b += b_i;
+        t.nm = (double *)b; b += b_nm;
+        t.na = (int *)b;
+        return t;
+    };
+    const int first_res = nseg - R;   // segments >= first_res stay taped from the forward
+    // step n's buffers in tape T (k = n - segment start; last: the segment's
+    // last step, whose output is the handle's state)
+    auto taped_step = [&](const Tape &T, int k, bool last) {
+        StepBufs bf{};
+        bf.w0 = T.w0 + W9 * k;
+        bf.u1 = T.u1 + W9 * k;
+        bf.p1 = T.p1 + W9 * k;
+        bf.u2 = T.u2 + W9 * k;
+        bf.wout = last ? h->d_w : T.w0 + W9 * (k + 1);
+        if (wdon) {
+            bf.wdprev = k == 0 ? h->d_wd : T.wb + P * (k - 1);
+            bf.wda = T.wa + P * k;
+            bf.wdb = T.wb + P * k;
+            bf.wdnext = last ? h->d_wdn : T.wa + P * (k + 1);
+            bf.wda_ready = true;
+        } else {
+            bf.wdprev = bf.wda = bf.wdb = h->d_wd;   // never written without wet/dry
+        }
+        bf.nodemm1 = T.nm + 6 * nn * (2 * k);
+        bf.nodemm2 = T.nm + 6 * nn * (2 * k + 1);
+        bf.nodearg1 = T.na + 6 * nn * (2 * k);
+        bf.nodearg2 = T.na + 6 * nn * (2 * k + 1);
+        return bf;
+    };
     // reverse buffers: slot/zbe pairs per stage (the edge part of a stage's
     // input adjoint is consumed by the limiter reverse that comes next)
     double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *nodeb, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
@@ -181,24 +261,41 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         DGR_HIP(hipMemsetAsync(actl, 0, nm * sizeof(AdjCtrl), s));
         DGR_HIP(hipStreamSynchronize(s));
     }
-    const size_t Wb = W9 * sizeof(double), Ib = P * sizeof(int), Cb = (size_t)nm * sizeof(Ctrl);
-    const bool wdon = h->ph.wetdry != 0;
 
-    // forward with checkpoints (run steps: each step's end records its max
-    // |lambda| and the next step's t, dt and tide)
+    // forward: checkpoints of the recomputed segments (and of the start, to
+    // restore the handle), taped steps in the resident ones; each step's end
+    // records its max |lambda| and the next step's t, dt and tide
     enqueue_run_prologue(h);
     launch_step_record(h->m, h->d_ctrl, rec, 0, s);
     for (int n = 0; n < N; ++n) {
-        if (n % K == 0) {
-            const int c = n / K;
+        const int c = n / K, k = n % K;
+        const int n1 = (c + 1) * K < N ? (c + 1) * K : N;
+        StepRec *re = rec + (size_t)nm * n, *rx = n + 1 < N ? rec + (size_t)nm * (n + 1) : nullptr;
+        if (k == 0 && (c < first_res || c == 0)) {
             DGR_HIP(hipMemcpyAsync(ckW + W9 * c, h->d_w, Wb, hipMemcpyDeviceToDevice, s));
             DGR_HIP(hipMemcpyAsync(ckWD + P * c, h->d_wd, Ib, hipMemcpyDeviceToDevice, s));
             DGR_HIP(hipMemcpyAsync(ckC + (size_t)nm * c, h->d_ctrl, Cb, hipMemcpyDeviceToDevice, s));
         }
-        enqueue_run_step(h, nullptr, rec + (size_t)nm * n, n + 1 < N ? rec + (size_t)nm * (n + 1) : nullptr);
+        const double *wout = h->d_w;
+        if (c < first_res) {
+            enqueue_run_step(h, nullptr, re, rx);
+        } else {
+            const Tape T = slot(c - first_res);
+            if (k == 0) {
+                // the segment's start state and stage-1 flags (the previous
+                // step's stage-2 limiter or the run prologue wrote them)
+                DGR_HIP(hipMemcpyAsync(T.w0, h->d_w, Wb, hipMemcpyDeviceToDevice, s));
+                if (wdon) DGR_HIP(hipMemcpyAsync(T.wa, h->d_wdn, Ib, hipMemcpyDeviceToDevice, s));
+            }
+            const StepBufs bf = taped_step(T, k, n + 1 == n1);
+            enqueue_run_step(h, &bf, re, rx);
+            // the next step's entry flags, where a plain step leaves them
+            if (wdon && n + 1 == n1) DGR_HIP(hipMemcpyAsync(h->d_wd, T.wb + P * k, Ib, hipMemcpyDeviceToDevice, s));
+            wout = bf.wout;
+        }
         if ((n + 1) % sp->sample_every == 0) {
-            const int k = (n + 1) / sp->sample_every - 1;
-            launch_gauge_sample(h->m, ng, d_gel, h->d_w, d_hv + (size_t)k * per, d_obs + (size_t)k * per, actl, s);
+            const int j = (n + 1) / sp->sample_every - 1;
+            launch_gauge_sample(h->m, ng, d_gel, wout, d_hv + (size_t)j * per, d_obs + (size_t)j * per, actl, s);
         }
     }
     DGR_HIP(hipGetLastError());
@@ -222,55 +319,42 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     bool have_slot1 = false;
     for (int c = nseg - 1; c >= 0; --c) {
         const int n0 = c * K, n1 = (n0 + K < N) ? n0 + K : N;
-        // recompute the segment into the tape: the checkpoint goes to slot 0,
-        // every step reads its slot and writes the next one (no copies); the
-        // stage-2 limiter of each step writes the next step's start flags
-        DGR_HIP(hipMemcpyAsync(tw0, ckW + W9 * c, Wb, hipMemcpyDeviceToDevice, s));
-        DGR_HIP(hipMemcpyAsync(h->d_wd, ckWD + P * c, Ib, hipMemcpyDeviceToDevice, s));
-        DGR_HIP(hipMemcpyAsync(h->d_ctrl, ckC + (size_t)nm * c, Cb, hipMemcpyDeviceToDevice, s));
-        launch_step_begin(h->m, h->d_ctrl, s);
-        if (wdon) launch_wd_status(h->m, h->ph, tw0, h->d_wd, twa, s);
-        for (int n = n0; n < n1; ++n) {
-            const int k = n - n0;
-            const bool last = n + 1 == n1;
-            StepBufs bf{};
-            bf.w0 = tw0 + W9 * k;
-            bf.u1 = tu1 + W9 * k;
-            bf.p1 = tp1 + W9 * k;
-            bf.u2 = tu2 + W9 * k;
-            bf.wout = last ? h->d_w : tw0 + W9 * (k + 1);
-            if (wdon) {
-                bf.wdprev = k == 0 ? h->d_wd : twb + P * (k - 1);
-                bf.wda = twa + P * k;
-                bf.wdb = twb + P * k;
-                bf.wdnext = last ? h->d_wdn : twa + P * (k + 1);
-                bf.wda_ready = true;
-            } else {
-                bf.wdprev = bf.wda = bf.wdb = h->d_wd;   // never written without wet/dry
+        const Tape T = slot(c >= first_res ? c - first_res : 0);
+        if (c < first_res) {
+            // recompute the segment into slot 0: the checkpoint goes to step
+            // 0's start, every step reads its slot and writes the next one
+            // (no copies); the stage-2 limiter of each step writes the next
+            // step's start flags
+            DGR_HIP(hipMemcpyAsync(T.w0, ckW + W9 * c, Wb, hipMemcpyDeviceToDevice, s));
+            DGR_HIP(hipMemcpyAsync(h->d_wd, ckWD + P * c, Ib, hipMemcpyDeviceToDevice, s));
+            DGR_HIP(hipMemcpyAsync(h->d_ctrl, ckC + (size_t)nm * c, Cb, hipMemcpyDeviceToDevice, s));
+            launch_step_begin(h->m, h->d_ctrl, s);
+            if (wdon) launch_wd_status(h->m, h->ph, T.w0, h->d_wd, T.wa, s);
+            for (int n = n0; n < n1; ++n) {
+                const StepBufs bf = taped_step(T, n - n0, n + 1 == n1);
+                enqueue_run_step(h, &bf);
             }
-            bf.nodemm1 = tnm + 6 * nn * (2 * k);
-            bf.nodemm2 = tnm + 6 * nn * (2 * k + 1);
-            bf.nodearg1 = tna + 6 * nn * (2 * k);
-            bf.nodearg2 = tna + 6 * nn * (2 * k + 1);
-            enqueue_run_step(h, &bf);
         }
         for (int n = n1 - 1; n >= n0; --n) {
             const int k = n - n0;
-            double *w0 = tw0 + W9 * k, *u1 = tu1 + W9 * k, *p1 = tp1 + W9 * k, *u2 = tu2 + W9 * k;
-            const int *wda = wdon ? twa + P * k : h->d_wd, *wdb = wdon ? twb + P * k : h->d_wd;
+            double *w0 = T.w0 + W9 * k, *u1 = T.u1 + W9 * k, *p1 = T.p1 + W9 * k, *u2 = T.u2 + W9 * k;
+            const int *wda = wdon ? T.wa + P * k : h->d_wd, *wdb = wdon ? T.wb + P * k : h->d_wd;
             const StepRec *rn = rec + (size_t)nm * n;
             if ((n + 1) % sp->sample_every == 0) {
                 const int j = (n + 1) / sp->sample_every - 1;
                 launch_adj_gauges(h->m, ng, d_gel, d_hv + (size_t)j * per, d_obs + (size_t)j * per, wb, zvb, s);
             }
-            const double *nm1 = tnm + 6 * nn * (2 * k), *nm2 = tnm + 6 * nn * (2 * k + 1);
-            const int *na1 = tna + 6 * nn * (2 * k), *na2 = tna + 6 * nn * (2 * k + 1);
+            const double *nm1 = T.nm + 6 * nn * (2 * k), *nm2 = T.nm + 6 * nn * (2 * k + 1);
+            const int *na1 = T.na + 6 * nn * (2 * k), *na2 = T.na + 6 * nn * (2 * k + 1);
             // stage 2: input p1, base w0, output u2 -> limited -> w^{n+1} (adjoint wb + slot1)
             launch_adj_limit(h->m, h->ph, u2, nm2, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
                              zvb, ub, vmb, actl, s);
             launch_adj_node(h->m, vmb, nodeb, s);
             launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
                            dtp + maxparts, s);
+            // the max-|lambda| adjoint (k_adj_scalar of step n+1) is shared
+            // by the edges whose operand attains the recorded max
+            if (n + 1 < N) launch_adj_ties(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, rn, rn, actl, s);
             launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slot2, zbe2, tdp + maxparts,
                             n + 1 < N, s);
             // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)

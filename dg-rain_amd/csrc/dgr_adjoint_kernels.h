@@ -74,13 +74,6 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
     double zv[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
-    double S[9], sb[9], ub[9], Vmin[3][3], Vmax[3][3], Vminb[3][3], Vmaxb[3][3];
-#pragma unroll
-    for (int f = 0; f < 9; ++f) {
-        S[f] = U[f * P + e];
-        sb[f] = Sb[f * P + e] + (slotb ? slotb[f * P + e] : 0.0);
-        ub[f] = 0.0;
-    }
     if (zbe) {
         // z_edge = (z_a + z_b)/2 of local edge k
         double z[3] = {0, 0, 0};
@@ -93,17 +86,60 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
 #pragma unroll
         for (int k = 0; k < 3; ++k) zvb[k * P + e] += z[k];
     }
+    // limit_adj field by field: the zeta pass decides modify, then each
+    // field's bounds are loaded, its pass replayed and reversed, and its
+    // adjoints stored, so one field's data is live at a time
+    double S[9];
 #pragma unroll
-    for (int f = 0; f < 3; ++f)
+    for (int f = 0; f < 9; ++f) S[f] = U[f * P + e];
+    auto sb_of = [&](int f) { return Sb[f * P + e] + (slotb ? slotb[f * P + e] : 0.0); };
+    double Vmin[3], Vmax[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) { Vmin[k] = nodemm[nd[k]]; Vmax[k] = nodemm[3 * nn + nd[k]]; }
+    LimFieldDec d;
+    limit_field_forward(S, 0, Vmin, Vmax, d);
+    int modify = 0;
+    {
+        const double w1 = S[0], w2 = S[1], w3 = S[2];
+        const double Ha = (w1 - w2 + w3) + zv[0], Hb_ = (w1 + w2 - w3) + zv[1], Hc = (-w1 + w2 + w3) + zv[2];
+        if (Ha > ph.h0 && Hb_ > ph.h0 && Hc > ph.h0 && Vmin[0] + zv[0] > ph.h0 && Vmin[1] + zv[1] > ph.h0 &&
+            Vmin[2] + zv[2] > ph.h0)
+            modify = 1;
+        if (d.Vfin[0] + zv[0] < ph.h0 || d.Vfin[1] + zv[1] < ph.h0 || d.Vfin[2] + zv[2] < ph.h0) modify = 0;
+    }
+    // the limited zeta midpoints (for the PDop check below)
+    double so[3] = {S[0], S[1], S[2]};
+    if (modify) {
+        so[0] = (d.Vfin[0] + d.Vfin[1]) / 2;
+        so[1] = (d.Vfin[1] + d.Vfin[2]) / 2;
+        so[2] = (d.Vfin[2] + d.Vfin[0]) / 2;
+    }
+#pragma unroll
+    for (int iv = 0; iv < 3; ++iv) {
+        double sb[9], ub[9], Vminb[3] = {0, 0, 0}, Vmaxb[3] = {0, 0, 0};
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { sb[3 * iv + k] = sb_of(3 * iv + k); ub[3 * iv + k] = 0.0; }
+        if (!modify) {
+#pragma unroll
+            for (int k = 0; k < 3; ++k) ub[3 * iv + k] += sb[3 * iv + k];
+        } else {
+            if (iv > 0) {
+#pragma unroll
+                for (int k = 0; k < 3; ++k) {
+                    Vmin[k] = nodemm[iv * nn + nd[k]];
+                    Vmax[k] = nodemm[(3 + iv) * nn + nd[k]];
+                }
+                limit_field_forward(S, iv, Vmin, Vmax, d);
+            }
+            limit_field_reverse(d, iv, sb, ub, Vminb, Vmaxb);
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            Vmin[f][k] = nodemm[f * nn + nd[k]];
-            Vmax[f][k] = nodemm[(3 + f) * nn + nd[k]];
-            Vminb[f][k] = 0.0;
-            Vmaxb[f][k] = 0.0;
+            Ub[(3 * iv + k) * P + e] = ub[3 * iv + k];
+            vmb[(iv * 3 + k) * P + e] = Vminb[k];
+            vmb[(9 + iv * 3 + k) * P + e] = Vmaxb[k];
         }
-    double so[9];
-    limit_adj(S, Vmin, Vmax, zv, ph.h0, sb, ub, Vminb, Vmaxb, so);
+    }
     if (ph.wetdry) {
         // PDop2D (wetDry2D.c:66-256) leaves an element unchanged iff every
         // vertex depth it forms from the limited state exceeds H0
@@ -112,15 +148,6 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
         const double H0 = ph.h0;
         if (mh0 - mh1 + mh2 <= H0 || mh0 + mh1 - mh2 <= H0 || -mh0 + mh1 + mh2 <= H0) actl[L.mid].unsupported = 1;
     }
-#pragma unroll
-    for (int f = 0; f < 9; ++f) Ub[f * P + e] = ub[f];
-#pragma unroll
-    for (int f = 0; f < 3; ++f)
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            vmb[(f * 3 + k) * P + e] = Vminb[f][k];
-            vmb[(9 + f * 3 + k) * P + e] = Vmaxb[f][k];
-        }
 }
 
 // node sums of the bound adjoints: nodeb[f][i] (min) and nodeb[3+f][i] (max)
@@ -696,14 +723,21 @@ void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, con
                      double *slotb, double *zbe, double *tideparts, bool with_lambda, hipStream_t s)
 {
     const dim3 g((unsigned)(m.ned_pad / 256)), b(256);
-    if (with_lambda)
-        hipLaunchKernelGGL(k_adj_ties, g, b, 0, s, m, ph, win, bzeta, bqn, wd, rec_lam, rec, actl);
     if (ph.wetdry)
         hipLaunchKernelGGL(k_adj_edge<true>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
                            with_lambda ? 1 : 0, slotb, zbe, tideparts);
     else
         hipLaunchKernelGGL(k_adj_edge<false>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
                            with_lambda ? 1 : 0, slotb, zbe, tideparts);
+}
+
+// the tie count of the recorded max |lambda| that k_adj_edge with
+// with_lambda divides the max-|lambda| adjoint by (after k_adj_scalar)
+void launch_adj_ties(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
+                     const int *wd, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_ties, dim3((unsigned)(m.ned_pad / 256)), dim3(256), 0, s, m, ph, win, bzeta, bqn, wd,
+                       rec_lam, rec, actl);
 }
 
 void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, const double *zbe, double *zvb,

@@ -43,6 +43,8 @@ void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double 
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
                      double *slotb, double *zbe, double *tideparts, bool with_lambda, hipStream_t s);
+void launch_adj_ties(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
+                     const int *wd, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl, hipStream_t s);
 void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, const double *zbe, double *zvb,
                         hipStream_t s);
 void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
@@ -105,6 +107,13 @@ struct dgr_handle {
     int use_graph;
     int graph_variant;
     hipGraphExec_t graph_exec;
+    // HBM the adjoint may keep taped segments in (dgr_set_tape_budget):
+    // < 0 half the free device memory, 0 recompute every segment
+    long long tape_budget = -1;
+    // the adjoint's tape space, kept for the next call (a gradient loop
+    // calls with the same sizes)
+    char *d_tape = nullptr;
+    size_t tape_bytes = 0;
 };
 
 #define DGR_HIP(call)                                                                   \

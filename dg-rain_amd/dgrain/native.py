@@ -108,6 +108,7 @@ def load():
         "dgr_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
         "dgr_member_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
         "dgr_set_infiltration": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
+        "dgr_set_tape_budget": (C.c_int, [vp, C.c_longlong]),
         "dgr_member_set_infiltration": (C.c_int, [vp, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double]),
         "dgr_member_set_time": (C.c_int, [vp, C.c_int, C.c_double, C.c_double]),
         "dgr_member_report": (C.c_int, [vp, C.c_int, C.POINTER(Report)]),
@@ -286,6 +287,11 @@ class _Handle:
             raise ValueError("hydrograph arrays differ in length")
         check(self.lib.dgr_member_set_hydrograph(self.h, int(member), int(code), t.size, _dp(t), _dp(z), _dp(q)),
               "dgr_member_set_hydrograph")
+
+    def set_tape_budget(self, nbytes):
+        """Device memory the adjoint may keep taped segments in (< 0: half the
+        free memory, 0: recompute every segment)."""
+        check(self.lib.dgr_set_tape_budget(self.h, int(nbytes)), "dgr_set_tape_budget")
 
     def set_infiltration(self, K, psi=0.0, theta_e=0.0, s_e=0.0, member=0):
         """Green-Ampt infiltration of a member (K <= 0 turns it off)."""

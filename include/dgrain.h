@@ -244,6 +244,14 @@ typedef struct {
 /* loss[members]; grad_z, grad_n: [sum of members' num_nodes], member order (may be NULL) */
 dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *spec, double *loss, double *grad_z,
                               double *grad_n);
+/* Device memory dgr_adjoint_gauges may spend on resident tapes (extension).
+ * The last R segments (as many as fit) are taped by the forward pass itself
+ * and reversed without recomputation; the others are recomputed from their
+ * checkpoints.  bytes < 0 (the default): half the free device memory at the
+ * call; 0: recompute every segment.  The gradient is the same bit for bit.
+ * The tape space stays allocated on the handle for the next call (a gradient
+ * loop repeats the same sizes) until this call or dgr_destroy releases it. */
+dgr_status dgr_set_tape_budget(dgr_handle *h, long long bytes);
 /* Forward run that samples the gauge depths: h_out[members][nsteps/sample_every][ngauges]. */
 dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every, int ngauges, const int *gauge_el,
                          double *h_out);

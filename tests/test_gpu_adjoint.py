@@ -70,6 +70,48 @@ def test_adjoint_loss_matches_forward(gpu):
     assert np.all(np.isfinite(gz)) and np.all(np.isfinite(gn)) and np.abs(gn).max() > 0
 
 
+def _segment_bytes(s, nn, K):
+    """Device bytes of one segment's tape (dgr_adjoint_api.hip): four state
+    copies, two flag sets, node bounds and their elements, per step."""
+    _, P = s.device_state()
+    up = lambda b: (b + 255) // 256 * 256
+    return 4 * up(9 * P * 8 * K) + 2 * up(P * 4 * K) + up(12 * nn * K * 8) + up(12 * nn * K * 4)
+
+
+@pytest.mark.parametrize("wetdry", [0, 1])
+def test_resident_tapes_give_the_same_gradient(gpu, wetdry):
+    """Segments taped by the forward pass and kept resident, segments
+    recomputed from checkpoints, and a mix of both (the last two of five
+    segments resident, the last one partial) give the same loss and
+    gradients bit for bit."""
+    from helpers import perturbed_mesh, random_state
+    nsteps, every, K = 30, 5, 7
+    if wetdry:
+        wl, gauges, obs = _problem(nsteps=nsteps, sample_every=every)
+        mesh, state, prm = wl.mesh, (wl.zeta, wl.qx, wl.qy), wl.solver_params()
+    else:
+        mesh = perturbed_mesh(N=10, seed=4)
+        state = random_state(mesh, seed=9, dz=0.05, q=0.05)
+        prm = native.default_params(cfl=0.3)
+        gauges = np.array([3, 40, 77, 150], np.int32)
+        s = native.Solver(mesh, prm)
+        s.set_state(*state)
+        s.set_nodal(n=mesh.n * 1.3)
+        obs = s.gauge_run(nsteps, every, gauges)
+        s.close()
+    s = native.Solver(mesh, prm)
+    seg = _segment_bytes(s, mesh.num_nodes, K)
+    out = []
+    for budget in (0, 2 * seg + seg // 2, -1, 1 << 40):
+        s.set_tape_budget(budget)
+        s.set_state(*state)
+        out.append(s.adjoint_gauges(nsteps, every, K, gauges, obs))
+    for loss, gz, gn in out[1:]:
+        assert loss[0] == out[0][0][0]
+        assert np.array_equal(gz, out[0][1]) and np.array_equal(gn, out[0][2])
+    assert np.abs(out[0][2]).max() > 0
+
+
 @pytest.mark.parametrize("param", ["z", "n"])
 def test_adjoint_directional_derivatives(gpu, param):
     """Tolerance: 1e-6 relative between <grad, d> and the central difference,
