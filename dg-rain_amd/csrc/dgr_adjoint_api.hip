@@ -312,7 +312,9 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         return fst;
     }
 
-    // reverse
+    // reverse.  Taped steps of the edge-kernel variant recorded their
+    // max-|lambda| tie counts (h->kind[1] is the variant's partials space)
+    const bool rec_ties = h->kind[1] == PARTS_EDGE;
     DGR_HIP(hipMemsetAsync(wb, 0, Wb, s));
     DGR_HIP(hipMemsetAsync(zvb, 0, 3 * P * sizeof(double), s));
     DGR_HIP(hipMemsetAsync(nvb, 0, 3 * P * sizeof(double), s));
@@ -330,9 +332,11 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             DGR_HIP(hipMemcpyAsync(h->d_ctrl, ckC + (size_t)nm * c, Cb, hipMemcpyDeviceToDevice, s));
             launch_step_begin(h->m, h->d_ctrl, s);
             if (wdon) launch_wd_status(h->m, h->ph, T.w0, h->d_wd, T.wa, s);
+            // (the step records are written again, with the same values, and
+            // with the tie counts of the taped edge kernel)
             for (int n = n0; n < n1; ++n) {
                 const StepBufs bf = taped_step(T, n - n0, n + 1 == n1);
-                enqueue_run_step(h, &bf);
+                enqueue_run_step(h, &bf, rec + (size_t)nm * n, n + 1 < N ? rec + (size_t)nm * (n + 1) : nullptr);
             }
         }
         for (int n = n1 - 1; n >= n0; --n) {
@@ -353,18 +357,24 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
                            dtp + maxparts, s);
             // the max-|lambda| adjoint (k_adj_scalar of step n+1) is shared
-            // by the edges whose operand attains the recorded max
-            if (n + 1 < N) launch_adj_ties(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, rn, rn, actl, s);
+            // by the edges whose operand attains the recorded max; the taped
+            // edge kernel counted them, the element-partials variants count
+            // them here
+            int lam_mode = 0;
+            if (n + 1 < N) {
+                lam_mode = rec_ties ? 2 : 1;
+                if (!rec_ties) launch_adj_ties(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, rn, rn, actl, s);
+            }
             launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slot2, zbe2, tdp + maxparts,
-                            n + 1 < N, s);
+                            lam_mode, s);
             // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)
             launch_adj_limit(h->m, h->ph, u1, nm1, pb, slot2, zbe2, zvb, ub, vmb, actl, s);
             launch_adj_node(h->m, vmb, nodeb, s);
             launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nodeb, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);
-            launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, false, s);
+            launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, 0, s);
             have_slot1 = true;
-            launch_adj_scalar(h->m, dtp, tdp, maxparts, rn, n > 0 ? rn - nm : rn, n > 0, actl, s);
+            launch_adj_scalar(h->m, dtp, tdp, maxparts, rn, n > 0 ? rn - nm : rn, n > 0, rec_ties ? 1 : 0, actl, s);
             double *t = wb;
             wb = wb0;
             wb0 = t;

@@ -425,7 +425,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_edge(MeshDev m, PhysDev ph, cons
         const EdgeState s = edge_state(m, M, rec[mid], win, bzeta, bqn, ed);
         double lam_share = 0.0;
         const AdjCtrl &ac = actl[mid];
-        if (use_lambda && ac.lamb != 0.0 && ac.ntie > 0) lam_share = ac.lamb / ac.ntie;
+        // the tie count: recorded by the taped forward (use_lambda 2) or
+        // counted by k_adj_ties (1)
+        const int ntie = (use_lambda == 2) ? rec_lam[mid].ntie : ac.ntie;
+        if (use_lambda && ac.lamb != 0.0 && ntie > 0) lam_share = ac.lamb / ntie;
         const double target = __longlong_as_double((long long)rec_lam[mid].lam2);
         double Fin_b[3], Fex_b[3];
 #pragma unroll
@@ -566,7 +569,7 @@ __global__ __launch_bounds__(256) void k_adj_combine(MeshDev m, double *Dst, con
 // (dt_n = cfl minEdgLength / lambda_{n-1} unless the final-time clamp acted).
 __global__ void k_adj_scalar(MeshDev m, const double *__restrict__ dtparts, const double *__restrict__ tideparts,
                              int maxparts, const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_prev,
-                             int has_prev, AdjCtrl *actl)
+                             int has_prev, int rec_ties, AdjCtrl *actl)
 {
     const int mid = blockIdx.x;
     const MemberDev &M = m.mem[mid];
@@ -592,6 +595,9 @@ __global__ void k_adj_scalar(MeshDev m, const double *__restrict__ dtparts, cons
         tbar = tb * M.tide_amp * w * cos(w * r.t + M.tide_phase);
     }
     a.S += tbar;
+    // the max-|lambda| adjoint this step's edges shared needs a recorded tie
+    // count; a fold that saw a NaN operand has none
+    if (rec_ties && a.lamb != 0.0 && r.ntie <= 0) a.unsupported = 1;
     a.lamb = 0.0;
     a.ntie = 0;
     if (has_prev) {
@@ -720,15 +726,15 @@ void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double 
 
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
-                     double *slotb, double *zbe, double *tideparts, bool with_lambda, hipStream_t s)
+                     double *slotb, double *zbe, double *tideparts, int with_lambda, hipStream_t s)
 {
     const dim3 g((unsigned)(m.ned_pad / 256)), b(256);
     if (ph.wetdry)
         hipLaunchKernelGGL(k_adj_edge<true>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
-                           with_lambda ? 1 : 0, slotb, zbe, tideparts);
+                           with_lambda, slotb, zbe, tideparts);
     else
         hipLaunchKernelGGL(k_adj_edge<false>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
-                           with_lambda ? 1 : 0, slotb, zbe, tideparts);
+                           with_lambda, slotb, zbe, tideparts);
 }
 
 // the tie count of the recorded max |lambda| that k_adj_edge with
@@ -747,10 +753,11 @@ void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, cons
 }
 
 void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
-                       const StepRec *rec, const StepRec *rec_prev, int has_prev, AdjCtrl *actl, hipStream_t s)
+                       const StepRec *rec, const StepRec *rec_prev, int has_prev, int rec_ties, AdjCtrl *actl,
+                       hipStream_t s)
 {
     hipLaunchKernelGGL(k_adj_scalar, dim3((unsigned)m.nmem), dim3(256), 0, s, m, dtparts, tideparts, maxparts, rec,
-                       rec_prev, has_prev, actl);
+                       rec_prev, has_prev, rec_ties, actl);
 }
 
 void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,

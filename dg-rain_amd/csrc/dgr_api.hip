@@ -62,7 +62,7 @@ static dgr_status free_handle(dgr_handle *h)
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_edges,
                     h->d_einfo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
                     h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts,
-                    h->d_nanparts, h->d_wdn, h->d_hydro, h->d_tape};
+                    h->d_nanparts, h->d_wdn, h->d_hydro, h->d_tape, h->d_tiemax, h->d_tiecnt};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -262,6 +262,8 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     DGR_TRY(dalloc(&h->d_ctrl, n));
     DGR_TRY(dalloc(&h->d_lamparts, 2 * (size_t)maxparts));
     DGR_TRY(dalloc(&h->d_nanparts, 2 * (size_t)maxparts));
+    DGR_TRY(dalloc(&h->d_tiemax, (size_t)maxparts));
+    DGR_TRY(dalloc(&h->d_tiecnt, (size_t)maxparts));
     DGR_TRY(dalloc(&h->d_wdn, P));
     DGR_TRY(hipMemcpyAsync(h->d_nodes, T.nodes.data(), nn_tot * sizeof(double4), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_ev, T.ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
@@ -309,6 +311,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     PhysDev &ph = h->ph;
     ph.g = prms[0].g; ph.h0 = prms[0].h0; ph.velzero = prms[0].velzero; ph.wetdry = prms[0].wetdry;
     ph.lamparts = h->d_lamparts; ph.nanparts = h->d_nanparts; ph.maxparts = maxparts;
+    ph.tiemax = h->d_tiemax; ph.tiecnt = h->d_tiecnt;
     size_t maxne = 0;
     for (int i = 0; i < n; ++i) maxne = maxne > (size_t)mem[i].ne ? maxne : (size_t)mem[i].ne;
     h->h_planes.resize(9 * maxne);
@@ -702,8 +705,12 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b)
                  wdon ? b->wdb : nullptr, s);
     const int *wd2 = wdon ? b->wdb : b->wda;
     mark(h);
+    // a taped step's edge kernel also counts the stage-2 max-|lambda| ties
+    // for the reverse (k_step_end records them)
+    const bool ties = b->nodemm2 != nullptr;
     h->kind[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, b->p1, b->u2, b->w0, h->d_bzeta, h->d_bqn, wd2,
-                              h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
+                              h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s, ties);
+    h->ties_last = ties && h->kind[1] == PARTS_EDGE;
     mark(h);
     // stage 2's max |lambda| sets the next dt.  Element partials: repair the
     // fold if a NaN operand was seen (edge partials carry the NaN position).
@@ -741,7 +748,7 @@ void enqueue_run_step(dgr_handle *h, const StepBufs *b, StepRec *rec_end, StepRe
     const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true,
                           nullptr, nullptr, nullptr, nullptr};
     enqueue_stages(h, b ? b : &run);
-    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], true, h->stream, rec_end, rec_next);
+    launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], true, h->stream, rec_end, rec_next, h->ties_last);
 }
 
 extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)

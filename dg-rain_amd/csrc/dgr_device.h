@@ -177,6 +177,11 @@ struct PhysDev {
     // operand in the workgroup's edge range (-1: none); lamparts then holds
     // the max of the operands after it (-1.0: none)
     int *nanparts;
+    // edge kernel of a taped stage 2 only, [maxparts]: the workgroup's
+    // largest non-NaN operand (raw bits, -1.0: none) and how many of its
+    // operands equal it; k_step_end turns them into StepRec::ntie
+    unsigned long long *tiemax;
+    int *tiecnt;
     int maxparts;
 };
 
@@ -193,6 +198,10 @@ struct AdjCtrl {
 struct StepRec {
     double t, dt, tide;         // at the step's start
     unsigned long long lam2;    // max |lambda| of the step's second stage (raw bits)
+    // operands of the stage-2 max-|lambda| fold equal to lam2 (taped edge-kernel
+    // steps; -1: the fold saw a NaN operand)
+    int ntie;
+    int pad;
 };
 
 // which workgroup space a flux launch wrote its partials in

@@ -13,7 +13,7 @@ namespace dgr {
 int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
                  double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
-                 hipStream_t s);
+                 hipStream_t s, bool ties = false);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
@@ -24,7 +24,7 @@ void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, cons
                       hipStream_t s);
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
-                     StepRec *rec_end = nullptr, StepRec *rec_next = nullptr);
+                     StepRec *rec_end = nullptr, StepRec *rec_next = nullptr, bool ties = false);
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
 void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s);
 void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s);
@@ -42,13 +42,14 @@ void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double 
                     double *dtparts, hipStream_t s);
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
-                     double *slotb, double *zbe, double *tideparts, bool with_lambda, hipStream_t s);
+                     double *slotb, double *zbe, double *tideparts, int with_lambda, hipStream_t s);
 void launch_adj_ties(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl, hipStream_t s);
 void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, const double *zbe, double *zvb,
                         hipStream_t s);
 void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
-                       const StepRec *rec, const StepRec *rec_prev, int has_prev, AdjCtrl *actl, hipStream_t s);
+                       const StepRec *rec, const StepRec *rec_prev, int has_prev, int rec_ties, AdjCtrl *actl,
+                       hipStream_t s);
 void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,
                        double *Wb, double *zvb, hipStream_t s);
 void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
@@ -93,6 +94,9 @@ struct dgr_handle {
     Ctrl *d_ctrl;              // [nmem]
     unsigned long long *d_lamparts;   // [2][maxparts] per-workgroup max |lambda|
     int *d_nanparts;                  // [2][maxparts] last NaN operand per edge workgroup
+    unsigned long long *d_tiemax;     // [maxparts] tie partials of a taped stage 2
+    int *d_tiecnt;
+    bool ties_last;                   // the last enqueued step counted max-|lambda| ties
     int *d_wdn;      // [pitch] wet/dry flags for the next step's start (dgr_run)
     double *d_hydro; // hydrograph tables of all members (MeshDev::hydro), or null
     int variant;     // 0 fused, 1 split element-centric, 2 split edge-centric

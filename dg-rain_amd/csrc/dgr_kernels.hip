@@ -612,6 +612,35 @@ __device__ __forceinline__ void seg_commit(int tp, double tm, int first, const P
     }
 }
 
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// the workgroup's largest non-NaN max-|lambda| operand and how many of its
+// operands equal it (lane values a, c), for the tie count of a taped step
+__device__ __forceinline__ void tie_commit(double a, int c, const PhysDev &ph, int lb)
+{
+    __shared__ double ra[4];
+    __shared__ int rc[4];
+    double A = wave_max(a);
+    if ((threadIdx.x & 63) == 0) ra[threadIdx.x >> 6] = A;
+    __syncthreads();
+    A = ra[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) A = (ra[i] > A) ? ra[i] : A;
+    const int cc = wave_sum((a == A) ? c : 0);
+    if ((threadIdx.x & 63) == 0) rc[threadIdx.x >> 6] = cc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        ph.tiemax[lb] = (unsigned long long)__double_as_longlong(A);
+        ph.tiecnt[lb] = rc[0] + rc[1] + rc[2] + rc[3];
+    }
+}
+
 // max of n partials by one block of 256 threads; the result is valid in thread 0
 __device__ __forceinline__ unsigned long long block_reduce_parts(const unsigned long long *p, int n)
 {
@@ -783,7 +812,7 @@ template <bool WETDRY>
 __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
                                                    const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                    const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrls,
-                                                   int slot)
+                                                   int slot, int ties)
 {
     const int lb = xcd_block();
     const int mid = m.dblk_mem[lb];
@@ -792,6 +821,8 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
     const int ed = lb * 256 + (int)threadIdx.x;
     int tp = -1, first = -2;
     double tm = -1.0;
+    double ta = -1.0;   // largest non-NaN operand of the lane and its count (ties)
+    int tc = 0;
     if (ed < M.ed_off + M.ned) {
         const int P = m.pitch;
         const double g = ph.g;
@@ -911,6 +942,8 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             const double x = ops.v[i];
             if (x != x) { tp = (int)ops.pos[i]; tm = -1.0; }
             else tm = (x > tm) ? x : tm;
+            if (x > ta) { ta = x; tc = 1; }
+            else if (x == ta) tc += 1;
         }
         first = (int)p0;
 #pragma unroll
@@ -921,6 +954,7 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
         }
     }
     seg_commit(tp, tm, first, ph, slot, lb);
+    if (ties) tie_commit(ta, tc, ph, lb);
 }
 
 // Split variants, part 2: inner products, RHS and RK update from the Fhat planes.
@@ -1427,18 +1461,53 @@ __device__ __forceinline__ double resolve_lam(const MeshDev &m, const PhysDev &p
     return ml;
 }
 
+// the member's count of stage-2 max-|lambda| operands equal to ml (the
+// resolved fold) from the taped edge launch's tie partials, or -1 when the
+// fold saw a NaN operand (the largest non-NaN operand is then not ml).  All
+// threads of the workgroup call it; the result is valid in thread 0.
+__device__ __forceinline__ int member_ties(const MeshDev &m, const PhysDev &ph, int mid, double ml)
+{
+    __shared__ double ra[4];
+    __shared__ int rc[4];
+    const MemberDev &M = m.mem[mid];
+    const unsigned long long *tp = ph.tiemax + M.dblk0;
+    const int *tn = ph.tiecnt + M.dblk0;
+    double A = -1.0;
+    for (int b = threadIdx.x; b < M.dblks; b += 256) {
+        const double v = __longlong_as_double((long long)tp[b]);
+        A = (v > A) ? v : A;
+    }
+    A = wave_max(A);
+    if ((threadIdx.x & 63) == 0) ra[threadIdx.x >> 6] = A;
+    __syncthreads();
+    A = ra[0];
+#pragma unroll
+    for (int i = 1; i < 4; ++i) A = (ra[i] > A) ? ra[i] : A;
+    int C = 0;
+    for (int b = threadIdx.x; b < M.dblks; b += 256)
+        if (__longlong_as_double((long long)tp[b]) == A) C += tn[b];
+    C = wave_sum(C);
+    if ((threadIdx.x & 63) == 0) rc[threadIdx.x >> 6] = C;
+    __syncthreads();
+    C = rc[0] + rc[1] + rc[2] + rc[3];
+    return (A == ml) ? C : -1;
+}
+
 // one workgroup per member.  With next != 0 the next step's bookkeeping
 // follows (the tide at the new time, the NaN bookkeeping cleared, the step
 // counted), which k_step_begin does otherwise.  Optional step records: the
 // step's max |lambda| into rec_end, the next step's t, dt and tide into
 // rec_next (after the next step's bookkeeping).
 __global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind, int next,
-                                                  StepRec *rec_end, StepRec *rec_next)
+                                                  StepRec *rec_end, StepRec *rec_next, int ties)
 {
     const int mid = blockIdx.x;
     Ctrl *c = ctrls + mid;
     const double ml = resolve_lam(m, ph, c, mid, 1, kind);
+    int ntie = 0;
+    if (ties) ntie = member_ties(m, ph, mid, ml);
     if (threadIdx.x != 0) return;
+    if (ties && rec_end) rec_end[mid].ntie = ntie;
     const MemberDev &M = m.mem[mid];
     const double time = c->t + c->dt;
     double h = M.cfl * M.min_edge / ml;
@@ -1507,7 +1576,7 @@ static inline dim3 member_grid(const MeshDev &m) { return dim3((unsigned)((m.nme
 int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const double *win,
                  double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
-                 hipStream_t s)
+                 hipStream_t s, bool ties)
 {
     const dim3 g = grid_for(m.pitch), b(256);
 #define DGR_LS(MD, WDF) \
@@ -1532,8 +1601,9 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
         kind = PARTS_ELEM;
     } else {
         const dim3 ge = grid_for(m.ned_pad);
-        if (wetdry) hipLaunchKernelGGL(k_edge_flux<true>, ge, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
-        else hipLaunchKernelGGL(k_edge_flux<false>, ge, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot);
+        const int t = ties ? 1 : 0;
+        if (wetdry) hipLaunchKernelGGL(k_edge_flux<true>, ge, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot, t);
+        else hipLaunchKernelGGL(k_edge_flux<false>, ge, b, 0, s, m, ph, win, bzeta, bqn, wd, fh, ctrl, slot, t);
         kind = PARTS_EDGE;
     }
     if (mode == 0) DGR_LR(0); else if (mode == 1) DGR_LR(1); else DGR_LR(2);
@@ -1582,10 +1652,10 @@ void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s)
 }
 
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
-                     StepRec *rec_end, StepRec *rec_next)
+                     StepRec *rec_end, StepRec *rec_next, bool ties)
 {
     hipLaunchKernelGGL(k_step_end, dim3((unsigned)m.nmem), dim3(256), 0, s, m, ph, c, kind, next ? 1 : 0, rec_end,
-                       rec_next);
+                       rec_next, ties ? 1 : 0);
 }
 
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s)

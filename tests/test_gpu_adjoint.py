@@ -112,6 +112,38 @@ def test_resident_tapes_give_the_same_gradient(gpu, wetdry):
     assert np.abs(out[0][2]).max() > 0
 
 
+@pytest.mark.parametrize("still", [True, False])
+def test_recorded_ties_equal_counted_ties(gpu, still):
+    """The edge-kernel variant records how many max-|lambda| operands equal
+    the step's max in the taped forward; the element variants count them in
+    the reverse (k_adj_ties).  Same forward bits, so the gradients must agree
+    bit for bit, also on still water over a flat bed where every edge ties."""
+    from helpers import perturbed_mesh, random_state
+    nsteps, every = 12, 3
+    if still:
+        wl = configs.lake_at_rest(N=8, bumps=False)
+        mesh, state = wl.mesh, (wl.zeta, wl.qx, wl.qy)
+        prm = native.default_params(**wl.params)
+    else:
+        mesh = perturbed_mesh(N=10, seed=4)
+        state = random_state(mesh, seed=9, dz=0.05, q=0.05)
+        prm = native.default_params(cfl=0.3)
+    gauges = np.array([3, 40, 77], np.int32)
+    s = native.Solver(mesh, prm)
+    s.set_state(*state)
+    s.set_nodal(n=mesh.n * 1.3 + 0.01)
+    obs = s.gauge_run(nsteps, every, gauges) + 1e-3
+    out = []
+    for variant in (2, 0, 1):
+        s.set_variant(variant)
+        s.set_state(*state)
+        out.append(s.adjoint_gauges(nsteps, every, 4, gauges, obs))
+    for loss, gz, gn in out[1:]:
+        assert loss[0] == out[0][0][0]
+        assert np.array_equal(gz, out[0][1]) and np.array_equal(gn, out[0][2])
+    assert np.abs(out[0][1]).max() > 0
+
+
 @pytest.mark.parametrize("param", ["z", "n"])
 def test_adjoint_directional_derivatives(gpu, param):
     """Tolerance: 1e-6 relative between <grad, d> and the central difference,
