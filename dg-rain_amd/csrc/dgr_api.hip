@@ -426,6 +426,8 @@ extern "C" dgr_status dgr_member_get_modify(dgr_handle *h, int mi, int stage, un
 // Hydrographs: every member's tables are concatenated into one device array
 // (MeshDev::hydro), rebuilt on each change; the captured step graph holds the
 // mesh descriptor by value, so it is dropped.
+static dgr_status refresh_forcing(dgr_handle *h, int mi);
+
 extern "C" dgr_status dgr_member_set_hydrograph(dgr_handle *h, int mi, int code, int npts, const double *t,
                                                 const double *zeta_b, const double *qn_b)
 {
@@ -465,7 +467,7 @@ extern "C" dgr_status dgr_member_set_hydrograph(dgr_handle *h, int mi, int code,
         DGR_HIP(hipGraphExecDestroy(h->graph_exec));
         h->graph_exec = nullptr;
     }
-    return DGR_OK;
+    return refresh_forcing(h, mi);
 }
 
 extern "C" dgr_status dgr_set_hydrograph(dgr_handle *h, int code, int npts, const double *t, const double *zeta_b,
@@ -496,8 +498,7 @@ extern "C" dgr_status dgr_member_set_infiltration(dgr_handle *h, int mi, double 
     md.infil_K = kk;
     md.infil_P = pp;
     DGR_HIP(hipMemcpyAsync(h->d_mem + mi, &md, sizeof(MemberDev), hipMemcpyHostToDevice, h->stream));
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
+    return refresh_forcing(h, mi);
 }
 
 extern "C" dgr_status dgr_set_infiltration(dgr_handle *h, double K, double psi, double theta_e, double s_e)
@@ -522,36 +523,41 @@ dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c)
     return DGR_OK;
 }
 
+// Member mi's time-dependent forcing (tide, hydrograph values, infiltration
+// rate) at its current t and dt, as k_step_begin evaluates it, so that single
+// operators see the forcing of the handle's time; the other members' Ctrl and
+// every step counter are left as they were.
+static dgr_status refresh_forcing(dgr_handle *h, int mi)
+{
+    std::vector<Ctrl> all(h->nmem), after(h->nmem);
+    DGR_HIP(hipMemcpyAsync(all.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    launch_step_begin(h->m, h->d_ctrl, h->stream);
+    DGR_HIP(hipGetLastError());
+    DGR_HIP(hipMemcpyAsync(after.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    all[mi].tide = after[mi].tide;
+    all[mi].infil = after[mi].infil;
+    for (int k = 0; k < 3; ++k) {
+        all[mi].hyd_z[k] = after[mi].hyd_z[k];
+        all[mi].hyd_q[k] = after[mi].hyd_q[k];
+    }
+    DGR_HIP(hipMemcpyAsync(h->d_ctrl, all.data(), h->nmem * sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
 extern "C" dgr_status dgr_member_set_time(dgr_handle *h, int mi, double t, double dt)
 {
     if (bad_member(h, mi)) return DGR_ERR_ARG;
-    // t, dt, then the tide value at t; the step counter (which only orders
+    // t, dt, then the forcing at t; the step counter (which only orders
     // error keys) is left as it was
     Ctrl c;
     dgr_status st = read_ctrl(h, mi, &c);
     if (st) return st;
     c.t = t;
     c.dt = dt;
-    const MemberHost &M = h->mem[mi];
-    if (M.prm.tide_code) {
-        // the device evaluates the tide in k_step_begin; run it for every
-        // member and restore the others' step counters
-        DGR_HIP(hipMemcpyAsync(h->d_ctrl + mi, &c, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
-        std::vector<Ctrl> all(h->nmem);
-        DGR_HIP(hipMemcpyAsync(all.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
-        DGR_HIP(hipStreamSynchronize(h->stream));
-        launch_step_begin(h->m, h->d_ctrl, h->stream);
-        DGR_HIP(hipGetLastError());
-        std::vector<Ctrl> after(h->nmem);
-        DGR_HIP(hipMemcpyAsync(after.data(), h->d_ctrl, h->nmem * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
-        DGR_HIP(hipStreamSynchronize(h->stream));
-        all[mi].tide = after[mi].tide;
-        DGR_HIP(hipMemcpyAsync(h->d_ctrl, all.data(), h->nmem * sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
-    } else {
-        DGR_HIP(hipMemcpyAsync(h->d_ctrl + mi, &c, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
-    }
-    DGR_HIP(hipStreamSynchronize(h->stream));
-    return DGR_OK;
+    DGR_HIP(hipMemcpyAsync(h->d_ctrl + mi, &c, sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
+    return refresh_forcing(h, mi);
 }
 
 static void fill_report(const Ctrl &c, dgr_report *rep)

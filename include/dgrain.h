@@ -143,7 +143,10 @@ dgr_status dgr_set_hydrograph(dgr_handle *h, int code, int npts, const double *t
  * step's mean rate (F(t + dt) - F(t)) / dt from every zeta DOF of the RHS of
  * each element not flagged dry (after params.rain).  K <= 0 turns it off;
  * otherwise psi > 0, 0 < theta_e <= 1 and 0 <= s_e < 1 or DGR_ERR_ARG.  The
- * rate is updated by dgr_run only.  dgr_adjoint_gauges returns
+ * rate is evaluated at each step's start by dgr_run; single operators
+ * (dgr_compute_rhs, ...) see the rate of the handle's current time and dt,
+ * as for the tide and hydrograph values (refreshed by this call, by
+ * dgr_set_hydrograph and by dgr_set_time).  dgr_adjoint_gauges returns
  * DGR_ERR_UNSUPPORTED while it is on. */
 dgr_status dgr_set_infiltration(dgr_handle *h, double K, double psi, double theta_e, double s_e);
 

@@ -253,6 +253,38 @@ def test_hydrograph_trajectory(gpu, wetdry):
     _compare_run(s, o, 40, 10)
 
 
+def test_forcing_follows_the_handle_time(gpu):
+    """Single operators see the hydrograph values and infiltration rate of the
+    handle's current time: right after the setters, and after dgr_set_time
+    moves a handle back to t = 0 from a later time."""
+    mesh = perturbed_mesh(N=10, seed=4)
+    z, qx, qy = random_state(mesh, seed=9, dz=0.05, q=0.05)
+
+    def forced(h):
+        _set_hydrographs(h)
+        h.set_infiltration(1e-5, *GA)
+
+    a = native.Solver(mesh, native.default_params(cfl=0.3))
+    forced(a)
+    a.set_state(z, qx, qy)
+    a.run(5)
+    a.set_state(z, qx, qy)
+    a.set_time(0.0, native.default_params().dt0)
+    b = native.Solver(mesh, native.default_params(cfl=0.3))
+    forced(b)
+    b.set_state(z, qx, qy)
+    c = native.Solver(mesh, native.default_params(cfl=0.3))
+    c.set_state(z, qx, qy)
+    ra, fa, _ = a.compute2DL()
+    rb, fb, _ = b.compute2DL()
+    rc, fc, _ = c.compute2DL()
+    for x, y in zip(ra + fa, rb + fb):
+        assert same(x, y)
+    assert not all(same(x, y) for x, y in zip(rb + fb, rc + fc))
+    for h in (a, b, c):
+        h.close()
+
+
 def test_hydrograph_rejected_by_adjoint(gpu):
     mesh = perturbed_mesh(N=6, seed=4)
     s = native.Solver(mesh, native.default_params(cfl=0.3))
