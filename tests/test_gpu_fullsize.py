@@ -1,9 +1,11 @@
-"""Parity at the benchmark sizes: the headline coastal workload (512x512
-cells, 524,288 triangles) bit for bit against the oracle over its first
-steps; still water at that size drifts exactly as the oracle's (the
-reference scheme's own rounding drift); and the 64-member
-ensemble of the batching benchmark (7.3 M triangles) advances every member
-bit for bit as its own handle does."""
+"""Parity at the extreme sizes: the headline coastal workload (512x512
+cells, 524,288 triangles) and the largest single-GPU workload (2048x2048,
+8.4 M triangles) bit for bit against the oracle over their first steps;
+still water at 512x512 drifts exactly as the oracle's (the reference
+scheme's own rounding drift); the 64-member ensemble of the batching
+benchmark (7.3 M triangles) advances every member bit for bit as its own
+handle does; and the smallest meshes (one and two triangles, alone and as
+ensemble members) against the oracle."""
 import os
 
 import numpy as np
@@ -11,6 +13,7 @@ import pytest
 
 import oracle
 from dgrain import configs, native
+from dgrain import mesh as M
 from helpers import same
 
 pytestmark = pytest.mark.gpu
@@ -87,4 +90,63 @@ def test_benchmark_ensemble_members_match_solo(gpu):
         assert reps[i].t == r.t and reps[i].dt == r.dt, i
         s.close()
     print("  %d members, %d triangles, bitwise equal to their solo runs" % (len(wls), sum(w.num_el for w in wls)))
+    ens.close()
+
+
+def test_large_2048_bitwise(gpu):
+    """The 8.4 M-triangle workload (tide, rain, wet/dry, limiter), 3 steps:
+    states, flags, t and dt equal to the oracle's (16 threads)."""
+    wl = configs.large(N=2048)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    rep = s.run(3)
+    o = oracle.Oracle.for_workload(wl, threads=min(16, os.cpu_count() or 1))
+    t, dt = o.run(3)
+    for a, b in zip(s.get_state(), o.get_state()):
+        assert same(a, b)
+    assert np.array_equal(s.get_wd(), o.get_wd())
+    assert rep.t == t and rep.dt == dt
+    print("  3 steps of %d triangles bitwise equal" % wl.num_el)
+    s.close()
+
+
+def _tiny(ntri):
+    """One triangle, or two sharing an edge, with water over a sloping bed
+    and some flow; walls."""
+    x = np.array([0.0, 10.0, 0.0, 10.0])[: ntri + 2]
+    y = np.array([0.0, 0.0, 10.0, 10.0])[: ntri + 2]
+    ev = np.array([[0, 1, 2], [1, 3, 2]][:ntri], np.int32).ravel()
+    mesh = M.from_elements(x, y, 1.0 + 0.05 * x, np.full(x.size, 0.03), ev)
+    rng = np.random.default_rng(ntri)
+    z = 0.1 + 0.01 * rng.standard_normal(3 * ntri)
+    q = 0.02 * rng.standard_normal((2, 3 * ntri))
+    return mesh, (z, q[0], q[1])
+
+
+@pytest.mark.parametrize("ntri", [1, 2])
+@pytest.mark.parametrize("wetdry", [0, 1])
+def test_tiny_meshes(gpu, ntri, wetdry):
+    """Meshes of one and two triangles (every edge a wall, a single member
+    workgroup mostly padding): 30 steps bit for bit the oracle's, alone and
+    as a member between two larger ones in an ensemble."""
+    mesh, st = _tiny(ntri)
+    prm = native.default_params(cfl=0.3, wetdry=wetdry)
+    s = native.Solver(mesh, prm)
+    s.set_state(*st)
+    rep = s.run(30)
+    o = oracle.Oracle(mesh, math=oracle.MATH_DEV, cfl=0.3, wetdry=wetdry)
+    o.set_state(*st)
+    t, dt = o.run(30)
+    for a, b in zip(s.get_state(), o.get_state()):
+        assert same(a, b)
+    assert rep.t == t and rep.dt == dt
+    big = configs.dam_break(N=9, nsteps=30, wetdry=wetdry)
+    ens = native.Ensemble([big.mesh, mesh, big.mesh], [big.solver_params(), prm, big.solver_params()])
+    for i, x in enumerate(((big.zeta, big.qx, big.qy), st, (big.zeta, big.qx, big.qy))):
+        ens.set_state(i, *x)
+    reps = ens.run(30)
+    for a, b in zip(ens.get_state(1), o.get_state()):
+        assert same(a, b)
+    assert reps[1].t == t and reps[1].dt == dt
+    s.close()
     ens.close()
