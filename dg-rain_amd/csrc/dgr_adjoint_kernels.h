@@ -342,8 +342,8 @@ __device__ __forceinline__ EdgeState edge_state(const MeshDev &m, const MemberDe
         s.tide = M.tide_code != 0 && s.code == M.tide_code;
         s.zex = s.tide ? rc.tide : bzeta[ed];
         double qnex = s.tide ? 1000000.0 : bqn[ed];
-        if (s.zex < 0) { s.zex = s.zin; s.zex_from_in = true; }
-        if (qnex > 1000) { qnex = s.qxin * s.nx + s.qyin * s.ny; s.qn_from_in = true; }
+        if (M.sentinels && s.zex < 0) { s.zex = s.zin; s.zex_from_in = true; }
+        if (M.sentinels && qnex > 1000) { qnex = s.qxin * s.nx + s.qyin * s.ny; s.qn_from_in = true; }
         const double qtex = s.qxin * tx + s.qyin * ty;
         s.qxex = (ty * qnex - s.ny * qtex) * s.dn;
         s.qyex = (-tx * qnex + s.nx * qtex) * s.dn;

@@ -4,6 +4,7 @@
 // stage sequence and its hipGraph capture.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -28,6 +29,7 @@ extern "C" void dgr_params_default(dgr_params *p)
     p->rain = 0.0;
     p->wetdry = 0;
     p->tide_code = 0;
+    p->bc_sentinels = 1;
     p->tide_period = 43200.0;
     p->record_modify = 1;
 }
@@ -62,7 +64,8 @@ static dgr_status free_handle(dgr_handle *h)
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_edges,
                     h->d_einfo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
                     h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts,
-                    h->d_nanparts, h->d_wdn, h->d_hydro, h->d_tape, h->d_tiemax, h->d_tiecnt};
+                    h->d_nanparts, h->d_wdn, h->d_hydro, h->d_tape, h->d_tiemax, h->d_tiecnt,
+                    h->d_cpl_spec, h->d_cpl_vals};
     for (void *p : ptrs)
         if (p) (void)hipFree(p);
     for (hipEvent_t e : h->ev_pool) (void)hipEventDestroy(e);
@@ -161,6 +164,7 @@ static dgr_status build_member(const dgr_mesh_view *mv, const MemberHost &M, int
     D.rain = M.prm.rain; D.cfl = M.prm.cfl; D.min_edge = M.min_edge; D.final_time = M.prm.final_time;
     D.tide_datum = M.prm.tide_datum; D.tide_amp = M.prm.tide_amp; D.tide_period = M.prm.tide_period;
     D.tide_phase = M.prm.tide_phase;
+    D.sentinels = M.prm.bc_sentinels != 0;
     for (int b = 0; b < D.eblks; ++b) T.eblk_mem[D.eblk0 + b] = mi;
     for (int b = 0; b < D.dblks; ++b) T.dblk_mem[D.dblk0 + b] = mi;
     return DGR_OK;
@@ -513,6 +517,72 @@ extern "C" dgr_status dgr_member_set_boundary(dgr_handle *h, int mi, const doubl
     DGR_HIP(hipMemcpyAsync(h->d_bzeta + M.ed_off, bz, M.ned * sizeof(double), hipMemcpyHostToDevice, h->stream));
     DGR_HIP(hipMemcpyAsync(h->d_bqn + M.ed_off, bq, M.ned * sizeof(double), hipMemcpyHostToDevice, h->stream));
     DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_member_get_boundary(dgr_handle *h, int mi, double *bz, double *bq)
+{
+    if (bad_member(h, mi) || !bz || !bq) return DGR_ERR_ARG;
+    const MemberHost &M = h->mem[mi];
+    DGR_HIP(hipMemcpyAsync(bz, h->d_bzeta + M.ed_off, M.ned * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipMemcpyAsync(bq, h->d_bqn + M.ed_off, M.ned * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_get_boundary(dgr_handle *h, double *bz, double *bq)
+{
+    return dgr_member_get_boundary(h, 0, bz, bq);
+}
+
+extern "C" dgr_status dgr_couple_channel_ends(dgr_handle *h, int nends, const int *end_off, const int *edge_member,
+                                              const int *edge_local, const int *end_dir, const double *end_A,
+                                              const double *end_Q, double *ghost_A, double *ghost_Q)
+{
+    if (!h || nends < 0) return DGR_ERR_ARG;
+    if (nends == 0) return DGR_OK;
+    if (!end_off || !end_dir || !end_A || !end_Q || !ghost_A || !ghost_Q || end_off[0] != 0) return DGR_ERR_ARG;
+    for (int e = 0; e < nends; ++e)
+        if (end_off[e + 1] < end_off[e] || (end_dir[e] != 1 && end_dir[e] != -1)) return DGR_ERR_ARG;
+    const int ne = end_off[nends];
+    if (ne > 0 && (!edge_member || !edge_local)) return DGR_ERR_ARG;
+    for (int j = 0; j < ne; ++j)
+        if (bad_member(h, edge_member[j]) || edge_local[j] < 0 || edge_local[j] >= h->mem[edge_member[j]].ned)
+            return DGR_ERR_ARG;
+    // spec: end_off, members, local edges, directions; vals: A, Q, then the
+    // ghost values come back in the same buffer
+    const size_t ns = (size_t)nends + 1 + 2 * (size_t)ne + nends, nv = 4 * (size_t)nends;
+    if (h->cpl_spec_n < ns) {
+        if (h->d_cpl_spec) DGR_HIP(hipFree(h->d_cpl_spec));
+        h->d_cpl_spec = nullptr;
+        h->cpl_spec_n = 0;
+        DGR_HIP(dalloc(&h->d_cpl_spec, ns));
+        h->cpl_spec_n = ns;
+    }
+    if (h->cpl_vals_n < nv) {
+        if (h->d_cpl_vals) DGR_HIP(hipFree(h->d_cpl_vals));
+        h->d_cpl_vals = nullptr;
+        h->cpl_vals_n = 0;
+        DGR_HIP(dalloc(&h->d_cpl_vals, nv));
+        h->cpl_vals_n = nv;
+    }
+    std::vector<int> spec(ns);
+    std::vector<double> vals(nv);
+    std::copy(end_off, end_off + nends + 1, spec.begin());
+    std::copy(edge_member, edge_member + ne, spec.begin() + nends + 1);
+    std::copy(edge_local, edge_local + ne, spec.begin() + nends + 1 + ne);
+    std::copy(end_dir, end_dir + nends, spec.begin() + nends + 1 + 2 * ne);
+    std::copy(end_A, end_A + nends, vals.begin());
+    std::copy(end_Q, end_Q + nends, vals.begin() + nends);
+    hipStream_t s = h->stream;
+    DGR_HIP(hipMemcpyAsync(h->d_cpl_spec, spec.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_HIP(hipMemcpyAsync(h->d_cpl_vals, vals.data(), 2 * nends * sizeof(double), hipMemcpyHostToDevice, s));
+    double *ghost = h->d_cpl_vals + 2 * (size_t)nends;
+    launch_couple_ends(h->m, h->d_w, h->d_bzeta, h->d_bqn, nends, h->d_cpl_spec, h->d_cpl_vals, ghost, s);
+    DGR_HIP(hipGetLastError());
+    DGR_HIP(hipMemcpyAsync(ghost_A, ghost, nends * sizeof(double), hipMemcpyDeviceToHost, s));
+    DGR_HIP(hipMemcpyAsync(ghost_Q, ghost + nends, nends * sizeof(double), hipMemcpyDeviceToHost, s));
+    DGR_HIP(hipStreamSynchronize(s));
     return DGR_OK;
 }
 

@@ -90,7 +90,7 @@ struct MemberDev {
     int tide_code;
     int hyd_n[3], hyd_off[3];  // hydrograph of boundary code c+1: hyd_n points at
                                // MeshDev::hydro + hyd_off (t, then zeta_b, then qn_b)
-    int pad;
+    int sentinels;             // prescribed edges: bzeta < 0 / bQn > 1000 take the interior value
     double rain, cfl, min_edge, final_time;
     double tide_datum, tide_amp, tide_period, tide_phase;
     double infil_K, infil_P;  // Green-Ampt conductivity and psi * delta_theta (K = 0: off)

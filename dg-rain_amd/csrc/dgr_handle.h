@@ -23,6 +23,8 @@ void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hip
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
                       hipStream_t s);
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
+void launch_couple_ends(const MeshDev &m, const double *w, double *bzeta, double *bqn, int nends, const int *spec,
+                        const double *vals, double *ghost, hipStream_t s);
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
                      StepRec *rec_end = nullptr, StepRec *rec_next = nullptr, bool ties = false);
 void launch_lam_reduce(const MeshDev &m, const PhysDev &ph, Ctrl *c, int slot, int kind, hipStream_t s);
@@ -118,6 +120,10 @@ struct dgr_handle {
     // calls with the same sizes)
     char *d_tape = nullptr;
     size_t tape_bytes = 0;
+    // channel-coupling staging (dgr_couple_channel_ends), grown on demand
+    int *d_cpl_spec = nullptr;
+    double *d_cpl_vals = nullptr;
+    size_t cpl_spec_n = 0, cpl_vals_n = 0;
 };
 
 #define DGR_HIP(call)                                                                   \

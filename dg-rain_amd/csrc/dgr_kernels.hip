@@ -484,8 +484,8 @@ __device__ __forceinline__ void element_fluxes(const MeshDev &m, const PhysDev &
             const int code = (inf >> EI_CODE_SHIFT) & 0xff;
             double qnex;
             prescribed_values(M, ctrl, code, bzeta, bqn, ed, zex, qnex);
-            if (zex < 0) zex = zin;
-            if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+            if (M.sentinels && zex < 0) zex = zin;
+            if (M.sentinels && qnex > 1000) qnex = qxin * nx + qyin * ny;
             const double qtex = qxin * tx + qyin * ty;
             qxex = (ty * qnex - ny * qtex) * dn;
             qyex = (-tx * qnex + nx * qtex) * dn;
@@ -879,8 +879,8 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
             if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
                 zex = in.bz;
                 double qnex = in.bq;
-                if (zex < 0) zex = zin;
-                if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+                if (M.sentinels && zex < 0) zex = zin;
+                if (M.sentinels && qnex > 1000) qnex = qxin * nx + qyin * ny;
                 const double qtex = qxin * tx + qyin * ty;
                 qxex = (ty * qnex - ny * qtex) * dn;
                 qyex = (-tx * qnex + nx * qtex) * dn;
@@ -1671,6 +1671,67 @@ void launch_step_mark(const MeshDev &m, Ctrl *c, double dt, hipStream_t s)
 void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s)
 {
     hipLaunchKernelGGL(k_reset_maxlam, member_grid(m), dim3(64), 0, s, m, c);
+}
+
+// ChanNet's channel-junction coupling, junction side (ChanNet/internal_BC.c:
+// 16-217), one thread per channel end, its edges in the list's order (the
+// order of the reference's sums).  spec: end_off[nends+1], the ends' edges'
+// members and member-local edge numbers, end_dir[nends]; vals: end_A, end_Q.
+// Writes the edges' bzeta/bqn and ghost[e] (area), ghost[nends+e] (discharge).
+__global__ __launch_bounds__(64) void k_couple_ends(MeshDev m, const double *__restrict__ w, double *bzeta,
+                                                    double *bqn, int nends, const int *__restrict__ spec,
+                                                    const double *__restrict__ vals, double *__restrict__ ghost)
+{
+    const int e = blockIdx.x * 64 + threadIdx.x;
+    if (e >= nends) return;
+    const int P = m.pitch;
+    const int *end_off = spec;
+    const int ne = end_off[nends];
+    const int *emem = spec + nends + 1, *eloc = emem + ne, *edir = eloc + ne;
+    const int j0 = end_off[e], j1 = end_off[e + 1], n = j1 - j0;
+    const double A = vals[e], Q = vals[nends + e];
+    const int dir = edir[e];
+    double area = 0, qn = 0;
+    for (int j = j0; j < j1; ++j) {
+        const MemberDev &M = m.mem[emem[j]];
+        const int ed = M.ed_off + eloc[j];
+        const int4 E = m.edges[ed];
+        const int el = E.x, kin = m.einfo[ed] & ED_KIN_MASK;
+        const double4 p1 = m.nodes[m.ev[el]], p2 = m.nodes[m.ev[P + el]], p3 = m.nodes[m.ev[2 * P + el]];
+        const double xv1 = p1.x, xv2 = p2.x, xv3 = p3.x, yv1 = p1.y, yv2 = p2.y, yv3 = p3.y;
+        double x0, x1, y0, y1, len;                                          // internal_BC.c:51-82
+        if (kin == 0) {
+            x0 = xv1; x1 = xv2; y0 = yv1; y1 = yv2;
+            len = sqrt((xv1 - xv2) * (xv1 - xv2) + (yv1 - yv2) * (yv1 - yv2));
+        } else if (kin == 1) {
+            x0 = xv2; x1 = xv3; y0 = yv2; y1 = yv3;
+            len = sqrt((xv2 - xv3) * (xv2 - xv3) + (yv2 - yv3) * (yv2 - yv3));
+        } else {
+            x0 = xv3; x1 = xv1; y0 = yv3; y1 = yv1;
+            len = sqrt((xv1 - xv3) * (xv1 - xv3) + (yv1 - yv3) * (yv1 - yv3));
+        }
+        const double zeta = w[kin * P + el], qx = w[(3 + kin) * P + el], qy = w[(6 + kin) * P + el];
+        const double zj = 0.5 * (m.nodes[E.z].z + m.nodes[E.w].z);            // :87-88
+        const double H = zeta + zj;
+        const double jac = (xv2 - xv1) * (yv3 - yv1) - (xv3 - xv1) * (yv2 - yv1);
+        const int sj = isign(jac);
+        const double nx = sj * (y1 - y0) / len;                              // :92-93
+        const double ny = sj * (x0 - x1) / len;
+        const double qnj = qx * nx + qy * ny;
+        area += len * H;                                                     // :97-98
+        qn += qnj * len;
+        bzeta[ed] = A / (n * len) - zj;                                      // :101-107, :197-203
+        bqn[ed] = (dir > 0 ? Q : -Q) / (n * len);
+    }
+    ghost[e] = area;                                                         // :110-114, :208-214
+    ghost[nends + e] = dir > 0 ? qn : -qn;
+}
+
+void launch_couple_ends(const MeshDev &m, const double *w, double *bzeta, double *bqn, int nends, const int *spec,
+                        const double *vals, double *ghost, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_couple_ends, dim3((unsigned)((nends + 63) / 64)), dim3(64), 0, s, m, w, bzeta, bqn, nends,
+                       spec, vals, ghost);
 }
 
 void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s)

@@ -41,7 +41,7 @@ class Params(C.Structure):
                 ("dt0", C.c_double), ("final_time", C.c_double), ("rain", C.c_double),
                 ("wetdry", C.c_int), ("tide_code", C.c_int),
                 ("tide_datum", C.c_double), ("tide_amp", C.c_double), ("tide_period", C.c_double),
-                ("tide_phase", C.c_double), ("record_modify", C.c_int)]
+                ("tide_phase", C.c_double), ("record_modify", C.c_int), ("bc_sentinels", C.c_int)]
 
 
 class AdjointSpec(C.Structure):
@@ -105,6 +105,9 @@ def load():
         "dgr_member_get_wd": (C.c_int, [vp, C.c_int, ip]),
         "dgr_member_get_modify": (C.c_int, [vp, C.c_int, C.c_int, C.POINTER(C.c_ubyte)]),
         "dgr_member_set_boundary": (C.c_int, [vp, C.c_int, dp, dp]),
+        "dgr_member_get_boundary": (C.c_int, [vp, C.c_int, dp, dp]),
+        "dgr_get_boundary": (C.c_int, [vp, dp, dp]),
+        "dgr_couple_channel_ends": (C.c_int, [vp, C.c_int, ip, ip, ip, ip, dp, dp, dp, dp]),
         "dgr_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
         "dgr_member_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, dp, dp, dp]),
         "dgr_set_infiltration": (C.c_int, [vp, C.c_double, C.c_double, C.c_double, C.c_double]),
@@ -256,6 +259,10 @@ class _Handle:
     def _nmem(self):
         return len(getattr(self, "meshes", None) or [self.mesh])
 
+    def _num_edges(self, member):
+        ms = getattr(self, "meshes", None) or [self.mesh]
+        return ms[member].num_edges
+
     def gauge_run(self, nsteps, sample_every, gauge_el):
         """Run nsteps steps; gauge depths after every sample_every-th step,
         shape [members, nsteps // sample_every, ngauges]."""
@@ -287,6 +294,28 @@ class _Handle:
             raise ValueError("hydrograph arrays differ in length")
         check(self.lib.dgr_member_set_hydrograph(self.h, int(member), int(code), t.size, _dp(t), _dp(z), _dp(q)),
               "dgr_member_set_hydrograph")
+
+    def get_boundary(self, member=0):
+        """(bzeta, bQn) of a member's edges."""
+        ned = self._num_edges(member)
+        a, b = np.empty(ned), np.empty(ned)
+        check(self.lib.dgr_member_get_boundary(self.h, int(member), _dp(a), _dp(b)), "dgr_member_get_boundary")
+        return a, b
+
+    def couple_channel_ends(self, end_off, edge_member, edge_local, end_dir, end_A, end_Q):
+        """ChanNet's internal_BC, junction side, over the members as junctions:
+        sets the coupled edges' bzeta/bQn, returns the ends' (ghost_A, ghost_Q)."""
+        iv = [np.ascontiguousarray(a, np.int32) for a in (end_off, edge_member, edge_local, end_dir)]
+        dv = [np.ascontiguousarray(a, np.float64) for a in (end_A, end_Q)]
+        n = iv[3].size
+        ga, gq = np.zeros(n), np.zeros(n)
+        check(self.lib.dgr_couple_channel_ends(self.h, n, *[_ip(a) for a in iv], *[_dp(a) for a in dv], _dp(ga),
+                                               _dp(gq)), "dgr_couple_channel_ends")
+        return ga, gq
+
+    def oneTimeStep2D(self, t, dt):
+        """oneTimeStep2D with a host dt, every member."""
+        check(self.lib.dgr_one_time_step(self.h, float(t), float(dt)), "dgr_one_time_step")
 
     def set_tape_budget(self, nbytes):
         """Device memory the adjoint may keep taped segments in (< 0: half the
@@ -390,8 +419,6 @@ class Solver(_Handle):
     def PDop2D(self):
         check(self.lib.dgr_pdop(self.h), "dgr_pdop")
 
-    def oneTimeStep2D(self, t, dt):
-        check(self.lib.dgr_one_time_step(self.h, float(t), float(dt)), "dgr_one_time_step")
 
 
 class Ensemble(_Handle):

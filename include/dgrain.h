@@ -81,6 +81,10 @@ typedef struct {
     int tide_code;       /* BdryPrescribed code driven by the tide; 0 = none (extension) */
     double tide_datum, tide_amp, tide_period, tide_phase;
     int record_modify;   /* keep the limiter's per-element activation flags */
+    int bc_sentinels;    /* 1 (default): 2DCode's prescribed-boundary sentinels
+                            (compute2DL.c:152-155): bzeta < 0 or bQn > 1000 take the
+                            interior value; 0: ChanNet's junction edges
+                            (ChanNet/compute2DL.c:154-161) take bzeta/bQn as given */
 } dgr_params;
 
 typedef struct {
@@ -215,6 +219,26 @@ dgr_status dgr_member_set_wd(dgr_handle *h, int member, const int *wd);
 dgr_status dgr_member_get_wd(dgr_handle *h, int member, int *wd);
 dgr_status dgr_member_get_modify(dgr_handle *h, int member, int stage, unsigned char *modify);
 dgr_status dgr_member_set_boundary(dgr_handle *h, int member, const double *bzeta, const double *bqn);
+dgr_status dgr_member_get_boundary(dgr_handle *h, int member, double *bzeta, double *bqn);
+dgr_status dgr_get_boundary(dgr_handle *h, double *bzeta, double *bqn);
+
+/* ChanNet's channel-junction coupling, junction side (extension; the
+ * reference's internal_BC, ChanNet/internal_BC.c:16-217), for junctions held
+ * as the members of this handle (ChanNet's 2D junction step: params.wetdry = 1,
+ * params.bc_sentinels = 0).  Channel end e couples to the junction edges
+ * [end_off[e], end_off[e+1]) of edge_member/edge_local (member-local edge
+ * numbers, EdgtoVert order); end_dir[e] = +1 for a channel's first node (an
+ * inflow end: end_A/end_Q are its A[1], Q[1]), -1 for its last node (an
+ * outflow end: A[2N-2], Q[2N-2]).  Each coupled edge takes
+ * bzeta = A/(n len) - z_edge and bQn = +-Q/(n len) (n = the end's edge
+ * count, len = the edge length), and the end's ghost values
+ * (ChanNet's A[0], Q[0] or A[2N-1], Q[2N-1]) come back as
+ * ghost_A[e] = sum of len H and ghost_Q[e] = +-sum of len qn over its edges,
+ * from the members' current states.  The channels themselves stay with the
+ * caller.  Bit-identical to the reference's internal_BC. */
+dgr_status dgr_couple_channel_ends(dgr_handle *h, int nends, const int *end_off, const int *edge_member,
+                                   const int *edge_local, const int *end_dir, const double *end_A,
+                                   const double *end_Q, double *ghost_A, double *ghost_Q);
 dgr_status dgr_member_set_time(dgr_handle *h, int member, double t, double dt);
 dgr_status dgr_member_set_hydrograph(dgr_handle *h, int member, int code, int npts, const double *t,
                                      const double *zeta_b, const double *qn_b);

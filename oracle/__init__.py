@@ -45,7 +45,7 @@ class _Params(C.Structure):
                 ("wetdry", C.c_int), ("math", C.c_int),
                 ("cfl", C.c_double), ("dt0", C.c_double), ("final_time", C.c_double), ("rain", C.c_double),
                 ("tide_code", C.c_int), ("tide_datum", C.c_double), ("tide_amp", C.c_double),
-                ("tide_period", C.c_double), ("tide_phase", C.c_double)]
+                ("tide_period", C.c_double), ("tide_phase", C.c_double), ("no_sentinels", C.c_int)]
 
 
 def _dp(a):
@@ -96,6 +96,7 @@ def _load_oracle():
             "orc_ga_cumulative": (C.c_double, [C.c_double, C.c_double, C.c_double]),
             "orc_log": (C.c_double, [C.c_double]),
             "orc_log1p": (C.c_double, [C.c_double]),
+            "orc_couple_ends": (C.c_int, [C.POINTER(vp), C.c_int, ip, ip, ip, ip, dp, dp, dp, dp]),
             "orc_one_time_step": (C.c_int, [vp, C.c_double, C.c_double]),
             "orc_run": (C.c_int, [vp, C.c_int, dp, dp]),
             "orc_pow43": (C.c_double, [C.c_double]),
@@ -127,6 +128,22 @@ def dlog(x):
 
 def dlog1p(x):
     return _load_oracle().orc_log1p(float(x))
+
+
+def couple_ends(juncs, end_off, edge_junc, edge_local, end_dir, end_A, end_Q):
+    """ChanNet's internal_BC, junction side, over Oracle objects (one per
+    junction): sets their bzeta/bQn at the coupled edges, returns the ends'
+    (ghost_A, ghost_Q)."""
+    lib = _load_oracle()
+    iv = [np.ascontiguousarray(a, np.int32) for a in (end_off, edge_junc, edge_local, end_dir)]
+    dv = [np.ascontiguousarray(a, np.float64) for a in (end_A, end_Q)]
+    n = iv[3].size
+    ga, gq = np.zeros(n), np.zeros(n)
+    hs = (C.c_void_p * len(juncs))(*[o.h for o in juncs])
+    rc = lib.orc_couple_ends(hs, n, *[_ip(a) for a in iv], *[_dp(a) for a in dv], _dp(ga), _dp(gq))
+    if rc:
+        raise ValueError("a coupled edge is not an edge of its element")
+    return ga, gq
 
 
 def pow43(x):
@@ -166,7 +183,7 @@ class Oracle:
 
     def __init__(self, mesh, math=MATH_DEV, wetdry=0, cfl=0.1, dt0=1e-7, final_time=np.inf, rain=0.0,
                  tide_code=0, tide_datum=0.0, tide_amp=0.0, tide_period=43200.0, tide_phase=0.0,
-                 g=9.81, h0=1e-3, velzero=1e-2, threads=1, **_ignored):
+                 g=9.81, h0=1e-3, velzero=1e-2, threads=1, bc_sentinels=1, **_ignored):
         lib = _load_oracle()
         self.lib = lib
         self.mesh = mesh
@@ -176,7 +193,7 @@ class Oracle:
         m = _Mesh(mesh.num_el, mesh.num_nodes, mesh.num_edges, _dp(x), _dp(y), _dp(z), _dp(n),
                   _ip(ev), _ip(e2v), _ip(e2e), _ip(bd))
         p = _Params(g, h0, velzero, wetdry, math, cfl, dt0, final_time, rain, tide_code, tide_datum,
-                    tide_amp, tide_period, tide_phase)
+                    tide_amp, tide_period, tide_phase, 0 if bc_sentinels else 1)
         self.p = p
         self.h = lib.orc_create(C.byref(m), C.byref(p))
         if threads > 1:
@@ -463,6 +480,10 @@ class RefChan:
             lib.refchan_compute_L.argtypes = [vp, dp, dp, dp]
             for nm in ("refchan_limiter", "refchan_wd_status", "refchan_pdop"):
                 getattr(lib, nm).argtypes = [vp]
+            lib.refchan_channel.restype = vp
+            lib.refchan_channel.argtypes = [C.c_int, dp, dp, dp, C.c_int, ip, C.c_int, ip]
+            lib.refchan_channel_destroy.argtypes = [vp]
+            lib.refchan_internal_bc.argtypes = [C.c_int, C.POINTER(vp), C.c_int, C.POINTER(vp)]
             RefChan._lib = lib
         self.lib = lib = RefChan._lib
         self.mesh = mesh
@@ -507,6 +528,39 @@ class RefChan:
 
     def SlopeLimiter(self):
         self.lib.refchan_limiter(self.j)
+
+    def get_boundary(self):
+        return self.a["bz"].copy(), self.a["bq"].copy()
+
+    @staticmethod
+    def internal_bc(juncs, end_off, edge_junc, edge_local, end_dir, end_A, end_Q):
+        """The reference's internal_BC over RefChan junctions, one two-node
+        channel per end (its A[1], Q[1] or A[2], Q[2] the end values): sets
+        the junctions' bzeta/bQn, returns the ends' (ghost_A, ghost_Q)."""
+        lib = RefChan._lib
+        dp, ip = C.POINTER(C.c_double), C.POINTER(C.c_int)
+        keep, chans = [], []
+        for e in range(len(end_dir)):
+            j0, j1 = int(end_off[e]), int(end_off[e + 1])
+            lst = np.array([[edge_local[j], edge_junc[j]] for j in range(j0, j1)], np.int32).ravel()
+            A, Q, b = np.zeros(4), np.zeros(4), np.ones(2)
+            at = 1 if end_dir[e] > 0 else 2
+            A[at], Q[at] = end_A[e], end_Q[e]
+            keep += [lst, A, Q, b]
+            n = j1 - j0
+            if end_dir[e] > 0:
+                c = lib.refchan_channel(2, _dp(A), _dp(Q), _dp(b), n, lst.ctypes.data_as(ip), 0, None)
+            else:
+                c = lib.refchan_channel(2, _dp(A), _dp(Q), _dp(b), 0, None, n, lst.ctypes.data_as(ip))
+            chans.append(c)
+        cs = (C.c_void_p * len(chans))(*chans)
+        js = (C.c_void_p * len(juncs))(*[j.j for j in juncs])
+        lib.refchan_internal_bc(len(chans), cs, len(juncs), js)
+        ga = np.array([keep[4 * e + 1][0 if end_dir[e] > 0 else 3] for e in range(len(end_dir))])
+        gq = np.array([keep[4 * e + 2][0 if end_dir[e] > 0 else 3] for e in range(len(end_dir))])
+        for c in chans:
+            lib.refchan_channel_destroy(c)
+        return ga, gq
 
     def wetDryStatus2D(self):
         self.lib.refchan_wd_status(self.j)

@@ -648,8 +648,8 @@ int orc_compute_L(orc_state *s)
         if (bp) {                                                                                /* :149-164 */
             zex = s->bzeta[i];
             double qnex = s->bqn[i];
-            if (zex < 0) zex = zin;
-            if (qnex > 1000) qnex = qxin * nx + qyin * ny;
+            if (!s->p.no_sentinels && zex < 0) zex = zin;
+            if (!s->p.no_sentinels && qnex > 1000) qnex = qxin * nx + qyin * ny;
             double qtex = qxin * tx + qyin * ty;
             double dn = 1. / (nx * ty - ny * tx);
             qxex = (ty * qnex - ny * qtex) * dn;
@@ -921,6 +921,65 @@ void orc_pdop(orc_state *s)
             }
         }
     }
+}
+
+/* ------------------------------------------------------------------------ */
+/* ChanNet's channel-junction coupling, junction side (ChanNet/internal_BC.c */
+/* :16-217): for each channel end e, the junction edges [end_off[e],         */
+/* end_off[e+1]) (junction edge_junc[j], edge edge_local[j]) take           */
+/* bzeta = A/(n len) - z_edge and bQn = +-Q/(n len) from the channel's      */
+/* values next to the junction (end_A, end_Q; end_dir +1 for the channel's  */
+/* first node, an inflow end, -1 for its last node), and the end's ghost    */
+/* values are the sums over its edges of len H and of len qn (negated for   */
+/* an outflow end), in edge order.  -1 if an edge is not an edge of its     */
+/* el_in (the reference then reads an unset local edge).                     */
+/* ------------------------------------------------------------------------ */
+int orc_couple_ends(orc_state *const *juncs, int nends, const int *end_off, const int *edge_junc,
+                    const int *edge_local, const int *end_dir, const double *end_A, const double *end_Q,
+                    double *ghost_A, double *ghost_Q)
+{
+    for (int e = 0; e < nends; ++e) {
+        const int j0 = end_off[e], j1 = end_off[e + 1], n = j1 - j0;
+        double area = 0, qn = 0;
+        for (int j = j0; j < j1; ++j) {
+            orc_state *s = juncs[edge_junc[j]];
+            const orc_mesh *m = &s->m;
+            const int ed = edge_local[j];
+            const int el = m->edg_to_els[2 * ed];                                   /* :35 */
+            const int v1 = m->elto_vert[3 * el], v2 = m->elto_vert[3 * el + 1], v3 = m->elto_vert[3 * el + 2];
+            const double xv1 = m->x[v1], xv2 = m->x[v2], xv3 = m->x[v3];
+            const double yv1 = m->y[v1], yv2 = m->y[v2], yv3 = m->y[v3];
+            const int a = m->edg_to_vert[2 * ed], b = m->edg_to_vert[2 * ed + 1];
+            int edg;
+            double x0, x1, y0, y1, len;                                             /* :51-82 */
+            if ((v1 == a && v2 == b) || (v1 == b && v2 == a)) {
+                edg = 0; x0 = xv1; x1 = xv2; y0 = yv1; y1 = yv2;
+                len = sqrt((xv1 - xv2) * (xv1 - xv2) + (yv1 - yv2) * (yv1 - yv2));
+            } else if ((v2 == a && v3 == b) || (v2 == b && v3 == a)) {
+                edg = 1; x0 = xv2; x1 = xv3; y0 = yv2; y1 = yv3;
+                len = sqrt((xv2 - xv3) * (xv2 - xv3) + (yv2 - yv3) * (yv2 - yv3));
+            } else if ((v3 == a && v1 == b) || (v3 == b && v1 == a)) {
+                edg = 2; x0 = xv3; x1 = xv1; y0 = yv3; y1 = yv1;
+                len = sqrt((xv1 - xv3) * (xv1 - xv3) + (yv1 - yv3) * (yv1 - yv3));
+            } else {
+                return -1;
+            }
+            const double zeta = s->zeta[3 * el + edg], qx = s->qx[3 * el + edg], qy = s->qy[3 * el + edg];
+            const double zj = 0.5 * (m->z[a] + m->z[b]);                           /* :87-88 */
+            const double H = zeta + zj;
+            const double jac = (xv2 - xv1) * (yv3 - yv1) - (xv3 - xv1) * (yv2 - yv1);
+            const double nx = orc_sign(jac) * (y1 - y0) / len;                      /* :92-93 */
+            const double ny = orc_sign(jac) * (x0 - x1) / len;
+            const double qnj = qx * nx + qy * ny;
+            area += len * H;                                                        /* :97-98 */
+            qn += qnj * len;
+            s->bzeta[ed] = end_A[e] / (n * len) - zj;                               /* :101-107 */
+            s->bqn[ed] = (end_dir[e] > 0 ? end_Q[e] : -end_Q[e]) / (n * len);       /* :203 for outflow */
+        }
+        ghost_A[e] = area;                                                          /* :110-114, :208-214 */
+        ghost_Q[e] = end_dir[e] > 0 ? qn : -qn;
+    }
+    return 0;
 }
 
 /* ------------------------------------------------------------------------ */

@@ -51,6 +51,7 @@ typedef struct {
     double rain;             /* m/s, added to every RHSZeta DOF (extension) */
     int tide_code;           /* BdryPrescribed code whose bzeta follows the tide; 0 = none */
     double tide_datum, tide_amp, tide_period, tide_phase;
+    int no_sentinels;        /* 1: prescribed edges take bzeta/bQn as given (ChanNet/compute2DL.c:154-161) */
 } orc_params;
 
 typedef struct orc_state orc_state;
@@ -95,6 +96,12 @@ double orc_hydro_eval(int n, const double *t, const double *v, double time);
  * orc_run takes the step's mean rate (F(t+dt) - F(t))/dt off the RHS of zeta
  * of every element not flagged dry.  K <= 0 turns it off.  0 or -1. */
 int orc_set_infiltration(orc_state *s, double K, double psi, double theta_e, double s_e);
+/* ChanNet's channel-junction coupling, junction side (ChanNet/internal_BC.c:16-217):
+ * sets the coupled junction edges' bzeta/bQn from the channel end values and
+ * returns the ends' ghost area and discharge.  0 or -1. */
+int orc_couple_ends(orc_state *const *juncs, int nends, const int *end_off, const int *edge_junc,
+                    const int *edge_local, const int *end_dir, const double *end_A, const double *end_Q,
+                    double *ghost_A, double *ghost_Q);
 double orc_ga_cumulative(double K, double P, double t);
 double orc_log(double x);
 double orc_log1p(double x);

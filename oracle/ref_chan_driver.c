@@ -51,5 +51,35 @@ double refchan_compute_L(struct junction *j, double *rz, double *rqx, double *rq
 }
 
 void refchan_limiter(struct junction *j) { SlopeLimiter(j); }
+
+/* The channel network globals of ChanNet/main.c:19-22 and the reference's
+ * coupling (ChanNet/internal_BC.c:16-217) over them.  A channel here carries
+ * only what internal_BC reads: its node count, A, Q, the widths, and the
+ * junction edge lists of its two ends (all borrowed from the caller). */
+int NumChannels, NumJunctions;
+struct channel **ChannelList;
+struct junction **JunctionList;
+void internal_BC(void);
+
+struct channel *refchan_channel(int num_nodes, double *A, double *Q, double *b, int n_in, int *in_edges, int n_out,
+                                int *out_edges)
+{
+    struct channel *c = calloc(1, sizeof(struct channel));
+    c->NumNodes = num_nodes;
+    c->NumEl = num_nodes - 1;
+    c->A = A; c->Q = Q; c->b = b;
+    c->NumInflowJunctionEdges = n_in; c->InflowJunctionEdges = in_edges;
+    c->NumOutflowJunctionEdges = n_out; c->OutflowJunctionEdges = out_edges;
+    return c;
+}
+
+void refchan_channel_destroy(struct channel *c) { free(c); }
+
+void refchan_internal_bc(int nchan, struct channel **chans, int njunc, struct junction **juncs)
+{
+    NumChannels = nchan; ChannelList = chans;
+    NumJunctions = njunc; JunctionList = juncs;
+    internal_BC();
+}
 void refchan_wd_status(struct junction *j) { wetDryStatus2D(j); }
 void refchan_pdop(struct junction *j) { PDop2D(j); }

@@ -61,3 +61,27 @@ def rel_l2(a, b):
     d = np.linalg.norm(a - b)
     n = np.linalg.norm(b)
     return d / n if n > 0 else d
+
+
+def coupling_network(seed):
+    """Two junction meshes and four channel ends: two inflow ends (one edge
+    each, and three edges), two outflow ends (two edges on the first
+    junction, one on the second), over boundary and interior edges, with
+    random junction states (surface below and above the datum) and end
+    values.  Returns (meshes, states, end spec)."""
+    rng = np.random.default_rng(seed)
+    meshes = [perturbed_mesh(N=5, seed=seed), perturbed_mesh(N=6, seed=seed + 1)]
+    states = []
+    for i, m in enumerate(meshes):
+        z, qx, qy = random_state(m, seed=300 + 10 * seed + i, dz=0.3, q=0.2)
+        states.append((z - 0.2, qx, qy))
+    bd = [np.nonzero(np.array(m.edg_to_els[0::2]) == np.array(m.edg_to_els[1::2]))[0] for m in meshes]
+    edges = [(0, int(bd[0][0]))], [(1, int(bd[1][3])), (1, int(bd[1][4])), (0, 7)], \
+        [(0, int(bd[0][5])), (0, int(bd[0][6]))], [(1, int(bd[1][9]))]
+    end_off = np.cumsum([0] + [len(e) for e in edges])
+    edge_junc = np.array([j for e in edges for j, _ in e], np.int32)
+    edge_local = np.array([k for e in edges for _, k in e], np.int32)
+    end_dir = np.array([1, 1, -1, -1], np.int32)
+    end_A = rng.uniform(0.5, 3.0, 4)
+    end_Q = rng.uniform(-1.0, 1.0, 4)
+    return meshes, states, (end_off, edge_junc, edge_local, end_dir, end_A, end_Q)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import perturbed_mesh, random_state, same
+from helpers import coupling_network, perturbed_mesh, random_state, same
 from conftest import require_ref
 
 pytestmark = pytest.mark.ref
@@ -94,6 +94,32 @@ def test_wetdry_flux_against_channet():
         assert all(same(a, b) for a, b in zip(r1, r2)) and l1 == l2
 
 
+def test_junction_boundaries_against_channet():
+    """ChanNet's junction edges (ChanNet/compute2DL.c:154-161) take bzeta and
+    bQn as given, with no sentinels: the oracle with bc_sentinels=0 against
+    ChanNet's compiled compute2DL on meshes with prescribed edges whose
+    values include negative surfaces and |qn| > 1000, on wet elements."""
+    for seed in range(3):
+        m = perturbed_mesh(N=8, seed=seed)
+        st = random_state(m, seed=seed + 60, dz=0.2, q=0.3)
+        rng = np.random.default_rng(seed)
+        bz = rng.uniform(-0.5, 0.5, m.num_edges)
+        bq = rng.uniform(-2000.0, 2000.0, m.num_edges)
+        ch = oracle.RefChan(m)
+        o = oracle.Oracle(m, math=oracle.MATH_REF, wetdry=1, bc_sentinels=0)
+        wd = np.ones(m.num_el, np.int32)
+        for impl in (ch, o):
+            impl.set_state(*st)
+            impl.set_wd(wd)
+        ch.a["bz"][:] = bz
+        ch.a["bq"][:] = bq
+        o.set_boundary(bz, bq)
+        assert np.any(np.array(m.bdry) > 0)
+        r1, l1 = ch.compute2DL()
+        r2, _, l2 = o.compute2DL()
+        assert all(same(a, b) for a, b in zip(r1, r2)) and l1 == l2
+
+
 def test_topology_random_meshes(tmp_path):
     from dgrain import mesh as M
     for N in (4, 9):
@@ -105,3 +131,22 @@ def test_topology_random_meshes(tmp_path):
         assert np.array_equal(t["edg_to_els"], m.edg_to_els)
         assert np.array_equal(t["bdry"], m.bdry)
         assert t["min_edge_length"] == m.min_edge_length
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_channel_coupling_against_reference(seed):
+    """The junction side of ChanNet's internal_BC: coupled edges' bzeta/bQn
+    and the ends' ghost area and discharge, bit for bit."""
+    meshes, states, spec = coupling_network(seed)
+    refs, ors = [oracle.RefChan(m) for m in meshes], [oracle.Oracle(m, math=oracle.MATH_REF) for m in meshes]
+    for r, o, st, m in zip(refs, ors, states, meshes):
+        r.set_state(*st)
+        o.set_state(*st)
+        o.set_boundary(np.zeros(m.num_edges), np.zeros(m.num_edges))   # RefChan starts at 0
+    ga_r, gq_r = oracle.RefChan.internal_bc(refs, *spec)
+    ga_o, gq_o = oracle.couple_ends(ors, *spec)
+    assert same(ga_r, ga_o) and same(gq_r, gq_o)
+    for r, o in zip(refs, ors):
+        for a, b in zip(r.get_boundary(), o.get_boundary()):
+            assert same(a, b)
+    assert np.any(ors[0].get_boundary()[0] < 0)   # a below-datum surface: no sentinel applies here
