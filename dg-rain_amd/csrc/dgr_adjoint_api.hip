@@ -315,6 +315,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     // reverse.  Taped steps of the edge-kernel variant recorded their
     // max-|lambda| tie counts (h->kind[1] is the variant's partials space)
     const bool rec_ties = h->kind[1] == PARTS_EDGE;
+    AdjScalarArgs pending{};
     DGR_HIP(hipMemsetAsync(wb, 0, Wb, s));
     DGR_HIP(hipMemsetAsync(zvb, 0, 3 * P * sizeof(double), s));
     DGR_HIP(hipMemsetAsync(nvb, 0, 3 * P * sizeof(double), s));
@@ -351,8 +352,10 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             const double *nm1 = T.nm + 6 * nn * (2 * k), *nm2 = T.nm + 6 * nn * (2 * k + 1);
             const int *na1 = T.na + 6 * nn * (2 * k), *na2 = T.na + 6 * nn * (2 * k + 1);
             // stage 2: input p1, base w0, output u2 -> limited -> w^{n+1} (adjoint wb + slot1)
+            // (the scalar pass of step n+1 rides on this launch)
             launch_adj_limit(h->m, h->ph, u2, nm2, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
-                             zvb, ub, vmb, actl, s);
+                             zvb, ub, vmb, actl, &pending, s);
+            pending.active = 0;
             launch_adj_node(h->m, vmb, nodeb, s);
             launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
                            dtp + maxparts, s);
@@ -368,13 +371,20 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slot2, zbe2, tdp + maxparts,
                             lam_mode, s);
             // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)
-            launch_adj_limit(h->m, h->ph, u1, nm1, pb, slot2, zbe2, zvb, ub, vmb, actl, s);
+            launch_adj_limit(h->m, h->ph, u1, nm1, pb, slot2, zbe2, zvb, ub, vmb, actl, nullptr, s);
             launch_adj_node(h->m, vmb, nodeb, s);
             launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nodeb, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);
             launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, 0, s);
             have_slot1 = true;
-            launch_adj_scalar(h->m, dtp, tdp, maxparts, rn, n > 0 ? rn - nm : rn, n > 0, rec_ties ? 1 : 0, actl, s);
+            // this step's scalar pass (dt and tide adjoints to the previous
+            // step's max |lambda|): inside the next step's first launch, or
+            // a launch of its own after the last step
+            pending = AdjScalarArgs{dtp, tdp, rn, n > 0 ? rn - nm : rn, maxparts, n > 0 ? 1 : 0, rec_ties ? 1 : 0, 1};
+            if (n == 0) {
+                launch_adj_scalar(h->m, pending, actl, s);
+                pending.active = 0;
+            }
             double *t = wb;
             wb = wb0;
             wb0 = t;

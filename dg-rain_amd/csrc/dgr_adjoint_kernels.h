@@ -54,6 +54,8 @@ __global__ __launch_bounds__(256) void k_node_arg(MeshDev m, const double *__res
     }
 }
 
+__device__ __forceinline__ void adj_scalar_member(const MeshDev &m, int mid, const AdjScalarArgs &sa, AdjCtrl *actl);
+
 // limiter reverse.  U: pre-limit stage output; Sb (+ slotb when given): the
 // adjoint of the limited (and PDop'd) output, slotb being the edge part left
 // by the following stage's k_adj_edge, whose z_edge adjoints (zbe) are moved
@@ -65,9 +67,12 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
                                                    const double *__restrict__ nodemm, const double *__restrict__ Sb,
                                                    const double *__restrict__ slotb, const double *__restrict__ zbe,
                                                    double *__restrict__ zvb, double *__restrict__ Ub,
-                                                   double *__restrict__ vmb, AdjCtrl *actl)
+                                                   double *__restrict__ vmb, AdjCtrl *actl, AdjScalarArgs sa)
 {
     const ElemLane L = elem_lane(m);
+    // the previous step's scalar pass, by each member's first element
+    // workgroup (its inputs are complete: the launches before wrote them)
+    if (sa.active && L.lb == m.mem[L.mid].eblk0) adj_scalar_member(m, L.mid, sa, actl);
     if (!L.valid) return;
     const int e = L.e, P = m.pitch, nn = m.nn;
     int nd[3];
@@ -567,12 +572,11 @@ __global__ __launch_bounds__(256) void k_adj_combine(MeshDev m, double *Dst, con
 // step n (workgroup partials of both stages plus the start-time term), the
 // tide time derivative, and the adjoint of the previous step's max |lambda|
 // (dt_n = cfl minEdgLength / lambda_{n-1} unless the final-time clamp acted).
-__global__ void k_adj_scalar(MeshDev m, const double *__restrict__ dtparts, const double *__restrict__ tideparts,
-                             int maxparts, const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_prev,
-                             int has_prev, int rec_ties, AdjCtrl *actl)
+__device__ __forceinline__ void adj_scalar_member(const MeshDev &m, int mid, const AdjScalarArgs &sa, AdjCtrl *actl)
 {
-    const int mid = blockIdx.x;
     const MemberDev &M = m.mem[mid];
+    const double *dtparts = sa.dtparts, *tideparts = sa.tideparts;
+    const int maxparts = sa.maxparts;
     __shared__ double red[2][4];
     double dtb = 0.0, tb = 0.0;
     for (int i = threadIdx.x; i < M.eblks; i += 256)
@@ -583,27 +587,34 @@ __global__ void k_adj_scalar(MeshDev m, const double *__restrict__ dtparts, cons
     for (int o = 32; o > 0; o >>= 1) { dtb += __shfl_xor(dtb, o, 64); tb += __shfl_xor(tb, o, 64); }
     if ((threadIdx.x & 63) == 0) { red[0][threadIdx.x >> 6] = dtb; red[1][threadIdx.x >> 6] = tb; }
     __syncthreads();
-    if (threadIdx.x != 0) return;
-    dtb = red[0][0] + red[0][1] + red[0][2] + red[0][3];
-    tb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
-    AdjCtrl &a = actl[mid];
-    const StepRec r = rec[mid];
-    dtb += a.S;   // t_k = t_0 + sum_{j<k} dt_j
-    double tbar = 0.0;
-    if (M.tide_code) {
-        const double w = 6.283185307179586 / M.tide_period;
-        tbar = tb * M.tide_amp * w * cos(w * r.t + M.tide_phase);
+    if (threadIdx.x == 0) {
+        dtb = red[0][0] + red[0][1] + red[0][2] + red[0][3];
+        tb = red[1][0] + red[1][1] + red[1][2] + red[1][3];
+        AdjCtrl &a = actl[mid];
+        const StepRec r = sa.rec[mid];
+        dtb += a.S;   // t_k = t_0 + sum_{j<k} dt_j
+        double tbar = 0.0;
+        if (M.tide_code) {
+            const double w = 6.283185307179586 / M.tide_period;
+            tbar = tb * M.tide_amp * w * cos(w * r.t + M.tide_phase);
+        }
+        a.S += tbar;
+        // the max-|lambda| adjoint this step's edges shared needs a recorded tie
+        // count; a fold that saw a NaN operand has none
+        if (sa.rec_ties && a.lamb != 0.0 && r.ntie <= 0) a.unsupported = 1;
+        a.lamb = 0.0;
+        a.ntie = 0;
+        if (sa.has_prev) {
+            const double lam = __longlong_as_double((long long)sa.rec_prev[mid].lam2);
+            if (r.dt == M.cfl * M.min_edge / lam) a.lamb = -r.dt / lam * dtb;
+        }
     }
-    a.S += tbar;
-    // the max-|lambda| adjoint this step's edges shared needs a recorded tie
-    // count; a fold that saw a NaN operand has none
-    if (rec_ties && a.lamb != 0.0 && r.ntie <= 0) a.unsupported = 1;
-    a.lamb = 0.0;
-    a.ntie = 0;
-    if (has_prev) {
-        const double lam = __longlong_as_double((long long)rec_prev[mid].lam2);
-        if (r.dt == M.cfl * M.min_edge / lam) a.lamb = -r.dt / lam * dtb;
-    }
+    __syncthreads();
+}
+
+__global__ void k_adj_scalar(MeshDev m, AdjScalarArgs sa, AdjCtrl *actl)
+{
+    adj_scalar_member(m, blockIdx.x, sa, actl);
 }
 
 // loss gradient of the gauges sampled at this state: h_g = mean over the
@@ -699,10 +710,11 @@ void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nod
 
 void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
                       const double *slotb, const double *zbe, double *zvb, double *Ub, double *vmb, AdjCtrl *actl,
-                      hipStream_t s)
+                      const AdjScalarArgs *sa, hipStream_t s)
 {
+    const AdjScalarArgs none{};
     hipLaunchKernelGGL(k_adj_limit, dim3((unsigned)(m.pitch / 256)), dim3(256), 0, s, m, ph, U, nodemm, Sb, slotb, zbe,
-                       zvb, Ub, vmb, actl);
+                       zvb, Ub, vmb, actl, sa ? *sa : none);
 }
 
 void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s)
@@ -752,12 +764,9 @@ void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, cons
     hipLaunchKernelGGL(k_adj_combine, dim3((unsigned)(m.pitch / 256)), dim3(256), 0, s, m, Dst, slotb, zbe, zvb);
 }
 
-void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
-                       const StepRec *rec, const StepRec *rec_prev, int has_prev, int rec_ties, AdjCtrl *actl,
-                       hipStream_t s)
+void launch_adj_scalar(const MeshDev &m, const AdjScalarArgs &sa, AdjCtrl *actl, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_adj_scalar, dim3((unsigned)m.nmem), dim3(256), 0, s, m, dtparts, tideparts, maxparts, rec,
-                       rec_prev, has_prev, rec_ties, actl);
+    hipLaunchKernelGGL(k_adj_scalar, dim3((unsigned)m.nmem), dim3(256), 0, s, m, sa, actl);
 }
 
 void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,

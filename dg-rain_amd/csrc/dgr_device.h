@@ -194,6 +194,16 @@ struct AdjCtrl {
     int unsupported; // set if the forward took a branch the reverse does not cover
 };
 
+// the reverse's per-step scalar pass (adj_scalar_member) for one step,
+// carried by the next stage-2 limiter reverse launch (active = 0: none)
+struct AdjScalarArgs {
+    const double *dtparts;          // [2][maxparts] dt adjoint partials of the step's RHS reverses
+    const double *tideparts;        // [2][maxparts] tide adjoint partials of its edge reverses
+    const struct StepRec *rec;      // the step's record
+    const struct StepRec *rec_prev; // the step before's
+    int maxparts, has_prev, rec_ties, active;
+};
+
 // per-step record of the forward pass, per member
 struct StepRec {
     double t, dt, tide;         // at the step's start

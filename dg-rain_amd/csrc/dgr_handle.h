@@ -36,7 +36,7 @@ void launch_math_selftest(int n, const double *x, const double *y, double *out, 
 void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nodearg, hipStream_t s);
 void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
                       const double *slotb, const double *zbe, double *zvb, double *Ub, double *vmb, AdjCtrl *actl,
-                      hipStream_t s);
+                      const AdjScalarArgs *sa, hipStream_t s);
 void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s);
 void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
                     const double *U, const double *Ubown, const double *nodeb, const int *nodearg, const int *wd,
@@ -49,9 +49,7 @@ void launch_adj_ties(const MeshDev &m, const PhysDev &ph, const double *win, con
                      const int *wd, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl, hipStream_t s);
 void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, const double *zbe, double *zvb,
                         hipStream_t s);
-void launch_adj_scalar(const MeshDev &m, const double *dtparts, const double *tideparts, int maxparts,
-                       const StepRec *rec, const StepRec *rec_prev, int has_prev, int rec_ties, AdjCtrl *actl,
-                       hipStream_t s);
+void launch_adj_scalar(const MeshDev &m, const AdjScalarArgs &sa, AdjCtrl *actl, hipStream_t s);
 void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,
                        double *Wb, double *zvb, hipStream_t s);
 void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
