@@ -231,7 +231,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     };
     // reverse buffers: slot/zbe pairs per stage (the edge part of a stage's
     // input adjoint is consumed by the limiter reverse that comes next)
-    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *nodeb, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
+    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
     DGR_HIP(B.alloc(&wb, W9));
     DGR_HIP(B.alloc(&wb0, W9));
     DGR_HIP(B.alloc(&pb, W9));
@@ -242,7 +242,6 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&zbe1, 3 * P));
     DGR_HIP(B.alloc(&zbe2, 3 * P));
     DGR_HIP(B.alloc(&vmb, 18 * P));
-    DGR_HIP(B.alloc(&nodeb, 6 * nn));
     DGR_HIP(B.alloc(&zvb, 3 * P));
     DGR_HIP(B.alloc(&nvb, 3 * P));
     DGR_HIP(B.alloc(&dtp, 2 * (size_t)maxparts));
@@ -356,8 +355,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             launch_adj_limit(h->m, h->ph, u2, nm2, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
                              zvb, ub, vmb, actl, &pending, s);
             pending.active = 0;
-            launch_adj_node(h->m, vmb, nodeb, s);
-            launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nodeb, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
+            launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, vmb, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
                            dtp + maxparts, s);
             // the max-|lambda| adjoint (k_adj_scalar of step n+1) is shared
             // by the edges whose operand attains the recorded max; the taped
@@ -372,8 +370,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
                             lam_mode, s);
             // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)
             launch_adj_limit(h->m, h->ph, u1, nm1, pb, slot2, zbe2, zvb, ub, vmb, actl, nullptr, s);
-            launch_adj_node(h->m, vmb, nodeb, s);
-            launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nodeb, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
+            launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, vmb, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);
             launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, 0, s);
             have_slot1 = true;

@@ -9,8 +9,8 @@
 //                   element that attains each (the limiter's active bounds)
 //   k_adj_limit     limiter pass reverse: adjoint of the pre-limit output
 //                   (own part) and of the node bounds at the element's vertices
-//   k_adj_node      node sums of the bound adjoints
-//   k_adj_rhs       bound adjoints -> element means -> stage output; RK update
+//   k_adj_rhs       node sums of the bound adjoints, gathered by the element
+//                   attaining each bound -> element means -> stage output; RK update
 //                   and element RHS reverse: adjoint of the stage input, of
 //                   Fhat, of dt (per-workgroup partial) and of nodal z, n
 //   k_adj_edge      Roe flux reverse per edge, written to the two element
@@ -155,25 +155,37 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
     }
 }
 
-// node sums of the bound adjoints: nodeb[f][i] (min) and nodeb[3+f][i] (max)
-__global__ __launch_bounds__(256) void k_adj_node(MeshDev m, const double *__restrict__ vmb, double *__restrict__ nodeb)
+// node sums of the bound adjoints of node nd: s[f] (min) and s[3+f] (max)
+// over the node's elements in n2e order.  Entries are taken four at a time
+// with clamped indices, so every load of a group is issued before the first
+// add; an entry past the end leaves the sums as they are.
+__device__ __forceinline__ void node_bound_sums(const MeshDev &m, const double *__restrict__ vmb, int b, int en,
+                                                double s[6])
 {
-    const int i = xcd_block() * 256 + (int)threadIdx.x;
-    if (i >= m.nn) return;
-    const int P = m.pitch, nn = m.nn;
-    double s[6] = {0, 0, 0, 0, 0, 0};
-    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
-    for (int j = b; j < en; ++j) {
-        const int v = m.n2e[j];
-        const int el = v & N2E_EL_MASK, k = (unsigned)v >> N2E_K_SHIFT;
+    const int P = m.pitch;
 #pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            s[f] += vmb[(f * 3 + k) * P + el];
-            s[3 + f] += vmb[(9 + f * 3 + k) * P + el];
+    for (int f = 0; f < 6; ++f) s[f] = 0.0;
+    for (int j = b; j < en; j += 4) {
+        int v[4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) v[i] = m.n2e[min(j + i, en - 1)];
+        double x[4][6];
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int el = v[i] & N2E_EL_MASK, k = (unsigned)v[i] >> N2E_K_SHIFT;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                x[i][f] = vmb[(f * 3 + k) * P + el];
+                x[i][3 + f] = vmb[(9 + f * 3 + k) * P + el];
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const bool in = j + i < en;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) s[f] = in ? s[f] + x[i][f] : s[f];
         }
     }
-#pragma unroll
-    for (int f = 0; f < 6; ++f) nodeb[f * nn + i] = s[f];
 }
 
 // sum over a workgroup of 256 threads, valid in thread 0
@@ -191,15 +203,16 @@ __device__ __forceinline__ double block_sum(double v)
 //   MODE 1: U = W + R dt             -> Wb += Ub + RHS'(W)^T (dt Ub)
 //   MODE 2: U = (W + B) 0.5 + R dt/2 -> Inb = 0.5 Ub + RHS'(W)^T (dt/2 Ub), Bb = 0.5 Ub
 // W: stage input, B: step-start state (MODE 2), U: pre-limit output, Ubown:
-// own part of the pre-limit adjoint; the node-bound part is added here from
-// nodeb for the elements that attain a bound.  Outputs: Inb (MODE 2: the
+// own part of the pre-limit adjoint; the node-bound part is added here, from
+// the bound adjoints vmb of the limiter reverse, for the elements that attain
+// a bound.  Outputs: Inb (MODE 2: the
 // stage-input adjoint, written; MODE 1: added to Bb, which is the step-start
 // adjoint), Fhb (adjoint of Fhat), zvb/nvb (nodal z/n adjoints per element
 // vertex, added), dtparts (this workgroup's dt adjoint).
 template <int MODE>
 __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const double *__restrict__ W,
                                                     const double *__restrict__ B, const double *__restrict__ U,
-                                                    const double *__restrict__ Ubown, const double *__restrict__ nodeb,
+                                                    const double *__restrict__ Ubown, const double *__restrict__ vmb,
                                                     const int *__restrict__ nodearg, const int *__restrict__ wd,
                                                     const StepRec *__restrict__ rec, double *Inb, double *Bb,
                                                     double *__restrict__ Fhb, double *__restrict__ zvb,
@@ -213,15 +226,33 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
         double w[9], u[9], ub[9];
 #pragma unroll
         for (int f = 0; f < 9; ++f) { w[f] = W[f * P + e]; u[f] = U[f * P + e]; ub[f] = Ubown[f * P + e]; }
-        // the element mean enters the node bounds it attains
+        // the element mean enters the node bounds it attains; a bound's
+        // adjoint is the sum of the bound adjoints (vmb) over the node's
+        // elements, formed here by the element that attains it
+        int nd[3], b[3], en[3];
+        bool own[3][6];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) nd[k] = m.ev[k * P + e];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            const int nd = m.ev[k * P + e];
+            b[k] = m.n2e_off[nd[k]];
+            en[k] = m.n2e_off[nd[k] + 1];
+#pragma unroll
+            for (int f = 0; f < 6; ++f) own[k][f] = nodearg[f * nn + nd[k]] == e;
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            bool any = false;
+#pragma unroll
+            for (int f = 0; f < 6; ++f) any |= own[k][f];
+            if (!any) continue;
+            double sb[6];
+            node_bound_sums(m, vmb, b[k], en[k], sb);
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 double add = 0.0;
-                if (nodearg[f * nn + nd] == e) add += nodeb[f * nn + nd];
-                if (nodearg[(3 + f) * nn + nd] == e) add += nodeb[(3 + f) * nn + nd];
+                if (own[k][f]) add += sb[f];
+                if (own[k][3 + f]) add += sb[3 + f];
                 add *= 1.0 / 3.0;
                 ub[3 * f] += add;
                 ub[3 * f + 1] += add;
@@ -717,22 +748,17 @@ void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, cons
                        zvb, Ub, vmb, actl, sa ? *sa : none);
 }
 
-void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_adj_node, dim3((unsigned)((m.nn + 255) / 256)), dim3(256), 0, s, m, vmb, nodeb);
-}
-
 void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
-                    const double *U, const double *Ubown, const double *nodeb, const int *nodearg, const int *wd,
+                    const double *U, const double *Ubown, const double *vmb, const int *nodearg, const int *wd,
                     const StepRec *rec, double *Inb, double *Bb, double *Fhb, double *zvb, double *nvb,
                     double *dtparts, hipStream_t s)
 {
     const dim3 g((unsigned)(m.pitch / 256)), b(256);
     if (mode == 1)
-        hipLaunchKernelGGL(k_adj_rhs<1>, g, b, 0, s, m, ph, W, B, U, Ubown, nodeb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
+        hipLaunchKernelGGL(k_adj_rhs<1>, g, b, 0, s, m, ph, W, B, U, Ubown, vmb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
                            nvb, dtparts);
     else
-        hipLaunchKernelGGL(k_adj_rhs<2>, g, b, 0, s, m, ph, W, B, U, Ubown, nodeb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
+        hipLaunchKernelGGL(k_adj_rhs<2>, g, b, 0, s, m, ph, W, B, U, Ubown, vmb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
                            nvb, dtparts);
 }
 

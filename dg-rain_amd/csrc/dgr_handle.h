@@ -37,9 +37,8 @@ void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nod
 void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
                       const double *slotb, const double *zbe, double *zvb, double *Ub, double *vmb, AdjCtrl *actl,
                       const AdjScalarArgs *sa, hipStream_t s);
-void launch_adj_node(const MeshDev &m, const double *vmb, double *nodeb, hipStream_t s);
 void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
-                    const double *U, const double *Ubown, const double *nodeb, const int *nodearg, const int *wd,
+                    const double *U, const double *Ubown, const double *vmb, const int *nodearg, const int *wd,
                     const StepRec *rec, double *Inb, double *Bb, double *Fhb, double *zvb, double *nvb,
                     double *dtparts, hipStream_t s);
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
