@@ -85,3 +85,46 @@ def coupling_network(seed):
     end_A = rng.uniform(0.5, 3.0, 4)
     end_Q = rng.uniform(-1.0, 1.0, 4)
     return meshes, states, (end_off, edge_junc, edge_local, end_dir, end_A, end_Q)
+
+
+def delaunay_mesh(n_side=8, n_in=80, L=100.0, seed=0, depth=2.0, codes=True):
+    """Unstructured mesh: the Delaunay triangulation of n_side + 1 evenly
+    spaced nodes per side of the square and n_in random interior nodes (kept
+    at least 0.4 of the mean spacing apart), with node and element numbers
+    shuffled, every element counter-clockwise, node valences of 3 to ~10,
+    random depth and n, and boundary sections of codes 1 (left, lower half),
+    3 (bottom) and 2 (right)."""
+    from scipy.spatial import Delaunay
+    rng = np.random.default_rng(seed)
+    t = np.arange(n_side) * (L / n_side)
+    bx = np.concatenate([t, np.full(n_side, L), L - t, np.zeros(n_side)])
+    by = np.concatenate([np.zeros(n_side), t, np.full(n_side, L), L - t])
+    hmin = 0.4 * L / np.sqrt(n_in + 4 * n_side)
+    px, py = list(bx), list(by)
+    while len(px) < 4 * n_side + n_in:
+        cx, cy = rng.uniform(0.05 * L, 0.95 * L, 2)
+        if np.min(np.hypot(np.asarray(px) - cx, np.asarray(py) - cy)) >= hmin:
+            px.append(cx)
+            py.append(cy)
+    x, y = np.asarray(px), np.asarray(py)
+    tri = Delaunay(np.c_[x, y]).simplices.astype(np.int64)
+    assert np.unique(tri).size == x.size, "a node was left out of the triangulation"
+    area = (x[tri[:, 1]] - x[tri[:, 0]]) * (y[tri[:, 2]] - y[tri[:, 0]]) - \
+        (x[tri[:, 2]] - x[tri[:, 0]]) * (y[tri[:, 1]] - y[tri[:, 0]])
+    assert np.all(np.abs(area) > 1e-6 * hmin * hmin)
+    tri[area < 0] = tri[area < 0][:, [0, 2, 1]]
+    # shuffle node and element numbering
+    perm = rng.permutation(x.size)
+    inv = np.empty_like(perm)
+    inv[perm] = np.arange(x.size)
+    x, y = x[perm], y[perm]
+    tri = inv[tri][rng.permutation(tri.shape[0])]
+    z = depth + rng.uniform(-0.25 * depth, 0.25 * depth, x.size)
+    n = rng.uniform(0.01, 0.05, x.size)
+    kw = {}
+    if codes:
+        def side(k0, k1):   # boundary nodes k0..k1 (in side order) as consecutive pairs
+            ids = inv[np.arange(k0, k1 + 1) % (4 * n_side)]
+            return np.ascontiguousarray(np.stack([ids[:-1], ids[1:]], 1), np.int32).ravel()
+        kw = dict(in1=side(3 * n_side + n_side // 2, 4 * n_side), in2=side(0, n_side), out=side(n_side, 2 * n_side))
+    return M.from_elements(x, y, z, n, tri.astype(np.int32).ravel(), **kw)

@@ -6,7 +6,7 @@ import numpy as np
 import pytest
 
 import oracle
-from helpers import coupling_network, perturbed_mesh, random_state, same
+from helpers import coupling_network, delaunay_mesh, perturbed_mesh, random_state, same
 from conftest import require_ref
 
 pytestmark = pytest.mark.ref
@@ -120,10 +120,41 @@ def test_junction_boundaries_against_channet():
         assert all(same(a, b) for a, b in zip(r1, r2)) and l1 == l2
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_unstructured_mesh_steps(seed):
+    """Delaunay meshes (shuffled numbering, valences 1 to 10, every boundary
+    code): compute2DL, the limiter and 12 wet/dry steps with boundary
+    forcing, bit for bit against the compiled 2DCode and, for the wet/dry
+    flux branches, ChanNet."""
+    m = delaunay_mesh(seed=seed)
+    st = random_state(m, seed=seed + 70, dz=0.1, q=0.1)
+    ref, o = oracle.Ref2D(m), oracle.Oracle(m, math=oracle.MATH_REF)
+    for impl in (ref, o):
+        impl.set_state(*st)
+    r1, f1, l1 = ref.compute2DL()
+    r2, f2, l2 = o.compute2DL()
+    assert all(same(a, b) for a, b in zip(f1 + r1, f2 + r2)) and l1 == l2
+    ref.SlopeLimiter()
+    o.SlopeLimiter()
+    assert all(same(a, b) for a, b in zip(ref.get_state(), o.get_state()))
+    for _ in range(12):
+        ref.oneTimeStep2D(0.0, 0.05)
+        o.oneTimeStep2D(0.0, 0.05)
+    assert all(same(a, b) for a, b in zip(ref.get_state(), o.get_state()))
+    mw = delaunay_mesh(seed=seed, codes=False)
+    wd = np.random.default_rng(seed).integers(0, 2, mw.num_el).astype(np.int32)
+    ch, ow = oracle.RefChan(mw), oracle.Oracle(mw, math=oracle.MATH_REF, wetdry=1)
+    for impl in (ch, ow):
+        impl.set_state(*random_state(mw, seed=seed + 80, dz=0.2, q=0.3))
+        impl.set_wd(wd)
+    r1, l1 = ch.compute2DL()
+    r2, _, l2 = ow.compute2DL()
+    assert all(same(a, b) for a, b in zip(r1, r2)) and l1 == l2
+
+
 def test_topology_random_meshes(tmp_path):
     from dgrain import mesh as M
-    for N in (4, 9):
-        m = perturbed_mesh(N=N, seed=N)
+    for N, m in ((4, perturbed_mesh(N=4, seed=4)), (9, perturbed_mesh(N=9, seed=9)), (0, delaunay_mesh(seed=5))):
         p = str(tmp_path / ("m%d.14" % N))
         M.write_fort14(p, m)
         t = oracle.Ref2D(m).topology_from_fort14(p)
