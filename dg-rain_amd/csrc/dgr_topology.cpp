@@ -2,8 +2,10 @@
 // built in O(N) with hash lookups instead of the reference's O(edges x
 // elements) scans (2DCode/extract_mesh_attributes.c:143-361).  Output is
 // identical: the same edge numbering, EdgtoEls pairs and BdryPrescribed codes.
+#include <algorithm>
 #include <cmath>
 #include <cstdint>
+#include <numeric>
 #include <unordered_map>
 #include <vector>
 
@@ -17,7 +19,73 @@ inline uint64_t edge_key(int a, int b)
     return ((uint64_t)lo << 32) | hi;
 }
 
+// position of cell (cx, cy) along the Hilbert curve through a 2^16 x 2^16
+// grid (the textbook quadrant recursion: at each level take the quadrant's
+// index, then rotate/reflect the cell into that quadrant's frame)
+inline uint64_t hilbert_index(uint32_t cx, uint32_t cy)
+{
+    const uint32_t n = 1u << 16;
+    uint64_t d = 0;
+    for (uint32_t s = n >> 1; s > 0; s >>= 1) {
+        const uint32_t rx = (cx & s) ? 1u : 0u, ry = (cy & s) ? 1u : 0u;
+        d += (uint64_t)s * s * ((3u * rx) ^ ry);
+        if (ry == 0) {
+            if (rx == 1) {
+                cx = n - 1 - cx;
+                cy = n - 1 - cy;
+            }
+            std::swap(cx, cy);
+        }
+    }
+    return d;
+}
+
 }  // namespace
+
+extern "C" dgr_status dgr_locality_order(int num_el, int num_nodes, const int *ev, const double *x, const double *y,
+                                         int *el_order, int *node_order)
+{
+    if (num_el < 0 || num_nodes < 0 || (num_el && (!ev || !el_order)) || (num_nodes && (!x || !y || !node_order)))
+        return DGR_ERR_ARG;
+    for (long long i = 0; i < 3LL * num_el; ++i)
+        if (ev[i] < 0 || ev[i] >= num_nodes) return DGR_ERR_MESH;
+    double x0 = INFINITY, x1 = -INFINITY, y0 = INFINITY, y1 = -INFINITY;
+    for (int i = 0; i < num_nodes; ++i) {
+        if (!std::isfinite(x[i]) || !std::isfinite(y[i])) return DGR_ERR_MESH;
+        x0 = std::min(x0, x[i]);
+        x1 = std::max(x1, x[i]);
+        y0 = std::min(y0, y[i]);
+        y1 = std::max(y1, y[i]);
+    }
+    // one scale for both axes keeps the curve's cells square
+    const double span = std::max(x1 - x0, y1 - y0);
+    const double scale = span > 0 ? 65535.0 / span : 0.0;
+    std::vector<uint64_t> key((size_t)num_el);
+    for (int e = 0; e < num_el; ++e) {
+        const int *v = ev + 3 * (size_t)e;
+        const double cx = (x[v[0]] + x[v[1]] + x[v[2]]) / 3.0, cy = (y[v[0]] + y[v[1]] + y[v[2]]) / 3.0;
+        const double gx = std::min(65535.0, std::max(0.0, (cx - x0) * scale));
+        const double gy = std::min(65535.0, std::max(0.0, (cy - y0) * scale));
+        key[(size_t)e] = hilbert_index((uint32_t)gx, (uint32_t)gy);
+    }
+    std::iota(el_order, el_order + num_el, 0);
+    std::stable_sort(el_order, el_order + num_el, [&](int a, int b) { return key[(size_t)a] < key[(size_t)b]; });
+    // nodes in order of first use; nodes no element uses keep their relative
+    // order at the end
+    std::vector<char> seen((size_t)num_nodes, 0);
+    int k = 0;
+    for (int i = 0; i < num_el; ++i)
+        for (int j = 0; j < 3; ++j) {
+            const int v = ev[3 * (size_t)el_order[i] + j];
+            if (!seen[(size_t)v]) {
+                seen[(size_t)v] = 1;
+                node_order[k++] = v;
+            }
+        }
+    for (int v = 0; v < num_nodes; ++v)
+        if (!seen[(size_t)v]) node_order[k++] = v;
+    return DGR_OK;
+}
 
 extern "C" dgr_status dgr_build_topology(int num_el, int num_nodes, const int *ev, const double *x,
                                          const double *y, int n_in1, const int *in1, int n_in2, const int *in2,

@@ -108,6 +108,73 @@ def nodal_to_midpoints(mesh, nodal):
     return out.ravel()
 
 
+@dataclass
+class Renumbering:
+    """A mesh renumbered by locality_order: new element i is old element
+    el_order[i], new node j is old node node_order[j]."""
+    el_order: np.ndarray
+    node_order: np.ndarray
+
+    def midpoints_to_new(self, a):
+        """Per-midpoint array (el*3+k) of the old numbering in the new one."""
+        return np.ascontiguousarray(np.asarray(a).reshape(-1, 3)[self.el_order].ravel())
+
+    def midpoints_to_old(self, a):
+        out = np.empty_like(np.asarray(a).reshape(-1, 3))
+        out[self.el_order] = np.asarray(a).reshape(-1, 3)
+        return out.ravel()
+
+    def elements_to_new(self, a):
+        return np.ascontiguousarray(np.asarray(a)[self.el_order])
+
+    def elements_to_old(self, a):
+        out = np.empty_like(np.asarray(a))
+        out[self.el_order] = a
+        return out
+
+    def element_ids_to_new(self, ids):
+        """New numbers of old elements (gauge lists and the like)."""
+        inv = np.empty_like(self.el_order)
+        inv[self.el_order] = np.arange(self.el_order.size, dtype=self.el_order.dtype)
+        return inv[np.asarray(ids)]
+
+    def nodes_to_new(self, a):
+        return np.ascontiguousarray(np.asarray(a)[self.node_order])
+
+    def nodes_to_old(self, a):
+        out = np.empty_like(np.asarray(a))
+        out[self.node_order] = a
+        return out
+
+
+def locality_order(mesh):
+    """The mesh renumbered for memory locality (dgr_locality_order: elements
+    along a Hilbert curve through their centroids, nodes in order of first
+    use); returns (mesh, Renumbering).  The edge table keeps its order and
+    each edge its orientation (which element is el_in), with element and
+    node numbers mapped: the reference evaluates each Roe flux in its edge's
+    orientation and restarts its max-|lambda| fold at NaN operands in edge
+    order, so rebuilding the table from the new numbering would change
+    results at the rounding level.  Kept this way, a run on the renumbered
+    mesh is the original run, renumbered."""
+    el_order, node_order = native.locality_order(mesh.elto_vert, mesh.x, mesh.y)
+    inv_n = np.empty_like(node_order)
+    inv_n[node_order] = np.arange(node_order.size, dtype=node_order.dtype)
+    inv_e = np.empty_like(el_order)
+    inv_e[el_order] = np.arange(el_order.size, dtype=el_order.dtype)
+
+    def nodes(a):
+        return np.ascontiguousarray(inv_n[np.asarray(a, np.int64)], np.int32) if np.size(a) else \
+            np.zeros(0, np.int32)
+
+    m = Mesh(mesh.x[node_order], mesh.y[node_order], mesh.z[node_order], mesh.n[node_order],
+             nodes(mesh.elto_vert.reshape(-1, 3)[el_order].ravel()), nodes(mesh.edg_to_vert),
+             np.ascontiguousarray(inv_e[np.asarray(mesh.edg_to_els, np.int64)], np.int32),
+             np.array(mesh.bdry, np.int32), mesh.min_edge_length,
+             nodes(mesh.in1), nodes(mesh.in2), nodes(mesh.out))
+    return m, Renumbering(el_order, node_order)
+
+
 # ---------------------------------------------------------------------------
 # fort.14 variant (extract_mesh_attributes.c:39-141)
 # ---------------------------------------------------------------------------

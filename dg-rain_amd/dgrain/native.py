@@ -122,6 +122,7 @@ def load():
         "dgr_checkpoint_write": (C.c_int, [vp, C.c_int, C.c_char_p]),
         "dgr_checkpoint_read": (C.c_int, [vp, C.c_int, C.c_char_p]),
         "dgr_profile_end": (C.c_int, [vp, dp, ip]),
+        "dgr_locality_order": (C.c_int, [C.c_int, C.c_int, ip, dp, dp, ip, ip]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -161,6 +162,20 @@ def default_params(**kw):
             raise KeyError(k)
         setattr(p, k, v)
     return p
+
+
+def locality_order(elto_vert, x, y):
+    """(el_order, node_order) of dgr_locality_order: new element i is old
+    element el_order[i], new node j is old node node_order[j]."""
+    lib = load()
+    ev = np.ascontiguousarray(elto_vert, dtype=np.int32).ravel()
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    y = np.ascontiguousarray(y, dtype=np.float64)
+    el_order = np.empty(ev.size // 3, np.int32)
+    node_order = np.empty(x.size, np.int32)
+    check(lib.dgr_locality_order(el_order.size, x.size, _ip(ev), _dp(x), _dp(y), _ip(el_order), _ip(node_order)),
+           "dgr_locality_order")
+    return el_order, node_order
 
 
 def build_topology(elto_vert, x, y, in1=None, in2=None, out=None):
