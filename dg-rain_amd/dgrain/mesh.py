@@ -114,6 +114,16 @@ class Renumbering:
     el_order[i], new node j is old node node_order[j]."""
     el_order: np.ndarray
     node_order: np.ndarray
+    edge_order: np.ndarray = None
+
+    def edges_to_new(self, a):
+        """Per-edge array (boundary values) of the old numbering in the new one."""
+        return np.ascontiguousarray(np.asarray(a)[self.edge_order])
+
+    def edges_to_old(self, a):
+        out = np.empty_like(np.asarray(a))
+        out[self.edge_order] = a
+        return out
 
     def midpoints_to_new(self, a):
         """Per-midpoint array (el*3+k) of the old numbering in the new one."""
@@ -147,16 +157,27 @@ class Renumbering:
         return out
 
 
-def locality_order(mesh):
+def locality_order(mesh, edges="keep"):
     """The mesh renumbered for memory locality (dgr_locality_order: elements
     along a Hilbert curve through their centroids, nodes in order of first
-    use); returns (mesh, Renumbering).  The edge table keeps its order and
-    each edge its orientation (which element is el_in), with element and
-    node numbers mapped: the reference evaluates each Roe flux in its edge's
-    orientation and restarts its max-|lambda| fold at NaN operands in edge
-    order, so rebuilding the table from the new numbering would change
-    results at the rounding level.  Kept this way, a run on the renumbered
-    mesh is the original run, renumbered."""
+    use); returns (mesh, Renumbering).  Each edge keeps its orientation
+    (which element is el_in), with element and node numbers mapped: the
+    reference evaluates each Roe flux in its edge's orientation, so
+    rebuilding the table from the new numbering would change results at the
+    rounding level.
+
+    edges="keep": the edge table also keeps its order.  A run on the
+    renumbered mesh is then the original run, renumbered, bit for bit.
+    edges="locality": edges ordered by the first new element that has them
+    (the edge-centric kernels then gather with locality too).  The reference
+    restarts its max-|lambda| fold at NaN operands (both-dry edges) in edge
+    order, so where a step has them its dt can differ from the original
+    run's; elsewhere the run is the original run, bit for bit.  Either way
+    the run is the reference's own run of the renumbered mesh.  The
+    Renumbering holds the edge order too (new edge e is old edge
+    edge_order[e]; the identity for "keep")."""
+    if edges not in ("keep", "locality"):
+        raise ValueError(edges)
     el_order, node_order = native.locality_order(mesh.elto_vert, mesh.x, mesh.y)
     inv_n = np.empty_like(node_order)
     inv_n[node_order] = np.arange(node_order.size, dtype=node_order.dtype)
@@ -167,12 +188,17 @@ def locality_order(mesh):
         return np.ascontiguousarray(inv_n[np.asarray(a, np.int64)], np.int32) if np.size(a) else \
             np.zeros(0, np.int32)
 
+    e2e = inv_e[np.asarray(mesh.edg_to_els, np.int64).reshape(-1, 2)]
+    edge_order = np.arange(e2e.shape[0], dtype=np.int32)
+    if edges == "locality":
+        edge_order = np.argsort(e2e.min(axis=1), kind="stable").astype(np.int32)
+    e2v = np.asarray(mesh.edg_to_vert).reshape(-1, 2)[edge_order]
     m = Mesh(mesh.x[node_order], mesh.y[node_order], mesh.z[node_order], mesh.n[node_order],
-             nodes(mesh.elto_vert.reshape(-1, 3)[el_order].ravel()), nodes(mesh.edg_to_vert),
-             np.ascontiguousarray(inv_e[np.asarray(mesh.edg_to_els, np.int64)], np.int32),
-             np.array(mesh.bdry, np.int32), mesh.min_edge_length,
+             nodes(mesh.elto_vert.reshape(-1, 3)[el_order].ravel()), nodes(e2v.ravel()),
+             np.ascontiguousarray(e2e[edge_order].ravel(), np.int32),
+             np.ascontiguousarray(np.asarray(mesh.bdry)[edge_order], np.int32), mesh.min_edge_length,
              nodes(mesh.in1), nodes(mesh.in2), nodes(mesh.out))
-    return m, Renumbering(el_order, node_order)
+    return m, Renumbering(el_order, node_order, edge_order)
 
 
 # ---------------------------------------------------------------------------

@@ -122,3 +122,29 @@ def test_errors():
         native.locality_order(np.array([0, 1, 2], np.int32), np.array([0.0, np.nan, 0.0]), y)
     eo, no = native.locality_order(np.zeros(0, np.int32), x, y)
     assert eo.size == 0 and np.array_equal(no, [0, 1, 2])
+
+
+def test_locality_edge_order():
+    """edges="locality": edges sorted by their first new element, each with
+    its orientation and code; a wet run with tide and rain (no NaN operand
+    in the max-|lambda| fold) is the original run, bit for bit."""
+    sh = _shuffled(delaunay_mesh(seed=8, n_in=300), seed=4)
+    m, rn = M.locality_order(sh, edges="locality")
+    e_new = np.asarray(m.edg_to_els).reshape(-1, 2)
+    e_old = np.asarray(sh.edg_to_els).reshape(-1, 2)
+    assert np.all(np.diff(e_new.min(axis=1)) >= 0)
+    assert np.array_equal(rn.el_order[e_new], e_old[rn.edge_order])
+    assert np.array_equal(m.bdry, rn.edges_to_new(sh.bdry))
+    assert np.array_equal(rn.edges_to_old(m.bdry), sh.bdry)
+    rng = np.random.default_rng(5)
+    st = [0.05 * rng.standard_normal(3 * sh.num_el) for _ in range(3)]
+    kw = dict(cfl=0.3, wetdry=1, tide_code=2, tide_datum=0.0, tide_amp=0.3, tide_period=600.0, rain=30.0 / 3.6e6)
+    a = oracle.Oracle(sh, math=oracle.MATH_DEV, **kw)
+    b = oracle.Oracle(m, math=oracle.MATH_DEV, **kw)
+    a.set_state(*st)
+    b.set_state(*[rn.midpoints_to_new(x) for x in st])
+    assert a.run(30) == b.run(30)
+    for x, y in zip(a.get_state(), b.get_state()):
+        assert same(x, rn.midpoints_to_old(y))
+    with pytest.raises(ValueError):
+        M.locality_order(sh, edges="sorted")
