@@ -192,9 +192,11 @@ def locality_order(mesh, edges="keep"):
     edge_order = np.arange(e2e.shape[0], dtype=np.int32)
     if edges == "locality":
         edge_order = np.argsort(e2e.min(axis=1), kind="stable").astype(np.int32)
-    e2v = np.asarray(mesh.edg_to_vert).reshape(-1, 2)[edge_order]
+    # each edge's vertex pair in ascending new numbers, as the reference
+    # stores it (its length and z_edge are symmetric in the two vertices)
+    e2v = np.sort(inv_n[np.asarray(mesh.edg_to_vert, np.int64).reshape(-1, 2)[edge_order]], axis=1)
     m = Mesh(mesh.x[node_order], mesh.y[node_order], mesh.z[node_order], mesh.n[node_order],
-             nodes(mesh.elto_vert.reshape(-1, 3)[el_order].ravel()), nodes(e2v.ravel()),
+             nodes(mesh.elto_vert.reshape(-1, 3)[el_order].ravel()), np.ascontiguousarray(e2v.ravel(), np.int32),
              np.ascontiguousarray(e2e[edge_order].ravel(), np.int32),
              np.ascontiguousarray(np.asarray(mesh.bdry)[edge_order], np.int32), mesh.min_edge_length,
              nodes(mesh.in1), nodes(mesh.in2), nodes(mesh.out))

@@ -54,9 +54,10 @@ def test_same_mesh_better_locality(make):
         b = np.asarray(sh.edg_to_els)[k::2]
         assert np.array_equal(rn.el_order[a], b)
         assert np.array_equal(m.x[ev_new[a]], sh.x[ev_old[b]])
-        va, vb = np.asarray(m.edg_to_vert)[k::2], np.asarray(sh.edg_to_vert)[k::2]
-        assert np.array_equal(m.x[va], sh.x[vb]) and np.array_equal(m.y[va], sh.y[vb])
+    pa, pb = np.asarray(m.edg_to_vert).reshape(-1, 2), np.asarray(sh.edg_to_vert).reshape(-1, 2)
+    assert np.array_equal(np.sort(m.x[pa] + 1j * m.y[pa], axis=1), np.sort(sh.x[pb] + 1j * sh.y[pb], axis=1))
     assert np.array_equal(m.bdry, sh.bdry)
+    assert np.all(np.diff(np.asarray(m.edg_to_vert).reshape(-1, 2), axis=1) > 0)   # v0 < v1
     # the table is the one the reference builds for the renumbered elements,
     # up to edge order and orientation
     rebuilt = M.from_elements(m.x, m.y, m.z, m.n, m.elto_vert, m.in1, m.in2, m.out)
