@@ -121,8 +121,9 @@ dgr_status dgr_build_topology(int num_el, int num_nodes, const int *elto_vert,
  * neighbours' states and vertex data, and a mesh numbered without locality
  * (a shuffled fort.14) runs the step ~3x slower than the same mesh in this
  * order.  Apply it once when the mesh is read, mapping EdgtoEls and
- * EdgtoVert through the new numbers but keeping the edge order and each
- * edge's orientation (dgrain.mesh.locality_order does): the run on the
+ * EdgtoVert through the new numbers (each EdgtoVert pair re-sorted
+ * ascending) but keeping the edge order and each edge's (el_in, el_ex)
+ * order (dgrain.mesh.locality_order does): the run on the
  * renumbered mesh is then the original run, bit for bit.  (Edges numbered
  * anew from the new element order would flip some edges' orientation, and
  * the Roe flux evaluated the other way round differs in rounding.)
