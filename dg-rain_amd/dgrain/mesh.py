@@ -111,10 +111,11 @@ def nodal_to_midpoints(mesh, nodal):
 @dataclass
 class Renumbering:
     """A mesh renumbered by locality_order: new element i is old element
-    el_order[i], new node j is old node node_order[j]."""
+    el_order[i], new node j is old node node_order[j], new edge e is old
+    edge edge_order[e]."""
     el_order: np.ndarray
     node_order: np.ndarray
-    edge_order: np.ndarray = None
+    edge_order: np.ndarray
 
     def edges_to_new(self, a):
         """Per-edge array (boundary values) of the old numbering in the new one."""
