@@ -153,8 +153,15 @@ def test_unstructured_mesh_steps(seed):
 
 
 def test_topology_random_meshes(tmp_path):
+    """The O(N) topology against the reference's extract_mesh_attributes on
+    the same fort.14: grids, a Delaunay mesh, and a Delaunay mesh renumbered
+    by locality_order and written back as a fort.14 (boundary sections
+    included)."""
     from dgrain import mesh as M
-    for N, m in ((4, perturbed_mesh(N=4, seed=4)), (9, perturbed_mesh(N=9, seed=9)), (0, delaunay_mesh(seed=5))):
+    r = M.locality_order(delaunay_mesh(seed=6))[0]
+    renumbered = M.from_elements(r.x, r.y, r.z, r.n, r.elto_vert, r.in1, r.in2, r.out)
+    for N, m in ((4, perturbed_mesh(N=4, seed=4)), (9, perturbed_mesh(N=9, seed=9)), (0, delaunay_mesh(seed=5)),
+                 (1, renumbered)):
         p = str(tmp_path / ("m%d.14" % N))
         M.write_fort14(p, m)
         t = oracle.Ref2D(m).topology_from_fort14(p)
