@@ -170,13 +170,15 @@ def locality_order(mesh, edges="keep"):
     edges="keep": the edge table also keeps its order.  A run on the
     renumbered mesh is then the original run, renumbered, bit for bit.
     edges="locality": edges ordered by the first new element that has them
-    (the edge-centric kernels then gather with locality too).  The reference
-    restarts its max-|lambda| fold at NaN operands (both-dry edges) in edge
-    order, so where a step has them its dt can differ from the original
-    run's; elsewhere the run is the original run, bit for bit.  Either way
-    the run is the reference's own run of the renumbered mesh.  The
-    Renumbering holds the edge order too (new edge e is old edge
-    edge_order[e]; the identity for "keep")."""
+    (the edge-centric kernels then gather with locality too), orientation
+    kept.  Without both-dry edges the run is the original run, bit for bit.
+    The reference restarts its max-|lambda| fold at NaN operands (both-dry
+    edges) in edge order, so a step that has them takes the dt of this
+    table's own fold order.  That is neither the original run's dt nor the
+    dt of the reference's run of a renumbered fort.14: extract_mesh_attributes
+    would order the edges by their lowest vertex and make the lowest-numbered
+    element el_in.  The Renumbering holds the edge order too (new edge e is
+    old edge edge_order[e]; the identity for "keep")."""
     if edges not in ("keep", "locality"):
         raise ValueError(edges)
     el_order, node_order = native.locality_order(mesh.elto_vert, mesh.x, mesh.y)

@@ -19,6 +19,43 @@ from helpers import same
 pytestmark = pytest.mark.gpu
 
 
+@pytest.mark.parametrize("N,walls", [(16, True), (512, False)])
+def test_still_coastal_start_fails_as_the_oracle(gpu, N, walls):
+    """The configured still-water coastal start (still water at 0 m over the
+    -10 m .. +2 m bed) cannot be run by the reference's wet/dry scheme: a
+    NaN flux stops it within a few steps.  The device stops in the same step
+    with the oracle's error code and edge, at 16x16 with walls only (the
+    compiled reference's case, tests/golden/still_coastal_failure.npz: step
+    3, edge 95) and at the full 512x512 size with the tide.  A normal error
+    return: nothing on the device faults."""
+    wl = configs.coastal(N=N, start="still")
+    if walls:
+        wl.mesh.bdry[:] = 0
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl, threads=min(16, os.cpu_count() or 1))
+    for step in range(1, 20):
+        try:
+            o.run(1)
+        except RuntimeError as ex:
+            msg = str(ex)
+            break
+        s.run(1)
+        for a, b in zip(s.get_state(), o.get_state()):
+            assert same(a, b), "state differs after %d steps" % step
+    else:
+        pytest.fail("expected the oracle to stop on this input")
+    with pytest.raises(native.DgrError) as ei:
+        s.run(1)
+    assert ei.value.code == 4
+    code, where = msg.split("error ")[1].split(" at ")
+    assert "error %s at %s" % (code, where) in str(ei.value), (msg, str(ei.value))
+    if walls:
+        d = np.load(os.path.join(os.path.dirname(__file__), "golden", "still_coastal_failure.npz"))
+        assert step == int(d["step"]) and ("edge %s," % where) in str(d["message"])
+    print("  %dx%d still start: step %d, %s" % (N, N, step, str(ei.value)), flush=True)
+
+
 def test_coastal_512_flood_bitwise(gpu):
     """The benchmark workload (tide, wet/dry and limiter on, CFL 0.3, flood
     start), 20 steps with dt on the device, every variant's default (the
