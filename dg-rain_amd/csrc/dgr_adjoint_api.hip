@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <mutex>
 #include <vector>
 
 #include "dgr_handle.h"
@@ -73,6 +74,94 @@ extern "C" dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every,
             for (int g = 0; g < ngauges; ++g)
                 h_out[((size_t)mi * nsamp + k) * ngauges + g] = hv[(size_t)k * per + (size_t)mi * ngauges + g];
     return any_error(h);
+}
+
+// ---------------------------------------------------------------------------
+// dgr_gauge_run_noalloc: dgr_gauge_run without device allocation.  Its
+// staging (the gauge element ids, then the sampled depths already in the
+// output's [member][sample][gauge] order) lives in `scratch`, a fixed array
+// in each device's memory for the life of the process.  On the host the
+// array's name is not a device address: each device's copy is found once
+// with hipGetSymbolAddress (that device current) and only that address is
+// handed to copies and kernels.  Every call on a device shares its copy, so
+// calls that overlap on one device race.
+// ---------------------------------------------------------------------------
+static __device__ __align__(256) unsigned char scratch[DGR_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+unsigned char *g_scratch_dev[kMaxDevices];   // this device's copy of `scratch`
+
+dgr_status scratch_of(int device, unsigned char **out)
+{
+    if (device < 0 || device >= kMaxDevices) return DGR_ERR_ARG;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_dev[device]) {
+        int cur = -1;
+        DGR_HIP(hipGetDevice(&cur));
+        if (cur != device) DGR_HIP(hipSetDevice(device));
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (cur != device) (void)hipSetDevice(cur);
+        if (e != hipSuccess || !p) {
+            fprintf(stderr, "libdgrain: hipGetSymbolAddress(scratch) failed: %s\n", hipGetErrorString(e));
+            return DGR_ERR_HIP;
+        }
+        g_scratch_dev[device] = (unsigned char *)p;
+    }
+    *out = g_scratch_dev[device];
+    return DGR_OK;
+}
+}  // namespace
+
+extern "C" dgr_status dgr_gauge_run_noalloc(dgr_handle *h, int nsteps, int sample_every, int ngauges,
+                                            const int *gauge_el, double *h_out)
+{
+    if (!h || nsteps < 0 || sample_every <= 0 || ngauges <= 0 || !gauge_el || !h_out) return DGR_ERR_ARG;
+    for (int mi = 0; mi < h->nmem; ++mi)
+        for (int g = 0; g < ngauges; ++g) {
+            const int e = gauge_el[mi * ngauges + g];
+            if (e < 0 || e >= h->mem[mi].ne) return DGR_ERR_ARG;
+        }
+    const size_t nsamp = (size_t)(nsteps / sample_every);
+    const size_t per = (size_t)h->nmem * ngauges;
+    const size_t id_bytes = (per * sizeof(int) + 255) / 256 * 256;
+    if (id_bytes + per * nsamp * sizeof(double) > DGR_NOALLOC_SCRATCH_BYTES) {
+        fprintf(stderr, "libdgrain: dgr_gauge_run_noalloc: %zu gauge ids and %zu samples exceed its %u-byte workspace\n",
+                per, per * nsamp, (unsigned)DGR_NOALLOC_SCRATCH_BYTES);
+        return DGR_ERR_TOO_LARGE;
+    }
+    unsigned char *base = nullptr;
+    const dgr_status st = scratch_of(h->device, &base);
+    if (st != DGR_OK) return st;
+    int *d_gel = (int *)base;
+    double *d_h = (double *)(base + id_bytes);
+    DGR_HIP(hipMemcpyAsync(d_gel, gauge_el, per * sizeof(int), hipMemcpyHostToDevice, h->stream));
+    if (nsteps > 0) enqueue_run_prologue(h);
+    const int mstride = (int)nsamp * ngauges;
+    for (int n = 0; n < nsteps; ++n) {
+        enqueue_run_step(h);
+        if ((n + 1) % sample_every == 0) {
+            const int k = (n + 1) / sample_every - 1;
+            launch_gauge_sample(h->m, ngauges, d_gel, h->d_w, d_h + (size_t)k * ngauges, nullptr, nullptr, h->stream,
+                                mstride);
+        }
+    }
+    DGR_HIP(hipGetLastError());
+    if (per * nsamp > 0)
+        DGR_HIP(hipMemcpyAsync(h_out, d_h, per * nsamp * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+    DGR_HIP(hipStreamSynchronize(h->stream));
+    // the members' error keys, a few control blocks at a time on the stack
+    Ctrl c[8];
+    for (int m0 = 0; m0 < h->nmem; m0 += 8) {
+        const int nc = std::min(8, h->nmem - m0);
+        DGR_HIP(hipMemcpyAsync(c, h->d_ctrl + m0, nc * sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));
+        DGR_HIP(hipStreamSynchronize(h->stream));
+        for (int i = 0; i < nc; ++i)
+            if (err_code_of(c[i].err_key)) return DGR_ERR_NAN;
+    }
+    return DGR_OK;
 }
 
 extern "C" dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, const double *n)

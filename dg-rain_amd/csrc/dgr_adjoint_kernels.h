@@ -668,10 +668,11 @@ __global__ void k_adj_gauges(MeshDev m, int ngauge, const int *__restrict__ gaug
     }
 }
 
-// forward sample: h at the gauges, one thread per (member, gauge); with obs
-// the squared misfits are summed per member in gauge order
+// forward sample: h at the gauges, one thread per (member, gauge), member
+// mid's gauge g at hval[mid * mstride + g]; with obs the squared misfits are
+// summed per member in gauge order
 __global__ void k_gauge_sample(MeshDev m, int ngauge, const int *__restrict__ gauge_el, const double *__restrict__ w,
-                               double *__restrict__ hval)
+                               double *__restrict__ hval, int mstride)
 {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= m.nmem * ngauge) return;
@@ -684,7 +685,7 @@ __global__ void k_gauge_sample(MeshDev m, int ngauge, const int *__restrict__ ga
         const double za = m.nodes[m.ev[k * P + e]].z, zb = m.nodes[m.ev[((k + 1) % 3) * P + e]].z;
         h += w[k * P + e] + 0.5 * (za + zb);
     }
-    hval[t] = h / 3.0;
+    hval[(t / ngauge) * mstride + t % ngauge] = h / 3.0;
 }
 
 __global__ void k_gauge_loss(MeshDev m, int ngauge, const double *__restrict__ hval, const double *__restrict__ obs,
@@ -804,11 +805,12 @@ void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const 
 }
 
 void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
-                         const double *obs, AdjCtrl *actl, hipStream_t s)
+                         const double *obs, AdjCtrl *actl, hipStream_t s, int mstride)
 {
     const int n = m.nmem * ngauge;
+    if (mstride <= 0) mstride = ngauge;
     hipLaunchKernelGGL(k_gauge_sample, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, m, ngauge, gauge_el, w,
-                       hval);
+                       hval, mstride);
     if (obs && actl)
         hipLaunchKernelGGL(k_gauge_loss, dim3((unsigned)((m.nmem + 63) / 64)), dim3(64), 0, s, m, ngauge, hval, obs,
                            actl);

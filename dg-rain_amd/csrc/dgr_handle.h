@@ -51,8 +51,9 @@ void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, cons
 void launch_adj_scalar(const MeshDev &m, const AdjScalarArgs &sa, AdjCtrl *actl, hipStream_t s);
 void launch_adj_gauges(const MeshDev &m, int ngauge, const int *gauge_el, const double *hval, const double *obs,
                        double *Wb, double *zvb, hipStream_t s);
+// member mid's gauge g goes to hval[mid * mstride + g]; mstride 0: ngauge
 void launch_gauge_sample(const MeshDev &m, int ngauge, const int *gauge_el, const double *w, double *hval,
-                         const double *obs, AdjCtrl *actl, hipStream_t s);
+                         const double *obs, AdjCtrl *actl, hipStream_t s, int mstride = 0);
 void launch_step_record(const MeshDev &m, const Ctrl *ctrls, StepRec *rec, int at_end, hipStream_t s);
 void launch_adj_gather(const MeshDev &m, const double *zvb, const double *nvb, double *gz, double *gn, hipStream_t s);
 }  // namespace dgr

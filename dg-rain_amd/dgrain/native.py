@@ -18,8 +18,11 @@ NUM_VARIANTS = 3   # DGR_NUM_VARIANTS in include/dgrain.h
 DGR_OK = 0
 DGR_ERR_ARG = 1
 DGR_ERR_UNSUPPORTED = 7
+DGR_ERR_TOO_LARGE = 8
+NOALLOC_SCRATCH_BYTES = 8 << 20   # DGR_NOALLOC_SCRATCH_BYTES in include/dgrain.h
 STATUS = {0: "ok", 1: "invalid argument", 2: "inconsistent mesh", 3: "HIP runtime error",
-          4: "NaN flux or wet/dry failure", 5: "out of memory", 6: "no HIP device", 7: "unsupported"}
+          4: "NaN flux or wet/dry failure", 5: "out of memory", 6: "no HIP device", 7: "unsupported",
+          8: "request exceeds the fixed workspace"}
 
 
 class DgrError(RuntimeError):
@@ -118,6 +121,7 @@ def load():
         "dgr_member_layout": (C.c_int, [vp, C.c_int, ip, ip]),
         "dgr_adjoint_gauges": (C.c_int, [vp, C.POINTER(AdjointSpec), dp, dp, dp]),
         "dgr_gauge_run": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, ip, dp]),
+        "dgr_gauge_run_noalloc": (C.c_int, [vp, C.c_int, C.c_int, C.c_int, ip, dp]),
         "dgr_set_nodal": (C.c_int, [vp, C.c_int, dp, dp]),
         "dgr_checkpoint_write": (C.c_int, [vp, C.c_int, C.c_char_p]),
         "dgr_checkpoint_read": (C.c_int, [vp, C.c_int, C.c_char_p]),
@@ -286,6 +290,16 @@ class _Handle:
         out = np.zeros((gel.shape[0], nsamp, gel.shape[1]))
         check(self.lib.dgr_gauge_run(self.h, int(nsteps), int(sample_every), gel.shape[1], _ip(gel), _dp(out)),
               "dgr_gauge_run")
+        return out
+
+    def gauge_run_noalloc(self, nsteps, sample_every, gauge_el):
+        """gauge_run through dgr_gauge_run_noalloc: no device allocation, the
+        staging in the library's fixed per-device workspace."""
+        gel = np.ascontiguousarray(np.reshape(gauge_el, (self._nmem(), -1)), dtype=np.int32)
+        nsamp = nsteps // sample_every
+        out = np.zeros((gel.shape[0], nsamp, gel.shape[1]))
+        check(self.lib.dgr_gauge_run_noalloc(self.h, int(nsteps), int(sample_every), gel.shape[1], _ip(gel),
+                                             _dp(out)), "dgr_gauge_run_noalloc")
         return out
 
     def adjoint_gauges(self, nsteps, sample_every, checkpoint_every, gauge_el, obs):

@@ -55,7 +55,8 @@ typedef enum {
                                wetDry2D.c:142-146,233-237); see dgr_last_error */
     DGR_ERR_NOMEM = 5,
     DGR_ERR_NODEV = 6,      /* no HIP device */
-    DGR_ERR_UNSUPPORTED = 7
+    DGR_ERR_UNSUPPORTED = 7,
+    DGR_ERR_TOO_LARGE = 8   /* request exceeds a fixed workspace (dgr_gauge_run_noalloc) */
 } dgr_status;
 
 typedef struct dgr_handle dgr_handle;
@@ -302,6 +303,18 @@ dgr_status dgr_set_tape_budget(dgr_handle *h, long long bytes);
 /* Forward run that samples the gauge depths: h_out[members][nsteps/sample_every][ngauges]. */
 dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every, int ngauges, const int *gauge_el,
                          double *h_out);
+/* dgr_gauge_run with no device allocation (extension; no counterpart in the
+ * reference).  For callers that cannot allocate device memory mid-run:
+ * serving loops and harnesses that forbid hipMalloc.  The gauge ids and the
+ * samples are staged in a fixed per-device workspace of
+ * DGR_NOALLOC_SCRATCH_BYTES that each device holds until the process ends:
+ * the ids rounded up to 256 bytes plus members*ngauges*(nsteps/sample_every)
+ * doubles must fit, else DGR_ERR_TOO_LARGE.  Same results as dgr_gauge_run.
+ * All calls on one device share the workspace: calls that overlap on one
+ * device (two threads, two handles) race. */
+#define DGR_NOALLOC_SCRATCH_BYTES (8u << 20)
+dgr_status dgr_gauge_run_noalloc(dgr_handle *h, int nsteps, int sample_every, int ngauges, const int *gauge_el,
+                                 double *h_out);
 /* Replace a member's nodal depth z and/or Manning n (NULL keeps it). */
 dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, const double *n);
 
