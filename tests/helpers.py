@@ -128,3 +128,48 @@ def delaunay_mesh(n_side=8, n_in=80, L=100.0, seed=0, depth=2.0, codes=True):
             return np.ascontiguousarray(np.stack([ids[:-1], ids[1:]], 1), np.int32).ravel()
         kw = dict(in1=side(3 * n_side + n_side // 2, 4 * n_side), in2=side(0, n_side), out=side(n_side, 2 * n_side))
     return M.from_elements(x, y, z, n, tri.astype(np.int32).ravel(), **kw)
+
+
+def smooth_field(x, y, rng, modes=4, kmax=3):
+    """A smooth random field over points (x, y): a few cosine modes of low
+    wave number over the bounding box."""
+    x = (x - x.min()) / (np.ptp(x) + 1e-300)
+    y = (y - y.min()) / (np.ptp(y) + 1e-300)
+    d = np.zeros(np.size(x))
+    for _ in range(modes):
+        kx, ky = rng.integers(0, kmax, 2)
+        d += rng.standard_normal() * np.cos(np.pi * kx * x + rng.uniform(0, 6.3)) * \
+            np.cos(np.pi * ky * y + rng.uniform(0, 6.3))
+    return d
+
+
+def smooth_start(wl, rng, amp=0.05, qamp=0.05):
+    """The workload started from its own state plus smooth random elevation
+    and discharge fields (amp, qamp).  Unlike a per-midpoint random state,
+    a smooth one leaves the limiter inactive almost everywhere, and unlike
+    still water it has no exact ties in the limiter bounds or the max-|lambda|
+    fold, so a loss of the run is differentiable at the start with few kinks
+    nearby."""
+    import dataclasses
+    m = wl.mesh
+    zn, un, vn = (a * smooth_field(m.x, m.y, rng) for a in (amp, qamp, qamp))
+    return dataclasses.replace(wl, zeta=wl.zeta + M.nodal_to_midpoints(m, zn), qx=M.nodal_to_midpoints(m, un),
+                               qy=M.nodal_to_midpoints(m, vn))
+
+
+def converged_difference(loss_of, eps_list, tol, avg=1):
+    """Central differences (L(+e) - L(-e)) / 2e for e in eps_list (largest
+    first), each the mean over the avg step sizes e * (1 + 0.1 k), k < avg
+    (the rounding noise of separate runs averages out).  Returns
+    (estimate, e) at the largest e whose estimate agrees with the next
+    smaller e's to within tol (relative): a kink of the piecewise-smooth loss
+    inside [-e, e] would show as a disagreement.  Returns (None, None) if no
+    successive pair agrees."""
+    def fd(e):
+        return sum((loss_of(x) - loss_of(-x)) / (2 * x) for x in (e * (1 + 0.1 * k) for k in range(avg))) / avg
+    fds = [fd(e) for e in eps_list]
+    for i in range(len(fds) - 1):
+        a, b = fds[i], fds[i + 1]
+        if abs(a - b) <= tol * max(abs(b), 1e-300):
+            return b, eps_list[i + 1]
+    return None, None
