@@ -533,6 +533,150 @@ DGR_HD int limit_adj(const double S[9], const double Vmin[3][3], const double Vm
     return 1;
 }
 
+// ---------------------------------------------------------------------------
+// PDop2D (wetDry2D.c:66-256) reverse for one element, on the branch the
+// forward takes.  S[9] is the element's state before PDop (the limited
+// state), zv[3] its vertex depths, Sb[9] the adjoint of PDop's output.
+// Writes Sin_b[9], the adjoint of S, and accumulates zvb[3] (the depth
+// enters through the midpoint bed bz_k = (zv_k + zv_k+1)/2).  Returns 1 if
+// PDop changed the element, else 0 (Sin_b = Sb).
+//
+// The forward, with T the midpoint-to-vertex map (a-b+c, a+b-c, -a+b+c):
+//   Ht = T (zeta + bz)                      vertex depths
+//   if some Ht_i <= H0:
+//     avg(Ht) <= H0: h_i = avg(Ht)          (flatten)
+//     else: h[m] = H0; h[n] = max(H0, Ht[n] - (H0 - Ht[m])/2);
+//           h[r] -= (h[m] - Ht[m]) + (h[n] - Ht[n])   (m, n, r as chosen
+//           below; n == m when vertex 1 is the minimum, wetDry2D.c:115-124)
+//     zeta_k = (h_k + h_k+1)/2 - bz_k
+//   f_i = h_i > H0; no f_i: momentum zeroed; some f_i zero: vertex velocities
+//   u = T q / h, the dry vertices' velocities (|T q| > VELZERO) summed and
+//   shared among the wet ones, q_k = (h_k u_k + h_k+1 u_k+1)/2.
+// ---------------------------------------------------------------------------
+DGR_HD int pdop_adj(const double S[9], const double zv[3], double H0, double VZ, const double Sb[9], double Sin_b[9],
+                    double zvb[3])
+{
+    const double bz[3] = {0.5 * (zv[0] + zv[1]), 0.5 * (zv[1] + zv[2]), 0.5 * (zv[2] + zv[0])};
+    const double mh0 = S[0] + bz[0], mh1 = S[1] + bz[1], mh2 = S[2] + bz[2];
+    const double Ht[3] = {mh0 - mh1 + mh2, mh0 + mh1 - mh2, -mh0 + mh1 + mh2};
+    if (!(Ht[0] <= H0 || Ht[1] <= H0 || Ht[2] <= H0)) {
+        for (int i = 0; i < 9; ++i) Sin_b[i] = Sb[i];
+        return 0;
+    }
+    // forward replay of the depth correction
+    const double avgH = (Ht[0] + Ht[1] + Ht[2]) / 3;
+    const bool flat = avgH <= H0;
+    double h[3] = {Ht[0], Ht[1], Ht[2]};
+    int mi = 0, ni = 1, ri = 2;
+    bool nact = false;   // the max(H0, .) took its second operand
+    if (flat) {
+        h[0] = h[1] = h[2] = avgH;
+    } else {
+        double mh = Ht[0];
+        if (Ht[1] < mh) { mi = 1; mh = Ht[1]; }
+        if (Ht[2] < mh) { mi = 2; mh = Ht[2]; }
+        double nh = Ht[1];
+        for (int i = 0; i < 3; ++i)
+            if (Ht[i] < nh && i != mi) { ni = i; nh = Ht[i]; }
+        for (int i = 2; i >= 0; --i)
+            if (i != mi && i != ni) ri = i;
+        h[mi] = H0;
+        const double cand = nh - 0.5 * (h[mi] - mh);
+        nact = cand > H0;
+        h[ni] = nact ? cand : H0;
+        h[ri] = h[ri] - (h[mi] - mh) - (h[ni] - nh);
+    }
+    // hb: adjoint of the corrected vertex depths h
+    double hb[3] = {0, 0, 0};
+    // momentum (both fields)
+    const int f[3] = {h[0] > H0, h[1] > H0, h[2] > H0};
+    const int npos = f[0] + f[1] + f[2];
+    for (int q = 1; q < 3; ++q) {
+        const double *Q = S + 3 * q;
+        const double *ob = Sb + 3 * q;
+        double *qb = Sin_b + 3 * q;
+        if (npos == 0) {
+            qb[0] = qb[1] = qb[2] = 0.0;
+            continue;
+        }
+        if (npos == 3) {
+            qb[0] = ob[0]; qb[1] = ob[1]; qb[2] = ob[2];
+            continue;
+        }
+        const double qe[3] = {Q[0] - Q[1] + Q[2], Q[0] + Q[1] - Q[2], -Q[0] + Q[1] + Q[2]};
+        double u[3], du = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            u[i] = qe[i] / h[i];
+            if (!f[i] && fabs(qe[i]) > VZ) du += u[i];
+        }
+        const double dun = du / npos;
+        double qob[3] = {0, 0, 0};
+        for (int k = 0; k < 3; ++k) {
+            qob[k] += 0.5 * ob[k];
+            qob[(k + 1) % 3] += 0.5 * ob[k];
+        }
+        double ub[3], dunb = 0.0;
+        for (int i = 0; i < 3; ++i) {
+            const double un = f[i] ? u[i] + dun : 0.0;
+            hb[i] += un * qob[i];
+            const double unb = h[i] * qob[i];
+            ub[i] = f[i] ? unb : 0.0;
+            dunb += f[i] ? unb : 0.0;
+        }
+        const double dub = dunb / npos;
+        double qeb[3];
+        for (int i = 0; i < 3; ++i) {
+            if (!f[i] && fabs(qe[i]) > VZ) ub[i] += dub;
+            qeb[i] = ub[i] / h[i];
+            hb[i] -= ub[i] * u[i] / h[i];
+        }
+        qb[0] = qeb[0] + qeb[1] - qeb[2];
+        qb[1] = -qeb[0] + qeb[1] + qeb[2];
+        qb[2] = qeb[0] - qeb[1] + qeb[2];
+    }
+    // zeta_k = (h_k + h_k+1)/2 - bz_k
+    double bzb[3];
+    for (int k = 0; k < 3; ++k) {
+        hb[k] += 0.5 * Sb[k];
+        hb[(k + 1) % 3] += 0.5 * Sb[k];
+        bzb[k] = -Sb[k];
+    }
+    // the depth correction, reversed operation by operation
+    double Htb[3];
+    if (flat) {
+        const double s = (hb[0] + hb[1] + hb[2]) / 3;
+        Htb[0] = Htb[1] = Htb[2] = s;
+    } else {
+        double mhb = 0.0, nhb = 0.0;
+        const double t3 = hb[ri];              // h[ri] -= (h[mi] - mh) + (h[ni] - nh)
+        hb[mi] -= t3;
+        hb[ni] -= t3;
+        mhb += t3;
+        nhb += t3;
+        const double t2 = hb[ni];              // h[ni] = max(H0, nh - (h[mi] - mh)/2)
+        hb[ni] = 0.0;
+        if (nact) {
+            nhb += t2;
+            hb[mi] -= 0.5 * t2;
+            mhb += 0.5 * t2;
+        }
+        hb[mi] = 0.0;                          // h[mi] = H0
+        for (int i = 0; i < 3; ++i) Htb[i] = hb[i];
+        Htb[mi] += mhb;
+        Htb[ni] += nhb;
+    }
+    // Ht = T (zeta + bz)
+    const double mhb3[3] = {Htb[0] + Htb[1] - Htb[2], -Htb[0] + Htb[1] + Htb[2], Htb[0] - Htb[1] + Htb[2]};
+    for (int k = 0; k < 3; ++k) {
+        Sin_b[k] = mhb3[k];
+        bzb[k] += mhb3[k];
+    }
+    zvb[0] += 0.5 * (bzb[0] + bzb[2]);
+    zvb[1] += 0.5 * (bzb[0] + bzb[1]);
+    zvb[2] += 0.5 * (bzb[1] + bzb[2]);
+    return 1;
+}
+
 #undef DGR_HD
 
 }  // namespace dgr

@@ -7,7 +7,8 @@
 // element workgroups, edge kernels on the edge workgroups):
 //   k_node_arg      node min/max of the stage output's element means and the
 //                   element that attains each (the limiter's active bounds)
-//   k_adj_limit     limiter pass reverse: adjoint of the pre-limit output
+//   k_adj_limit     PDop2D reverse (wet/dry runs, elements it changed), then
+//                   limiter pass reverse: adjoint of the pre-limit output
 //                   (own part) and of the node bounds at the element's vertices
 //   k_adj_rhs       node sums of the bound adjoints, gathered by the element
 //                   attaining each bound -> element means -> stage output; RK update
@@ -59,10 +60,11 @@ __device__ __forceinline__ void adj_scalar_member(const MeshDev &m, int mid, con
 // limiter reverse.  U: pre-limit stage output; Sb (+ slotb when given): the
 // adjoint of the limited (and PDop'd) output, slotb being the edge part left
 // by the following stage's k_adj_edge, whose z_edge adjoints (zbe) are moved
-// to the element's vertices here too.  Writes Ub (own part of the pre-limit
-// adjoint) and vmb[18][pitch]: adjoint of Vmin (f*3+k) and Vmax (9+f*3+k) at
-// vertex k.  PDop is the identity when every vertex depth exceeds H0; an
-// element where it is not raises the member's `unsupported` flag.
+// to the element's vertices here too.  With wet/dry on, PDop2D's reverse
+// (pdop_adj, on the branch the forward took) turns that into the adjoint of
+// the limited state first, and adds the adjoint of the vertex depths it
+// reads to zvb.  Writes Ub (own part of the pre-limit adjoint) and
+// vmb[18][pitch]: adjoint of Vmin (f*3+k) and Vmax (9+f*3+k) at vertex k.
 __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const double *__restrict__ U,
                                                    const double *__restrict__ nodemm, const double *__restrict__ Sb,
                                                    const double *__restrict__ slotb, const double *__restrict__ zbe,
@@ -79,17 +81,15 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
     double zv[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
+    double zadd[3] = {0, 0, 0};   // this element's additions to zvb
     if (zbe) {
         // z_edge = (z_a + z_b)/2 of local edge k
-        double z[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const double b = zbe[k * P + e];
-            z[k] += 0.5 * b;
-            z[(k + 1) % 3] += 0.5 * b;
+            zadd[k] += 0.5 * b;
+            zadd[(k + 1) % 3] += 0.5 * b;
         }
-#pragma unroll
-        for (int k = 0; k < 3; ++k) zvb[k * P + e] += z[k];
     }
     // limit_adj field by field: the zeta pass decides modify, then each
     // field's bounds are loaded, its pass replayed and reversed, and its
@@ -97,28 +97,73 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
     double S[9];
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = U[f * P + e];
-    auto sb_of = [&](int f) { return Sb[f * P + e] + (slotb ? slotb[f * P + e] : 0.0); };
     double Vmin[3], Vmax[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { Vmin[k] = nodemm[nd[k]]; Vmax[k] = nodemm[3 * nn + nd[k]]; }
-    LimFieldDec d;
-    limit_field_forward(S, 0, Vmin, Vmax, d);
+    // the zeta pass decides modify (its decisions are replayed again in the
+    // field loop rather than kept live through the PDop reverse)
     int modify = 0;
+    double lim[9];
     {
+        LimFieldDec d;
+        limit_field_forward(S, 0, Vmin, Vmax, d);
         const double w1 = S[0], w2 = S[1], w3 = S[2];
         const double Ha = (w1 - w2 + w3) + zv[0], Hb_ = (w1 + w2 - w3) + zv[1], Hc = (-w1 + w2 + w3) + zv[2];
         if (Ha > ph.h0 && Hb_ > ph.h0 && Hc > ph.h0 && Vmin[0] + zv[0] > ph.h0 && Vmin[1] + zv[1] > ph.h0 &&
             Vmin[2] + zv[2] > ph.h0)
             modify = 1;
         if (d.Vfin[0] + zv[0] < ph.h0 || d.Vfin[1] + zv[1] < ph.h0 || d.Vfin[2] + zv[2] < ph.h0) modify = 0;
+        lim[0] = S[0]; lim[1] = S[1]; lim[2] = S[2];
+        if (modify) {
+            lim[0] = (d.Vfin[0] + d.Vfin[1]) / 2;
+            lim[1] = (d.Vfin[1] + d.Vfin[2]) / 2;
+            lim[2] = (d.Vfin[2] + d.Vfin[0]) / 2;
+        }
     }
-    // the limited zeta midpoints (for the PDop check below)
-    double so[3] = {S[0], S[1], S[2]};
-    if (modify) {
-        so[0] = (d.Vfin[0] + d.Vfin[1]) / 2;
-        so[1] = (d.Vfin[1] + d.Vfin[2]) / 2;
-        so[2] = (d.Vfin[2] + d.Vfin[0]) / 2;
+    // where PDop acted, its reverse leaves the adjoint of the limited state in
+    // Ub (read back field by field below, each slot before it is written)
+    bool pdop = false;
+    if (ph.wetdry) {
+        // PDop2D (wetDry2D.c:66-256) changes an element iff a vertex depth it
+        // forms from the limited state is at most H0
+        const double bz0 = 0.5 * (zv[0] + zv[1]), bz1 = 0.5 * (zv[1] + zv[2]), bz2 = 0.5 * (zv[2] + zv[0]);
+        const double mh0 = lim[0] + bz0, mh1 = lim[1] + bz1, mh2 = lim[2] + bz2;
+        const double H0 = ph.h0;
+        if (mh0 - mh1 + mh2 <= H0 || mh0 + mh1 - mh2 <= H0 || -mh0 + mh1 + mh2 <= H0) {
+            pdop = true;
+            // the limited momentum, then PDop's reverse on the limited state
+#pragma unroll
+            for (int iv = 1; iv < 3; ++iv) {
+                if (modify) {
+                    LimFieldDec dq;
+                    double mn[3], mx[3];
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) {
+                        mn[k] = nodemm[iv * nn + nd[k]];
+                        mx[k] = nodemm[(3 + iv) * nn + nd[k]];
+                    }
+                    limit_field_forward(S, iv, mn, mx, dq);
+                    lim[3 * iv] = (dq.Vfin[0] + dq.Vfin[1]) / 2;
+                    lim[3 * iv + 1] = (dq.Vfin[1] + dq.Vfin[2]) / 2;
+                    lim[3 * iv + 2] = (dq.Vfin[2] + dq.Vfin[0]) / 2;
+                } else {
+#pragma unroll
+                    for (int k = 0; k < 3; ++k) lim[3 * iv + k] = S[3 * iv + k];
+                }
+            }
+            double sb[9], lb[9];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) sb[f] = Sb[f * P + e] + (slotb ? slotb[f * P + e] : 0.0);
+            pdop_adj(lim, zv, H0, ph.velzero, sb, lb, zadd);
+#pragma unroll
+            for (int f = 0; f < 9; ++f) Ub[f * P + e] = lb[f];
+        }
     }
+    if (zbe || pdop) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) zvb[k * P + e] += zadd[k];
+    }
+    auto sb_of = [&](int f) { return pdop ? Ub[f * P + e] : Sb[f * P + e] + (slotb ? slotb[f * P + e] : 0.0); };
 #pragma unroll
     for (int iv = 0; iv < 3; ++iv) {
         double sb[9], ub[9], Vminb[3] = {0, 0, 0}, Vmaxb[3] = {0, 0, 0};
@@ -134,8 +179,9 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
                     Vmin[k] = nodemm[iv * nn + nd[k]];
                     Vmax[k] = nodemm[(3 + iv) * nn + nd[k]];
                 }
-                limit_field_forward(S, iv, Vmin, Vmax, d);
             }
+            LimFieldDec d;
+            limit_field_forward(S, iv, Vmin, Vmax, d);
             limit_field_reverse(d, iv, sb, ub, Vminb, Vmaxb);
         }
 #pragma unroll
@@ -144,14 +190,6 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
             vmb[(iv * 3 + k) * P + e] = Vminb[k];
             vmb[(9 + iv * 3 + k) * P + e] = Vmaxb[k];
         }
-    }
-    if (ph.wetdry) {
-        // PDop2D (wetDry2D.c:66-256) leaves an element unchanged iff every
-        // vertex depth it forms from the limited state exceeds H0
-        const double bz0 = 0.5 * (zv[0] + zv[1]), bz1 = 0.5 * (zv[1] + zv[2]), bz2 = 0.5 * (zv[2] + zv[0]);
-        const double mh0 = so[0] + bz0, mh1 = so[1] + bz1, mh2 = so[2] + bz2;
-        const double H0 = ph.h0;
-        if (mh0 - mh1 + mh2 <= H0 || mh0 + mh1 - mh2 <= H0 || -mh0 + mh1 + mh2 <= H0) actl[L.mid].unsupported = 1;
     }
 }
 

@@ -278,8 +278,9 @@ dgr_status dgr_member_layout(dgr_handle *h, int member, int *el_offset, int *num
  * step (Roe flux, inner products, Manning friction, RK update, limiter with
  * its active set, dt = cfl*minEdgLength/max|lambda| and the tide's time) over
  * checkpoints every `checkpoint_every` steps.  The forward's wet/dry flags
- * enter as recorded; an element where PDop2D changed the state makes the call
- * return DGR_ERR_UNSUPPORTED.  The final-time clamp of dt is taken as inactive.
+ * enter as recorded, and PDop2D's mass and momentum corrections are
+ * differentiated on the branch each element took.  The final-time clamp of dt
+ * is taken as inactive.
  * The handle's state, flags and time are restored on return. */
 typedef struct {
     int nsteps;

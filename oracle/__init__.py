@@ -89,6 +89,7 @@ def _load_oracle():
             "orc_slope_limiter": (None, [vp, ub]),
             "orc_wetdry_status": (None, [vp]),
             "orc_pdop": (None, [vp]),
+            "orc_pdop_changed": (C.c_longlong, [vp]),
             "orc_boundary_conditions": (None, [vp, C.c_double]),
             "orc_set_hydrograph": (C.c_int, [vp, C.c_int, C.c_int, dp, dp, dp]),
             "orc_hydro_eval": (C.c_double, [C.c_int, dp, dp, C.c_double]),
@@ -312,6 +313,11 @@ class Oracle:
     @property
     def max_lambda(self):
         return self.lib.orc_max_lambda(self.h)
+
+    @property
+    def pdop_changed(self):
+        """Elements whose state PDop2D changed, summed over all its calls."""
+        return self.lib.orc_pdop_changed(self.h)
 
     @property
     def min_edge_length(self):

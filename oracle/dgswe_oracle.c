@@ -76,6 +76,9 @@ struct orc_state {
     /* Green-Ampt infiltration (extension): conductivity K, psi * delta_theta,
      * and the mean rate over the current step */
     double ga_K, ga_P, infil_rate;
+    /* elements whose state PDop2D changed, summed over its calls (a probe
+     * for tests that must know the wet/dry correction acted) */
+    long long pdop_changed;
 };
 
 /* mathfunctions.c:4-11 */
@@ -860,6 +863,12 @@ void orc_pdop(orc_state *s)
     const orc_mesh *m = &s->m;
     const double H0 = s->p.h0, VZ = s->p.velzero;
     for (int el = 0; el < m->num_el; ++el) {                          /* :66-258 */
+        double before[9];
+        for (int k = 0; k < 3; ++k) {
+            before[k] = s->zeta[3 * el + k];
+            before[3 + k] = s->qx[3 * el + k];
+            before[6 + k] = s->qy[3 * el + k];
+        }
         double mz0 = s->zeta[3 * el], mz1 = s->zeta[3 * el + 1], mz2 = s->zeta[3 * el + 2];
         double z0 = m->z[m->elto_vert[3 * el]], z1 = m->z[m->elto_vert[3 * el + 1]], z2 = m->z[m->elto_vert[3 * el + 2]];
         double bz0 = 0.5 * (z0 + z1), bz1 = 0.5 * (z1 + z2), bz2 = 0.5 * (z2 + z0);
@@ -920,8 +929,15 @@ void orc_pdop(orc_state *s)
                 s->qy[3 * el + i] = 0.5 * (qye[i1] + qye[i2]);
             }
         }
+        int changed = 0;
+        for (int k = 0; k < 3; ++k)
+            changed |= before[k] != s->zeta[3 * el + k] || before[3 + k] != s->qx[3 * el + k] ||
+                       before[6 + k] != s->qy[3 * el + k];
+        s->pdop_changed += changed;
     }
 }
+
+long long orc_pdop_changed(const orc_state *s) { return s->pdop_changed; }
 
 /* ------------------------------------------------------------------------ */
 /* ChanNet's channel-junction coupling, junction side (ChanNet/internal_BC.c */

@@ -84,6 +84,8 @@ int orc_compute_L(orc_state *s);
 void orc_slope_limiter(orc_state *s, unsigned char *modify_out);
 void orc_wetdry_status(orc_state *s);
 void orc_pdop(orc_state *s);
+/* elements whose state orc_pdop changed, summed over all its calls (test probe) */
+long long orc_pdop_changed(const orc_state *s);
 void orc_boundary_conditions(orc_state *s, double t);
 /* Hydrograph extension: boundary code 1..3 takes zeta_b(t), qn_b(t),
  * piecewise linear through n points (t strictly increasing), held at the end

@@ -107,6 +107,53 @@ static void limit_eval(const double S[9], const double Vmin[3][3], const double 
         for (int i = 0; i < 9; ++i) out[i] = res[i];
 }
 
+// PDop2D on one element (wetDry2D.c:66-256) in plain double arithmetic
+static void pdop_eval(const double S[9], const double zv[3], double H0, double VZ, double out[9])
+{
+    for (int i = 0; i < 9; ++i) out[i] = S[i];
+    const double mz[3] = {0.5 * (zv[0] + zv[1]), 0.5 * (zv[1] + zv[2]), 0.5 * (zv[2] + zv[0])};
+    const double m0 = S[0] + mz[0], m1 = S[1] + mz[1], m2 = S[2] + mz[2];
+    double Ht[3] = {m0 - m1 + m2, m0 + m1 - m2, -m0 + m1 + m2};
+    const double avgH = (Ht[0] + Ht[1] + Ht[2]) / 3;
+    if (Ht[0] <= H0 || Ht[1] <= H0 || Ht[2] <= H0) {
+        if (avgH <= H0) {
+            for (int i = 0; i < 3; ++i) Ht[i] = avgH;
+        } else {
+            int mi = 0;
+            double mh = Ht[0];
+            for (int i = 1; i < 3; ++i)
+                if (Ht[i] < mh) { mi = i; mh = Ht[i]; }
+            int ni = 1;
+            double nh = Ht[1];
+            for (int i = 0; i < 3; ++i)
+                if (Ht[i] < nh && i != mi) { ni = i; nh = Ht[i]; }
+            int ri = 2;
+            for (int i = 0; i < 3; ++i)
+                if (i != mi && i != ni) { ri = i; break; }
+            Ht[mi] = H0;
+            Ht[ni] = std::fmax(H0, nh - 0.5 * (Ht[mi] - mh));
+            Ht[ri] = Ht[ri] - (Ht[mi] - mh) - (Ht[ni] - nh);
+        }
+        for (int k = 0; k < 3; ++k) out[k] = 0.5 * (Ht[k] + Ht[(k + 1) % 3]) - mz[k];
+    }
+    int f[3], npos = 0;
+    for (int i = 0; i < 3; ++i) { f[i] = Ht[i] > H0; npos += f[i]; }
+    if (npos == 3) return;
+    for (int q = 1; q < 3; ++q) {
+        const double *Q = S + 3 * q;
+        if (npos == 0) { out[3 * q] = out[3 * q + 1] = out[3 * q + 2] = 0; continue; }
+        const double qe[3] = {Q[0] - Q[1] + Q[2], Q[0] + Q[1] - Q[2], -Q[0] + Q[1] + Q[2]};
+        double u[3], du = 0;
+        for (int i = 0; i < 3; ++i) {
+            u[i] = qe[i] / Ht[i];
+            du += u[i] * (1 - f[i]) * (std::fabs(qe[i]) > VZ);
+        }
+        double qo[3];
+        for (int i = 0; i < 3; ++i) qo[i] = Ht[i] * (f[i] * (u[i] + du / npos));
+        for (int k = 0; k < 3; ++k) out[3 * q + k] = 0.5 * (qo[k] + qo[(k + 1) % 3]);
+    }
+}
+
 static int report(const char *name, double err, double tol)
 {
     printf("%-28s max rel err %.3e (tol %.0e) %s\n", name, err, tol, err <= tol ? "ok" : "FAIL");
@@ -262,11 +309,76 @@ static int check_limit(int trials)
     return report("limit_adj", worst, 1e-5);
 }
 
+static int check_pdop(int trials)
+{
+    const double H0 = 1e-3, VZ = 1e-2;
+    double worst = 0;
+    int acted = 0, flat = 0, min1 = 0, zeroed = 0, moved = 0;
+    for (int t = 0; t < trials; ++t) {
+        // vertex depths around H0 (some dry), the midpoint state that has them
+        double zv[3], Hv[3], S[9], Sb[9];
+        for (int k = 0; k < 3; ++k) zv[k] = U(0.01, 0.05);
+        const int kind = t % 4;
+        for (int k = 0; k < 3; ++k)
+            Hv[k] = kind == 0 ? U(-2e-3, 1.5e-3)                   // mostly flattened
+                  : kind == 1 ? U(-3e-3, 0.05)                     // lift the minimum
+                  : kind == 2 ? (k == 1 ? U(-3e-3, 8e-4) : U(4e-3, 0.05))   // vertex 1 the minimum
+                              : U(-1e-3, 0.02);
+        for (int k = 0; k < 3; ++k) S[k] = 0.5 * (Hv[k] + Hv[(k + 1) % 3]) - 0.5 * (zv[k] + zv[(k + 1) % 3]);
+        for (int k = 3; k < 9; ++k) S[k] = U(-0.03, 0.03);
+        for (double &v : Sb) v = U(-1, 1);
+        double Sin_b[9], zvb[3] = {0, 0, 0};
+        const int a = pdop_adj(S, zv, H0, VZ, Sb, Sin_b, zvb);
+        acted += a;
+        if (a) {
+            const double avg = (Hv[0] + Hv[1] + Hv[2]) / 3;
+            flat += avg <= H0;
+            min1 += avg > H0 && Hv[1] < Hv[0] && Hv[1] <= Hv[2];
+            double o[9];
+            pdop_eval(S, zv, H0, VZ, o);
+            zeroed += o[3] == 0 && o[4] == 0 && o[5] == 0;
+            moved += !(o[3] == 0 && o[4] == 0 && o[5] == 0) && (o[3] != S[3] || o[4] != S[4] || o[5] != S[5]);
+        }
+        for (int dir = 0; dir < 4; ++dir) {
+            double dS[9], dz[3];
+            for (double &v : dS) v = U(-1, 1);
+            for (double &v : dz) v = U(-1, 1);
+            const double h = 1e-9;
+            double y[2];
+            for (int s = 0; s < 2; ++s) {
+                const double sg = s ? 1.0 : -1.0;
+                double S2[9], z2[3], out[9];
+                for (int i = 0; i < 9; ++i) S2[i] = S[i] + sg * h * dS[i];
+                for (int k = 0; k < 3; ++k) z2[k] = zv[k] + sg * h * dz[k];
+                pdop_eval(S2, z2, H0, VZ, out);
+                y[s] = 0;
+                for (int i = 0; i < 9; ++i) y[s] += Sb[i] * out[i];
+            }
+            const double fd = (y[1] - y[0]) / (2 * h);
+            double ad = 0, mag = 0;
+            for (int i = 0; i < 9; ++i) { ad += Sin_b[i] * dS[i]; mag += std::fabs(Sin_b[i] * dS[i]); }
+            for (int k = 0; k < 3; ++k) { ad += zvb[k] * dz[k]; mag += std::fabs(zvb[k] * dz[k]); }
+            const double e = std::fabs(ad - fd) / (mag + 1e-300);
+            if (e > 1e-5 && getenv("ADJ_VERBOSE")) printf("pdop trial %d dir %d ad %.10e fd %.10e\n", t, dir, ad, fd);
+            worst = e > worst ? e : worst;
+        }
+    }
+    printf("pdop_adj: %d of %d elements changed (%d flattened, %d with vertex 1 the minimum, %d momentum zeroed, "
+           "%d momentum redistributed)\n", acted, trials, flat, min1, zeroed, moved);
+    int bad = report("pdop_adj", worst, 1e-6);
+    if (flat == 0 || min1 == 0 || zeroed == 0 || moved == 0 || acted - flat == 0) {
+        printf("pdop_adj: a branch was not exercised\n");
+        bad = 1;
+    }
+    return bad;
+}
+
 int main()
 {
     int bad = 0;
     bad += check_roe(2000);
     bad += check_rhs(500);
     bad += check_limit(2000);
+    bad += check_pdop(4000);
     return bad ? 1 : 0;
 }

@@ -214,6 +214,120 @@ def test_adjoint_against_oracle_differences(gpu):
     assert worst <= 1e-6, worst
 
 
+def _dry_bed_problem(N=24, W=40, seed=5):
+    """The dry-bed dam break (5 mm reservoir for x <= 5 m, dry film beyond,
+    walls) made tie-free so that the loss is differentiable where it is
+    evaluated.  Three exact ties of the plain workload are removed:
+    * the dry film sits at 0.8 H0 instead of exactly H0, PDop2D's own
+      threshold (wetDry2D.c:93: Ht <= H0);
+    * the reservoir surface carries a smooth 0.2 mm perturbation (still water
+      of uniform depth makes every wet edge tie for max |lambda|, the dt
+      operand);
+    * the bed carries a smooth 20 um relief.
+    Manning n is 0.002: with n = 0.02 the explicit friction of the reference
+    scheme diverges in the thin films (its NaN failure) within a few steps.
+    Every element is a gauge, sampled after each step; the observations come
+    from a run with n scaled by 1.2."""
+    from dgrain import mesh as M
+    from helpers import smooth_field
+    wl = configs.dry_bed_dam_break(N=N, nsteps=W)
+    rng = np.random.default_rng(seed)
+    mesh = dataclasses.replace(wl.mesh, z=wl.mesh.z + 2e-5 * smooth_field(wl.mesh.x, wl.mesh.y, rng),
+                               n=np.full(wl.mesh.num_nodes, 0.002))
+    surf = np.where(mesh.x <= 5.0, 0.005 + 2e-4 * smooth_field(mesh.x, mesh.y, rng), 0.8e-3) - mesh.z
+    wl = dataclasses.replace(wl, mesh=mesh, zeta=M.nodal_to_midpoints(mesh, surf))
+    gauges = np.arange(wl.num_el, dtype=np.int32)
+    obs = _gauges_of(wl, gauges, W, n=mesh.n * 1.2)
+    return wl, gauges, obs
+
+
+def _gauges_of(wl, gauges, W, z=None, n=None):
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_nodal(z=z, n=n)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    h = s.gauge_run(W, 1, gauges)
+    s.close()
+    return h
+
+
+def test_adjoint_through_pdop(gpu):
+    """The gradient through PDop2D's mass and momentum corrections
+    (wetDry2D.c:93-253), differentiated on the branch each element took:
+    40 steps of the tie-free dry-bed dam break (24x24 cells), in which the
+    oracle counts thousands of element-stages that PDop changed.  dL/dn in 8
+    smooth random directions and dL/dz in 8 smooth directions over the
+    reservoir, against settled central differences of the GPU forward;
+    tolerance 1e-6.
+
+    A bed perturbation under the dry film is left out on purpose: PDop pins
+    a lifted vertex at exactly H0, and the next stage's PDop switches its
+    momentum redistribution on a vertex depth <= H0 test, so moving the bed
+    under a pinned vertex jumps the state (the reference scheme is
+    discontinuous there, and no central difference settles)."""
+    from helpers import converged_difference, smooth_field
+    W = 40
+    wl, gauges, obs = _dry_bed_problem(W=W)
+    o = oracle.Oracle.for_workload(wl)
+    o.run(W)
+    changed = o.pdop_changed
+    o.close()
+    assert changed > 1000, changed
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    loss, gz, gn = s.adjoint_gauges(W, 1, 8, gauges, obs)
+    s.close()
+    ref = float(((_gauges_of(wl, gauges, W) - obs) ** 2).sum())
+    assert abs(loss[0] - ref) <= 1e-13 * ref
+    print("  PDop changed %d element-stages; loss %.6e" % (changed, loss[0]), flush=True)
+    rng = np.random.default_rng(9)
+    m = wl.mesh
+    worst = 0.0
+    for param, g in (("n", gn), ("z", gz)):
+        base = m.n if param == "n" else m.z
+        for j in range(8):
+            d = smooth_field(m.x, m.y, rng)
+            d = d * np.abs(base).mean() if param == "n" else d * np.clip((3.5 - m.x) / 1.5, 0.0, 1.0)
+            eps = [1e-5, 1e-6, 1e-7] if param == "n" else [1e-7, 1e-8, 1e-9, 1e-10]
+
+            def L(e):
+                kw = {param: base + e * d}
+                return float(((_gauges_of(wl, gauges, W, **kw) - obs) ** 2).sum())
+
+            fd, e = converged_difference(L, eps, 1e-6)
+            assert fd is not None, "%s direction %d: central differences did not settle" % (param, j)
+            ad = float(g @ d)
+            rel = abs(ad - fd) / abs(fd)
+            print("  %s dir %d: adjoint %.12e  central difference %.12e (eps %.0e)  rel %.2e" % (param, j, ad, fd, e, rel),
+                  flush=True)
+            worst = max(worst, rel)
+    assert worst <= 1e-6, worst
+
+
+def test_adjoint_runs_on_the_plain_dry_bed(gpu):
+    """The plain dry-bed dam break (film at exactly H0): PDop acts on every
+    dry element each stage and the adjoint returns a gradient (it used to
+    refuse such runs); its loss is the forward's.  Central differences do
+    not apply here: the film sits on PDop's threshold, so the loss has kinks
+    at the evaluation point."""
+    W = 40
+    wl = configs.dry_bed_dam_break(N=24, nsteps=W)
+    gauges = np.arange(0, wl.num_el, 7, dtype=np.int32)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    obs = s.gauge_run(W, 5, gauges) + 1e-3    # (a larger n makes the thin films' friction diverge)
+    s.close()
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    loss, gz, gn = s.adjoint_gauges(W, 5, 10, gauges, obs)
+    s.close()
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    ref = float(((s.gauge_run(W, 5, gauges) - obs) ** 2).sum())
+    s.close()
+    assert abs(loss[0] - ref) <= 1e-13 * ref
+    assert np.all(np.isfinite(gz)) and np.all(np.isfinite(gn)) and np.abs(gn).max() > 0
+
+
 def test_autograd_harness(gpu):
     """torch.autograd over the C ABI: backward gives the adjoint gradients,
     and a few gradient steps on Manning n reduce the misfit."""
