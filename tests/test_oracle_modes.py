@@ -80,3 +80,47 @@ def test_device_math_inside_the_reference_own_spread(tmp_path):
         return
     assert spread_ref > 1e-11          # the reference cannot hold 1e-11 against itself
     assert spread_dev < 100 * spread_ref
+
+
+def test_flags_identical_between_the_modes():
+    """Over the full plain dam break (100x100 grid, 1000 SSP-RK2 steps, CFL
+    0.3, wet/dry and limiter on), the device-math and reference-math oracles
+    take the same discrete decisions at every step: wet/dry flags and both
+    stages' limiter activations, although their states drift apart (see
+    test_device_math_inside_the_reference_own_spread)."""
+    wl = configs.dam_break(N=100, nsteps=1000)
+    a = oracle.Oracle.for_workload(wl, math=oracle.MATH_DEV, threads=8)
+    b = oracle.Oracle.for_workload(wl, math=oracle.MATH_REF, threads=8)
+    limited = 0
+    for step in range(1, 1001):
+        a.run(1)
+        b.run(1)
+        assert np.array_equal(a.get_wd(), b.get_wd()), "wet/dry flags differ after step %d" % step
+        for st in (0, 1):
+            ma, mb = a.get_modify(st), b.get_modify(st)
+            assert np.array_equal(ma, mb), "stage %d limiter activations differ after step %d" % (st + 1, step)
+            limited += int(ma.sum())
+    drift = rel_l2(np.concatenate(a.get_state()), np.concatenate(b.get_state()))
+    print("limiter activations %d; state drift %.2e after 1000 steps" % (limited, drift))
+    assert limited > 0 and drift > 0
+
+
+def test_flags_on_a_dry_bed_until_the_modes_drift_apart():
+    """Thin films put states next to the wet/dry thresholds, where the modes'
+    one-ulp differences in H^(4/3) eventually flip a flag: on the dry-bed
+    dam break (24x24, film at H0) the flags and limiter activations are the
+    same for the first 50 steps, while wet/dry flags change and PDop acts
+    (on this host they first differ at step 72)."""
+    wl = configs.dry_bed_dam_break(N=24, nsteps=50)
+    a = oracle.Oracle.for_workload(wl, math=oracle.MATH_DEV)
+    b = oracle.Oracle.for_workload(wl, math=oracle.MATH_REF)
+    wet = set()
+    for step in range(1, 51):
+        a.run(1)
+        b.run(1)
+        wd = a.get_wd()
+        assert np.array_equal(wd, b.get_wd()), step
+        for st in (0, 1):
+            assert np.array_equal(a.get_modify(st), b.get_modify(st)), (step, st)
+        wet.add(int(wd.sum()))
+    assert len(wet) > 10 and a.pdop_changed > 1000
