@@ -92,3 +92,4 @@ def test_compat_time_evolution_writes_reference_files(gpu, tmp_path):
         assert os.path.exists(str(tmp_path / name)), name
     vals = np.loadtxt(str(tmp_path / "001_H.dat")).ravel()
     assert np.allclose(vals, o.get_state()[0], rtol=0, atol=1e-15)
+
