@@ -33,10 +33,15 @@ VALU_PEAK_GWIS = 256 * 4 * 2.4 / 4
 KSTAGE_BYTES_PER_EL = 244.0
 # algorithmic bytes per element per RK stage for the whole stage
 STAGE_BYTES_PER_EL = 268.0
+# FP64 vector peak of MI355X (FLOP/s)
+FP64_PEAK_TFLOPS = 78.6
 # the kernel(s) each stage-kernel variant launches for the flux/RHS/update
 STAGE_KERNELS = {0: "k_stage (fused flux gather + RHS + RK update)",
                  1: "k_flux + k_rhs (element-centric fluxes, then RHS + RK update)",
                  2: "k_edge_flux + k_rhs (edge-centric fluxes, then RHS + RK update)"}
+STAGE_ALL = {0: "k_stage, k_node_minmax, k_limit",
+             1: "k_flux, k_rhs, k_node_minmax, k_limit",
+             2: "k_edge_flux, k_rhs, k_node_minmax, k_limit"}
 
 
 def progress(msg):
@@ -53,6 +58,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ensemble", action="store_true", help="skip the batched-ensemble measurement")
+    ap.add_argument("--no-wetdry", action="store_true", help="skip the wet/dry-active (dry-bed dam break) leg")
     ap.add_argument("--no-adjoint", action="store_true", help="skip the adjoint-cost measurement")
     ap.add_argument("--adj-N", type=int, default=256)
     ap.add_argument("--no-large", action="store_true", help="skip the 2048x2048 run and the batched adjoints")
@@ -82,39 +88,100 @@ def copy_calibration():
     print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "dispatches": 5}))
 
 
-def load_traffic(variant):
-    """HBM bytes per stage launch of the stage kernels, from the FETCH_SIZE /
-    WRITE_SIZE passes summarised in profiles/ (None when they cover another
-    variant)."""
-    p = os.path.join(ROOT, "profiles", "stage_kernels_coastal_512_traffic.json")
+PAIR = {"k_edge_flux<true>": 2, "k_rhs<1>": 1, "k_rhs<2>": 1}   # launches per step of the variant-2 pair
+
+
+def load_pair(variant):
+    """(HBM bytes, VALU wave-instructions) per stage launch of the
+    flux/RHS/update kernel pair, from the counter summary in profiles/."""
+    c = load_counters(variant)
+    if c is None or variant != 2:
+        return None, None
+    kk = c["kernels"]
+    return (sum(kk[k]["bytes"] * n for k, n in PAIR.items()) / 2.0,
+            sum(kk[k]["valu_insts"] * n for k, n in PAIR.items()) / 2.0)
+
+
+def load_counters(variant):
+    """Per RK stage of the whole step (all kernels, k_step_end's share
+    included): HBM bytes from FETCH_SIZE/WRITE_SIZE, wave-level VALU
+    instructions from SQ_INSTS_VALU, FP64 FLOPs from SQ_INSTS_VALU_FLOPS_FP64
+    (+ _TRANS), from the PMC passes summarised in profiles/ (None when they
+    cover another variant)."""
+    p = os.path.join(ROOT, "profiles", "coastal_512_stage_counters.json")
     if os.path.exists(p):
         with open(p) as f:
             d = json.load(f)
         if d.get("variant") == variant:
-            return d.get("bytes_per_launch")
+            return d
     return None
 
 
-def load_valu(variant):
-    """Wave-level VALU instructions per stage launch of the stage kernel pair,
-    from the SQ_INSTS_VALU pass summarised in profiles/ (None when it covers
-    another variant)."""
-    p = os.path.join(ROOT, "profiles", "stage_kernels_coastal_512_valu.json")
-    if os.path.exists(p):
-        with open(p) as f:
-            d = json.load(f)
-        if d.get("variant") == variant:
-            return d.get("valu_wave_insts_per_launch")
-    return None
+def timed_run(s, steps):
+    """s.run(steps) timed by the wall clock and by two HIP events recorded on
+    the handle's own stream (every kernel of the run is on it); no per-kernel
+    events inside the loop."""
+    import torch
+    st = torch.cuda.ExternalStream(s.stream())
+    e0 = torch.cuda.Event(enable_timing=True)
+    e1 = torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    e0.record(st)
+    rep = s.run(steps)
+    e1.record(st)
+    e1.synchronize()
+    wall = time.perf_counter() - t0
+    return rep, wall, e0.elapsed_time(e1) * 1e-3
 
 
-def cpu_baseline(wl, budget_s, threads=1):
-    """The oracle (C restatement, device-math mode) timed on a bounded number
-    of steps of the same workload: one thread (the reference is
-    single-threaded), or `threads` OpenMP threads over the element and edge
-    loops (same bits)."""
+def wetdry_counts(mesh, wd):
+    """(dry elements, wet/dry edges: interior edges with one side flagged dry
+    and the other wet)."""
+    import numpy as np
+    e2e = np.asarray(mesh.edg_to_els).reshape(-1, 2)
+    a, b = wd[e2e[:, 0]], wd[e2e[:, 1]]
+    return int((wd == 0).sum()), int(((a == 0) != (b == 0)).sum())
+
+
+def wetdry_bench(a, variant):
+    """A wet/dry-active leg at the headline size: the dam break onto a dry bed
+    (N x N cells over 10 m x 10 m, 5 mm reservoir on x <= 5 m, dry film at H0
+    elsewhere, walls, wet/dry and limiter on, CFL 0.3), where the wet/dry
+    status, the dry-edge Roe fluxes and PDop act every stage.  Timed like the
+    headline; the dry elements and wet/dry edges are counted at the start and
+    the end of the timed steps."""
+    from dgrain import configs, native
+    wl = configs.dry_bed_dam_break(N=a.N, nsteps=a.steps)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_variant(variant)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    s.run(a.warmup)
+    s.sync()
+    start = wetdry_counts(wl.mesh, s.get_wd())
+    rep, wall, ev = timed_run(s, a.steps)
+    end = wetdry_counts(wl.mesh, s.get_wd())
+    s.close()
+    ne = wl.num_el
+    return {"workload": "dry_bed_dam_break_%dx%d" % (a.N, a.N), "triangles": ne, "steps": a.steps,
+            "value": 9.0 * ne * 2 * a.steps / ev, "unit": "DOF-updates/s", "ms_per_step": ev / a.steps * 1e3,
+            "wall_ms_per_step": wall / a.steps * 1e3, "sim_time_s": rep.t,
+            "dry_elements": {"start": start[0], "end": end[0]},
+            "wet_dry_edges": {"start": start[1], "end": end[1]},
+            "stage_roofline_frac": STAGE_BYTES_PER_EL * ne * 2 * a.steps / ev / 1e9 / HBM_PEAK_GBS}
+
+
+def cpu_baseline(wl, budget_s, threads=1, ref_flags=False):
+    """The oracle (C restatement) timed on a bounded number of steps of the
+    same workload: device-math mode at -O2, one thread (the reference is
+    single-threaded) or `threads` OpenMP threads over the element and edge
+    loops (same bits); or (ref_flags) reference-math mode built at the
+    reference's own flags, -g with no optimisation (2DCode/Makefile:8)."""
     import oracle
-    o = oracle.Oracle.for_workload(wl, math=oracle.MATH_DEV, threads=threads)
+    if ref_flags:
+        o = oracle.Oracle.for_workload(wl, math=oracle.MATH_REF, threads=threads, lib=oracle.LIB_G)
+    else:
+        o = oracle.Oracle.for_workload(wl, math=oracle.MATH_DEV, threads=threads)
     o.run(1)                     # first step uses dt0 = 1e-7; not timed
     steps = 0
     t0 = time.perf_counter()
@@ -126,8 +193,9 @@ def cpu_baseline(wl, budget_s, threads=1):
             break
     dofs = 9.0 * wl.num_el * 2 * steps
     return {"value": dofs / el, "unit": "DOF-updates/s per RK stage", "cores": threads, "kind": "port",
-            "sample": "%d SSP-RK2 steps of the same %d-triangle workload, oracle C restatement, %d thread%s, %.1f s"
-                      % (steps, wl.num_el, threads, "" if threads == 1 else "s", el)}
+            "sample": "%d SSP-RK2 steps of the same %d-triangle workload, oracle C restatement (%s), %d thread%s, %.1f s"
+                      % (steps, wl.num_el, "reference math, -g -O0 as 2DCode/Makefile:8" if ref_flags
+                         else "device math, -O2", threads, "" if threads == 1 else "s", el)}
 
 
 def ensemble_bench(a, variant):
@@ -316,22 +384,39 @@ def main():
     else:
         s.set_variant(a.variant)
         variant, tune_ms = a.variant, None
-    torch.cuda.synchronize()
-    s.profile_begin()
-    t0 = time.perf_counter()
-    rep = s.run(a.steps)
-    s.sync()
-    torch.cuda.synchronize()
-    el = time.perf_counter() - t0
-    k_ms, k_n = s.profile_end()
+    # the headline: HIP events on the handle's stream around the timed steps
+    # (no per-kernel events inside), wall clock beside them
+    rep, wall, ev = timed_run(s, a.steps)
     ne = wl.num_el
     dof_updates = 9.0 * ne * 2 * a.steps
-    value = dof_updates / el
+    value = dof_updates / ev
+    stage_s = ev / (2 * a.steps)                 # one RK stage, all its kernels
+    stage_gbs = STAGE_BYTES_PER_EL * ne / stage_s / 1e9
+    # the flux/RHS/update pair alone, in a separate profiled pass (per-kernel
+    # event pairs slow a run down, so they stay out of the headline)
+    s.profile_begin()
+    s.run(min(a.steps, 50))
+    k_ms, k_n = s.profile_end()
     k_avg_s = (k_ms / k_n) * 1e-3
-    achieved = KSTAGE_BYTES_PER_EL * ne / k_avg_s / 1e9
-    traffic = load_traffic(variant) if a.N == 512 else None
-    valu = load_valu(variant) if a.N == 512 else None
-    stage_gbs = STAGE_BYTES_PER_EL * ne * 2 * a.steps / el / 1e9
+    pair_gbs = KSTAGE_BYTES_PER_EL * ne / k_avg_s / 1e9
+    traffic, valu = load_pair(variant) if a.N == 512 else (None, None)
+    cnt = load_counters(variant) if a.N == 512 else None
+    stage_roof = {"kernel": "one RK stage (%s; k_step_end once per step, half to each stage)" % STAGE_ALL[variant],
+                  "achieved": stage_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stage_gbs / HBM_PEAK_GBS,
+                  "traffic": None, "avg_launch_ms": stage_s * 1e3, "launches": 2 * a.steps,
+                  "bytes_per_element": STAGE_BYTES_PER_EL, "bound": "hbm"}
+    fp64 = None
+    if cnt is not None:
+        ps = cnt["per_stage"]
+        hbm_frac = ps["bytes"] / stage_s / 1e9 / HBM_PEAK_GBS
+        valu_frac = ps["valu_insts"] / stage_s / 1e9 / VALU_PEAK_GWIS
+        flops = ps["f64_flops"] / stage_s      # lane FLOPs: 64 x the per-wave counters
+        stage_roof.update({"traffic": ps["bytes"], "traffic_frac_of_peak": hbm_frac, "valu_issue_frac": valu_frac,
+                           "bound": "valu" if valu_frac > hbm_frac else "hbm"})
+        fp64 = {"achieved": flops / 1e12, "peak": FP64_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": flops / 1e12 / FP64_PEAK_TFLOPS, "flops_per_stage": ps["f64_flops"],
+                "flops_per_element_stage": ps["f64_flops"] / ne,
+                "counters": "64 x (SQ_INSTS_VALU_FLOPS_FP64 + SQ_INSTS_VALU_FLOPS_FP64_TRANS): the counters count per wave-instruction"}
     out = {
         "metric": "FP64 DOF-updates/s per RK stage",
         "value": value,
@@ -339,7 +424,7 @@ def main():
         "n_gpus": 1,
         "steps": a.steps,
         "warmup": a.warmup,
-        "ms_per_step": el / a.steps * 1e3,
+        "ms_per_step": ev / a.steps * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
@@ -348,22 +433,31 @@ def main():
         "config": {"workload": "coastal_inundation_flood_start_%dx%d" % (a.N, a.N), "triangles": ne,
                    "dofs": 9 * ne, "p": 1, "rk": "SSP-RK2", "cfl": 0.3, "wetdry": True, "limiter": True,
                    "tide_m": 1.5, "sim_time_s": rep.t},
-        "roofline": {"bound": "hbm", "kernel": STAGE_KERNELS[variant], "achieved": achieved,
-                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": traffic, "avg_launch_ms": k_avg_s * 1e3, "launches": k_n,
-                     "bytes_per_element": KSTAGE_BYTES_PER_EL},
-        # the pair's real bound: FP64 VALU issue (wave-instructions/s against
-        # 1024 SIMDs x 2.4 GHz / 4 cycles), instruction count from the PMC pass
+        "roofline": stage_roof,
+        "fp64": fp64,
+        "wall_ms_per_step": wall / a.steps * 1e3,
+        # the flux/RHS/update kernel pair alone (HIP events around each
+        # launch of the pair, profiled pass)
+        "roofline_pair": {"bound": "valu", "kernel": STAGE_KERNELS[variant], "achieved": pair_gbs,
+                          "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": pair_gbs / HBM_PEAK_GBS,
+                          "traffic": traffic, "avg_launch_ms": k_avg_s * 1e3, "launches": k_n,
+                          "bytes_per_element": KSTAGE_BYTES_PER_EL},
+        # FP64 VALU issue of the pair (wave-instructions/s against 1024 SIMDs x
+        # 2.4 GHz / 4 cycles), instruction count from the PMC pass
         "valu_issue": None if valu is None else {
             "achieved": valu / k_avg_s / 1e9, "peak": VALU_PEAK_GWIS, "unit": "G wave-instructions/s",
             "frac": valu / k_avg_s / 1e9 / VALU_PEAK_GWIS, "insts_per_launch": valu},
-        "stage_roofline_frac": stage_gbs / HBM_PEAK_GBS,
         "stage_variant": {"chosen": variant, "autotune_ms_per_step": tune_ms},
         "ensemble_speedup": None,
         "adjoint_forward_cost": None,
     }
+    el = ev
     s.close()
     progress("single instance: %.3e DOF-updates/s, %.4f ms/step" % (value, el / a.steps * 1e3))
+    if not a.no_wetdry:
+        out["wetdry"] = wetdry_bench(a, variant)
+        progress("wet/dry leg: %.3e DOF-updates/s, %.4f ms/step" % (out["wetdry"]["value"],
+                                                                   out["wetdry"]["ms_per_step"]))
     if not a.no_ensemble:
         e = ensemble_bench(a, variant)
         out["ensemble"] = e
@@ -384,6 +478,7 @@ def main():
         nt = min(16, os.cpu_count() or 1)
         if nt > 1:
             out["cpu_baseline_threads"] = cpu_baseline(wl, a.cpu_seconds / 2, threads=nt)
+        out["cpu_baseline_reference_flags"] = cpu_baseline(wl, a.cpu_seconds, ref_flags=True)
     print(json.dumps(out))
 
 

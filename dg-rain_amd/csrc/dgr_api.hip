@@ -924,6 +924,13 @@ extern "C" dgr_status dgr_last_error(dgr_handle *h, int *code, int *where)
     return DGR_OK;
 }
 
+extern "C" dgr_status dgr_get_stream(dgr_handle *h, void **stream)
+{
+    if (!h || !stream) return DGR_ERR_ARG;
+    *stream = (void *)h->stream;
+    return DGR_OK;
+}
+
 extern "C" dgr_status dgr_sync(dgr_handle *h)
 {
     if (!h) return DGR_ERR_ARG;

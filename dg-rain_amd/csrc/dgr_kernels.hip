@@ -646,6 +646,7 @@ __device__ __forceinline__ unsigned long long block_reduce_parts(const unsigned 
 {
     __shared__ unsigned long long red[4];
     unsigned long long b = 0ULL;
+#pragma unroll 8
     for (int i = threadIdx.x; i < n; i += 256) b = (p[i] > b) ? p[i] : b;
     b = wave_max(b);
     if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = b;
@@ -1410,6 +1411,9 @@ __device__ __forceinline__ double member_fold_edges(const MeshDev &m, const Phys
     // a second one only when the member saw a NaN operand
     int P = -1;
     double R = -1.0;
+    // unrolled so that several partials' loads are in flight at once (one
+    // workgroup folds every partial of the member)
+#pragma unroll 8
     for (int b = threadIdx.x; b < M.dblks; b += 256) {
         const double v = __longlong_as_double((long long)lp[b]);
         P = (np[b] > P) ? np[b] : P;
@@ -1426,6 +1430,7 @@ __device__ __forceinline__ double member_fold_edges(const MeshDev &m, const Phys
     if (P < 0) return (R > 0.0) ? R : 0.0;
     __syncthreads();
     R = -1.0;
+#pragma unroll 8
     for (int b = threadIdx.x; b < M.dblks; b += 256) {
         const double v = __longlong_as_double((long long)lp[b]);
         if ((np[b] == P || 512LL * b > P) && v > R) R = v;
@@ -1473,6 +1478,7 @@ __device__ __forceinline__ int member_ties(const MeshDev &m, const PhysDev &ph, 
     const unsigned long long *tp = ph.tiemax + M.dblk0;
     const int *tn = ph.tiecnt + M.dblk0;
     double A = -1.0;
+#pragma unroll 8
     for (int b = threadIdx.x; b < M.dblks; b += 256) {
         const double v = __longlong_as_double((long long)tp[b]);
         A = (v > A) ? v : A;
@@ -1484,6 +1490,7 @@ __device__ __forceinline__ int member_ties(const MeshDev &m, const PhysDev &ph, 
 #pragma unroll
     for (int i = 1; i < 4; ++i) A = (ra[i] > A) ? ra[i] : A;
     int C = 0;
+#pragma unroll 8
     for (int b = threadIdx.x; b < M.dblks; b += 256)
         if (__longlong_as_double((long long)tp[b]) == A) C += tn[b];
     C = wave_sum(C);

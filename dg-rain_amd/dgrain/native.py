@@ -126,6 +126,7 @@ def load():
         "dgr_checkpoint_write": (C.c_int, [vp, C.c_int, C.c_char_p]),
         "dgr_checkpoint_read": (C.c_int, [vp, C.c_int, C.c_char_p]),
         "dgr_profile_end": (C.c_int, [vp, dp, ip]),
+        "dgr_get_stream": (C.c_int, [vp, C.POINTER(vp)]),
         "dgr_locality_order": (C.c_int, [C.c_int, C.c_int, ip, dp, dp, ip, ip]),
     }
     for name, (res, args) in sig.items():
@@ -262,6 +263,13 @@ class _Handle:
 
     def profile_begin(self):
         check(self.lib.dgr_profile_begin(self.h))
+
+    def stream(self):
+        """The handle's HIP stream as an integer handle (every kernel of the
+        handle runs on it), e.g. for torch.cuda.ExternalStream."""
+        p = C.c_void_p()
+        check(self.lib.dgr_get_stream(self.h, C.byref(p)), "dgr_get_stream")
+        return p.value
 
     def profile_end(self):
         """(summed ms, launches) of the flux/RHS/update kernels since profile_begin."""

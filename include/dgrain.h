@@ -328,6 +328,10 @@ dgr_status dgr_checkpoint_read(dgr_handle *h, int member, const char *path);
 
 /* Synchronise the handle's stream. */
 dgr_status dgr_sync(dgr_handle *h);
+/* The handle's HIP stream (a hipStream_t), on which every kernel of the
+ * handle runs: callers time it with their own HIP events, or order their own
+ * work after the handle's. */
+dgr_status dgr_get_stream(dgr_handle *h, void **stream);
 
 /* Device-resident state: pointer to the 9 SoA planes [zeta0..2, qx0..2, qy0..2]
  * of `pitch` doubles each (pitch >= num_el).  Valid until dgr_destroy. */

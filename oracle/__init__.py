@@ -16,6 +16,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "liboracle.so")
+# the same source built at the reference's flags (-g, no optimisation)
+LIB_G = os.path.join(HERE, "liboracle_g.so")
 REF2D = os.path.join(HERE, "_ref", "libref2dcode.so")
 REFCHAN = os.path.join(HERE, "_ref", "librefchan2d.so")
 
@@ -56,15 +58,14 @@ def _ip(a):
     return a.ctypes.data_as(C.POINTER(C.c_int))
 
 
-_olib = None
+_olibs = {}
 
 
-def _load_oracle():
-    global _olib
-    if _olib is None:
-        if not os.path.exists(LIB):
+def _load_oracle(path=LIB):
+    if path not in _olibs:
+        if not os.path.exists(path):
             build(ref=False)
-        lib = C.CDLL(LIB)
+        lib = C.CDLL(path)
         dp, ip, vp = C.POINTER(C.c_double), C.POINTER(C.c_int), C.c_void_p
         ub = C.POINTER(C.c_ubyte)
         sig = {
@@ -109,8 +110,8 @@ def _load_oracle():
             fn = getattr(lib, name)
             fn.restype = res
             fn.argtypes = args
-        _olib = lib
-    return _olib
+        _olibs[path] = lib
+    return _olibs[path]
 
 
 def hydro_eval(t, v, time):
@@ -184,8 +185,8 @@ class Oracle:
 
     def __init__(self, mesh, math=MATH_DEV, wetdry=0, cfl=0.1, dt0=1e-7, final_time=np.inf, rain=0.0,
                  tide_code=0, tide_datum=0.0, tide_amp=0.0, tide_period=43200.0, tide_phase=0.0,
-                 g=9.81, h0=1e-3, velzero=1e-2, threads=1, bc_sentinels=1, **_ignored):
-        lib = _load_oracle()
+                 g=9.81, h0=1e-3, velzero=1e-2, threads=1, bc_sentinels=1, lib=LIB, **_ignored):
+        lib = _load_oracle(lib)
         self.lib = lib
         self.mesh = mesh
         self._keep = [np.ascontiguousarray(a) for a in
@@ -203,8 +204,8 @@ class Oracle:
         self.dt = dt0
 
     @classmethod
-    def for_workload(cls, wl, math=MATH_DEV, threads=1):
-        o = cls(wl.mesh, math=math, threads=threads, **wl.params)
+    def for_workload(cls, wl, math=MATH_DEV, threads=1, lib=LIB):
+        o = cls(wl.mesh, math=math, threads=threads, lib=lib, **wl.params)
         o.set_state(wl.zeta, wl.qx, wl.qy)
         return o
 

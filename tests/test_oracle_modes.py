@@ -124,3 +124,16 @@ def test_flags_on_a_dry_bed_until_the_modes_drift_apart():
             assert np.array_equal(a.get_modify(st), b.get_modify(st)), (step, st)
         wet.add(int(wd.sum()))
     assert len(wet) > 10 and a.pdop_changed > 1000
+
+
+def test_oracle_at_the_reference_flags_gives_the_same_bits():
+    """liboracle_g.so (the restatement at 2DCode/Makefile:8's -g, no
+    optimisation; the CPU baseline at the reference's flags) and the -O2
+    build give the same trajectory bit for bit in both math modes."""
+    wl = configs.coastal(N=24, nsteps=20, rain_mm_h=10.0)
+    for math in (oracle.MATH_REF, oracle.MATH_DEV):
+        a = oracle.Oracle.for_workload(wl, math=math)
+        b = oracle.Oracle.for_workload(wl, math=math, lib=oracle.LIB_G)
+        assert a.run(20) == b.run(20)
+        for x, y in zip(a.get_state(), b.get_state()):
+            assert np.array_equal(x, y)
