@@ -38,10 +38,12 @@ FP64_PEAK_TFLOPS = 78.6
 # the kernel(s) each stage-kernel variant launches for the flux/RHS/update
 STAGE_KERNELS = {0: "k_stage (fused flux gather + RHS + RK update)",
                  1: "k_flux + k_rhs (element-centric fluxes, then RHS + RK update)",
-                 2: "k_edge_flux + k_rhs (edge-centric fluxes, then RHS + RK update)"}
+                 2: "k_edge_flux + k_rhs (edge-centric fluxes, then RHS + RK update)",
+                 3: "k_stage_persist (edge fluxes, RHS + RK update, node bounds, limiter in one cooperative launch)"}
 STAGE_ALL = {0: "k_stage, k_node_minmax, k_limit",
              1: "k_flux, k_rhs, k_node_minmax, k_limit",
-             2: "k_edge_flux, k_rhs, k_node_minmax, k_limit"}
+             2: "k_edge_flux, k_rhs, k_node_minmax, k_limit",
+             3: "k_stage_persist: edge fluxes, RHS, node bounds, limiter between grid barriers"}
 
 
 def progress(msg):

@@ -819,8 +819,52 @@ void enqueue_run_prologue(dgr_handle *h)
     if (h->ph.wetdry) launch_wd_status(h->m, h->ph, h->d_w, h->d_wd, h->d_wdn, h->stream);
 }
 
+// a run step of the persistent variant: two cooperative launches, the
+// buffers of the plain run step (see enqueue_run_step)
+static void enqueue_run_step_persist(dgr_handle *h)
+{
+    const bool wdon = h->ph.wetdry != 0;
+    PersistArgs a{};
+    a.bzeta = h->d_bzeta;
+    a.bqn = h->d_bqn;
+    a.fhat = h->d_fhat;
+    a.means = h->d_means;
+    a.nodemm = h->d_nodemm;
+    a.ctrls = h->d_ctrl;
+    // stage 1: w -> w1 (flags d_wdn; the limiter writes the mid-step flags to d_wd)
+    a.win = h->d_w;
+    a.u = h->d_w1;
+    a.wbase = nullptr;
+    a.wd = h->d_wdn;
+    a.lout = h->d_w1;
+    a.modify = modify_ptr(h, 0);
+    a.wd_lin = h->d_wdn;
+    a.wd_lout = wdon ? h->d_wd : nullptr;
+    a.next = 0;
+    hipError_t e = launch_stage_persist(1, wdon, h->m, h->ph, a, h->stream);
+    // stage 2: w1 -> w in place (flags d_wd; the limiter writes the next
+    // step's start flags to d_wdn), then the step end
+    a.win = h->d_w1;
+    a.u = h->d_w;
+    a.wbase = h->d_w;
+    a.wd = wdon ? h->d_wd : h->d_wdn;
+    a.lout = h->d_w;
+    a.modify = modify_ptr(h, 1);
+    a.wd_lin = wdon ? h->d_wd : nullptr;
+    a.wd_lout = wdon ? h->d_wdn : nullptr;
+    a.next = 1;
+    if (e == hipSuccess) e = launch_stage_persist(2, wdon, h->m, h->ph, a, h->stream);
+    if (e != hipSuccess) fprintf(stderr, "libdgrain: persistent stage launch failed: %s\n", hipGetErrorString(e));
+    h->kind[0] = h->kind[1] = PARTS_EDGE;
+    h->ties_last = false;
+}
+
 void enqueue_run_step(dgr_handle *h, const StepBufs *b, StepRec *rec_end, StepRec *rec_next)
 {
+    if (!b && !rec_end && !rec_next && h->variant == DGR_VARIANT_PERSIST) {
+        enqueue_run_step_persist(h);
+        return;
+    }
     const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true,
                           nullptr, nullptr, nullptr, nullptr};
     enqueue_stages(h, b ? b : &run);
@@ -877,7 +921,7 @@ extern "C" dgr_status dgr_set_graph(dgr_handle *h, int on)
 extern "C" dgr_status dgr_run(dgr_handle *h, int nsteps, dgr_report *rep)
 {
     if (!h || nsteps < 0) return DGR_ERR_ARG;
-    if (nsteps > 0 && h->use_graph && !h->profiling) {
+    if (nsteps > 0 && h->use_graph && !h->profiling && h->variant != DGR_VARIANT_PERSIST) {
         dgr_status st = step_graph(h);
         if (st) return st;
         enqueue_run_prologue(h);

@@ -217,4 +217,20 @@ struct StepRec {
 // which workgroup space a flux launch wrote its partials in
 enum PartKind : int { PARTS_ELEM = 0, PARTS_EDGE = 1 };
 
+// buffers of one persistent stage launch (k_stage_persist)
+struct PersistArgs {
+    const double *win;     // stage input
+    double *u;             // stage output before the limiter (RK update)
+    const double *wbase;   // step-start state (stage 2)
+    const double *bzeta, *bqn;
+    const int *wd;         // wet/dry flags the stage uses
+    double *fhat, *means, *nodemm;
+    double *lout;          // limiter (+ PDop) output
+    unsigned char *modify;
+    Ctrl *ctrls;
+    const int *wd_lin;     // the limiter's status update: flags in and out (null: none)
+    int *wd_lout;
+    int next;              // stage 2: the step end does the next step's bookkeeping
+};
+
 }  // namespace dgr

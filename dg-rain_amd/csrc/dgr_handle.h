@@ -14,6 +14,8 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
                  double *wout, const double *wbase, const double *bzeta, const double *bqn, const int *wd,
                  double *means, double *fhat, Ctrl *ctrl, int slot, double *dbg_rhs, double *dbg_fhat,
                  hipStream_t s, bool ties = false);
+hipError_t launch_stage_persist(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const PersistArgs &a,
+                                hipStream_t s);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,

@@ -18,9 +18,11 @@
 // reference's evaluation order operation for operation; the only intended
 // differences from the reference binary are pow(x,2) -> x*x and
 // pow(H,4./3) -> dgr_pow43 (see dgr_math.h).
+#include <hip/hip_cooperative_groups.h>
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/dgrain.h"
 #include "dgr_adjoint.h"
 #include "dgr_device.h"
 #include "dgr_math.h"
@@ -36,12 +38,12 @@ __device__ __forceinline__ int isign(double a) { return a < 0 ? -1 : (a == 0 ? 0
 // this bijection gives each XCD a contiguous range of workgroups (elements or
 // edges), so neighbour gathers and partial-line stores of one region meet in
 // one L2 instead of eight.
-__device__ __forceinline__ int xcd_block()
+__device__ __forceinline__ int xcd_map(int b, int nb)
 {
-    const int nb = (int)gridDim.x, b = (int)blockIdx.x;
     const int q = nb >> 3, r = nb & 7, x = b & 7, j = b >> 3;
     return (x < r) ? x * (q + 1) + j : r * (q + 1) + (x - r) * q + j;
 }
+__device__ __forceinline__ int xcd_block() { return xcd_map((int)blockIdx.x, (int)gridDim.x); }
 
 // record an error; the smallest key (earliest step, stage, phase, then
 // lowest member-local edge/element id) wins, which is the error the
@@ -688,16 +690,17 @@ struct ElemLane {
     int lb, mid, e;
     bool valid;
 };
-__device__ __forceinline__ ElemLane elem_lane(const MeshDev &m)
+__device__ __forceinline__ ElemLane elem_lane_at(const MeshDev &m, int lb)
 {
     ElemLane L;
-    L.lb = xcd_block();
+    L.lb = lb;
     L.mid = m.eblk_mem[L.lb];
     L.e = L.lb * 256 + (int)threadIdx.x;
     const MemberDev &M = m.mem[L.mid];
     L.valid = L.e < M.el_off + M.ne;
     return L;
 }
+__device__ __forceinline__ ElemLane elem_lane(const MeshDev &m) { return elem_lane_at(m, xcd_block()); }
 
 // ---------------------------------------------------------------------------
 // k_stage: flux gather + RHS + RK update of one element per thread (fused).
@@ -810,12 +813,11 @@ __global__ __launch_bounds__(256) void k_flux(MeshDev m, PhysDev ph, const doubl
 // flux.  The max-|lambda| operands are the edge's own positions 2*edge and
 // 2*edge+1, so the fold bookkeeping is that of k_flux.
 template <bool WETDRY>
-__global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
-                                                   const double *__restrict__ bzeta, const double *__restrict__ bqn,
-                                                   const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrls,
-                                                   int slot, int ties)
+__device__ __forceinline__ void edge_flux_block(const MeshDev &m, const PhysDev &ph, const double *__restrict__ win,
+                                                const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                                const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrls,
+                                                int slot, int ties, int lb)
 {
-    const int lb = xcd_block();
     const int mid = m.dblk_mem[lb];
     const MemberDev &M = m.mem[mid];
     Ctrl *ctrl = ctrls + mid;
@@ -958,15 +960,24 @@ __global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const 
     if (ties) tie_commit(ta, tc, ph, lb);
 }
 
+template <bool WETDRY>
+__global__ __launch_bounds__(256) void k_edge_flux(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                                   const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                                   const int *__restrict__ wd, double *__restrict__ fhat, Ctrl *ctrls,
+                                                   int slot, int ties)
+{
+    edge_flux_block<WETDRY>(m, ph, win, bzeta, bqn, wd, fhat, ctrls, slot, ties, xcd_block());
+}
+
 // Split variants, part 2: inner products, RHS and RK update from the Fhat planes.
 //   MODE 0: RHS only into dbg_rhs; MODE 1/2 as in k_stage.
 template <int MODE>
-__global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const double *__restrict__ win, double *wout,
-                                             const double *wbase, const double *__restrict__ fhat,
-                                             const int *__restrict__ wd, double *__restrict__ means,
-                                             const Ctrl *ctrls, double *__restrict__ dbg_rhs)
+__device__ __forceinline__ void rhs_block(const MeshDev &m, const PhysDev &ph, const double *__restrict__ win,
+                                          double *wout, const double *wbase, const double *__restrict__ fhat,
+                                          const int *__restrict__ wd, double *__restrict__ means, const Ctrl *ctrls,
+                                          double *__restrict__ dbg_rhs, int lb)
 {
-    const ElemLane L = elem_lane(m);
+    const ElemLane L = elem_lane_at(m, lb);
     if (!L.valid) return;
     const MemberDev &M = m.mem[L.mid];
     const Ctrl *ctrl = ctrls + L.mid;
@@ -1015,6 +1026,15 @@ __global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const dou
     }
 }
 
+template <int MODE>
+__global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const double *__restrict__ win, double *wout,
+                                             const double *wbase, const double *__restrict__ fhat,
+                                             const int *__restrict__ wd, double *__restrict__ means,
+                                             const Ctrl *ctrls, double *__restrict__ dbg_rhs)
+{
+    rhs_block<MODE>(m, ph, win, wout, wbase, fhat, wd, means, ctrls, dbg_rhs, xcd_block());
+}
+
 // element means of the current state (standalone limiter calls)
 __global__ __launch_bounds__(256) void k_means(MeshDev m, const double *__restrict__ w, double *__restrict__ means)
 {
@@ -1029,10 +1049,10 @@ __global__ __launch_bounds__(256) void k_means(MeshDev m, const double *__restri
 // ARG: also the first element that attains each bound (-1: the sentinel
 // stood), which the limiter's reverse routes the bound adjoints to.
 template <bool ARG>
-__global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means,
-                                                     double *__restrict__ nodemm, int *__restrict__ nodearg)
+__device__ __forceinline__ void node_minmax_block(const MeshDev &m, const double *__restrict__ means,
+                                                  double *__restrict__ nodemm, int *__restrict__ nodearg, int lb)
 {
-    const int i = xcd_block() * 256 + (int)threadIdx.x;
+    const int i = lb * 256 + (int)threadIdx.x;
     if (i >= m.nn) return;
     double mn[3] = {99999, 99999, 99999}, mx[3] = {-99999, -99999, -99999};
     int amn[3] = {-1, -1, -1}, amx[3] = {-1, -1, -1};
@@ -1056,6 +1076,13 @@ __global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__
             nodearg[(3 + f) * nn + i] = amx[f];
         }
     }
+}
+
+template <bool ARG>
+__global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means,
+                                                     double *__restrict__ nodemm, int *__restrict__ nodearg)
+{
+    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_block());
 }
 
 // PDop2D on one element (wetDry2D.c:66-256); zeta/qx/qy are the element's
@@ -1170,12 +1197,12 @@ __device__ __forceinline__ int wd_status_of(double a, double b, double c, const 
 // state follows (the reference's next wetDryStatus2D call, which always comes
 // right after the limiter): wd_out from wd_in (may alias).
 template <bool PDOP>
-__global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const double *win, double *wout,
-                                               const double *__restrict__ nodemm,
-                                               unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
-                                               const int *wd_in, int *wd_out)
+__device__ __forceinline__ void limit_block(const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
+                                            const double *__restrict__ nodemm,
+                                            unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
+                                            const int *wd_in, int *wd_out, int lb)
 {
-    const ElemLane L = elem_lane(m);
+    const ElemLane L = elem_lane_at(m, lb);
     if (!L.valid) return;
     const int e = L.e;
     const int P = m.pitch, nn = m.nn;
@@ -1278,6 +1305,15 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const doub
 #pragma unroll
     for (int f = 0; f < 9; ++f) wout[f * P + e] = S[f];
     if (modify_out) modify_out[e] = (unsigned char)modify;
+}
+
+template <bool PDOP>
+__global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const double *win, double *wout,
+                                               const double *__restrict__ nodemm,
+                                               unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
+                                               const int *wd_in, int *wd_out)
+{
+    limit_block<PDOP>(m, ph, win, wout, nodemm, modify_out, ctrls, stage, wd_in, wd_out, xcd_block());
 }
 
 // standalone PDop2D
@@ -1505,10 +1541,9 @@ __device__ __forceinline__ int member_ties(const MeshDev &m, const PhysDev &ph, 
 // counted), which k_step_begin does otherwise.  Optional step records: the
 // step's max |lambda| into rec_end, the next step's t, dt and tide into
 // rec_next (after the next step's bookkeeping).
-__global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind, int next,
-                                                  StepRec *rec_end, StepRec *rec_next, int ties)
+__device__ __forceinline__ void step_end_member(const MeshDev &m, const PhysDev &ph, Ctrl *ctrls, int kind, int next,
+                                                StepRec *rec_end, StepRec *rec_next, int ties, int mid)
 {
-    const int mid = blockIdx.x;
     Ctrl *c = ctrls + mid;
     const double ml = resolve_lam(m, ph, c, mid, 1, kind);
     int ntie = 0;
@@ -1531,6 +1566,50 @@ __global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *c
         rec_next[mid].t = c->t;
         rec_next[mid].dt = c->dt;
         rec_next[mid].tide = c->tide;
+    }
+}
+
+__global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *ctrls, int kind, int next,
+                                                  StepRec *rec_end, StepRec *rec_next, int ties)
+{
+    step_end_member(m, ph, ctrls, kind, next, rec_end, rec_next, ties, (int)blockIdx.x);
+}
+
+// ---------------------------------------------------------------------------
+// Persistent variant: one cooperative launch runs a whole RK stage of every
+// member: edge fluxes, grid barrier, RHS + RK update, barrier, node bounds,
+// barrier, limiter (+ PDop and the wet/dry status), and in stage 2 a barrier
+// and the step end.  The grid is as many workgroups as fit on the device at
+// once (a multiple of 8); workgroup b runs virtual workgroups b, b + G, ...
+// of each phase, which lie on b's XCD, through the same XCD mapping as the
+// separate launches.  Same device functions, same bits.
+// ---------------------------------------------------------------------------
+template <int MODE, bool WETDRY>
+__global__ __launch_bounds__(256, 2) void k_stage_persist(MeshDev m, PhysDev ph, PersistArgs a)
+{
+    cooperative_groups::grid_group grid = cooperative_groups::this_grid();
+    const int G = (int)gridDim.x, b = (int)blockIdx.x;
+    const int nv_ed = m.ned_pad / 256, nv_el = m.pitch / 256, nv_nd = (m.nn + 255) / 256;
+    const int slot = MODE - 1;
+    for (int v = b; v < nv_ed; v += G) {
+        edge_flux_block<WETDRY>(m, ph, a.win, a.bzeta, a.bqn, a.wd, a.fhat, a.ctrls, slot, 0, xcd_map(v, nv_ed));
+        __syncthreads();   // the workgroup reductions' LDS is reused by the next virtual workgroup
+    }
+    grid.sync();
+    for (int v = b; v < nv_el; v += G)
+        rhs_block<MODE>(m, ph, a.win, a.u, a.wbase, a.fhat, a.wd, a.means, a.ctrls, nullptr, xcd_map(v, nv_el));
+    grid.sync();
+    for (int v = b; v < nv_nd; v += G) node_minmax_block<false>(m, a.means, a.nodemm, nullptr, xcd_map(v, nv_nd));
+    grid.sync();
+    for (int v = b; v < nv_el; v += G)
+        limit_block<WETDRY>(m, ph, a.u, a.lout, a.nodemm, a.modify, a.ctrls, slot, a.wd_lin, a.wd_lout,
+                            xcd_map(v, nv_el));
+    if (MODE == 2) {
+        grid.sync();
+        for (int mid = b; mid < m.nmem; mid += G) {
+            step_end_member(m, ph, a.ctrls, PARTS_EDGE, a.next, nullptr, nullptr, 0, mid);
+            __syncthreads();
+        }
     }
 }
 
@@ -1590,6 +1669,7 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
     hipLaunchKernelGGL((k_stage<MD, WDF>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
 #define DGR_LR(MD) \
     hipLaunchKernelGGL((k_rhs<MD>), g, b, 0, s, m, ph, win, wout, wbase, mode == 0 ? dbg_fhat : fhat, wd, means, ctrl, dbg_rhs)
+    if (variant == DGR_VARIANT_PERSIST) variant = 2;   // single stages: the separate launches
     if (variant == 0 || mode == 3) {
         if (wetdry) {
             if (mode == 0) DGR_LS(0, true); else if (mode == 1) DGR_LS(1, true);
@@ -1656,6 +1736,30 @@ void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, cons
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s)
 {
     hipLaunchKernelGGL(k_step_begin, member_grid(m), dim3(64), 0, s, m, c);
+}
+
+// workgroups of a persistent launch: as many as fit at once, a multiple of 8
+static int persist_grid(const void *kernel)
+{
+    int dev = 0, cus = 0, per_cu = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess) return 0;
+    return (per_cu * cus) & ~7;
+}
+
+hipError_t launch_stage_persist(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const PersistArgs &a,
+                                hipStream_t s)
+{
+    const void *k = mode == 1 ? (wetdry ? (const void *)k_stage_persist<1, true> : (const void *)k_stage_persist<1, false>)
+                              : (wetdry ? (const void *)k_stage_persist<2, true> : (const void *)k_stage_persist<2, false>);
+    const int G = persist_grid(k);
+    if (G < 8) return hipErrorInvalidConfiguration;
+    MeshDev mm = m;
+    PhysDev pp = ph;
+    PersistArgs aa = a;
+    void *args[] = {(void *)&mm, (void *)&pp, (void *)&aa};
+    return hipLaunchCooperativeKernel(k, dim3((unsigned)G), dim3(256), args, 0, s);
 }
 
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,

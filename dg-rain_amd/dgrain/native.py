@@ -13,7 +13,7 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 PKG_ROOT = os.path.dirname(_HERE)                      # .../dg-rain_amd
 REPO_ROOT = os.path.dirname(PKG_ROOT)
 LIB_PATH = os.path.join(PKG_ROOT, "lib", "libdgrain.so")
-NUM_VARIANTS = 3   # DGR_NUM_VARIANTS in include/dgrain.h
+NUM_VARIANTS = 4   # DGR_NUM_VARIANTS in include/dgrain.h
 
 DGR_OK = 0
 DGR_ERR_ARG = 1

@@ -207,11 +207,17 @@ dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *launches);
  *       products and RK update);
  *   1 = element-centric fluxes into Fhat planes, then the RHS/update kernel;
  *   2 = edge-centric fluxes (one Roe evaluation per edge) into Fhat planes,
- *       then the RHS/update kernel.
+ *       then the RHS/update kernel;
+ *   3 = variant 2's kernels as one persistent cooperative launch per RK stage
+ *       of a run step (grid barriers between the flux, RHS, node-bound and
+ *       limiter phases; stage 2 also ends the step).  Single operators and
+ *       taped steps use variant 2's separate launches; a run step in this
+ *       variant is not captured into a hipGraph.
  * dgr_autotune times `steps` steps of each from the current state (restored
  * afterwards), keeps the fastest and reports ms/step per variant in
  * ms_per_step[DGR_NUM_VARIANTS] (may be NULL). */
-#define DGR_NUM_VARIANTS 3
+#define DGR_NUM_VARIANTS 4
+#define DGR_VARIANT_PERSIST 3
 dgr_status dgr_set_variant(dgr_handle *h, int variant);
 dgr_status dgr_get_variant(dgr_handle *h, int *variant);
 dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step);
