@@ -71,6 +71,8 @@ def parse():
     ap.add_argument("--ens-steps", type=int, default=20, help="timed steps of the ensemble and of each solo run")
     ap.add_argument("--variant", type=int, default=-1,
                     help="stage-kernel variant to time (default: pick the fastest with dgr_autotune)")
+    ap.add_argument("--block", type=int, default=0,
+                    help="workgroup size of the element/node kernels (default: pick with dgr_autotune_block)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run 5 device copies of 512 MiB (HBM counter calibration) and exit")
     return ap.parse_args()
@@ -216,6 +218,7 @@ def ensemble_bench(a, variant):
     ens.run(2)
     ens.sync()
     ens_variant, ens_tune = ens.autotune(3)
+    ens_block, ens_block_ms = ens.autotune_block(3)
     ens.run(1)
     ens.sync()
     t0 = time.perf_counter()
@@ -227,6 +230,7 @@ def ensemble_bench(a, variant):
     for w in wls:
         s = native.Solver(w.mesh, w.solver_params())
         s.set_variant(ens_variant)
+        s.set_block(ens_block)
         s.set_state(w.zeta, w.qx, w.qy)
         s.run(3)
         s.sync()
@@ -239,6 +243,7 @@ def ensemble_bench(a, variant):
     v_ens, v_solo = dofs / t_ens, dofs / t_solo
     return {"members": len(wls), "triangles": ne_tot, "sizes_N": [int(round((w.num_el / 2) ** 0.5)) for w in wls],
             "steps": a.ens_steps, "variant": ens_variant, "autotune_ms_per_step": ens_tune,
+            "block": ens_block, "block_autotune_ms_per_step_64_128_256": ens_block_ms,
             "value": v_ens, "unit": "DOF-updates/s", "ms_per_step": t_ens / a.ens_steps * 1e3,
             "solo_value": v_solo, "solo_ms_per_step": t_solo / a.ens_steps * 1e3,
             "speedup": v_ens / v_solo,
@@ -386,6 +391,12 @@ def main():
     else:
         s.set_variant(a.variant)
         variant, tune_ms = a.variant, None
+    # and the workgroup size of the element and node kernels
+    if a.block <= 0:
+        block, block_ms = s.autotune_block(10)
+    else:
+        s.set_block(a.block)
+        block, block_ms = a.block, None
     # the headline: HIP events on the handle's stream around the timed steps
     # (no per-kernel events inside), wall clock beside them
     rep, wall, ev = timed_run(s, a.steps)
@@ -449,7 +460,8 @@ def main():
         "valu_issue": None if valu is None else {
             "achieved": valu / k_avg_s / 1e9, "peak": VALU_PEAK_GWIS, "unit": "G wave-instructions/s",
             "frac": valu / k_avg_s / 1e9 / VALU_PEAK_GWIS, "insts_per_launch": valu},
-        "stage_variant": {"chosen": variant, "autotune_ms_per_step": tune_ms},
+        "stage_variant": {"chosen": variant, "autotune_ms_per_step": tune_ms, "block": block,
+                          "block_autotune_ms_per_step_64_128_256": block_ms},
         "ensemble_speedup": None,
         "adjoint_forward_cost": None,
     }

@@ -312,6 +312,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     m.nodes = h->d_nodes; m.ev = h->d_ev; m.nb = h->d_nb; m.info = h->d_info; m.eid = h->d_eid;
     m.n2e_off = h->d_n2e_off; m.n2e = h->d_n2e; m.edges = h->d_edges; m.einfo = h->d_einfo;
     m.mem = h->d_mem; m.eblk_mem = h->d_eblk_mem; m.dblk_mem = h->d_dblk_mem;
+    m.wg = 256;
     PhysDev &ph = h->ph;
     ph.g = prms[0].g; ph.h0 = prms[0].h0; ph.velzero = prms[0].velzero; ph.wetdry = prms[0].wetdry;
     ph.lamparts = h->d_lamparts; ph.nanparts = h->d_nanparts; ph.maxparts = maxparts;
@@ -885,7 +886,7 @@ extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)
 // one step of every member captured as a hipGraph (rebuilt when the variant changes)
 static dgr_status step_graph(dgr_handle *h)
 {
-    if (h->graph_exec && h->graph_variant == h->variant) return DGR_OK;
+    if (h->graph_exec && h->graph_variant == h->variant && h->graph_wg == h->m.wg) return DGR_OK;
     if (h->graph_exec) {
         DGR_HIP(hipGraphExecDestroy(h->graph_exec));
         h->graph_exec = nullptr;
@@ -908,6 +909,7 @@ static dgr_status step_graph(dgr_handle *h)
         return DGR_ERR_HIP;
     }
     h->graph_variant = h->variant;
+    h->graph_wg = h->m.wg;
     return DGR_OK;
 }
 
@@ -1112,12 +1114,13 @@ extern "C" dgr_status dgr_get_variant(dgr_handle *h, int *variant)
     return DGR_OK;
 }
 
-// Times `steps` steps of every kernel variant from the current state and keeps
-// the fastest.  All variants compute the same bits; the state, WD flags and
-// step bookkeeping of every member are restored afterwards.
-extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step)
+// Times run steps (the path dgr_run takes) of each configuration that try()
+// sets up, from the current state, which is restored after each (state, WD
+// flags and step bookkeeping of every member); ms[i]: ms per step of
+// configuration i.  All configurations compute the same bits.
+template <typename Try>
+static dgr_status time_configs(dgr_handle *h, int steps, int n, Try &&try_config, double *ms_out)
 {
-    if (!h || steps <= 0) return DGR_ERR_ARG;
     const size_t P = h->pitch;
     double *w_save = nullptr;
     int *wd_save = nullptr;
@@ -1133,20 +1136,19 @@ extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double
     DGR_HIP(hipEventCreate(&e1));
     const int prof = h->profiling;
     h->profiling = 0;
-    int best = h->variant;
-    float best_ms = 1e30f;
-    for (int v = 0; v < DGR_NUM_VARIANTS; ++v) {
-        h->variant = v;
-        // one untimed step to warm the code object, then the timed steps
-        enqueue_step(h, nullptr);
+    for (int i = 0; i < n; ++i) {
+        try_config(i);
+        // one untimed step to warm the code objects, then the timed steps
+        enqueue_run_prologue(h);
+        enqueue_run_step(h);
         DGR_HIP(hipEventRecord(e0, h->stream));
-        for (int n = 0; n < steps; ++n) enqueue_step(h, nullptr);
+        for (int k = 0; k < steps; ++k) enqueue_run_step(h);
         DGR_HIP(hipEventRecord(e1, h->stream));
         DGR_HIP(hipEventSynchronize(e1));
+        DGR_HIP(hipGetLastError());
         float ms = 0.f;
         DGR_HIP(hipEventElapsedTime(&ms, e0, e1));
-        if (ms_per_step) ms_per_step[v] = ms / steps;
-        if (ms < best_ms) { best_ms = ms; best = v; }
+        ms_out[i] = ms / steps;
         DGR_HIP(hipMemcpyAsync(h->d_w, w_save, 9 * P * sizeof(double), hipMemcpyDeviceToDevice, h->stream));
         DGR_HIP(hipMemcpyAsync(h->d_wd, wd_save, P * sizeof(int), hipMemcpyDeviceToDevice, h->stream));
         DGR_HIP(hipMemcpyAsync(h->d_ctrl, c_save.data(), h->nmem * sizeof(Ctrl), hipMemcpyHostToDevice, h->stream));
@@ -1157,7 +1159,62 @@ extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double
     (void)hipEventDestroy(e1);
     (void)hipFree(w_save);
     (void)hipFree(wd_save);
+    return DGR_OK;
+}
+
+// Times `steps` run steps of every kernel variant from the current state and
+// keeps the fastest.
+extern "C" dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step)
+{
+    if (!h || steps <= 0) return DGR_ERR_ARG;
+    double ms[DGR_NUM_VARIANTS];
+    const int old = h->variant;
+    const dgr_status st = time_configs(h, steps, DGR_NUM_VARIANTS, [&](int v) { h->variant = v; }, ms);
+    if (st != DGR_OK) {
+        h->variant = old;
+        return st;
+    }
+    int best = 0;
+    for (int v = 0; v < DGR_NUM_VARIANTS; ++v) {
+        if (ms_per_step) ms_per_step[v] = ms[v];
+        if (ms[v] < ms[best]) best = v;
+    }
     h->variant = best;
     if (chosen) *chosen = best;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_set_block(dgr_handle *h, int threads)
+{
+    if (!h || (threads != 64 && threads != 128 && threads != 256)) return DGR_ERR_ARG;
+    h->m.wg = threads;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_get_block(dgr_handle *h, int *threads)
+{
+    if (!h || !threads) return DGR_ERR_ARG;
+    *threads = h->m.wg;
+    return DGR_OK;
+}
+
+extern "C" dgr_status dgr_autotune_block(dgr_handle *h, int steps, int *chosen, double *ms_per_step)
+{
+    if (!h || steps <= 0) return DGR_ERR_ARG;
+    static const int sizes[3] = {64, 128, 256};
+    double ms[3];
+    const int old = h->m.wg;
+    const dgr_status st = time_configs(h, steps, 3, [&](int i) { h->m.wg = sizes[i]; }, ms);
+    if (st != DGR_OK) {
+        h->m.wg = old;
+        return st;
+    }
+    int best = 2;
+    for (int i = 0; i < 3; ++i) {
+        if (ms_per_step) ms_per_step[i] = ms[i];
+        if (ms[i] < ms[best]) best = i;
+    }
+    h->m.wg = sizes[best];
+    if (chosen) *chosen = sizes[best];
     return DGR_OK;
 }

@@ -111,6 +111,7 @@ struct MeshDev {
     const int *eblk_mem;
     const int *dblk_mem;
     const double *hydro;      // hydrograph tables of all members
+    int wg;                   // workgroup size of k_rhs, k_node_minmax and k_limit (64, 128 or 256)
 };
 
 // one per member

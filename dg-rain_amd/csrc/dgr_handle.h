@@ -112,6 +112,7 @@ struct dgr_handle {
     // one captured step (dgr_set_graph)
     int use_graph;
     int graph_variant;
+    int graph_wg;    // MeshDev::wg the graph was captured with
     hipGraphExec_t graph_exec;
     // HBM the adjoint may keep taped segments in (dgr_set_tape_budget):
     // < 0 half the free device memory, 0 recompute every segment

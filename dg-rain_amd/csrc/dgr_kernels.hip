@@ -701,6 +701,18 @@ __device__ __forceinline__ ElemLane elem_lane_at(const MeshDev &m, int lb)
     return L;
 }
 __device__ __forceinline__ ElemLane elem_lane(const MeshDev &m) { return elem_lane_at(m, xcd_block()); }
+// element kernels whose workgroups carry no partials may run with fewer than
+// 256 threads: the element of a thread, its 256-element range and member
+__device__ __forceinline__ ElemLane elem_lane_thread(const MeshDev &m)
+{
+    ElemLane L;
+    L.e = xcd_block() * (int)blockDim.x + (int)threadIdx.x;
+    L.lb = L.e >> 8;
+    L.mid = m.eblk_mem[L.lb];
+    const MemberDev &M = m.mem[L.mid];
+    L.valid = L.e < M.el_off + M.ne;
+    return L;
+}
 
 // ---------------------------------------------------------------------------
 // k_stage: flux gather + RHS + RK update of one element per thread (fused).
@@ -975,9 +987,8 @@ template <int MODE>
 __device__ __forceinline__ void rhs_block(const MeshDev &m, const PhysDev &ph, const double *__restrict__ win,
                                           double *wout, const double *wbase, const double *__restrict__ fhat,
                                           const int *__restrict__ wd, double *__restrict__ means, const Ctrl *ctrls,
-                                          double *__restrict__ dbg_rhs, int lb)
+                                          double *__restrict__ dbg_rhs, const ElemLane &L)
 {
-    const ElemLane L = elem_lane_at(m, lb);
     if (!L.valid) return;
     const MemberDev &M = m.mem[L.mid];
     const Ctrl *ctrl = ctrls + L.mid;
@@ -1032,7 +1043,7 @@ __global__ __launch_bounds__(256, 2) void k_rhs(MeshDev m, PhysDev ph, const dou
                                              const int *__restrict__ wd, double *__restrict__ means,
                                              const Ctrl *ctrls, double *__restrict__ dbg_rhs)
 {
-    rhs_block<MODE>(m, ph, win, wout, wbase, fhat, wd, means, ctrls, dbg_rhs, xcd_block());
+    rhs_block<MODE>(m, ph, win, wout, wbase, fhat, wd, means, ctrls, dbg_rhs, elem_lane_thread(m));
 }
 
 // element means of the current state (standalone limiter calls)
@@ -1050,9 +1061,8 @@ __global__ __launch_bounds__(256) void k_means(MeshDev m, const double *__restri
 // stood), which the limiter's reverse routes the bound adjoints to.
 template <bool ARG>
 __device__ __forceinline__ void node_minmax_block(const MeshDev &m, const double *__restrict__ means,
-                                                  double *__restrict__ nodemm, int *__restrict__ nodearg, int lb)
+                                                  double *__restrict__ nodemm, int *__restrict__ nodearg, int i)
 {
-    const int i = lb * 256 + (int)threadIdx.x;
     if (i >= m.nn) return;
     double mn[3] = {99999, 99999, 99999}, mx[3] = {-99999, -99999, -99999};
     int amn[3] = {-1, -1, -1}, amx[3] = {-1, -1, -1};
@@ -1082,7 +1092,7 @@ template <bool ARG>
 __global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means,
                                                      double *__restrict__ nodemm, int *__restrict__ nodearg)
 {
-    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_block());
+    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_block() * (int)blockDim.x + (int)threadIdx.x);
 }
 
 // PDop2D on one element (wetDry2D.c:66-256); zeta/qx/qy are the element's
@@ -1200,9 +1210,8 @@ template <bool PDOP>
 __device__ __forceinline__ void limit_block(const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
                                             const double *__restrict__ nodemm,
                                             unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
-                                            const int *wd_in, int *wd_out, int lb)
+                                            const int *wd_in, int *wd_out, const ElemLane &L)
 {
-    const ElemLane L = elem_lane_at(m, lb);
     if (!L.valid) return;
     const int e = L.e;
     const int P = m.pitch, nn = m.nn;
@@ -1313,7 +1322,7 @@ __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const doub
                                                unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
                                                const int *wd_in, int *wd_out)
 {
-    limit_block<PDOP>(m, ph, win, wout, nodemm, modify_out, ctrls, stage, wd_in, wd_out, xcd_block());
+    limit_block<PDOP>(m, ph, win, wout, nodemm, modify_out, ctrls, stage, wd_in, wd_out, elem_lane_thread(m));
 }
 
 // standalone PDop2D
@@ -1597,13 +1606,15 @@ __global__ __launch_bounds__(256, 2) void k_stage_persist(MeshDev m, PhysDev ph,
     }
     grid.sync();
     for (int v = b; v < nv_el; v += G)
-        rhs_block<MODE>(m, ph, a.win, a.u, a.wbase, a.fhat, a.wd, a.means, a.ctrls, nullptr, xcd_map(v, nv_el));
+        rhs_block<MODE>(m, ph, a.win, a.u, a.wbase, a.fhat, a.wd, a.means, a.ctrls, nullptr,
+                        elem_lane_at(m, xcd_map(v, nv_el)));
     grid.sync();
-    for (int v = b; v < nv_nd; v += G) node_minmax_block<false>(m, a.means, a.nodemm, nullptr, xcd_map(v, nv_nd));
+    for (int v = b; v < nv_nd; v += G)
+        node_minmax_block<false>(m, a.means, a.nodemm, nullptr, xcd_map(v, nv_nd) * 256 + (int)threadIdx.x);
     grid.sync();
     for (int v = b; v < nv_el; v += G)
         limit_block<WETDRY>(m, ph, a.u, a.lout, a.nodemm, a.modify, a.ctrls, slot, a.wd_lin, a.wd_lout,
-                            xcd_map(v, nv_el));
+                            elem_lane_at(m, xcd_map(v, nv_el)));
     if (MODE == 2) {
         grid.sync();
         for (int mid = b; mid < m.nmem; mid += G) {
@@ -1665,10 +1676,11 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
                  hipStream_t s, bool ties)
 {
     const dim3 g = grid_for(m.pitch), b(256);
+    const dim3 gw((unsigned)(m.pitch / m.wg)), bw((unsigned)m.wg);
 #define DGR_LS(MD, WDF) \
     hipLaunchKernelGGL((k_stage<MD, WDF>), g, b, 0, s, m, ph, win, wout, wbase, bzeta, bqn, wd, means, ctrl, slot, dbg_rhs, dbg_fhat)
 #define DGR_LR(MD) \
-    hipLaunchKernelGGL((k_rhs<MD>), g, b, 0, s, m, ph, win, wout, wbase, mode == 0 ? dbg_fhat : fhat, wd, means, ctrl, dbg_rhs)
+    hipLaunchKernelGGL((k_rhs<MD>), gw, bw, 0, s, m, ph, win, wout, wbase, mode == 0 ? dbg_fhat : fhat, wd, means, ctrl, dbg_rhs)
     if (variant == DGR_VARIANT_PERSIST) variant = 2;   // single stages: the separate launches
     if (variant == 0 || mode == 3) {
         if (wetdry) {
@@ -1706,20 +1718,20 @@ void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t 
 
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s)
 {
-    if (nodearg) hipLaunchKernelGGL(k_node_minmax<true>, grid_for(m.nn), dim3(256), 0, s, m, means, nodemm, nodearg);
-    else hipLaunchKernelGGL(k_node_minmax<false>, grid_for(m.nn), dim3(256), 0, s, m, means, nodemm, nodearg);
+    const dim3 g((unsigned)((m.nn + m.wg - 1) / m.wg)), b((unsigned)m.wg);
+    if (nodearg) hipLaunchKernelGGL(k_node_minmax<true>, g, b, 0, s, m, means, nodemm, nodearg);
+    else hipLaunchKernelGGL(k_node_minmax<false>, g, b, 0, s, m, means, nodemm, nodearg);
 }
 
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
                   const double *nodemm, unsigned char *modify, Ctrl *ctrl, int stage, const int *wd_in, int *wd_out,
                   hipStream_t s)
 {
+    const dim3 g((unsigned)(m.pitch / m.wg)), b((unsigned)m.wg);
     if (pdop)
-        hipLaunchKernelGGL(k_limit<true>, grid_for(m.pitch), dim3(256), 0, s, m, ph, win, wout, nodemm, modify, ctrl,
-                           stage, wd_in, wd_out);
+        hipLaunchKernelGGL(k_limit<true>, g, b, 0, s, m, ph, win, wout, nodemm, modify, ctrl, stage, wd_in, wd_out);
     else
-        hipLaunchKernelGGL(k_limit<false>, grid_for(m.pitch), dim3(256), 0, s, m, ph, win, wout, nodemm, modify, ctrl,
-                           stage, wd_in, wd_out);
+        hipLaunchKernelGGL(k_limit<false>, g, b, 0, s, m, ph, win, wout, nodemm, modify, ctrl, stage, wd_in, wd_out);
 }
 
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s)

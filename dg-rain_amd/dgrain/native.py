@@ -98,6 +98,9 @@ def load():
         "dgr_set_variant": (C.c_int, [vp, C.c_int]),
         "dgr_get_variant": (C.c_int, [vp, ip]),
         "dgr_autotune": (C.c_int, [vp, C.c_int, ip, dp]),
+        "dgr_set_block": (C.c_int, [vp, C.c_int]),
+        "dgr_get_block": (C.c_int, [vp, ip]),
+        "dgr_autotune_block": (C.c_int, [vp, C.c_int, ip, dp]),
         "dgr_set_final_time": (C.c_int, [vp, C.c_double]),
         "dgr_set_graph": (C.c_int, [vp, C.c_int]),
         "dgr_ensemble_create": (C.c_int, [C.c_int, C.POINTER(MeshView), C.POINTER(Params), C.POINTER(vp)]),
@@ -257,6 +260,23 @@ class _Handle:
         ms = np.zeros(NUM_VARIANTS)
         check(self.lib.dgr_autotune(self.h, int(steps), C.byref(v), _dp(ms)))
         return v.value, [float(x) for x in ms]
+
+    def set_block(self, threads):
+        """Workgroup size (64, 128, 256) of the RHS, node-bound and limiter kernels."""
+        check(self.lib.dgr_set_block(self.h, int(threads)), "dgr_set_block")
+
+    def block(self):
+        t = C.c_int(0)
+        check(self.lib.dgr_get_block(self.h, C.byref(t)))
+        return t.value
+
+    def autotune_block(self, steps=5):
+        """Pick the fastest workgroup size for the current variant; returns
+        (chosen, ms/step at 64, 128 and 256 threads)."""
+        t = C.c_int(0)
+        ms = np.zeros(3)
+        check(self.lib.dgr_autotune_block(self.h, int(steps), C.byref(t), _dp(ms)), "dgr_autotune_block")
+        return t.value, [float(x) for x in ms]
 
     def set_final_time(self, t):
         check(self.lib.dgr_set_final_time(self.h, float(t)))

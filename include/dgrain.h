@@ -220,6 +220,15 @@ dgr_status dgr_profile_end(dgr_handle *h, double *total_ms, int *launches);
 #define DGR_VARIANT_PERSIST 3
 dgr_status dgr_set_variant(dgr_handle *h, int variant);
 dgr_status dgr_get_variant(dgr_handle *h, int *variant);
+/* Workgroup size of the RHS/update, node-bound and limiter kernels: 64, 128
+ * or 256 threads (default 256; the edge kernels keep 256, their workgroups
+ * carry the max-|lambda| fold partials).  Same bits at every size.
+ * dgr_autotune_block times `steps` run steps at each size from the current
+ * state (restored afterwards), keeps the fastest and reports it in *chosen and
+ * ms/step per size (64, 128, 256) in ms_per_step[3] (may be NULL). */
+dgr_status dgr_set_block(dgr_handle *h, int threads);
+dgr_status dgr_get_block(dgr_handle *h, int *threads);
+dgr_status dgr_autotune_block(dgr_handle *h, int steps, int *chosen, double *ms_per_step);
 dgr_status dgr_autotune(dgr_handle *h, int steps, int *chosen, double *ms_per_step);
 
 /* Run dgr_run's steps as replays of one captured hipGraph of the step

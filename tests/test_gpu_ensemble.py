@@ -47,6 +47,34 @@ def test_ensemble_members_match_solo(gpu, variant):
     print("  %d members, %d elements, bitwise equal to their solo runs" % (len(wls), sum(w.num_el for w in wls)))
 
 
+@pytest.mark.parametrize("threads", [64, 128])
+def test_ensemble_block_sizes_match_solo(gpu, threads):
+    """Members of an ensemble stepped with 64- or 128-thread workgroups in the
+    RHS, node-bound and limiter kernels (several workgroups per 256-element
+    member range, padding workgroups at a member's end) are bit for bit their
+    solo runs at the default 256, with and without the captured step."""
+    wls = _members()
+    for graph in (False, True):
+        ens = native.Ensemble([w.mesh for w in wls], [w.solver_params() for w in wls])
+        ens.set_variant(2)
+        ens.set_block(threads)
+        ens.set_graph(graph)
+        for i, w in enumerate(wls):
+            ens.set_state(i, w.zeta, w.qx, w.qy)
+        reps = ens.run(30)
+        for i, w in enumerate(wls):
+            s = native.Solver(w.mesh, w.solver_params())
+            s.set_variant(2)
+            s.set_state(w.zeta, w.qx, w.qy)
+            r = s.run(30)
+            for a, b in zip(ens.get_state(i), s.get_state()):
+                assert same(a, b), "member %d state differs at %d threads" % (i, threads)
+            assert np.array_equal(ens.get_wd(i), s.get_wd()), i
+            assert reps[i].t == r.t and reps[i].dt == r.dt, i
+            s.close()
+        ens.close()
+
+
 def test_ensemble_member_hydrographs(gpu):
     """Per-member hydrographs (none on member 0, different ones on members 1
     and 2) under graph replay: each member bit for bit its solo run."""

@@ -483,6 +483,50 @@ def test_stage_variants_bitwise(gpu, variant):
     _compare_run(s, oracle.Oracle.for_workload(wl), 40, 20)
 
 
+@pytest.mark.parametrize("threads", [64, 128])
+def test_block_sizes_bitwise(gpu, threads):
+    """The RHS, node-bound and limiter kernels at 64 or 128 threads per
+    workgroup (the 256-element member ranges then span several workgroups)
+    give the oracle's bits: a wet/dry dry-bed dam break with PDop acting,
+    and a tide+rain run on a mesh whose element count is no multiple of 64."""
+    wl = configs.dry_bed_dam_break(N=24, nsteps=60)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_variant(2)
+    s.set_block(threads)
+    assert s.block() == threads
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    _compare_run(s, oracle.Oracle.for_workload(wl), 60, 30)
+    wl = configs.coastal(N=23, nsteps=30, rain_mm_h=30.0)
+    assert wl.num_el % 64
+    for v in range(native.NUM_VARIANTS):
+        s = native.Solver(wl.mesh, wl.solver_params())
+        s.set_variant(v)
+        s.set_block(threads)
+        s.set_state(wl.zeta, wl.qx, wl.qy)
+        _compare_run(s, oracle.Oracle.for_workload(wl), 30, 30)
+
+
+def test_autotune_block_keeps_state(gpu):
+    """dgr_autotune_block times 64, 128 and 256 threads from the current
+    state and restores it; a graph captured before the change is
+    recaptured, and the run stays bitwise equal to the oracle."""
+    wl = configs.coastal(N=32, nsteps=30)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    s.set_graph(True)
+    o = oracle.Oracle.for_workload(wl)
+    _compare_run(s, o, 10, 10)
+    chosen, ms = s.autotune_block(3)
+    assert chosen in (64, 128, 256) and s.block() == chosen
+    assert len(ms) == 3 and all(m > 0 for m in ms), ms
+    _compare_run(s, o, 10, 10)
+    for t in (64, 128, 256):
+        s.set_block(t)
+        _compare_run(s, o, 2, 2)
+    with pytest.raises(native.DgrError):
+        s.set_block(96)
+
+
 def test_autotune_keeps_state(gpu):
     """dgr_autotune times every variant from the current state and restores
     it: the run that follows is still bitwise equal to the oracle."""
