@@ -408,6 +408,181 @@ __device__ __forceinline__ void element_rhs(const Geo &G, const double W[9], con
     }
 }
 
+// ---------------------------------------------------------------------------
+// The same RHS as geometry_of + element_rhs, each geometry quantity computed
+// with the same expression but next to its first use, in three phases:
+// surface integrals (Gauss points and basis values one edge at a time),
+// volume terms, combine.  Only the vertex data, the RT0 matrix and the
+// running sums stay live across phases, instead of all 63 geometry values:
+// the RHS kernel then fits fewer registers (more waves per SIMD).
+// 2DInnerProducts.c:14-566, compute2DL.c:344-409.
+// ---------------------------------------------------------------------------
+template <class DV>
+__device__ __forceinline__ void element_rhs_phased(const double4 P[3], const double W[9], const double Fh[9],
+                                                   int wd_self, double g, double rain, double infil, double R[9],
+                                                   DV &D)
+{
+    const double xv1 = P[0].x, xv2 = P[1].x, xv3 = P[2].x;                  // 2DInnerProducts.c:22-27
+    const double yv1 = P[0].y, yv2 = P[1].y, yv3 = P[2].y;
+    const double x1 = (xv1 + xv2) / 2, x2 = (xv2 + xv3) / 2, x3 = (xv3 + xv1) / 2;   // :30-35
+    const double y1 = (yv1 + yv2) / 2, y2 = (yv2 + yv3) / 2, y3 = (yv3 + yv1) / 2;
+    const double el0 = sqrt((xv2 - xv1) * (xv2 - xv1) + (yv2 - yv1) * (yv2 - yv1));   // :37-39
+    const double el1 = sqrt((xv3 - xv2) * (xv3 - xv2) + (yv3 - yv2) * (yv3 - yv2));
+    const double el2 = sqrt((xv3 - xv1) * (xv3 - xv1) + (yv3 - yv1) * (yv3 - yv1));
+    const double jac = 4 * (x2 - x3) * (y2 - y1) - 4 * (x2 - x1) * (y2 - y3);  // :110
+    const int sj = isign(jac);
+    double nx1, ny1, nx2, ny2, nx3, ny3;
+    {
+        const typename DV::Den De0 = D.den(el0), De1 = D.den(el1), De2 = D.den(el2);   // :111-116
+        nx1 = D.q(sj * 2 * (y2 - y3), De0); ny1 = D.q(sj * 2 * (x3 - x2), De0);
+        nx2 = D.q(sj * 2 * (y3 - y1), De1); ny2 = D.q(sj * 2 * (x1 - x3), De1);
+        nx3 = D.q(sj * 2 * (y1 - y2), De2); ny3 = D.q(sj * 2 * (x2 - x1), De2);
+    }
+    const double M11 = nx1, M12 = x1 * nx1 + y1 * ny1, M13 = ny1;            // :126-128
+    const double M21 = nx2, M22 = x2 * nx2 + y2 * ny2, M23 = ny2;
+    const double M31 = nx3, M32 = x3 * nx3 + y3 * ny3, M33 = ny3;
+    const typename DV::Den den =
+        D.den(M13 * M22 * M31 - M12 * M23 * M31 - M13 * M21 * M32 + M11 * M23 * M32 + M12 * M21 * M33 - M11 * M22 * M33);
+    const double nxs[3] = {nx1, nx2, nx3}, nys[3] = {ny1, ny2, ny3};
+
+    // surface integrals SI[f*3+b] (:134-239), Gauss points (:51-65) and
+    // basis values (:73-87) of one edge at a time
+    double SI[9];
+    {
+        double a[3], b[3], c[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f)
+            rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den, Fh[f * 3 + 0], Fh[f * 3 + 1], Fh[f * 3 + 2],
+                a[f], b[f], c[f], D);
+        const double sqrt3 = sqrt(3.0);
+        const typename DV::Den D3 = D.den_const(sqrt3);
+        const double dn0 = x1 * y2 - x1 * y3 + x2 * y3 - x2 * y1 + x3 * y1 - x3 * y2;   // :73
+        const typename DV::Den Ddn = D.den(dn0);
+        const double els[3] = {el0, el1, el2};
+#pragma unroll
+        for (int ed = 0; ed < 3; ++ed) {
+            const double ej = els[ed] / 2;
+            double xe[2], ye[2];
+            if (ed == 0) {
+                xe[0] = D.q(sqrt3 * x1 - x2 + x3, D3);  ye[0] = D.q(sqrt3 * y1 - y2 + y3, D3);
+                xe[1] = D.q(sqrt3 * x1 + x2 - x3, D3);  ye[1] = D.q(sqrt3 * y1 + y2 - y3, D3);
+            } else if (ed == 1) {
+                xe[0] = D.q(x1 + sqrt3 * x2 - x3, D3);  ye[0] = D.q(y1 + sqrt3 * y2 - y3, D3);
+                xe[1] = D.q(-x1 + sqrt3 * x2 + x3, D3); ye[1] = D.q(-y1 + sqrt3 * y2 + y3, D3);
+            } else {
+                xe[0] = D.q(-x1 + x2 + sqrt3 * x3, D3); ye[0] = D.q(-y1 + y2 + sqrt3 * y3, D3);
+                xe[1] = D.q(x1 - x2 + sqrt3 * x3, D3);  ye[1] = D.q(y1 - y2 + sqrt3 * y3, D3);
+            }
+            double Fn[3][2], ps[3][2];
+#pragma unroll
+            for (int n = 0; n < 2; ++n) {
+                const double xq = xe[n], yq = ye[n];
+#pragma unroll
+                for (int f = 0; f < 3; ++f) Fn[f][n] = (a[f] + b[f] * xq) * nxs[ed] + (c[f] + b[f] * yq) * nys[ed];
+                ps[0][n] = D.q(-(yq * x2 - yq * x3 - xq * y2 + x3 * y2 + xq * y3 - x2 * y3), Ddn);
+                ps[1][n] = D.q(-(-x1 * yq + x3 * yq - x3 * y1 + xq * y1 - xq * y3 + x1 * y3), Ddn);
+                ps[2][n] = D.q(-(yq * x1 - yq * x2 - xq * y1 + x2 * y1 + xq * y2 - x1 * y2), Ddn);
+            }
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb) {
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const double term = ej * (Fn[f][0] * ps[bb][0] + Fn[f][1] * ps[bb][1]);
+                    // ((term_e0 + term_e1) + term_e2), the reference's left-to-right sum
+                    SI[f * 3 + bb] = (ed == 0) ? term : SI[f * 3 + bb] + term;
+                }
+            }
+        }
+    }
+
+    // volume terms (:247-566)
+    const double weight = 1.0 / 6;
+    const double lg = (wd_self == 0) ? 0.0 : g;
+    const double mz[3] = {0.5 * (P[0].z + P[1].z), 0.5 * (P[1].z + P[2].z), 0.5 * (P[2].z + P[0].z)};
+    const double aj = fabs(jac);
+    double tau[3], VF[3][3];   // VF[field][basis]
+    double bgx, bgy;
+    {
+        double Fx[3][3], Fy[3][3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const double mnk = 0.5 * (P[k].w + P[(k + 1) % 3].w);
+            const double gnn = g * mnk * mnk;
+            const double zr = W[k], qx = W[3 + k], qy = W[6 + k];
+            const double H = zr + mz[k];
+            const typename DV::Den DH = D.den(H);
+            const double u = D.q(qx, DH);
+            const double v = D.q(qy, DH);
+            tau[k] = D.q(gnn * sqrt(u * u + v * v), D.den(dgr_pow43(H)));
+            const double uv = D.q(qx * qy, DH);
+            Fx[0][k] = qx;
+            Fx[1][k] = D.q(qx * qx, DH) + 0.5 * lg * ((H * H) - (mz[k] * mz[k]));
+            Fx[2][k] = uv;
+            Fy[0][k] = qy;
+            Fy[1][k] = uv;
+            Fy[2][k] = D.q(qy * qy, DH) + 0.5 * lg * ((H * H) - (mz[k] * mz[k]));
+        }
+        const double fj = D.q(4.0, D.den(jac));                              // 4 / jac
+        const double gP[3][2] = {{fj * (y2 - y3), fj * (x3 - x2)},           // :303-308
+                                 {fj * (y3 - y1), fj * (x1 - x3)},
+                                 {fj * (y1 - y2), fj * (x2 - x1)}};
+        const double sx = x1 + x2 + x3, sy = y1 + y2 + y3;
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            double va, vb, vc;
+            rt0(M11, M12, M13, M21, M22, M23, M31, M32, M33, den,
+                Fx[f][0] * nx1 + Fy[f][0] * ny1, Fx[f][1] * nx2 + Fy[f][1] * ny2, Fx[f][2] * nx3 + Fy[f][2] * ny3,
+                va, vb, vc, D);
+#pragma unroll
+            for (int bb = 0; bb < 3; ++bb) {
+                const double gs = gP[bb][0] * sx + gP[bb][1] * sy;
+                VF[f][bb] = aj * (0.5 * va * gP[bb][0] + 0.5 * vc * gP[bb][1] + vb * weight * gs);
+            }
+        }
+        const double gz0 = mz[0] * gP[0][0] + mz[1] * gP[1][0] + mz[2] * gP[2][0];   // :343-347
+        const double gz1 = mz[0] * gP[0][1] + mz[1] * gP[1][1] + mz[2] * gP[2][1];
+        bgx = g * aj * gz0 * weight;                                          // :535
+        bgy = g * aj * gz1 * weight;
+    }
+
+    // rhs combine (compute2DL.c:344-354) and Minv (:380-404)
+    const double jv = P[0].x * (P[1].y - P[2].y) + P[1].x * (P[2].y - P[0].y) + P[2].x * (P[0].y - P[1].y);
+    const double Mi = D.q(6.0, D.den(fabs(jv)));                              // compute2DL.c:380-382
+    const double aw = aj * weight;
+    double ze[3], qe[3], re[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        ze[k] = -SI[k] + VF[0][k];
+        // with lg = 0 the bed term is (0*|jac|*dz*weight)*zeta = +-0 (a NaN zeta stays NaN)
+        const double bx = (wd_self == 0) ? 0.0 * W[k] : bgx * W[k];
+        const double fx = aw * tau[k] * W[3 + k];
+        qe[k] = -SI[3 + k] + VF[1][k] + bx - fx;
+        const double by = (wd_self == 0) ? 0.0 * W[k] : bgy * W[k];
+        const double fy = aw * tau[k] * W[6 + k];
+        re[k] = -SI[6 + k] + VF[2][k] + by - fy;
+    }
+    R[0] = (Mi * ze[0] + 0.0 * ze[1]) + 0.0 * ze[2];
+    R[1] = (0.0 * ze[0] + Mi * ze[1]) + 0.0 * ze[2];
+    R[2] = (0.0 * ze[0] + 0.0 * ze[1]) + Mi * ze[2];
+    R[3] = (Mi * qe[0] + 0.0 * qe[1]) + 0.0 * qe[2];
+    R[4] = (0.0 * qe[0] + Mi * qe[1]) + 0.0 * qe[2];
+    R[5] = (0.0 * qe[0] + 0.0 * qe[1]) + Mi * qe[2];
+    R[6] = (Mi * re[0] + 0.0 * re[1]) + 0.0 * re[2];
+    R[7] = (0.0 * re[0] + Mi * re[1]) + 0.0 * re[2];
+    R[8] = (0.0 * re[0] + 0.0 * re[1]) + Mi * re[2];
+    if (rain != 0.0) {
+        R[0] = R[0] + rain;
+        R[1] = R[1] + rain;
+        R[2] = R[2] + rain;
+    }
+    // infiltration leaves elements not flagged dry
+    if (infil != 0.0 && wd_self != 0) {
+        R[0] = R[0] - infil;
+        R[1] = R[1] - infil;
+        R[2] = R[2] - infil;
+    }
+}
+
 // wall reflection of (qx, qy) about the edge normal (compute2DL.c:179-189);
 // dn = 1/(nx*ty - ny*tx) with (tx, ty) = (-ny, nx)
 __device__ __forceinline__ void reflect(double qx, double qy, double nx, double ny, double dn, double &rx, double &ry)
@@ -1003,9 +1178,10 @@ __device__ __forceinline__ void rhs_block(const MeshDev &m, const PhysDev &ph, c
     double R[9];
     {
         DivFast D;
-        GeoRegs G;
-        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
-        element_rhs(G, W, Fh, wd_self, ph.g, M.rain, ctrl->infil, R, D);
+        double4 Pv[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Pv[k] = m.nodes[m.ev[k * P + e]];
+        element_rhs_phased(Pv, W, Fh, wd_self, ph.g, M.rain, ctrl->infil, R, D);
         if (!D.ok()) {
             // an operand outside the fast division's range: plain divisions.
             // The inputs are loaded again through pointers the compiler
