@@ -20,7 +20,11 @@ void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t 
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
                   const double *nodemm, unsigned char *modify, Ctrl *ctrl, int stage, const int *wd_in, int *wd_out,
-                  hipStream_t s);
+                  hipStream_t s, int step_adj = 0);
+// stage-2 node bounds with the step end of every member (k_step_end, next = 1)
+// in the same launch
+void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *means, double *nodemm, Ctrl *c,
+                            int kind, hipStream_t s);
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
                       hipStream_t s);
@@ -165,7 +169,9 @@ struct StepBufs {
     int *nodearg1, *nodearg2;
 };
 
-void enqueue_stages(dgr_handle *h, const StepBufs *b);
+// fuse_end: the step end (k_step_end, next = 1) rides on the stage-2 node
+// bounds launch; the caller then launches no step end
+void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end = false);
 void enqueue_step(dgr_handle *h, const StepBufs *b);
 // the steps of a run: the prologue once (step bookkeeping, start flags into
 // d_wdn), then run steps that chain the start flags through the stage-2

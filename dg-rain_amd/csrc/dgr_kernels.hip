@@ -48,10 +48,12 @@ __device__ __forceinline__ int xcd_block() { return xcd_map((int)blockIdx.x, (in
 // record an error; the smallest key (earliest step, stage, phase, then
 // lowest member-local edge/element id) wins, which is the error the
 // reference's serial loops would have stopped at (compute2DL.c:245-302,
-// wetDry2D.c:142-146,233-237)
-__device__ __forceinline__ void raise_error(Ctrl *c, int code, int where, int phase)
+// wetDry2D.c:142-146,233-237).  step_adj: steps by which Ctrl::step is
+// already ahead of the step being computed (1 in a stage-2 limiter launch
+// that follows a step end folded into the node bounds)
+__device__ __forceinline__ void raise_error(Ctrl *c, int code, int where, int phase, int step_adj = 0)
 {
-    const unsigned long long key = ((unsigned long long)c->step << 34) | ((unsigned long long)phase << 32) |
+    const unsigned long long key = ((unsigned long long)(c->step - step_adj) << 34) | ((unsigned long long)phase << 32) |
                                    ((unsigned long long)(where & 0x1fffffff) << 3) | (unsigned long long)code;
     atomicMin(&c->err_key, key);
 }
@@ -1386,7 +1388,7 @@ template <bool PDOP>
 __device__ __forceinline__ void limit_block(const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
                                             const double *__restrict__ nodemm,
                                             unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
-                                            const int *wd_in, int *wd_out, const ElemLane &L)
+                                            const int *wd_in, int *wd_out, const ElemLane &L, int step_adj)
 {
     if (!L.valid) return;
     const int e = L.e;
@@ -1485,7 +1487,7 @@ __device__ __forceinline__ void limit_block(const MeshDev &m, const PhysDev &ph,
             eval(load(ev2, nodes2, win2, nm2, wd2), X);
         }
     }
-    if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 2 * stage + 1);
+    if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 2 * stage + 1, step_adj);
     if (PDOP && wd_out) wd_out[e] = wdv;
 #pragma unroll
     for (int f = 0; f < 9; ++f) wout[f * P + e] = S[f];
@@ -1496,9 +1498,10 @@ template <bool PDOP>
 __global__ __launch_bounds__(256) void k_limit(MeshDev m, PhysDev ph, const double *win, double *wout,
                                                const double *__restrict__ nodemm,
                                                unsigned char *__restrict__ modify_out, Ctrl *ctrls, int stage,
-                                               const int *wd_in, int *wd_out)
+                                               const int *wd_in, int *wd_out, int step_adj)
 {
-    limit_block<PDOP>(m, ph, win, wout, nodemm, modify_out, ctrls, stage, wd_in, wd_out, elem_lane_thread(m));
+    limit_block<PDOP>(m, ph, win, wout, nodemm, modify_out, ctrls, stage, wd_in, wd_out, elem_lane_thread(m),
+                      step_adj);
 }
 
 // standalone PDop2D
@@ -1760,6 +1763,25 @@ __global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *c
     step_end_member(m, ph, ctrls, kind, next, rec_end, rec_next, ties, (int)blockIdx.x);
 }
 
+// The stage-2 node bounds of a plain run step with the step end folded in
+// (one launch fewer per step): the first nf workgroups (nmem rounded up to a
+// multiple of 8, so that the node workgroups keep their XCD mapping) end
+// one member's step each, as k_step_end with next = 1 does; the others
+// compute node bounds.  Every input of the step end (the stage-2 flux
+// partials) is complete when this launch starts, and of what it writes the
+// following stage-2 limiter reads only the step count (its error keys
+// subtract the step it has advanced).
+__global__ __launch_bounds__(256) void k_node_minmax_end(MeshDev m, PhysDev ph, const double *__restrict__ means,
+                                                         double *__restrict__ nodemm, Ctrl *ctrls, int kind, int nf)
+{
+    const int b = (int)blockIdx.x;
+    if (b < nf) {
+        if (b < m.nmem) step_end_member(m, ph, ctrls, kind, 1, nullptr, nullptr, 0, b);
+        return;
+    }
+    node_minmax_block<false>(m, means, nodemm, nullptr, xcd_map(b - nf, (int)gridDim.x - nf) * 256 + (int)threadIdx.x);
+}
+
 // ---------------------------------------------------------------------------
 // Persistent variant: one cooperative launch runs a whole RK stage of every
 // member: edge fluxes, grid barrier, RHS + RK update, barrier, node bounds,
@@ -1790,7 +1812,7 @@ __global__ __launch_bounds__(256, 2) void k_stage_persist(MeshDev m, PhysDev ph,
     grid.sync();
     for (int v = b; v < nv_el; v += G)
         limit_block<WETDRY>(m, ph, a.u, a.lout, a.nodemm, a.modify, a.ctrls, slot, a.wd_lin, a.wd_lout,
-                            elem_lane_at(m, xcd_map(v, nv_el)));
+                            elem_lane_at(m, xcd_map(v, nv_el)), 0);
     if (MODE == 2) {
         grid.sync();
         for (int mid = b; mid < m.nmem; mid += G) {
@@ -1901,13 +1923,15 @@ void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, i
 
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
                   const double *nodemm, unsigned char *modify, Ctrl *ctrl, int stage, const int *wd_in, int *wd_out,
-                  hipStream_t s)
+                  hipStream_t s, int step_adj)
 {
     const dim3 g((unsigned)(m.pitch / m.wg)), b((unsigned)m.wg);
     if (pdop)
-        hipLaunchKernelGGL(k_limit<true>, g, b, 0, s, m, ph, win, wout, nodemm, modify, ctrl, stage, wd_in, wd_out);
+        hipLaunchKernelGGL(k_limit<true>, g, b, 0, s, m, ph, win, wout, nodemm, modify, ctrl, stage, wd_in, wd_out,
+                           step_adj);
     else
-        hipLaunchKernelGGL(k_limit<false>, g, b, 0, s, m, ph, win, wout, nodemm, modify, ctrl, stage, wd_in, wd_out);
+        hipLaunchKernelGGL(k_limit<false>, g, b, 0, s, m, ph, win, wout, nodemm, modify, ctrl, stage, wd_in, wd_out,
+                           step_adj);
 }
 
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s)
@@ -1948,6 +1972,14 @@ hipError_t launch_stage_persist(int mode, bool wetdry, const MeshDev &m, const P
     PersistArgs aa = a;
     void *args[] = {(void *)&mm, (void *)&pp, (void *)&aa};
     return hipLaunchCooperativeKernel(k, dim3((unsigned)G), dim3(256), args, 0, s);
+}
+
+void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *means, double *nodemm, Ctrl *c,
+                            int kind, hipStream_t s)
+{
+    const int nf = (m.nmem + 7) & ~7;
+    const dim3 g((unsigned)(nf + (m.nn + 255) / 256)), b(256);
+    hipLaunchKernelGGL(k_node_minmax_end, g, b, 0, s, m, ph, means, nodemm, c, kind, nf);
 }
 
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
