@@ -204,3 +204,47 @@ def test_renumbered_edges_wet_run_the_same(gpu, variant):
     assert np.array_equal(rn.elements_to_old(s.get_wd()), o.get_wd())
     assert rep.t == t and rep.dt == dt
     s.close()
+
+
+def _fan_mesh(K):
+    """A centre node shared by K triangles inside a ring of 2K annulus
+    triangles."""
+    from dgrain import mesh as M
+    ang = 2 * np.pi * np.arange(K) / K
+    x = np.concatenate([[0.0], 10 * np.cos(ang), 20 * np.cos(ang + np.pi / K)])
+    y = np.concatenate([[0.0], 10 * np.sin(ang), 20 * np.sin(ang + np.pi / K)])
+    ev = []
+    for i in range(K):
+        a, b = 1 + i, 1 + (i + 1) % K
+        c, d = 1 + K + i, 1 + K + (i + 1) % K
+        ev += [(0, a, b), (a, c, b), (b, c, d)]
+    rng = np.random.default_rng(K)
+    z = 2.0 + rng.uniform(-0.3, 0.3, x.size)
+    n = rng.uniform(0.01, 0.04, x.size)
+    return M.from_elements(x, y, z, n, np.array(ev, np.int32))
+
+
+@pytest.mark.parametrize("K", [7, 12, 24])
+def test_node_valence_paths(gpu, K):
+    """Nodes shared by 7, 12 and 24 elements (the node bounds walk each
+    node's element list in order): 40 wet/dry steps of each, bit for bit the
+    oracle's."""
+    mesh = _fan_mesh(K)
+    rng = np.random.default_rng(K + 1)
+    n3 = 3 * mesh.num_el
+    wl = configs.Workload("fan_%d" % K, mesh, 0.1 * rng.standard_normal(n3), 0.2 * rng.standard_normal(n3),
+                          0.2 * rng.standard_normal(n3), dict(cfl=0.3, wetdry=1), 40)
+    for variant in (0, 2):
+        s = native.Solver(wl.mesh, wl.solver_params())
+        s.set_variant(variant)
+        s.set_state(wl.zeta, wl.qx, wl.qy)
+        o = oracle.Oracle.for_workload(wl)
+        rep = s.run(40)
+        t, dt = o.run(40)
+        for a, b in zip(s.get_state(), o.get_state()):
+            assert same(a, b)
+        assert np.array_equal(s.get_wd(), o.get_wd())
+        for st in (0, 1):
+            assert np.array_equal(s.get_modify(st), o.get_modify(st))
+        assert rep.t == t and rep.dt == dt
+        s.close()
