@@ -257,8 +257,8 @@ def adjoint_bench(a, variant):
     field, the gradient taken at a perturbed n (truth times a lognormal
     factor, sigma 0.2) with checkpoints every 50 steps.  Cost = time of one
     dgr_adjoint_gauges call over the time of the plain forward run of the
-    same steps, twice: with the default tape budget (half the free HBM: the
-    forward tapes every segment that fits and those are reversed without
+    same steps, twice: with the default tape budget (the free HBM less 1/16
+    of the device memory: the forward tapes every segment that fits and those are reversed without
     recomputation) and with budget 0 (every segment recomputed from its
     checkpoint).  Both give the same gradient bit for bit.  Each is timed on
     a second call, as in a gradient loop: the first call also allocates the
@@ -350,6 +350,12 @@ def large_bench(a, variant):
     obs = ens.gauge_run(S, 10, gauges) * (1.0 + 1e-4)
     for i in range(nb):
         ens.set_state(i, sub.zeta, sub.qx, sub.qy)
+    # timed on the second call, as in a gradient loop (the first also
+    # allocates the tape space the handle keeps)
+    t0 = time.perf_counter()
+    ens.adjoint_gauges(S, 10, 50, gauges, obs)
+    ens.sync()
+    t_first = time.perf_counter() - t0
     t0 = time.perf_counter()
     loss, gz, gn = ens.adjoint_gauges(S, 10, 50, gauges, obs)
     ens.sync()
@@ -361,7 +367,8 @@ def large_bench(a, variant):
     ens.close()
     part2 = {"workload": "16 gradients, coastal_flood_512x512 sub-mesh, 64 gauges each", "members": nb,
              "triangles": nb * sub.num_el, "steps": S, "checkpoint_every": 50, "forward_s": t_fwd,
-             "adjoint_s": t_adj, "ratio": t_adj / t_fwd, "loss_per_member": [float(x) for x in loss]}
+             "adjoint_s": t_adj, "ratio": t_adj / t_fwd, "adjoint_first_call_s": t_first,
+             "loss_per_member": [float(x) for x in loss]}
     return {"forward": part1, "batched_adjoint": part2}
 
 

@@ -278,17 +278,24 @@ DGR_HD Rt0 rt0_map(double M11, double M12, double M13, double M21, double M22, d
 // Inputs: W[9] state, wd_self; outputs adjoint Rb[9].
 // Accumulates Wb[9], Fhb[9], mzb[3], gnnb[3], bgxb, bgyb.
 // ---------------------------------------------------------------------------
+// The reverse in two parts that read different geometry: the surface
+// integrals (-> Fhb) and the volume, friction and bed terms (-> Wb, mzb,
+// gnnb, bgxb, bgyb).  A kernel may evaluate the geometry separately for
+// each, so that only one part's geometry is live at a time.
 template <typename Geo>
-DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double g, const double Rb[9], double Wb[9],
-                            double Fhb[9], double mzb[3], double gnnb[3], double &bgxb, double &bgyb)
+DGR_HD Rt0 rt0_map_of(const Geo &G)
+{
+    return rt0_map(G(GX_NX), G(GX_M2), G(GX_NY), G(GX_NX + 1), G(GX_M2 + 1), G(GX_NY + 1), G(GX_NX + 2),
+                   G(GX_M2 + 2), G(GX_NY + 2), G(GX_DEN));
+}
+
+template <typename Geo>
+DGR_HD void element_rhs_adj_surface(const Geo &G, const double Rb[9], double Fhb[9])
 {
     const double nxs[3] = {G(GX_NX), G(GX_NX + 1), G(GX_NX + 2)};
     const double nys[3] = {G(GX_NY), G(GX_NY + 1), G(GX_NY + 2)};
-    const Rt0 T = rt0_map(nxs[0], G(GX_M2), nys[0], nxs[1], G(GX_M2 + 1), nys[1], nxs[2], G(GX_M2 + 2), nys[2],
-                          G(GX_DEN));
-    const double Mi = G(GX_MI), aw = G(GX_AW), aj = G(GX_AJ), bgx = G(GX_BGX), bgy = G(GX_BGY);
-    const double weight = 1.0 / 6;
-    const double lg = (wd_self == 0) ? 0.0 : g;
+    const Rt0 T = rt0_map_of(G);
+    const double Mi = G(GX_MI);
     // R = Mi (..) : adjoints of ze, qe, re
     double eb[9];
     for (int i = 0; i < 9; ++i) eb[i] = Mi * Rb[i];
@@ -311,6 +318,20 @@ DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double
     }
     for (int f = 0; f < 3; ++f)
         for (int k = 0; k < 3; ++k) Fhb[f * 3 + k] += T.A[k] * ab[f] + T.B[k] * bb_[f] + T.C[k] * cb_[f];
+}
+
+template <typename Geo>
+DGR_HD void element_rhs_adj_volume(const Geo &G, const double W[9], int wd_self, double g, const double Rb[9],
+                                   double Wb[9], double mzb[3], double gnnb[3], double &bgxb, double &bgyb)
+{
+    const double nxs[3] = {G(GX_NX), G(GX_NX + 1), G(GX_NX + 2)};
+    const double nys[3] = {G(GX_NY), G(GX_NY + 1), G(GX_NY + 2)};
+    const Rt0 T = rt0_map_of(G);
+    const double Mi = G(GX_MI), aw = G(GX_AW), aj = G(GX_AJ), bgx = G(GX_BGX), bgy = G(GX_BGY);
+    const double weight = 1.0 / 6;
+    const double lg = (wd_self == 0) ? 0.0 : g;
+    double eb[9];
+    for (int i = 0; i < 9; ++i) eb[i] = Mi * Rb[i];
     // volume term VF[f][b] = aj (0.5 va GPx_b + 0.5 vc GPy_b + vb weight GS_b), (va,vb,vc) = rt0(Fnp)
     double Fnpb[3][3];   // [field][k]
     for (int f = 0; f < 3; ++f) {
@@ -382,6 +403,14 @@ DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double
         Wb[3 + k] += qxb;
         Wb[6 + k] += qyb;
     }
+}
+
+template <typename Geo>
+DGR_HD void element_rhs_adj(const Geo &G, const double W[9], int wd_self, double g, const double Rb[9], double Wb[9],
+                            double Fhb[9], double mzb[3], double gnnb[3], double &bgxb, double &bgyb)
+{
+    element_rhs_adj_surface(G, Rb, Fhb);
+    element_rhs_adj_volume(G, W, wd_self, g, Rb, Wb, mzb, gnnb, bgxb, bgyb);
 }
 
 // forward decisions of one field's limiter pass (SlopeLimiter2D.c:166-303):

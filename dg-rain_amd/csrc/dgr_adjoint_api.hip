@@ -236,6 +236,25 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&d_obs, per * (nsamp > 0 ? nsamp : 1)));
     DGR_HIP(B.alloc(&d_hv, per * (nsamp > 0 ? nsamp : 1)));
     DGR_HIP(B.alloc(&actl, nm));
+    // reverse buffers: slot/zbe pairs per stage (the edge part of a stage's
+    // input adjoint is consumed by the limiter reverse that comes next)
+    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
+    DGR_HIP(B.alloc(&wb, W9));
+    DGR_HIP(B.alloc(&wb0, W9));
+    DGR_HIP(B.alloc(&pb, W9));
+    DGR_HIP(B.alloc(&ub, W9));
+    DGR_HIP(B.alloc(&fhb, W9));
+    DGR_HIP(B.alloc(&slot1, W9));
+    DGR_HIP(B.alloc(&slot2, W9));
+    DGR_HIP(B.alloc(&zbe1, 3 * P));
+    DGR_HIP(B.alloc(&zbe2, 3 * P));
+    DGR_HIP(B.alloc(&vmb, 18 * P));
+    DGR_HIP(B.alloc(&zvb, 3 * P));
+    DGR_HIP(B.alloc(&nvb, 3 * P));
+    DGR_HIP(B.alloc(&dtp, 2 * (size_t)maxparts));
+    DGR_HIP(B.alloc(&tdp, 2 * (size_t)maxparts));
+    DGR_HIP(B.alloc(&gz, nn));
+    DGR_HIP(B.alloc(&gn, nn));
     // Tapes: each step's start state, both stage outputs before and after
     // the limiter, both stages' flags, node bounds and the elements attaining
     // them.  The last R segments are taped by the forward pass itself and
@@ -250,9 +269,13 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     const size_t seg_bytes = 4 * b_w + 2 * b_i + b_nm + b_na;
     long long budget = h->tape_budget;
     if (budget < 0) {
-        // half the free memory, counting the tape space the handle holds
+        // the free memory (every other buffer of the call is allocated by
+        // now), counting the tape space the handle holds, less a reserve of
+        // 1/16 of the device memory
         size_t fr = 0, tot = 0;
-        budget = hipMemGetInfo(&fr, &tot) == hipSuccess ? (long long)((fr + h->tape_bytes) / 2) : 0;
+        budget = hipMemGetInfo(&fr, &tot) == hipSuccess
+                     ? std::max<long long>(0, (long long)(fr + h->tape_bytes) - (long long)(tot / 16))
+                     : 0;
     }
     int R = (int)std::min<long long>(nseg, budget / (long long)seg_bytes);
     if (h->tape_bytes < seg_bytes * (size_t)(R > 1 ? R : 1)) {
@@ -318,25 +341,6 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         bf.nodearg2 = T.na + 6 * nn * (2 * k + 1);
         return bf;
     };
-    // reverse buffers: slot/zbe pairs per stage (the edge part of a stage's
-    // input adjoint is consumed by the limiter reverse that comes next)
-    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
-    DGR_HIP(B.alloc(&wb, W9));
-    DGR_HIP(B.alloc(&wb0, W9));
-    DGR_HIP(B.alloc(&pb, W9));
-    DGR_HIP(B.alloc(&ub, W9));
-    DGR_HIP(B.alloc(&fhb, W9));
-    DGR_HIP(B.alloc(&slot1, W9));
-    DGR_HIP(B.alloc(&slot2, W9));
-    DGR_HIP(B.alloc(&zbe1, 3 * P));
-    DGR_HIP(B.alloc(&zbe2, 3 * P));
-    DGR_HIP(B.alloc(&vmb, 18 * P));
-    DGR_HIP(B.alloc(&zvb, 3 * P));
-    DGR_HIP(B.alloc(&nvb, 3 * P));
-    DGR_HIP(B.alloc(&dtp, 2 * (size_t)maxparts));
-    DGR_HIP(B.alloc(&tdp, 2 * (size_t)maxparts));
-    DGR_HIP(B.alloc(&gz, nn));
-    DGR_HIP(B.alloc(&gn, nn));
     DGR_HIP(hipMemcpyAsync(d_gel, sp->gauge_el, per * sizeof(int), hipMemcpyHostToDevice, s));
     {
         // obs arrive [member][sample][gauge]; device keeps [sample][member][gauge]

@@ -313,14 +313,30 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
                 dtb += 0.5 * R[f] * ub[f];
             }
         }
-        // geometry enters derivatives only: reciprocal products
+        // geometry enters derivatives only: reciprocal products.  Evaluated
+        // once for the surface part and again, from vertex data loaded anew
+        // (the compiler cannot reuse the first evaluation), for the volume
+        // part: only one part's geometry is live at a time.
+        {
+            GeoRegs G;
+            DivRecip D;
+            geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
+            double Fb[9];
+#pragma unroll
+            for (int f = 0; f < 9; ++f) Fb[f] = 0.0;
+            element_rhs_adj_surface(G, Rb, Fb);
+#pragma unroll
+            for (int f = 0; f < 9; ++f) Fhb[f * P + e] = Fb[f];
+        }
+        MeshDev m2 = m;
+        asm volatile("" : "+s"(m2.nodes), "+s"(m2.ev), "+s"(m2.info));
         GeoRegs G;
         DivRecip D;
-        geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
-        double Wb[9], Fb[9], mzb[3] = {0, 0, 0}, gnnb[3] = {0, 0, 0}, bgxb = 0.0, bgyb = 0.0;
+        geometry_of(m2, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
+        double Wb[9], mzb[3] = {0, 0, 0}, gnnb[3] = {0, 0, 0}, bgxb = 0.0, bgyb = 0.0;
 #pragma unroll
-        for (int f = 0; f < 9; ++f) { Wb[f] = 0.0; Fb[f] = 0.0; }
-        element_rhs_adj(G, w, wd[e], ph.g, Rb, Wb, Fb, mzb, gnnb, bgxb, bgyb);
+        for (int f = 0; f < 9; ++f) Wb[f] = 0.0;
+        element_rhs_adj_volume(G, w, wd[e], ph.g, Rb, Wb, mzb, gnnb, bgxb, bgyb);
         if (MODE == 1) {
 #pragma unroll
             for (int f = 0; f < 9; ++f) Bb[f * P + e] += ub[f] + Wb[f];
@@ -331,8 +347,6 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
                 Bb[f * P + e] = 0.5 * ub[f];   // first write of the step-start adjoint
             }
         }
-#pragma unroll
-        for (int f = 0; f < 9; ++f) Fhb[f * P + e] = Fb[f];
         // geometry -> nodal z and n.  bgx = g |jac| weight sum_k mz_k gP_k,x
         const double cbed = ph.g * G(GX_AJ) * (1.0 / 6);
 #pragma unroll

@@ -119,7 +119,7 @@ struct dgr_handle {
     int graph_wg;    // MeshDev::wg the graph was captured with
     hipGraphExec_t graph_exec;
     // HBM the adjoint may keep taped segments in (dgr_set_tape_budget):
-    // < 0 half the free device memory, 0 recompute every segment
+    // < 0 the free device memory less 1/16 of it, 0 recompute every segment
     long long tape_budget = -1;
     // the adjoint's tape space, kept for the next call (a gradient loop
     // calls with the same sizes)

@@ -311,8 +311,9 @@ dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *spec, doubl
 /* Device memory dgr_adjoint_gauges may spend on resident tapes (extension).
  * The last R segments (as many as fit) are taped by the forward pass itself
  * and reversed without recomputation; the others are recomputed from their
- * checkpoints.  bytes < 0 (the default): half the free device memory at the
- * call; 0: recompute every segment.  The gradient is the same bit for bit.
+ * checkpoints.  bytes < 0 (the default): the device memory still free when
+ * the call has allocated its other buffers, less 1/16 of the device's
+ * memory; 0: recompute every segment.  The gradient is the same bit for bit.
  * The tape space stays allocated on the handle for the next call (a gradient
  * loop repeats the same sizes) until this call or dgr_destroy releases it. */
 dgr_status dgr_set_tape_budget(dgr_handle *h, long long bytes);
