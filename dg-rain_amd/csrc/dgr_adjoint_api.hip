@@ -255,6 +255,12 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&tdp, 2 * (size_t)maxparts));
     DGR_HIP(B.alloc(&gz, nn));
     DGR_HIP(B.alloc(&gn, nn));
+    // one stage's node bounds and the elements attaining them, when the
+    // tapes leave them out (compact tapes, below)
+    double *cnm;
+    int *cna;
+    DGR_HIP(B.alloc(&cnm, 6 * nn));
+    DGR_HIP(B.alloc(&cna, 6 * nn));
     // Tapes: each step's start state, both stage outputs before and after
     // the limiter, both stages' flags, node bounds and the elements attaining
     // them.  The last R segments are taped by the forward pass itself and
@@ -266,7 +272,13 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_w = up(Wb * K), b_i = up(Ib * K), b_nm = up(12 * nn * K * sizeof(double)),
                  b_na = up(12 * nn * K * sizeof(int));
-    const size_t seg_bytes = 4 * b_w + 2 * b_i + b_nm + b_na;
+    const size_t seg_full = 4 * b_w + 2 * b_i + b_nm + b_na;
+    // compact tapes: states and flags only; the reverse recomputes each
+    // stage's node bounds from the taped pre-limiter state (means, then the
+    // node kernel: the forward's kernels on the forward's inputs, so the
+    // same bits).  Chosen when they keep more segments resident than full
+    // tapes do.
+    const size_t seg_compact = 4 * b_w + 2 * b_i;
     long long budget = h->tape_budget;
     if (budget < 0) {
         // the free memory (every other buffer of the call is allocated by
@@ -277,7 +289,11 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
                      ? std::max<long long>(0, (long long)(fr + h->tape_bytes) - (long long)(tot / 16))
                      : 0;
     }
-    int R = (int)std::min<long long>(nseg, budget / (long long)seg_bytes);
+    const int R_full = (int)std::min<long long>(nseg, budget / (long long)seg_full);
+    const int R_compact = (int)std::min<long long>(nseg, budget / (long long)seg_compact);
+    const bool compact = R_compact > R_full;
+    const size_t seg_bytes = compact ? seg_compact : seg_full;
+    int R = compact ? R_compact : R_full;
     if (h->tape_bytes < seg_bytes * (size_t)(R > 1 ? R : 1)) {
         // grow the handle's tape space (kept for the next call)
         if (h->d_tape) {
@@ -312,8 +328,8 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         t.u2 = (double *)b; b += b_w;
         t.wa = (int *)b; b += b_i;
         t.wb = (int *)b; b += b_i;
-        t.nm = (double *)b; b += b_nm;
-        t.na = (int *)b;
+        t.nm = compact ? nullptr : (double *)b; b += compact ? 0 : b_nm;
+        t.na = compact ? nullptr : (int *)b;
         return t;
     };
     const int first_res = nseg - R;   // segments >= first_res stay taped from the forward
@@ -335,10 +351,13 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         } else {
             bf.wdprev = bf.wda = bf.wdb = h->d_wd;   // never written without wet/dry
         }
-        bf.nodemm1 = T.nm + 6 * nn * (2 * k);
-        bf.nodemm2 = T.nm + 6 * nn * (2 * k + 1);
-        bf.nodearg1 = T.na + 6 * nn * (2 * k);
-        bf.nodearg2 = T.na + 6 * nn * (2 * k + 1);
+        if (!compact) {
+            bf.nodemm1 = T.nm + 6 * nn * (2 * k);
+            bf.nodemm2 = T.nm + 6 * nn * (2 * k + 1);
+            bf.nodearg1 = T.na + 6 * nn * (2 * k);
+            bf.nodearg2 = T.na + 6 * nn * (2 * k + 1);
+        }
+        bf.taped = true;
         return bf;
     };
     DGR_HIP(hipMemcpyAsync(d_gel, sp->gauge_el, per * sizeof(int), hipMemcpyHostToDevice, s));
@@ -441,10 +460,18 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
                 const int j = (n + 1) / sp->sample_every - 1;
                 launch_adj_gauges(h->m, ng, d_gel, d_hv + (size_t)j * per, d_obs + (size_t)j * per, wb, zvb, s);
             }
-            const double *nm1 = T.nm + 6 * nn * (2 * k), *nm2 = T.nm + 6 * nn * (2 * k + 1);
-            const int *na1 = T.na + 6 * nn * (2 * k), *na2 = T.na + 6 * nn * (2 * k + 1);
+            const double *nm1 = compact ? cnm : T.nm + 6 * nn * (2 * k);
+            const double *nm2 = compact ? cnm : T.nm + 6 * nn * (2 * k + 1);
+            const int *na1 = compact ? cna : T.na + 6 * nn * (2 * k);
+            const int *na2 = compact ? cna : T.na + 6 * nn * (2 * k + 1);
+            // a stage's node bounds from its taped pre-limiter state
+            auto node_bounds = [&](const double *u) {
+                launch_means(h->m, u, h->d_means, s);
+                launch_node_minmax(h->m, h->d_means, cnm, cna, s);
+            };
             // stage 2: input p1, base w0, output u2 -> limited -> w^{n+1} (adjoint wb + slot1)
             // (the scalar pass of step n+1 rides on this launch)
+            if (compact) node_bounds(u2);
             launch_adj_limit(h->m, h->ph, u2, nm2, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
                              zvb, ub, vmb, actl, &pending, s);
             pending.active = 0;
@@ -462,6 +489,8 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slot2, zbe2, tdp + maxparts,
                             lam_mode, s);
             // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)
+            // (the stage-2 kernels that read cna are ahead on the stream)
+            if (compact) node_bounds(u1);
             launch_adj_limit(h->m, h->ph, u1, nm1, pb, slot2, zbe2, zvb, ub, vmb, actl, nullptr, s);
             launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, vmb, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);

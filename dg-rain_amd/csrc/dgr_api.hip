@@ -769,7 +769,7 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end)
     const bool wdon = h->ph.wetdry != 0;
     hipStream_t s = h->stream;
     const StepBufs plain = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wd, h->d_wd, nullptr, false,
-                            nullptr, nullptr, nullptr, nullptr};
+                            nullptr, nullptr, nullptr, nullptr, false};
     if (!b) b = &plain;
     if (wdon && !b->wda_ready) launch_wd_status(h->m, h->ph, b->w0, b->wdprev, b->wda, s);
     mark(h);
@@ -784,7 +784,7 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end)
     mark(h);
     // a taped step's edge kernel also counts the stage-2 max-|lambda| ties
     // for the reverse (k_step_end records them)
-    const bool ties = b->nodemm2 != nullptr;
+    const bool ties = b->taped;
     h->kind[1] = launch_stage(h->variant, 2, wdon, h->m, h->ph, b->p1, b->u2, b->w0, h->d_bzeta, h->d_bqn, wd2,
                               h->d_means, h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s, ties);
     h->ties_last = ties && h->kind[1] == PARTS_EDGE;
@@ -868,7 +868,7 @@ void enqueue_run_step(dgr_handle *h, const StepBufs *b, StepRec *rec_end, StepRe
         return;
     }
     const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true,
-                          nullptr, nullptr, nullptr, nullptr};
+                          nullptr, nullptr, nullptr, nullptr, false};
     // a plain step (no tape, no records) ends inside its stage-2 node bounds launch
     const bool plain = !b && !rec_end && !rec_next;
     enqueue_stages(h, b ? b : &run, plain);

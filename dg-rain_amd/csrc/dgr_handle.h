@@ -167,6 +167,9 @@ struct StepBufs {
     // them) go here instead of the handle's scratch buffer
     double *nodemm1, *nodemm2;
     int *nodearg1, *nodearg2;
+    // a taped step: the stage-2 edge kernel also counts the max-|lambda|
+    // ties the reverse needs
+    bool taped;
 };
 
 // fuse_end: the step end (k_step_end, next = 1) rides on the stage-2 node

@@ -70,20 +70,24 @@ def test_adjoint_loss_matches_forward(gpu):
     assert np.all(np.isfinite(gz)) and np.all(np.isfinite(gn)) and np.abs(gn).max() > 0
 
 
-def _segment_bytes(s, nn, K):
+def _segment_bytes(s, nn, K, compact=False):
     """Device bytes of one segment's tape (dgr_adjoint_api.hip): four state
-    copies, two flag sets, node bounds and their elements, per step."""
+    copies and two flag sets per step, plus node bounds and their elements
+    unless the tape is compact (the reverse then recomputes them)."""
     _, P = s.device_state()
     up = lambda b: (b + 255) // 256 * 256
-    return 4 * up(9 * P * 8 * K) + 2 * up(P * 4 * K) + up(12 * nn * K * 8) + up(12 * nn * K * 4)
+    b = 4 * up(9 * P * 8 * K) + 2 * up(P * 4 * K)
+    return b if compact else b + up(12 * nn * K * 8) + up(12 * nn * K * 4)
 
 
 @pytest.mark.parametrize("wetdry", [0, 1])
 def test_resident_tapes_give_the_same_gradient(gpu, wetdry):
     """Segments taped by the forward pass and kept resident, segments
-    recomputed from checkpoints, and a mix of both (the last two of five
-    segments resident, the last one partial) give the same loss and
-    gradients bit for bit."""
+    recomputed from checkpoints, and mixes of both (of five segments, the
+    last one partial: the last two resident with full tapes; the last three
+    with compact tapes, which a budget picks when they keep more segments
+    resident, the reverse then recomputing the node bounds) give the same
+    loss and gradients bit for bit."""
     from helpers import perturbed_mesh, random_state
     nsteps, every, K = 30, 5, 7
     if wetdry:
@@ -101,8 +105,10 @@ def test_resident_tapes_give_the_same_gradient(gpu, wetdry):
         s.close()
     s = native.Solver(mesh, prm)
     seg = _segment_bytes(s, mesh.num_nodes, K)
+    cseg = _segment_bytes(s, mesh.num_nodes, K, compact=True)
+    assert 2 * seg < 3 * cseg <= 2 * seg + seg // 2 < 3 * seg   # full: 2 resident; compact: 3
     out = []
-    for budget in (0, 2 * seg + seg // 2, -1, 1 << 40):
+    for budget in (0, 2 * seg, 2 * seg + seg // 2, -1, 1 << 40):
         s.set_tape_budget(budget)
         s.set_state(*state)
         out.append(s.adjoint_gauges(nsteps, every, K, gauges, obs))
