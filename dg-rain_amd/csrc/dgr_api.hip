@@ -587,6 +587,58 @@ extern "C" dgr_status dgr_couple_channel_ends(dgr_handle *h, int nends, const in
     return DGR_OK;
 }
 
+extern "C" dgr_status dgr_couple_dgshed(dgr_handle *h, int n, const int *edge_member, const int *edge_local,
+                                       const int *conn, const double *chan, const double *geom, double *swe,
+                                       double *ghost)
+{
+    if (!h || n < 0) return DGR_ERR_ARG;
+    if (n == 0) return DGR_OK;
+    if (!edge_member || !edge_local || !conn || !chan || !geom || !ghost) return DGR_ERR_ARG;
+    for (int i = 0; i < n; ++i)
+        if (bad_member(h, edge_member[i]) || edge_local[i] < 0 || edge_local[i] >= h->mem[edge_member[i]].ned ||
+            (conn[i] != 0 && conn[i] != 1))
+            return DGR_ERR_ARG;
+    // spec: members, local edges, conn, flags; vals: chan, geom, swe, ghost
+    const size_t ns = 4 * (size_t)n, nv = 14 * (size_t)n;
+    if (h->cpl_spec_n < ns) {
+        if (h->d_cpl_spec) DGR_HIP(hipFree(h->d_cpl_spec));
+        h->d_cpl_spec = nullptr;
+        h->cpl_spec_n = 0;
+        DGR_HIP(dalloc(&h->d_cpl_spec, ns));
+        h->cpl_spec_n = ns;
+    }
+    if (h->cpl_vals_n < nv) {
+        if (h->d_cpl_vals) DGR_HIP(hipFree(h->d_cpl_vals));
+        h->d_cpl_vals = nullptr;
+        h->cpl_vals_n = 0;
+        DGR_HIP(dalloc(&h->d_cpl_vals, nv));
+        h->cpl_vals_n = nv;
+    }
+    std::vector<int> spec(ns, 0);
+    std::copy(edge_member, edge_member + n, spec.begin());
+    std::copy(edge_local, edge_local + n, spec.begin() + n);
+    std::copy(conn, conn + n, spec.begin() + 2 * n);
+    std::vector<double> vals(10 * (size_t)n);
+    std::copy(chan, chan + 6 * (size_t)n, vals.begin());
+    std::copy(geom, geom + 4 * (size_t)n, vals.begin() + 6 * (size_t)n);
+    hipStream_t s = h->stream;
+    DGR_HIP(hipMemcpyAsync(h->d_cpl_spec, spec.data(), ns * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_HIP(hipMemcpyAsync(h->d_cpl_vals, vals.data(), vals.size() * sizeof(double), hipMemcpyHostToDevice, s));
+    launch_couple_dgshed(h->m, h->ph, h->d_w, h->d_bzeta, h->d_bqn, n, h->d_cpl_spec, h->d_cpl_vals, s);
+    DGR_HIP(hipGetLastError());
+    std::vector<double> out(4 * (size_t)n);
+    DGR_HIP(hipMemcpyAsync(out.data(), h->d_cpl_vals + 10 * (size_t)n, out.size() * sizeof(double),
+                           hipMemcpyDeviceToHost, s));
+    DGR_HIP(hipMemcpyAsync(spec.data() + 3 * n, h->d_cpl_spec + 3 * (size_t)n, n * sizeof(int), hipMemcpyDeviceToHost,
+                           s));
+    DGR_HIP(hipStreamSynchronize(s));
+    for (int i = 0; i < n; ++i)
+        if (spec[3 * (size_t)n + i]) return DGR_ERR_ARG;   // not a boundary edge (left unchanged)
+    if (swe) std::copy(out.begin(), out.begin() + 2 * (size_t)n, swe);
+    std::copy(out.begin() + 2 * (size_t)n, out.end(), ghost);
+    return DGR_OK;
+}
+
 dgr_status read_ctrl(dgr_handle *h, int mi, Ctrl *c)
 {
     DGR_HIP(hipMemcpyAsync(c, h->d_ctrl + mi, sizeof(Ctrl), hipMemcpyDeviceToHost, h->stream));

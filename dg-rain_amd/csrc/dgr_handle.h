@@ -29,6 +29,8 @@ void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hip
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
                       hipStream_t s);
 void launch_step_begin(const MeshDev &m, Ctrl *c, hipStream_t s);
+void launch_couple_dgshed(const MeshDev &m, const PhysDev &ph, const double *w, double *bzeta, double *bqn, int n,
+                          int *spec, double *vals, hipStream_t s);
 void launch_couple_ends(const MeshDev &m, const double *w, double *bzeta, double *bqn, int nends, const int *spec,
                         const double *vals, double *ghost, hipStream_t s);
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,

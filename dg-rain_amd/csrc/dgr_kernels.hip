@@ -2058,6 +2058,96 @@ __global__ __launch_bounds__(64) void k_couple_ends(MeshDev m, const double *__r
     ghost[nends + e] = dir > 0 ? qn : -qn;
 }
 
+// DG-SWEM's coupling edge with DGSHED's 1-D channels, 2-D side: the
+// channel end's values become the edge's boundary data
+// (obtain_channels_values, DGSHED/couple_domains.c:248-295, with getH of
+// DGSHED/constitutive_equations.c:6-24, as CHANNEL_EDGE_HYDRO reads them,
+// src/channel_edge_hydro.F:72-75), and el_in's trace averaged over the
+// edge's two Gauss points goes to the channel end's ghost values
+// (POPULATE_SWE_ARRAY, src/couple2dgshed.F:22-81; receive_swe_values,
+// DGSHED/couple_domains.c:297-335).  The trace uses this element's CR basis
+// at the Gauss points as Compute2DInnerProducts evaluates it.  One thread per
+// coupled edge, the oracle's orc_couple_dgshed operation for operation.
+// spec: members[n], member-local edges[n], conn[n], then flag[n] (written:
+// 1 if the edge is no boundary edge, which is left unchanged); vals: chan[6n]
+// (A, Q, b, m1, m2, z), geom[4n] (b, m1, m2, z), then swe[2n], ghost[2n]
+// (written).
+__device__ __forceinline__ double channel_depth(double A, double b, double m1, double m2)
+{
+    double H;
+    if (m1 > 0 || m2 > 0)
+        H = (-b + sqrt(b * b + 2 * A * m1 + 2 * A * m2)) / (m1 + m2);
+    else
+        H = A / b;
+    return H;
+}
+
+__global__ __launch_bounds__(64) void k_couple_dgshed(MeshDev m, PhysDev ph, const double *__restrict__ w,
+                                                      double *bzeta, double *bqn, int n, int *spec,
+                                                      double *vals)
+{
+    const int i = blockIdx.x * 64 + threadIdx.x;
+    if (i >= n) return;
+    const int *emem = spec, *eloc = spec + n, *conn = spec + 2 * n;
+    int *flag = spec + 3 * n;
+    const double *chan = vals + 6 * i, *gm = vals + 6 * n + 4 * i;
+    double *swe = vals + 10 * n, *ghost = vals + 12 * n;
+    const MemberDev &M = m.mem[emem[i]];
+    const int ed = M.ed_off + eloc[i];
+    const int inf = m.einfo[ed];
+    if (!(inf & EI_BOUNDARY)) {
+        flag[i] = 1;
+        return;
+    }
+    flag[i] = 0;
+    const int4 E = m.edges[ed];
+    const int el = E.x, kin = inf & ED_KIN_MASK;
+    const double4 A = m.nodes[E.z], B = m.nodes[E.w];
+    const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));   // compute2DL.c:98-132
+    const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
+    const double nx = sg * (B.y - A.y) / len;
+    const double ny = sg * (A.x - B.x) / len;
+    // channel -> 2-D
+    const double zeta_ch = channel_depth(chan[0], chan[2], chan[3], chan[4]) - chan[5];
+    const double qn_ch = conn[i] == 0 ? chan[1] : -chan[1];
+    // 2-D -> channel: the trace at the Gauss points of edge kin
+    GeoRegs G;
+    DivExact D;
+    geometry_of(m, el, ph.g, [&](int j, double v) { G.v[j] = v; }, D);
+    const int P = m.pitch;
+    double ze_avg = 0.0, q_n_sum = 0.0;
+#pragma unroll
+    for (int q = 0; q < 2; ++q) {
+        const double P1 = G(GX_PS + 2 * kin + q), P2 = G(GX_PS + 6 + 2 * kin + q), P3 = G(GX_PS + 12 + 2 * kin + q);
+        double tr[3];
+#pragma unroll
+        for (int f = 0; f < 3; ++f) {
+            double t = w[(3 * f) * P + el] * P1;
+            t = t + w[(3 * f + 1) * P + el] * P2;
+            t = t + w[(3 * f + 2) * P + el] * P3;
+            tr[f] = t;
+        }
+        const double q_n_int = tr[1] * nx + tr[2] * ny;
+        ze_avg = ze_avg + tr[0] * 1.0;
+        q_n_sum = q_n_sum + q_n_int * 1.0;
+    }
+    const double sz = 0.5 * ze_avg, sq = 0.5 * len * q_n_sum;
+    swe[2 * i] = sz;
+    swe[2 * i + 1] = sq;
+    const double htval = sz + gm[3];
+    ghost[2 * i] = htval * gm[0] + 0.5 * (gm[1] + gm[2]) * htval * htval;
+    ghost[2 * i + 1] = conn[i] == 0 ? sq : -sq;
+    bzeta[ed] = zeta_ch;
+    bqn[ed] = qn_ch / len;
+}
+
+void launch_couple_dgshed(const MeshDev &m, const PhysDev &ph, const double *w, double *bzeta, double *bqn, int n,
+                          int *spec, double *vals, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_couple_dgshed, dim3((unsigned)((n + 63) / 64)), dim3(64), 0, s, m, ph, w, bzeta, bqn, n,
+                       spec, vals);
+}
+
 void launch_couple_ends(const MeshDev &m, const double *w, double *bzeta, double *bqn, int nends, const int *spec,
                         const double *vals, double *ghost, hipStream_t s)
 {

@@ -102,6 +102,7 @@ def load():
         "dgr_get_block": (C.c_int, [vp, ip]),
         "dgr_autotune_block": (C.c_int, [vp, C.c_int, ip, dp]),
         "dgr_set_final_time": (C.c_int, [vp, C.c_double]),
+        "dgr_couple_dgshed": (C.c_int, [vp, C.c_int, ip, ip, ip, dp, dp, dp, dp]),
         "dgr_set_graph": (C.c_int, [vp, C.c_int]),
         "dgr_ensemble_create": (C.c_int, [C.c_int, C.POINTER(MeshView), C.POINTER(Params), C.POINTER(vp)]),
         "dgr_num_members": (C.c_int, [vp, ip]),
@@ -370,13 +371,30 @@ class _Handle:
                                                _dp(gq)), "dgr_couple_channel_ends")
         return ga, gq
 
+    def couple_dgshed(self, edge_member, edge_local, conn, chan, geom):
+        """DG-SWEM's channel coupling edge, 2-D side: sets the coupled boundary
+        edges' bzeta/bQn from the channel ends (chan: [n][6] = A, Q, b, m1, m2,
+        z) and returns (swe [n][2] = edge-averaged zeta and len x edge-averaged
+        q.n, ghost [n][2] = the ends' ghost area and discharge, from geom:
+        [n][4] = b, m1, m2, z)."""
+        mem = np.ascontiguousarray(edge_member, np.int32)
+        loc = np.ascontiguousarray(edge_local, np.int32)
+        cn = np.ascontiguousarray(conn, np.int32)
+        ch = np.ascontiguousarray(chan, np.float64).reshape(-1, 6)
+        gm = np.ascontiguousarray(geom, np.float64).reshape(-1, 4)
+        n = mem.size
+        swe, ghost = np.zeros((n, 2)), np.zeros((n, 2))
+        check(self.lib.dgr_couple_dgshed(self.h, n, _ip(mem), _ip(loc), _ip(cn), _dp(ch), _dp(gm), _dp(swe),
+                                         _dp(ghost)), "dgr_couple_dgshed")
+        return swe, ghost
+
     def oneTimeStep2D(self, t, dt):
         """oneTimeStep2D with a host dt, every member."""
         check(self.lib.dgr_one_time_step(self.h, float(t), float(dt)), "dgr_one_time_step")
 
     def set_tape_budget(self, nbytes):
-        """Device memory the adjoint may keep taped segments in (< 0: half the
-        free memory, 0: recompute every segment)."""
+        """Device memory the adjoint may keep taped segments in (< 0: the free
+        memory less 1/16 of the device's, 0: recompute every segment)."""
         check(self.lib.dgr_set_tape_budget(self.h, int(nbytes)), "dgr_set_tape_budget")
 
     def set_infiltration(self, K, psi=0.0, theta_e=0.0, s_e=0.0, member=0):

@@ -275,6 +275,36 @@ dgr_status dgr_get_boundary(dgr_handle *h, double *bzeta, double *bqn);
 dgr_status dgr_couple_channel_ends(dgr_handle *h, int nends, const int *end_off, const int *edge_member,
                                    const int *edge_local, const int *end_dir, const double *end_A,
                                    const double *end_Q, double *ghost_A, double *ghost_Q);
+
+/* DG-SWEM's coupling edge with DGSHED's 1-D channels, 2-D side (extension:
+ * this handle's members stand where DG-SWEM's mesh stands).  Replaces the
+ * exchange of src/DG_timestep.F:55-59 -- GET_CHANNELS_VALUES
+ * (obtain_channels_values, DGSHED/couple_domains.c:248-295),
+ * POPULATE_SWE_ARRAY (src/couple2dgshed.F:22-81) and SHARE_SWE_VALUES
+ * (receive_swe_values, DGSHED/couple_domains.c:297-335) -- for n coupled
+ * boundary edges edge_member[i]/edge_local[i] (member-local edge numbers):
+ *  - channel -> 2-D: chan[6i..] = (A, Q, b, m1, m2, z) of the channel end
+ *    (conn[i] = 0, the channel's first node: A[1], Q[1] and element 0's
+ *    b, m1, m2, z; conn[i] = 1, its last node: A[NumNodes], Q[NumNodes] and
+ *    the last element's, Q negated) give zeta = getH(A, b, m1, m2) - z
+ *    (DGSHED/constitutive_equations.c:6-24) and Qn; the edge takes
+ *    bzeta = zeta and bQn = Qn / len, the exterior state CHANNEL_EDGE_HYDRO
+ *    builds (src/channel_edge_hydro.F:72-75).  The edge needs a
+ *    BdryPrescribed code and params.bc_sentinels = 0 (a channel surface may
+ *    lie below the datum);
+ *  - 2-D -> channel: the element's trace at the edge's two Gauss points
+ *    (its CR basis as Compute2DInnerProducts evaluates it, weights 1) gives
+ *    swe[2i] = 0.5 ZE_AVG and swe[2i+1] = 0.5 len Q_N_SUM (may be NULL), and
+ *    with geom[4i..] = (b, m1, m2, z) of the channel end (conn 0: element
+ *    0's; conn 1: the last node's NodalB, Nodalm1, Nodalm2, NodalZ) the end's
+ *    ghost area ghost[2i] = h b + 0.5 (m1 + m2) h^2, h = swe[2i] + z, and
+ *    discharge ghost[2i+1] = +-swe[2i+1] (the caller's A[0], Q[0] or
+ *    A[NumNodes+1], Q[NumNodes+1]).
+ * The trace uses the state before the new boundary data.  DGR_ERR_ARG if an
+ * edge is no boundary edge (those edges are left unchanged). */
+dgr_status dgr_couple_dgshed(dgr_handle *h, int n, const int *edge_member, const int *edge_local,
+                             const int *conn, const double *chan, const double *geom, double *swe,
+                             double *ghost);
 dgr_status dgr_member_set_time(dgr_handle *h, int member, double t, double dt);
 dgr_status dgr_member_set_hydrograph(dgr_handle *h, int member, int code, int npts, const double *t,
                                      const double *zeta_b, const double *qn_b);

@@ -99,6 +99,7 @@ def _load_oracle(path=LIB):
             "orc_log": (C.c_double, [C.c_double]),
             "orc_log1p": (C.c_double, [C.c_double]),
             "orc_couple_ends": (C.c_int, [C.POINTER(vp), C.c_int, ip, ip, ip, ip, dp, dp, dp, dp]),
+            "orc_couple_dgshed": (C.c_int, [vp, C.c_int, ip, ip, dp, dp, dp, dp]),
             "orc_one_time_step": (C.c_int, [vp, C.c_double, C.c_double]),
             "orc_run": (C.c_int, [vp, C.c_int, dp, dp]),
             "orc_pow43": (C.c_double, [C.c_double]),
@@ -130,6 +131,24 @@ def dlog(x):
 
 def dlog1p(x):
     return _load_oracle().orc_log1p(float(x))
+
+
+def couple_dgshed(o, edges, conn, chan, geom):
+    """DG-SWEM's channel coupling edge, 2-D side, on an Oracle: sets the
+    coupled boundary edges' bzeta/bQn from the channel ends (chan: [n][6] =
+    A, Q, b, m1, m2, z) and returns (swe [n][2] = edge-averaged zeta and len x
+    edge-averaged q.n, ghost [n][2] = the ends' ghost area and discharge,
+    from geom: [n][4] = b, m1, m2, z)."""
+    lib = _load_oracle()
+    e = np.ascontiguousarray(edges, np.int32)
+    c = np.ascontiguousarray(conn, np.int32)
+    ch = np.ascontiguousarray(chan, np.float64).reshape(-1, 6)
+    gm = np.ascontiguousarray(geom, np.float64).reshape(-1, 4)
+    n = e.size
+    swe, ghost = np.zeros((n, 2)), np.zeros((n, 2))
+    if lib.orc_couple_dgshed(o.h, n, _ip(e), _ip(c), _dp(ch), _dp(gm), _dp(swe), _dp(ghost)):
+        raise ValueError("a coupled edge is not a boundary edge")
+    return swe, ghost
 
 
 def couple_ends(juncs, end_off, edge_junc, edge_local, end_dir, end_A, end_Q):

@@ -32,6 +32,13 @@
  *   tide / rain / cfl     extensions with no reference code: pinned by closed
  *                            forms (closed basin under rain rises at exactly R,
  *                            tide value equals datum + amp*orc_sin(...)).
+ *   orc_couple_dgshed     DG-SWEM's channel coupling edge: Fortran (no
+ *                            compiler here) and DGSHED sources whose headers
+ *                            are absent, so parity unpinned against compiled
+ *                            code; pinned by closed forms (getH inverts the
+ *                            trapezoid area, a linear trace averages to the
+ *                            edge midpoint value, an exchange of a channel at
+ *                            the 2-D level returns its own area).
  */
 #include "dgswe_oracle.h"
 
@@ -994,6 +1001,120 @@ int orc_couple_ends(orc_state *const *juncs, int nends, const int *end_off, cons
         }
         ghost_A[e] = area;                                                          /* :110-114, :208-214 */
         ghost_Q[e] = end_dir[e] > 0 ? qn : -qn;
+    }
+    return 0;
+}
+
+/* ------------------------------------------------------------------------ */
+/* DG-SWEM's coupling edge with DGSHED's 1-D channels, 2-D side.             */
+/* getH: depth of a trapezoidal channel of wet area A, bed width b and side  */
+/* slopes m1, m2 (DGSHED/constitutive_equations.c:6-24).                     */
+/* ------------------------------------------------------------------------ */
+static double orc_get_h(double A, double b, double m1, double m2)
+{
+    double H;
+    if (m1 > 0 || m2 > 0)
+        H = (-b + sqrt(b * b + 2 * A * m1 + 2 * A * m2)) / (m1 + m2);
+    else
+        H = A / b;
+    return H;
+}
+
+/* f (one field's three midpoint values of element el) at the two Gauss     */
+/* points of the element's local edge k: the CR basis at the points, as     */
+/* Compute2DInnerProducts evaluates it (2DInnerProducts.c:30-87), summed    */
+/* over the three basis functions (the DG-SWEM trace, src/couple2dgshed.F:  */
+/* 55-64, with this element's basis in place of the modal one).             */
+static void orc_edge_trace(const orc_state *s, int el, int k, const double *f, double out[2])
+{
+    const orc_mesh *m = &s->m;
+    const int v0 = m->elto_vert[el * 3], v1 = m->elto_vert[el * 3 + 1], v2 = m->elto_vert[el * 3 + 2];
+    const double x1 = (m->x[v0] + m->x[v1]) / 2, x2 = (m->x[v1] + m->x[v2]) / 2, x3 = (m->x[v2] + m->x[v0]) / 2;
+    const double y1 = (m->y[v0] + m->y[v1]) / 2, y2 = (m->y[v1] + m->y[v2]) / 2, y3 = (m->y[v2] + m->y[v0]) / 2;
+    const double sqrt3 = sqrt(3);
+    double xe[2], ye[2];
+    if (k == 0) {
+        xe[0] = (sqrt3 * x1 - x2 + x3) / sqrt3;  ye[0] = (sqrt3 * y1 - y2 + y3) / sqrt3;
+        xe[1] = (sqrt3 * x1 + x2 - x3) / sqrt3;  ye[1] = (sqrt3 * y1 + y2 - y3) / sqrt3;
+    } else if (k == 1) {
+        xe[0] = (x1 + sqrt3 * x2 - x3) / sqrt3;  ye[0] = (y1 + sqrt3 * y2 - y3) / sqrt3;
+        xe[1] = (-x1 + sqrt3 * x2 + x3) / sqrt3; ye[1] = (-y1 + sqrt3 * y2 + y3) / sqrt3;
+    } else {
+        xe[0] = (-x1 + x2 + sqrt3 * x3) / sqrt3; ye[0] = (-y1 + y2 + sqrt3 * y3) / sqrt3;
+        xe[1] = (x1 - x2 + sqrt3 * x3) / sqrt3;  ye[1] = (y1 - y2 + sqrt3 * y3) / sqrt3;
+    }
+    const double den = x1 * y2 - x1 * y3 + x2 * y3 - x2 * y1 + x3 * y1 - x3 * y2;
+    for (int n = 0; n < 2; ++n) {
+        const double xq = xe[n], yq = ye[n];
+        const double P1 = -(yq * x2 - yq * x3 - xq * y2 + x3 * y2 + xq * y3 - x2 * y3) / den;
+        const double P2 = -(-x1 * yq + x3 * yq - x3 * y1 + xq * y1 - xq * y3 + x1 * y3) / den;
+        const double P3 = -(yq * x1 - yq * x2 - xq * y1 + x2 * y1 + xq * y2 - x1 * y2) / den;
+        double t = f[3 * el] * P1;
+        t = t + f[3 * el + 1] * P2;
+        t = t + f[3 * el + 2] * P3;
+        out[n] = t;
+    }
+}
+
+/* For coupled boundary edge i (edges[i]):                                  */
+/*  channel -> 2-D (obtain_channels_values, DGSHED/couple_domains.c:248-295; */
+/*  CHANNEL_EDGE_HYDRO, src/channel_edge_hydro.F:72-75): the channel end's   */
+/*  chan[6i..] = (A, Q, b, m1, m2, z) (conn[i] 0: A[1], Q[1] and element 0's */
+/*  geometry; 1: A[NumNodes], Q[NumNodes] and the last element's, Q         */
+/*  negated) give zeta = getH(A, b, m1, m2) - z and Qn; the edge takes       */
+/*  bzeta = zeta and bQn = Qn / len;                                         */
+/*  2-D -> channel (POPULATE_SWE_ARRAY, src/couple2dgshed.F:22-81;           */
+/*  receive_swe_values, DGSHED/couple_domains.c:297-335): el_in's trace at  */
+/*  the edge's two Gauss points (weights 1) gives swe[2i] = 0.5 ZE_AVG and   */
+/*  swe[2i+1] = 0.5 len Q_N_SUM (q.n with el_in's outward normal of         */
+/*  compute2DL.c:98-132), and with geom[4i..] = (b, m1, m2, z) (conn 0:      */
+/*  element 0's; 1: the last node's Nodal*) the channel end's ghost area     */
+/*  ghost[2i] = h b + 0.5 (m1 + m2) h^2, h = swe[2i] + z, and discharge      */
+/*  ghost[2i+1] = +-swe[2i+1].  -1 if an edge is no boundary edge.           */
+int orc_couple_dgshed(orc_state *s, int n, const int *edges, const int *conn, const double *chan,
+                      const double *geom, double *swe, double *ghost)
+{
+    const orc_mesh *m = &s->m;
+    for (int i = 0; i < n; ++i) {
+        const int ed = edges[i];
+        if (ed < 0 || ed >= m->num_edges) return -1;
+        const int el = m->edg_to_els[2 * ed];
+        if (m->edg_to_els[2 * ed + 1] != el) return -1;
+        const int v0 = m->edg_to_vert[2 * ed], v1 = m->edg_to_vert[2 * ed + 1];
+        const int *ein = &m->elto_vert[3 * el];
+        const int kin = orc_local_edge(ein, v0, v1);
+        if (kin < 0) return -1;
+        const double *X = m->x, *Y = m->y;
+        const double len = sqrt((X[v1] - X[v0]) * (X[v1] - X[v0]) + (Y[v1] - Y[v0]) * (Y[v1] - Y[v0]));
+        const double jac = X[ein[0]] * (Y[ein[1]] - Y[ein[2]]) + X[ein[1]] * (Y[ein[2]] - Y[ein[0]]) +
+                           X[ein[2]] * (Y[ein[0]] - Y[ein[1]]);
+        const int a = ein[kin], b = ein[(kin + 1) % 3];
+        const double nx = orc_sign(jac) * (Y[b] - Y[a]) / len;
+        const double ny = orc_sign(jac) * (X[a] - X[b]) / len;
+        /* channel -> 2-D */
+        const double *c = chan + 6 * i;
+        const double ht = orc_get_h(c[0], c[2], c[3], c[4]);
+        const double zeta_ch = ht - c[5];
+        const double qn_ch = conn[i] == 0 ? c[1] : -c[1];
+        /* 2-D -> channel, from the state before the new boundary data */
+        double zt[2], qxt[2], qyt[2];
+        orc_edge_trace(s, el, kin, s->zeta, zt);
+        orc_edge_trace(s, el, kin, s->qx, qxt);
+        orc_edge_trace(s, el, kin, s->qy, qyt);
+        double ze_avg = 0.0, q_n_sum = 0.0;
+        for (int q = 0; q < 2; ++q) {
+            const double q_n_int = qxt[q] * nx + qyt[q] * ny;
+            ze_avg = ze_avg + zt[q] * 1.0;
+            q_n_sum = q_n_sum + q_n_int * 1.0;
+        }
+        const double sz = 0.5 * ze_avg, sq = 0.5 * len * q_n_sum;
+        if (swe) { swe[2 * i] = sz; swe[2 * i + 1] = sq; }
+        const double *gm = geom + 4 * i;
+        const double htval = sz + gm[3];
+        ghost[2 * i] = htval * gm[0] + 0.5 * (gm[1] + gm[2]) * htval * htval;
+        ghost[2 * i + 1] = conn[i] == 0 ? sq : -sq;
+        s->bzeta[ed] = zeta_ch;
+        s->bqn[ed] = qn_ch / len;
     }
     return 0;
 }

@@ -104,6 +104,13 @@ int orc_set_infiltration(orc_state *s, double K, double psi, double theta_e, dou
 int orc_couple_ends(orc_state *const *juncs, int nends, const int *end_off, const int *edge_junc,
                     const int *edge_local, const int *end_dir, const double *end_A, const double *end_Q,
                     double *ghost_A, double *ghost_Q);
+/* DG-SWEM's coupling edge with DGSHED's 1-D channels, 2-D side
+ * (src/couple2dgshed.F:22-81, src/channel_edge_hydro.F:72-75,
+ * DGSHED/couple_domains.c:248-335): sets the coupled boundary edges'
+ * bzeta/bQn from the channel ends and returns the 2-D edge values (swe) and
+ * the ends' ghost area and discharge (ghost).  0 or -1. */
+int orc_couple_dgshed(orc_state *s, int n, const int *edges, const int *conn, const double *chan,
+                      const double *geom, double *swe, double *ghost);
 double orc_ga_cumulative(double K, double P, double t);
 double orc_log(double x);
 double orc_log1p(double x);

@@ -370,3 +370,102 @@ def test_adjoint_pieces_against_finite_differences(tmp_path):
     r = subprocess.run([exe], capture_output=True, text=True)
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def _dgshed_case(seed, n=6):
+    """A perturbed mesh with a random state and n channel ends on its
+    prescribed boundary edges (rectangular and trapezoidal sections)."""
+    rng = np.random.default_rng(seed)
+    mesh = perturbed_mesh(N=6, seed=seed)
+    edges = rng.choice(np.nonzero(np.asarray(mesh.bdry) > 0)[0], n, replace=False).astype(np.int32)
+    conn = rng.integers(0, 2, n).astype(np.int32)
+    b = rng.uniform(1.0, 5.0, n)
+    m1 = np.where(np.arange(n) % 2 == 0, 0.0, rng.uniform(0.2, 2.0, n))
+    m2 = np.where(m1 == 0, 0.0, rng.uniform(0.2, 2.0, n))
+    z = rng.uniform(1.5, 2.5, n)
+    return rng, mesh, edges, conn, b, m1, m2, z
+
+
+def test_dgshed_channel_depth_inverts_the_section_area():
+    """Channel -> 2-D (obtain_channels_values, DGSHED/couple_domains.c:
+    248-295): a channel end holding depth h over bed z in a trapezoidal or
+    rectangular section of area A = h b + (m1 + m2) h^2 / 2 hands the edge
+    bzeta = getH(A) - z = h - z (getH, constitutive_equations.c:6-24) and
+    bQn = +-Q / len (CHANNEL_EDGE_HYDRO reads Q_N_EXT = CHVALS(2)/EDG_LEN)."""
+    rng, mesh, edges, conn, b, m1, m2, z = _dgshed_case(3)
+    n = edges.size
+    h = rng.uniform(0.05, 4.0, n)
+    A = h * b + 0.5 * (m1 + m2) * h * h
+    Q = rng.uniform(-3.0, 3.0, n)
+    o = oracle.Oracle(mesh, math=oracle.MATH_DEV, bc_sentinels=0)
+    o.set_state(*random_state(mesh, seed=4))
+    oracle.couple_dgshed(o, edges, conn, np.stack([A, Q, b, m1, m2, z], 1), np.stack([b, m1, m2, z], 1))
+    bz, bq = o.get_boundary()
+    assert np.allclose(bz[edges], h - z, rtol=0, atol=1e-12)
+    x, y = np.asarray(mesh.x), np.asarray(mesh.y)
+    v = np.asarray(mesh.edg_to_vert).reshape(-1, 2)[edges]
+    length = np.hypot(x[v[:, 1]] - x[v[:, 0]], y[v[:, 1]] - y[v[:, 0]])
+    assert np.allclose(bq[edges], np.where(conn == 0, Q, -Q) / length, rtol=1e-14, atol=0)
+
+
+def test_dgshed_trace_of_a_linear_field():
+    """2-D -> channel (POPULATE_SWE_ARRAY, src/couple2dgshed.F:22-81): the
+    Gauss-point average of the trace of a linear field is its value at the
+    edge midpoint, and 0.5 len Q_N_SUM is len times the midpoint normal
+    discharge with the element's outward normal (the CR DOFs of a linear
+    field are its midpoint values)."""
+    rng, mesh, edges, conn, b, m1, m2, z = _dgshed_case(5)
+    x, y = np.asarray(mesh.x), np.asarray(mesh.y)
+    ev = np.asarray(mesh.elto_vert).reshape(-1, 3)
+    mx = 0.5 * (x[ev] + x[np.roll(ev, -1, axis=1)])   # midpoint k between vertices k, k+1
+    my = 0.5 * (y[ev] + y[np.roll(ev, -1, axis=1)])
+    cz, cx, cy = rng.normal(size=3), rng.normal(size=3), rng.normal(size=3)
+    lin = lambda c: (c[0] + c[1] * mx / 100 + c[2] * my / 100).ravel()
+    o = oracle.Oracle(mesh, math=oracle.MATH_DEV, bc_sentinels=0)
+    o.set_state(lin(cz), lin(cx), lin(cy))
+    n = edges.size
+    chan = np.stack([np.ones(n), np.zeros(n), b, m1, m2, z], 1)
+    swe, ghost = oracle.couple_dgshed(o, edges, conn, chan, np.stack([b, m1, m2, z], 1))
+    v = np.asarray(mesh.edg_to_vert).reshape(-1, 2)[edges]
+    ex, ey = 0.5 * (x[v[:, 0]] + x[v[:, 1]]), 0.5 * (y[v[:, 0]] + y[v[:, 1]])
+    f = lambda c: c[0] + c[1] * ex / 100 + c[2] * ey / 100
+    assert np.allclose(swe[:, 0], f(cz), rtol=0, atol=1e-12)
+    # outward normal of the boundary element, from its centroid
+    el = np.asarray(mesh.edg_to_els).reshape(-1, 2)[edges, 0]
+    tx, ty = x[v[:, 1]] - x[v[:, 0]], y[v[:, 1]] - y[v[:, 0]]
+    length = np.hypot(tx, ty)
+    nx, ny = ty / length, -tx / length
+    cxe, cye = x[ev[el]].mean(1), y[ev[el]].mean(1)
+    flip = np.sign(nx * (ex - cxe) + ny * (ey - cye))
+    nx, ny = nx * flip, ny * flip
+    assert np.allclose(swe[:, 1], length * (f(cx) * nx + f(cy) * ny), rtol=0, atol=1e-11)
+    assert np.array_equal(ghost[:, 1], np.where(conn == 0, swe[:, 1], -swe[:, 1]))
+
+
+def test_dgshed_exchange_at_one_level_returns_the_channel_area():
+    """A 2-D surface at the channel's own level (zeta = h - z everywhere,
+    still): the channel end gets back its own area (receive_swe_values,
+    DGSHED/couple_domains.c:297-335, with the end's geometry) and zero
+    discharge, and the edge gets bzeta = zeta."""
+    rng, mesh, edges, conn, b, m1, m2, z = _dgshed_case(9)
+    n = edges.size
+    h = rng.uniform(0.5, 3.0, n)
+    level = 0.7
+    z = h - level                       # every channel end at the same level
+    A = h * b + 0.5 * (m1 + m2) * h * h
+    o = oracle.Oracle(mesh, math=oracle.MATH_DEV, bc_sentinels=0)
+    n3 = 3 * mesh.num_el
+    o.set_state(np.full(n3, level), np.zeros(n3), np.zeros(n3))
+    swe, ghost = oracle.couple_dgshed(o, edges, conn, np.stack([A, np.zeros(n), b, m1, m2, z], 1),
+                                      np.stack([b, m1, m2, z], 1))
+    # the three basis values sum to one up to rounding of the basis
+    # evaluation (coordinates ~100, element size ~15)
+    assert np.allclose(swe[:, 0], level, rtol=1e-13, atol=0)
+    assert np.all(swe[:, 1] == 0)
+    assert np.allclose(ghost[:, 0], A, rtol=1e-13, atol=0)
+    bz, _ = o.get_boundary()
+    assert np.allclose(bz[edges], level, rtol=0, atol=1e-13)
+    with pytest.raises(ValueError):      # an interior edge is refused
+        inner = np.nonzero(np.asarray(mesh.edg_to_els[0::2]) != np.asarray(mesh.edg_to_els[1::2]))[0][:1]
+        oracle.couple_dgshed(o, inner.astype(np.int32), conn[:1], np.stack([A, A, b, m1, m2, z], 1)[:1],
+                             np.stack([b, m1, m2, z], 1)[:1])
