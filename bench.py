@@ -40,9 +40,9 @@ STAGE_KERNELS = {0: "k_stage (fused flux gather + RHS + RK update)",
                  1: "k_flux + k_rhs (element-centric fluxes, then RHS + RK update)",
                  2: "k_edge_flux + k_rhs (edge-centric fluxes, then RHS + RK update)",
                  3: "k_stage_persist (edge fluxes, RHS + RK update, node bounds, limiter in one cooperative launch)"}
-STAGE_ALL = {0: "k_stage, k_node_minmax, k_limit",
-             1: "k_flux, k_rhs, k_node_minmax, k_limit",
-             2: "k_edge_flux, k_rhs, k_node_minmax, k_limit",
+STAGE_ALL = {0: "k_stage, k_node_minmax, k_limit; the step end rides on the stage-2 node-bounds launch",
+             1: "k_flux, k_rhs, k_node_minmax, k_limit; the step end rides on the stage-2 node-bounds launch",
+             2: "k_edge_flux, k_rhs, k_node_minmax, k_limit; the step end rides on the stage-2 node-bounds launch",
              3: "k_stage_persist: edge fluxes, RHS, node bounds, limiter between grid barriers"}
 
 
@@ -107,8 +107,8 @@ def load_pair(variant):
 
 
 def load_counters(variant):
-    """Per RK stage of the whole step (all kernels, k_step_end's share
-    included): HBM bytes from FETCH_SIZE/WRITE_SIZE, wave-level VALU
+    """Per RK stage of the whole step (all kernels, the step end included):
+    HBM bytes from FETCH_SIZE/WRITE_SIZE, wave-level VALU
     instructions from SQ_INSTS_VALU, FP64 FLOPs from SQ_INSTS_VALU_FLOPS_FP64
     (+ _TRANS), from the PMC passes summarised in profiles/ (None when they
     cover another variant)."""
@@ -421,7 +421,7 @@ def main():
     pair_gbs = KSTAGE_BYTES_PER_EL * ne / k_avg_s / 1e9
     traffic, valu = load_pair(variant) if a.N == 512 else (None, None)
     cnt = load_counters(variant) if a.N == 512 else None
-    stage_roof = {"kernel": "one RK stage (%s; k_step_end once per step, half to each stage)" % STAGE_ALL[variant],
+    stage_roof = {"kernel": "one RK stage (%s)" % STAGE_ALL[variant],
                   "achieved": stage_gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": stage_gbs / HBM_PEAK_GBS,
                   "traffic": None, "avg_launch_ms": stage_s * 1e3, "launches": 2 * a.steps,
                   "bytes_per_element": STAGE_BYTES_PER_EL, "bound": "hbm"}
