@@ -816,7 +816,7 @@ static void mark(dgr_handle *h)
 // places ChanNet/time_evolution.c:218-427 puts them), enqueued on h->stream
 // for every member at once.  The wetDryStatus2D call between the stages runs
 // inside the stage-1 limiter kernel.  b == nullptr: the plain in-place buffers.
-void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end)
+void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end, StepRec *rec_end, StepRec *rec_next)
 {
     const bool wdon = h->ph.wetdry != 0;
     hipStream_t s = h->stream;
@@ -847,8 +847,11 @@ void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end)
         launch_stage(h->variant, 3, wdon, h->m, h->ph, b->p1, nullptr, nullptr, h->d_bzeta, h->d_bqn, wd2, nullptr,
                      h->d_fhat, h->d_ctrl, 1, nullptr, nullptr, s);
     double *nm2 = b->nodemm2 ? b->nodemm2 : h->d_nodemm;
-    if (fuse_end) launch_node_minmax_end(h->m, h->ph, h->d_means, nm2, h->d_ctrl, h->kind[1], s);
-    else launch_node_minmax(h->m, h->d_means, nm2, b->nodearg2, s);
+    if (fuse_end)
+        launch_node_minmax_end(h->m, h->ph, h->d_means, nm2, b->nodearg2, h->d_ctrl, h->kind[1], rec_end, rec_next,
+                               h->ties_last, s);
+    else
+        launch_node_minmax(h->m, h->d_means, nm2, b->nodearg2, s);
     const bool wdn = wdon && b->wdnext;
     launch_limit(wdon, h->m, h->ph, b->u2, b->wout, nm2, modify_ptr(h, 1), h->d_ctrl, 1,
                  wdn ? wd2 : nullptr, wdn ? b->wdnext : nullptr, s, fuse_end ? 1 : 0);
@@ -921,11 +924,8 @@ void enqueue_run_step(dgr_handle *h, const StepBufs *b, StepRec *rec_end, StepRe
     }
     const StepBufs run = {h->d_w, h->d_w1, h->d_w1, h->d_w, h->d_w, h->d_wd, h->d_wdn, h->d_wd, h->d_wdn, true,
                           nullptr, nullptr, nullptr, nullptr, false};
-    // a plain step (no tape, no records) ends inside its stage-2 node bounds launch
-    const bool plain = !b && !rec_end && !rec_next;
-    enqueue_stages(h, b ? b : &run, plain);
-    if (!plain)
-        launch_step_end(h->m, h->ph, h->d_ctrl, h->kind[1], true, h->stream, rec_end, rec_next, h->ties_last);
+    // the step ends inside its stage-2 node bounds launch
+    enqueue_stages(h, b ? b : &run, true, rec_end, rec_next);
 }
 
 extern "C" dgr_status dgr_one_time_step(dgr_handle *h, double t, double dt)

@@ -23,8 +23,8 @@ void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *
                   hipStream_t s, int step_adj = 0);
 // stage-2 node bounds with the step end of every member (k_step_end, next = 1)
 // in the same launch
-void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *means, double *nodemm, Ctrl *c,
-                            int kind, hipStream_t s);
+void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *means, double *nodemm, int *nodearg,
+                            Ctrl *c, int kind, StepRec *rec_end, StepRec *rec_next, bool ties, hipStream_t s);
 void launch_pdop(const MeshDev &m, const PhysDev &ph, double *w, Ctrl *ctrl, hipStream_t s);
 void launch_wd_status(const MeshDev &m, const PhysDev &ph, const double *w, const int *wd_in, int *wd_out,
                       hipStream_t s);
@@ -174,9 +174,11 @@ struct StepBufs {
     bool taped;
 };
 
-// fuse_end: the step end (k_step_end, next = 1) rides on the stage-2 node
-// bounds launch; the caller then launches no step end
-void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end = false);
+// fuse_end: the step end (k_step_end, next = 1, with the optional step
+// records) rides on the stage-2 node bounds launch; the caller then launches
+// no step end
+void enqueue_stages(dgr_handle *h, const StepBufs *b, bool fuse_end = false, StepRec *rec_end = nullptr,
+                    StepRec *rec_next = nullptr);
 void enqueue_step(dgr_handle *h, const StepBufs *b);
 // the steps of a run: the prologue once (step bookkeeping, start flags into
 // d_wdn), then run steps that chain the start flags through the stage-2

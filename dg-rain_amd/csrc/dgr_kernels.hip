@@ -1763,23 +1763,27 @@ __global__ __launch_bounds__(256) void k_step_end(MeshDev m, PhysDev ph, Ctrl *c
     step_end_member(m, ph, ctrls, kind, next, rec_end, rec_next, ties, (int)blockIdx.x);
 }
 
-// The stage-2 node bounds of a plain run step with the step end folded in
-// (one launch fewer per step): the first nf workgroups (nmem rounded up to a
+// The stage-2 node bounds of a run step with the step end folded in (one
+// launch fewer per step): the first nf workgroups (nmem rounded up to a
 // multiple of 8, so that the node workgroups keep their XCD mapping) end
-// one member's step each, as k_step_end with next = 1 does; the others
-// compute node bounds.  Every input of the step end (the stage-2 flux
-// partials) is complete when this launch starts, and of what it writes the
-// following stage-2 limiter reads only the step count (its error keys
-// subtract the step it has advanced).
+// one member's step each, as k_step_end with next = 1 does (with the step
+// records and tie counts of a taped step); the others compute node bounds
+// (ARG: and the elements attaining them).  Every input of the step end (the
+// stage-2 flux partials) is complete when this launch starts, and of what
+// it writes the following stage-2 limiter reads only the step count (its
+// error keys subtract the step it has advanced).
+template <bool ARG>
 __global__ __launch_bounds__(256) void k_node_minmax_end(MeshDev m, PhysDev ph, const double *__restrict__ means,
-                                                         double *__restrict__ nodemm, Ctrl *ctrls, int kind, int nf)
+                                                         double *__restrict__ nodemm, int *__restrict__ nodearg,
+                                                         Ctrl *ctrls, int kind, int nf, StepRec *rec_end,
+                                                         StepRec *rec_next, int ties)
 {
     const int b = (int)blockIdx.x;
     if (b < nf) {
-        if (b < m.nmem) step_end_member(m, ph, ctrls, kind, 1, nullptr, nullptr, 0, b);
+        if (b < m.nmem) step_end_member(m, ph, ctrls, kind, 1, rec_end, rec_next, ties, b);
         return;
     }
-    node_minmax_block<false>(m, means, nodemm, nullptr, xcd_map(b - nf, (int)gridDim.x - nf) * 256 + (int)threadIdx.x);
+    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_map(b - nf, (int)gridDim.x - nf) * 256 + (int)threadIdx.x);
 }
 
 // ---------------------------------------------------------------------------
@@ -1974,12 +1978,17 @@ hipError_t launch_stage_persist(int mode, bool wetdry, const MeshDev &m, const P
     return hipLaunchCooperativeKernel(k, dim3((unsigned)G), dim3(256), args, 0, s);
 }
 
-void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *means, double *nodemm, Ctrl *c,
-                            int kind, hipStream_t s)
+void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *means, double *nodemm, int *nodearg,
+                            Ctrl *c, int kind, StepRec *rec_end, StepRec *rec_next, bool ties, hipStream_t s)
 {
     const int nf = (m.nmem + 7) & ~7;
     const dim3 g((unsigned)(nf + (m.nn + 255) / 256)), b(256);
-    hipLaunchKernelGGL(k_node_minmax_end, g, b, 0, s, m, ph, means, nodemm, c, kind, nf);
+    if (nodearg)
+        hipLaunchKernelGGL(k_node_minmax_end<true>, g, b, 0, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
+                           rec_next, ties ? 1 : 0);
+    else
+        hipLaunchKernelGGL(k_node_minmax_end<false>, g, b, 0, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
+                           rec_next, ties ? 1 : 0);
 }
 
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
