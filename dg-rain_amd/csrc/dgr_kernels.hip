@@ -1447,11 +1447,13 @@ __device__ __forceinline__ void limit_block(const MeshDev &m, const PhysDev &ph,
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     // sumdiff*sd / max(1, kdp) with kdp in {0,1,2,3}: x/1 = x and
-                    // x/2 = x*0.5 exactly
+                    // x/2 = x*0.5 exactly.  kdp = 3 means all three diffs are
+                    // positive, and k = 0 takes it down to 2: only k = 0 can
+                    // divide by 3.
                     double redfac;
                     if (diff[k] > 0) {
                         const double num = sumdiff * sd;
-                        redfac = (kdp > 2.0) ? D.q(num, D3) : (kdp > 1.0) ? num * 0.5 : num;
+                        redfac = (k == 0 && kdp > 2.0) ? D.q(num, D3) : (kdp > 1.0) ? num * 0.5 : num;
                         kdp -= 1;
                     } else redfac = 0;
                     const double redmax = (sd > 0) ? V[k] - Vmin[k] : Vmax[k] - V[k];
