@@ -451,11 +451,14 @@ DGR_HD void limit_field_forward(const double S[9], int iv, const double Vmin[3],
         const double diff[3] = {(V[0] - avg) * s, (V[1] - avg) * s, (V[2] - avg) * s};
         double kdp = (double)((diff[0] > 0) + (diff[1] > 0) + (diff[2] > 0));
         for (int k = 0; k < 3; ++k) {
-            const double dv = fmax(1.0, kdp);
             double redfac;
             if (diff[k] > 0) {
                 d.pos |= 1u << (3 * ll + k);
-                redfac = sumdiff * s / dv;
+                // sumdiff*s / max(1, kdp), kdp >= 1 here: x/1 = x and x/2 =
+                // x*0.5 exactly, and only k = 0 can see kdp = 3 (all three
+                // positive)
+                const double num = sumdiff * s;
+                redfac = (k == 0 && kdp > 2.0) ? num / 3 : (kdp > 1.0) ? num * 0.5 : num;
                 kdp -= 1;
             } else {
                 redfac = 0;
@@ -493,7 +496,8 @@ DGR_HD void limit_field_reverse(const LimFieldDec &d, int iv, const double Sb[9]
                     // the divisor max(1, kdp): positives at k and after in this pass
                     const unsigned later = (d.pos >> (3 * ll + k)) & (7u >> k);
                     const int cnt = (int)(later & 1u) + (int)((later >> 1) & 1u) + (int)((later >> 2) & 1u);
-                    sumdiffb += redb * s / (cnt > 1 ? cnt : 1);
+                    const double x = redb * s;
+                    sumdiffb += (cnt == 3) ? x / 3 : (cnt == 2) ? x * 0.5 : x;
                 }
             } else {
                 if (s > 0) { Vb[k] += redb; Vminb[k] -= redb; }
@@ -513,9 +517,10 @@ DGR_HD void limit_field_reverse(const LimFieldDec &d, int iv, const double Sb[9]
         else Vminb[k] += Vb[k];
     }
     // V0 = (w1-w2+w3, w1+w2-w3, -w1+w2+w3), avg = (w1+w2+w3)/3
-    Sin_b[3 * iv] += V0b[0] + V0b[1] - V0b[2] + avgb / 3;
-    Sin_b[3 * iv + 1] += -V0b[0] + V0b[1] + V0b[2] + avgb / 3;
-    Sin_b[3 * iv + 2] += V0b[0] - V0b[1] + V0b[2] + avgb / 3;
+    const double avgb3 = avgb / 3;
+    Sin_b[3 * iv] += V0b[0] + V0b[1] - V0b[2] + avgb3;
+    Sin_b[3 * iv + 1] += -V0b[0] + V0b[1] + V0b[2] + avgb3;
+    Sin_b[3 * iv + 2] += V0b[0] - V0b[1] + V0b[2] + avgb3;
 }
 
 // ---------------------------------------------------------------------------
