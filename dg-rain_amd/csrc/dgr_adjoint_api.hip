@@ -178,6 +178,8 @@ extern "C" dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, 
     }
     DGR_HIP(hipMemcpyAsync(h->d_nodes + M.nd_off, nodes.data(), M.nn * sizeof(double4), hipMemcpyHostToDevice,
                            h->stream));
+    if (z) launch_edge_geometry(h->m, h->d_egeo, h->stream);   // z_edge follows the bed
+    DGR_HIP(hipGetLastError());
     DGR_HIP(hipStreamSynchronize(h->stream));
     return DGR_OK;
 }
@@ -238,7 +240,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&actl, nm));
     // reverse buffers: slot/zbe pairs per stage (the edge part of a stage's
     // input adjoint is consumed by the limiter reverse that comes next)
-    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
+    double *wb, *wb0, *pb, *ub, *fhb, *slot1, *slot2, *zbe1, *zbe2, *vmb, *nsum, *zvb, *nvb, *dtp, *tdp, *gz, *gn;
     DGR_HIP(B.alloc(&wb, W9));
     DGR_HIP(B.alloc(&wb0, W9));
     DGR_HIP(B.alloc(&pb, W9));
@@ -249,6 +251,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&zbe1, 3 * P));
     DGR_HIP(B.alloc(&zbe2, 3 * P));
     DGR_HIP(B.alloc(&vmb, 18 * P));
+    DGR_HIP(B.alloc(&nsum, 8 * nn));
     DGR_HIP(B.alloc(&zvb, 3 * P));
     DGR_HIP(B.alloc(&nvb, 3 * P));
     DGR_HIP(B.alloc(&dtp, 2 * (size_t)maxparts));
@@ -260,7 +263,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     double *cnm;
     int *cna;
     DGR_HIP(B.alloc(&cnm, 6 * nn));
-    DGR_HIP(B.alloc(&cna, 6 * nn));
+    DGR_HIP(B.alloc(&cna, 8 * nn));
     // Tapes: each step's start state, both stage outputs before and after
     // the limiter, both stages' flags, node bounds and the elements attaining
     // them.  The last R segments are taped by the forward pass itself and
@@ -271,7 +274,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     const bool wdon = h->ph.wetdry != 0;
     auto up = [](size_t b) { return (b + 255) & ~(size_t)255; };
     const size_t b_w = up(Wb * K), b_i = up(Ib * K), b_nm = up(12 * nn * K * sizeof(double)),
-                 b_na = up(12 * nn * K * sizeof(int));
+                 b_na = up(16 * nn * K * sizeof(int));
     const size_t seg_full = 4 * b_w + 2 * b_i + b_nm + b_na;
     // compact tapes: states and flags only; the reverse recomputes each
     // stage's node bounds from the taped pre-limiter state (means, then the
@@ -354,8 +357,8 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
         if (!compact) {
             bf.nodemm1 = T.nm + 6 * nn * (2 * k);
             bf.nodemm2 = T.nm + 6 * nn * (2 * k + 1);
-            bf.nodearg1 = T.na + 6 * nn * (2 * k);
-            bf.nodearg2 = T.na + 6 * nn * (2 * k + 1);
+            bf.nodearg1 = T.na + 8 * nn * (2 * k);
+            bf.nodearg2 = T.na + 8 * nn * (2 * k + 1);
         }
         bf.taped = true;
         return bf;
@@ -462,8 +465,8 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             }
             const double *nm1 = compact ? cnm : T.nm + 6 * nn * (2 * k);
             const double *nm2 = compact ? cnm : T.nm + 6 * nn * (2 * k + 1);
-            const int *na1 = compact ? cna : T.na + 6 * nn * (2 * k);
-            const int *na2 = compact ? cna : T.na + 6 * nn * (2 * k + 1);
+            const int *na1 = compact ? cna : T.na + 8 * nn * (2 * k);
+            const int *na2 = compact ? cna : T.na + 8 * nn * (2 * k + 1);
             // a stage's node bounds from its taped pre-limiter state
             auto node_bounds = [&](const double *u) {
                 launch_means(h->m, u, h->d_means, s);
@@ -475,7 +478,8 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             launch_adj_limit(h->m, h->ph, u2, nm2, wb, have_slot1 ? slot1 : nullptr, have_slot1 ? zbe1 : nullptr,
                              zvb, ub, vmb, actl, &pending, s);
             pending.active = 0;
-            launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, vmb, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
+            launch_adj_node_sums(h->m, vmb, nsum, s);
+            launch_adj_rhs(2, h->m, h->ph, p1, w0, u2, ub, nsum, na2, wdb, rn, pb, wb0, fhb, zvb, nvb,
                            dtp + maxparts, s);
             // the max-|lambda| adjoint (k_adj_scalar of step n+1) is shared
             // by the edges whose operand attains the recorded max; the taped
@@ -492,7 +496,8 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             // (the stage-2 kernels that read cna are ahead on the stream)
             if (compact) node_bounds(u1);
             launch_adj_limit(h->m, h->ph, u1, nm1, pb, slot2, zbe2, zvb, ub, vmb, actl, nullptr, s);
-            launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, vmb, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
+            launch_adj_node_sums(h->m, vmb, nsum, s);
+            launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nsum, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);
             launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, 0, s);
             have_slot1 = true;

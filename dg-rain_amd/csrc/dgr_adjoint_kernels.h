@@ -23,38 +23,6 @@
 // max-|lambda| adjoint (dt = cfl * minEdgLength / max|lambda|).
 #pragma once
 
-// ---------------------------------------------------------------------------
-// node min/max (SlopeLimiter2D.c:106-153) of the element means of U, and the
-// first element in ascending order that attains each (-1: the sentinel stood)
-// ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_node_arg(MeshDev m, const double *__restrict__ U, double *__restrict__ nodemm,
-                                                  int *__restrict__ nodearg)
-{
-    const int i = xcd_block() * 256 + (int)threadIdx.x;
-    if (i >= m.nn) return;
-    const int P = m.pitch;
-    double mn[3] = {99999, 99999, 99999}, mx[3] = {-99999, -99999, -99999};
-    int amn[3] = {-1, -1, -1}, amx[3] = {-1, -1, -1};
-    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
-    for (int j = b; j < en; ++j) {
-        const int el = m.n2e[j] & N2E_EL_MASK;
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            const double a = (U[(3 * f) * P + el] + U[(3 * f + 1) * P + el] + U[(3 * f + 2) * P + el]) / 3;
-            if (a < mn[f]) { mn[f] = a; amn[f] = el; }
-            if (a > mx[f]) { mx[f] = a; amx[f] = el; }
-        }
-    }
-    const int nn = m.nn;
-#pragma unroll
-    for (int f = 0; f < 3; ++f) {
-        nodemm[f * nn + i] = mn[f];
-        nodemm[(3 + f) * nn + i] = mx[f];
-        nodearg[f * nn + i] = amn[f];
-        nodearg[(3 + f) * nn + i] = amx[f];
-    }
-}
-
 __device__ __forceinline__ void adj_scalar_member(const MeshDev &m, int mid, const AdjScalarArgs &sa, AdjCtrl *actl);
 
 // limiter reverse.  U: pre-limit stage output; Sb (+ slotb when given): the
@@ -65,7 +33,7 @@ __device__ __forceinline__ void adj_scalar_member(const MeshDev &m, int mid, con
 // the limited state first, and adds the adjoint of the vertex depths it
 // reads to zvb.  Writes Ub (own part of the pre-limit adjoint) and
 // vmb[18][pitch]: adjoint of Vmin (f*3+k) and Vmax (9+f*3+k) at vertex k.
-__global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const double *__restrict__ U,
+__global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, const double *__restrict__ U,
                                                    const double *__restrict__ nodemm, const double *__restrict__ Sb,
                                                    const double *__restrict__ slotb, const double *__restrict__ zbe,
                                                    double *__restrict__ zvb, double *__restrict__ Ub,
@@ -193,37 +161,47 @@ __global__ __launch_bounds__(256) void k_adj_limit(MeshDev m, PhysDev ph, const 
     }
 }
 
-// node sums of the bound adjoints of node nd: s[f] (min) and s[3+f] (max)
-// over the node's elements in n2e order.  Entries are taken four at a time
-// with clamped indices, so every load of a group is issued before the first
-// add; an entry past the end leaves the sums as they are.
-__device__ __forceinline__ void node_bound_sums(const MeshDev &m, const double *__restrict__ vmb, int b, int en,
-                                                double s[6])
+// node sums of the bound adjoints: for every node, the sums of vmb (the
+// limiter reverse's bound adjoints, [18][pitch]) over the node's elements in
+// n2e order, as nsum[8 i + f], f = 0..2 the min bounds of the three fields,
+// 3..5 the max bounds (6..7 unused).  Only the element that attains a bound
+// reads its sum (k_adj_rhs).  Entries are taken four at a time with clamped
+// indices, so every load of a group is issued before the first add.
+__global__ __launch_bounds__(256) void k_adj_node_sums(MeshDev m, const double *__restrict__ vmb,
+                                                       double *__restrict__ nsum)
 {
+    const int i = xcd_block() * 256 + (int)threadIdx.x;
+    if (i >= m.nn) return;
     const int P = m.pitch;
+    double s[6];
 #pragma unroll
     for (int f = 0; f < 6; ++f) s[f] = 0.0;
+    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
     for (int j = b; j < en; j += 4) {
         int v[4];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) v[i] = m.n2e[min(j + i, en - 1)];
+        for (int q = 0; q < 4; ++q) v[q] = m.n2e[min(j + q, en - 1)];
         double x[4][6];
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int el = v[i] & N2E_EL_MASK, k = (unsigned)v[i] >> N2E_K_SHIFT;
+        for (int q = 0; q < 4; ++q) {
+            const int el = v[q] & N2E_EL_MASK, k = (unsigned)v[q] >> N2E_K_SHIFT;
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
-                x[i][f] = vmb[(f * 3 + k) * P + el];
-                x[i][3 + f] = vmb[(9 + f * 3 + k) * P + el];
+                x[q][f] = vmb[(f * 3 + k) * P + el];
+                x[q][3 + f] = vmb[(9 + f * 3 + k) * P + el];
             }
         }
 #pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const bool in = j + i < en;
+        for (int q = 0; q < 4; ++q) {
+            const bool in = j + q < en;
 #pragma unroll
-            for (int f = 0; f < 6; ++f) s[f] = in ? s[f] + x[i][f] : s[f];
+            for (int f = 0; f < 6; ++f) s[f] = in ? s[f] + x[q][f] : s[f];
         }
     }
+    double2 *o = reinterpret_cast<double2 *>(nsum) + 4 * (size_t)i;
+    o[0] = make_double2(s[0], s[1]);
+    o[1] = make_double2(s[2], s[3]);
+    o[2] = make_double2(s[4], s[5]);
 }
 
 // sum over a workgroup of 256 threads, valid in thread 0
@@ -242,15 +220,15 @@ __device__ __forceinline__ double block_sum(double v)
 //   MODE 2: U = (W + B) 0.5 + R dt/2 -> Inb = 0.5 Ub + RHS'(W)^T (dt/2 Ub), Bb = 0.5 Ub
 // W: stage input, B: step-start state (MODE 2), U: pre-limit output, Ubown:
 // own part of the pre-limit adjoint; the node-bound part is added here, from
-// the bound adjoints vmb of the limiter reverse, for the elements that attain
-// a bound.  Outputs: Inb (MODE 2: the
+// the node sums nsum of the limiter reverse's bound adjoints (k_adj_node_sums),
+// for the elements that attain a bound.  Outputs: Inb (MODE 2: the
 // stage-input adjoint, written; MODE 1: added to Bb, which is the step-start
 // adjoint), Fhb (adjoint of Fhat), zvb/nvb (nodal z/n adjoints per element
 // vertex, added), dtparts (this workgroup's dt adjoint).
 template <int MODE>
 __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const double *__restrict__ W,
                                                     const double *__restrict__ B, const double *__restrict__ U,
-                                                    const double *__restrict__ Ubown, const double *__restrict__ vmb,
+                                                    const double *__restrict__ Ubown, const double *__restrict__ nsum,
                                                     const int *__restrict__ nodearg, const int *__restrict__ wd,
                                                     const StepRec *__restrict__ rec, double *Inb, double *Bb,
                                                     double *__restrict__ Fhb, double *__restrict__ zvb,
@@ -265,18 +243,18 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
 #pragma unroll
         for (int f = 0; f < 9; ++f) { w[f] = W[f * P + e]; u[f] = U[f * P + e]; ub[f] = Ubown[f * P + e]; }
         // the element mean enters the node bounds it attains; a bound's
-        // adjoint is the sum of the bound adjoints (vmb) over the node's
-        // elements, formed here by the element that attains it
-        int nd[3], b[3], en[3];
+        // adjoint is the sum of the bound adjoints over the node's elements
+        // (nsum), read here by the element that attains it
+        int nd[3];
         bool own[3][6];
 #pragma unroll
         for (int k = 0; k < 3; ++k) nd[k] = m.ev[k * P + e];
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            b[k] = m.n2e_off[nd[k]];
-            en[k] = m.n2e_off[nd[k] + 1];
-#pragma unroll
-            for (int f = 0; f < 6; ++f) own[k][f] = nodearg[f * nn + nd[k]] == e;
+            const int4 *na = reinterpret_cast<const int4 *>(nodearg) + 2 * nd[k];
+            const int4 a0 = na[0], a1 = na[1];
+            own[k][0] = a0.x == e; own[k][1] = a0.y == e; own[k][2] = a0.z == e;
+            own[k][3] = a0.w == e; own[k][4] = a1.x == e; own[k][5] = a1.y == e;
         }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
@@ -284,8 +262,9 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
 #pragma unroll
             for (int f = 0; f < 6; ++f) any |= own[k][f];
             if (!any) continue;
-            double sb[6];
-            node_bound_sums(m, vmb, b[k], en[k], sb);
+            const double2 *ns = reinterpret_cast<const double2 *>(nsum) + 4 * (size_t)nd[k];
+            const double2 s01 = ns[0], s23 = ns[1], s45 = ns[2];
+            const double sb[6] = {s01.x, s01.y, s23.x, s23.y, s45.x, s45.y};
 #pragma unroll
             for (int f = 0; f < 3; ++f) {
                 double add = 0.0;
@@ -406,14 +385,12 @@ __device__ __forceinline__ EdgeState edge_state(const MeshDev &m, const MemberDe
     s.boundary = (s.inf & EI_BOUNDARY) != 0;
     s.prescribed = (s.inf & EI_PRESCRIBED) != 0;
     s.code = (s.inf >> EI_CODE_SHIFT) & 0xff;
-    const double4 A = m.nodes[E.z], B = m.nodes[E.w];
-    const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
-    const double sg = (double)(((s.inf >> EI_SGN_SHIFT) & 3) - 1);
-    s.nx = sg * (B.y - A.y) / len;
-    s.ny = sg * (A.x - B.x) / len;
+    const double4 G = m.egeo[ed];   // the forward's edge geometry (k_edge_geometry)
+    s.nx = G.x;
+    s.ny = G.y;
     const double tx = -s.ny, ty = s.nx;
-    s.zb = (A.z + B.z) / 2;
-    s.dn = 1. / (s.nx * ty - s.ny * tx);
+    s.zb = G.z;
+    s.dn = G.w;
     s.zin = win[s.kin * P + s.el_in];
     s.qxin = win[(3 + s.kin) * P + s.el_in];
     s.qyin = win[(6 + s.kin) * P + s.el_in];
@@ -787,10 +764,6 @@ __global__ __launch_bounds__(256) void k_adj_gather(MeshDev m, const double *__r
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nodearg, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_node_arg, dim3((unsigned)((m.nn + 255) / 256)), dim3(256), 0, s, m, U, nodemm, nodearg);
-}
 
 void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
                       const double *slotb, const double *zbe, double *zvb, double *Ub, double *vmb, AdjCtrl *actl,
@@ -801,18 +774,23 @@ void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, cons
                        zvb, Ub, vmb, actl, sa ? *sa : none);
 }
 
+void launch_adj_node_sums(const MeshDev &m, const double *vmb, double *nsum, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_adj_node_sums, dim3((unsigned)((m.nn + 255) / 256)), dim3(256), 0, s, m, vmb, nsum);
+}
+
 void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
-                    const double *U, const double *Ubown, const double *vmb, const int *nodearg, const int *wd,
+                    const double *U, const double *Ubown, const double *nsum, const int *nodearg, const int *wd,
                     const StepRec *rec, double *Inb, double *Bb, double *Fhb, double *zvb, double *nvb,
                     double *dtparts, hipStream_t s)
 {
     const dim3 g((unsigned)(m.pitch / 256)), b(256);
     if (mode == 1)
-        hipLaunchKernelGGL(k_adj_rhs<1>, g, b, 0, s, m, ph, W, B, U, Ubown, vmb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
-                           nvb, dtparts);
+        hipLaunchKernelGGL(k_adj_rhs<1>, g, b, 0, s, m, ph, W, B, U, Ubown, nsum, nodearg, wd, rec, Inb, Bb, Fhb,
+                           zvb, nvb, dtparts);
     else
-        hipLaunchKernelGGL(k_adj_rhs<2>, g, b, 0, s, m, ph, W, B, U, Ubown, vmb, nodearg, wd, rec, Inb, Bb, Fhb, zvb,
-                           nvb, dtparts);
+        hipLaunchKernelGGL(k_adj_rhs<2>, g, b, 0, s, m, ph, W, B, U, Ubown, nsum, nodearg, wd, rec, Inb, Bb, Fhb,
+                           zvb, nvb, dtparts);
 }
 
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,

@@ -62,7 +62,7 @@ static dgr_status free_handle(dgr_handle *h)
 {
     if (!h) return DGR_OK;
     void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_edges,
-                    h->d_einfo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
+                    h->d_einfo, h->d_egeo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
                     h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts,
                     h->d_nanparts, h->d_wdn, h->d_hydro, h->d_tape, h->d_tiemax, h->d_tiecnt,
                     h->d_cpl_spec, h->d_cpl_vals};
@@ -251,6 +251,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     DGR_TRY(dalloc(&h->d_n2e, n2e_tot));
     DGR_TRY(dalloc(&h->d_edges, ned_pad));
     DGR_TRY(dalloc(&h->d_einfo, ned_pad));
+    DGR_TRY(dalloc(&h->d_egeo, ned_pad));
     DGR_TRY(dalloc(&h->d_mem, n));
     DGR_TRY(dalloc(&h->d_eblk_mem, P / 256));
     DGR_TRY(dalloc(&h->d_dblk_mem, (size_t)ned_pad / 256));
@@ -311,6 +312,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     m.pitch = pitch; m.nn = nn_tot; m.ned_pad = ned_pad; m.nmem = n;
     m.nodes = h->d_nodes; m.ev = h->d_ev; m.nb = h->d_nb; m.info = h->d_info; m.eid = h->d_eid;
     m.n2e_off = h->d_n2e_off; m.n2e = h->d_n2e; m.edges = h->d_edges; m.einfo = h->d_einfo;
+    m.egeo = h->d_egeo;
     m.mem = h->d_mem; m.eblk_mem = h->d_eblk_mem; m.dblk_mem = h->d_dblk_mem;
     m.wg = 256;
     PhysDev &ph = h->ph;
@@ -320,6 +322,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     size_t maxne = 0;
     for (int i = 0; i < n; ++i) maxne = maxne > (size_t)mem[i].ne ? maxne : (size_t)mem[i].ne;
     h->h_planes.resize(9 * maxne);
+    launch_edge_geometry(h->m, h->d_egeo, s);
     // tide value for t = 0
     launch_step_begin(h->m, h->d_ctrl, s);
     if (hipStreamSynchronize(s) != hipSuccess) { free_handle(h); return DGR_ERR_HIP; }

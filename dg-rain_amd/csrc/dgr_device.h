@@ -24,6 +24,9 @@
 //   edges        int4   [ned_pad]    {el_in, el_ex, va, vb}: EdgtoEls and el_in's
 //                                     vertices of the edge in el_in's order
 //   einfo        int    [ned_pad]    packed edge facts, see ED_* below
+//   egeo         double4[ned_pad]    {nx, ny, z_edge, 1/(nx*ty - ny*tx)} of each
+//                                     edge in el_in's orientation, from the nodes
+//                                     (one coalesced 32-byte load per edge)
 //   eblk_mem     int    [pitch/256]  member of each element workgroup
 //   dblk_mem     int    [ned_pad/256] member of each edge workgroup
 #pragma once
@@ -107,6 +110,7 @@ struct MeshDev {
     const int *n2e;
     const int4 *edges;
     const int *einfo;
+    const double4 *egeo;
     const MemberDev *mem;
     const int *eblk_mem;
     const int *dblk_mem;

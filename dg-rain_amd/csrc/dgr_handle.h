@@ -17,6 +17,7 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
 hipError_t launch_stage_persist(int mode, bool wetdry, const MeshDev &m, const PhysDev &ph, const PersistArgs &a,
                                 hipStream_t s);
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s);
+void launch_edge_geometry(const MeshDev &m, double4 *egeo, hipStream_t s);
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s);
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
                   const double *nodemm, unsigned char *modify, Ctrl *ctrl, int stage, const int *wd_in, int *wd_out,
@@ -41,12 +42,12 @@ void launch_reset_maxlam(const MeshDev &m, Ctrl *c, hipStream_t s);
 void launch_math_selftest(int n, const double *x, const double *y, double *out, hipStream_t s);
 
 // adjoint sweep (dgr_adjoint_kernels.h)
-void launch_node_arg(const MeshDev &m, const double *U, double *nodemm, int *nodearg, hipStream_t s);
 void launch_adj_limit(const MeshDev &m, const PhysDev &ph, const double *U, const double *nodemm, const double *Sb,
                       const double *slotb, const double *zbe, double *zvb, double *Ub, double *vmb, AdjCtrl *actl,
                       const AdjScalarArgs *sa, hipStream_t s);
+void launch_adj_node_sums(const MeshDev &m, const double *vmb, double *nsum, hipStream_t s);
 void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double *W, const double *B,
-                    const double *U, const double *Ubown, const double *vmb, const int *nodearg, const int *wd,
+                    const double *U, const double *Ubown, const double *nsum, const int *nodearg, const int *wd,
                     const StepRec *rec, double *Inb, double *Bb, double *Fhb, double *zvb, double *nvb,
                     double *dtparts, hipStream_t s);
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
@@ -91,6 +92,7 @@ struct dgr_handle {
     int *d_ev, *d_nb, *d_info, *d_eid, *d_n2e_off, *d_n2e;
     int4 *d_edges;   // {el_in, el_ex, va, vb} per edge
     int *d_einfo;    // packed edge facts (ED_*/EI_* bits)
+    double4 *d_egeo; // canonical edge geometry (MeshDev::egeo), refreshed when the nodes change
     MemberDev *d_mem;
     int *d_eblk_mem, *d_dblk_mem;
     double *d_bzeta, *d_bqn;

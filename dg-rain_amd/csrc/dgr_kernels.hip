@@ -994,6 +994,27 @@ __global__ __launch_bounds__(256) void k_flux(MeshDev m, PhysDev ph, const doubl
     lam_commit(lam, ph, slot, L.lb);
 }
 
+// Canonical geometry of every edge, from el_in's side as the edge loop of
+// compute2DL evaluates it (compute2DL.c:98-144): the unit normal (nx, ny),
+// the bed at the edge midpoint and 1/(nx*ty - ny*tx), (tx, ty) = (-ny, nx).
+// It depends on the nodes only, so it is computed once per mesh (and again
+// when the bed changes) instead of from two vertex gathers in every stage.
+// The squares and the vertex sum do not depend on the vertex order.
+__global__ __launch_bounds__(256) void k_edge_geometry(MeshDev m, double4 *__restrict__ egeo)
+{
+    const int ed = (int)blockIdx.x * 256 + (int)threadIdx.x;
+    if (ed >= m.ned_pad) return;
+    const int4 E = m.edges[ed];
+    const int inf = m.einfo[ed];
+    const double4 A = m.nodes[E.z], B = m.nodes[E.w];
+    const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
+    const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
+    const double nx = sg * (B.y - A.y) / len;
+    const double ny = sg * (A.x - B.x) / len;
+    const double tx = -ny, ty = nx;
+    egeo[ed] = make_double4(nx, ny, (A.z + B.z) / 2, 1.0 / (nx * ty - ny * tx));
+}
+
 // Edge variant of part 1: one thread per edge, the edge loop of compute2DL
 // (compute2DL.c:17-331) as the reference runs it, each Roe flux evaluated
 // once and stored for el_in (slot in_k) and, negated, for el_ex (slot ex_k).
@@ -1026,17 +1047,16 @@ __device__ __forceinline__ void edge_flux_block(const MeshDev &m, const PhysDev 
         const int code = (inf >> EI_CODE_SHIFT) & 0xff;
         const int led = ed - M.ed_off;                 // member-local edge id
         const long long p0 = 2LL * led;
-        // the edge's inputs: vertices, the two midpoint states, boundary data
+        // the edge's inputs: its geometry, the two midpoint states, boundary data
         struct EdgeIn {
-            double4 A, B;
+            double4 G;
             double zin, qxin, qyin, zex0, qxex0, qyex0, bz, bq;
             int wd_in, wd_ex;
         };
-        auto load = [&](const double4 *nodes, const double *w, const double *bzp, const double *bqp,
+        auto load = [&](const double4 *egeo, const double *w, const double *bzp, const double *bqp,
                         const int *wdp) {
             EdgeIn in;
-            in.A = nodes[E.z];
-            in.B = nodes[E.w];
+            in.G = egeo[ed];
             in.zin = w[kin * P + el_in]; in.qxin = w[(3 + kin) * P + el_in]; in.qyin = w[(6 + kin) * P + el_in];
             in.zex0 = in.zin; in.qxex0 = in.qxin; in.qyex0 = in.qyin;
             if (!boundary) {
@@ -1052,21 +1072,15 @@ __device__ __forceinline__ void edge_flux_block(const MeshDev &m, const PhysDev 
         LamOps ops;
         double Fi[3], Fx[3];
         bool nan_flux = false;
-        // the edge's geometry, boundary data and Roe flux(es) with division
-        // policy D.  Geometry from el_in (compute2DL.c:98-133); the squares and
-        // the vertex sum do not depend on the vertex order.
+        // the edge's boundary data and Roe flux(es) with division policy D;
+        // its geometry (compute2DL.c:98-144) comes from egeo
         auto eval = [&](const EdgeIn &in, auto &D) {
-            const double4 A = in.A, B = in.B;
             const double zin = in.zin, qxin = in.qxin, qyin = in.qyin;
             const int wd_in = in.wd_in, wd_ex = in.wd_ex;
-            const double len = sqrt((B.x - A.x) * (B.x - A.x) + (B.y - A.y) * (B.y - A.y));
-            const double sg = (double)(((inf >> EI_SGN_SHIFT) & 3) - 1);
-            const auto Dl = D.den(len);
-            const double nx = D.q(sg * (B.y - A.y), Dl);
-            const double ny = D.q(sg * (A.x - B.x), Dl);
+            const double nx = in.G.x, ny = in.G.y;
             const double tx = -ny, ty = nx;
-            const double zb = (A.z + B.z) / 2;
-            const double dn = D.q(1.0, D.den(nx * ty - ny * tx));
+            const double zb = in.G.z;
+            const double dn = in.G.w;
             double zex = in.zex0, qxex = in.qxex0, qyex = in.qyex0;
             if (inf & EI_PRESCRIBED) {                                   // compute2DL.c:149-164
                 zex = in.bz;
@@ -1111,18 +1125,18 @@ __device__ __forceinline__ void edge_flux_block(const MeshDev &m, const PhysDev 
         };
         {
             DivFast D;
-            eval(load(m.nodes, win, bzeta, bqn, wd), D);
+            eval(load(m.egeo, win, bzeta, bqn, wd), D);
             if (!D.ok()) {
                 // an operand outside the fast division's range: plain
                 // divisions, on inputs loaded again through pointers the
                 // compiler cannot match with the first loads (so none of them
                 // stays live through the fast evaluation)
-                const double4 *nodes2 = m.nodes;
+                const double4 *egeo2 = m.egeo;
                 const double *win2 = win, *bz2 = bzeta, *bq2 = bqn;
                 const int *wd2 = wd;
-                asm volatile("" : "+s"(nodes2), "+s"(win2), "+s"(bz2), "+s"(bq2), "+s"(wd2));
+                asm volatile("" : "+s"(egeo2), "+s"(win2), "+s"(bz2), "+s"(bq2), "+s"(wd2));
                 DivExact X;
-                eval(load(nodes2, win2, bz2, bq2, wd2), X);
+                eval(load(egeo2, win2, bz2, bq2, wd2), X);
             }
         }
         if (nan_flux) raise_error(ctrl, 3, led, 2 * slot);
@@ -1236,7 +1250,8 @@ __global__ __launch_bounds__(256) void k_means(MeshDev m, const double *__restri
 
 // node min/max of element means, ascending element order (SlopeLimiter2D.c:106-153).
 // ARG: also the first element that attains each bound (-1: the sentinel
-// stood), which the limiter's reverse routes the bound adjoints to.
+// stood), which the limiter's reverse routes the bound adjoints to, as one
+// 32-byte record per node: nodearg[8 i + f], f = 0..2 min, 3..5 max, 6..7 pad.
 template <bool ARG>
 __device__ __forceinline__ void node_minmax_block(const MeshDev &m, const double *__restrict__ means,
                                                   double *__restrict__ nodemm, int *__restrict__ nodearg, int i)
@@ -1259,10 +1274,11 @@ __device__ __forceinline__ void node_minmax_block(const MeshDev &m, const double
     for (int f = 0; f < 3; ++f) {
         nodemm[f * nn + i] = mn[f];
         nodemm[(3 + f) * nn + i] = mx[f];
-        if (ARG) {
-            nodearg[f * nn + i] = amn[f];
-            nodearg[(3 + f) * nn + i] = amx[f];
-        }
+    }
+    if (ARG) {
+        int4 *na = reinterpret_cast<int4 *>(nodearg) + 2 * i;
+        na[0] = make_int4(amn[0], amn[1], amn[2], amx[0]);
+        na[1] = make_int4(amx[1], amx[2], -1, -1);
     }
 }
 
@@ -1918,6 +1934,11 @@ int launch_stage(int variant, int mode, bool wetdry, const MeshDev &m, const Phy
 void launch_means(const MeshDev &m, const double *w, double *means, hipStream_t s)
 {
     hipLaunchKernelGGL(k_means, grid_for(m.pitch), dim3(256), 0, s, m, w, means);
+}
+
+void launch_edge_geometry(const MeshDev &m, double4 *egeo, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_edge_geometry, grid_for(m.ned_pad), dim3(256), 0, s, m, egeo);
 }
 
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s)
