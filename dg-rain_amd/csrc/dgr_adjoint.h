@@ -76,9 +76,10 @@ DGR_HD double roe_eval(double F[3], const RoeIn &a, double nx, double ny, double
 }
 
 // Reverse of roe_eval: Fb[3] and lamb (adjoint of the returned max|lambda|)
-// accumulate into ib.
-DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, const double Fb[3], double lamb,
-                    RoeIn &ib)
+// accumulate into ib.  With tie, lamb applies only if the max|lambda| that
+// roe_eval would return (the same operands, the same fold) equals target.
+DGR_HD void roe_adj_impl(const RoeIn &a, double nx, double ny, double lg, double g, const double Fb[3], double lamb,
+                         bool tie, double target, RoeIn &ib)
 {
     // forward values
     const double Hi = a.zi + a.zb, He = a.ze + a.zb;
@@ -90,13 +91,22 @@ DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, c
     const double c = sqrt(g * Hh);
     const double den = -2 * c;
     const double num[3] = {-c - nx * uh - ny * vh, -2 * c * ny * uh + 2 * c * nx * vh, -c + nx * uh + ny * vh};
-    const double Q[3][3] = {{num[0] / den, nx / den, ny / den},
-                            {num[1] / den, 2 * c * ny / den, -2 * c * nx / den},
-                            {num[2] / den, -nx / den, -ny / den}};
+    // Q only weighs derivatives: reciprocal products
+    const double rden = 1.0 / den;
+    const double Q[3][3] = {{num[0] * rden, nx * rden, ny * rden},
+                            {num[1] * rden, 2 * c * ny * rden, -2 * c * nx * rden},
+                            {num[2] * rden, -nx * rden, -ny * rden}};
     const double sgi = sqrt(g * Hi), sge = sqrt(g * He);
     const double li[3] = {ui * nx + vi * ny - sgi, ui * nx + vi * ny, ui * nx + vi * ny + sgi};
     const double le[3] = {ue * nx + ve * ny - sge, ue * nx + ve * ny, ue * nx + ve * ny + sge};
     const double lh[3] = {uh * nx + vh * ny - c, uh * nx + vh * ny, uh * nx + vh * ny + c};
+    if (tie) {
+        // roe_eval's max|lambda| fold over the same operands
+        double mx = fabs(li[0]);
+        const double ops[5] = {fabs(le[0]), fabs(li[1]), fabs(le[1]), fabs(li[2]), fabs(le[2])};
+        for (int i = 0; i < 5; ++i) mx = ops[i] > mx ? ops[i] : mx;
+        if (!(mx == target)) lamb = 0.0;
+    }
     double lam[3];
     int act[3];   // 0: zero, 1: lh-li, 2: le-lh, 3: |lh|
     for (int k = 0; k < 3; ++k) {
@@ -177,7 +187,6 @@ DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, c
     Hib += sgib * (0.5 * g / sgi);
     Heb += sgeb * (0.5 * g / sge);
     // Q = num / den and the constant-numerator entries
-    const double rden = 1.0 / den;
     double denb = 0;
     for (int r = 0; r < 3; ++r)
         for (int s = 0; s < 3; ++s) denb += -Q[r][s] * Qb[r][s];
@@ -243,6 +252,20 @@ DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, c
     ib.zi += Hib;
     ib.ze += Heb;
     ib.zb += Hib + Heb + zbb;
+}
+
+DGR_HD void roe_adj(const RoeIn &a, double nx, double ny, double lg, double g, const double Fb[3], double lamb,
+                    RoeIn &ib)
+{
+    roe_adj_impl(a, nx, ny, lg, g, Fb, lamb, false, 0.0, ib);
+}
+
+// the tie-gated form: lamb is shared only if this evaluation's max|lambda|
+// equals target (the recorded max of the stage)
+DGR_HD void roe_adj_tie(const RoeIn &a, double nx, double ny, double lg, double g, const double Fb[3], double lamb,
+                        double target, RoeIn &ib)
+{
+    roe_adj_impl(a, nx, ny, lg, g, Fb, lamb, true, target, ib);
 }
 
 // ---------------------------------------------------------------------------

@@ -553,10 +553,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_edge(MeshDev m, PhysDev ph, cons
                     for (int f = 0; f < 3; ++f) Fb[f] = (in_wet ? 0.0 : Fin_b[f]) + (ex_wet ? 0.0 : Fex_b[f]);
                 }
             }
-            double F[3];
-            const double l = roe_eval(F, a, s.nx, s.ny, g, g);   // |lambda| does not depend on lg
+            // (|lambda| does not depend on lg: the tie test inside roe_adj_tie
+            // sees the operands the forward folded)
             RoeIn ib = (mode == 2) ? RoeIn{0, 0, 0, 0, 0, 0, 0} : acc;
-            roe_adj(a, s.nx, s.ny, lg, g, Fb, (lam_op && l == target) ? lam_share : 0.0, ib);
+            roe_adj_tie(a, s.nx, s.ny, lg, g, Fb, lam_op ? lam_share : 0.0, target, ib);
             if (mode != 2) {
                 acc = ib;
             } else if (j == 0) {
