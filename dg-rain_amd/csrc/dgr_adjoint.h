@@ -496,6 +496,28 @@ DGR_HD void limit_field_forward(const double S[9], int iv, const double Vmin[3],
     for (int k = 0; k < 3; ++k) d.Vfin[k] = V[k];
 }
 
+// a field pass's decisions (all the reverse reads) in one word: clamp
+// selections (bits 0-5), pass signs + 1 (6-11), positive diffs (12-20),
+// redistributions that kept redfac (21-29)
+DGR_HD unsigned lim_dec_pack(const LimFieldDec &d)
+{
+    unsigned p = 0;
+    for (int k = 0; k < 3; ++k) p |= (unsigned)d.clampsel[k] << (2 * k);
+    for (int ll = 0; ll < 3; ++ll) p |= (unsigned)(d.sd[ll] + 1) << (6 + 2 * ll);
+    return p | (d.pos << 12) | (d.use0 << 21);
+}
+
+DGR_HD LimFieldDec lim_dec_unpack(unsigned p)
+{
+    LimFieldDec d;
+    for (int k = 0; k < 3; ++k) d.clampsel[k] = (int)((p >> (2 * k)) & 3u);
+    for (int ll = 0; ll < 3; ++ll) d.sd[ll] = (int)((p >> (6 + 2 * ll)) & 3u) - 1;
+    d.pos = (p >> 12) & 0x1ffu;
+    d.use0 = (p >> 21) & 0x1ffu;
+    for (int k = 0; k < 3; ++k) d.Vfin[k] = 0.0;
+    return d;
+}
+
 // reverse of one field's pass given its decisions: Sb the adjoint of the
 // limited midpoints; accumulates Sin_b and the bound adjoints
 DGR_HD void limit_field_reverse(const LimFieldDec &d, int iv, const double Sb[9], double Sin_b[9], double Vminb[3],

@@ -61,7 +61,10 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
     }
     // limit_adj field by field: the zeta pass decides modify, then each
     // field's bounds are loaded, its pass replayed and reversed, and its
-    // adjoints stored, so one field's data is live at a time
+    // adjoints stored, so one field's data is live at a time.  A pass that
+    // has already run keeps its decisions, packed in one word (dec), and is
+    // not replayed: the zeta pass always, the momentum passes where PDop
+    // acted
     double S[9];
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = U[f * P + e];
@@ -72,9 +75,11 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
     // field loop rather than kept live through the PDop reverse)
     int modify = 0;
     double lim[9];
+    unsigned dec[3] = {0u, 0u, 0u};
     {
         LimFieldDec d;
         limit_field_forward(S, 0, Vmin, Vmax, d);
+        dec[0] = lim_dec_pack(d);
         const double w1 = S[0], w2 = S[1], w3 = S[2];
         const double Ha = (w1 - w2 + w3) + zv[0], Hb_ = (w1 + w2 - w3) + zv[1], Hc = (-w1 + w2 + w3) + zv[2];
         if (Ha > ph.h0 && Hb_ > ph.h0 && Hc > ph.h0 && Vmin[0] + zv[0] > ph.h0 && Vmin[1] + zv[1] > ph.h0 &&
@@ -111,6 +116,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
                         mx[k] = nodemm[(3 + iv) * nn + nd[k]];
                     }
                     limit_field_forward(S, iv, mn, mx, dq);
+                    dec[iv] = lim_dec_pack(dq);
                     lim[3 * iv] = (dq.Vfin[0] + dq.Vfin[1]) / 2;
                     lim[3 * iv + 1] = (dq.Vfin[1] + dq.Vfin[2]) / 2;
                     lim[3 * iv + 2] = (dq.Vfin[2] + dq.Vfin[0]) / 2;
@@ -141,15 +147,17 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
 #pragma unroll
             for (int k = 0; k < 3; ++k) ub[3 * iv + k] += sb[3 * iv + k];
         } else {
-            if (iv > 0) {
+            LimFieldDec d;
+            if (iv == 0 || pdop) {
+                d = lim_dec_unpack(dec[iv]);
+            } else {
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
                     Vmin[k] = nodemm[iv * nn + nd[k]];
                     Vmax[k] = nodemm[(3 + iv) * nn + nd[k]];
                 }
+                limit_field_forward(S, iv, Vmin, Vmax, d);
             }
-            LimFieldDec d;
-            limit_field_forward(S, iv, Vmin, Vmax, d);
             limit_field_reverse(d, iv, sb, ub, Vminb, Vmaxb);
         }
 #pragma unroll

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <cstring>
 #include <cstdio>
 #include <cstdlib>
 #include <random>
@@ -309,6 +310,44 @@ static int check_limit(int trials)
     return report("limit_adj", worst, 1e-5);
 }
 
+// the packed form of a field pass's decisions (lim_dec_pack) gives back
+// every decision, and the reverse from the unpacked form equals, bit for
+// bit, the reverse from the pass's own record; random passes with clipped
+// and unclipped vertices, both signs of the redistribution and ties
+static int check_lim_dec(int trials)
+{
+    int bad = 0, clipped = 0;
+    for (int t = 0; t < trials; ++t) {
+        double S[9], Vmin[3], Vmax[3], Sb[9];
+        const int iv = t % 3;
+        const double base = U(-1, 1);
+        for (double &v : S) v = base + U(-0.3, 0.3);
+        for (int k = 0; k < 3; ++k) {
+            Vmin[k] = base + U(-0.35, -0.02);
+            Vmax[k] = base + U(0.02, 0.35);
+        }
+        if (t % 7 == 0) S[3 * iv] = S[3 * iv + 1] = S[3 * iv + 2] = base;   // sumdiff 0 at once
+        for (double &v : Sb) v = U(-1, 1);
+        LimFieldDec d;
+        limit_field_forward(S, iv, Vmin, Vmax, d);
+        const LimFieldDec u = lim_dec_unpack(lim_dec_pack(d));
+        bool same = d.pos == u.pos && d.use0 == u.use0;
+        for (int k = 0; k < 3; ++k) {
+            same = same && d.clampsel[k] == u.clampsel[k] && d.sd[k] == u.sd[k];
+            clipped += d.clampsel[k] != 0;
+        }
+        double a[9] = {0}, b[9] = {0}, amn[3] = {0}, amx[3] = {0}, bmn[3] = {0}, bmx[3] = {0};
+        limit_field_reverse(d, iv, Sb, a, amn, amx);
+        limit_field_reverse(u, iv, Sb, b, bmn, bmx);
+        for (int i = 0; i < 9; ++i) same = same && std::memcmp(&a[i], &b[i], 8) == 0;
+        for (int k = 0; k < 3; ++k)
+            same = same && std::memcmp(&amn[k], &bmn[k], 8) == 0 && std::memcmp(&amx[k], &bmx[k], 8) == 0;
+        bad += !same;
+    }
+    printf("lim_dec_pack: %d of %d passes differ after packing (%d clipped vertices)\n", bad, trials, clipped);
+    return bad ? 1 : 0;
+}
+
 static int check_pdop(int trials)
 {
     const double H0 = 1e-3, VZ = 1e-2;
@@ -379,6 +418,7 @@ int main()
     bad += check_roe(2000);
     bad += check_rhs(500);
     bad += check_limit(2000);
+    bad += check_lim_dec(20000);
     bad += check_pdop(4000);
     return bad ? 1 : 0;
 }
