@@ -301,13 +301,21 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
             }
         }
         // geometry enters derivatives only: reciprocal products.  Evaluated
-        // once for the surface part and again, from vertex data loaded anew
-        // (the compiler cannot reuse the first evaluation), for the volume
-        // part: only one part's geometry is live at a time.
+        // once for the surface part and again for the volume part, so only
+        // one part's geometry is live at a time; the vertex data the second
+        // evaluation and the nodal adjoints need comes back from this
+        // thread's own LDS slots, not from three more memory gathers
+        __shared__ double4 vtx[3][256];
         {
+            double4 Pv[3];
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                Pv[k] = m.nodes[m.ev[k * P + e]];
+                vtx[k][threadIdx.x] = Pv[k];
+            }
             GeoRegs G;
             DivRecip D;
-            geometry_of(m, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
+            geometry_of_vertices(m, e, Pv, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
             double Fb[9];
 #pragma unroll
             for (int f = 0; f < 9; ++f) Fb[f] = 0.0;
@@ -315,11 +323,15 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
 #pragma unroll
             for (int f = 0; f < 9; ++f) Fhb[f * P + e] = Fb[f];
         }
-        MeshDev m2 = m;
-        asm volatile("" : "+s"(m2.nodes), "+s"(m2.ev), "+s"(m2.info));
+        // (the compiler may not carry the first evaluation's vertex values
+        // across: they come back from LDS)
+        asm volatile("" ::: "memory");
+        double4 Pw[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) Pw[k] = vtx[k][threadIdx.x];
         GeoRegs G;
         DivRecip D;
-        geometry_of(m2, e, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
+        geometry_of_vertices(m, e, Pw, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
         double Wb[9], mzb[3] = {0, 0, 0}, gnnb[3] = {0, 0, 0}, bgxb = 0.0, bgyb = 0.0;
 #pragma unroll
         for (int f = 0; f < 9; ++f) Wb[f] = 0.0;
@@ -338,9 +350,10 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
         const double cbed = ph.g * G(GX_AJ) * (1.0 / 6);
 #pragma unroll
         for (int k = 0; k < 3; ++k) mzb[k] += cbed * (G(GX_GP + 2 * k) * bgxb + G(GX_GP + 2 * k + 1) * bgyb);
+        asm volatile("" ::: "memory");
         double4 Pv[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) Pv[k] = m.nodes[m.ev[k * P + e]];
+        for (int k = 0; k < 3; ++k) Pv[k] = vtx[k][threadIdx.x];
         double zb_[3] = {0, 0, 0}, nb_[3] = {0, 0, 0};
 #pragma unroll
         for (int k = 0; k < 3; ++k) {

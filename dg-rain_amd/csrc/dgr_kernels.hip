@@ -179,12 +179,10 @@ __device__ __forceinline__ void rt0(double M11, double M12, double M13, double M
 // expressions, handed to put(GeoIdx, value).
 // ---------------------------------------------------------------------------
 template <typename Put, class DV>
-__device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, Put &&put, DV &D)
+__device__ __forceinline__ void geometry_of_vertices(const MeshDev &m, int e, const double4 P[3], double g, Put &&put,
+                                                     DV &D)
 {
     const int Pt = m.pitch;
-    double4 P[3];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) P[k] = m.nodes[m.ev[k * Pt + e]];
     const double xv1 = P[0].x, xv2 = P[1].x, xv3 = P[2].x;                  // 2DInnerProducts.c:22-27
     const double yv1 = P[0].y, yv2 = P[1].y, yv3 = P[2].y;
     const double x1 = (xv1 + xv2) / 2, x2 = (xv2 + xv3) / 2, x3 = (xv3 + xv1) / 2;   // :30-35
@@ -277,6 +275,15 @@ __device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, P
         put(GX_EZB + k, (A.z + B.z) / 2);
         put(GX_EDN + k, D.q(1.0, D.den(nx * ty - ny * tx)));
     }
+}
+
+template <typename Put, class DV>
+__device__ __forceinline__ void geometry_of(const MeshDev &m, int e, double g, Put &&put, DV &D)
+{
+    double4 P[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) P[k] = m.nodes[m.ev[k * m.pitch + e]];
+    geometry_of_vertices(m, e, P, g, put, D);
 }
 
 template <typename Put>
