@@ -1192,9 +1192,17 @@ __device__ __forceinline__ void rhs_block(const MeshDev &m, const PhysDev &ph, c
     const Ctrl *ctrl = ctrls + L.mid;
     const int e = L.e;
     const int P = m.pitch;
+    // the stage input, also kept in this thread's LDS slots for the RK
+    // update (so neither registers through the inner products nor a second
+    // load from memory hold it)
+    __shared__ double wlds[9][256];
+    const int tl = (int)threadIdx.x;
     double W[9], Fh[9];
 #pragma unroll
-    for (int f = 0; f < 9; ++f) W[f] = win[f * P + e];
+    for (int f = 0; f < 9; ++f) {
+        W[f] = win[f * P + e];
+        wlds[f][tl] = W[f];
+    }
 #pragma unroll
     for (int f = 0; f < 9; ++f) Fh[f] = fhat[f * P + e];
     const int wd_self = wd[e];
@@ -1226,12 +1234,9 @@ __device__ __forceinline__ void rhs_block(const MeshDev &m, const PhysDev &ph, c
 #pragma unroll
         for (int f = 0; f < 9; ++f) dbg_rhs[f * P + e] = R[f];
     } else {
-        // the stage input again (a cache hit) rather than kept live through
-        // the RHS
-        const double *w3 = win;
-        asm volatile("" : "+s"(w3));
+        asm volatile("" ::: "memory");
 #pragma unroll
-        for (int f = 0; f < 9; ++f) W[f] = w3[f * P + e];
+        for (int f = 0; f < 9; ++f) W[f] = wlds[f][tl];
         rk_update<MODE>(m, e, W, R, wout, wbase, means, ctrl);
     }
 }
