@@ -571,6 +571,33 @@ def test_lake_at_rest_on_gpu(gpu, wetdry):
     assert np.max(np.abs(qxr)) <= 1e-11 and np.max(np.abs(qyr)) <= 1e-11
 
 
+def test_set_nodal_steps_like_a_fresh_handle(gpu):
+    """A bed and Manning n changed through dgr_set_nodal step exactly as a
+    handle created with them: the edge geometry (the bed at each edge's
+    midpoint, which every Roe flux reads) follows the new bed.  Coastal
+    physics: tide on prescribed edges, wet/dry on, limiter."""
+    import dataclasses
+    wl = configs.coastal(N=24, nsteps=40)
+    rng = np.random.default_rng(11)
+    mesh = wl.mesh
+    z1 = mesh.z + 0.4 * np.cos(mesh.x / 700.0) * np.sin(mesh.y / 900.0) + 0.05 * rng.standard_normal(mesh.num_nodes)
+    n1 = mesh.n * np.exp(0.3 * rng.standard_normal(mesh.num_nodes))
+    fresh = native.Solver(dataclasses.replace(mesh, z=z1, n=n1), wl.solver_params())
+    moved = native.Solver(mesh, wl.solver_params())
+    moved.set_nodal(z=z1, n=n1)
+    old = native.Solver(mesh, wl.solver_params())
+    outs = []
+    for s in (fresh, moved, old):
+        s.set_state(wl.zeta, wl.qx, wl.qy)
+        rep = s.run(wl.nsteps)
+        outs.append((s.get_state(), s.get_wd(), rep.t, rep.dt))
+        s.close()
+    (a, wa, ta, da), (b, wb, tb, db), (c, _, _, _) = outs
+    assert all(same(x, y) for x, y in zip(a, b))
+    assert np.array_equal(wa, wb) and ta == tb and da == db
+    assert not same(a[0], c[0])   # the new bed changes the run
+
+
 def test_checkpoint_restart(gpu, tmp_path):
     """A restart file written after 15 steps and read into a new handle gives
     the bits of the uninterrupted run (state, flags, t, dt)."""
