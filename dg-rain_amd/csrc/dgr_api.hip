@@ -61,7 +61,7 @@ static int local_edge(const int *ev, int v0, int v1)
 static dgr_status free_handle(dgr_handle *h)
 {
     if (!h) return DGR_OK;
-    void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_edges,
+    void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_nl_off, h->d_nl, h->d_loc_off, h->d_locw, h->d_edges,
                     h->d_einfo, h->d_egeo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
                     h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts,
                     h->d_nanparts, h->d_wdn, h->d_hydro, h->d_tape, h->d_tiemax, h->d_tiecnt,
@@ -80,6 +80,8 @@ static inline int pad256(int n) { return (n + 255) / 256 * 256; }
 // Host tables of all members, in the concatenated (padded) index space.
 struct HostTables {
     std::vector<int> ev, nb, info, eid, off, n2e, einfo, eblk_mem, dblk_mem;
+    std::vector<int> nl_off, nl, loc_off;
+    std::vector<unsigned> locw;
     std::vector<int4> edges;
     std::vector<double4> nodes;
     std::vector<double> bz, bq;
@@ -229,6 +231,40 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
         dgr_status st = build_member(&mvs[i], mem[i], i, pitch, T);
         if (st) return st;
     }
+    // per group of 64 nodes (a wave of the node kernels): the distinct
+    // elements of its CSR entries, ascending, and each entry's position in
+    // that list (three 10-bit positions per word)
+    {
+        const int ngrp = (nn_tot + 63) / 64;
+        T.nl_off.assign((size_t)ngrp + 1, 0);
+        T.loc_off.assign((size_t)nn_tot + 1, 0);
+        std::vector<int> els;
+        for (int g = 0; g < ngrp; ++g) {
+            const int a = 64 * g, b = std::min(a + 64, nn_tot);
+            els.clear();
+            for (int j = T.off[a]; j < T.off[b]; ++j) els.push_back(T.n2e[j] & N2E_EL_MASK);
+            std::sort(els.begin(), els.end());
+            els.erase(std::unique(els.begin(), els.end()), els.end());
+            T.nl_off[g] = (int)T.nl.size();
+            T.nl.insert(T.nl.end(), els.begin(), els.end());
+            for (int i = a; i < b; ++i) {
+                T.loc_off[i] = (int)T.locw.size();
+                unsigned word = 0;
+                int t = 0;
+                for (int j = T.off[i]; j < T.off[i + 1]; ++j) {
+                    const int el = T.n2e[j] & N2E_EL_MASK;
+                    const unsigned loc = (unsigned)(std::lower_bound(els.begin(), els.end(), el) - els.begin());
+                    word |= (loc & 1023u) << (10 * t);
+                    if (++t == 3) { T.locw.push_back(word); word = 0; t = 0; }
+                }
+                if (t) T.locw.push_back(word);
+            }
+        }
+        T.nl_off[ngrp] = (int)T.nl.size();
+        T.loc_off[nn_tot] = (int)T.locw.size();
+        if (T.nl.empty()) T.nl.push_back(0);
+        T.locw.push_back(0);   // padding: the node kernels read a node's second word unconditionally
+    }
 
     dgr_handle *h = new dgr_handle();   // value-initialised: all pointers null
     h->nmem = n; h->pitch = pitch; h->nn = nn_tot; h->ned_pad = ned_pad;
@@ -249,6 +285,10 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     DGR_TRY(dalloc(&h->d_eid, 3 * P));
     DGR_TRY(dalloc(&h->d_n2e_off, (size_t)nn_tot + 1));
     DGR_TRY(dalloc(&h->d_n2e, n2e_tot));
+    DGR_TRY(dalloc(&h->d_nl_off, T.nl_off.size()));
+    DGR_TRY(dalloc(&h->d_nl, T.nl.size()));
+    DGR_TRY(dalloc(&h->d_loc_off, T.loc_off.size()));
+    DGR_TRY(dalloc(&h->d_locw, T.locw.size()));
     DGR_TRY(dalloc(&h->d_edges, ned_pad));
     DGR_TRY(dalloc(&h->d_einfo, ned_pad));
     DGR_TRY(dalloc(&h->d_egeo, ned_pad));
@@ -277,6 +317,10 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     DGR_TRY(hipMemcpyAsync(h->d_eid, T.eid.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_n2e_off, T.off.data(), ((size_t)nn_tot + 1) * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_n2e, T.n2e.data(), n2e_tot * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_nl_off, T.nl_off.data(), T.nl_off.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_nl, T.nl.data(), T.nl.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_loc_off, T.loc_off.data(), T.loc_off.size() * sizeof(int), hipMemcpyHostToDevice, s));
+    DGR_TRY(hipMemcpyAsync(h->d_locw, T.locw.data(), T.locw.size() * sizeof(unsigned), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_edges, T.edges.data(), ned_pad * sizeof(int4), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_einfo, T.einfo.data(), ned_pad * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_mem, T.mdev.data(), n * sizeof(MemberDev), hipMemcpyHostToDevice, s));
@@ -311,7 +355,8 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     MeshDev &m = h->m;
     m.pitch = pitch; m.nn = nn_tot; m.ned_pad = ned_pad; m.nmem = n;
     m.nodes = h->d_nodes; m.ev = h->d_ev; m.nb = h->d_nb; m.info = h->d_info; m.eid = h->d_eid;
-    m.n2e_off = h->d_n2e_off; m.n2e = h->d_n2e; m.edges = h->d_edges; m.einfo = h->d_einfo;
+    m.n2e_off = h->d_n2e_off; m.n2e = h->d_n2e; m.edges = h->d_edges;
+    m.nl_off = h->d_nl_off; m.nl = h->d_nl; m.loc_off = h->d_loc_off; m.locw = h->d_locw; m.einfo = h->d_einfo;
     m.egeo = h->d_egeo;
     m.mem = h->d_mem; m.eblk_mem = h->d_eblk_mem; m.dblk_mem = h->d_dblk_mem;
     m.wg = 256;

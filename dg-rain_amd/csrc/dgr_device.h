@@ -21,6 +21,10 @@
 //   nodemm       double [6][nn]      node min (zeta,qx,qy) then max
 //   n2e_off/n2e  node -> element CSR in ascending element order; an entry is
 //                el | k << 30, k the node's local vertex index in el
+//   nl_off/nl    per group of 64 nodes (one wave of the node kernels): the
+//                distinct elements of the group's CSR entries, ascending
+//   loc_off/locw per node: the positions of its CSR entries in its group's
+//                list, three 10-bit positions per word, in CSR order
 //   edges        int4   [ned_pad]    {el_in, el_ex, va, vb}: EdgtoEls and el_in's
 //                                     vertices of the edge in el_in's order
 //   einfo        int    [ned_pad]    packed edge facts, see ED_* below
@@ -50,6 +54,10 @@ enum : int {
     N2E_EL_MASK = 0x3fffffff,
     N2E_K_SHIFT = 30,
 };
+
+// the node kernels stage a 64-node group's element means in LDS when the
+// group touches at most NL_CAP elements (a structured grid: 258)
+constexpr int NL_CAP = 264;
 
 // edge info word: bits 4..15 as in the element info word, plus
 enum : int {
@@ -108,6 +116,8 @@ struct MeshDev {
     const int *eid;
     const int *n2e_off;
     const int *n2e;
+    const int *nl_off, *nl, *loc_off;
+    const unsigned *locw;
     const int4 *edges;
     const int *einfo;
     const double4 *egeo;

@@ -90,6 +90,8 @@ struct dgr_handle {
     // device buffers
     double4 *d_nodes;
     int *d_ev, *d_nb, *d_info, *d_eid, *d_n2e_off, *d_n2e;
+    int *d_nl_off, *d_nl, *d_loc_off;   // node groups' element lists, entries' positions in them
+    unsigned *d_locw;
     int4 *d_edges;   // {el_in, el_ex, va, vb} per edge
     int *d_einfo;    // packed edge facts (ED_*/EI_* bits)
     double4 *d_egeo; // canonical edge geometry (MeshDev::egeo), refreshed when the nodes change

@@ -1264,21 +1264,86 @@ __global__ __launch_bounds__(256) void k_means(MeshDev m, const double *__restri
 // ARG: also the first element that attains each bound (-1: the sentinel
 // stood), which the limiter's reverse routes the bound adjoints to, as one
 // 32-byte record per node: nodearg[8 i + f], f = 0..2 min, 3..5 max, 6..7 pad.
+// With lds (a wave's region of NL_CAP x 3 doubles, and NL_CAP ints for ARG)
+// the wave first loads the means of the elements its 64 nodes touch (the
+// group's sorted list, once each) and each node then reads its entries'
+// means there through their packed list positions; the entries, their order
+// and the comparisons are those of the CSR loop.  A group whose list is
+// longer than NL_CAP, or a null lds, takes the CSR loop over memory.
 template <bool ARG>
 __device__ __forceinline__ void node_minmax_block(const MeshDev &m, const double *__restrict__ means,
-                                                  double *__restrict__ nodemm, int *__restrict__ nodearg, int i)
+                                                  double *__restrict__ nodemm, int *__restrict__ nodearg, int i,
+                                                  double *lds = nullptr)
 {
-    if (i >= m.nn) return;
+    const int P = m.pitch;
+    const int grp = i >> 6, lane = (int)threadIdx.x & 63;
+    const bool live = i < m.nn;
     double mn[3] = {99999, 99999, 99999}, mx[3] = {-99999, -99999, -99999};
     int amn[3] = {-1, -1, -1}, amx[3] = {-1, -1, -1};
-    const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
-    for (int j = b; j < en; ++j) {
-        const int el = m.n2e[j] & N2E_EL_MASK;
+    bool staged = false;
+    if (lds && grp < ((m.nn + 63) >> 6)) {          // uniform over the wave
+        const int l0 = m.nl_off[grp], cnt = m.nl_off[grp + 1] - l0;
+        if (cnt <= NL_CAP) {
+            staged = true;
+            // this node's entries and their first two position words (the
+            // word array has one word of padding at its end), issued ahead of
+            // the wave's loads
+            int b = 0, en = 0, w0 = 0;
+            unsigned wa = 0, wb = 0;
+            if (live) {
+                b = m.n2e_off[i]; en = m.n2e_off[i + 1]; w0 = m.loc_off[i];
+                wa = m.locw[w0]; wb = m.locw[w0 + 1];
+            }
+            constexpr int NQ = (NL_CAP + 63) / 64;
+            int el[NQ];
 #pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            const double a = means[f * m.pitch + el];
-            if (a < mn[f]) { mn[f] = a; if (ARG) amn[f] = el; }
-            if (a > mx[f]) { mx[f] = a; if (ARG) amx[f] = el; }
+            for (int r = 0; r < NQ; ++r) el[r] = lane + 64 * r < cnt ? m.nl[l0 + lane + 64 * r] : -1;
+            double v[NQ][3];
+#pragma unroll
+            for (int r = 0; r < NQ; ++r)
+#pragma unroll
+                for (int f = 0; f < 3; ++f) v[r][f] = el[r] >= 0 ? means[f * P + el[r]] : 0.0;
+            int *lid = reinterpret_cast<int *>(lds + 3 * NL_CAP);
+#pragma unroll
+            for (int r = 0; r < NQ; ++r) {
+                if (el[r] < 0) continue;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) lds[f * NL_CAP + lane + 64 * r] = v[r][f];
+                if (ARG) lid[lane + 64 * r] = el[r];
+            }
+            // the wave's stores complete before any lane reads them
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+            if (!live) return;
+            unsigned word = wa;
+            for (int k = 0, t = 0, wi = 1; k < en - b; ++k) {
+                if (t == 3) {
+                    word = wi == 1 ? wb : m.locw[w0 + wi];
+                    ++wi; t = 0;
+                }
+                const int loc = (int)(word & 1023u);
+                word >>= 10; ++t;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) {
+                    const double a = lds[f * NL_CAP + loc];
+                    if (a < mn[f]) { mn[f] = a; if (ARG) amn[f] = lid[loc]; }
+                    if (a > mx[f]) { mx[f] = a; if (ARG) amx[f] = lid[loc]; }
+                }
+            }
+        }
+    }
+    if (!live) return;
+    if (!staged) {
+        const int b = m.n2e_off[i], en = m.n2e_off[i + 1];
+        for (int j = b; j < en; ++j) {
+            const int el = m.n2e[j] & N2E_EL_MASK;
+#pragma unroll
+            for (int f = 0; f < 3; ++f) {
+                const double a = means[f * P + el];
+                if (a < mn[f]) { mn[f] = a; if (ARG) amn[f] = el; }
+                if (a > mx[f]) { mx[f] = a; if (ARG) amx[f] = el; }
+            }
         }
     }
     const int nn = m.nn;
@@ -1294,11 +1359,16 @@ __device__ __forceinline__ void node_minmax_block(const MeshDev &m, const double
     }
 }
 
+// LDS of the staged node kernels: one region per wave
+constexpr size_t NL_WAVE_BYTES = 3 * NL_CAP * sizeof(double) + NL_CAP * sizeof(int);
+
 template <bool ARG>
 __global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means,
                                                      double *__restrict__ nodemm, int *__restrict__ nodearg)
 {
-    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_block() * (int)blockDim.x + (int)threadIdx.x);
+    extern __shared__ double nl_lds[];
+    double *lds = nl_lds + ((int)threadIdx.x >> 6) * (NL_WAVE_BYTES / sizeof(double));
+    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_block() * (int)blockDim.x + (int)threadIdx.x, lds);
 }
 
 // PDop2D on one element (wetDry2D.c:66-256); zeta/qx/qy are the element's
@@ -1813,7 +1883,10 @@ __global__ __launch_bounds__(256) void k_node_minmax_end(MeshDev m, PhysDev ph, 
         if (b < m.nmem) step_end_member(m, ph, ctrls, kind, 1, rec_end, rec_next, ties, b);
         return;
     }
-    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_map(b - nf, (int)gridDim.x - nf) * 256 + (int)threadIdx.x);
+    extern __shared__ double nl_lds[];
+    double *lds = nl_lds + ((int)threadIdx.x >> 6) * (NL_WAVE_BYTES / sizeof(double));
+    node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_map(b - nf, (int)gridDim.x - nf) * 256 + (int)threadIdx.x,
+                           lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -1956,8 +2029,9 @@ void launch_edge_geometry(const MeshDev &m, double4 *egeo, hipStream_t s)
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s)
 {
     const dim3 g((unsigned)((m.nn + m.wg - 1) / m.wg)), b((unsigned)m.wg);
-    if (nodearg) hipLaunchKernelGGL(k_node_minmax<true>, g, b, 0, s, m, means, nodemm, nodearg);
-    else hipLaunchKernelGGL(k_node_minmax<false>, g, b, 0, s, m, means, nodemm, nodearg);
+    const size_t lds = (size_t)(m.wg / 64) * NL_WAVE_BYTES;
+    if (nodearg) hipLaunchKernelGGL(k_node_minmax<true>, g, b, lds, s, m, means, nodemm, nodearg);
+    else hipLaunchKernelGGL(k_node_minmax<false>, g, b, lds, s, m, means, nodemm, nodearg);
 }
 
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
@@ -2018,11 +2092,12 @@ void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *m
 {
     const int nf = (m.nmem + 7) & ~7;
     const dim3 g((unsigned)(nf + (m.nn + 255) / 256)), b(256);
+    const size_t lds = 4 * NL_WAVE_BYTES;
     if (nodearg)
-        hipLaunchKernelGGL(k_node_minmax_end<true>, g, b, 0, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
+        hipLaunchKernelGGL(k_node_minmax_end<true>, g, b, lds, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
                            rec_next, ties ? 1 : 0);
     else
-        hipLaunchKernelGGL(k_node_minmax_end<false>, g, b, 0, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
+        hipLaunchKernelGGL(k_node_minmax_end<false>, g, b, lds, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
                            rec_next, ties ? 1 : 0);
 }
 
