@@ -178,6 +178,7 @@ extern "C" dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, 
     }
     DGR_HIP(hipMemcpyAsync(h->d_nodes + M.nd_off, nodes.data(), M.nn * sizeof(double4), hipMemcpyHostToDevice,
                            h->stream));
+    if (z) DGR_HIP(hipMemcpyAsync(h->d_zn + M.nd_off, z, M.nn * sizeof(double), hipMemcpyHostToDevice, h->stream));
     if (z) launch_edge_geometry(h->m, h->d_egeo, h->stream);   // z_edge follows the bed
     DGR_HIP(hipGetLastError());
     DGR_HIP(hipStreamSynchronize(h->stream));

@@ -48,7 +48,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
     int nd[3];
     double zv[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.nodes[nd[k]].z; }
+    for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.zn[nd[k]]; }
     double zadd[3] = {0, 0, 0};   // this element's additions to zvb
     if (zbe) {
         // z_edge = (z_a + z_b)/2 of local edge k
@@ -732,7 +732,7 @@ __global__ void k_gauge_sample(MeshDev m, int ngauge, const int *__restrict__ ga
     double h = 0.0;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double za = m.nodes[m.ev[k * P + e]].z, zb = m.nodes[m.ev[((k + 1) % 3) * P + e]].z;
+        const double za = m.zn[m.ev[k * P + e]], zb = m.zn[m.ev[((k + 1) % 3) * P + e]];
         h += w[k * P + e] + 0.5 * (za + zb);
     }
     hval[(t / ngauge) * mstride + t % ngauge] = h / 3.0;

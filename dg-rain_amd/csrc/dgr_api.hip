@@ -61,7 +61,7 @@ static int local_edge(const int *ev, int v0, int v1)
 static dgr_status free_handle(dgr_handle *h)
 {
     if (!h) return DGR_OK;
-    void *ptrs[] = {h->d_nodes, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_nl_off, h->d_nl, h->d_loc_off, h->d_locw, h->d_edges,
+    void *ptrs[] = {h->d_nodes, h->d_zn, h->d_ev, h->d_nb, h->d_info, h->d_eid, h->d_n2e_off, h->d_n2e, h->d_nl_off, h->d_nl, h->d_loc_off, h->d_locw, h->d_edges,
                     h->d_einfo, h->d_egeo, h->d_mem, h->d_eblk_mem, h->d_dblk_mem, h->d_bzeta, h->d_bqn, h->d_w, h->d_w1,
                     h->d_means, h->d_nodemm, h->d_dbg, h->d_fhat, h->d_wd, h->d_modify, h->d_ctrl, h->d_lamparts,
                     h->d_nanparts, h->d_wdn, h->d_hydro, h->d_tape, h->d_tiemax, h->d_tiecnt,
@@ -279,6 +279,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     hipStream_t s = h->stream;
     const int maxparts = (pitch > ned_pad ? pitch : ned_pad) / 256;
     DGR_TRY(dalloc(&h->d_nodes, nn_tot));
+    DGR_TRY(dalloc(&h->d_zn, nn_tot));
     DGR_TRY(dalloc(&h->d_ev, 3 * P));
     DGR_TRY(dalloc(&h->d_nb, 3 * P));
     DGR_TRY(dalloc(&h->d_info, 3 * P));
@@ -311,6 +312,9 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
     DGR_TRY(dalloc(&h->d_tiecnt, (size_t)maxparts));
     DGR_TRY(dalloc(&h->d_wdn, P));
     DGR_TRY(hipMemcpyAsync(h->d_nodes, T.nodes.data(), nn_tot * sizeof(double4), hipMemcpyHostToDevice, s));
+    std::vector<double> zn(nn_tot);
+    for (int i = 0; i < nn_tot; ++i) zn[i] = T.nodes[i].z;
+    DGR_TRY(hipMemcpyAsync(h->d_zn, zn.data(), nn_tot * sizeof(double), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_ev, T.ev.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_nb, T.nb.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
     DGR_TRY(hipMemcpyAsync(h->d_info, T.info.data(), 3 * P * sizeof(int), hipMemcpyHostToDevice, s));
@@ -354,7 +358,7 @@ static dgr_status create_members(int n, const dgr_mesh_view *mvs, const dgr_para
 #undef DGR_TRY
     MeshDev &m = h->m;
     m.pitch = pitch; m.nn = nn_tot; m.ned_pad = ned_pad; m.nmem = n;
-    m.nodes = h->d_nodes; m.ev = h->d_ev; m.nb = h->d_nb; m.info = h->d_info; m.eid = h->d_eid;
+    m.nodes = h->d_nodes; m.zn = h->d_zn; m.ev = h->d_ev; m.nb = h->d_nb; m.info = h->d_info; m.eid = h->d_eid;
     m.n2e_off = h->d_n2e_off; m.n2e = h->d_n2e; m.edges = h->d_edges;
     m.nl_off = h->d_nl_off; m.nl = h->d_nl; m.loc_off = h->d_loc_off; m.locw = h->d_locw; m.einfo = h->d_einfo;
     m.egeo = h->d_egeo;

@@ -110,6 +110,7 @@ struct MemberDev {
 struct MeshDev {
     int pitch, nn, ned_pad, nmem;
     const double4 *nodes;
+    const double *zn;   // nodes[i].z as a plane, for kernels that need only the bed
     const int *ev;
     const int *nb;
     const int *info;

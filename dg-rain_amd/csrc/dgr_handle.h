@@ -89,6 +89,7 @@ struct dgr_handle {
     int record_modify;
     // device buffers
     double4 *d_nodes;
+    double *d_zn;   // per-node bed, a copy of d_nodes[i].z
     int *d_ev, *d_nb, *d_info, *d_eid, *d_n2e_off, *d_n2e;
     int *d_nl_off, *d_nl, *d_loc_off;   // node groups' element lists, entries' positions in them
     unsigned *d_locw;

@@ -1496,11 +1496,11 @@ __device__ __forceinline__ void limit_block(const MeshDev &m, const PhysDev &ph,
         double zv[3], S[9], Vmin[3][3], Vmax[3][3];
         int old;
     };
-    auto load = [&](const int *ev, const double4 *nodes, const double *w, const double *nm, const int *wdi) {
+    auto load = [&](const int *ev, const double *zn, const double *w, const double *nm, const int *wdi) {
         LimIn in;
         int nd[3];
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { nd[k] = ev[k * P + e]; in.zv[k] = nodes[nd[k]].z; }
+        for (int k = 0; k < 3; ++k) { nd[k] = ev[k * P + e]; in.zv[k] = zn[nd[k]]; }
 #pragma unroll
         for (int f = 0; f < 9; ++f) in.S[f] = w[f * P + e];
 #pragma unroll
@@ -1575,16 +1575,16 @@ __device__ __forceinline__ void limit_block(const MeshDev &m, const PhysDev &ph,
     };
     {
         DivFast D;
-        eval(load(m.ev, m.nodes, win, nodemm, wd_in), D);
+        eval(load(m.ev, m.zn, win, nodemm, wd_in), D);
         if (!D.ok()) {
             // an operand outside the fast division's range: plain divisions on
             // inputs loaded again (nothing of the fast pass stays live for it)
             const int *ev2 = m.ev, *wd2 = wd_in;
-            const double4 *nodes2 = m.nodes;
+            const double *zn2 = m.zn;
             const double *win2 = win, *nm2 = nodemm;
-            asm volatile("" : "+s"(ev2), "+s"(nodes2), "+s"(win2), "+s"(nm2), "+s"(wd2));
+            asm volatile("" : "+s"(ev2), "+s"(zn2), "+s"(win2), "+s"(nm2), "+s"(wd2));
             DivExact X;
-            eval(load(ev2, nodes2, win2, nm2, wd2), X);
+            eval(load(ev2, zn2, win2, nm2, wd2), X);
         }
     }
     if (err) raise_error(ctrls + L.mid, err, e - m.mem[L.mid].el_off, 2 * stage + 1, step_adj);
@@ -1613,7 +1613,7 @@ __global__ __launch_bounds__(256) void k_pdop(MeshDev m, PhysDev ph, double *w, 
     const int P = m.pitch;
     double zv[3], S[9];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) zv[k] = m.nodes[m.ev[k * P + e]].z;
+    for (int k = 0; k < 3; ++k) zv[k] = m.zn[m.ev[k * P + e]];
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = w[f * P + e];
     DivExact D;
@@ -1633,7 +1633,7 @@ __global__ __launch_bounds__(256) void k_wd_status(MeshDev m, PhysDev ph, const 
     const int P = m.pitch;
     double zz[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) zz[k] = m.nodes[m.ev[k * P + e]].z;
+    for (int k = 0; k < 3; ++k) zz[k] = m.zn[m.ev[k * P + e]];
     DivExact D;
     wd_out[e] = wd_status_of(w[e], w[P + e], w[2 * P + e], zz, ph.h0, wd_in[e], D);
 }
