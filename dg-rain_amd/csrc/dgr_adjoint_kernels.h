@@ -68,9 +68,16 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
     double S[9];
 #pragma unroll
     for (int f = 0; f < 9; ++f) S[f] = U[f * P + e];
+    // every field's bounds, loaded together with the zeta bounds (one round
+    // of gathers instead of one per field)
+    double Bmin[3][3], Bmax[3][3];
+#pragma unroll
+    for (int iv = 0; iv < 3; ++iv)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { Bmin[iv][k] = nodemm[iv * nn + nd[k]]; Bmax[iv][k] = nodemm[(3 + iv) * nn + nd[k]]; }
     double Vmin[3], Vmax[3];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) { Vmin[k] = nodemm[nd[k]]; Vmax[k] = nodemm[3 * nn + nd[k]]; }
+    for (int k = 0; k < 3; ++k) { Vmin[k] = Bmin[0][k]; Vmax[k] = Bmax[0][k]; }
     // the zeta pass decides modify (its decisions are replayed again in the
     // field loop rather than kept live through the PDop reverse)
     int modify = 0;
@@ -109,13 +116,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
             for (int iv = 1; iv < 3; ++iv) {
                 if (modify) {
                     LimFieldDec dq;
-                    double mn[3], mx[3];
-#pragma unroll
-                    for (int k = 0; k < 3; ++k) {
-                        mn[k] = nodemm[iv * nn + nd[k]];
-                        mx[k] = nodemm[(3 + iv) * nn + nd[k]];
-                    }
-                    limit_field_forward(S, iv, mn, mx, dq);
+                    limit_field_forward(S, iv, Bmin[iv], Bmax[iv], dq);
                     dec[iv] = lim_dec_pack(dq);
                     lim[3 * iv] = (dq.Vfin[0] + dq.Vfin[1]) / 2;
                     lim[3 * iv + 1] = (dq.Vfin[1] + dq.Vfin[2]) / 2;
@@ -151,12 +152,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
             if (iv == 0 || pdop) {
                 d = lim_dec_unpack(dec[iv]);
             } else {
-#pragma unroll
-                for (int k = 0; k < 3; ++k) {
-                    Vmin[k] = nodemm[iv * nn + nd[k]];
-                    Vmax[k] = nodemm[(3 + iv) * nn + nd[k]];
-                }
-                limit_field_forward(S, iv, Vmin, Vmax, d);
+                limit_field_forward(S, iv, Bmin[iv], Bmax[iv], d);
             }
             limit_field_reverse(d, iv, sb, ub, Vminb, Vmaxb);
         }
