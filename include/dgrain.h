@@ -362,7 +362,9 @@ dgr_status dgr_gauge_run(dgr_handle *h, int nsteps, int sample_every, int ngauge
 #define DGR_NOALLOC_SCRATCH_BYTES (8u << 20)
 dgr_status dgr_gauge_run_noalloc(dgr_handle *h, int nsteps, int sample_every, int ngauges, const int *gauge_el,
                                  double *h_out);
-/* Replace a member's nodal depth z and/or Manning n (NULL keeps it). */
+/* Replace a member's nodal depth z and/or Manning n (NULL keeps it).  The
+ * handle's derived bed data (edge geometry, per-node depth plane) follow, so
+ * the next steps are those of a handle created on the new bed. */
 dgr_status dgr_set_nodal(dgr_handle *h, int member, const double *z, const double *n);
 
 /* Restart files (extension): one member's state, wet/dry flags, time and dt.
