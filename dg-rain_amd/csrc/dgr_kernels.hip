@@ -1724,24 +1724,54 @@ __device__ __forceinline__ unsigned long long member_lam(const MeshDev &m, const
 // fold starts at 0; a NaN operand restarts it, so the result is the max of
 // the operands after the last NaN, or NaN if none follows.  One workgroup of
 // 256 threads; the result is valid in every thread.
-__device__ __forceinline__ double member_fold_edges(const MeshDev &m, const PhysDev &ph, int mid, int slot)
+// With tie != nullptr (a taped step of the edge kernel) the same pass also
+// reduces the tie partials (tie_commit): the largest non-NaN operand A and
+// how many operands equal it, as (max, count) pairs (a larger max takes its
+// count, an equal one adds it; exact, so any order gives the same pair);
+// *tie gets the count when A is the fold's result, else -1 (a NaN operand
+// restarted the fold).
+__device__ __forceinline__ double member_fold_edges(const MeshDev &m, const PhysDev &ph, int mid, int slot,
+                                                    int *tie = nullptr)
 {
-    __shared__ int rp[4];
-    __shared__ double rm[4];
+    __shared__ int rp[4], rc[4];
+    __shared__ double rm[4], ra[4];
     const MemberDev &M = m.mem[mid];
     const int *np = ph.nanparts + (size_t)slot * ph.maxparts + M.dblk0;
     const unsigned long long *lp = ph.lamparts + (size_t)slot * ph.maxparts + M.dblk0;
+    const unsigned long long *tp = ph.tiemax + M.dblk0;
+    const int *tn = ph.tiecnt + M.dblk0;
     // one pass for P and the max of all partials (the answer when P = -1),
     // a second one only when the member saw a NaN operand
-    int P = -1;
-    double R = -1.0;
+    int P = -1, C = 0;
+    double R = -1.0, A = -1.0;
     // unrolled so that several partials' loads are in flight at once (one
     // workgroup folds every partial of the member)
+    if (tie) {
 #pragma unroll 8
-    for (int b = threadIdx.x; b < M.dblks; b += 256) {
-        const double v = __longlong_as_double((long long)lp[b]);
-        P = (np[b] > P) ? np[b] : P;
-        R = (v > R) ? v : R;
+        for (int b = threadIdx.x; b < M.dblks; b += 256) {
+            const double v = __longlong_as_double((long long)lp[b]);
+            P = (np[b] > P) ? np[b] : P;
+            R = (v > R) ? v : R;
+            const double a = __longlong_as_double((long long)tp[b]);
+            const int c = tn[b];
+            C = (a > A) ? c : (a == A) ? C + c : C;
+            A = (a > A) ? a : A;
+        }
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) {
+            const double a = __shfl_xor(A, o, 64);
+            const int c = __shfl_xor(C, o, 64);
+            C = (a > A) ? c : (a == A) ? C + c : C;
+            A = (a > A) ? a : A;
+        }
+        if ((threadIdx.x & 63) == 0) { ra[threadIdx.x >> 6] = A; rc[threadIdx.x >> 6] = C; }
+    } else {
+#pragma unroll 8
+        for (int b = threadIdx.x; b < M.dblks; b += 256) {
+            const double v = __longlong_as_double((long long)lp[b]);
+            P = (np[b] > P) ? np[b] : P;
+            R = (v > R) ? v : R;
+        }
     }
     P = wave_max(P);
     R = wave_max(R);
@@ -1751,7 +1781,20 @@ __device__ __forceinline__ double member_fold_edges(const MeshDev &m, const Phys
     R = rm[0];
 #pragma unroll
     for (int i = 1; i < 4; ++i) { P = (rp[i] > P) ? rp[i] : P; R = (rm[i] > R) ? rm[i] : R; }
-    if (P < 0) return (R > 0.0) ? R : 0.0;
+    if (tie) {
+        A = ra[0];
+        C = rc[0];
+#pragma unroll
+        for (int i = 1; i < 4; ++i) {
+            C = (ra[i] > A) ? rc[i] : (ra[i] == A) ? C + rc[i] : C;
+            A = (ra[i] > A) ? ra[i] : A;
+        }
+    }
+    if (P < 0) {
+        const double ml = (R > 0.0) ? R : 0.0;
+        if (tie) *tie = (A == ml) ? C : -1;
+        return ml;
+    }
     __syncthreads();
     R = -1.0;
 #pragma unroll 8
@@ -1765,16 +1808,18 @@ __device__ __forceinline__ double member_fold_edges(const MeshDev &m, const Phys
     R = rm[0];
 #pragma unroll
     for (int i = 1; i < 4; ++i) R = (rm[i] > R) ? rm[i] : R;
-    return (R < 0.0) ? __longlong_as_double(0x7ff8000000000000LL) : R;
+    const double ml = (R < 0.0) ? __longlong_as_double(0x7ff8000000000000LL) : R;
+    if (tie) *tie = (A == ml) ? C : -1;
+    return ml;
 }
 
 // the fold of one member and stage from the last flux launch's partials,
 // stored resolved in Ctrl::maxlam[slot] (thread 0)
 __device__ __forceinline__ double resolve_lam(const MeshDev &m, const PhysDev &ph, Ctrl *c, int mid, int slot,
-                                              int kind)
+                                              int kind, int *tie = nullptr)
 {
     if (kind == PARTS_EDGE) {
-        const double ml = member_fold_edges(m, ph, mid, slot);
+        const double ml = member_fold_edges(m, ph, mid, slot, tie);
         if (threadIdx.x == 0) {
             c->maxlam[slot] = (unsigned long long)__double_as_longlong(ml);
             c->nan_pos[slot] = -1;
@@ -1833,9 +1878,11 @@ __device__ __forceinline__ void step_end_member(const MeshDev &m, const PhysDev 
                                                 StepRec *rec_end, StepRec *rec_next, int ties, int mid)
 {
     Ctrl *c = ctrls + mid;
-    const double ml = resolve_lam(m, ph, c, mid, 1, kind);
+    // taped steps of the edge kernel count the max's ties in the fold's pass
     int ntie = 0;
-    if (ties) ntie = member_ties(m, ph, mid, ml);
+    const bool fused = ties && kind == PARTS_EDGE;
+    const double ml = resolve_lam(m, ph, c, mid, 1, kind, fused ? &ntie : nullptr);
+    if (ties && !fused) ntie = member_ties(m, ph, mid, ml);
     if (threadIdx.x != 0) return;
     if (ties && rec_end) rec_end[mid].ntie = ntie;
     const MemberDev &M = m.mem[mid];
