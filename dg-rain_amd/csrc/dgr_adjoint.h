@@ -18,6 +18,7 @@
 #include <math.h>
 
 #include "dgr_device.h"
+#include "dgr_math.h"
 
 namespace dgr {
 
@@ -92,7 +93,7 @@ DGR_HD void roe_adj_impl(const RoeIn &a, double nx, double ny, double lg, double
     const double den = -2 * c;
     const double num[3] = {-c - nx * uh - ny * vh, -2 * c * ny * uh + 2 * c * nx * vh, -c + nx * uh + ny * vh};
     // Q only weighs derivatives: reciprocal products
-    const double rden = 1.0 / den;
+    const double rden = recip_d(den);
     const double Q[3][3] = {{num[0] * rden, nx * rden, ny * rden},
                             {num[1] * rden, 2 * c * ny * rden, -2 * c * nx * rden},
                             {num[2] * rden, -nx * rden, -ny * rden}};
@@ -184,8 +185,8 @@ DGR_HD void roe_adj_impl(const RoeIn &a, double nx, double ny, double lg, double
     cb += -lhb[0] + lhb[2];
     // the reverse sweep multiplies by reciprocals (the forward values above
     // keep the reference's divisions: they decide the active branches)
-    Hib += sgib * (0.5 * g / sgi);
-    Heb += sgeb * (0.5 * g / sge);
+    Hib += sgib * ((0.5 * g) * recip_d(sgi));
+    Heb += sgeb * ((0.5 * g) * recip_d(sge));
     // Q = num / den and the constant-numerator entries
     double denb = 0;
     for (int r = 0; r < 3; ++r)
@@ -204,11 +205,11 @@ DGR_HD void roe_adj_impl(const RoeIn &a, double nx, double ny, double lg, double
     cb += (2 * ny * Qb[1][1] - 2 * nx * Qb[1][2]) * rden;
     cb += -2 * denb;
     // c = sqrt(g Hh)
-    const double Hhb = cb * (0.5 * g / c);
+    const double Hhb = cb * ((0.5 * g) * recip_d(c));
     Hib += 0.5 * Hhb;
     Heb += 0.5 * Hhb;
     // Roe averages
-    const double rS = 1.0 / S;
+    const double rS = recip_d(S);
     const double numub = uhb * rS, numvb = vhb * rS;
     const double Sb = -(uh * uhb + vh * vhb) * rS;
     uib += sHi * numub;
@@ -216,8 +217,8 @@ DGR_HD void roe_adj_impl(const RoeIn &a, double nx, double ny, double lg, double
     vib += sHi * numvb;
     veb += sHe * numvb;
     const double sHib = ui * numub + vi * numvb + Sb, sHeb = ue * numub + ve * numvb + Sb;
-    Hib += sHib * (0.5 / sHi);
-    Heb += sHeb * (0.5 / sHe);
+    Hib += sHib * (0.5 * recip_d(sHi));
+    Heb += sHeb * (0.5 * recip_d(sHe));
     // physical normal fluxes
     {
         const double f1x = nx * fnb[0], f1y = ny * fnb[0], f2x = nx * fnb[1], f2y = ny * fnb[1];
@@ -234,7 +235,7 @@ DGR_HD void roe_adj_impl(const RoeIn &a, double nx, double ny, double lg, double
         zbb += -lg * a.zb * (f2x + f3y);
     }
     // velocities
-    const double rHi = 1.0 / Hi, rHe = 1.0 / He;
+    const double rHi = recip_d(Hi), rHe = recip_d(He);
     ib.qxi += uib * rHi;
     ib.qyi += vib * rHi;
     Hib += -(ui * uib + vi * vib) * rHi;
@@ -280,7 +281,7 @@ DGR_HD Rt0 rt0_map(double M11, double M12, double M13, double M21, double M22, d
                    double M33, double den)
 {
     Rt0 m;
-    const double r = 1.0 / den;
+    const double r = recip_d(den);
     m.A[0] = (M23 * M32 - M22 * M33) * r;
     m.A[1] = (M12 * M33 - M13 * M32) * r;
     m.A[2] = (M13 * M22 - M12 * M23) * r;
@@ -372,10 +373,10 @@ DGR_HD void element_rhs_adj_volume(const Geo &G, const double W[9], int wd_self,
         const double mz = G(GX_MZ + k), gnn = G(GX_GNN + k);
         const double zr = W[k], qx = W[3 + k], qy = W[6 + k];
         const double H = zr + mz;
-        const double rH = 1.0 / H;
+        const double rH = recip_d(H);
         const double u = qx * rH, v = qy * rH;
         const double sp = sqrt(u * u + v * v);
-        const double rH43 = 1.0 / (H * cbrt(H));
+        const double rH43 = recip_d(H * cbrt(H));
         const double tau = gnn * sp * rH43;
         double Hb = 0, qxb = 0, qyb = 0, zrb = 0, mzb_explicit = 0;
         // Fx/Fy adjoints from the normal point fluxes

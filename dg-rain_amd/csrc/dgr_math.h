@@ -167,14 +167,30 @@ struct DivFast {
     __device__ __forceinline__ bool ok() const { return emin >= -699 && amax <= 0x1p320 && dmin >= 0x1p-320; }
 };
 
-// quotients as products with a rounded reciprocal (within a few ulps of
-// x / y): for values that only enter derivatives, never a forward result
+// reciprocal for values that only enter derivatives: the hardware estimate
+// refined by two Newton steps (within about an ulp of 1/y, not correctly
+// rounded); zero, infinite and NaN inputs give what 1/y gives.  Host builds
+// divide.
+__host__ __device__ __forceinline__ double recip_d(double y)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    const double r0 = __builtin_amdgcn_rcp(y);
+    double r = fma(r0, fma(-y, r0, 1.0), r0);
+    r = fma(r, fma(-y, r, 1.0), r);
+    return __builtin_isfinite(r) ? r : r0;
+#else
+    return 1.0 / y;
+#endif
+}
+
+// quotients as products with a reciprocal (within a few ulps of x / y): for
+// values that only enter derivatives, never a forward result
 struct DivRecip {
     struct Den {
         double r;
     };
-    __device__ __forceinline__ Den den(double y) { return Den{1.0 / y}; }
-    __device__ __forceinline__ Den den_const(double y) { return Den{1.0 / y}; }
+    __device__ __forceinline__ Den den(double y) { return Den{recip_d(y)}; }
+    __device__ __forceinline__ Den den_const(double y) { return Den{recip_d(y)}; }
     __device__ __forceinline__ double q(double x, const Den &d) { return x * d.r; }
 };
 
