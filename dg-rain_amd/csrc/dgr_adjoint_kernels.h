@@ -253,6 +253,15 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
         bool own[3][6];
 #pragma unroll
         for (int k = 0; k < 3; ++k) nd[k] = m.ev[k * P + e];
+        // the vertex records (into this thread's LDS slots) and the nodal
+        // adjoints this element adds to, in the same round of loads as the
+        // attaining-element records
+        __shared__ double4 vtx[3][256];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) vtx[k][threadIdx.x] = m.nodes[nd[k]];
+        double zv0[3], nv0[3];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) { zv0[k] = zvb[k * P + e]; nv0[k] = nvb[k * P + e]; }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int4 *na = reinterpret_cast<const int4 *>(nodearg) + 2 * nd[k];
@@ -298,17 +307,13 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
         }
         // geometry enters derivatives only: reciprocal products.  Evaluated
         // once for the surface part and again for the volume part, so only
-        // one part's geometry is live at a time; the vertex data the second
-        // evaluation and the nodal adjoints need comes back from this
-        // thread's own LDS slots, not from three more memory gathers
-        __shared__ double4 vtx[3][256];
+        // one part's geometry is live at a time; the vertex data both
+        // evaluations and the nodal adjoints need come from this thread's
+        // own LDS slots
         {
             double4 Pv[3];
 #pragma unroll
-            for (int k = 0; k < 3; ++k) {
-                Pv[k] = m.nodes[m.ev[k * P + e]];
-                vtx[k][threadIdx.x] = Pv[k];
-            }
+            for (int k = 0; k < 3; ++k) Pv[k] = vtx[k][threadIdx.x];
             GeoRegs G;
             DivRecip D;
             geometry_of_vertices(m, e, Pv, ph.g, [&](int i, double v) { G.v[i] = v; }, D);
@@ -362,7 +367,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
             nb_[k1] += ph.g * mn * gnnb[k];
         }
 #pragma unroll
-        for (int k = 0; k < 3; ++k) { zvb[k * P + e] += zb_[k]; nvb[k * P + e] += nb_[k]; }
+        for (int k = 0; k < 3; ++k) { zvb[k * P + e] = zv0[k] + zb_[k]; nvb[k * P + e] = nv0[k] + nb_[k]; }
     }
     const double s = block_sum(dtb);
     if (threadIdx.x == 0) dtparts[L.lb] = s;
