@@ -49,6 +49,10 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
     double zv[3];
 #pragma unroll
     for (int k = 0; k < 3; ++k) { nd[k] = m.ev[k * P + e]; zv[k] = m.zn[nd[k]]; }
+    // the nodal z adjoints this element adds to, loaded with its inputs
+    double zv0[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) zv0[k] = zvb[k * P + e];
     double zadd[3] = {0, 0, 0};   // this element's additions to zvb
     if (zbe) {
         // z_edge = (z_a + z_b)/2 of local edge k
@@ -136,7 +140,7 @@ __global__ __launch_bounds__(256, 3) void k_adj_limit(MeshDev m, PhysDev ph, con
     }
     if (zbe || pdop) {
 #pragma unroll
-        for (int k = 0; k < 3; ++k) zvb[k * P + e] += zadd[k];
+        for (int k = 0; k < 3; ++k) zvb[k * P + e] = zv0[k] + zadd[k];
     }
     auto sb_of = [&](int f) { return pdop ? Ub[f * P + e] : Sb[f * P + e] + (slotb ? slotb[f * P + e] : 0.0); };
 #pragma unroll
