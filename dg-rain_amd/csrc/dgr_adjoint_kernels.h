@@ -266,6 +266,13 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
         double zv0[3], nv0[3];
 #pragma unroll
         for (int k = 0; k < 3; ++k) { zv0[k] = zvb[k * P + e]; nv0[k] = nvb[k * P + e]; }
+        // MODE 1 adds to the step-start adjoint: its current value waits in
+        // this thread's LDS slots instead of being loaded at the end
+        __shared__ double bbl[MODE == 1 ? 9 : 1][256];
+        if (MODE == 1) {
+#pragma unroll
+            for (int f = 0; f < 9; ++f) bbl[f][threadIdx.x] = Bb[f * P + e];
+        }
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const int4 *na = reinterpret_cast<const int4 *>(nodearg) + 2 * nd[k];
@@ -343,7 +350,7 @@ __global__ __launch_bounds__(256, 2) void k_adj_rhs(MeshDev m, PhysDev ph, const
         element_rhs_adj_volume(G, w, wd[e], ph.g, Rb, Wb, mzb, gnnb, bgxb, bgyb);
         if (MODE == 1) {
 #pragma unroll
-            for (int f = 0; f < 9; ++f) Bb[f * P + e] += ub[f] + Wb[f];
+            for (int f = 0; f < 9; ++f) Bb[f * P + e] = bbl[f][threadIdx.x] + (ub[f] + Wb[f]);
         } else {
 #pragma unroll
             for (int f = 0; f < 9; ++f) {
