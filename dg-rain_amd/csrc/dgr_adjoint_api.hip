@@ -259,6 +259,13 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(B.alloc(&tdp, 2 * (size_t)maxparts));
     DGR_HIP(B.alloc(&gz, nn));
     DGR_HIP(B.alloc(&gn, nn));
+    AdjEdgeQueue eq{};
+    const bool split = h->ph.wetdry && h->m.ned_pad >= ADJ_EDGE_SPLIT;
+    const size_t neq = split ? (size_t)h->m.ned_pad : 256;
+    DGR_HIP(B.alloc(&eq.blist, neq));
+    DGR_HIP(B.alloc(&eq.tb, neq));
+    DGR_HIP(B.alloc(&eq.queue, neq / 256));
+    DGR_HIP(B.alloc(&eq.qcount, 2));
     // one stage's node bounds and the elements attaining them, when the
     // tapes leave them out (compact tapes, below)
     double *cnm;
@@ -434,6 +441,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(hipMemsetAsync(wb, 0, Wb, s));
     DGR_HIP(hipMemsetAsync(zvb, 0, 3 * P * sizeof(double), s));
     DGR_HIP(hipMemsetAsync(nvb, 0, 3 * P * sizeof(double), s));
+    DGR_HIP(hipMemsetAsync(eq.qcount, 0, 2 * sizeof(int), s));
     bool have_slot1 = false;
     for (int c = nseg - 1; c >= 0; --c) {
         const int n0 = c * K, n1 = (n0 + K < N) ? n0 + K : N;
@@ -492,7 +500,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
                 if (!rec_ties) launch_adj_ties(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, rn, rn, actl, s);
             }
             launch_adj_edge(h->m, h->ph, p1, h->d_bzeta, h->d_bqn, wdb, fhb, rn, rn, actl, slot2, zbe2, tdp + maxparts,
-                            lam_mode, s);
+                            lam_mode, eq, s);
             // stage 1: input w0, output u1 -> limited -> p1 (adjoint pb + slot2)
             // (the stage-2 kernels that read cna are ahead on the stream)
             if (compact) node_bounds(u1);
@@ -500,7 +508,7 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
             launch_adj_node_sums(h->m, vmb, nsum, s);
             launch_adj_rhs(1, h->m, h->ph, w0, nullptr, u1, ub, nsum, na1, wda, rn, nullptr, wb0, fhb, zvb, nvb,
                            dtp, s);
-            launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, 0, s);
+            launch_adj_edge(h->m, h->ph, w0, h->d_bzeta, h->d_bqn, wda, fhb, rn, rn, actl, slot1, zbe1, tdp, 0, eq, s);
             have_slot1 = true;
             // this step's scalar pass (dt and tide adjoints to the previous
             // step's max |lambda|): inside the next step's first launch, or

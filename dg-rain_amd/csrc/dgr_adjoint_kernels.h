@@ -497,14 +497,147 @@ __global__ __launch_bounds__(256) void k_adj_ties(MeshDev m, PhysDev ph, const d
     if (c) atomicAdd(&actl[mid].ntie, c);
 }
 
-// Roe flux reverse per edge.  W: stage input; Fhb: adjoint of the Fhat slots.
-// Writes slotb[9][pitch] (adjoint of the stage input at the two slots the
-// edge feeds; a boundary edge writes its one slot) and zbe[3][pitch] (the
-// z_edge adjoint in el_in's slot, 0 in el_ex's), and tideparts (workgroup sum
-// of the tide value's adjoint).  With lamb != 0 the edges whose |lambda|
-// operand equals the recorded max (rec_lam) share lamb.
+// Roe flux reverse of one edge.  W: stage input; Fhb: adjoint of the Fhat
+// slots.  Writes slotb[9][pitch] (adjoint of the stage input at the two
+// slots the edge feeds; a boundary edge writes its one slot) and zbe[3][pitch]
+// (the z_edge adjoint in el_in's slot, 0 in el_ex's) and returns the edge's
+// tide-value adjoint.  With lamb != 0 the edges whose |lambda| operand equals
+// the recorded max (rec_lam) share lamb.  WETDRY false: the edge's sides are
+// wet (or wet/dry is off): one Roe evaluation.
 template <bool WETDRY>
-__global__ __launch_bounds__(256, 2) void k_adj_edge(MeshDev m, PhysDev ph, const double *__restrict__ win,
+__device__ __forceinline__ double adj_edge_one(const MeshDev &m, const PhysDev &ph, const MemberDev &M, int mid,
+                                               int ed, const double *__restrict__ win,
+                                               const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                               const int *__restrict__ wd, const double *__restrict__ Fhb,
+                                               const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_lam,
+                                               const AdjCtrl *__restrict__ actl, int use_lambda,
+                                               double *__restrict__ slotb, double *__restrict__ zbe)
+{
+    double tideb = 0.0;
+    const int P = m.pitch;
+    const double g = ph.g;
+    const EdgeState s = edge_state(m, M, rec[mid], win, bzeta, bqn, ed);
+    double lam_share = 0.0;
+    const AdjCtrl &ac = actl[mid];
+    // the tie count: recorded by the taped forward (use_lambda 2) or
+    // counted by k_adj_ties (1)
+    const int ntie = (use_lambda == 2) ? rec_lam[mid].ntie : ac.ntie;
+    if (use_lambda && ac.lamb != 0.0 && ntie > 0) lam_share = ac.lamb / ntie;
+    const double target = __longlong_as_double((long long)rec_lam[mid].lam2);
+    double Fin_b[3], Fex_b[3];
+#pragma unroll
+    for (int f = 0; f < 3; ++f) {
+        Fin_b[f] = Fhb[(f * 3 + s.kin) * P + s.el_in];
+        Fex_b[f] = s.boundary ? 0.0 : -Fhb[(f * 3 + s.kex) * P + s.el_ex];
+    }
+    // adjoints of the edge-level inputs.  Up to two Roe evaluations per
+    // edge: (wet) one with g; (one side dry) the g flux and the lg = 0
+    // flux of the same states; (both dry) el_in's and el_ex's reflection
+    // fluxes.  One roe_eval/roe_adj in a loop keeps a single copy of the
+    // reverse code (and its registers) for every case.
+    double zinb = 0, qxinb = 0, qyinb = 0, zexb = 0, qxexb = 0, qyexb = 0, zbb = 0;
+    double Ar[2][2];
+    reflect_matrix(s.nx, s.ny, s.dn, Ar);
+    int mode = 0;   // 0 wet, 1 one side dry, 2 both dry
+    bool in_wet = true, ex_wet = true;
+    if (WETDRY) {
+        const int wd_in = wd[s.el_in], wd_ex = s.boundary ? wd_in : wd[s.el_ex];
+        in_wet = wd_in == 1;
+        ex_wet = s.boundary ? in_wet : (wd_ex == 1);
+        if (wd_in == 0 && wd_ex == 0) mode = 2;
+        else if (!(in_wet && ex_wet)) mode = 1;
+    }
+    RoeIn acc = {0, 0, 0, 0, 0, 0, 0};   // identity-mapped input adjoints (modes 0, 1)
+    const int nev = (mode == 0) ? 1 : 2;
+#pragma unroll 1
+    for (int j = 0; j < nev; ++j) {
+        RoeIn a;
+        double lg = g, Fb[3];
+        bool lam_op = true;
+        if (mode == 2) {
+            lg = 0.0;
+            if (j == 0) {
+                double rx, ry;
+                reflect(s.qxin, s.qyin, s.nx, s.ny, s.dn, rx, ry);
+                a = RoeIn{s.zin, s.zin, s.qxin, rx, s.qyin, ry, s.zb};
+            } else {
+                double sx, sy;
+                reflect(s.qxex, s.qyex, s.nx, s.ny, s.dn, sx, sy);
+                a = RoeIn{s.zex, s.zex, sx, s.qxex, sy, s.qyex, s.zb};
+            }
+#pragma unroll
+            for (int f = 0; f < 3; ++f) Fb[f] = (j == 0) ? Fin_b[f] : Fex_b[f];
+        } else {
+            a = RoeIn{s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
+            if (mode == 0) {
+#pragma unroll
+                for (int f = 0; f < 3; ++f) Fb[f] = Fin_b[f] + Fex_b[f];
+            } else if (j == 0) {   // the g flux feeds the wet sides
+#pragma unroll
+                for (int f = 0; f < 3; ++f) Fb[f] = (in_wet ? Fin_b[f] : 0.0) + (ex_wet ? Fex_b[f] : 0.0);
+            } else {               // the lg = 0 flux the dry sides (|lambda| counted once)
+                lg = 0.0;
+                lam_op = false;
+#pragma unroll
+                for (int f = 0; f < 3; ++f) Fb[f] = (in_wet ? 0.0 : Fin_b[f]) + (ex_wet ? 0.0 : Fex_b[f]);
+            }
+        }
+        // (|lambda| does not depend on lg: the tie test inside roe_adj_tie
+        // sees the operands the forward folded)
+        RoeIn ib = (mode == 2) ? RoeIn{0, 0, 0, 0, 0, 0, 0} : acc;
+        roe_adj_tie(a, s.nx, s.ny, lg, g, Fb, lam_op ? lam_share : 0.0, target, ib);
+        if (mode != 2) {
+            acc = ib;
+        } else if (j == 0) {
+            zinb += ib.zi + ib.ze;
+            qxinb += ib.qxi + Ar[0][0] * ib.qxe + Ar[1][0] * ib.qye;
+            qyinb += ib.qyi + Ar[0][1] * ib.qxe + Ar[1][1] * ib.qye;
+            zbb += ib.zb;
+        } else {
+            zexb += ib.zi + ib.ze;
+            qxexb += ib.qxe + Ar[0][0] * ib.qxi + Ar[1][0] * ib.qyi;
+            qyexb += ib.qye + Ar[0][1] * ib.qxi + Ar[1][1] * ib.qyi;
+            zbb += ib.zb;
+        }
+    }
+    if (mode != 2) {
+        zinb += acc.zi; zexb += acc.ze; qxinb += acc.qxi; qxexb += acc.qxe;
+        qyinb += acc.qyi; qyexb += acc.qye; zbb += acc.zb;
+    }
+    // boundary data -> slots (compute2DL.c:149-199)
+    double exz = 0, exqx = 0, exqy = 0;   // adjoint of the el_ex slot
+    if (s.prescribed) {
+        if (s.zex_from_in) zinb += zexb;
+        else if (s.tide) tideb += zexb;
+        const double tx = -s.ny, ty = s.nx;
+        const double ax = s.qn_from_in ? s.nx : 0.0, ay = s.qn_from_in ? s.ny : 0.0;
+        // qxex = (ty qn - ny qt) dn, qyex = (-tx qn + nx qt) dn, qt = qxin tx + qyin ty
+        qxinb += ((ty * ax - s.ny * tx) * qxexb + (-tx * ax + s.nx * tx) * qyexb) * s.dn;
+        qyinb += ((ty * ay - s.ny * ty) * qxexb + (-tx * ay + s.nx * ty) * qyexb) * s.dn;
+    } else if (s.boundary) {
+        zinb += zexb;
+        qxinb += Ar[0][0] * qxexb + Ar[1][0] * qyexb;
+        qyinb += Ar[0][1] * qxexb + Ar[1][1] * qyexb;
+    } else {
+        exz = zexb; exqx = qxexb; exqy = qyexb;
+    }
+    slotb[s.kin * P + s.el_in] = zinb;
+    slotb[(3 + s.kin) * P + s.el_in] = qxinb;
+    slotb[(6 + s.kin) * P + s.el_in] = qyinb;
+    zbe[s.kin * P + s.el_in] = zbb;
+    if (!s.boundary) {
+        slotb[s.kex * P + s.el_ex] = exz;
+        slotb[(3 + s.kex) * P + s.el_ex] = exqx;
+        slotb[(6 + s.kex) * P + s.el_ex] = exqy;
+        zbe[s.kex * P + s.el_ex] = 0.0;
+    }
+    return tideb;
+}
+
+// every edge in one launch (WETDRY false: wet/dry off, one Roe evaluation per
+// edge, 3 waves/SIMD; true: up to two, 2 waves/SIMD)
+template <bool WETDRY>
+__global__ __launch_bounds__(256, WETDRY ? 2 : 3) void k_adj_edge(MeshDev m, PhysDev ph, const double *__restrict__ win,
                                                   const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                   const int *__restrict__ wd, const double *__restrict__ Fhb,
                                                   const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_lam,
@@ -517,127 +650,100 @@ __global__ __launch_bounds__(256, 2) void k_adj_edge(MeshDev m, PhysDev ph, cons
     const MemberDev &M = m.mem[mid];
     const int ed = lb * 256 + (int)threadIdx.x;
     double tideb = 0.0;
-    if (ed < M.ed_off + M.ned) {
-        const int P = m.pitch;
-        const double g = ph.g;
-        const EdgeState s = edge_state(m, M, rec[mid], win, bzeta, bqn, ed);
-        double lam_share = 0.0;
-        const AdjCtrl &ac = actl[mid];
-        // the tie count: recorded by the taped forward (use_lambda 2) or
-        // counted by k_adj_ties (1)
-        const int ntie = (use_lambda == 2) ? rec_lam[mid].ntie : ac.ntie;
-        if (use_lambda && ac.lamb != 0.0 && ntie > 0) lam_share = ac.lamb / ntie;
-        const double target = __longlong_as_double((long long)rec_lam[mid].lam2);
-        double Fin_b[3], Fex_b[3];
-#pragma unroll
-        for (int f = 0; f < 3; ++f) {
-            Fin_b[f] = Fhb[(f * 3 + s.kin) * P + s.el_in];
-            Fex_b[f] = s.boundary ? 0.0 : -Fhb[(f * 3 + s.kex) * P + s.el_ex];
-        }
-        // adjoints of the edge-level inputs.  Up to two Roe evaluations per
-        // edge: (wet) one with g; (one side dry) the g flux and the lg = 0
-        // flux of the same states; (both dry) el_in's and el_ex's reflection
-        // fluxes.  One roe_eval/roe_adj in a loop keeps a single copy of the
-        // reverse code (and its registers) for every case.
-        double zinb = 0, qxinb = 0, qyinb = 0, zexb = 0, qxexb = 0, qyexb = 0, zbb = 0;
-        double Ar[2][2];
-        reflect_matrix(s.nx, s.ny, s.dn, Ar);
-        int mode = 0;   // 0 wet, 1 one side dry, 2 both dry
-        bool in_wet = true, ex_wet = true;
-        if (WETDRY) {
-            const int wd_in = wd[s.el_in], wd_ex = s.boundary ? wd_in : wd[s.el_ex];
-            in_wet = wd_in == 1;
-            ex_wet = s.boundary ? in_wet : (wd_ex == 1);
-            if (wd_in == 0 && wd_ex == 0) mode = 2;
-            else if (!(in_wet && ex_wet)) mode = 1;
-        }
-        RoeIn acc = {0, 0, 0, 0, 0, 0, 0};   // identity-mapped input adjoints (modes 0, 1)
-        const int nev = (mode == 0) ? 1 : 2;
-#pragma unroll 1
-        for (int j = 0; j < nev; ++j) {
-            RoeIn a;
-            double lg = g, Fb[3];
-            bool lam_op = true;
-            if (mode == 2) {
-                lg = 0.0;
-                if (j == 0) {
-                    double rx, ry;
-                    reflect(s.qxin, s.qyin, s.nx, s.ny, s.dn, rx, ry);
-                    a = RoeIn{s.zin, s.zin, s.qxin, rx, s.qyin, ry, s.zb};
-                } else {
-                    double sx, sy;
-                    reflect(s.qxex, s.qyex, s.nx, s.ny, s.dn, sx, sy);
-                    a = RoeIn{s.zex, s.zex, sx, s.qxex, sy, s.qyex, s.zb};
-                }
-#pragma unroll
-                for (int f = 0; f < 3; ++f) Fb[f] = (j == 0) ? Fin_b[f] : Fex_b[f];
-            } else {
-                a = RoeIn{s.zin, s.zex, s.qxin, s.qxex, s.qyin, s.qyex, s.zb};
-                if (mode == 0) {
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) Fb[f] = Fin_b[f] + Fex_b[f];
-                } else if (j == 0) {   // the g flux feeds the wet sides
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) Fb[f] = (in_wet ? Fin_b[f] : 0.0) + (ex_wet ? Fex_b[f] : 0.0);
-                } else {               // the lg = 0 flux the dry sides (|lambda| counted once)
-                    lg = 0.0;
-                    lam_op = false;
-#pragma unroll
-                    for (int f = 0; f < 3; ++f) Fb[f] = (in_wet ? 0.0 : Fin_b[f]) + (ex_wet ? 0.0 : Fex_b[f]);
-                }
-            }
-            // (|lambda| does not depend on lg: the tie test inside roe_adj_tie
-            // sees the operands the forward folded)
-            RoeIn ib = (mode == 2) ? RoeIn{0, 0, 0, 0, 0, 0, 0} : acc;
-            roe_adj_tie(a, s.nx, s.ny, lg, g, Fb, lam_op ? lam_share : 0.0, target, ib);
-            if (mode != 2) {
-                acc = ib;
-            } else if (j == 0) {
-                zinb += ib.zi + ib.ze;
-                qxinb += ib.qxi + Ar[0][0] * ib.qxe + Ar[1][0] * ib.qye;
-                qyinb += ib.qyi + Ar[0][1] * ib.qxe + Ar[1][1] * ib.qye;
-                zbb += ib.zb;
-            } else {
-                zexb += ib.zi + ib.ze;
-                qxexb += ib.qxe + Ar[0][0] * ib.qxi + Ar[1][0] * ib.qyi;
-                qyexb += ib.qye + Ar[0][1] * ib.qxi + Ar[1][1] * ib.qyi;
-                zbb += ib.zb;
-            }
-        }
-        if (mode != 2) {
-            zinb += acc.zi; zexb += acc.ze; qxinb += acc.qxi; qxexb += acc.qxe;
-            qyinb += acc.qyi; qyexb += acc.qye; zbb += acc.zb;
-        }
-        // boundary data -> slots (compute2DL.c:149-199)
-        double exz = 0, exqx = 0, exqy = 0;   // adjoint of the el_ex slot
-        if (s.prescribed) {
-            if (s.zex_from_in) zinb += zexb;
-            else if (s.tide) tideb += zexb;
-            const double tx = -s.ny, ty = s.nx;
-            const double ax = s.qn_from_in ? s.nx : 0.0, ay = s.qn_from_in ? s.ny : 0.0;
-            // qxex = (ty qn - ny qt) dn, qyex = (-tx qn + nx qt) dn, qt = qxin tx + qyin ty
-            qxinb += ((ty * ax - s.ny * tx) * qxexb + (-tx * ax + s.nx * tx) * qyexb) * s.dn;
-            qyinb += ((ty * ay - s.ny * ty) * qxexb + (-tx * ay + s.nx * ty) * qyexb) * s.dn;
-        } else if (s.boundary) {
-            zinb += zexb;
-            qxinb += Ar[0][0] * qxexb + Ar[1][0] * qyexb;
-            qyinb += Ar[0][1] * qxexb + Ar[1][1] * qyexb;
-        } else {
-            exz = zexb; exqx = qxexb; exqy = qyexb;
-        }
-        slotb[s.kin * P + s.el_in] = zinb;
-        slotb[(3 + s.kin) * P + s.el_in] = qxinb;
-        slotb[(6 + s.kin) * P + s.el_in] = qyinb;
-        zbe[s.kin * P + s.el_in] = zbb;
-        if (!s.boundary) {
-            slotb[s.kex * P + s.el_ex] = exz;
-            slotb[(3 + s.kex) * P + s.el_ex] = exqx;
-            slotb[(6 + s.kex) * P + s.el_ex] = exqy;
-            zbe[s.kex * P + s.el_ex] = 0.0;
-        }
-    }
+    if (ed < M.ed_off + M.ned)
+        tideb = adj_edge_one<WETDRY>(m, ph, M, mid, ed, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl, use_lambda,
+                                     slotb, zbe);
     const double t = block_sum(tideb);
     if (threadIdx.x == 0) tideparts[lb] = t;
+}
+
+// On large meshes with wet/dry the reverse runs in two launches instead.
+// k_adj_edge_wet reverses the edges whose sides are both wet (one Roe
+// evaluation, 3 waves/SIMD), lists each workgroup's other edges in thread
+// order (blist[256 lb + i]) and queues the workgroups that have any ({lb,
+// count}, one atomic per such workgroup), keeping their wet edges' tide
+// adjoints (tb[ed]).  k_adj_edge_dry (up to two evaluations per edge, the
+// register budget of 2 waves/SIMD) takes the queued workgroups on a small
+// grid, one workgroup each, and forms their tide partial over all 256 edges
+// in thread order, as k_adj_edge<true> does: same bits, whatever the queue
+// order.
+__global__ __launch_bounds__(256, 3) void k_adj_edge_wet(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                                      const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                                      const int *__restrict__ wd, const double *__restrict__ Fhb,
+                                                      const StepRec *__restrict__ rec,
+                                                      const StepRec *__restrict__ rec_lam,
+                                                      const AdjCtrl *__restrict__ actl, int use_lambda,
+                                                      double *__restrict__ slotb, double *__restrict__ zbe,
+                                                      double *__restrict__ tideparts, int *__restrict__ blist,
+                                                      double *__restrict__ tb, int2 *__restrict__ queue,
+                                                      int *qcount)
+{
+    __shared__ int wcnt[4];
+    const int lb = xcd_block();
+    const int mid = m.dblk_mem[lb];
+    const MemberDev &M = m.mem[mid];
+    const int ed = lb * 256 + (int)threadIdx.x;
+    double tideb = 0.0;
+    bool dry = false;
+    if (ed < M.ed_off + M.ned) {
+        // both sides wet (a boundary edge's outside is its inside)
+        const int4 E = m.edges[ed];
+        const bool bnd = (m.einfo[ed] & EI_BOUNDARY) != 0;
+        if (wd[E.x] == 1 && (bnd || wd[E.y] == 1))
+            tideb = adj_edge_one<false>(m, ph, M, mid, ed, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
+                                        use_lambda, slotb, zbe);
+        else
+            dry = true;
+    }
+    const unsigned long long bal = __ballot(dry);
+    const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
+    if (lane == 0) wcnt[wv] = __popcll(bal);
+    __syncthreads();
+    int base = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) base += (i < wv) ? wcnt[i] : 0;
+    const int tot = wcnt[0] + wcnt[1] + wcnt[2] + wcnt[3];
+    if (dry) blist[lb * 256 + base + __popcll(bal & ((1ull << lane) - 1))] = ed;
+    if (tot) tb[ed] = tideb;
+    const double t = block_sum(tideb);
+    if (threadIdx.x == 0) {
+        tideparts[lb] = t;
+        if (tot) queue[atomicAdd(qcount, 1)] = make_int2(lb, tot);
+    }
+}
+
+__global__ __launch_bounds__(256, 2) void k_adj_edge_dry(MeshDev m, PhysDev ph, const double *__restrict__ win,
+                                                      const double *__restrict__ bzeta, const double *__restrict__ bqn,
+                                                      const int *__restrict__ wd, const double *__restrict__ Fhb,
+                                                      const StepRec *__restrict__ rec,
+                                                      const StepRec *__restrict__ rec_lam,
+                                                      const AdjCtrl *__restrict__ actl, int use_lambda,
+                                                      double *__restrict__ slotb, double *__restrict__ zbe,
+                                                      double *__restrict__ tideparts, const int *__restrict__ blist,
+                                                      const double *__restrict__ tb, const int2 *__restrict__ queue,
+                                                      const int *qcount, int *qnext)
+{
+    __shared__ double tbl[256];
+    // the counter of the next k_adj_edge_wet, unused by this launch
+    if (blockIdx.x == 0 && threadIdx.x == 0) *qnext = 0;
+    const int nq = *qcount;
+    for (int q = (int)blockIdx.x; q < nq; q += (int)gridDim.x) {
+        const int2 Q = queue[q];
+        const int b = Q.x;
+        const int mid = m.dblk_mem[b];
+        const MemberDev &M = m.mem[mid];
+        tbl[threadIdx.x] = tb[b * 256 + (int)threadIdx.x];
+        __syncthreads();
+        if ((int)threadIdx.x < Q.y) {
+            const int ed = blist[b * 256 + (int)threadIdx.x];
+            tbl[ed - b * 256] = adj_edge_one<true>(m, ph, M, mid, ed, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
+                                                   use_lambda, slotb, zbe);
+        }
+        __syncthreads();
+        const double t = block_sum(tbl[threadIdx.x]);
+        if (threadIdx.x == 0) tideparts[b] = t;
+        __syncthreads();   // tbl and block_sum's shared slots are reused next round
+    }
 }
 
 // Dst += slotb (stage input adjoint); the z_edge adjoints to the edge's two
@@ -828,15 +934,28 @@ void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double 
 
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
-                     double *slotb, double *zbe, double *tideparts, int with_lambda, hipStream_t s)
+                     double *slotb, double *zbe, double *tideparts, int with_lambda, const AdjEdgeQueue &eq,
+                     hipStream_t s)
 {
-    const dim3 g((unsigned)(m.ned_pad / 256)), b(256);
-    if (ph.wetdry)
-        hipLaunchKernelGGL(k_adj_edge<true>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
-                           with_lambda, slotb, zbe, tideparts);
+    const int nblk = m.ned_pad / 256;
+    const dim3 g((unsigned)nblk), b(256);
+    if (ph.wetdry && m.ned_pad >= ADJ_EDGE_SPLIT) {
+        // the queue counter alternates between calls; each dry launch zeroes
+        // the one the next call uses (both start at zero)
+        int *qc = eq.qcount + eq.parity, *qn = eq.qcount + (1 - eq.parity);
+        hipLaunchKernelGGL(k_adj_edge_wet, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
+                           with_lambda, slotb, zbe, tideparts, eq.blist, eq.tb, eq.queue, qc);
+        // two workgroups per CU at the dry kernel's 2 waves/SIMD
+        const dim3 gd((unsigned)(nblk < 512 ? nblk : 512));
+        hipLaunchKernelGGL(k_adj_edge_dry, gd, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
+                           with_lambda, slotb, zbe, tideparts, eq.blist, eq.tb, eq.queue, qc, qn);
+        eq.parity = 1 - eq.parity;
+    } else if (ph.wetdry)
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_edge<true>), g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec,
+                           rec_lam, actl, with_lambda, slotb, zbe, tideparts);
     else
-        hipLaunchKernelGGL(k_adj_edge<false>, g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
-                           with_lambda, slotb, zbe, tideparts);
+        hipLaunchKernelGGL(HIP_KERNEL_NAME(k_adj_edge<false>), g, b, 0, s, m, ph, win, bzeta, bqn, wd, Fhb, rec,
+                           rec_lam, actl, with_lambda, slotb, zbe, tideparts);
 }
 
 // the tie count of the recorded max |lambda| that k_adj_edge with

@@ -107,6 +107,20 @@ struct MemberDev {
     double infil_K, infil_P;  // Green-Ampt conductivity and psi * delta_theta (K = 0: off)
 };
 
+// the edge reverse's list of non-wet edges per workgroup, the wet edges'
+// tide adjoints of the listed workgroups and the queue of those workgroups
+// (wet/dry on meshes of ADJ_EDGE_SPLIT padded edges or more): blist[ned_pad],
+// tb[ned_pad], queue[ned_pad / 256], two counters used alternately (zero
+// before the first call)
+constexpr int ADJ_EDGE_SPLIT = 1 << 21;
+struct AdjEdgeQueue {
+    int *blist;
+    double *tb;
+    int2 *queue;
+    int *qcount;
+    mutable int parity;
+};
+
 struct MeshDev {
     int pitch, nn, ned_pad, nmem;
     const double4 *nodes;

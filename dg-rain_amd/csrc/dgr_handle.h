@@ -52,7 +52,8 @@ void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double 
                     double *dtparts, hipStream_t s);
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
-                     double *slotb, double *zbe, double *tideparts, int with_lambda, hipStream_t s);
+                     double *slotb, double *zbe, double *tideparts, int with_lambda, const AdjEdgeQueue &eq,
+                     hipStream_t s);
 void launch_adj_ties(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl, hipStream_t s);
 void launch_adj_combine(const MeshDev &m, double *Dst, const double *slotb, const double *zbe, double *zvb,

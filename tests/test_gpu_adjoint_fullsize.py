@@ -185,3 +185,56 @@ def test_batched_gradients_equal_solo(gpu):
     rng = np.random.default_rng(23)
     _check_directions(wl, g, s, o, W, every, solo[0][1][1], "z", 2, [4e-7, 1e-7, 2.5e-8, 6.25e-9], 1e-6, rng,
                       _gauges_gpu, avg=2)
+
+
+# padded edges from which the library reverses wet/dry edge fluxes in two
+# launches (wet edges, then the listed others)
+EDGE_SPLIT = 1 << 21
+
+
+def _padded_edges(wl):
+    ned = int(np.asarray(wl.mesh.edg_to_vert).size // 2)
+    return (ned + 255) // 256 * 256
+
+
+def test_split_edge_reverse_equals_one_pass(gpu):
+    """Three 512x512 members in one ensemble (2.36 M padded edges: the edge
+    reverse runs as a wet-edge launch plus a launch over the listed non-wet
+    edges): two coastal members with their own bumps, tide and gauges, and a
+    dry-bed dam break (half the domain dry, the other edges one- or both-side
+    dry as the front moves).  Every member's loss and gradients equal its
+    solo gradient (one launch per edge reverse) bit for bit."""
+    W, every, K = 20, 10, 10
+    probs = []
+    for i in range(3):
+        if i < 2:
+            wl = configs.coastal(N=512, nsteps=W, seed=70 + i, tide_amp=0.5 + 0.5 * i, level=3.5)
+            rng = np.random.default_rng(80 + i)
+            wl = smooth_start(wl, rng)
+        else:
+            wl = configs.dry_bed_dam_break(N=512, nsteps=W)
+            rng = np.random.default_rng(82)
+        gauges = np.sort(rng.choice(wl.num_el, 64, replace=False)).astype(np.int32)
+        truth = wl.mesh.n.copy()
+        start = truth * np.exp(0.2 * rng.standard_normal(truth.size))
+        obs = _gauges_gpu(wl, gauges, W, every, wl.mesh.z, truth)
+        probs.append((wl, gauges, start, obs))
+    assert all(_padded_edges(p[0]) < EDGE_SPLIT for p in probs)
+    assert sum(_padded_edges(p[0]) for p in probs) >= EDGE_SPLIT
+    solo = [_adjoint(wl, g, s, o, W, every, K, -1) for wl, g, s, o in probs]
+    e = native.Ensemble([p[0].mesh for p in probs], [p[0].solver_params() for p in probs])
+    for i, (wl, g, s, o) in enumerate(probs):
+        e.set_nodal(n=s, member=i)
+        e.set_state(i, wl.zeta, wl.qx, wl.qy)
+    loss, gz, gn = e.adjoint_gauges(W, every, K, np.stack([p[1] for p in probs]),
+                                    np.concatenate([p[3] for p in probs]))
+    e.close()
+    off = 0
+    for i, (wl, g, s, o) in enumerate(probs):
+        nn = wl.mesh.num_nodes
+        sl, sz, sn = solo[i]
+        assert np.all(np.isfinite(sz)) and np.any(sz != 0), i
+        assert loss[i] == sl[0], i
+        assert np.array_equal(gz[off:off + nn], sz) and np.array_equal(gn[off:off + nn], sn), i
+        off += nn
+    print("  %d members of a split edge reverse bit for bit their one-launch gradients" % len(probs), flush=True)
