@@ -312,7 +312,7 @@ def adjoint_bench(a, variant):
 
 
 def large_bench(a, variant):
-    """part 1, the 2048 x 2048 mesh (8,388,608 triangles, 40 km,
+    """The large workload: part 1, the 2048 x 2048 mesh (8,388,608 triangles, 40 km,
     128 bumps, tide + 20 mm/h rain) timed over a few steps; part 2, 16 gauge
     gradients on one 512 x 512 sub-mesh (64 seeded gauges each), batched as
     the 16 members of one ensemble handle (one launch per reverse kernel for
@@ -495,7 +495,7 @@ def main():
                  % (out["large"]["forward"]["value"], out["large"]["batched_adjoint"]["ratio"]))
     if not a.no_cpu:
         out["cpu_baseline"] = cpu_baseline(wl, a.cpu_seconds)
-        # the host's share of cores on a GPU box is 16
+        # the host's share of cores per GPU is 16 on the measurement machines
         nt = min(16, os.cpu_count() or 1)
         if nt > 1:
             out["cpu_baseline_threads"] = cpu_baseline(wl, a.cpu_seconds / 2, threads=nt)

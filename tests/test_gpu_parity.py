@@ -4,7 +4,7 @@ deterministic algorithm).  The bar is bit-for-bit equality up to the sign of
 zero, for states, RHS, edge fluxes, max_lambda, dt, wet/dry flags and limiter
 activations.  (The oracle's reference-math mode is pinned to the compiled
 reference in test_oracle_vs_reference.py; the two modes are compared to the
-north-star tolerance rel-L2 <= 1e-11 in test_oracle_modes.py.)"""
+tolerance rel-L2 <= 1e-11 in test_oracle_modes.py.)"""
 import os
 
 import numpy as np
@@ -545,7 +545,7 @@ def test_autotune_keeps_state(gpu):
 @pytest.mark.parametrize("wetdry", [0, 1])
 def test_lake_at_rest_on_gpu(gpu, wetdry):
     """Still water over a bumpy bed (walls, 32x32 grid, 500 steps, limiter on):
-    the surface stays flat to 1e-13 (the north-star bar on h).  The momentum
+    the surface stays flat to 1e-13 (the bar on h).  The momentum
     is not exactly balanced by the reference scheme itself: it drifts by
     rounding to ~6e-12 after 500 steps, identically in the oracle's
     reference-math mode (checked here to 1e-11) and bit for bit in its
