@@ -1360,14 +1360,16 @@ __device__ __forceinline__ void node_minmax_block(const MeshDev &m, const double
 }
 
 // LDS of the staged node kernels: one region per wave
-constexpr size_t NL_WAVE_BYTES = 3 * NL_CAP * sizeof(double) + NL_CAP * sizeof(int);
+// (the attaining elements' ids, ARG only, after the means)
+template <bool ARG>
+constexpr size_t nl_wave_bytes() { return 3 * NL_CAP * sizeof(double) + (ARG ? NL_CAP * sizeof(int) : 0); }
 
 template <bool ARG>
 __global__ __launch_bounds__(256) void k_node_minmax(MeshDev m, const double *__restrict__ means,
                                                      double *__restrict__ nodemm, int *__restrict__ nodearg)
 {
     extern __shared__ double nl_lds[];
-    double *lds = nl_lds + ((int)threadIdx.x >> 6) * (NL_WAVE_BYTES / sizeof(double));
+    double *lds = nl_lds + ((int)threadIdx.x >> 6) * (nl_wave_bytes<ARG>() / sizeof(double));
     node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_block() * (int)blockDim.x + (int)threadIdx.x, lds);
 }
 
@@ -1931,7 +1933,7 @@ __global__ __launch_bounds__(256) void k_node_minmax_end(MeshDev m, PhysDev ph, 
         return;
     }
     extern __shared__ double nl_lds[];
-    double *lds = nl_lds + ((int)threadIdx.x >> 6) * (NL_WAVE_BYTES / sizeof(double));
+    double *lds = nl_lds + ((int)threadIdx.x >> 6) * (nl_wave_bytes<ARG>() / sizeof(double));
     node_minmax_block<ARG>(m, means, nodemm, nodearg, xcd_map(b - nf, (int)gridDim.x - nf) * 256 + (int)threadIdx.x,
                            lds);
 }
@@ -2076,9 +2078,9 @@ void launch_edge_geometry(const MeshDev &m, double4 *egeo, hipStream_t s)
 void launch_node_minmax(const MeshDev &m, const double *means, double *nodemm, int *nodearg, hipStream_t s)
 {
     const dim3 g((unsigned)((m.nn + m.wg - 1) / m.wg)), b((unsigned)m.wg);
-    const size_t lds = (size_t)(m.wg / 64) * NL_WAVE_BYTES;
-    if (nodearg) hipLaunchKernelGGL(k_node_minmax<true>, g, b, lds, s, m, means, nodemm, nodearg);
-    else hipLaunchKernelGGL(k_node_minmax<false>, g, b, lds, s, m, means, nodemm, nodearg);
+    const size_t w = (size_t)(m.wg / 64);
+    if (nodearg) hipLaunchKernelGGL(k_node_minmax<true>, g, b, w * nl_wave_bytes<true>(), s, m, means, nodemm, nodearg);
+    else hipLaunchKernelGGL(k_node_minmax<false>, g, b, w * nl_wave_bytes<false>(), s, m, means, nodemm, nodearg);
 }
 
 void launch_limit(bool pdop, const MeshDev &m, const PhysDev &ph, const double *win, double *wout,
@@ -2139,13 +2141,12 @@ void launch_node_minmax_end(const MeshDev &m, const PhysDev &ph, const double *m
 {
     const int nf = (m.nmem + 7) & ~7;
     const dim3 g((unsigned)(nf + (m.nn + 255) / 256)), b(256);
-    const size_t lds = 4 * NL_WAVE_BYTES;
     if (nodearg)
-        hipLaunchKernelGGL(k_node_minmax_end<true>, g, b, lds, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
-                           rec_next, ties ? 1 : 0);
+        hipLaunchKernelGGL(k_node_minmax_end<true>, g, b, 4 * nl_wave_bytes<true>(), s, m, ph, means, nodemm, nodearg, c,
+                           kind, nf, rec_end, rec_next, ties ? 1 : 0);
     else
-        hipLaunchKernelGGL(k_node_minmax_end<false>, g, b, lds, s, m, ph, means, nodemm, nodearg, c, kind, nf, rec_end,
-                           rec_next, ties ? 1 : 0);
+        hipLaunchKernelGGL(k_node_minmax_end<false>, g, b, 4 * nl_wave_bytes<false>(), s, m, ph, means, nodemm, nodearg,
+                           c, kind, nf, rec_end, rec_next, ties ? 1 : 0);
 }
 
 void launch_step_end(const MeshDev &m, const PhysDev &ph, Ctrl *c, int kind, bool next, hipStream_t s,
