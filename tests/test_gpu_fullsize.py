@@ -19,19 +19,23 @@ from helpers import same
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("N,walls", [(16, True), (512, False)])
-def test_still_coastal_start_fails_as_the_oracle(gpu, N, walls):
+@pytest.mark.parametrize("N,walls,variant", [(16, True, None), (512, False, None), (512, False, 2)])
+def test_still_coastal_start_fails_as_the_oracle(gpu, N, walls, variant):
     """The configured still-water coastal start (still water at 0 m over the
     -10 m .. +2 m bed) cannot be run by the reference's wet/dry scheme: a
     NaN flux stops it within a few steps.  The device stops in the same step
     with the oracle's error code and edge, at 16x16 with walls only (the
     compiled reference's case, tests/golden/still_coastal_failure.npz: step
-    3, edge 95) and at the full 512x512 size with the tide.  A normal error
-    return: nothing on the device faults."""
+    3, edge 95) and at the full 512x512 size with the tide, in the default
+    variant and in the edge-centric one (whose workgroup partials carry the
+    NaN operands' positions).  A normal error return: nothing on the device
+    faults."""
     wl = configs.coastal(N=N, start="still")
     if walls:
         wl.mesh.bdry[:] = 0
     s = native.Solver(wl.mesh, wl.solver_params())
+    if variant is not None:
+        s.set_variant(variant)
     s.set_state(wl.zeta, wl.qx, wl.qy)
     o = oracle.Oracle.for_workload(wl, threads=min(16, os.cpu_count() or 1))
     for step in range(1, 20):
@@ -75,6 +79,33 @@ def test_coastal_512_flood_bitwise(gpu):
             assert np.array_equal(s.get_modify(st), o.get_modify(st)), done
         assert rep.t == t and rep.dt == dt, (done, rep.t, t, rep.dt, dt)
         print("  %d steps of %d triangles bitwise equal (t=%.6g)" % (done, wl.num_el, t), flush=True)
+    s.close()
+
+
+def test_dry_bed_512_bitwise(gpu):
+    """The wet/dry-active benchmark leg (dry-bed dam break at 512x512, half
+    the domain dry) with the edge-centric variant the benchmark picks: 20
+    steps, states, wet/dry flags, limiter activations, t and dt equal to the
+    oracle's after 10 and 20 steps, with one-side-dry and both-dry edges
+    present throughout."""
+    wl = configs.dry_bed_dam_break(N=512, nsteps=20)
+    s = native.Solver(wl.mesh, wl.solver_params())
+    s.set_variant(2)
+    s.set_state(wl.zeta, wl.qx, wl.qy)
+    o = oracle.Oracle.for_workload(wl, threads=min(16, os.cpu_count() or 1))
+    for done in (10, 20):
+        rep = s.run(10)
+        t, dt = o.run(10)
+        for a, b in zip(s.get_state(), o.get_state()):
+            assert same(a, b), "state differs after %d steps" % done
+        wd = s.get_wd()
+        assert np.array_equal(wd, o.get_wd()), done
+        assert 0 < np.count_nonzero(wd == 0) < wd.size, done
+        for st in (0, 1):
+            assert np.array_equal(s.get_modify(st), o.get_modify(st)), done
+        assert rep.t == t and rep.dt == dt, (done, rep.t, t, rep.dt, dt)
+        print("  %d steps of %d triangles bitwise equal (t=%.6g, %d dry)" % (done, wl.num_el, t,
+                                                                            np.count_nonzero(wd == 0)), flush=True)
     s.close()
 
 
