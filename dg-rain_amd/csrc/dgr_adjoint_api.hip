@@ -441,6 +441,21 @@ extern "C" dgr_status dgr_adjoint_gauges(dgr_handle *h, const dgr_adjoint_spec *
     DGR_HIP(hipMemsetAsync(wb, 0, Wb, s));
     DGR_HIP(hipMemsetAsync(zvb, 0, 3 * P * sizeof(double), s));
     DGR_HIP(hipMemsetAsync(nvb, 0, 3 * P * sizeof(double), s));
+    if (split) {
+        // the split edge reverse pays off only where nearly every element is
+        // wet (its second launch works through the listed edges' workgroups
+        // a few hundred at a time): decided on the flags the reverse starts
+        // from, at most 1 in 1000 elements not wet
+        int notwet = 0;
+        size_t nel = 0;
+        for (int mi = 0; mi < nm; ++mi) nel += (size_t)h->mem[mi].ne;
+        DGR_HIP(hipMemsetAsync(eq.qcount, 0, sizeof(int), s));
+        launch_count_notwet(h->m, h->d_wd, eq.qcount, s);
+        DGR_HIP(hipGetLastError());
+        DGR_HIP(hipMemcpyAsync(&notwet, eq.qcount, sizeof(int), hipMemcpyDeviceToHost, s));
+        DGR_HIP(hipStreamSynchronize(s));
+        eq.split = (size_t)notwet * 1000 <= nel;
+    }
     DGR_HIP(hipMemsetAsync(eq.qcount, 0, 2 * sizeof(int), s));
     bool have_slot1 = false;
     for (int c = nseg - 1; c >= 0; --c) {

@@ -932,6 +932,19 @@ void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double 
                            zvb, nvb, dtparts);
 }
 
+// the elements whose wet/dry flag is not wet, counted into *out
+__global__ __launch_bounds__(256) void k_count_notwet(MeshDev m, const int *__restrict__ wd, int *out)
+{
+    const ElemLane L = elem_lane(m);
+    const unsigned long long bal = __ballot(L.valid && wd[L.e] != 1);
+    if (((int)threadIdx.x & 63) == 0 && bal) atomicAdd(out, __popcll(bal));
+}
+
+void launch_count_notwet(const MeshDev &m, const int *wd, int *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_count_notwet, dim3((unsigned)(m.pitch / 256)), dim3(256), 0, s, m, wd, out);
+}
+
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
                      double *slotb, double *zbe, double *tideparts, int with_lambda, const AdjEdgeQueue &eq,
@@ -939,7 +952,7 @@ void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, con
 {
     const int nblk = m.ned_pad / 256;
     const dim3 g((unsigned)nblk), b(256);
-    if (ph.wetdry && m.ned_pad >= ADJ_EDGE_SPLIT) {
+    if (ph.wetdry && eq.split) {
         // the queue counter alternates between calls; each dry launch zeroes
         // the one the next call uses (both start at zero)
         int *qc = eq.qcount + eq.parity, *qn = eq.qcount + (1 - eq.parity);

@@ -109,9 +109,10 @@ struct MemberDev {
 
 // the edge reverse's list of non-wet edges per workgroup, the wet edges'
 // tide adjoints of the listed workgroups and the queue of those workgroups
-// (wet/dry on meshes of ADJ_EDGE_SPLIT padded edges or more): blist[ned_pad],
-// tb[ned_pad], queue[ned_pad / 256], two counters used alternately (zero
-// before the first call)
+// (wet/dry on meshes of ADJ_EDGE_SPLIT padded edges or more whose elements
+// are nearly all wet when the reverse starts): blist[ned_pad], tb[ned_pad],
+// queue[ned_pad / 256], two counters used alternately (zero before the first
+// call)
 constexpr int ADJ_EDGE_SPLIT = 1 << 21;
 struct AdjEdgeQueue {
     int *blist;
@@ -119,6 +120,7 @@ struct AdjEdgeQueue {
     int2 *queue;
     int *qcount;
     mutable int parity;
+    bool split;   // two launches (else the one-launch kernel)
 };
 
 struct MeshDev {

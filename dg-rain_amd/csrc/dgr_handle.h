@@ -50,6 +50,7 @@ void launch_adj_rhs(int mode, const MeshDev &m, const PhysDev &ph, const double 
                     const double *U, const double *Ubown, const double *nsum, const int *nodearg, const int *wd,
                     const StepRec *rec, double *Inb, double *Bb, double *Fhb, double *zvb, double *nvb,
                     double *dtparts, hipStream_t s);
+void launch_count_notwet(const MeshDev &m, const int *wd, int *out, hipStream_t s);
 void launch_adj_edge(const MeshDev &m, const PhysDev &ph, const double *win, const double *bzeta, const double *bqn,
                      const int *wd, const double *Fhb, const StepRec *rec, const StepRec *rec_lam, AdjCtrl *actl,
                      double *slotb, double *zbe, double *tideparts, int with_lambda, const AdjEdgeQueue &eq,

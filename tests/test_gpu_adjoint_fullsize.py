@@ -188,7 +188,8 @@ def test_batched_gradients_equal_solo(gpu):
 
 
 # padded edges from which the library reverses wet/dry edge fluxes in two
-# launches (wet edges, then the listed others)
+# launches (wet edges, then the listed others), when at most 1 in 1000
+# elements is not wet as the reverse starts
 EDGE_SPLIT = 1 << 21
 
 
@@ -198,22 +199,26 @@ def _padded_edges(wl):
 
 
 def test_split_edge_reverse_equals_one_pass(gpu):
-    """Three 512x512 members in one ensemble (2.36 M padded edges: the edge
-    reverse runs as a wet-edge launch plus a launch over the listed non-wet
-    edges): two coastal members with their own bumps, tide and gauges, and a
-    dry-bed dam break (half the domain dry, the other edges one- or both-side
-    dry as the front moves).  Every member's loss and gradients equal its
-    solo gradient (one launch per edge reverse) bit for bit."""
+    """Three 512x512 coastal members (all wet, own bumps, tide and gauges)
+    and a 32x32 dry-bed dam break (half its domain dry, its other edges one-
+    or both-side dry as the front moves) in one ensemble: 2.36 M padded edges
+    and 0.07% of the elements not wet, so the edge reverse runs as a wet-edge
+    launch plus a launch over the listed non-wet edges.  Every member's loss
+    and gradients equal its solo gradient (one launch per edge reverse) bit
+    for bit."""
     W, every, K = 20, 10, 10
     probs = []
-    for i in range(3):
-        if i < 2:
+    for i in range(4):
+        if i < 3:
             wl = configs.coastal(N=512, nsteps=W, seed=70 + i, tide_amp=0.5 + 0.5 * i, level=3.5)
             rng = np.random.default_rng(80 + i)
             wl = smooth_start(wl, rng)
         else:
-            wl = configs.dry_bed_dam_break(N=512, nsteps=W)
-            rng = np.random.default_rng(82)
+            # Manning n 0.002: with 0.02 the reference scheme's explicit
+            # friction diverges in the thin films of this small mesh
+            wl = configs.dry_bed_dam_break(N=32, nsteps=W)
+            wl = dataclasses.replace(wl, mesh=dataclasses.replace(wl.mesh, n=np.full(wl.mesh.num_nodes, 0.002)))
+            rng = np.random.default_rng(83)
         gauges = np.sort(rng.choice(wl.num_el, 64, replace=False)).astype(np.int32)
         truth = wl.mesh.n.copy()
         start = truth * np.exp(0.2 * rng.standard_normal(truth.size))
