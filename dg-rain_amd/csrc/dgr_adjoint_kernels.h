@@ -503,15 +503,18 @@ __global__ __launch_bounds__(256) void k_adj_ties(MeshDev m, PhysDev ph, const d
 // (the z_edge adjoint in el_in's slot, 0 in el_ex's) and returns the edge's
 // tide-value adjoint.  With lamb != 0 the edges whose |lambda| operand equals
 // the recorded max (rec_lam) share lamb.  WETDRY false: the edge's sides are
-// wet (or wet/dry is off): one Roe evaluation.
-template <bool WETDRY>
+// wet (or wet/dry is off): one Roe evaluation.  GATE: the edge's wet/dry
+// flags are loaded with its other inputs, and an edge with a side not wet
+// is left alone (*dry set, nothing written).
+template <bool WETDRY, bool GATE = false>
 __device__ __forceinline__ double adj_edge_one(const MeshDev &m, const PhysDev &ph, const MemberDev &M, int mid,
                                                int ed, const double *__restrict__ win,
                                                const double *__restrict__ bzeta, const double *__restrict__ bqn,
                                                const int *__restrict__ wd, const double *__restrict__ Fhb,
                                                const StepRec *__restrict__ rec, const StepRec *__restrict__ rec_lam,
                                                const AdjCtrl *__restrict__ actl, int use_lambda,
-                                               double *__restrict__ slotb, double *__restrict__ zbe)
+                                               double *__restrict__ slotb, double *__restrict__ zbe,
+                                               bool *dry = nullptr)
 {
     double tideb = 0.0;
     const int P = m.pitch;
@@ -529,6 +532,14 @@ __device__ __forceinline__ double adj_edge_one(const MeshDev &m, const PhysDev &
     for (int f = 0; f < 3; ++f) {
         Fin_b[f] = Fhb[(f * 3 + s.kin) * P + s.el_in];
         Fex_b[f] = s.boundary ? 0.0 : -Fhb[(f * 3 + s.kex) * P + s.el_ex];
+    }
+    if (GATE) {
+        // both sides wet (a boundary edge's outside is its inside)
+        const int wi = wd[s.el_in];
+        if (!(wi == 1 && (s.boundary || wd[s.el_ex] == 1))) {
+            *dry = true;
+            return 0.0;
+        }
     }
     // adjoints of the edge-level inputs.  Up to two Roe evaluations per
     // edge: (wet) one with g; (one side dry) the g flux and the lg = 0
@@ -685,16 +696,9 @@ __global__ __launch_bounds__(256, 3) void k_adj_edge_wet(MeshDev m, PhysDev ph, 
     const int ed = lb * 256 + (int)threadIdx.x;
     double tideb = 0.0;
     bool dry = false;
-    if (ed < M.ed_off + M.ned) {
-        // both sides wet (a boundary edge's outside is its inside)
-        const int4 E = m.edges[ed];
-        const bool bnd = (m.einfo[ed] & EI_BOUNDARY) != 0;
-        if (wd[E.x] == 1 && (bnd || wd[E.y] == 1))
-            tideb = adj_edge_one<false>(m, ph, M, mid, ed, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
-                                        use_lambda, slotb, zbe);
-        else
-            dry = true;
-    }
+    if (ed < M.ed_off + M.ned)
+        tideb = adj_edge_one<false, true>(m, ph, M, mid, ed, win, bzeta, bqn, wd, Fhb, rec, rec_lam, actl,
+                                          use_lambda, slotb, zbe, &dry);
     const unsigned long long bal = __ballot(dry);
     const int lane = (int)threadIdx.x & 63, wv = (int)threadIdx.x >> 6;
     if (lane == 0) wcnt[wv] = __popcll(bal);
